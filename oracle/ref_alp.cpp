@@ -1,0 +1,336 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- never part of the product path.
+ *
+ * A C shim over ALP/GraphBLAS's own `reference` / `reference_omp` backend.
+ * oracle/Makefile compiles this file together with the reference's sources
+ * where they lie (include/ plus src/graphblas/{descriptors,rc}.cpp and
+ * src/graphblas/reference/{init,config,io}.cpp) into oracle/_ref/. The
+ * resulting libraries are used for two things only:
+ *   - writing golden fixtures (tests/golden/make_golden.py), and
+ *   - the CPU baseline leg of bench.py (timing reference_omp on the host).
+ * Nothing under alp_amd/ links or loads them.
+ *
+ * Every entry point builds its inputs with the reference's own generators
+ * (grb::algorithms::build_ndims_system_matrix, build_hpcg_system, ...) and
+ * calls the reference's own primitives/algorithms:
+ *   - grb::mxv                      include/graphblas/reference/blas2.hpp:1946
+ *   - grb::dot                      include/graphblas/reference/blas1.hpp:9520
+ *   - conjugate_gradient            include/graphblas/algorithms/conjugate_gradient.hpp:507
+ *   - hpcg                          include/graphblas/algorithms/hpcg/hpcg.hpp:97
+ *   - simple_pagerank               include/graphblas/algorithms/simple_pagerank.hpp:146
+ *   - knn                           include/graphblas/algorithms/knn.hpp:82
+ */
+#include <graphblas.hpp>
+#include <graphblas/algorithms/conjugate_gradient.hpp>
+#include <graphblas/algorithms/hpcg/hpcg.hpp>
+#include <graphblas/algorithms/hpcg/system_building_utils.hpp>
+#include <graphblas/algorithms/hpcg/matrix_building_utils.hpp>
+#include <graphblas/algorithms/simple_pagerank.hpp>
+#include <graphblas/algorithms/knn.hpp>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#define REF_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+typedef grb::Semiring<
+	grb::operators::add< double >, grb::operators::mul< double >,
+	grb::identities::zero, grb::identities::one
+> RealRing;
+
+bool g_inited = false;
+
+int ensure_init() {
+	if( g_inited ) { return 0; }
+	grb::RC rc = grb::init();
+	if( rc != grb::SUCCESS ) { return static_cast< int >( rc ) + 100; }
+	g_inited = true;
+	return 0;
+}
+
+double now_ms() {
+	return std::chrono::duration< double, std::milli >(
+		std::chrono::steady_clock::now().time_since_epoch() ).count();
+}
+
+int build_stencil( grb::Matrix< double > &A, size_t nx, size_t ny, size_t nz ) {
+	const std::array< size_t, 3 > sizes = { nx, ny, nz };
+	return static_cast< int >( grb::algorithms::build_ndims_system_matrix< 3, double >(
+		A, sizes, 1, 26.0, -1.0 ) );
+}
+
+int to_host( const grb::Vector< double > &v, double * out ) {
+	const size_t n = grb::size( v );
+	std::fill( out, out + n, 0.0 );
+	for( const auto &pair : v ) { out[ pair.first ] = pair.second; }
+	return 0;
+}
+
+int from_host( grb::Vector< double > &v, const double * in ) {
+	const size_t n = grb::size( v );
+	return static_cast< int >( grb::buildVector( v, in, in + n, grb::SEQUENTIAL ) );
+}
+
+} // namespace
+
+/** Which backend this library was compiled against: 0 = reference, 1 = reference_omp. */
+REF_API int ref_backend() {
+#ifdef ALPREF_OMP
+	return 1;
+#else
+	return 0;
+#endif
+}
+
+/** Number of nonzeros of the nx*ny*nz 27-point stencil built by the reference. */
+REF_API int ref_stencil_nnz( size_t nx, size_t ny, size_t nz, size_t * nnz_out ) {
+	if( ensure_init() ) { return -1; }
+	grb::Matrix< double > A( nx * ny * nz, nx * ny * nz );
+	int rc = build_stencil( A, nx, ny, nz );
+	*nnz_out = grb::nnz( A );
+	return rc;
+}
+
+/**
+ * Dumps the reference's CRS (and CCS) arrays of the 27-point stencil, exactly as
+ * stored by Matrix<double,reference>::buildMatrixUnique (reference/matrix.hpp:1854).
+ */
+REF_API int ref_stencil_storage( size_t nx, size_t ny, size_t nz, int which,
+	uint64_t * offsets, uint32_t * indices, double * values
+) {
+	if( ensure_init() ) { return -1; }
+	const size_t n = nx * ny * nz;
+	grb::Matrix< double > A( n, n );
+	int rc = build_stencil( A, nx, ny, nz );
+	if( rc ) { return rc; }
+	const size_t nnz = grb::nnz( A );
+	if( which == 0 ) {
+		const auto &S = grb::internal::getCRS( A );
+		for( size_t i = 0; i <= n; ++i ) { offsets[ i ] = S.col_start[ i ]; }
+		for( size_t k = 0; k < nnz; ++k ) { indices[ k ] = S.row_index[ k ]; values[ k ] = S.values[ k ]; }
+	} else {
+		const auto &S = grb::internal::getCCS( A );
+		for( size_t i = 0; i <= n; ++i ) { offsets[ i ] = S.col_start[ i ]; }
+		for( size_t k = 0; k < nnz; ++k ) { indices[ k ] = S.row_index[ k ]; values[ k ] = S.values[ k ]; }
+	}
+	return 0;
+}
+
+/** y (in/out) <- y + A x on the 27-point stencil, plus-times over doubles. */
+REF_API int ref_stencil_mxv( size_t nx, size_t ny, size_t nz, const double * x, double * y ) {
+	if( ensure_init() ) { return -1; }
+	const size_t n = nx * ny * nz;
+	grb::Matrix< double > A( n, n );
+	int rc = build_stencil( A, nx, ny, nz );
+	grb::Vector< double > vx( n ), vy( n );
+	rc = rc ? rc : from_host( vx, x );
+	rc = rc ? rc : from_host( vy, y );
+	rc = rc ? rc : static_cast< int >( grb::mxv< grb::descriptors::dense >( vy, A, vx, RealRing() ) );
+	if( !rc ) { to_host( vy, y ); }
+	return rc;
+}
+
+/** y (in/out) <- y + A^T x on the 27-point stencil (the CCS gather path). */
+REF_API int ref_stencil_mxv_transposed( size_t nx, size_t ny, size_t nz, const double * x, double * y ) {
+	if( ensure_init() ) { return -1; }
+	const size_t n = nx * ny * nz;
+	grb::Matrix< double > A( n, n );
+	int rc = build_stencil( A, nx, ny, nz );
+	grb::Vector< double > vx( n ), vy( n );
+	rc = rc ? rc : from_host( vx, x );
+	rc = rc ? rc : from_host( vy, y );
+	rc = rc ? rc : static_cast< int >( grb::mxv< grb::descriptors::dense |
+		grb::descriptors::transpose_matrix >( vy, A, vx, RealRing() ) );
+	if( !rc ) { to_host( vy, y ); }
+	return rc;
+}
+
+/** z <- z + dot(x, y) under plus-times (reference/blas1.hpp:9520). */
+REF_API int ref_dot( size_t n, const double * x, const double * y, double * z ) {
+	if( ensure_init() ) { return -1; }
+	grb::Vector< double > vx( n ), vy( n );
+	int rc = from_host( vx, x );
+	rc = rc ? rc : from_host( vy, y );
+	double acc = *z;
+	rc = rc ? rc : static_cast< int >( grb::dot< grb::descriptors::dense >( acc, vx, vy, RealRing() ) );
+	*z = acc;
+	return rc;
+}
+
+/**
+ * Unpreconditioned CG on the 27-point stencil (conjugate_gradient.hpp:507).
+ * b == nullptr means b = A*1 (computed by the reference's mxv).
+ * Returns the grb::RC code (FAILED = not converged).
+ */
+REF_API int ref_stencil_cg( size_t nx, size_t ny, size_t nz, const double * x0, const double * b,
+	size_t max_iter, double tol, double * x_out, size_t * iters, double * residual
+) {
+	if( ensure_init() ) { return -1; }
+	const size_t n = nx * ny * nz;
+	grb::Matrix< double > A( n, n );
+	int rc = build_stencil( A, nx, ny, nz );
+	grb::Vector< double > x( n ), vb( n ), r( n ), u( n ), temp( n );
+	if( b ) {
+		rc = rc ? rc : from_host( vb, b );
+	} else {
+		grb::Vector< double > ones( n );
+		rc = rc ? rc : static_cast< int >( grb::set( ones, 1.0 ) );
+		rc = rc ? rc : static_cast< int >( grb::set( vb, 0.0 ) );
+		rc = rc ? rc : static_cast< int >( grb::mxv( vb, A, ones, RealRing() ) );
+	}
+	rc = rc ? rc : from_host( x, x0 );
+	if( rc ) { return rc; }
+	size_t it = 0;
+	double res = 0;
+	grb::RC grc = grb::algorithms::conjugate_gradient( x, A, vb, max_iter, tol, it, res, r, u, temp );
+	*iters = it;
+	*residual = res;
+	to_host( x, x_out );
+	return static_cast< int >( grc );
+}
+
+/**
+ * The reference's HPCG (hpcg.hpp:97) on an nx*ny*nz 27-point system built with
+ * build_hpcg_system (system_building_utils.hpp:84): coarsening step 2, minimum
+ * physical size 4, `colors` colour masks by index mod colors, b = A*1, x0 = 0.
+ */
+REF_API int ref_hpcg( size_t nx, size_t ny, size_t nz, size_t max_levels, size_t colors,
+	size_t pre, size_t post, size_t max_iter, double tol, int with_precond,
+	double * x_out, size_t * iters, double * residual
+) {
+	if( ensure_init() ) { return -1; }
+	const std::array< size_t, 3 > sizes = { nx, ny, nz };
+	grb::algorithms::hpcg_system_params< 3, double > params {
+		sizes, 1, colors, 26.0, -1.0, 4, max_levels, 2
+	};
+	std::unique_ptr< grb::algorithms::hpcg_data< double, double, double > > holder;
+	int rc = static_cast< int >( grb::algorithms::build_hpcg_system< 3, double >( holder, params ) );
+	if( rc ) { return rc; }
+	auto &A = holder->A;
+	auto &x = holder->x;
+	auto &b = holder->b;
+	rc = rc ? rc : static_cast< int >( grb::set( x, 1.0 ) );
+	rc = rc ? rc : static_cast< int >( grb::set( b, 0.0 ) );
+	rc = rc ? rc : static_cast< int >( grb::mxv( b, A, x, RealRing() ) );
+	rc = rc ? rc : static_cast< int >( grb::set( x, 0.0 ) );
+	if( rc ) { return rc; }
+	size_t it = 0;
+	double res = 0;
+	rc = static_cast< int >( grb::algorithms::hpcg( *holder, with_precond != 0, pre, post,
+		max_iter, tol, it, res ) );
+	*iters = it;
+	*residual = res;
+	to_host( x, x_out );
+	return rc;
+}
+
+/** Number of MG levels build_hpcg_system creates below the finest for these sizes. */
+REF_API int ref_hpcg_levels( size_t nx, size_t ny, size_t nz, size_t max_levels, size_t * levels ) {
+	if( ensure_init() ) { return -1; }
+	const std::array< size_t, 3 > sizes = { nx, ny, nz };
+	grb::algorithms::hpcg_system_params< 3, double > params { sizes, 1, 8, 26.0, -1.0, 4, max_levels, 2 };
+	std::unique_ptr< grb::algorithms::hpcg_data< double, double, double > > holder;
+	int rc = static_cast< int >( grb::algorithms::build_hpcg_system< 3, double >( holder, params ) );
+	size_t l = 0;
+	for( auto * c = holder->coarser_level; c != nullptr; c = c->coarser_level ) { ++l; }
+	*levels = l;
+	return rc;
+}
+
+/**
+ * simple_pagerank (simple_pagerank.hpp:146) on a pattern matrix given as an
+ * edge list (row = src, col = dst), built with buildMatrixUnique.
+ */
+REF_API int ref_pagerank( size_t n, size_t nedges, const uint32_t * src, const uint32_t * dst,
+	double alpha, double conv, size_t max_iter, double * pr_out, size_t * iters, double * quality
+) {
+	if( ensure_init() ) { return -1; }
+	grb::Matrix< void > L( n, n );
+	std::vector< size_t > I( src, src + nedges ), J( dst, dst + nedges );
+	int rc = static_cast< int >( grb::buildMatrixUnique( L, I.data(), J.data(), nedges, grb::SEQUENTIAL ) );
+	if( rc ) { return rc; }
+	grb::Vector< double > pr( n ), pr_next( n ), pr_nextnext( n ), row_sum( n );
+	grb::Vector< bool > is_dangling( n );
+	size_t it = 0;
+	double q = 0;
+	rc = static_cast< int >( grb::algorithms::simple_pagerank< grb::descriptors::no_operation >(
+		pr, L, pr_next, pr_nextnext, row_sum, is_dangling, alpha, conv, max_iter, &it, &q ) );
+	*iters = it;
+	*quality = q;
+	to_host( pr, pr_out );
+	return rc;
+}
+
+/** knn (knn.hpp:82): the k-hop neighbourhood of `source` on a pattern matrix. */
+REF_API int ref_knn( size_t n, size_t nedges, const uint32_t * src, const uint32_t * dst,
+	size_t source, size_t k, uint8_t * out
+) {
+	if( ensure_init() ) { return -1; }
+	grb::Matrix< void > A( n, n );
+	std::vector< size_t > I( src, src + nedges ), J( dst, dst + nedges );
+	int rc = static_cast< int >( grb::buildMatrixUnique( A, I.data(), J.data(), nedges, grb::SEQUENTIAL ) );
+	if( rc ) { return rc; }
+	grb::Vector< bool > u( n ), buf1( n );
+	rc = static_cast< int >( grb::algorithms::knn< grb::descriptors::no_operation >( u, A, source, k, buf1 ) );
+	std::fill( out, out + n, 0 );
+	for( const auto &pair : u ) { out[ pair.first ] = pair.second ? 1 : 2; }
+	return rc;
+}
+
+/**
+ * CPU baseline timing: builds the nx*ny*nz stencil, then times `reps`
+ * grb::set(y,0) + grb::mxv<dense>(y, A, x) pairs after one warm-up; returns the
+ * median milliseconds of one mxv (the set is timed apart and not included).
+ */
+REF_API double ref_time_stencil_mxv( size_t nx, size_t ny, size_t nz, size_t reps ) {
+	if( ensure_init() ) { return -1; }
+	const size_t n = nx * ny * nz;
+	grb::Matrix< double > A( n, n );
+	if( build_stencil( A, nx, ny, nz ) ) { return -1; }
+	grb::Vector< double > x( n ), y( n );
+	grb::set( x, 1.0 );
+	grb::set( y, 0.0 );
+	grb::mxv< grb::descriptors::dense >( y, A, x, RealRing() );
+	std::vector< double > t;
+	for( size_t r = 0; r < reps; ++r ) {
+		grb::set( y, 0.0 );
+		const double t0 = now_ms();
+		grb::mxv< grb::descriptors::dense >( y, A, x, RealRing() );
+		t.push_back( now_ms() - t0 );
+	}
+	std::sort( t.begin(), t.end() );
+	return t[ t.size() / 2 ];
+}
+
+/**
+ * CPU baseline timing: unpreconditioned CG on the stencil, b = A*1, x0 = 0,
+ * to relative tolerance `tol` or `max_iter`; returns wall milliseconds of the
+ * conjugate_gradient call and its iteration count.
+ */
+REF_API double ref_time_stencil_cg( size_t nx, size_t ny, size_t nz, size_t max_iter, double tol,
+	size_t * iters
+) {
+	if( ensure_init() ) { return -1; }
+	const size_t n = nx * ny * nz;
+	grb::Matrix< double > A( n, n );
+	if( build_stencil( A, nx, ny, nz ) ) { return -1; }
+	grb::Vector< double > x( n ), b( n ), r( n ), u( n ), temp( n ), ones( n );
+	grb::set( ones, 1.0 );
+	grb::set( b, 0.0 );
+	grb::mxv( b, A, ones, RealRing() );
+	grb::set( x, 0.0 );
+	size_t it = 0;
+	double res = 0;
+	const double t0 = now_ms();
+	grb::algorithms::conjugate_gradient( x, A, b, max_iter, tol, it, res, r, u, temp );
+	const double dt = now_ms() - t0;
+	*iters = it;
+	return dt;
+}

@@ -1,0 +1,184 @@
+"""Writes the golden fixtures in this folder from ALP's own reference backend.
+
+Run in a container that holds the reference sources (``make -C oracle ref``
+must have built ``oracle/_ref/libalpref_seq.so``, ALP's sequential
+``reference`` backend behind ``oracle/ref_alp.cpp``):
+
+    python tests/golden/make_golden.py
+
+Every input is generated from a stated seed; outputs are what the reference
+returned. The fixtures are data only (inputs where they are not regenerable
+from a seed, and expected outputs); ``manifest.json`` records each file's
+seed, shape and SHA-256.
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle as O  # noqa: E402
+
+P = O._ptr
+S = ctypes.c_size_t
+D = ctypes.c_double
+
+
+def save(name, manifest, meta, **arrays):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    with open(path, "rb") as f:
+        digest = hashlib.sha256(f.read()).hexdigest()
+    manifest[name] = dict(meta, sha256=digest,
+                          arrays={k: list(np.asarray(v).shape) for k, v in arrays.items()})
+
+
+def main():
+    R = O.ref("seq")
+    manifest = {}
+
+    # -- storage: the reference's CRS and CCS of small stencils -------------------
+    for (nx, ny, nz) in [(3, 3, 3), (4, 5, 6)]:
+        n = nx * ny * nz
+        nnz = S()
+        R.ref_stencil_nnz(nx, ny, nz, ctypes.byref(nnz))
+        out = {}
+        for which, tag in ((0, "crs"), (1, "ccs")):
+            off = np.zeros(n + 1, np.uint64)
+            idx = np.zeros(nnz.value, np.uint32)
+            val = np.zeros(nnz.value)
+            R.ref_stencil_storage(nx, ny, nz, which, P(off), P(idx), P(val))
+            out[f"{tag}_off"], out[f"{tag}_idx"], out[f"{tag}_val"] = off, idx, val
+        save(f"stencil_storage_{nx}x{ny}x{nz}", manifest,
+             dict(kind="storage", sizes=[nx, ny, nz], diag=26.0, offdiag=-1.0), **out)
+
+    # -- mxv / transposed mxv: y <- y + A x, y0 and x ~ U[0,1) ---------------------
+    for (nx, ny, nz, seed) in [(8, 8, 8, 7), (5, 9, 7, 11), (16, 16, 16, 3)]:
+        n = nx * ny * nz
+        rng = np.random.default_rng(seed)
+        x = rng.random(n)
+        y0 = rng.random(n)
+        y = y0.copy()
+        R.ref_stencil_mxv(nx, ny, nz, P(x), P(y))
+        yt = y0.copy()
+        R.ref_stencil_mxv_transposed(nx, ny, nz, P(x), P(yt))
+        save(f"mxv_{nx}x{ny}x{nz}", manifest,
+             dict(kind="mxv", sizes=[nx, ny, nz], seed=seed, rng="numpy default_rng: x, then y0"),
+             x=x, y0=y0, y=y, yt=yt)
+
+    # -- dot ----------------------------------------------------------------------
+    dots = {}
+    for n, seed in [(1, 1), (2, 2), (9, 3), (10, 4), (17, 5), (1000, 6), (4097, 7)]:
+        rng = np.random.default_rng(seed)
+        x = rng.standard_normal(n)
+        y = rng.standard_normal(n)
+        z = D(0.0)
+        R.ref_dot(n, P(x), P(y), ctypes.byref(z))
+        dots[f"x_{n}"], dots[f"y_{n}"], dots[f"z_{n}"] = x, y, np.array([z.value])
+    save("dot", manifest, dict(kind="dot", rng="numpy default_rng(seed): x, y ~ N(0,1)",
+                               cases=[[1, 1], [2, 2], [9, 3], [10, 4], [17, 5], [1000, 6], [4097, 7]]),
+         **dots)
+
+    # -- CG: b = A*1, x0 ~ U[0,1) --------------------------------------------------
+    for (nx, ny, nz, seed, max_iter, tol) in [(8, 8, 8, 42, 1000, 1e-10), (12, 10, 9, 42, 1000, 1e-10),
+                                              (16, 16, 16, 42, 20, 1e-10)]:
+        n = nx * ny * nz
+        x0 = np.random.default_rng(seed).random(n)
+        x = np.zeros(n)
+        it, res = S(), D()
+        rc = R.ref_stencil_cg(nx, ny, nz, P(x0), None, max_iter, tol, P(x), ctypes.byref(it),
+                              ctypes.byref(res))
+        save(f"cg_{nx}x{ny}x{nz}_it{max_iter}", manifest,
+             dict(kind="cg", sizes=[nx, ny, nz], seed=seed, rng="numpy default_rng(seed).random(n)",
+                  b="A*1", max_iter=max_iter, tol=tol),
+             x0=x0, x=x, rc=np.array([rc]), iterations=np.array([it.value]),
+             residual=np.array([res.value]))
+
+    # -- C1: 64^3 stencil, 50 CG iterations (summary only; x is 2 MB) -------------
+    nx = ny = nz = 64
+    n = nx * ny * nz
+    x0 = np.random.default_rng(42).random(n)
+    x = np.zeros(n)
+    it, res = S(), D()
+    rc = R.ref_stencil_cg(nx, ny, nz, P(x0), None, 50, 1e-300, P(x), ctypes.byref(it),
+                          ctypes.byref(res))
+    probe = np.random.default_rng(99).integers(0, n, 64)
+    save("cg_64x64x64_it50", manifest,
+         dict(kind="cg_summary", sizes=[64, 64, 64], seed=42, rng="numpy default_rng(42).random(n)",
+              b="A*1", max_iter=50, tol=1e-300, probe_rng="numpy default_rng(99).integers(0,n,64)"),
+         rc=np.array([rc]), iterations=np.array([it.value]), residual=np.array([res.value]),
+         x_probe_idx=probe, x_probe=x[probe], x_sum=np.array([x.sum()]),
+         x_sha256=np.frombuffer(hashlib.sha256(x.tobytes()).digest(), np.uint8))
+
+    # -- HPCG: b = A*1, x0 = 0, mod-k colours -------------------------------------
+    for (nx, ny, nz, levels, colors, iters, tol) in [
+        (16, 16, 16, 1, 27, 56, 0.0),      # the reference smoke test's default problem
+        (16, 16, 16, 3, 8, 12, 0.0),
+        (8, 12, 16, 2, 5, 10, 0.0),
+        (32, 32, 32, 3, 8, 200, 1e-10),    # to 1e-10 relative residual
+    ]:
+        n = nx * ny * nz
+        x = np.zeros(n)
+        it, res = S(), D()
+        rc = R.ref_hpcg(nx, ny, nz, levels, colors, 1, 1, iters, tol, 1, P(x), ctypes.byref(it),
+                        ctypes.byref(res))
+        lv = S()
+        R.ref_hpcg_levels(nx, ny, nz, levels, ctypes.byref(lv))
+        save(f"hpcg_{nx}x{ny}x{nz}_l{levels}_c{colors}", manifest,
+             dict(kind="hpcg", sizes=[nx, ny, nz], max_levels=levels, colors=colors,
+                  coarse_levels_built=lv.value, presmooth=1, postsmooth=1, max_iter=iters,
+                  tol=tol, b="A*1", x0="0"),
+             x=x, rc=np.array([rc]), iterations=np.array([it.value]), residual=np.array([res.value]))
+
+    # -- PageRank and k-hop on graphs -------------------------------------------
+    n = 10
+    I = np.array(list(range(n)) + [n - 3], np.uint32)
+    J = np.array([(i + 1) % n for i in range(n)] + [n - 1], np.uint32)
+    pr = np.zeros(n)
+    it, q = S(), D()
+    rc = R.ref_pagerank(n, len(I), P(I), P(J), 0.85, 1e-7, 1000, P(pr), ctypes.byref(it),
+                        ctypes.byref(q))
+    save("pagerank_smoke10", manifest,
+         dict(kind="pagerank", graph="ring i->(i+1)%10 plus 7->9 (tests/smoke/hook/small_simple_pagerank.cpp:76-82)",
+              alpha=0.85, conv=1e-7, max_iter=1000),
+         I=I, J=J, pr=pr, rc=np.array([rc]), iterations=np.array([it.value]), quality=np.array([q.value]))
+
+    for scale, seed in [(9, 1), (11, 5)]:
+        n = 1 << scale
+        s, d = O.rmat_edges(scale, 16 * n, seed=seed)
+        I, J = O.symmetrize_dedup(n, s, d)
+        perm = np.random.default_rng(seed + 100).permutation(len(I))
+        I, J = I[perm], J[perm]
+        pr = np.zeros(n)
+        it, q = S(), D()
+        rc = R.ref_pagerank(n, len(I), P(I), P(J), 0.85, 1e-9, 1000, P(pr), ctypes.byref(it),
+                            ctypes.byref(q))
+        crs = O.coo_to_compressed(n, n, I, J, which=0)
+        src = int(np.argmax(np.diff(crs.off.astype(np.int64))))
+        knn = {}
+        for k in (1, 2, 3):
+            out = np.zeros(n, np.uint8)
+            R.ref_knn(n, len(I), P(I), P(J), src, k, P(out))
+            knn[f"knn_{k}"] = (out == 1).astype(np.uint8)
+        save(f"graph_rmat_s{scale}", manifest,
+             dict(kind="graph", generator="orc_rmat_edges scale, 16*2^scale edges, a,b,c=.57,.19,.19",
+                  seed=seed, symmetrised=True, dedup=True, self_loops=False,
+                  input_order="numpy default_rng(seed+100).permutation", alpha=0.85, conv=1e-9,
+                  max_iter=1000, source=src),
+             I=I, J=J, pr=pr, rc=np.array([rc]), iterations=np.array([it.value]),
+             quality=np.array([q.value]), source=np.array([src]), **knn)
+
+    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
+    print(f"wrote {len(manifest)} fixtures")
+
+
+if __name__ == "__main__":
+    main()

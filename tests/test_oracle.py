@@ -1,0 +1,315 @@
+"""Pins the CPU oracle (oracle/alp_oracle.c) before anything is checked against it.
+
+Each oracle function is checked against (1) the golden fixtures written from
+ALP's own sequential reference backend (tests/golden/make_golden.py), bit-exact,
+and (2) something the mathematics fixes independently of the reference:
+closed forms, brute force on tiny inputs, or invariants, chosen so that a
+dropped term, a wrong sign or a transposed operand fails.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+# ---------------------------------------------------------------- storage ----
+
+@pytest.mark.parametrize("sizes", [(3, 3, 3), (4, 5, 6)])
+def test_stencil_storage_matches_reference(golden, sizes):
+    g = golden("stencil_storage_%dx%dx%d" % sizes)
+    A = O.stencil(*sizes)
+    assert np.array_equal(A.off, g["crs_off"])
+    assert np.array_equal(A.idx, g["crs_idx"])
+    assert np.array_equal(A.val, g["crs_val"])
+    # the stencil is symmetric, so the reference's CCS equals its CRS
+    assert np.array_equal(g["ccs_idx"], g["crs_idx"])
+
+
+def closed_form_A1(nx, ny, nz):
+    """b_i = 27 - prod_d c_d(i), c_d = 2 on a face of dimension d, 3 inside (diag 26, off -1)."""
+    def c(n):
+        v = np.full(n, 3.0)
+        v[0] = v[-1] = 2.0
+        return v
+    cx, cy, cz = c(nx), c(ny), c(nz)
+    prod = cz[:, None, None] * cy[None, :, None] * cx[None, None, :]
+    return (27.0 - prod).reshape(-1)
+
+
+@pytest.mark.parametrize("sizes", [(3, 3, 3), (8, 8, 8), (5, 7, 9), (64, 64, 64)])
+def test_stencil_closed_form(sizes):
+    A = O.stencil(*sizes)
+    b = O.spmv(A, np.ones(A.nrows))
+    assert np.array_equal(b, closed_form_A1(*sizes))
+    assert A.nnz == (3 * sizes[0] - 2) * (3 * sizes[1] - 2) * (3 * sizes[2] - 2)
+
+
+def test_stencil_64_norm():
+    # ||A*1||^2 at 64^3 as measured on the reference (interior 0, face 9, edge 15, corner 19)
+    b = O.spmv(O.stencil(64, 64, 64), np.ones(64 ** 3))
+    assert float(b @ b) == 2038472.0
+
+
+def test_stencil_bruteforce_dense():
+    nx, ny, nz = 3, 4, 5
+    A = O.stencil(nx, ny, nz).dense()
+    n = nx * ny * nz
+    coords = [(i % nx, (i // nx) % ny, i // (nx * ny)) for i in range(n)]
+    for i, ci in enumerate(coords):
+        for j, cj in enumerate(coords):
+            near = all(abs(a - b) <= 1 for a, b in zip(ci, cj))
+            want = 26.0 if i == j else (-1.0 if near else 0.0)
+            assert A[i, j] == want
+
+
+def test_coo_to_compressed_reverse_order_and_dense():
+    rng = np.random.default_rng(0)
+    n, m = 13, 9
+    key = rng.choice(n * m, 40, replace=False)
+    I, J = (key // m).astype(np.uint32), (key % m).astype(np.uint32)
+    V = rng.standard_normal(40)
+    crs = O.coo_to_compressed(n, m, I, J, V, which=0)
+    ccs = O.coo_to_compressed(n, m, I, J, V, which=1)
+    D = np.zeros((n, m))
+    D[I, J] = V
+    assert np.array_equal(crs.dense(), D)
+    assert np.array_equal(ccs.dense(), D.T)
+    # within a row, entries appear in reverse input order (matrix.hpp:1918-1930)
+    for i in range(n):
+        inp = [J[k] for k in range(40) if I[k] == i]
+        assert list(crs.idx[crs.off[i]:crs.off[i + 1]]) == inp[::-1]
+
+
+# ------------------------------------------------------------------- SpMV ----
+
+@pytest.mark.parametrize("sizes", [(8, 8, 8), (5, 9, 7), (16, 16, 16)])
+def test_spmv_matches_reference(golden, sizes):
+    g = golden("mxv_%dx%dx%d" % sizes)
+    A = O.stencil(*sizes)
+    assert np.array_equal(O.spmv(A, g["x"], g["y0"]), g["y"])
+    # A is symmetric: the reference's transposed (CCS) product gives the same bits
+    assert np.array_equal(O.spmv(A, g["x"], g["y0"]), g["yt"])
+
+
+def test_spmv_dense_bruteforce_tolerance():
+    rng = np.random.default_rng(1)
+    A = O.stencil(6, 5, 4)
+    x = rng.standard_normal(A.nrows)
+    y0 = rng.standard_normal(A.nrows)
+    ref = y0 + A.dense() @ x
+    assert np.allclose(O.spmv(A, x, y0), ref, rtol=0, atol=1e-12)
+    # an off-by-one column index would not pass: perturb and check detection
+    bad = O.CSR(A.nrows, A.ncols, A.off, np.minimum(A.idx + 1, A.ncols - 1), A.val)
+    assert not np.allclose(O.spmv(bad, x, y0), ref, rtol=0, atol=1e-6)
+
+
+def test_spmv_f32_matches_rounded_f64_within_ulps():
+    rng = np.random.default_rng(2)
+    A = O.stencil(8, 8, 8)
+    x = rng.random(A.nrows).astype(np.float32)
+    y32 = O.spmv(A, x)
+    y64 = O.spmv(A, x.astype(np.float64))
+    assert np.allclose(y32, y64, rtol=1e-5, atol=1e-5)
+
+
+def test_lorland_spmv_bruteforce():
+    rng = np.random.default_rng(3)
+    n = 50
+    D = (rng.random((n, n)) < 0.08)
+    I, J = np.nonzero(D)
+    A = O.coo_to_compressed(n, n, I, J, which=0)
+    x = (rng.random(n) < 0.3).astype(np.uint8)
+    y0 = (rng.random(n) < 0.2).astype(np.uint8)
+    want = y0 | (D.astype(np.uint8) @ x > 0)
+    assert np.array_equal(O.spmv_lorland(A, x, y0), want.astype(np.uint8))
+    mask = (rng.random(n) < 0.5).astype(np.uint8)
+    got = O.spmv_lorland(A, x, y0, mask=mask, invert_mask=True)
+    want2 = np.where(mask == 0, want, y0)
+    assert np.array_equal(got, want2.astype(np.uint8))
+
+
+# -------------------------------------------------------------------- dot ----
+
+def test_dot_matches_reference(golden):
+    g = golden("dot")
+    for n in (1, 2, 9, 10, 17, 1000, 4097):
+        assert O.dot(g[f"x_{n}"], g[f"y_{n}"]) == g[f"z_{n}"][0], n
+
+
+def test_dot_exact_integers_and_closed_forms():
+    for n in (0, 1, 7, 8, 9, 16, 17, 1000):
+        x = np.arange(n, dtype=np.float64)
+        assert O.dot(x, np.ones(n)) == n * (n - 1) / 2
+        assert O.dot(x, x) == sum(i * i for i in range(n))
+    # the reference's dot.cpp closed form: x = 1.5, y = -1 except y[0] = 2.55 -> 2.55 - 1.5(n-1)... sign pinned
+    n = 1024
+    x = np.full(n, 1.5)
+    y = -np.ones(n)
+    assert O.dot(x, y) == -1.5 * n
+
+
+# --------------------------------------------------------------------- CG ----
+
+@pytest.mark.parametrize("name,sizes", [("cg_8x8x8_it1000", (8, 8, 8)),
+                                        ("cg_12x10x9_it1000", (12, 10, 9)),
+                                        ("cg_16x16x16_it20", (16, 16, 16))])
+def test_cg_matches_reference(golden, name, sizes):
+    g = golden(name)
+    A = O.stencil(*sizes)
+    b = O.spmv(A, np.ones(A.nrows))
+    max_iter = int(name.split("_it")[1])
+    rc, x, it, res, _ = O.cg(A, b, g["x0"], max_iter, 1e-10)
+    assert it == g["iterations"][0]
+    assert res == g["residual"][0]
+    assert np.array_equal(x, g["x"])
+    assert (rc == 0) == (g["rc"][0] == 0)
+
+
+def test_cg_64_50_iterations_matches_reference(golden):
+    import hashlib
+    g = golden("cg_64x64x64_it50")
+    A = O.stencil(64, 64, 64)
+    b = O.spmv(A, np.ones(A.nrows))
+    rc, x, it, res, _ = O.cg(A, b, g["x0"] if "x0" in g else np.random.default_rng(42).random(A.nrows),
+                             50, 1e-300)
+    assert it == 50 and rc == 1
+    assert res == g["residual"][0]
+    assert np.array_equal(x[g["x_probe_idx"]], g["x_probe"])
+    assert hashlib.sha256(x.tobytes()).digest() == bytes(g["x_sha256"])
+
+
+def test_cg_against_direct_solve():
+    A = O.stencil(5, 6, 7)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(A.nrows)
+    rc, x, it, res, hist = O.cg(A, b, np.zeros(A.nrows), 500, 1e-12)
+    assert rc == 0
+    xs = np.linalg.solve(A.dense(), b)
+    assert np.max(np.abs(x - xs)) < 1e-10 * np.max(np.abs(xs))
+    # the returned residual is ||r||^2 of the recursively updated r
+    assert res == hist[-1] and res < (1e-12 ** 2) * float(b @ b)
+
+
+# ------------------------------------------------------------------- HPCG ----
+
+@pytest.mark.parametrize("name,sizes,levels,colors", [
+    ("hpcg_16x16x16_l1_c27", (16, 16, 16), 1, 27),
+    ("hpcg_16x16x16_l3_c8", (16, 16, 16), 3, 8),
+    ("hpcg_8x12x16_l2_c5", (8, 12, 16), 2, 5),
+    ("hpcg_32x32x32_l3_c8", (32, 32, 32), 3, 8),
+])
+def test_hpcg_matches_reference(golden, name, sizes, levels, colors):
+    import json
+    import os
+    meta = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "manifest.json")))[name]
+    g = golden(name)
+    cfg = O.hpcg_cfg(*sizes, levels, colors=colors)
+    assert cfg.levels == meta["coarse_levels_built"] + 1
+    A = O.stencil(*sizes)
+    b = O.spmv(A, np.ones(A.nrows))
+    x, it, res, _ = O.hpcg(cfg, b, np.zeros(A.nrows), meta["max_iter"], meta["tol"])
+    assert it == g["iterations"][0]
+    assert res == g["residual"][0]
+    assert np.array_equal(x, g["x"])
+
+
+def test_hpcg_smoke_criterion():
+    # tests/smoke/hpcg.cpp:93: ||x - 1|| < 4e-14 after 56 iterations at 16^3, 27 colours
+    A = O.stencil(16, 16, 16)
+    b = O.spmv(A, np.ones(A.nrows))
+    x, it, res, _ = O.hpcg(O.hpcg_cfg(16, 16, 16, 1, colors=27), b, np.zeros(A.nrows), 56, 0.0)
+    assert np.linalg.norm(x - 1) < 4e-14
+
+
+def test_rbgs_geometric_colouring_is_proper_and_converges():
+    cfg = O.hpcg_cfg(16, 16, 16, 3, geometric=True)
+    A = O.stencil(16, 16, 16)
+    b = O.spmv(A, np.ones(A.nrows))
+    x, it, res, hist = O.hpcg(cfg, b, np.zeros(A.nrows), 100, 1e-10)
+    assert res / np.linalg.norm(b) <= 1e-10 * 1.01
+    assert np.max(np.abs(x - 1)) < 1e-8
+    # parity colouring: no stencil neighbour shares a colour
+    n = A.nrows
+    col = np.array([(i % 16 & 1) + 2 * ((i // 16) % 16 & 1) + 4 * ((i // 256) & 1) for i in range(n)])
+    rows = np.repeat(np.arange(n), np.diff(A.off.astype(np.int64)))
+    off_diag = rows != A.idx
+    assert not np.any(col[rows[off_diag]] == col[A.idx[off_diag]])
+
+
+def test_mg_vcycle_is_symmetric_preconditioner():
+    # a symmetric V-cycle (forward then backward sweeps, R^T prolongation) is a
+    # symmetric operator: <M r1, r2> == <r1, M r2>
+    cfg = O.hpcg_cfg(8, 8, 8, 2, geometric=True)
+    rng = np.random.default_rng(9)
+    r1, r2 = rng.standard_normal(512), rng.standard_normal(512)
+    a = O.mg_vcycle(cfg, r1) @ r2
+    b = r1 @ O.mg_vcycle(cfg, r2)
+    assert abs(a - b) < 1e-12 * abs(a)
+
+
+# ----------------------------------------------------------------- graphs ----
+
+def test_pagerank_smoke_vector(golden):
+    g = golden("pagerank_smoke10")
+    n = 10
+    crs = O.coo_to_compressed(n, n, g["I"], g["J"], which=0)
+    ccs = O.coo_to_compressed(n, n, g["I"], g["J"], which=1)
+    rc, pr, it, q = O.pagerank(crs, ccs, 0.85, 1e-7, 1000)
+    assert rc == 0 and it == g["iterations"][0]
+    assert np.array_equal(pr, g["pr"])
+    # the printed vector of tests/smoke/smoketests.sh:154 (6 significant digits)
+    want = [0.106896, 0.105862, 0.104983, 0.104235, 0.1036, 0.10306, 0.102601, 0.102211,
+            0.0584396, 0.108113]
+    assert np.allclose(pr, want, rtol=5e-6, atol=0)
+
+
+@pytest.mark.parametrize("scale", [9, 11])
+def test_graph_fixtures(golden, scale):
+    g = golden(f"graph_rmat_s{scale}")
+    n = 1 << scale
+    crs = O.coo_to_compressed(n, n, g["I"], g["J"], which=0)
+    ccs = O.coo_to_compressed(n, n, g["I"], g["J"], which=1)
+    rc, pr, it, q = O.pagerank(crs, ccs, 0.85, 1e-9, 1000)
+    assert it == g["iterations"][0] and q == g["quality"][0]
+    assert np.array_equal(pr, g["pr"])
+    # PageRank invariant: entries positive and sum to 1 up to rounding
+    assert abs(pr.sum() - 1.0) < 1e-9 and pr.min() > 0
+    src = int(g["source"][0])
+    lev = O.bfs_levels(crs, src)
+    for k in (1, 2, 3):
+        assert np.array_equal(O.khop(ccs, src, k), g[f"knn_{k}"])
+        assert np.array_equal(((lev >= 0) & (lev <= k)).astype(np.uint8), g[f"knn_{k}"])
+
+
+def test_bfs_levels_bruteforce():
+    rng = np.random.default_rng(4)
+    n = 80
+    s, d = rng.integers(0, n, 160), rng.integers(0, n, 160)
+    I, J = O.symmetrize_dedup(n, s.astype(np.uint32), d.astype(np.uint32))
+    A = O.coo_to_compressed(n, n, I, J, which=0)
+    D = A.dense() > 0
+    lev = O.bfs_levels(A, 0)
+    reach = np.zeros(n, bool)
+    reach[0] = True
+    want = np.full(n, -1)
+    want[0] = 0
+    for h in range(1, n):
+        nxt = reach | (D.astype(int) @ reach.astype(int) > 0)
+        want[nxt & ~reach] = h
+        if (nxt == reach).all():
+            break
+        reach = nxt
+    assert np.array_equal(lev, want)
+
+
+def test_rmat_generator_properties():
+    s, d = O.rmat_edges(10, 200000, seed=1)
+    s2, d2 = O.rmat_edges(10, 200000, seed=1)
+    assert np.array_equal(s, s2) and np.array_equal(d, d2)
+    assert s.max() < 1024 and d.max() < 1024
+    # top-level quadrant frequencies: a = .57, b = .19, c = .19, d = .05
+    top_s, top_d = s >> 9, d >> 9
+    f = [np.mean((top_s == i) & (top_d == j)) for i, j in ((0, 0), (0, 1), (1, 0), (1, 1))]
+    assert np.allclose(f, [0.57, 0.19, 0.19, 0.05], atol=0.005)
+    s3, _ = O.rmat_edges(10, 1000, seed=2)
+    assert not np.array_equal(s3, s[:1000])
