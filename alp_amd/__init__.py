@@ -1,0 +1,313 @@
+"""alp_amd -- Python binding of the MI355X-native `hip` backend for ALP/GraphBLAS.
+
+The product is ``libalp_hip.so`` (hand-written gfx950 kernels behind the C ABI
+declared in ``include/alp_hip.h``). This module binds that ABI with ctypes for
+tests, benchmarks and Python callers. It never falls back to a CPU path: if the
+library is missing, importing :func:`lib` raises.
+
+Objects mirror ALP's containers (include/graphblas/base/{vector,matrix}.hpp):
+:class:`Vector` owns a device array, :class:`Matrix` owns device CRS/CCS
+storage; the functions named after grb:: primitives (``mxv``, ``vxm``, ``dot``,
+...) follow the reference's argument order and return codes (grb::RC values).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libalp_hip.so")
+
+# grb::RC (include/graphblas/rc.hpp:47-155)
+SUCCESS, PANIC, OUTOFMEM, MISMATCH, OVERLAP, OVERFLW, UNSUPPORTED, ILLEGAL, FAILED = range(9)
+RC_NAMES = ["SUCCESS", "PANIC", "OUTOFMEM", "MISMATCH", "OVERLAP", "OVERFLW", "UNSUPPORTED",
+            "ILLEGAL", "FAILED"]
+
+# semirings (alp_semiring_t)
+PLUS_TIMES_F64, PLUS_TIMES_F32, LOR_LAND_BOOL, PLUS_PATTERN_F64, MIN_PLUS_F64 = range(5)
+
+# descriptors (include/graphblas/descriptors.hpp:63-267)
+INVERT_MASK = 1
+TRANSPOSE_MATRIX = 2
+STRUCTURAL = 8
+DENSE = 16
+ADD_IDENTITY = 32
+SAFE_OVERLAP = 1024
+OVERWRITE = 1 << 20
+
+VAL_F64, VAL_F32, VAL_PATTERN = 0, 1, 2
+MAT_WITH_CCS, MAT_WITH_F32 = 1, 2
+
+
+class AlpError(RuntimeError):
+    def __init__(self, rc: int, what: str):
+        self.rc = rc
+        super().__init__(f"{what} returned {RC_NAMES[rc] if 0 <= rc < 9 else rc}")
+
+
+def build(quiet: bool = True) -> None:
+    """Compile libalp_hip.so for gfx950 in-tree (hipcc cross-compiles without a GPU)."""
+    out = subprocess.run(["make", "-C", HERE, f"-j{min(16, os.cpu_count() or 4)}"],
+                         capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("libalp_hip build failed:\n" + out.stdout[-4000:] + out.stderr[-8000:])
+    if not quiet:
+        print(out.stdout)
+
+
+_P = ctypes.c_void_p
+_S = ctypes.c_size_t
+_D = ctypes.c_double
+_I = ctypes.c_int
+_U = ctypes.c_uint
+_RC = ctypes.c_int
+
+# every exported entry point: name -> (restype, argtypes)
+SIGNATURES = {
+    "alp_hip_init": (_RC, [_I]),
+    "alp_hip_finalize": (_RC, []),
+    "alp_hip_set_stream": (_RC, [_P]),
+    "alp_hip_sync": (_RC, []),
+    "alp_hip_version": (ctypes.c_char_p, []),
+    "alp_hip_device_count": (_I, []),
+    "alp_hip_malloc": (_RC, [ctypes.POINTER(_P), _S]),
+    "alp_hip_free": (_RC, [_P]),
+    "alp_hip_memcpy_h2d": (_RC, [_P, _P, _S]),
+    "alp_hip_memcpy_d2h": (_RC, [_P, _P, _S]),
+    "alp_hip_memcpy_d2d": (_RC, [_P, _P, _S]),
+    "alp_hip_matrix_create_crs": (_RC, [ctypes.POINTER(_P), _S, _S, _S, _P, _P, _P, _I, _U]),
+    "alp_hip_matrix_create_coo": (_RC, [ctypes.POINTER(_P), _S, _S, _S, _P, _P, _P, _I, _U]),
+    "alp_hip_matrix_stencil27": (_RC, [ctypes.POINTER(_P), _S, _S, _S, _D, _D, _U]),
+    "alp_hip_matrix_destroy": (_RC, [_P]),
+    "alp_hip_matrix_info": (_RC, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S)]),
+    "alp_hip_matrix_download": (_RC, [_P, _I, _P, _P, _P]),
+    "alp_hip_matrix_set_kernel": (_RC, [_P, _I, _I]),
+    "alp_hip_matrix_get_kernel": (_RC, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "alp_hip_matrix_autotune": (_RC, [_P]),
+    "alp_hip_mxv": (_RC, [_P, _I, _U, _P, _P, _P]),
+    "alp_hip_vxm": (_RC, [_P, _I, _U, _P, _P, _P]),
+    "alp_hip_dot_f64": (_RC, [_S, _P, _P, ctypes.POINTER(_D)]),
+    "alp_hip_dot_f32": (_RC, [_S, _P, _P, ctypes.POINTER(_D)]),
+    "alp_hip_set_f64": (_RC, [_S, _P, _D]),
+    "alp_hip_set_f32": (_RC, [_S, _P, ctypes.c_float]),
+    "alp_hip_set_u8": (_RC, [_S, _P, ctypes.c_uint8]),
+    "alp_hip_axpy_f64": (_RC, [_S, _D, _P, _P]),
+    "alp_hip_xpay_f64": (_RC, [_S, _D, _P, _P, _P]),
+    "alp_hip_ewise_f64": (_RC, [_S, _I, _P, _P, _P]),
+    "alp_hip_scale_f64": (_RC, [_S, _D, _P]),
+    "alp_hip_sum_f64": (_RC, [_S, _P, ctypes.POINTER(_D)]),
+    "alp_hip_convert_f64_to_f32": (_RC, [_S, _P, _P]),
+    "alp_hip_convert_f32_to_f64": (_RC, [_S, _P, _P]),
+    "alp_hip_cg": (_RC, [_P, _P, _P, _S, _D, _I, ctypes.POINTER(_S), ctypes.POINTER(_D)]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded libalp_hip.so (raises if it was not built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `make -C alp_amd` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != SUCCESS:
+        raise AlpError(rc, what)
+
+
+def init(device: int = 0) -> None:
+    check(lib().alp_hip_init(device), "alp_hip_init")
+
+
+def sync() -> None:
+    check(lib().alp_hip_sync(), "alp_hip_sync")
+
+
+def device_count() -> int:
+    return int(lib().alp_hip_device_count())
+
+
+# ---------------------------------------------------------------------------
+# containers
+# ---------------------------------------------------------------------------
+
+class Vector:
+    """A dense device vector (grb::Vector<D, hip>), owning hipMalloc'd memory."""
+
+    def __init__(self, n: int, dtype=np.float64, fill=None):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = _P()
+        check(lib().alp_hip_malloc(ctypes.byref(p), max(1, self.n * self.dtype.itemsize)), "alp_hip_malloc")
+        self.ptr = p.value
+        if fill is not None:
+            self.set(fill)
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "Vector":
+        a = np.ascontiguousarray(a)
+        v = cls(a.size, a.dtype)
+        v.upload(a)
+        return v
+
+    @property
+    def nbytes(self) -> int:
+        return self.n * self.dtype.itemsize
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(lib().alp_hip_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes), "memcpy_h2d")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        check(lib().alp_hip_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes), "memcpy_d2h")
+        return out
+
+    def set(self, v) -> None:
+        L = lib()
+        if self.dtype == np.float64:
+            check(L.alp_hip_set_f64(self.n, self.ptr, float(v)), "set")
+        elif self.dtype == np.float32:
+            check(L.alp_hip_set_f32(self.n, self.ptr, float(v)), "set")
+        elif self.dtype.itemsize == 1:
+            check(L.alp_hip_set_u8(self.n, self.ptr, int(v)), "set")
+        else:
+            self.upload(np.full(self.n, v, self.dtype))
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().alp_hip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Matrix:
+    """Device-resident CRS (+ optional CCS) storage: grb::Matrix<D, hip>."""
+
+    def __init__(self, handle: int):
+        self.h = handle
+        r, c, z = _S(), _S(), _S()
+        check(lib().alp_hip_matrix_info(self.h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z)),
+              "matrix_info")
+        self.nrows, self.ncols, self.nnz = r.value, c.value, z.value
+
+    @classmethod
+    def stencil27(cls, nx, ny, nz, diag=26.0, offdiag=-1.0, flags=0) -> "Matrix":
+        h = _P()
+        check(lib().alp_hip_matrix_stencil27(ctypes.byref(h), nx, ny, nz, diag, offdiag, flags),
+              "matrix_stencil27")
+        return cls(h.value)
+
+    @classmethod
+    def from_crs(cls, nrows, ncols, off, idx, val=None, value_type=None, flags=0) -> "Matrix":
+        off = np.ascontiguousarray(off, np.uint64)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        if value_type is None:
+            value_type = VAL_PATTERN if val is None else (VAL_F32 if np.asarray(val).dtype == np.float32 else VAL_F64)
+        vp = None
+        if val is not None:
+            val = np.ascontiguousarray(val, np.float32 if value_type == VAL_F32 else np.float64)
+            vp = val.ctypes.data
+        h = _P()
+        check(lib().alp_hip_matrix_create_crs(ctypes.byref(h), nrows, ncols, int(off[-1]),
+                                              off.ctypes.data, idx.ctypes.data, vp, value_type, flags),
+              "matrix_create_crs")
+        return cls(h.value)
+
+    @classmethod
+    def from_coo(cls, nrows, ncols, I, J, V=None, value_type=None, flags=0) -> "Matrix":
+        I = np.ascontiguousarray(I, np.uint32)
+        J = np.ascontiguousarray(J, np.uint32)
+        if value_type is None:
+            value_type = VAL_PATTERN if V is None else (VAL_F32 if np.asarray(V).dtype == np.float32 else VAL_F64)
+        vp = None
+        if V is not None:
+            V = np.ascontiguousarray(V, np.float32 if value_type == VAL_F32 else np.float64)
+            vp = V.ctypes.data
+        h = _P()
+        check(lib().alp_hip_matrix_create_coo(ctypes.byref(h), nrows, ncols, len(I), I.ctypes.data,
+                                              J.ctypes.data, vp, value_type, flags),
+              "matrix_create_coo")
+        return cls(h.value)
+
+    def download(self, which=0, with_values=True, value_dtype=np.float64):
+        nmaj = self.nrows if which == 0 else self.ncols
+        off = np.zeros(nmaj + 1, np.uint64)
+        idx = np.zeros(max(self.nnz, 1), np.uint32)
+        val = np.zeros(max(self.nnz, 1), value_dtype) if with_values else None
+        check(lib().alp_hip_matrix_download(self.h, which, off.ctypes.data, idx.ctypes.data,
+                                            val.ctypes.data if with_values else None), "matrix_download")
+        return off, idx[: self.nnz], (val[: self.nnz] if with_values else None)
+
+    def set_kernel(self, kernel: int, lanes: int = 0) -> None:
+        check(lib().alp_hip_matrix_set_kernel(self.h, kernel, lanes), "matrix_set_kernel")
+
+    def get_kernel(self):
+        k, l = _I(), _I()
+        check(lib().alp_hip_matrix_get_kernel(self.h, ctypes.byref(k), ctypes.byref(l)), "matrix_get_kernel")
+        return k.value, l.value
+
+    def autotune(self):
+        check(lib().alp_hip_matrix_autotune(self.h), "matrix_autotune")
+        return self.get_kernel()
+
+    def free(self) -> None:
+        if getattr(self, "h", None):
+            lib().alp_hip_matrix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# primitives (grb:: names)
+# ---------------------------------------------------------------------------
+
+def mxv(y: Vector, A: Matrix, x: Vector, ring: int = PLUS_TIMES_F64, descr: int = 0,
+        mask: Vector | None = None) -> int:
+    """y (+)= A (x) x (grb::mxv, reference/blas2.hpp:1946); returns the grb::RC code."""
+    return lib().alp_hip_mxv(A.h, ring, descr, y.ptr, x.ptr, mask.ptr if mask is not None else None)
+
+
+def vxm(y: Vector, x: Vector, A: Matrix, ring: int = PLUS_TIMES_F64, descr: int = 0,
+        mask: Vector | None = None) -> int:
+    """y (+)= x (x) A (grb::vxm, reference/blas2.hpp:1540-1694); returns the grb::RC code."""
+    return lib().alp_hip_vxm(A.h, ring, descr, y.ptr, x.ptr, mask.ptr if mask is not None else None)
+
+
+def dot(x: Vector, y: Vector, z: float = 0.0) -> float:
+    """z + dot(x, y) with a deterministic reduction (grb::dot, reference/blas1.hpp:9520)."""
+    out = _D(z)
+    f = lib().alp_hip_dot_f32 if x.dtype == np.float32 else lib().alp_hip_dot_f64
+    check(f(x.n, x.ptr, y.ptr, ctypes.byref(out)), "dot")
+    return out.value
+
+
+def cg(A: Matrix, b: Vector, x: Vector, max_iter: int, tol: float, precision: int = 0):
+    """grb::algorithms::conjugate_gradient on the device; returns (rc, iterations, ||r||^2)."""
+    it, res = _S(), _D()
+    rc = lib().alp_hip_cg(A.h, b.ptr, x.ptr, max_iter, tol, precision, ctypes.byref(it), ctypes.byref(res))
+    return rc, it.value, res.value

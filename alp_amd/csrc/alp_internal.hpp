@@ -1,0 +1,172 @@
+// Internal definitions shared by the HIP translation units of libalp_hip.so.
+// gfx950 only: 64-wide wavefronts, 256 CUs in 8 XCDs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+
+#include "../../include/alp_hip.h"
+
+#define ALP_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace alp {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kNumXcd = 8;
+
+// ---- error handling ---------------------------------------------------------
+
+void report_hip_error(hipError_t e, const char *what, const char *file, int line);
+
+#define ALP_HIP_TRY(call)                                                        \
+	do {                                                                         \
+		hipError_t alp_e_ = (call);                                              \
+		if (alp_e_ != hipSuccess) {                                              \
+			::alp::report_hip_error(alp_e_, #call, __FILE__, __LINE__);          \
+			return alp_e_ == hipErrorOutOfMemory ? ALP_OUTOFMEM : ALP_PANIC;     \
+		}                                                                        \
+	} while (0)
+
+#define ALP_TRY(expr)                                                            \
+	do {                                                                         \
+		alp_rc_t alp_rc_ = (expr);                                               \
+		if (alp_rc_ != ALP_SUCCESS) return alp_rc_;                              \
+	} while (0)
+
+#define ALP_LAUNCH_CHECK() ALP_HIP_TRY(hipGetLastError())
+
+// ---- runtime context (grb::init<hip>) ---------------------------------------
+
+struct Context {
+	int device = -1;
+	hipStream_t own_stream = nullptr;
+	hipStream_t stream = nullptr;   // current stream (own or caller's)
+	int num_cus = 0;
+	// small device scratch for reductions: partial sums, tickets and results
+	double *d_partials = nullptr;   // kMaxPartials doubles
+	unsigned *d_tickets = nullptr;  // 64 counters, kept at zero between uses
+	double *d_scalars = nullptr;    // 64 result slots
+	double *h_pinned = nullptr;     // 64 pinned host doubles
+};
+constexpr int kMaxPartials = 4096;
+
+Context &ctx();
+alp_rc_t ensure_init();
+inline hipStream_t stream() { return ctx().stream; }
+
+// number of blocks that fill the chip `per_cu` deep with kBlock-thread blocks
+inline unsigned fill_blocks(int per_cu) {
+	return static_cast<unsigned>(ctx().num_cus > 0 ? ctx().num_cus * per_cu : 256 * per_cu);
+}
+
+// ---- device helpers ---------------------------------------------------------
+
+// Blocks are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8). Remap
+// so that each XCD walks one contiguous range of logical blocks: neighbouring
+// rows then share that XCD's L2 (stencil and graph locality).
+__device__ __forceinline__ unsigned xcd_logical_block(unsigned b, unsigned nb) {
+	const unsigned xcd = b % kNumXcd, local = b / kNumXcd;
+	const unsigned q = nb / kNumXcd, rem = nb % kNumXcd;
+	const unsigned start = xcd * q + (xcd < rem ? xcd : rem);
+	return start + local;
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_xor(T v, int mask) {
+	return __shfl_xor(v, mask, kWave);
+}
+
+// sum over the aligned group of G lanes that holds this lane
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+	for (int s = G / 2; s > 0; s >>= 1) v += shfl_xor(v, s);
+	return v;
+}
+
+// deterministic block-wide sum (fixed tree); every thread gets the result
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *lds /* >= kBlock / kWave entries */) {
+	v = group_sum<kWave>(v);
+	const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+	__syncthreads();
+	if (lane == 0) lds[wave] = v;
+	__syncthreads();
+	T t = T(0);
+#pragma unroll
+	for (int w = 0; w < kBlock / kWave; ++w) t += lds[w];
+	return t;
+}
+
+// Last-block-done ticket: returns true in exactly one block (the last to arrive),
+// after all blocks' prior global writes are visible to it. The counter is reset
+// to zero by that block so no memset is needed between uses (graph friendly).
+__device__ __forceinline__ bool last_block(unsigned *ticket, bool *lds_flag) {
+	__threadfence();
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		const unsigned t = atomicAdd(ticket, 1u);
+		*lds_flag = (t == gridDim.x - 1);
+		if (*lds_flag) atomicExch(ticket, 0u);
+	}
+	__syncthreads();
+	if (*lds_flag) __threadfence();
+	return *lds_flag;
+}
+
+// fixed-order sum of n partials by one block (n <= kMaxPartials)
+template <typename T>
+__device__ __forceinline__ T block_sum_partials(const T *p, unsigned n, T *lds) {
+	T v = T(0);
+	for (unsigned i = threadIdx.x; i < n; i += kBlock) v += __builtin_nontemporal_load(p + i);
+	return block_sum(v, lds);
+}
+
+// ---- matrices ---------------------------------------------------------------
+
+struct Storage {
+	uint64_t *off = nullptr;  // nmajor + 1
+	uint32_t *idx = nullptr;  // nnz
+	void *val = nullptr;      // nnz values of the matrix type, or null (pattern)
+	float *val32 = nullptr;   // optional f32 copy of f64 values
+	size_t nmajor = 0;
+	size_t nminor = 0;
+	// CSR-stream row blocks: row_blocks[b] .. row_blocks[b+1] is block b's row range
+	uint32_t *row_blocks = nullptr;
+	size_t n_row_blocks = 0;
+	// statistics for kernel selection
+	double avg_len = 0;
+	size_t max_len = 0;
+	int kernel = 0;  // 0 auto, 1 vector, 2 stream
+	int lanes = 0;
+};
+
+}  // namespace alp
+
+struct alp_hip_matrix_s {
+	size_t nrows = 0, ncols = 0, nnz = 0;
+	int value_type = ALP_VAL_F64;
+	unsigned flags = 0;
+	alp::Storage crs, ccs;
+};
+
+namespace alp {
+
+alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32);
+void storage_free(Storage &S);
+alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A);
+
+// SpMV entry used by mxv/vxm and by the solvers (one storage, one semiring).
+alp_rc_t spmv(const Storage &S, int value_type, alp_semiring_t ring, unsigned descr, void *y,
+              const void *x, const uint8_t *mask, hipStream_t s);
+
+// device scalars shared with the host
+alp_rc_t read_scalars(double *h, const double *d, int count);
+
+}  // namespace alp

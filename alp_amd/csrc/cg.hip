@@ -1,0 +1,457 @@
+// Device-resident conjugate gradients: grb::algorithms::conjugate_gradient
+// (algorithms/conjugate_gradient.hpp:191-484, unpreconditioned).
+//
+// The reference issues, per iteration, set + mxv + dot + apply + eWiseMul + foldr
+// + foldl + dot + foldr + foldr (:387-471), each a full pass over n with a host
+// round trip for every scalar. Here one iteration is three kernels and no host
+// round trip; all scalars live in a device state block:
+//   k_cg_spmv_dot  temp = A u and the partial sums of temp.u in one pass; the
+//                  last block to finish forms alpha = sigma / (temp.u)
+//   k_cg_update    x += alpha u, r -= alpha temp and the partials of r.r; the last
+//                  block forms the residual, the stopping test (||r|| < tol ||b||,
+//                  :378,437) and beta = r.r / sigma
+//   k_cg_dir       u = r + beta u
+// Every reduction has a fixed tree (fixed grid, fixed per-thread strides, the
+// partials folded in index order), so results are run-to-run deterministic.
+// Iterations are enqueued in batches replayed from one captured hipGraph; the
+// kernels turn into no-ops once the state says done, so the host only checks
+// between batches and the iteration count is still exact.
+//
+// Mixed precision (precision = 1): FP32 CG iterations on the FP32 copy of A
+// inside FP64 iterative refinement: r = b - A x in FP64, the inner solve on
+// r/||r|| in FP32 to a relative 1e-5, then x += ||r|| d in FP64.
+#include "spmv_device.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace alp {
+
+struct CgState {
+	double sigma;      // r.r of the current residual
+	double alpha;
+	double ratio;      // beta / sigma of the direction update
+	double rr;         // last r.r (the reference's `residual`)
+	double tol_abs;    // tol * sqrt(b.b)
+	double bnorm2;
+	double r0norm;     // sqrt of the first r.r (inner solves)
+	double pad;
+	unsigned long long iter;
+	unsigned long long max_iter;
+	int done;
+	int converged;
+	int pad2[2];
+};
+
+template <typename T>
+struct CgArgs {
+	size_t n;
+	const uint64_t *off;
+	const uint32_t *idx;
+	const T *val;
+	T *x, *r, *u, *temp;
+	const T *b;
+	double *partials;
+	unsigned *ticket;
+	CgState *st;
+};
+
+// ---- kernels ------------------------------------------------------------------
+
+template <int G, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	if (a.st->done) return;
+	constexpr unsigned kRows = kBlock / G;
+	const size_t nchunks = (a.n + kRows - 1) / kRows;
+	const WorkSplit w = xcd_split(nchunks);
+	const int lane = threadIdx.x % G;
+	double part = 0.0;
+	for (size_t c = w.first; c < w.last; c += w.step) {
+		const size_t row = c * kRows + threadIdx.x / G;
+		if (row >= a.n) continue;
+		const uint64_t beg = a.off[row], end = a.off[row + 1];
+		T acc = T(0);
+		if (beg != end) acc = row_gather<G, PlusTimes<T>>(beg, end, lane, a.idx, a.val, a.u);
+		if (lane == 0) {
+			a.temp[row] = acc;
+			part += (double)acc * (double)a.u[row];
+		}
+	}
+	part = block_sum(part, lds);
+	if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
+	if (last_block(a.ticket, &is_last)) {
+		const double pAp = block_sum_partials(a.partials, gridDim.x, lds);
+		if (threadIdx.x == 0) a.st->alpha = a.st->sigma / pAp;
+	}
+}
+
+// generic-SpMV path: only the dot (temp.u) and alpha
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_dot_alpha(CgArgs<T> a) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	if (a.st->done) return;
+	double part = 0.0;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (size_t)gridDim.x * kBlock)
+		part += (double)a.temp[i] * (double)a.u[i];
+	part = block_sum(part, lds);
+	if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
+	if (last_block(a.ticket, &is_last)) {
+		const double pAp = block_sum_partials(a.partials, gridDim.x, lds);
+		if (threadIdx.x == 0) a.st->alpha = a.st->sigma / pAp;
+	}
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	if (a.st->done) return;
+	const T alpha = (T)a.st->alpha;
+	double part = 0.0;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (size_t)gridDim.x * kBlock) {
+		a.x[i] = alpha * a.u[i] + a.x[i];
+		const T ri = a.r[i] - alpha * a.temp[i];
+		a.r[i] = ri;
+		part += (double)ri * (double)ri;
+	}
+	part = block_sum(part, lds);
+	if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
+	if (last_block(a.ticket, &is_last)) {
+		const double rr = block_sum_partials(a.partials, gridDim.x, lds);
+		if (threadIdx.x == 0) {
+			CgState *st = a.st;
+			st->rr = rr;
+			st->iter += 1;
+			const bool conv = std::sqrt(rr) < st->tol_abs;
+			if (conv || st->iter >= st->max_iter) {
+				st->done = 1;
+				st->converged = conv ? 1 : 0;
+			} else {
+				st->ratio = rr / st->sigma;
+				st->sigma = rr;
+			}
+		}
+	}
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs<T> a) {
+	if (a.st->done) return;
+	const T ratio = (T)a.st->ratio;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (size_t)gridDim.x * kBlock)
+		a.u[i] = ratio * a.u[i] + a.r[i];
+}
+
+// prelude: r = b - temp (temp = A x), u = r, sigma = r.r, bnorm2 = bb.bb where bb
+// is the right-hand side used for the stopping rule
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs<T> a, const double *bb_f64,
+                                                   const T *bb_t, double tol, size_t max_iter) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	double prr = 0.0, pbb = 0.0;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (size_t)gridDim.x * kBlock) {
+		const T ri = a.b[i] - a.temp[i];
+		a.r[i] = ri;
+		a.u[i] = ri;
+		prr += (double)ri * (double)ri;
+		const double bi = bb_f64 ? bb_f64[i] : (double)bb_t[i];
+		pbb += bi * bi;
+	}
+	prr = block_sum(prr, lds);
+	pbb = block_sum(pbb, lds);
+	if (threadIdx.x == 0) {
+		a.partials[blockIdx.x] = prr;
+		a.partials[gridDim.x + blockIdx.x] = pbb;
+	}
+	if (last_block(a.ticket, &is_last)) {
+		const double rr = block_sum_partials(a.partials, gridDim.x, lds);
+		const double bn = block_sum_partials(a.partials + gridDim.x, gridDim.x, lds);
+		if (threadIdx.x == 0) {
+			CgState *st = a.st;
+			st->sigma = rr;
+			st->rr = rr;
+			st->bnorm2 = bn;
+			st->tol_abs = tol * std::sqrt(bn);
+			st->r0norm = std::sqrt(rr);
+			st->iter = 0;
+			st->max_iter = max_iter;
+			st->done = 0;
+			st->converged = 0;
+			st->ratio = 0.0;
+			st->alpha = 0.0;
+		}
+	}
+}
+
+// ---- iterative refinement helpers (FP64 outer loop) -----------------------------
+
+struct IrState {
+	double rnorm;       // ||b - A x||
+	double bnorm;
+	double inv_rnorm;
+	int done;
+	int pad;
+};
+
+// r64 = b - t64 (t64 = A x), partial ||r||^2 -> last block: rnorm, stop test
+__global__ __launch_bounds__(kBlock) void k_ir_residual(size_t n, const double *b, const double *t,
+                                                       double *r, double *partials, unsigned *ticket,
+                                                       IrState *ir, double tol, int first) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	double p = 0.0, pb = 0.0;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+		const double ri = b[i] - t[i];
+		r[i] = ri;
+		p += ri * ri;
+		if (first) pb += b[i] * b[i];
+	}
+	p = block_sum(p, lds);
+	pb = block_sum(pb, lds);
+	if (threadIdx.x == 0) {
+		partials[blockIdx.x] = p;
+		partials[gridDim.x + blockIdx.x] = pb;
+	}
+	if (last_block(ticket, &is_last)) {
+		const double rr = block_sum_partials(partials, gridDim.x, lds);
+		const double bb = block_sum_partials(partials + gridDim.x, gridDim.x, lds);
+		if (threadIdx.x == 0) {
+			if (first) ir->bnorm = std::sqrt(bb);
+			ir->rnorm = std::sqrt(rr);
+			ir->inv_rnorm = ir->rnorm > 0 ? 1.0 / ir->rnorm : 0.0;
+			ir->done = ir->rnorm < tol * ir->bnorm ? 1 : 0;
+		}
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_ir_scale_down(size_t n, const double *r, float *r32,
+                                                         float *x32, const IrState *ir) {
+	const double s = ir->inv_rnorm;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+		r32[i] = (float)(r[i] * s);
+		x32[i] = 0.0f;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_ir_correct(size_t n, double *x, const float *d32,
+                                                      const IrState *ir) {
+	const double s = ir->rnorm;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		x[i] = s * (double)d32[i] + x[i];
+}
+
+// ---- host driver ----------------------------------------------------------------
+
+static unsigned vec_grid(size_t n) {
+	size_t g = (n + kBlock - 1) / kBlock;
+	const size_t cap = fill_blocks(4);
+	if (g > cap) g = cap;
+	if (g < 1) g = 1;
+	if (g > kMaxPartials / 2) g = kMaxPartials / 2;
+	return static_cast<unsigned>(g);
+}
+
+int default_lanes(const Storage &S);
+
+template <typename T>
+struct CgRunner {
+	const Storage &S;
+	CgArgs<T> a;
+	int G;
+	bool fused;
+	unsigned gspmv, gvec;
+
+	CgRunner(const Storage &S_, CgArgs<T> a_) : S(S_), a(a_) {
+		G = S.lanes ? S.lanes : default_lanes(S);
+		fused = S.kernel != 2 && G >= 2;
+		const size_t nchunks = (a.n + (kBlock / G) - 1) / (kBlock / G);
+		size_t g = fill_blocks(8);
+		if (g > nchunks) g = nchunks;
+		if (g < 1) g = 1;
+		gspmv = static_cast<unsigned>(g);
+		gvec = vec_grid(a.n);
+	}
+
+	alp_rc_t iteration(hipStream_t s) {
+		if (fused) {
+#define ALP_CG_CASE(GG)                                                                     \
+	case GG:                                                                                \
+		hipLaunchKernelGGL((k_cg_spmv_dot<GG, T>), dim3(gspmv), dim3(kBlock), 0, s, a);     \
+		break;
+			switch (G) {
+				ALP_CG_CASE(2)
+				ALP_CG_CASE(4)
+				ALP_CG_CASE(8)
+				ALP_CG_CASE(16)
+				ALP_CG_CASE(32)
+				ALP_CG_CASE(64)
+				default: return ALP_PANIC;
+			}
+#undef ALP_CG_CASE
+		} else {
+			ALP_TRY(spmv(S, std::is_same<T, float>::value ? ALP_VAL_F32 : ALP_VAL_F64,
+			             std::is_same<T, float>::value ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64,
+			             ALP_DESC_OVERWRITE, a.temp, a.u, nullptr, s));
+			hipLaunchKernelGGL(k_cg_dot_alpha<T>, dim3(gvec), dim3(kBlock), 0, s, a);
+		}
+		hipLaunchKernelGGL(k_cg_update<T>, dim3(gvec), dim3(kBlock), 0, s, a);
+		hipLaunchKernelGGL(k_cg_dir<T>, dim3(gvec), dim3(kBlock), 0, s, a);
+		ALP_LAUNCH_CHECK();
+		return ALP_SUCCESS;
+	}
+
+	// run until done; batches of `batch` iterations replayed from a captured graph
+	alp_rc_t run(hipStream_t s, CgState *h_state) {
+		const int batch = 16;
+		hipGraphExec_t exec = nullptr;
+		hipGraph_t graph = nullptr;
+		bool use_graph = false;
+		if (s != nullptr && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+			alp_rc_t rc = ALP_SUCCESS;
+			for (int i = 0; i < batch && rc == ALP_SUCCESS; ++i) rc = iteration(s);
+			if (hipStreamEndCapture(s, &graph) == hipSuccess && rc == ALP_SUCCESS &&
+			    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
+				use_graph = true;
+			(void)hipGetLastError();
+		}
+		alp_rc_t rc = ALP_SUCCESS;
+		for (;;) {
+			if (use_graph) {
+				if (hipGraphLaunch(exec, s) != hipSuccess) { rc = ALP_PANIC; break; }
+			} else {
+				for (int i = 0; i < batch && rc == ALP_SUCCESS; ++i) rc = iteration(s);
+				if (rc) break;
+			}
+			if (hipMemcpyAsync(h_state, a.st, sizeof(CgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
+			    hipStreamSynchronize(s) != hipSuccess) {
+				rc = ALP_PANIC;
+				break;
+			}
+			if (h_state->done) break;
+		}
+		if (exec) hipGraphExecDestroy(exec);
+		if (graph) hipGraphDestroy(graph);
+		return rc;
+	}
+};
+
+struct DeviceBuffers {
+	std::vector<void *> ptrs;
+	~DeviceBuffers() {
+		for (void *p : ptrs) hipFree(p);
+	}
+	template <typename T>
+	alp_rc_t get(T **p, size_t count) {
+		void *q = nullptr;
+		ALP_HIP_TRY(hipMalloc(&q, sizeof(T) * (count ? count : 1)));
+		ptrs.push_back(q);
+		*p = static_cast<T *>(q);
+		return ALP_SUCCESS;
+	}
+};
+
+static alp_rc_t cg_fp64(alp_hip_matrix_t A, const double *b, double *x, size_t max_iter,
+                        double tol, size_t *iters, double *residual) {
+	const size_t n = A->nrows;
+	hipStream_t s = stream();
+	DeviceBuffers buf;
+	double *r, *u, *temp;
+	CgState *st;
+	ALP_TRY(buf.get(&r, n));
+	ALP_TRY(buf.get(&u, n));
+	ALP_TRY(buf.get(&temp, n));
+	ALP_TRY(buf.get(&st, 1));
+	CgArgs<double> a{n, A->crs.off, A->crs.idx, (const double *)A->crs.val, x, r, u, temp, b,
+	                 ctx().d_partials, ctx().d_tickets, st};
+	// temp = A x; r = b - temp; u = r; sigma = r.r; tol *= sqrt(b.b) (:332-378)
+	ALP_TRY(spmv(A->crs, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, temp, x, nullptr, s));
+	hipLaunchKernelGGL(k_cg_init<double>, dim3(vec_grid(n)), dim3(kBlock), 0, s, a,
+	                   (const double *)b, (const double *)nullptr, tol, max_iter);
+	ALP_LAUNCH_CHECK();
+	CgRunner<double> runner(A->crs, a);
+	CgState h;
+	ALP_TRY(runner.run(s, &h));
+	*iters = h.iter;
+	*residual = h.rr;
+	return h.converged ? ALP_SUCCESS : ALP_FAILED;
+}
+
+static alp_rc_t cg_mixed(alp_hip_matrix_t A, const double *b, double *x, size_t max_iter,
+                         double tol, size_t *iters, double *residual) {
+	const size_t n = A->nrows;
+	if (!A->crs.val32) {
+		ALP_HIP_TRY(hipMalloc(&A->crs.val32, sizeof(float) * (A->nnz ? A->nnz : 1)));
+		ALP_TRY(alp_hip_convert_f64_to_f32(A->nnz, (const double *)A->crs.val, A->crs.val32));
+	}
+	hipStream_t s = stream();
+	DeviceBuffers buf;
+	double *r64, *t64;
+	float *x32, *r32, *u32, *temp32, *b32;
+	CgState *st;
+	IrState *ir;
+	ALP_TRY(buf.get(&r64, n));
+	ALP_TRY(buf.get(&t64, n));
+	ALP_TRY(buf.get(&x32, n));
+	ALP_TRY(buf.get(&r32, n));
+	ALP_TRY(buf.get(&u32, n));
+	ALP_TRY(buf.get(&temp32, n));
+	ALP_TRY(buf.get(&b32, n));
+	ALP_TRY(buf.get(&st, 1));
+	ALP_TRY(buf.get(&ir, 1));
+	CgArgs<float> a{n, A->crs.off, A->crs.idx, A->crs.val32, x32, r32, u32, temp32, b32,
+	                ctx().d_partials, ctx().d_tickets, st};
+	CgRunner<float> runner(A->crs, a);
+	const double inner_tol = 1e-5;
+	size_t total = 0;
+	IrState hir{};
+	CgState h{};
+	for (int outer = 0; outer < 100; ++outer) {
+		ALP_TRY(spmv(A->crs, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, t64, x, nullptr, s));
+		hipLaunchKernelGGL(k_ir_residual, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, b, t64, r64,
+		                   ctx().d_partials, ctx().d_tickets, ir, tol, outer == 0 ? 1 : 0);
+		ALP_LAUNCH_CHECK();
+		ALP_HIP_TRY(hipMemcpyAsync(&hir, ir, sizeof(IrState), hipMemcpyDeviceToHost, s));
+		ALP_HIP_TRY(hipStreamSynchronize(s));
+		if (hir.done || total >= max_iter) break;
+		// inner FP32 solve of A d = r / ||r|| from d = 0 (temp = A 0 = 0)
+		hipLaunchKernelGGL(k_ir_scale_down, dim3(vec_grid(n)), dim3(kBlock), 0, s, n,
+		                   (const double *)r64, b32, x32, (const IrState *)ir);
+		ALP_HIP_TRY(hipMemsetAsync(temp32, 0, sizeof(float) * n, s));
+		hipLaunchKernelGGL(k_cg_init<float>, dim3(vec_grid(n)), dim3(kBlock), 0, s, a,
+		                   (const double *)nullptr, (const float *)b32, inner_tol, max_iter - total);
+		ALP_LAUNCH_CHECK();
+		ALP_TRY(runner.run(s, &h));
+		total += h.iter;
+		hipLaunchKernelGGL(k_ir_correct, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, x,
+		                   (const float *)x32, (const IrState *)ir);
+		ALP_LAUNCH_CHECK();
+	}
+	*iters = total;
+	*residual = hir.rnorm * hir.rnorm;
+	return hir.done ? ALP_SUCCESS : ALP_FAILED;
+}
+
+}  // namespace alp
+
+using namespace alp;
+
+ALP_EXPORT alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *b, double *x, size_t max_iter,
+                               double tol, int precision, size_t *iters, double *residual) {
+	ALP_TRY(ensure_init());
+	if (!A || !b || !x || !iters || !residual) return ALP_ILLEGAL;
+	if (A->nrows != A->ncols) return ALP_ILLEGAL;
+	if (A->value_type != ALP_VAL_F64) return ALP_MISMATCH;
+	// the reference's argument checks (conjugate_gradient.hpp:300-312)
+	if (!(tol > 0)) return ALP_ILLEGAL;
+	if (max_iter == 0) return ALP_ILLEGAL;
+	*iters = 0;
+	*residual = std::numeric_limits<double>::infinity();
+	if (A->nrows == 0) return ALP_SUCCESS;
+	if (precision == 0) return cg_fp64(A, b, x, max_iter, tol, iters, residual);
+	if (precision == 1) return cg_mixed(A, b, x, max_iter, tol, iters, residual);
+	return ALP_ILLEGAL;
+}

@@ -1,0 +1,757 @@
+// Runtime, device memory, matrix construction and BLAS1 for the hip backend.
+//
+//   alp_hip_init / finalize    grb::init / grb::finalize (src/graphblas/reference/init.cpp:46,81)
+//   matrix construction        Matrix<D,reference>::buildMatrixUnique (reference/matrix.hpp:1854-1951)
+//   27-point stencil           build_ndims_system_matrix (algorithms/hpcg/matrix_building_utils.hpp:60-73)
+//   BLAS1                      reference/blas1.hpp (dot 9262-9639, eWiseMulAdd 6347-6760, foldl/foldr)
+#include "alp_internal.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstdlib>
+#include <mutex>
+#include <vector>
+
+namespace alp {
+
+void report_hip_error(hipError_t e, const char *what, const char *file, int line) {
+	if (std::getenv("ALP_HIP_QUIET") == nullptr)
+		std::fprintf(stderr, "alp_hip: %s failed: %s (%s:%d)\n", what, hipGetErrorString(e), file,
+		             line);
+}
+
+static Context g_ctx;
+static std::mutex g_init_mutex;
+
+Context &ctx() { return g_ctx; }
+
+alp_rc_t ensure_init() {
+	if (g_ctx.device >= 0) return ALP_SUCCESS;
+	return alp_hip_init(0);
+}
+
+alp_rc_t read_scalars(double *h, const double *d, int count) {
+	ALP_HIP_TRY(hipMemcpyAsync(ctx().h_pinned, d, sizeof(double) * count, hipMemcpyDeviceToHost,
+	                           stream()));
+	ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	std::memcpy(h, ctx().h_pinned, sizeof(double) * count);
+	return ALP_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// kernels: fills, conversions, BLAS1
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(size_t n, T *x, T v) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		x[i] = v;
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(kBlock) void k_convert(size_t n, const S *x, D *y) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		y[i] = static_cast<D>(x[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_axpy(size_t n, double a, const double *x, double *y) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		y[i] = a * x[i] + y[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_xpay(size_t n, double a, const double *x,
+                                                 const double *y, double *z) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		z[i] = a * x[i] + y[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale(size_t n, double a, double *x) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		x[i] = a * x[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_ewise(size_t n, int op, const double *x,
+                                                  const double *y, double *z) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+		const double a = x[i], b = y[i];
+		z[i] = op == 0 ? a + b : op == 1 ? a - b : op == 2 ? a * b : a / b;
+	}
+}
+
+// Deterministic dot: a fixed grid (a function of n only), fixed per-thread strides
+// and a fixed block tree; the last block folds the partials in index order and
+// writes init + sum. Y == nullptr means sum(x).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(size_t n, const T *__restrict__ x,
+                                                const T *__restrict__ y, double *partials,
+                                                unsigned *ticket, double *result, double init) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	double acc = 0.0;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		acc += y ? (double)x[i] * (double)y[i] : (double)x[i];
+	acc = block_sum(acc, lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+	if (last_block(ticket, &is_last)) {
+		const double s = block_sum_partials(partials, gridDim.x, lds);
+		if (threadIdx.x == 0) *result = init + s;
+	}
+}
+
+static unsigned reduction_grid(size_t n) {
+	size_t g = (n + kBlock * 8 - 1) / (kBlock * 8);
+	if (g < 1) g = 1;
+	if (g > 1024) g = 1024;
+	return static_cast<unsigned>(g);
+}
+
+static unsigned stream_grid(size_t n) {
+	size_t g = (n + kBlock - 1) / kBlock;
+	const size_t cap = fill_blocks(8);
+	if (g > cap) g = cap;
+	if (g < 1) g = 1;
+	return static_cast<unsigned>(g);
+}
+
+template <typename T>
+static alp_rc_t dot_impl(size_t n, const T *x, const T *y, double *h_z) {
+	ALP_TRY(ensure_init());
+	if (!h_z) return ALP_ILLEGAL;
+	if (n == 0) return ALP_SUCCESS;
+	if (!x || !y) return ALP_ILLEGAL;
+	const unsigned g = reduction_grid(n);
+	hipLaunchKernelGGL(k_dot<T>, dim3(g), dim3(kBlock), 0, stream(), n, x, y, ctx().d_partials,
+	                   ctx().d_tickets, ctx().d_scalars, *h_z);
+	ALP_LAUNCH_CHECK();
+	return read_scalars(h_z, ctx().d_scalars, 1);
+}
+
+// ---------------------------------------------------------------------------
+// matrices
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_row_stats(size_t n, const uint64_t *off,
+                                                      unsigned long long *maxlen) {
+	unsigned long long m = 0;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+		const unsigned long long l = off[i + 1] - off[i];
+		m = l > m ? l : m;
+	}
+	for (int s = kWave / 2; s > 0; s >>= 1) {
+		const unsigned long long o = __shfl_xor(m, s, kWave);
+		m = o > m ? o : m;
+	}
+	if (threadIdx.x % kWave == 0) atomicMax(maxlen, m);
+}
+
+// CSR-stream row blocks: greedy packing of whole rows into tiles of at most
+// `tile` nonzeros (rows longer than a tile get a block of their own). Built on
+// the host from the downloaded offsets; O(nrows).
+static alp_rc_t build_row_blocks(Storage &S, size_t tile) {
+	std::vector<uint64_t> off(S.nmajor + 1);
+	ALP_HIP_TRY(hipMemcpy(off.data(), S.off, sizeof(uint64_t) * (S.nmajor + 1), hipMemcpyDeviceToHost));
+	std::vector<uint32_t> rb;
+	rb.push_back(0);
+	size_t r = 0;
+	while (r < S.nmajor) {
+		size_t e = r + 1;
+		while (e < S.nmajor && off[e + 1] - off[r] <= tile && e - r < kBlock) ++e;
+		rb.push_back(static_cast<uint32_t>(e));
+		r = e;
+	}
+	if (S.row_blocks) hipFree(S.row_blocks);
+	S.row_blocks = nullptr;
+	ALP_HIP_TRY(hipMalloc(&S.row_blocks, sizeof(uint32_t) * rb.size()));
+	ALP_HIP_TRY(hipMemcpy(S.row_blocks, rb.data(), sizeof(uint32_t) * rb.size(), hipMemcpyHostToDevice));
+	S.n_row_blocks = rb.size() - 1;
+	return ALP_SUCCESS;
+}
+
+alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32) {
+	uint64_t total = 0;
+	ALP_HIP_TRY(hipMemcpy(&total, S.off + S.nmajor, sizeof(uint64_t), hipMemcpyDeviceToHost));
+	if (want_f32 && value_type == ALP_VAL_F64 && total > 0) {
+		ALP_HIP_TRY(hipMalloc(&S.val32, sizeof(float) * total));
+		hipLaunchKernelGGL((k_convert<double, float>), dim3(stream_grid(total)), dim3(kBlock), 0,
+		                   stream(), (size_t)total, (const double *)S.val, S.val32);
+		ALP_LAUNCH_CHECK();
+	}
+	unsigned long long *d_max = nullptr;
+	ALP_HIP_TRY(hipMalloc(&d_max, sizeof(unsigned long long)));
+	ALP_HIP_TRY(hipMemsetAsync(d_max, 0, sizeof(unsigned long long), stream()));
+	if (S.nmajor) {
+		hipLaunchKernelGGL(k_row_stats, dim3(stream_grid(S.nmajor)), dim3(kBlock), 0, stream(),
+		                   S.nmajor, (const uint64_t *)S.off, d_max);
+		ALP_LAUNCH_CHECK();
+	}
+	unsigned long long mx = 0;
+	ALP_HIP_TRY(hipMemcpyAsync(&mx, d_max, sizeof(mx), hipMemcpyDeviceToHost, stream()));
+	ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	hipFree(d_max);
+	S.max_len = mx;
+	S.avg_len = S.nmajor ? double(total) / double(S.nmajor) : 0.0;
+	return build_row_blocks(S, 2048);
+}
+
+void storage_free(Storage &S) {
+	if (S.off) hipFree(S.off);
+	if (S.idx) hipFree(S.idx);
+	if (S.val) hipFree(S.val);
+	if (S.val32) hipFree(S.val32);
+	if (S.row_blocks) hipFree(S.row_blocks);
+	S = Storage();
+}
+
+static size_t value_size(int value_type) {
+	return value_type == ALP_VAL_F64 ? 8 : value_type == ALP_VAL_F32 ? 4 : 0;
+}
+
+// ---- COO -> compressed, reproducing buildMatrixUniqueImplSeq's order ----------
+//
+// The reference counts entries per row, prefix-sums, then places input entry k at
+// --start[row(k)] (reference/matrix.hpp:1918-1930), i.e. the j-th entry of a row in
+// input order lands at row_end - 1 - j. A stable radix sort by row of the input
+// positions gives each entry's rank j within its row, and hence that slot.
+
+__global__ __launch_bounds__(kBlock) void k_histogram(size_t nnz, const uint32_t *major,
+                                                      uint64_t *counts) {
+	for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < nnz; k += (size_t)gridDim.x * kBlock)
+		atomicAdd(reinterpret_cast<unsigned long long *>(counts + major[k]), 1ull);
+}
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_place(size_t nnz, const uint32_t *sorted_major,
+                                                  const uint32_t *sorted_pos,
+                                                  const uint64_t *off, const uint32_t *minor,
+                                                  const V *vals, uint32_t *idx, V *out_vals) {
+	for (size_t p = (size_t)blockIdx.x * kBlock + threadIdx.x; p < nnz; p += (size_t)gridDim.x * kBlock) {
+		const uint32_t r = sorted_major[p];
+		const uint64_t dest = off[r] + off[r + 1] - 1 - p;
+		const uint32_t k = sorted_pos[p];
+		idx[dest] = minor[k];
+		if (vals) out_vals[dest] = vals[k];
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(size_t n, uint32_t *x) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		x[i] = static_cast<uint32_t>(i);
+}
+
+// d_major/d_minor/d_vals are device arrays in input order
+static alp_rc_t compress_device(Storage &S, size_t nmajor, size_t nminor, size_t nnz,
+                                const uint32_t *d_major, const uint32_t *d_minor,
+                                const void *d_vals, int value_type) {
+	const size_t vs = value_size(value_type);
+	S.nmajor = nmajor;
+	S.nminor = nminor;
+	ALP_HIP_TRY(hipMalloc(&S.off, sizeof(uint64_t) * (nmajor + 1)));
+	ALP_HIP_TRY(hipMalloc(&S.idx, sizeof(uint32_t) * (nnz ? nnz : 1)));
+	if (vs) ALP_HIP_TRY(hipMalloc(&S.val, vs * (nnz ? nnz : 1)));
+	uint64_t *counts = nullptr;
+	ALP_HIP_TRY(hipMalloc(&counts, sizeof(uint64_t) * (nmajor + 1)));
+	ALP_HIP_TRY(hipMemsetAsync(counts, 0, sizeof(uint64_t) * (nmajor + 1), stream()));
+	if (nnz) {
+		hipLaunchKernelGGL(k_histogram, dim3(stream_grid(nnz)), dim3(kBlock), 0, stream(), nnz,
+		                   d_major, counts);
+		ALP_LAUNCH_CHECK();
+	}
+	// exclusive scan of counts[0..nmajor] -> off (off[nmajor] = nnz since counts[nmajor] = 0)
+	size_t tmp_bytes = 0;
+	ALP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, S.off, nmajor + 1,
+	                                             stream()));
+	void *tmp = nullptr;
+	ALP_HIP_TRY(hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1));
+	ALP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, S.off, nmajor + 1,
+	                                             stream()));
+	hipFree(tmp);
+	hipFree(counts);
+	if (nnz) {
+		uint32_t *pos = nullptr, *sorted_major = nullptr, *sorted_pos = nullptr;
+		ALP_HIP_TRY(hipMalloc(&pos, sizeof(uint32_t) * nnz));
+		ALP_HIP_TRY(hipMalloc(&sorted_major, sizeof(uint32_t) * nnz));
+		ALP_HIP_TRY(hipMalloc(&sorted_pos, sizeof(uint32_t) * nnz));
+		hipLaunchKernelGGL(k_iota, dim3(stream_grid(nnz)), dim3(kBlock), 0, stream(), nnz, pos);
+		ALP_LAUNCH_CHECK();
+		int end_bit = 1;
+		while (end_bit < 32 && (size_t(1) << end_bit) < nmajor) ++end_bit;
+		tmp_bytes = 0;
+		ALP_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_major, sorted_major,
+		                                               pos, sorted_pos, nnz, 0, end_bit, stream()));
+		ALP_HIP_TRY(hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1));
+		ALP_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, d_major, sorted_major, pos,
+		                                               sorted_pos, nnz, 0, end_bit, stream()));
+		hipFree(tmp);
+		if (value_type == ALP_VAL_F64)
+			hipLaunchKernelGGL(k_place<double>, dim3(stream_grid(nnz)), dim3(kBlock), 0, stream(),
+			                   nnz, sorted_major, sorted_pos, S.off, d_minor,
+			                   (const double *)d_vals, S.idx, (double *)S.val);
+		else if (value_type == ALP_VAL_F32)
+			hipLaunchKernelGGL(k_place<float>, dim3(stream_grid(nnz)), dim3(kBlock), 0, stream(),
+			                   nnz, sorted_major, sorted_pos, S.off, d_minor,
+			                   (const float *)d_vals, S.idx, (float *)S.val);
+		else
+			hipLaunchKernelGGL(k_place<double>, dim3(stream_grid(nnz)), dim3(kBlock), 0, stream(),
+			                   nnz, sorted_major, sorted_pos, S.off, d_minor,
+			                   (const double *)nullptr, S.idx, (double *)nullptr);
+		ALP_LAUNCH_CHECK();
+		ALP_HIP_TRY(hipStreamSynchronize(stream()));
+		hipFree(pos);
+		hipFree(sorted_major);
+		hipFree(sorted_pos);
+	}
+	return ALP_SUCCESS;
+}
+
+// expand CRS into row coordinates (input order for the CCS = CRS order)
+__global__ __launch_bounds__(kBlock) void k_expand_rows(size_t nrows, const uint64_t *off,
+                                                        uint32_t *rows) {
+	const size_t r = (size_t)blockIdx.x * kBlock + threadIdx.x;
+	if (r >= nrows) return;
+	for (uint64_t k = off[r]; k < off[r + 1]; ++k) rows[k] = static_cast<uint32_t>(r);
+}
+
+alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A) {
+	Storage &C = A->crs;
+	uint32_t *rows = nullptr;
+	ALP_HIP_TRY(hipMalloc(&rows, sizeof(uint32_t) * (A->nnz ? A->nnz : 1)));
+	if (A->nrows) {
+		hipLaunchKernelGGL(k_expand_rows, dim3((A->nrows + kBlock - 1) / kBlock), dim3(kBlock), 0,
+		                   stream(), A->nrows, (const uint64_t *)C.off, rows);
+		ALP_LAUNCH_CHECK();
+	}
+	alp_rc_t rc = compress_device(A->ccs, A->ncols, A->nrows, A->nnz, C.idx, rows, C.val,
+	                              A->value_type);
+	hipFree(rows);
+	if (rc) return rc;
+	return storage_finalize(A->ccs, A->value_type, (A->flags & ALP_MAT_WITH_F32) != 0);
+}
+
+// ---- 27-point stencil on the device ----------------------------------------
+//
+// Row r = x + nx (y + ny z); its columns are the halo-1 neighbours. The
+// reference's generator emits them ascending and its counting sort stores each
+// row reversed, so rows are written with descending column indices.
+
+__device__ __forceinline__ unsigned span1(size_t c, size_t n) {
+	return (c > 0 ? 1u : 0u) + 1u + (c + 1 < n ? 1u : 0u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_stencil_counts(size_t nx, size_t ny, size_t nz,
+                                                           uint64_t *counts) {
+	const size_t n = nx * ny * nz;
+	for (size_t r = (size_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (size_t)gridDim.x * kBlock) {
+		const size_t x = r % nx, y = (r / nx) % ny, z = r / (nx * ny);
+		counts[r] = (uint64_t)span1(x, nx) * span1(y, ny) * span1(z, nz);
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_stencil_fill(size_t nx, size_t ny, size_t nz,
+                                                         double diag, double offdiag,
+                                                         const uint64_t *off, uint32_t *idx,
+                                                         double *val) {
+	const size_t n = nx * ny * nz;
+	for (size_t r = (size_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += (size_t)gridDim.x * kBlock) {
+		const size_t x = r % nx, y = (r / nx) % ny, z = r / (nx * ny);
+		uint64_t k = off[r];
+		const size_t zl = z > 0 ? z - 1 : 0, yl = y > 0 ? y - 1 : 0, xl = x > 0 ? x - 1 : 0;
+		const size_t zh = z + 1 < nz ? z + 1 : nz - 1, yh = y + 1 < ny ? y + 1 : ny - 1,
+		             xh = x + 1 < nx ? x + 1 : nx - 1;
+		for (size_t cz = zh + 1; cz-- > zl;)
+			for (size_t cy = yh + 1; cy-- > yl;)
+				for (size_t cx = xh + 1; cx-- > xl;) {
+					const size_t c = cx + nx * (cy + ny * cz);
+					idx[k] = static_cast<uint32_t>(c);
+					val[k] = c == r ? diag : offdiag;
+					++k;
+				}
+	}
+}
+
+}  // namespace alp
+
+using namespace alp;
+
+// ---------------------------------------------------------------------------
+// exported C ABI
+// ---------------------------------------------------------------------------
+
+ALP_EXPORT const char *alp_hip_version(void) { return "alp_hip 0.1 (gfx950)"; }
+
+ALP_EXPORT int alp_hip_device_count(void) {
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+	return n;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_init(int device) {
+	std::lock_guard<std::mutex> lock(g_init_mutex);
+	if (g_ctx.device >= 0) return g_ctx.device == device ? ALP_SUCCESS : ALP_ILLEGAL;
+	int count = 0;
+	if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return ALP_PANIC;
+	if (device < 0 || device >= count) return ALP_ILLEGAL;
+	ALP_HIP_TRY(hipSetDevice(device));
+	hipDeviceProp_t prop;
+	ALP_HIP_TRY(hipGetDeviceProperties(&prop, device));
+	g_ctx.num_cus = prop.multiProcessorCount;
+	ALP_HIP_TRY(hipStreamCreateWithFlags(&g_ctx.own_stream, hipStreamNonBlocking));
+	g_ctx.stream = g_ctx.own_stream;
+	ALP_HIP_TRY(hipMalloc(&g_ctx.d_partials, sizeof(double) * kMaxPartials));
+	ALP_HIP_TRY(hipMalloc(&g_ctx.d_tickets, sizeof(unsigned) * 64));
+	ALP_HIP_TRY(hipMemset(g_ctx.d_tickets, 0, sizeof(unsigned) * 64));
+	ALP_HIP_TRY(hipMalloc(&g_ctx.d_scalars, sizeof(double) * 64));
+	ALP_HIP_TRY(hipHostMalloc(&g_ctx.h_pinned, sizeof(double) * 64, hipHostMallocDefault));
+	g_ctx.device = device;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_finalize(void) {
+	std::lock_guard<std::mutex> lock(g_init_mutex);
+	if (g_ctx.device < 0) return ALP_SUCCESS;
+	hipStreamSynchronize(g_ctx.stream);
+	hipFree(g_ctx.d_partials);
+	hipFree(g_ctx.d_tickets);
+	hipFree(g_ctx.d_scalars);
+	hipHostFree(g_ctx.h_pinned);
+	hipStreamDestroy(g_ctx.own_stream);
+	g_ctx = Context();
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_set_stream(void *s) {
+	ALP_TRY(ensure_init());
+	g_ctx.stream = s ? static_cast<hipStream_t>(s) : g_ctx.own_stream;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_sync(void) {
+	ALP_TRY(ensure_init());
+	ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_malloc(void **p, size_t bytes) {
+	ALP_TRY(ensure_init());
+	if (!p) return ALP_ILLEGAL;
+	ALP_HIP_TRY(hipMalloc(p, bytes ? bytes : 1));
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_free(void *p) {
+	if (!p) return ALP_SUCCESS;
+	ALP_HIP_TRY(hipFree(p));
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_memcpy_h2d(void *d, const void *h, size_t bytes) {
+	ALP_TRY(ensure_init());
+	if (bytes == 0) return ALP_SUCCESS;
+	ALP_HIP_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream()));
+	ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_memcpy_d2h(void *h, const void *d, size_t bytes) {
+	ALP_TRY(ensure_init());
+	if (bytes == 0) return ALP_SUCCESS;
+	ALP_HIP_TRY(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, stream()));
+	ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_memcpy_d2d(void *d, const void *s, size_t bytes) {
+	ALP_TRY(ensure_init());
+	if (bytes == 0) return ALP_SUCCESS;
+	ALP_HIP_TRY(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, stream()));
+	return ALP_SUCCESS;
+}
+
+// ---- BLAS1 ----
+
+ALP_EXPORT alp_rc_t alp_hip_dot_f64(size_t n, const double *x, const double *y, double *h_z) {
+	return dot_impl<double>(n, x, y, h_z);
+}
+
+ALP_EXPORT alp_rc_t alp_hip_dot_f32(size_t n, const float *x, const float *y, double *h_z) {
+	return dot_impl<float>(n, x, y, h_z);
+}
+
+ALP_EXPORT alp_rc_t alp_hip_sum_f64(size_t n, const double *x, double *h_z) {
+	ALP_TRY(ensure_init());
+	if (!h_z) return ALP_ILLEGAL;
+	if (n == 0) return ALP_SUCCESS;
+	const unsigned g = reduction_grid(n);
+	hipLaunchKernelGGL(k_dot<double>, dim3(g), dim3(kBlock), 0, stream(), n, x,
+	                   (const double *)nullptr, ctx().d_partials, ctx().d_tickets,
+	                   ctx().d_scalars, *h_z);
+	ALP_LAUNCH_CHECK();
+	return read_scalars(h_z, ctx().d_scalars, 1);
+}
+
+template <typename T>
+static alp_rc_t fill(size_t n, T *x, T v) {
+	ALP_TRY(ensure_init());
+	if (n == 0) return ALP_SUCCESS;
+	if (!x) return ALP_ILLEGAL;
+	hipLaunchKernelGGL(k_fill<T>, dim3(stream_grid(n)), dim3(kBlock), 0, stream(), n, x, v);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_set_f64(size_t n, double *x, double v) { return fill(n, x, v); }
+ALP_EXPORT alp_rc_t alp_hip_set_f32(size_t n, float *x, float v) { return fill(n, x, v); }
+ALP_EXPORT alp_rc_t alp_hip_set_u8(size_t n, uint8_t *x, uint8_t v) { return fill(n, x, v); }
+
+ALP_EXPORT alp_rc_t alp_hip_axpy_f64(size_t n, double a, const double *x, double *y) {
+	ALP_TRY(ensure_init());
+	if (n == 0) return ALP_SUCCESS;
+	hipLaunchKernelGGL(k_axpy, dim3(stream_grid(n)), dim3(kBlock), 0, stream(), n, a, x, y);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_xpay_f64(size_t n, double a, const double *x, const double *y,
+                                     double *z) {
+	ALP_TRY(ensure_init());
+	if (n == 0) return ALP_SUCCESS;
+	hipLaunchKernelGGL(k_xpay, dim3(stream_grid(n)), dim3(kBlock), 0, stream(), n, a, x, y, z);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_ewise_f64(size_t n, int op, const double *x, const double *y,
+                                      double *z) {
+	ALP_TRY(ensure_init());
+	if (op < 0 || op > 3) return ALP_ILLEGAL;
+	if (n == 0) return ALP_SUCCESS;
+	hipLaunchKernelGGL(k_ewise, dim3(stream_grid(n)), dim3(kBlock), 0, stream(), n, op, x, y, z);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_scale_f64(size_t n, double a, double *x) {
+	ALP_TRY(ensure_init());
+	if (n == 0) return ALP_SUCCESS;
+	hipLaunchKernelGGL(k_scale, dim3(stream_grid(n)), dim3(kBlock), 0, stream(), n, a, x);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_convert_f64_to_f32(size_t n, const double *x, float *y) {
+	ALP_TRY(ensure_init());
+	if (n == 0) return ALP_SUCCESS;
+	hipLaunchKernelGGL((k_convert<double, float>), dim3(stream_grid(n)), dim3(kBlock), 0, stream(),
+	                   n, x, y);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_convert_f32_to_f64(size_t n, const float *x, double *y) {
+	ALP_TRY(ensure_init());
+	if (n == 0) return ALP_SUCCESS;
+	hipLaunchKernelGGL((k_convert<float, double>), dim3(stream_grid(n)), dim3(kBlock), 0, stream(),
+	                   n, x, y);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
+// ---- matrices ----
+
+static alp_rc_t new_matrix(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz,
+                           int value_type, unsigned flags) {
+	if (!A) return ALP_ILLEGAL;
+	if (value_type < 0 || value_type > 2) return ALP_ILLEGAL;
+	if (nrows > 0xFFFFFFFFull || ncols > 0xFFFFFFFFull) return ALP_OVERFLW;
+	auto *M = new (std::nothrow) alp_hip_matrix_s();
+	if (!M) return ALP_OUTOFMEM;
+	M->nrows = nrows;
+	M->ncols = ncols;
+	M->nnz = nnz;
+	M->value_type = value_type;
+	M->flags = flags;
+	M->crs.nmajor = nrows;
+	M->crs.nminor = ncols;
+	*A = M;
+	return ALP_SUCCESS;
+}
+
+static alp_rc_t finish_matrix(alp_hip_matrix_t M) {
+	ALP_TRY(storage_finalize(M->crs, M->value_type, (M->flags & ALP_MAT_WITH_F32) != 0));
+	if (M->flags & ALP_MAT_WITH_CCS) ALP_TRY(build_ccs_from_crs(M));
+	ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_matrix_destroy(alp_hip_matrix_t A) {
+	if (!A) return ALP_SUCCESS;
+	hipStreamSynchronize(stream());
+	storage_free(A->crs);
+	storage_free(A->ccs);
+	delete A;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_matrix_create_crs(alp_hip_matrix_t *A, size_t nrows, size_t ncols,
+                                              size_t nnz, const uint64_t *h_off,
+                                              const uint32_t *h_idx, const void *h_val,
+                                              int value_type, unsigned flags) {
+	ALP_TRY(ensure_init());
+	if (!h_off || (nnz && !h_idx) || (nnz && value_type != ALP_VAL_PATTERN && !h_val))
+		return ALP_ILLEGAL;
+	if (h_off[0] != 0 || h_off[nrows] != nnz) return ALP_ILLEGAL;
+	for (size_t i = 0; i < nrows; ++i)
+		if (h_off[i + 1] < h_off[i]) return ALP_ILLEGAL;
+	for (size_t k = 0; k < nnz; ++k)
+		if (h_idx[k] >= ncols) return ALP_MISMATCH;
+	ALP_TRY(new_matrix(A, nrows, ncols, nnz, value_type, flags));
+	alp_hip_matrix_t M = *A;
+	const size_t vs = value_size(value_type);
+	alp_rc_t rc = ALP_SUCCESS;
+	do {
+		if (hipMalloc(&M->crs.off, sizeof(uint64_t) * (nrows + 1)) != hipSuccess ||
+		    hipMalloc(&M->crs.idx, sizeof(uint32_t) * (nnz ? nnz : 1)) != hipSuccess ||
+		    (vs && hipMalloc(&M->crs.val, vs * (nnz ? nnz : 1)) != hipSuccess)) {
+			rc = ALP_OUTOFMEM;
+			break;
+		}
+		if (hipMemcpy(M->crs.off, h_off, sizeof(uint64_t) * (nrows + 1), hipMemcpyHostToDevice) ||
+		    (nnz && hipMemcpy(M->crs.idx, h_idx, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice)) ||
+		    (nnz && vs && hipMemcpy(M->crs.val, h_val, vs * nnz, hipMemcpyHostToDevice))) {
+			rc = ALP_PANIC;
+			break;
+		}
+		rc = finish_matrix(M);
+	} while (0);
+	if (rc) {
+		alp_hip_matrix_destroy(M);
+		*A = nullptr;
+	}
+	return rc;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_matrix_create_coo(alp_hip_matrix_t *A, size_t nrows, size_t ncols,
+                                              size_t nnz, const uint32_t *h_I,
+                                              const uint32_t *h_J, const void *h_V,
+                                              int value_type, unsigned flags) {
+	ALP_TRY(ensure_init());
+	if (nnz && (!h_I || !h_J || (value_type != ALP_VAL_PATTERN && !h_V))) return ALP_ILLEGAL;
+	if (nnz >= 0xFFFFFFFFull) return ALP_OVERFLW;
+	for (size_t k = 0; k < nnz; ++k)
+		if (h_I[k] >= nrows || h_J[k] >= ncols) return ALP_MISMATCH;
+	ALP_TRY(new_matrix(A, nrows, ncols, nnz, value_type, flags));
+	alp_hip_matrix_t M = *A;
+	const size_t vs = value_size(value_type);
+	uint32_t *dI = nullptr, *dJ = nullptr;
+	void *dV = nullptr;
+	alp_rc_t rc = ALP_SUCCESS;
+	do {
+		if (hipMalloc(&dI, sizeof(uint32_t) * (nnz ? nnz : 1)) ||
+		    hipMalloc(&dJ, sizeof(uint32_t) * (nnz ? nnz : 1)) ||
+		    (vs && hipMalloc(&dV, vs * (nnz ? nnz : 1)))) {
+			rc = ALP_OUTOFMEM;
+			break;
+		}
+		if (nnz && (hipMemcpy(dI, h_I, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice) ||
+		            hipMemcpy(dJ, h_J, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice) ||
+		            (vs && hipMemcpy(dV, h_V, vs * nnz, hipMemcpyHostToDevice)))) {
+			rc = ALP_PANIC;
+			break;
+		}
+		rc = compress_device(M->crs, nrows, ncols, nnz, dI, dJ, dV, value_type);
+		if (rc) break;
+		rc = storage_finalize(M->crs, value_type, (flags & ALP_MAT_WITH_F32) != 0);
+		if (rc) break;
+		if (flags & ALP_MAT_WITH_CCS) {
+			// the reference builds its CCS from the same input order (matrix.hpp:1928-1930)
+			rc = compress_device(M->ccs, ncols, nrows, nnz, dJ, dI, dV, value_type);
+			if (rc) break;
+			rc = storage_finalize(M->ccs, value_type, (flags & ALP_MAT_WITH_F32) != 0);
+		}
+	} while (0);
+	hipStreamSynchronize(stream());
+	if (dI) hipFree(dI);
+	if (dJ) hipFree(dJ);
+	if (dV) hipFree(dV);
+	if (rc) {
+		alp_hip_matrix_destroy(M);
+		*A = nullptr;
+	}
+	return rc;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_matrix_stencil27(alp_hip_matrix_t *A, size_t nx, size_t ny,
+                                             size_t nz, double diag, double offdiag,
+                                             unsigned flags) {
+	ALP_TRY(ensure_init());
+	// the reference's generator rejects sizes below 2*halo+1 (ndim_matrix_builders.hpp:229-233)
+	if (nx < 3 || ny < 3 || nz < 3) return ALP_ILLEGAL;
+	const size_t n = nx * ny * nz;
+	const size_t nnz = (3 * nx - 2) * (3 * ny - 2) * (3 * nz - 2);
+	ALP_TRY(new_matrix(A, n, n, nnz, ALP_VAL_F64, flags));
+	alp_hip_matrix_t M = *A;
+	alp_rc_t rc = ALP_SUCCESS;
+	uint64_t *counts = nullptr;
+	void *tmp = nullptr;
+	do {
+		if (hipMalloc(&M->crs.off, sizeof(uint64_t) * (n + 1)) ||
+		    hipMalloc(&M->crs.idx, sizeof(uint32_t) * nnz) ||
+		    hipMalloc(&M->crs.val, sizeof(double) * nnz) ||
+		    hipMalloc(&counts, sizeof(uint64_t) * (n + 1))) {
+			rc = ALP_OUTOFMEM;
+			break;
+		}
+		hipMemsetAsync(counts + n, 0, sizeof(uint64_t), stream());
+		hipLaunchKernelGGL(k_stencil_counts, dim3(stream_grid(n)), dim3(kBlock), 0, stream(), nx,
+		                   ny, nz, counts);
+		size_t tmp_bytes = 0;
+		hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, M->crs.off, n + 1, stream());
+		if (hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1)) {
+			rc = ALP_OUTOFMEM;
+			break;
+		}
+		if (hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, M->crs.off, n + 1, stream())) {
+			rc = ALP_PANIC;
+			break;
+		}
+		hipLaunchKernelGGL(k_stencil_fill, dim3(stream_grid(n)), dim3(kBlock), 0, stream(), nx, ny,
+		                   nz, diag, offdiag, (const uint64_t *)M->crs.off, M->crs.idx,
+		                   (double *)M->crs.val);
+		if (hipGetLastError() != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		rc = finish_matrix(M);
+	} while (0);
+	hipStreamSynchronize(stream());
+	if (counts) hipFree(counts);
+	if (tmp) hipFree(tmp);
+	if (rc) {
+		alp_hip_matrix_destroy(M);
+		*A = nullptr;
+	}
+	return rc;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_matrix_info(alp_hip_matrix_t A, size_t *nrows, size_t *ncols,
+                                        size_t *nnz) {
+	if (!A) return ALP_ILLEGAL;
+	if (nrows) *nrows = A->nrows;
+	if (ncols) *ncols = A->ncols;
+	if (nnz) *nnz = A->nnz;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_matrix_download(alp_hip_matrix_t A, int which, uint64_t *h_off,
+                                            uint32_t *h_idx, void *h_val) {
+	ALP_TRY(ensure_init());
+	if (!A || (which != 0 && which != 1)) return ALP_ILLEGAL;
+	Storage &S = which == 0 ? A->crs : A->ccs;
+	if (!S.off) return ALP_ILLEGAL;
+	ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	if (h_off) ALP_HIP_TRY(hipMemcpy(h_off, S.off, sizeof(uint64_t) * (S.nmajor + 1), hipMemcpyDeviceToHost));
+	if (h_idx && A->nnz) ALP_HIP_TRY(hipMemcpy(h_idx, S.idx, sizeof(uint32_t) * A->nnz, hipMemcpyDeviceToHost));
+	const size_t vs = value_size(A->value_type);
+	if (h_val && vs && A->nnz) ALP_HIP_TRY(hipMemcpy(h_val, S.val, vs * A->nnz, hipMemcpyDeviceToHost));
+	return ALP_SUCCESS;
+}

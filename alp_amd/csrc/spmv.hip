@@ -1,0 +1,318 @@
+// Semiring SpMV for the hip backend: grb::mxv / grb::vxm (reference/blas2.hpp:870-2200).
+//
+// ALP funnels every product into vxm_generic (blas2.hpp:870-1528), which picks a
+// CRS or CCS gather per output element (vxm_inner_kernel_gather, :295-568) and
+// folds y = y (+) row. Here the same gather runs as one of two HIP kernels over
+// device-resident storage, both on a persistent, XCD-aware grid:
+//   CSR-vector  an aligned group of G lanes (G = 2..64) per row; lanes stride the
+//               row so each load is one contiguous segment; a shuffle tree folds.
+//   CSR-stream  a block takes a tile of whole rows (<= 2048 nonzeros, <= 256
+//               rows), streams their products through LDS with fully coalesced
+//               loads, then one thread per row folds its slice; rows longer than
+//               a tile are folded by the whole block.
+// mxv gathers over the CRS; transpose_matrix (and vxm) over the CCS.
+#include "spmv_device.hpp"
+
+#include <vector>
+
+namespace alp {
+
+constexpr int kStreamTile = 2048;
+
+template <int G, typename R, int MASK, bool OVERWRITE>
+__global__ __launch_bounds__(kBlock) void k_spmv_vector(
+    size_t nrows, const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx,
+    const void *__restrict__ vval, const typename R::X *__restrict__ x,
+    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
+	using A = typename R::A;
+	using Y = typename R::Y;
+	constexpr unsigned kRows = kBlock / G;
+	const auto *val = static_cast<const typename R::V *>(vval);
+	const size_t nchunks = (nrows + kRows - 1) / kRows;
+	const WorkSplit w = xcd_split(nchunks);
+	const int lane = threadIdx.x % G;
+	for (size_t c = w.first; c < w.last; c += w.step) {
+		const size_t row = c * kRows + threadIdx.x / G;
+		if (row >= nrows) continue;
+		if (MASK) {
+			const bool m = mask[row] != 0;
+			if ((MASK == 1) != m) continue;
+		}
+		const uint64_t beg = off[row], end = off[row + 1];
+		if (beg == end) {
+			if (OVERWRITE && lane == 0) y[row] = static_cast<Y>(R::zero());
+			continue;
+		}
+		const A acc = row_gather<G, R>(beg, end, lane, idx, val, x);
+		if (lane == 0) y[row] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(static_cast<A>(y[row]), acc));
+	}
+}
+
+template <typename R, int MASK, bool OVERWRITE>
+__global__ __launch_bounds__(kBlock) void k_spmv_stream(
+    size_t nrb, const uint32_t *__restrict__ rb, const uint64_t *__restrict__ off,
+    const uint32_t *__restrict__ idx, const void *__restrict__ vval,
+    const typename R::X *__restrict__ x, typename R::Y *__restrict__ y,
+    const uint8_t *__restrict__ mask) {
+	using A = typename R::A;
+	using Y = typename R::Y;
+	__shared__ A tile[kStreamTile];
+	__shared__ A red[kBlock / kWave];
+	const auto *val = static_cast<const typename R::V *>(vval);
+	const WorkSplit w = xcd_split(nrb);
+	for (size_t b = w.first; b < w.last; b += w.step) {
+		const uint32_t r0 = rb[b], r1 = rb[b + 1];
+		const uint64_t base = off[r0], top = off[r1];
+		if (top - base <= (uint64_t)kStreamTile) {
+			for (uint64_t k = base + threadIdx.x; k < top; k += kBlock)
+				tile[k - base] = Prod<R>::at(val, x, k, __builtin_nontemporal_load(idx + k));
+			__syncthreads();
+			const uint32_t r = r0 + threadIdx.x;
+			if (r < r1) {
+				bool active = true;
+				if (MASK) active = (MASK == 1) == (mask[r] != 0);
+				if (active) {
+					const uint64_t e = off[r + 1];
+					uint64_t k = off[r];
+					if (k == e) {
+						if (OVERWRITE) y[r] = static_cast<Y>(R::zero());
+					} else {
+						A acc = R::zero();
+						for (; k < e; ++k) acc = R::add(acc, tile[k - base]);
+						y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(static_cast<A>(y[r]), acc));
+					}
+				}
+			}
+			__syncthreads();
+		} else {
+			// a single row longer than a tile: the whole block folds it
+			A acc = R::zero();
+			for (uint64_t k = base + threadIdx.x; k < top; k += kBlock)
+				acc = R::add(acc, Prod<R>::at(val, x, k, __builtin_nontemporal_load(idx + k)));
+			acc = group_fold<kWave, R>(acc);
+			__syncthreads();
+			if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = acc;
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				A t = red[0];
+				for (int i = 1; i < kBlock / kWave; ++i) t = R::add(t, red[i]);
+				bool active = true;
+				if (MASK) active = (MASK == 1) == (mask[r0] != 0);
+				if (active) y[r0] = OVERWRITE ? static_cast<Y>(t) : static_cast<Y>(R::add(static_cast<A>(y[r0]), t));
+			}
+			__syncthreads();
+		}
+	}
+}
+
+// ---- launch ------------------------------------------------------------------
+
+template <typename R, int MASK, bool OW>
+static void launch_vector(int G, const Storage &S, const void *val, const void *x, void *y,
+                          const uint8_t *mask, hipStream_t s) {
+	const size_t rows_per = kBlock / G;
+	const size_t nchunks = (S.nmajor + rows_per - 1) / rows_per;
+	size_t grid = nchunks < fill_blocks(8) ? nchunks : fill_blocks(8);
+	if (grid == 0) return;
+	using X = typename R::X;
+	using Y = typename R::Y;
+#define ALP_VEC_CASE(GG)                                                                             \
+	case GG:                                                                                         \
+		hipLaunchKernelGGL((k_spmv_vector<GG, R, MASK, OW>), dim3(grid), dim3(kBlock), 0, s,       \
+		                   S.nmajor, (const uint64_t *)S.off, (const uint32_t *)S.idx, val,          \
+		                   (const X *)x, (Y *)y, mask);                                              \
+		break;
+	switch (G) {
+		ALP_VEC_CASE(2)
+		ALP_VEC_CASE(4)
+		ALP_VEC_CASE(8)
+		ALP_VEC_CASE(16)
+		ALP_VEC_CASE(32)
+		ALP_VEC_CASE(64)
+		default: break;
+	}
+#undef ALP_VEC_CASE
+}
+
+template <typename R, int MASK, bool OW>
+static void launch_stream(const Storage &S, const void *val, const void *x, void *y,
+                          const uint8_t *mask, hipStream_t s) {
+	size_t grid = S.n_row_blocks < fill_blocks(4) ? S.n_row_blocks : fill_blocks(4);
+	if (grid == 0) return;
+	hipLaunchKernelGGL((k_spmv_stream<R, MASK, OW>), dim3(grid), dim3(kBlock), 0, s,
+	                   S.n_row_blocks, (const uint32_t *)S.row_blocks, (const uint64_t *)S.off,
+	                   (const uint32_t *)S.idx, val, (const typename R::X *)x,
+	                   (typename R::Y *)y, mask);
+}
+
+int default_lanes(const Storage &S) {
+	const double a = S.avg_len;
+	if (a <= 3) return 2;
+	if (a <= 6) return 4;
+	if (a <= 12) return 8;
+	if (a <= 40) return 32;
+	return 64;
+}
+
+template <typename R>
+static void dispatch(const Storage &S, const void *val, unsigned descr, void *y, const void *x,
+                     const uint8_t *mask, hipStream_t s) {
+	const bool ow = (descr & ALP_DESC_OVERWRITE) != 0;
+	const int mmode = mask ? ((descr & ALP_DESC_INVERT_MASK) ? 2 : 1) : 0;
+	int kernel = S.kernel;
+	int G = S.lanes ? S.lanes : default_lanes(S);
+	if (kernel == 0) kernel = (S.avg_len < 4 && S.max_len > 64) ? 2 : 1;
+#define ALP_GO(MM, OWV)                                                 \
+	if (kernel == 2)                                                    \
+		launch_stream<R, MM, OWV>(S, val, x, y, mask, s);               \
+	else                                                                \
+		launch_vector<R, MM, OWV>(G, S, val, x, y, mask, s);
+	if (ow) {
+		if (mmode == 0) { ALP_GO(0, true) }
+		else if (mmode == 1) { ALP_GO(1, true) }
+		else { ALP_GO(2, true) }
+	} else {
+		if (mmode == 0) { ALP_GO(0, false) }
+		else if (mmode == 1) { ALP_GO(1, false) }
+		else { ALP_GO(2, false) }
+	}
+#undef ALP_GO
+}
+
+alp_rc_t spmv(const Storage &S, int value_type, alp_semiring_t ring, unsigned descr, void *y,
+              const void *x, const uint8_t *mask, hipStream_t s) {
+	if (!S.off) return ALP_PANIC;
+	if (S.nmajor == 0) return ALP_SUCCESS;
+	if (!x || !y) return ALP_ILLEGAL;
+	switch (ring) {
+		case ALP_PLUS_TIMES_F64:
+			if (value_type == ALP_VAL_F64)
+				dispatch<PlusTimes<double>>(S, S.val, descr, y, x, mask, s);
+			else if (value_type == ALP_VAL_PATTERN)
+				dispatch<PlusPattern>(S, nullptr, descr, y, x, mask, s);
+			else
+				return ALP_MISMATCH;
+			break;
+		case ALP_PLUS_TIMES_F32:
+			if (value_type == ALP_VAL_F32)
+				dispatch<PlusTimes<float>>(S, S.val, descr, y, x, mask, s);
+			else if (value_type == ALP_VAL_F64 && S.val32)
+				dispatch<PlusTimes<float>>(S, S.val32, descr, y, x, mask, s);
+			else
+				return ALP_MISMATCH;
+			break;
+		case ALP_LOR_LAND_BOOL:
+			dispatch<LorLand>(S, nullptr, descr, y, x, mask, s);
+			break;
+		case ALP_PLUS_PATTERN_F64:
+			dispatch<PlusPattern>(S, nullptr, descr, y, x, mask, s);
+			break;
+		case ALP_MIN_PLUS_F64:
+			if (value_type != ALP_VAL_F64) return ALP_MISMATCH;
+			dispatch<MinPlus>(S, S.val, descr, y, x, mask, s);
+			break;
+		default:
+			return ALP_ILLEGAL;
+	}
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
+}  // namespace alp
+
+using namespace alp;
+
+// mxv: u = u (+) A v over the CRS (blas2.hpp:1836-1946; mxv flips transpose_matrix
+// into vxm_generic, :1848); with transpose_matrix, over the CCS.
+ALP_EXPORT alp_rc_t alp_hip_mxv(alp_hip_matrix_t A, alp_semiring_t ring, unsigned descr,
+                                void *y, const void *x, const uint8_t *mask) {
+	ALP_TRY(ensure_init());
+	if (!A) return ALP_ILLEGAL;
+	if (descr & ALP_DESC_TRANSPOSE_MATRIX) {
+		if (!A->ccs.off) ALP_TRY(build_ccs_from_crs(A));
+		return spmv(A->ccs, A->value_type, ring, descr, y, x, mask, stream());
+	}
+	return spmv(A->crs, A->value_type, ring, descr, y, x, mask, stream());
+}
+
+// vxm: u = u (+) v A, a gather over the CCS (blas2.hpp:1432-1466); transpose_matrix
+// gathers over the CRS (:1222-1255).
+ALP_EXPORT alp_rc_t alp_hip_vxm(alp_hip_matrix_t A, alp_semiring_t ring, unsigned descr,
+                                void *y, const void *x, const uint8_t *mask) {
+	ALP_TRY(ensure_init());
+	if (!A) return ALP_ILLEGAL;
+	if (descr & ALP_DESC_TRANSPOSE_MATRIX)
+		return spmv(A->crs, A->value_type, ring, descr, y, x, mask, stream());
+	if (!A->ccs.off) ALP_TRY(build_ccs_from_crs(A));
+	return spmv(A->ccs, A->value_type, ring, descr, y, x, mask, stream());
+}
+
+ALP_EXPORT alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes) {
+	if (!A || kernel < 0 || kernel > 2) return ALP_ILLEGAL;
+	if (lanes != 0 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 &&
+	    lanes != 64)
+		return ALP_ILLEGAL;
+	A->crs.kernel = A->ccs.kernel = kernel;
+	A->crs.lanes = A->ccs.lanes = lanes;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes) {
+	if (!A) return ALP_ILLEGAL;
+	int k = A->crs.kernel, l = A->crs.lanes ? A->crs.lanes : default_lanes(A->crs);
+	if (k == 0) k = (A->crs.avg_len < 4 && A->crs.max_len > 64) ? 2 : 1;
+	if (kernel) *kernel = k;
+	if (lanes) *lanes = l;
+	return ALP_SUCCESS;
+}
+
+// Autotuner: times each candidate on this matrix's CRS with its natural semiring
+// (3 warm-up + 10 timed runs each, hipEvents on the library stream) and keeps
+// the fastest. Candidates: CSR-vector with G = 2..64, CSR-stream.
+ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
+	ALP_TRY(ensure_init());
+	if (!A) return ALP_ILLEGAL;
+	if (A->nrows == 0) return ALP_SUCCESS;
+	const alp_semiring_t ring = A->value_type == ALP_VAL_F64 ? ALP_PLUS_TIMES_F64
+	                            : A->value_type == ALP_VAL_F32 ? ALP_PLUS_TIMES_F32
+	                                                           : ALP_PLUS_PATTERN_F64;
+	const size_t es = A->value_type == ALP_VAL_F32 ? 4 : 8;
+	void *x = nullptr, *y = nullptr;
+	ALP_HIP_TRY(hipMalloc(&x, es * A->ncols));
+	ALP_HIP_TRY(hipMalloc(&y, es * A->nrows));
+	ALP_HIP_TRY(hipMemsetAsync(x, 0, es * A->ncols, stream()));
+	hipEvent_t e0, e1;
+	ALP_HIP_TRY(hipEventCreate(&e0));
+	ALP_HIP_TRY(hipEventCreate(&e1));
+	struct Cand { int kernel, lanes; };
+	std::vector<Cand> cands = {{1, 2}, {1, 4}, {1, 8}, {1, 16}, {1, 32}, {1, 64}, {2, 0}};
+	float best = 1e30f;
+	Cand pick = {1, default_lanes(A->crs)};
+	const int k0 = A->crs.kernel, l0 = A->crs.lanes;
+	for (const Cand &c : cands) {
+		A->crs.kernel = c.kernel;
+		A->crs.lanes = c.lanes;
+		for (int i = 0; i < 3; ++i)
+			ALP_TRY(spmv(A->crs, A->value_type, ring, ALP_DESC_OVERWRITE, y, x, nullptr, stream()));
+		ALP_HIP_TRY(hipEventRecord(e0, stream()));
+		for (int i = 0; i < 10; ++i)
+			ALP_TRY(spmv(A->crs, A->value_type, ring, ALP_DESC_OVERWRITE, y, x, nullptr, stream()));
+		ALP_HIP_TRY(hipEventRecord(e1, stream()));
+		ALP_HIP_TRY(hipEventSynchronize(e1));
+		float ms = 0;
+		ALP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+		if (ms < best) {
+			best = ms;
+			pick = c;
+		}
+	}
+	(void)k0;
+	(void)l0;
+	A->crs.kernel = A->ccs.kernel = pick.kernel;
+	A->crs.lanes = A->ccs.lanes = pick.lanes;
+	hipEventDestroy(e0);
+	hipEventDestroy(e1);
+	hipFree(x);
+	hipFree(y);
+	return ALP_SUCCESS;
+}

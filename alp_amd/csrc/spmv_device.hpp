@@ -1,0 +1,120 @@
+// Device-side building blocks of the semiring SpMV (ALP's vxm_inner_kernel_gather,
+// reference/blas2.hpp:295-568): semiring functors, the per-row CSR-vector gather,
+// and the XCD-aware split of rows over a persistent grid. Shared by the generic
+// mxv/vxm kernels and by the fused solver kernels.
+#pragma once
+
+#include "alp_internal.hpp"
+
+namespace alp {
+
+// ---- semirings -------------------------------------------------------------
+// Each functor gives the additive identity, multiplication of a matrix value
+// (or the pattern's implicit one) with an input element, and the additive
+// fold. V = matrix value type (void for pattern), X = input, Y = output,
+// A = accumulator.
+
+template <typename T>
+struct PlusTimes {
+	using V = T; using X = T; using Y = T; using A = T;
+	static constexpr bool kPattern = false;
+	__device__ static A zero() { return A(0); }
+	__device__ static A mul(V a, X x) { return a * x; }
+	__device__ static A add(A a, A b) { return a + b; }
+};
+
+// lor-land over a pattern matrix: bytes 0/1 (monoid.hpp:139-147: lor has
+// immutable nonzeroes, so a row that is already true needs no work)
+struct LorLand {
+	using V = void; using X = uint8_t; using Y = uint8_t; using A = uint32_t;
+	static constexpr bool kPattern = true;
+	__device__ static A zero() { return 0u; }
+	__device__ static A one_mul(X x) { return x != 0 ? 1u : 0u; }
+	__device__ static A add(A a, A b) { return a | b; }
+};
+
+// add / left_assign_if over a pattern matrix: sums the input over the row
+// (simple_pagerank.hpp:224-237 row sums; the PageRank vxm over Matrix<void>)
+struct PlusPattern {
+	using V = void; using X = double; using Y = double; using A = double;
+	static constexpr bool kPattern = true;
+	__device__ static A zero() { return 0.0; }
+	__device__ static A one_mul(X x) { return x; }
+	__device__ static A add(A a, A b) { return a + b; }
+};
+
+struct MinPlus {
+	using V = double; using X = double; using Y = double; using A = double;
+	static constexpr bool kPattern = false;
+	__device__ static A zero() { return __builtin_inf(); }
+	__device__ static A mul(V a, X x) { return a + x; }
+	__device__ static A add(A a, A b) { return a < b ? a : b; }
+};
+
+template <typename R, bool P = R::kPattern>
+struct Prod;
+template <typename R>
+struct Prod<R, false> {
+	__device__ static typename R::A at(const typename R::V *__restrict__ val,
+	                                   const typename R::X *__restrict__ x, uint64_t k,
+	                                   uint32_t j) {
+		return R::mul(__builtin_nontemporal_load(val + k), x[j]);
+	}
+};
+template <typename R>
+struct Prod<R, true> {
+	__device__ static typename R::A at(const void *, const typename R::X *__restrict__ x, uint64_t,
+	                                   uint32_t j) {
+		return R::one_mul(x[j]);
+	}
+};
+
+template <int G, typename R>
+__device__ __forceinline__ typename R::A group_fold(typename R::A v) {
+#pragma unroll
+	for (int s = G / 2; s > 0; s >>= 1) v = R::add(v, shfl_xor(v, s));
+	return v;
+}
+
+// One row gathered by an aligned group of G lanes; every lane of the group
+// returns the row's fold. Lanes stride the row's nonzeros so each load
+// instruction of the group is one contiguous segment.
+template <int G, typename R, typename VP>
+__device__ __forceinline__ typename R::A row_gather(uint64_t beg, uint64_t end, int lane,
+                                                    const uint32_t *__restrict__ idx, VP val,
+                                                    const typename R::X *__restrict__ x) {
+	typename R::A acc = R::zero();
+	uint64_t k = beg + lane;
+	// two loads in flight per lane before folding
+	for (; k + G < end; k += 2 * G) {
+		const uint32_t j0 = __builtin_nontemporal_load(idx + k);
+		const uint32_t j1 = __builtin_nontemporal_load(idx + k + G);
+		const typename R::A p0 = Prod<R>::at(val, x, k, j0);
+		const typename R::A p1 = Prod<R>::at(val, x, k + G, j1);
+		acc = R::add(acc, R::add(p0, p1));
+	}
+	if (k < end) acc = R::add(acc, Prod<R>::at(val, x, k, __builtin_nontemporal_load(idx + k)));
+	return group_fold<G, R>(acc);
+}
+
+// ---- persistent-grid work split -----------------------------------------------
+// `nitems` units (row chunks) are split into one contiguous region per XCD
+// (sized by that XCD's share of the grid); inside a region the XCD's blocks take
+// units round-robin, so resident blocks of one XCD touch neighbouring rows.
+struct WorkSplit {
+	size_t first, last, step;
+};
+
+__device__ __forceinline__ WorkSplit xcd_split(size_t nitems) {
+	const unsigned B = gridDim.x, b = blockIdx.x;
+	if (B < kNumXcd) return WorkSplit{b, nitems, B};
+	const unsigned xcd = b % kNumXcd, local = b / kNumXcd;
+	const unsigned q = B / kNumXcd, rem = B % kNumXcd;
+	const unsigned start = xcd * q + (xcd < rem ? xcd : rem);
+	const unsigned nbx = q + (xcd < rem ? 1u : 0u);
+	const size_t r0 = nitems * (size_t)start / B;
+	const size_t r1 = nitems * (size_t)(start + nbx) / B;
+	return WorkSplit{r0 + local, r1, nbx};
+}
+
+}  // namespace alp

@@ -1,0 +1,164 @@
+/*
+ * alp_hip.h -- C ABI of the MI355X-native `hip` backend for ALP/GraphBLAS.
+ *
+ * This is the thin boundary between host code (the grb:: C++ layer in
+ * alp_amd/include/graphblas/hip/, the sparse_cg_* transition ABI in
+ * alp_hip_solver.h, or any FFI caller) and the hand-written gfx950 kernels in
+ * libalp_hip.so. Signatures carry plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every function returns an alp_rc_t whose values equal grb::RC
+ *     (include/graphblas/rc.hpp:47-155 in ALP): 0 = SUCCESS.
+ *   - Pointers named d_* are device pointers (from alp_hip_malloc or any HIP
+ *     allocation on the current device); h_* are host pointers.
+ *   - Matrices are opaque handles owning device-resident compressed storage in
+ *     ALP's layout (reference/compressed_storage.hpp:48-77): uint64 offsets,
+ *     uint32 indices, values; CRS always, CCS when requested.
+ *   - All work is enqueued on the library's stream (alp_hip_set_stream); a call
+ *     that returns a host scalar synchronises that stream. Calls are blocking
+ *     with respect to host-visible results, as ALP's reference backends are
+ *     (reference/properties.hpp: isBlockingExecution).
+ *   - Calls that share containers are not re-entrant, as in ALP.
+ */
+#ifndef ALP_HIP_H
+#define ALP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* grb::RC (include/graphblas/rc.hpp:47-155) */
+typedef enum {
+	ALP_SUCCESS = 0,
+	ALP_PANIC = 1,
+	ALP_OUTOFMEM = 2,
+	ALP_MISMATCH = 3,
+	ALP_OVERLAP = 4,
+	ALP_OVERFLW = 5,
+	ALP_UNSUPPORTED = 6,
+	ALP_ILLEGAL = 7,
+	ALP_FAILED = 8
+} alp_rc_t;
+
+/* semirings, each the device form of a grb::Semiring (include/graphblas/semiring.hpp:190) */
+typedef enum {
+	ALP_PLUS_TIMES_F64 = 0,     /* add<double>, mul<double>, zero, one */
+	ALP_PLUS_TIMES_F32 = 1,     /* add<float>, mul<float>, zero, one */
+	ALP_LOR_LAND_BOOL = 2,      /* logical_or<bool>, logical_and<bool>, false, true */
+	ALP_PLUS_PATTERN_F64 = 3,   /* add<double>, left_assign_if<double,bool> over a pattern
+	                               matrix (simple_pagerank.hpp:224-237; vxm over void) */
+	ALP_MIN_PLUS_F64 = 4        /* min<double>, add<double>, infinity, zero */
+} alp_semiring_t;
+
+/* descriptor bits, same values as grb::descriptors (include/graphblas/descriptors.hpp:63-267) */
+#define ALP_DESC_INVERT_MASK 1u
+#define ALP_DESC_TRANSPOSE_MATRIX 2u
+#define ALP_DESC_STRUCTURAL 8u
+#define ALP_DESC_DENSE 16u
+#define ALP_DESC_ADD_IDENTITY 32u
+#define ALP_DESC_SAFE_OVERLAP 1024u
+/* library-only bit: the output is known to be all zero (a grb::set(y,0) folded into
+ * the product), so y is written without being read */
+#define ALP_DESC_OVERWRITE (1u << 20)
+
+typedef struct alp_hip_matrix_s *alp_hip_matrix_t;
+
+/* ---- runtime: grb::init / grb::finalize (src/graphblas/reference/init.cpp:46,81) ---- */
+alp_rc_t alp_hip_init(int device);
+alp_rc_t alp_hip_finalize(void);
+alp_rc_t alp_hip_set_stream(void *hip_stream); /* NULL = the library's own stream */
+alp_rc_t alp_hip_sync(void);
+const char *alp_hip_version(void);
+/* number of gfx950 devices; 0 without a GPU (never initialises the runtime's device) */
+int alp_hip_device_count(void);
+
+/* ---- device memory ---- */
+alp_rc_t alp_hip_malloc(void **d_ptr, size_t bytes);
+alp_rc_t alp_hip_free(void *d_ptr);
+alp_rc_t alp_hip_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
+alp_rc_t alp_hip_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
+alp_rc_t alp_hip_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
+
+/* ---- matrices (Matrix<D,reference>, reference/matrix.hpp:1018) ---- */
+/* value_type: 0 = f64, 1 = f32, 2 = pattern (Matrix<void>) */
+#define ALP_VAL_F64 0
+#define ALP_VAL_F32 1
+#define ALP_VAL_PATTERN 2
+/* flags */
+#define ALP_MAT_WITH_CCS 1u      /* also build the CCS (transpose / vxm gathers) */
+#define ALP_MAT_WITH_F32 2u      /* also keep an f32 copy of f64 values (mixed precision) */
+
+/* from host CRS arrays (copied; the caller keeps ownership of its arrays) */
+alp_rc_t alp_hip_matrix_create_crs(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz,
+                                   const uint64_t *h_off, const uint32_t *h_idx,
+                                   const void *h_val, int value_type, unsigned flags);
+/* from host coordinates, deduplicated by the caller: buildMatrixUnique
+ * (reference/io.hpp:2304 -> matrix.hpp:1854-1951); the counting sort runs on the
+ * device and reproduces the reference's within-row order (reverse input order) */
+alp_rc_t alp_hip_matrix_create_coo(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz,
+                                   const uint32_t *h_I, const uint32_t *h_J, const void *h_V,
+                                   int value_type, unsigned flags);
+/* the 27-point stencil of build_ndims_system_matrix (algorithms/hpcg/
+ * matrix_building_utils.hpp:60-73), generated on the device */
+alp_rc_t alp_hip_matrix_stencil27(alp_hip_matrix_t *A, size_t nx, size_t ny, size_t nz,
+                                  double diag, double offdiag, unsigned flags);
+alp_rc_t alp_hip_matrix_destroy(alp_hip_matrix_t A);
+alp_rc_t alp_hip_matrix_info(alp_hip_matrix_t A, size_t *nrows, size_t *ncols, size_t *nnz);
+/* download CRS (which = 0) or CCS (which = 1); h_val may be NULL */
+alp_rc_t alp_hip_matrix_download(alp_hip_matrix_t A, int which, uint64_t *h_off,
+                                 uint32_t *h_idx, void *h_val);
+/* select the SpMV kernel: 0 = auto (autotuned), 1 = CSR-vector, 2 = CSR-stream */
+alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes);
+alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes);
+/* time each candidate SpMV kernel on this matrix and keep the fastest */
+alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A);
+
+/* ---- level-2: mxv / vxm (reference/blas2.hpp:870-2200) ----
+ * y (+)= A (x) x  (mxv; ALP_DESC_TRANSPOSE_MATRIX gathers over the CCS instead)
+ * d_mask (nullable) is an output mask of bytes (non-zero = true); with
+ * ALP_DESC_INVERT_MASK rows with a true mask are skipped. Element types follow the
+ * semiring: f64, f32, or bytes (0/1) for ALP_LOR_LAND_BOOL. */
+alp_rc_t alp_hip_mxv(alp_hip_matrix_t A, alp_semiring_t ring, unsigned descr, void *d_y,
+                     const void *d_x, const uint8_t *d_mask);
+/* y (+)= x (x) A   (vxm: the CCS gather; ALP_DESC_TRANSPOSE_MATRIX gathers over CRS) */
+alp_rc_t alp_hip_vxm(alp_hip_matrix_t A, alp_semiring_t ring, unsigned descr, void *d_y,
+                     const void *d_x, const uint8_t *d_mask);
+
+/* ---- level-1 (reference/blas1.hpp), f64 ---- */
+/* z = z + dot(x, y) with a fixed, deterministic reduction tree */
+alp_rc_t alp_hip_dot_f64(size_t n, const double *d_x, const double *d_y, double *h_z);
+alp_rc_t alp_hip_dot_f32(size_t n, const float *d_x, const float *d_y, double *h_z);
+alp_rc_t alp_hip_set_f64(size_t n, double *d_x, double v);
+alp_rc_t alp_hip_set_f32(size_t n, float *d_x, float v);
+alp_rc_t alp_hip_set_u8(size_t n, uint8_t *d_x, uint8_t v);
+/* y = alpha * x + y   (eWiseMul(y, alpha, x, ring) / eWiseMulAdd, blas1.hpp:8684,6566) */
+alp_rc_t alp_hip_axpy_f64(size_t n, double alpha, const double *d_x, double *d_y);
+/* z = alpha * x + y   (eWiseMulAdd into a cleared z, hpcg.hpp:179-180) */
+alp_rc_t alp_hip_xpay_f64(size_t n, double alpha, const double *d_x, const double *d_y,
+                          double *d_z);
+/* z = x op y, op: 0 add, 1 subtract, 2 multiply, 3 divide (eWiseApply, blas1.hpp) */
+alp_rc_t alp_hip_ewise_f64(size_t n, int op, const double *d_x, const double *d_y, double *d_z);
+/* x = alpha * x  (foldr(alpha, x, mulMonoid), blas1.hpp:1685) */
+alp_rc_t alp_hip_scale_f64(size_t n, double alpha, double *d_x);
+/* h_z = z + sum(x)  (foldl(z, x, addMonoid), blas1.hpp:2765) */
+alp_rc_t alp_hip_sum_f64(size_t n, const double *d_x, double *h_z);
+alp_rc_t alp_hip_convert_f64_to_f32(size_t n, const double *d_x, float *d_y);
+alp_rc_t alp_hip_convert_f32_to_f64(size_t n, const float *d_x, double *d_y);
+
+/* ---- CG (algorithms/conjugate_gradient.hpp:191-484), device resident ----
+ * Solves A x = b, stopping when ||r|| < tol*||b|| or after max_iter iterations;
+ * *residual = ||r||^2 (the reference's convention), *iters = iterations done.
+ * Returns ALP_FAILED when not converged, as the reference does.
+ * precision: 0 = FP64 throughout; 1 = FP32 inner iterations inside FP64 iterative
+ * refinement (residual b - A x and the solution update stay FP64). */
+alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *d_b, double *d_x, size_t max_iter,
+                    double tol, int precision, size_t *iters, double *residual);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALP_HIP_H */

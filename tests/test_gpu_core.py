@@ -1,0 +1,314 @@
+"""Parity of the hip backend's storage, SpMV, BLAS1 and CG against the oracle.
+
+All calls go through the C ABI (libalp_hip.so via ctypes). Bars: bit-exact for
+index arrays, integer and boolean work; <= 1e-12 relative for FP64 SpMV; CG
+iteration counts within +-1 of the reference's at the same tolerance.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_RTOL = 1e-12  # FP64 SpMV bar: ||y - y_ref||_inf <= 1e-12 ||y_ref||_inf
+
+
+@pytest.fixture(scope="module")
+def A():
+    import alp_amd
+    alp_amd.init(0)
+    return alp_amd
+
+
+def rel_inf(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ---------------------------------------------------------------- storage ----
+
+@pytest.mark.parametrize("sizes", [(3, 3, 3), (4, 5, 6), (16, 16, 16), (7, 9, 11)])
+def test_device_stencil_matches_reference_storage(A, sizes):
+    M = A.Matrix.stencil27(*sizes, flags=A.MAT_WITH_CCS)
+    ref = O.stencil(*sizes)
+    off, idx, val = M.download(0)
+    assert np.array_equal(off, ref.off) and np.array_equal(idx, ref.idx) and np.array_equal(val, ref.val)
+    coff, cidx, cval = M.download(1)  # symmetric: the reference's CCS equals its CRS
+    assert np.array_equal(coff, ref.off) and np.array_equal(cidx, ref.idx)
+
+
+def test_device_stencil_fixture(A, golden):
+    g = golden("stencil_storage_4x5x6")
+    off, idx, val = A.Matrix.stencil27(4, 5, 6).download(0)
+    assert np.array_equal(off, g["crs_off"]) and np.array_equal(idx, g["crs_idx"])
+    assert np.array_equal(val, g["crs_val"])
+
+
+def test_coo_build_reproduces_reference_order(A):
+    rng = np.random.default_rng(0)
+    n, m = 300, 200
+    key = rng.choice(n * m, 5000, replace=False)
+    I, J = (key // m).astype(np.uint32), (key % m).astype(np.uint32)
+    V = rng.standard_normal(5000)
+    M = A.Matrix.from_coo(n, m, I, J, V, flags=A.MAT_WITH_CCS)
+    for which in (0, 1):
+        ref = O.coo_to_compressed(n, m, I, J, V, which=which)
+        off, idx, val = M.download(which)
+        assert np.array_equal(off, ref.off)
+        assert np.array_equal(idx, ref.idx)
+        assert np.array_equal(val, ref.val)
+
+
+def test_coo_build_empty_rows_and_empty_matrix(A):
+    I = np.array([5, 5, 0], np.uint32)
+    J = np.array([1, 3, 2], np.uint32)
+    M = A.Matrix.from_coo(8, 4, I, J, np.array([1.0, 2.0, 3.0]))
+    off, idx, val = M.download(0)
+    assert list(off) == [0, 1, 1, 1, 1, 1, 3, 3, 3]
+    assert list(idx) == [2, 3, 1] and list(val) == [3.0, 2.0, 1.0]
+    E = A.Matrix.from_coo(4, 4, np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0))
+    assert E.nnz == 0
+    y = A.Vector.from_numpy(np.arange(4.0))
+    x = A.Vector(4, fill=1.0)
+    assert A.mxv(y, E, x) == A.SUCCESS
+    assert np.array_equal(y.numpy(), np.arange(4.0))  # empty rows leave y untouched
+
+
+def test_bad_inputs_are_rejected(A):
+    import ctypes
+    h = ctypes.c_void_p()
+    L = A.lib()
+    # sizes below the generator's 2*halo+1 (ndim_matrix_builders.hpp:229-233)
+    assert L.alp_hip_matrix_stencil27(ctypes.byref(h), 2, 5, 5, 26.0, -1.0, 0) == A.ILLEGAL
+    I = np.array([0, 9], np.uint32)
+    J = np.array([0, 1], np.uint32)
+    assert L.alp_hip_matrix_create_coo(ctypes.byref(h), 4, 4, 2, I.ctypes.data, J.ctypes.data, None,
+                                       A.VAL_PATTERN, 0) == A.MISMATCH
+    M = A.Matrix.stencil27(3, 3, 3)
+    x = A.Vector(27, fill=1.0)
+    y = A.Vector(27, fill=0.0)
+    assert A.mxv(y, M, x, ring=A.PLUS_TIMES_F32) == A.MISMATCH  # no f32 copy was requested
+    assert L.alp_hip_cg(M.h, x.ptr, y.ptr, 10, 0.0, 0, ctypes.byref(ctypes.c_size_t()),
+                        ctypes.byref(ctypes.c_double())) == A.ILLEGAL  # tol must be > 0
+
+
+# ------------------------------------------------------------------- SpMV ----
+
+def closed_form_A1(nx, ny, nz):
+    def c(n):
+        v = np.full(n, 3.0)
+        v[0] = v[-1] = 2.0
+        return v
+    prod = c(nz)[:, None, None] * c(ny)[None, :, None] * c(nx)[None, None, :]
+    return (27.0 - prod).reshape(-1)
+
+
+KERNELS = [(1, 2), (1, 4), (1, 8), (1, 16), (1, 32), (1, 64), (2, 0)]
+
+
+@pytest.mark.parametrize("kernel,lanes", KERNELS)
+@pytest.mark.parametrize("sizes", [(64, 64, 64), (192, 192, 192)])
+def test_mxv_ones_closed_form_bitexact(A, sizes, kernel, lanes):
+    M = A.Matrix.stencil27(*sizes)
+    M.set_kernel(kernel, lanes)
+    n = M.nrows
+    x = A.Vector(n, fill=1.0)
+    y = A.Vector(n, fill=0.0)
+    assert A.mxv(y, M, x, descr=A.DENSE) == A.SUCCESS
+    assert np.array_equal(y.numpy(), closed_form_A1(*sizes))
+    # overwrite mode (set(y, 0) folded into the product) gives the same bits
+    y.set(123.0)
+    assert A.mxv(y, M, x, descr=A.DENSE | A.OVERWRITE) == A.SUCCESS
+    assert np.array_equal(y.numpy(), closed_form_A1(*sizes))
+
+
+@pytest.mark.parametrize("kernel,lanes", KERNELS)
+@pytest.mark.parametrize("name,sizes", [("mxv_8x8x8", (8, 8, 8)), ("mxv_5x9x7", (5, 9, 7)),
+                                        ("mxv_16x16x16", (16, 16, 16))])
+def test_mxv_matches_reference_fixture(A, golden, name, sizes, kernel, lanes):
+    g = golden(name)
+    M = A.Matrix.stencil27(*sizes)
+    M.set_kernel(kernel, lanes)
+    x = A.Vector.from_numpy(g["x"])
+    y = A.Vector.from_numpy(g["y0"])
+    assert A.mxv(y, M, x) == A.SUCCESS
+    assert rel_inf(y.numpy(), g["y"]) <= FP64_RTOL
+    yt = A.Vector.from_numpy(g["y0"])
+    assert A.mxv(yt, M, x, descr=A.TRANSPOSE_MATRIX) == A.SUCCESS
+    assert rel_inf(yt.numpy(), g["yt"]) <= FP64_RTOL
+
+
+def test_mxv_random_matrix_vs_oracle_all_kernels(A):
+    rng = np.random.default_rng(7)
+    n, m = 5000, 4000
+    # skewed row lengths: a few long rows, many short and empty ones
+    lens = np.minimum(rng.zipf(1.6, n), 3000)
+    lens[rng.random(n) < 0.1] = 0
+    I = np.repeat(np.arange(n), lens).astype(np.uint32)
+    J = rng.integers(0, m, I.size).astype(np.uint32)
+    key = np.unique(I.astype(np.uint64) * m + J)
+    I, J = (key // m).astype(np.uint32), (key % m).astype(np.uint32)
+    V = rng.standard_normal(I.size)
+    ref = O.coo_to_compressed(n, m, I, J, V, which=0)
+    refT = O.coo_to_compressed(n, m, I, J, V, which=1)
+    x = rng.standard_normal(m)
+    y0 = rng.standard_normal(n)
+    want = O.spmv(ref, x, y0)
+    xt = rng.standard_normal(n)
+    yt0 = rng.standard_normal(m)
+    wantT = O.spmv(refT, xt, yt0)
+    M = A.Matrix.from_coo(n, m, I, J, V, flags=A.MAT_WITH_CCS)
+    for kernel, lanes in KERNELS:
+        M.set_kernel(kernel, lanes)
+        y = A.Vector.from_numpy(y0)
+        assert A.mxv(y, M, A.Vector.from_numpy(x)) == A.SUCCESS
+        assert rel_inf(y.numpy(), want) <= FP64_RTOL, (kernel, lanes)
+        yt = A.Vector.from_numpy(yt0)
+        assert A.vxm(yt, A.Vector.from_numpy(xt), M) == A.SUCCESS  # x A = A^T x: the CCS gather
+        assert rel_inf(yt.numpy(), wantT) <= FP64_RTOL, (kernel, lanes)
+
+
+def test_mxv_f32(A):
+    M = A.Matrix.stencil27(32, 32, 32, flags=A.MAT_WITH_F32)
+    rng = np.random.default_rng(3)
+    x = rng.random(M.nrows).astype(np.float32)
+    y = A.Vector(M.nrows, np.float32, fill=0.0)
+    assert A.mxv(y, M, A.Vector.from_numpy(x), ring=A.PLUS_TIMES_F32) == A.SUCCESS
+    want = O.spmv(O.stencil(32, 32, 32), x.astype(np.float64))
+    assert np.max(np.abs(y.numpy() - want)) <= 1e-5 * np.max(np.abs(want))
+
+
+def test_lorland_masked_mxv_bitexact(A):
+    rng = np.random.default_rng(11)
+    n = 3000
+    s, d = rng.integers(0, n, 20000), rng.integers(0, n, 20000)
+    I, J = O.symmetrize_dedup(n, s.astype(np.uint32), d.astype(np.uint32))
+    ref = O.coo_to_compressed(n, n, I, J, which=0)
+    M = A.Matrix.from_coo(n, n, I, J)
+    for kernel, lanes in KERNELS:
+        M.set_kernel(kernel, lanes)
+        x = (rng.random(n) < 0.05).astype(np.uint8)
+        y0 = (rng.random(n) < 0.1).astype(np.uint8)
+        mask = (rng.random(n) < 0.5).astype(np.uint8)
+        for inv in (False, True):
+            y = A.Vector.from_numpy(y0)
+            rc = A.mxv(y, M, A.Vector.from_numpy(x), ring=A.LOR_LAND_BOOL,
+                       descr=A.INVERT_MASK if inv else 0, mask=A.Vector.from_numpy(mask))
+            assert rc == A.SUCCESS
+            assert np.array_equal(y.numpy(), O.spmv_lorland(ref, x, y0, mask=mask, invert_mask=inv))
+
+
+def test_pattern_row_sums(A):
+    # vxm<transpose_matrix>(row_sum, ones, L) with add / left_assign_if = out-degrees
+    rng = np.random.default_rng(5)
+    n = 1000
+    s, d = rng.integers(0, n, 8000), rng.integers(0, n, 8000)
+    key = np.unique(s.astype(np.uint64) * n + d)
+    I, J = (key // n).astype(np.uint32), (key % n).astype(np.uint32)
+    M = A.Matrix.from_coo(n, n, I, J, flags=A.MAT_WITH_CCS)
+    y = A.Vector(n, fill=0.0)
+    assert A.vxm(y, A.Vector(n, fill=1.0), M, ring=A.PLUS_PATTERN_F64, descr=A.TRANSPOSE_MATRIX) == A.SUCCESS
+    assert np.array_equal(y.numpy(), np.bincount(I, minlength=n).astype(np.float64))
+    yc = A.Vector(n, fill=0.0)
+    assert A.vxm(yc, A.Vector(n, fill=1.0), M, ring=A.PLUS_PATTERN_F64) == A.SUCCESS
+    assert np.array_equal(yc.numpy(), np.bincount(J, minlength=n).astype(np.float64))
+
+
+def test_autotune_picks_a_valid_kernel(A):
+    M = A.Matrix.stencil27(64, 64, 64)
+    k, lanes = M.autotune()
+    assert k in (1, 2)
+    x = A.Vector(M.nrows, fill=1.0)
+    y = A.Vector(M.nrows, fill=0.0)
+    A.mxv(y, M, x)
+    assert np.array_equal(y.numpy(), closed_form_A1(64, 64, 64))
+
+
+# -------------------------------------------------------------------- dot ----
+
+def test_dot_matches_reference(A, golden):
+    g = golden("dot")
+    for n in (1, 2, 9, 10, 17, 1000, 4097):
+        x, y = g[f"x_{n}"], g[f"y_{n}"]
+        got = A.dot(A.Vector.from_numpy(x), A.Vector.from_numpy(y))
+        want = g[f"z_{n}"][0]
+        assert abs(got - want) <= 1e-12 * np.sum(np.abs(x * y)), n
+
+
+def test_dot_deterministic_and_exact_on_integers(A):
+    rng = np.random.default_rng(1)
+    n = 7_077_888
+    x = A.Vector.from_numpy(rng.standard_normal(n))
+    y = A.Vector.from_numpy(rng.standard_normal(n))
+    a, b = A.dot(x, y), A.dot(x, y)
+    assert a == b
+    k = A.Vector.from_numpy(np.arange(n, dtype=np.float64) % 1024)
+    assert A.dot(k, A.Vector(n, fill=1.0)) == float(np.sum(np.arange(n) % 1024))
+    assert A.dot(k, k, z=5.0) == 5.0 + float(np.sum((np.arange(n) % 1024) ** 2))
+
+
+def test_blas1_ops(A):
+    L = A.lib()
+    rng = np.random.default_rng(2)
+    n = 100_003
+    xh, yh = rng.standard_normal(n), rng.standard_normal(n)
+    x, y = A.Vector.from_numpy(xh), A.Vector.from_numpy(yh)
+    z = A.Vector(n)
+    assert L.alp_hip_axpy_f64(n, 0.5, x.ptr, y.ptr) == 0
+    assert np.allclose(y.numpy(), 0.5 * xh + yh, rtol=1e-15, atol=1e-15)
+    y.upload(yh)
+    for op, f in enumerate([np.add, np.subtract, np.multiply, np.divide]):
+        assert L.alp_hip_ewise_f64(n, op, x.ptr, y.ptr, z.ptr) == 0
+        assert np.array_equal(z.numpy(), f(xh, yh))
+    assert L.alp_hip_xpay_f64(n, -2.0, x.ptr, y.ptr, z.ptr) == 0
+    assert np.allclose(z.numpy(), -2.0 * xh + yh, rtol=1e-15, atol=1e-15)
+    assert L.alp_hip_scale_f64(n, 3.0, x.ptr) == 0
+    assert np.array_equal(x.numpy(), 3.0 * xh)
+
+
+# --------------------------------------------------------------------- CG ----
+
+@pytest.mark.parametrize("name,sizes", [("cg_8x8x8_it1000", (8, 8, 8)), ("cg_12x10x9_it1000", (12, 10, 9))])
+def test_cg_matches_reference_iterations(A, golden, name, sizes):
+    g = golden(name)
+    M = A.Matrix.stencil27(*sizes)
+    n = M.nrows
+    b = A.Vector(n, fill=0.0)
+    A.mxv(b, M, A.Vector(n, fill=1.0))
+    x = A.Vector.from_numpy(g["x0"])
+    rc, it, res = A.cg(M, b, x, 1000, 1e-10)
+    assert rc == A.SUCCESS
+    assert abs(it - int(g["iterations"][0])) <= 1
+    assert np.max(np.abs(x.numpy() - g["x"])) <= 1e-9
+    assert res < (1e-10 ** 2) * float(closed_form_A1(*sizes) @ closed_form_A1(*sizes))
+
+
+def test_cg_64_fifty_iterations(A, golden):
+    g = golden("cg_64x64x64_it50")
+    M = A.Matrix.stencil27(64, 64, 64)
+    n = M.nrows
+    b = A.Vector(n, fill=0.0)
+    A.mxv(b, M, A.Vector(n, fill=1.0))
+    x = A.Vector.from_numpy(np.random.default_rng(42).random(n))
+    rc, it, res = A.cg(M, b, x, 50, 1e-300)
+    assert rc == A.FAILED and it == 50  # not converged: the reference's FAILED
+    assert abs(res - g["residual"][0]) <= 1e-8 * g["residual"][0]
+    xp = x.numpy()[g["x_probe_idx"]]
+    assert np.max(np.abs(xp - g["x_probe"])) <= 1e-9
+
+
+def test_cg_to_1e10_fp64_and_mixed(A):
+    M = A.Matrix.stencil27(48, 48, 48, flags=A.MAT_WITH_F32)
+    n = M.nrows
+    b = A.Vector(n, fill=0.0)
+    A.mxv(b, M, A.Vector(n, fill=1.0))
+    bh = b.numpy()
+    ref = O.stencil(48, 48, 48)
+    for precision in (0, 1):
+        x = A.Vector(n, fill=0.0)
+        rc, it, res = A.cg(M, b, x, 5000, 1e-10, precision)
+        assert rc == A.SUCCESS, precision
+        xh = x.numpy()
+        r = bh - O.spmv(ref, xh)
+        assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(bh), precision
+        assert np.max(np.abs(xh - 1.0)) < 1e-7
