@@ -73,6 +73,8 @@ SIGNATURES = {
     "alp_hip_sync": (_RC, []),
     "alp_hip_version": (ctypes.c_char_p, []),
     "alp_hip_device_count": (_I, []),
+    "alp_hip_timer_start": (_RC, []),
+    "alp_hip_timer_stop": (_RC, [ctypes.POINTER(_D)]),
     "alp_hip_malloc": (_RC, [ctypes.POINTER(_P), _S]),
     "alp_hip_free": (_RC, [_P]),
     "alp_hip_memcpy_h2d": (_RC, [_P, _P, _S]),
@@ -84,8 +86,8 @@ SIGNATURES = {
     "alp_hip_matrix_destroy": (_RC, [_P]),
     "alp_hip_matrix_info": (_RC, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S)]),
     "alp_hip_matrix_download": (_RC, [_P, _I, _P, _P, _P]),
-    "alp_hip_matrix_set_kernel": (_RC, [_P, _I, _I]),
-    "alp_hip_matrix_get_kernel": (_RC, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "alp_hip_matrix_set_kernel": (_RC, [_P, _I, _I, _I]),
+    "alp_hip_matrix_get_kernel": (_RC, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "alp_hip_matrix_autotune": (_RC, [_P]),
     "alp_hip_mxv": (_RC, [_P, _I, _U, _P, _P, _P]),
     "alp_hip_vxm": (_RC, [_P, _I, _U, _P, _P, _P]),
@@ -258,13 +260,15 @@ class Matrix:
                                             val.ctypes.data if with_values else None), "matrix_download")
         return off, idx[: self.nnz], (val[: self.nnz] if with_values else None)
 
-    def set_kernel(self, kernel: int, lanes: int = 0) -> None:
-        check(lib().alp_hip_matrix_set_kernel(self.h, kernel, lanes), "matrix_set_kernel")
+    def set_kernel(self, kernel: int, lanes: int = 0, rows: int = 0) -> None:
+        check(lib().alp_hip_matrix_set_kernel(self.h, kernel, lanes, rows), "matrix_set_kernel")
 
     def get_kernel(self):
-        k, l = _I(), _I()
-        check(lib().alp_hip_matrix_get_kernel(self.h, ctypes.byref(k), ctypes.byref(l)), "matrix_get_kernel")
-        return k.value, l.value
+        """(kernel, lanes, rows): 1 = CSR-vector with `lanes` per row and `rows` per group, 2 = CSR-stream."""
+        k, l, r = _I(), _I(), _I()
+        check(lib().alp_hip_matrix_get_kernel(self.h, ctypes.byref(k), ctypes.byref(l), ctypes.byref(r)),
+              "matrix_get_kernel")
+        return k.value, l.value, r.value
 
     def autotune(self):
         check(lib().alp_hip_matrix_autotune(self.h), "matrix_autotune")

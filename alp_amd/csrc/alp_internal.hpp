@@ -139,12 +139,18 @@ struct Storage {
 	size_t nminor = 0;
 	// CSR-stream row blocks: row_blocks[b] .. row_blocks[b+1] is block b's row range
 	uint32_t *row_blocks = nullptr;
+	uint64_t *row_block_base = nullptr;  // off[row_blocks[b]]
 	size_t n_row_blocks = 0;
+	// the same for 4096-nonzero tiles
+	uint32_t *row_blocks_l = nullptr;
+	uint64_t *row_block_base_l = nullptr;
+	size_t n_row_blocks_l = 0;
 	// statistics for kernel selection
 	double avg_len = 0;
 	size_t max_len = 0;
 	int kernel = 0;  // 0 auto, 1 vector, 2 stream
 	int lanes = 0;
+	int rows = 0;    // rows per lane group of the CSR-vector kernel
 };
 
 }  // namespace alp

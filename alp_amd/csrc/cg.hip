@@ -58,25 +58,41 @@ struct CgArgs {
 
 // ---- kernels ------------------------------------------------------------------
 
-template <int G, typename T>
+template <int G, int RR, typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a) {
 	__shared__ double lds[kBlock / kWave];
 	__shared__ bool is_last;
 	if (a.st->done) return;
-	constexpr unsigned kRows = kBlock / G;
+	constexpr unsigned kGroups = kBlock / G;
+	constexpr unsigned kRows = kGroups * RR;
 	const size_t nchunks = (a.n + kRows - 1) / kRows;
 	const WorkSplit w = xcd_split(nchunks);
 	const int lane = threadIdx.x % G;
+	const unsigned grp = threadIdx.x / G;
 	double part = 0.0;
 	for (size_t c = w.first; c < w.last; c += w.step) {
-		const size_t row = c * kRows + threadIdx.x / G;
-		if (row >= a.n) continue;
-		const uint64_t beg = a.off[row], end = a.off[row + 1];
-		T acc = T(0);
-		if (beg != end) acc = row_gather<G, PlusTimes<T>>(beg, end, lane, a.idx, a.val, a.u);
-		if (lane == 0) {
-			a.temp[row] = acc;
-			part += (double)acc * (double)a.u[row];
+		uint64_t beg[RR], end[RR];
+		size_t row[RR];
+		T ur[RR];
+#pragma unroll
+		for (int i = 0; i < RR; ++i) {
+			row[i] = c * kRows + (size_t)i * kGroups + grp;
+			beg[i] = end[i] = 0;
+			ur[i] = T(0);
+			if (row[i] < a.n) {
+				beg[i] = a.off[row[i]];
+				end[i] = a.off[row[i] + 1];
+				if (lane == 0) ur[i] = a.u[row[i]];
+			}
+		}
+		T acc[RR];
+		rows_gather<G, RR, PlusTimes<T>>(beg, end, lane, a.idx, a.val, a.u, acc);
+#pragma unroll
+		for (int i = 0; i < RR; ++i) {
+			if (lane == 0 && row[i] < a.n) {
+				a.temp[row[i]] = acc[i];
+				part += (double)acc[i] * (double)ur[i];
+			}
 		}
 	}
 	part = block_sum(part, lds);
@@ -261,14 +277,16 @@ template <typename T>
 struct CgRunner {
 	const Storage &S;
 	CgArgs<T> a;
-	int G;
+	int G, RR;
 	bool fused;
 	unsigned gspmv, gvec;
 
 	CgRunner(const Storage &S_, CgArgs<T> a_) : S(S_), a(a_) {
 		G = S.lanes ? S.lanes : default_lanes(S);
+		RR = S.rows ? S.rows : 1;
 		fused = S.kernel != 2 && G >= 2;
-		const size_t nchunks = (a.n + (kBlock / G) - 1) / (kBlock / G);
+		const size_t rows_per = (size_t)(kBlock / G) * RR;
+		const size_t nchunks = (a.n + rows_per - 1) / rows_per;
 		size_t g = fill_blocks(8);
 		if (g > nchunks) g = nchunks;
 		if (g < 1) g = 1;
@@ -278,9 +296,14 @@ struct CgRunner {
 
 	alp_rc_t iteration(hipStream_t s) {
 		if (fused) {
-#define ALP_CG_CASE(GG)                                                                     \
-	case GG:                                                                                \
-		hipLaunchKernelGGL((k_cg_spmv_dot<GG, T>), dim3(gspmv), dim3(kBlock), 0, s, a);     \
+#define ALP_CG_CASE(GG)                                                                          \
+	case GG:                                                                                     \
+		if (RR == 4)                                                                             \
+			hipLaunchKernelGGL((k_cg_spmv_dot<GG, 4, T>), dim3(gspmv), dim3(kBlock), 0, s, a);   \
+		else if (RR == 2)                                                                        \
+			hipLaunchKernelGGL((k_cg_spmv_dot<GG, 2, T>), dim3(gspmv), dim3(kBlock), 0, s, a);   \
+		else                                                                                     \
+			hipLaunchKernelGGL((k_cg_spmv_dot<GG, 1, T>), dim3(gspmv), dim3(kBlock), 0, s, a);   \
 		break;
 			switch (G) {
 				ALP_CG_CASE(2)
@@ -293,7 +316,8 @@ struct CgRunner {
 			}
 #undef ALP_CG_CASE
 		} else {
-			ALP_TRY(spmv(S, std::is_same<T, float>::value ? ALP_VAL_F32 : ALP_VAL_F64,
+			// the matrix is FP64; the FP32 semiring selects its FP32 value copy
+			ALP_TRY(spmv(S, ALP_VAL_F64,
 			             std::is_same<T, float>::value ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64,
 			             ALP_DESC_OVERWRITE, a.temp, a.u, nullptr, s));
 			hipLaunchKernelGGL(k_cg_dot_alpha<T>, dim3(gvec), dim3(kBlock), 0, s, a);

@@ -148,24 +148,40 @@ __global__ __launch_bounds__(kBlock) void k_row_stats(size_t n, const uint64_t *
 // CSR-stream row blocks: greedy packing of whole rows into tiles of at most
 // `tile` nonzeros (rows longer than a tile get a block of their own). Built on
 // the host from the downloaded offsets; O(nrows).
-static alp_rc_t build_row_blocks(Storage &S, size_t tile) {
-	std::vector<uint64_t> off(S.nmajor + 1);
-	ALP_HIP_TRY(hipMemcpy(off.data(), S.off, sizeof(uint64_t) * (S.nmajor + 1), hipMemcpyDeviceToHost));
+static alp_rc_t upload_row_blocks(const std::vector<uint64_t> &off, size_t nrows, size_t tile,
+                                  uint32_t **d_rb, uint64_t **d_base, size_t *count) {
 	std::vector<uint32_t> rb;
 	rb.push_back(0);
 	size_t r = 0;
-	while (r < S.nmajor) {
+	while (r < nrows) {
 		size_t e = r + 1;
-		while (e < S.nmajor && off[e + 1] - off[r] <= tile && e - r < kBlock) ++e;
+		while (e < nrows && off[e + 1] - off[r] <= tile && e - r < kBlock) ++e;
 		rb.push_back(static_cast<uint32_t>(e));
 		r = e;
 	}
-	if (S.row_blocks) hipFree(S.row_blocks);
-	S.row_blocks = nullptr;
-	ALP_HIP_TRY(hipMalloc(&S.row_blocks, sizeof(uint32_t) * rb.size()));
-	ALP_HIP_TRY(hipMemcpy(S.row_blocks, rb.data(), sizeof(uint32_t) * rb.size(), hipMemcpyHostToDevice));
-	S.n_row_blocks = rb.size() - 1;
+	std::vector<uint64_t> rbase(rb.size());
+	for (size_t b = 0; b < rb.size(); ++b) rbase[b] = off[rb[b]];
+	if (*d_rb) hipFree(*d_rb);
+	if (*d_base) hipFree(*d_base);
+	*d_rb = nullptr;
+	*d_base = nullptr;
+	ALP_HIP_TRY(hipMalloc(d_rb, sizeof(uint32_t) * rb.size()));
+	ALP_HIP_TRY(hipMemcpy(*d_rb, rb.data(), sizeof(uint32_t) * rb.size(), hipMemcpyHostToDevice));
+	ALP_HIP_TRY(hipMalloc(d_base, sizeof(uint64_t) * rbase.size()));
+	ALP_HIP_TRY(hipMemcpy(*d_base, rbase.data(), sizeof(uint64_t) * rbase.size(), hipMemcpyHostToDevice));
+	*count = rb.size() - 1;
 	return ALP_SUCCESS;
+}
+
+// CSR-stream row tiles: greedy packing of whole rows into tiles of at most
+// 2048 (and, separately, 4096) nonzeros and 256 rows; a row longer than a tile
+// gets a tile of its own. Built on the host from the downloaded offsets; O(nrows).
+static alp_rc_t build_row_blocks(Storage &S) {
+	std::vector<uint64_t> off(S.nmajor + 1);
+	ALP_HIP_TRY(hipMemcpy(off.data(), S.off, sizeof(uint64_t) * (S.nmajor + 1), hipMemcpyDeviceToHost));
+	ALP_TRY(upload_row_blocks(off, S.nmajor, 2048, &S.row_blocks, &S.row_block_base, &S.n_row_blocks));
+	return upload_row_blocks(off, S.nmajor, 4096, &S.row_blocks_l, &S.row_block_base_l,
+	                         &S.n_row_blocks_l);
 }
 
 alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32) {
@@ -191,7 +207,7 @@ alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32) {
 	hipFree(d_max);
 	S.max_len = mx;
 	S.avg_len = S.nmajor ? double(total) / double(S.nmajor) : 0.0;
-	return build_row_blocks(S, 2048);
+	return build_row_blocks(S);
 }
 
 void storage_free(Storage &S) {
@@ -200,6 +216,9 @@ void storage_free(Storage &S) {
 	if (S.val) hipFree(S.val);
 	if (S.val32) hipFree(S.val32);
 	if (S.row_blocks) hipFree(S.row_blocks);
+	if (S.row_block_base) hipFree(S.row_block_base);
+	if (S.row_blocks_l) hipFree(S.row_blocks_l);
+	if (S.row_block_base_l) hipFree(S.row_block_base_l);
 	S = Storage();
 }
 
@@ -428,6 +447,27 @@ ALP_EXPORT alp_rc_t alp_hip_set_stream(void *s) {
 ALP_EXPORT alp_rc_t alp_hip_sync(void) {
 	ALP_TRY(ensure_init());
 	ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	return ALP_SUCCESS;
+}
+
+static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+
+ALP_EXPORT alp_rc_t alp_hip_timer_start(void) {
+	ALP_TRY(ensure_init());
+	if (!g_t0) ALP_HIP_TRY(hipEventCreate(&g_t0));
+	if (!g_t1) ALP_HIP_TRY(hipEventCreate(&g_t1));
+	ALP_HIP_TRY(hipEventRecord(g_t0, stream()));
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_timer_stop(double *ms) {
+	ALP_TRY(ensure_init());
+	if (!ms || !g_t0 || !g_t1) return ALP_ILLEGAL;
+	ALP_HIP_TRY(hipEventRecord(g_t1, stream()));
+	ALP_HIP_TRY(hipEventSynchronize(g_t1));
+	float f = 0;
+	ALP_HIP_TRY(hipEventElapsedTime(&f, g_t0, g_t1));
+	*ms = f;
 	return ALP_SUCCESS;
 }
 

@@ -13,74 +13,137 @@
 // mxv gathers over the CRS; transpose_matrix (and vxm) over the CCS.
 #include "spmv_device.hpp"
 
+#include <cstdlib>
 #include <vector>
 
 namespace alp {
 
-constexpr int kStreamTile = 2048;
 
-template <int G, typename R, int MASK, bool OVERWRITE>
+template <int G, int RR, typename R, int MASK, bool OVERWRITE>
 __global__ __launch_bounds__(kBlock) void k_spmv_vector(
     size_t nrows, const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx,
     const void *__restrict__ vval, const typename R::X *__restrict__ x,
     typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
 	using A = typename R::A;
 	using Y = typename R::Y;
-	constexpr unsigned kRows = kBlock / G;
+	constexpr unsigned kGroups = kBlock / G;
+	constexpr unsigned kRows = kGroups * RR;
 	const auto *val = static_cast<const typename R::V *>(vval);
 	const size_t nchunks = (nrows + kRows - 1) / kRows;
 	const WorkSplit w = xcd_split(nchunks);
 	const int lane = threadIdx.x % G;
+	const unsigned grp = threadIdx.x / G;
 	for (size_t c = w.first; c < w.last; c += w.step) {
-		const size_t row = c * kRows + threadIdx.x / G;
-		if (row >= nrows) continue;
-		if (MASK) {
-			const bool m = mask[row] != 0;
-			if ((MASK == 1) != m) continue;
+		uint64_t beg[RR], end[RR];
+		size_t row[RR];
+		bool act[RR];
+		A yv[RR];
+#pragma unroll
+		for (int i = 0; i < RR; ++i) {
+			// at step i the block covers kGroups consecutive rows: coalesced streams
+			row[i] = c * kRows + (size_t)i * kGroups + grp;
+			act[i] = row[i] < nrows;
+			if (MASK && act[i]) act[i] = (MASK == 1) == (mask[row[i]] != 0);
+			beg[i] = end[i] = 0;
+			yv[i] = R::zero();
+			if (act[i]) {
+				beg[i] = off[row[i]];
+				end[i] = off[row[i] + 1];
+				if (!OVERWRITE && lane == 0) yv[i] = static_cast<A>(y[row[i]]);
+			}
 		}
-		const uint64_t beg = off[row], end = off[row + 1];
-		if (beg == end) {
-			if (OVERWRITE && lane == 0) y[row] = static_cast<Y>(R::zero());
-			continue;
+		A acc[RR];
+		rows_gather<G, RR, R>(beg, end, lane, idx, val, x, acc);
+#pragma unroll
+		for (int i = 0; i < RR; ++i) {
+			if (!act[i] || lane != 0) continue;
+			if (beg[i] == end[i]) {
+				if (OVERWRITE) y[row[i]] = static_cast<Y>(R::zero());
+			} else {
+				y[row[i]] = OVERWRITE ? static_cast<Y>(acc[i]) : static_cast<Y>(R::add(yv[i], acc[i]));
+			}
 		}
-		const A acc = row_gather<G, R>(beg, end, lane, idx, val, x);
-		if (lane == 0) y[row] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(static_cast<A>(y[row]), acc));
 	}
 }
 
-template <typename R, int MASK, bool OVERWRITE>
+template <typename R, int MASK, bool OVERWRITE, int J, bool XCD>
 __global__ __launch_bounds__(kBlock) void k_spmv_stream(
-    size_t nrb, const uint32_t *__restrict__ rb, const uint64_t *__restrict__ off,
+    size_t nrb, const uint32_t *__restrict__ rb, const uint64_t *__restrict__ rbase,
+    const uint64_t *__restrict__ off,
     const uint32_t *__restrict__ idx, const void *__restrict__ vval,
     const typename R::X *__restrict__ x, typename R::Y *__restrict__ y,
     const uint8_t *__restrict__ mask) {
 	using A = typename R::A;
 	using Y = typename R::Y;
-	__shared__ A tile[kStreamTile];
+	using VT = typename Val<R>::T;
+	constexpr int kTile = J * kBlock;
+	__shared__ A tile[kTile];
 	__shared__ A red[kBlock / kWave];
 	const auto *val = static_cast<const typename R::V *>(vval);
-	const WorkSplit w = xcd_split(nrb);
+	const WorkSplit w = XCD ? xcd_split(nrb) : WorkSplit{blockIdx.x, nrb, gridDim.x};
+	// tile metadata (row bounds and nonzero bounds) is one independent load
+	// trip; the next tile's is fetched while the current one is processed
+	uint32_t nr0 = 0, nr1 = 0;
+	uint64_t nbase = 0, ntop = 0;
+	if (w.first < w.last) {
+		nr0 = rb[w.first];
+		nr1 = rb[w.first + 1];
+		nbase = rbase[w.first];
+		ntop = rbase[w.first + 1];
+	}
 	for (size_t b = w.first; b < w.last; b += w.step) {
-		const uint32_t r0 = rb[b], r1 = rb[b + 1];
-		const uint64_t base = off[r0], top = off[r1];
-		if (top - base <= (uint64_t)kStreamTile) {
-			for (uint64_t k = base + threadIdx.x; k < top; k += kBlock)
-				tile[k - base] = Prod<R>::at(val, x, k, __builtin_nontemporal_load(idx + k));
-			__syncthreads();
-			const uint32_t r = r0 + threadIdx.x;
-			if (r < r1) {
-				bool active = true;
+		const uint32_t r0 = nr0, r1 = nr1;
+		const uint64_t base = nbase, top = ntop;
+		const size_t bn = b + w.step;
+		if (bn < w.last) {
+			nr0 = rb[bn];
+			nr1 = rb[bn + 1];
+			nbase = rbase[bn];
+			ntop = rbase[bn + 1];
+		}
+		if (top - base <= (uint64_t)kTile) {
+			// tpr threads per row fold the row's slice of the tile; the row's
+			// bounds and old y are loaded up front with the tile's streams
+			const unsigned nr = r1 - r0;
+			unsigned tpr = 1;
+			while (tpr < 64 && tpr * 2 * nr <= kBlock) tpr *= 2;
+			const unsigned r = r0 + threadIdx.x / tpr, sub = threadIdx.x % tpr;
+			bool active = r < r1;
+			uint64_t rk = 0, re = 0;
+			A yv = R::zero();
+			if (active) {
+				rk = off[r];
+				re = off[r + 1];
 				if (MASK) active = (MASK == 1) == (mask[r] != 0);
-				if (active) {
-					const uint64_t e = off[r + 1];
-					uint64_t k = off[r];
-					if (k == e) {
-						if (OVERWRITE) y[r] = static_cast<Y>(R::zero());
-					} else {
-						A acc = R::zero();
-						for (; k < e; ++k) acc = R::add(acc, tile[k - base]);
-						y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(static_cast<A>(y[r]), acc));
-					}
+				if (!OVERWRITE && active && sub == 0) yv = static_cast<A>(y[r]);
+			}
+			// all J loads of a thread in flight before the first multiply
+			uint32_t jj[J];
+			VT vv[J];
+#pragma unroll
+			for (int q = 0; q < J; ++q) {
+				const uint64_t k = base + threadIdx.x + (uint64_t)q * kBlock;
+				jj[q] = 0;
+				vv[q] = VT(0);
+				if (k < top) {
+					jj[q] = __builtin_nontemporal_load(idx + k);
+					vv[q] = Val<R>::load(val, k);
+				}
+			}
+#pragma unroll
+			for (int q = 0; q < J; ++q) {
+				const uint64_t k = base + threadIdx.x + (uint64_t)q * kBlock;
+				if (k < top) tile[k - base] = Val<R>::mul(vv[q], x[jj[q]]);
+			}
+			__syncthreads();
+			A acc = R::zero();
+			for (uint64_t q = rk + sub; q < re; q += tpr) acc = R::add(acc, tile[q - base]);
+			for (unsigned s = tpr / 2; s > 0; s >>= 1) acc = R::add(acc, shfl_xor(acc, s));
+			if (active && sub == 0) {
+				if (rk == re) {
+					if (OVERWRITE) y[r] = static_cast<Y>(R::zero());
+				} else {
+					y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(yv, acc));
 				}
 			}
 			__syncthreads();
@@ -88,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
 			// a single row longer than a tile: the whole block folds it
 			A acc = R::zero();
 			for (uint64_t k = base + threadIdx.x; k < top; k += kBlock)
-				acc = R::add(acc, Prod<R>::at(val, x, k, __builtin_nontemporal_load(idx + k)));
+				acc = R::add(acc, Val<R>::mul(Val<R>::load(val, k), x[__builtin_nontemporal_load(idx + k)]));
 			acc = group_fold<kWave, R>(acc);
 			__syncthreads();
 			if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = acc;
@@ -107,10 +170,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
 
 // ---- launch ------------------------------------------------------------------
 
-template <typename R, int MASK, bool OW>
-static void launch_vector(int G, const Storage &S, const void *val, const void *x, void *y,
-                          const uint8_t *mask, hipStream_t s) {
-	const size_t rows_per = kBlock / G;
+template <typename R, int MASK, bool OW, int RR>
+static void launch_vector_rr(int G, const Storage &S, const void *val, const void *x, void *y,
+                             const uint8_t *mask, hipStream_t s) {
+	const size_t rows_per = (kBlock / G) * RR;
 	const size_t nchunks = (S.nmajor + rows_per - 1) / rows_per;
 	size_t grid = nchunks < fill_blocks(8) ? nchunks : fill_blocks(8);
 	if (grid == 0) return;
@@ -118,7 +181,7 @@ static void launch_vector(int G, const Storage &S, const void *val, const void *
 	using Y = typename R::Y;
 #define ALP_VEC_CASE(GG)                                                                             \
 	case GG:                                                                                         \
-		hipLaunchKernelGGL((k_spmv_vector<GG, R, MASK, OW>), dim3(grid), dim3(kBlock), 0, s,       \
+		hipLaunchKernelGGL((k_spmv_vector<GG, RR, R, MASK, OW>), dim3(grid), dim3(kBlock), 0, s,   \
 		                   S.nmajor, (const uint64_t *)S.off, (const uint32_t *)S.idx, val,          \
 		                   (const X *)x, (Y *)y, mask);                                              \
 		break;
@@ -135,14 +198,40 @@ static void launch_vector(int G, const Storage &S, const void *val, const void *
 }
 
 template <typename R, int MASK, bool OW>
-static void launch_stream(const Storage &S, const void *val, const void *x, void *y,
+static void launch_vector(int G, int RR, const Storage &S, const void *val, const void *x, void *y,
                           const uint8_t *mask, hipStream_t s) {
-	size_t grid = S.n_row_blocks < fill_blocks(4) ? S.n_row_blocks : fill_blocks(4);
+	// masked products keep one row per group (the mask already thins the rows)
+	if (MASK != 0 || RR <= 1)
+		launch_vector_rr<R, MASK, OW, 1>(G, S, val, x, y, mask, s);
+	else if (RR == 2)
+		launch_vector_rr<R, 0, OW, 2>(G, S, val, x, y, mask, s);
+	else
+		launch_vector_rr<R, 0, OW, 4>(G, S, val, x, y, mask, s);
+}
+
+// variant (`rows`): 0/1 = persistent grid, 4 blocks per CU; 2 = persistent, 8 per
+// CU; 4 = one block per row tile; 8 = one block per tile in plain block order.
+// tile (`lanes`): 0/8 = 2048-nonzero tiles, 16 = 4096-nonzero tiles.
+template <typename R, int MASK, bool OW>
+static void launch_stream(int variant, int tile, const Storage &S, const void *val, const void *x,
+                          void *y, const uint8_t *mask, hipStream_t s) {
+	const bool big = tile == 16;
+	const size_t nrb = big ? S.n_row_blocks_l : S.n_row_blocks;
+	const uint32_t *rb = big ? S.row_blocks_l : S.row_blocks;
+	const uint64_t *rbase = big ? S.row_block_base_l : S.row_block_base;
+	size_t cap = variant >= 4 ? nrb : fill_blocks(variant == 2 ? 8 : 4);
+	size_t grid = nrb < cap ? nrb : cap;
 	if (grid == 0) return;
-	hipLaunchKernelGGL((k_spmv_stream<R, MASK, OW>), dim3(grid), dim3(kBlock), 0, s,
-	                   S.n_row_blocks, (const uint32_t *)S.row_blocks, (const uint64_t *)S.off,
-	                   (const uint32_t *)S.idx, val, (const typename R::X *)x,
-	                   (typename R::Y *)y, mask);
+#define ALP_STREAM(JJ, XX)                                                                      \
+	hipLaunchKernelGGL((k_spmv_stream<R, MASK, OW, JJ, XX>), dim3(grid), dim3(kBlock), 0, s, nrb, \
+	                   rb, rbase, (const uint64_t *)S.off, (const uint32_t *)S.idx, val,          \
+	                   (const typename R::X *)x, (typename R::Y *)y, mask)
+	if (big) {
+		if (variant == 8) ALP_STREAM(16, false); else ALP_STREAM(16, true);
+	} else {
+		if (variant == 8) ALP_STREAM(8, false); else ALP_STREAM(8, true);
+	}
+#undef ALP_STREAM
 }
 
 int default_lanes(const Storage &S) {
@@ -161,12 +250,13 @@ static void dispatch(const Storage &S, const void *val, unsigned descr, void *y,
 	const int mmode = mask ? ((descr & ALP_DESC_INVERT_MASK) ? 2 : 1) : 0;
 	int kernel = S.kernel;
 	int G = S.lanes ? S.lanes : default_lanes(S);
+	int RR = S.rows ? S.rows : 1;  // CSR-stream: the launch variant
 	if (kernel == 0) kernel = (S.avg_len < 4 && S.max_len > 64) ? 2 : 1;
 #define ALP_GO(MM, OWV)                                                 \
 	if (kernel == 2)                                                    \
-		launch_stream<R, MM, OWV>(S, val, x, y, mask, s);               \
+		launch_stream<R, MM, OWV>(RR, S.lanes, S, val, x, y, mask, s);               \
 	else                                                                \
-		launch_vector<R, MM, OWV>(G, S, val, x, y, mask, s);
+		launch_vector<R, MM, OWV>(G, RR, S, val, x, y, mask, s);
 	if (ow) {
 		if (mmode == 0) { ALP_GO(0, true) }
 		else if (mmode == 1) { ALP_GO(1, true) }
@@ -247,28 +337,33 @@ ALP_EXPORT alp_rc_t alp_hip_vxm(alp_hip_matrix_t A, alp_semiring_t ring, unsigne
 	return spmv(A->ccs, A->value_type, ring, descr, y, x, mask, stream());
 }
 
-ALP_EXPORT alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes) {
+ALP_EXPORT alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes, int rows) {
 	if (!A || kernel < 0 || kernel > 2) return ALP_ILLEGAL;
 	if (lanes != 0 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 &&
 	    lanes != 64)
 		return ALP_ILLEGAL;
+	if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && !(kernel == 2 && rows == 8))
+		return ALP_ILLEGAL;
 	A->crs.kernel = A->ccs.kernel = kernel;
 	A->crs.lanes = A->ccs.lanes = lanes;
+	A->crs.rows = A->ccs.rows = rows;
 	return ALP_SUCCESS;
 }
 
-ALP_EXPORT alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes) {
+ALP_EXPORT alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes, int *rows) {
 	if (!A) return ALP_ILLEGAL;
 	int k = A->crs.kernel, l = A->crs.lanes ? A->crs.lanes : default_lanes(A->crs);
 	if (k == 0) k = (A->crs.avg_len < 4 && A->crs.max_len > 64) ? 2 : 1;
 	if (kernel) *kernel = k;
 	if (lanes) *lanes = l;
+	if (rows) *rows = A->crs.rows ? A->crs.rows : 1;
 	return ALP_SUCCESS;
 }
 
 // Autotuner: times each candidate on this matrix's CRS with its natural semiring
 // (3 warm-up + 10 timed runs each, hipEvents on the library stream) and keeps
-// the fastest. Candidates: CSR-vector with G = 2..64, CSR-stream.
+// the fastest. Candidates: CSR-vector with G = 2..64 lanes per row and 1, 2 or 4
+// rows per lane group, and CSR-stream. The CCS gets the same choice.
 ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 	ALP_TRY(ensure_init());
 	if (!A) return ALP_ILLEGAL;
@@ -284,14 +379,18 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 	hipEvent_t e0, e1;
 	ALP_HIP_TRY(hipEventCreate(&e0));
 	ALP_HIP_TRY(hipEventCreate(&e1));
-	struct Cand { int kernel, lanes; };
-	std::vector<Cand> cands = {{1, 2}, {1, 4}, {1, 8}, {1, 16}, {1, 32}, {1, 64}, {2, 0}};
+	struct Cand { int kernel, lanes, rows; };
+	std::vector<Cand> cands;
+	for (int g : {2, 4, 8, 16, 32, 64})
+		for (int r : {1, 2, 4}) cands.push_back({1, g, r});
+	for (int t : {8, 16})
+		for (int v : {1, 2, 4, 8}) cands.push_back({2, t, v});
 	float best = 1e30f;
-	Cand pick = {1, default_lanes(A->crs)};
-	const int k0 = A->crs.kernel, l0 = A->crs.lanes;
+	Cand pick = {1, default_lanes(A->crs), 1};
 	for (const Cand &c : cands) {
 		A->crs.kernel = c.kernel;
 		A->crs.lanes = c.lanes;
+		A->crs.rows = c.rows;
 		for (int i = 0; i < 3; ++i)
 			ALP_TRY(spmv(A->crs, A->value_type, ring, ALP_DESC_OVERWRITE, y, x, nullptr, stream()));
 		ALP_HIP_TRY(hipEventRecord(e0, stream()));
@@ -301,15 +400,17 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 		ALP_HIP_TRY(hipEventSynchronize(e1));
 		float ms = 0;
 		ALP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+		if (std::getenv("ALP_HIP_AUTOTUNE_VERBOSE"))
+			std::fprintf(stderr, "autotune kernel=%d lanes=%d rows=%d: %.1f us\n", c.kernel, c.lanes,
+			             c.rows, 100.0f * ms);
 		if (ms < best) {
 			best = ms;
 			pick = c;
 		}
 	}
-	(void)k0;
-	(void)l0;
 	A->crs.kernel = A->ccs.kernel = pick.kernel;
 	A->crs.lanes = A->ccs.lanes = pick.lanes;
+	A->crs.rows = A->ccs.rows = pick.rows;
 	hipEventDestroy(e0);
 	hipEventDestroy(e1);
 	hipFree(x);

@@ -97,6 +97,63 @@ __device__ __forceinline__ typename R::A row_gather(uint64_t beg, uint64_t end, 
 	return group_fold<G, R>(acc);
 }
 
+// Value load and multiply split apart so that all loads of a batch can be
+// issued before the first multiply (memory-level parallelism).
+template <typename R, bool P = R::kPattern>
+struct Val;
+template <typename R>
+struct Val<R, false> {
+	using T = typename R::V;
+	__device__ static T load(const T *__restrict__ val, uint64_t k) {
+		return __builtin_nontemporal_load(val + k);
+	}
+	__device__ static typename R::A mul(T v, typename R::X xj) { return R::mul(v, xj); }
+};
+template <typename R>
+struct Val<R, true> {
+	using T = char;
+	__device__ static T load(const void *, uint64_t) { return 0; }
+	__device__ static typename R::A mul(T, typename R::X xj) { return R::one_mul(xj); }
+};
+
+// RR rows per lane group, gathered together: each lane first issues the loads
+// of its first two elements of all RR rows (indices and values, then the x
+// gathers), and only then folds; rows longer than 2G finish in a tail loop.
+template <int G, int RR, typename R, typename VP>
+__device__ __forceinline__ void rows_gather(const uint64_t (&beg)[RR], const uint64_t (&end)[RR],
+                                            int lane, const uint32_t *__restrict__ idx, VP val,
+                                            const typename R::X *__restrict__ x,
+                                            typename R::A (&acc)[RR]) {
+	using VT = typename Val<R>::T;
+	uint32_t j[RR][2];
+	VT v[RR][2];
+#pragma unroll
+	for (int i = 0; i < RR; ++i)
+#pragma unroll
+		for (int h = 0; h < 2; ++h) {
+			const uint64_t k = beg[i] + lane + h * G;
+			j[i][h] = 0;
+			v[i][h] = VT(0);
+			if (k < end[i]) {
+				j[i][h] = __builtin_nontemporal_load(idx + k);
+				v[i][h] = Val<R>::load(val, k);
+			}
+		}
+#pragma unroll
+	for (int i = 0; i < RR; ++i) {
+		typename R::A p0 = R::zero(), p1 = R::zero();
+		if (beg[i] + lane < end[i]) p0 = Val<R>::mul(v[i][0], x[j[i][0]]);
+		if (beg[i] + lane + G < end[i]) p1 = Val<R>::mul(v[i][1], x[j[i][1]]);
+		acc[i] = R::add(p0, p1);
+	}
+#pragma unroll
+	for (int i = 0; i < RR; ++i)
+		for (uint64_t k = beg[i] + lane + 2 * G; k < end[i]; k += G)
+			acc[i] = R::add(acc[i], Val<R>::mul(Val<R>::load(val, k), x[__builtin_nontemporal_load(idx + k)]));
+#pragma unroll
+	for (int i = 0; i < RR; ++i) acc[i] = group_fold<G, R>(acc[i]);
+}
+
 // ---- persistent-grid work split -----------------------------------------------
 // `nitems` units (row chunks) are split into one contiguous region per XCD
 // (sized by that XCD's share of the grid); inside a region the XCD's blocks take

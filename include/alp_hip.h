@@ -75,6 +75,13 @@ const char *alp_hip_version(void);
 /* number of gfx950 devices; 0 without a GPU (never initialises the runtime's device) */
 int alp_hip_device_count(void);
 
+/* ---- timing on the library stream (the role of grb::utils::Timer,
+ *      include/graphblas/utils/timer.hpp, for device work) ----
+ * alp_hip_timer_start records a HIP event on the current stream; alp_hip_timer_stop
+ * records a second one, waits for it and returns the elapsed milliseconds. */
+alp_rc_t alp_hip_timer_start(void);
+alp_rc_t alp_hip_timer_stop(double *ms);
+
 /* ---- device memory ---- */
 alp_rc_t alp_hip_malloc(void **d_ptr, size_t bytes);
 alp_rc_t alp_hip_free(void *d_ptr);
@@ -110,9 +117,11 @@ alp_rc_t alp_hip_matrix_info(alp_hip_matrix_t A, size_t *nrows, size_t *ncols, s
 /* download CRS (which = 0) or CCS (which = 1); h_val may be NULL */
 alp_rc_t alp_hip_matrix_download(alp_hip_matrix_t A, int which, uint64_t *h_off,
                                  uint32_t *h_idx, void *h_val);
-/* select the SpMV kernel: 0 = auto (autotuned), 1 = CSR-vector, 2 = CSR-stream */
-alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes);
-alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes);
+/* select the SpMV kernel: 0 = auto, 1 = CSR-vector (`lanes` per row: 2..64, `rows`
+ * per lane group: 1, 2 or 4; 0 = default), 2 = CSR-stream (`rows` picks the launch:
+ * 1 = persistent grid 4 blocks/CU, 2 = persistent 8 blocks/CU, 4 = a block per tile) */
+alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes, int rows);
+alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes, int *rows);
 /* time each candidate SpMV kernel on this matrix and keep the fastest */
 alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A);
 

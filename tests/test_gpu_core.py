@@ -103,14 +103,15 @@ def closed_form_A1(nx, ny, nz):
     return (27.0 - prod).reshape(-1)
 
 
-KERNELS = [(1, 2), (1, 4), (1, 8), (1, 16), (1, 32), (1, 64), (2, 0)]
+KERNELS = [(1, 2, 1), (1, 4, 2), (1, 8, 4), (1, 16, 1), (1, 16, 4), (1, 32, 2), (1, 64, 1), (2, 0, 1),
+           (2, 0, 2), (2, 0, 4), (2, 8, 8), (2, 16, 1), (2, 16, 4), (2, 16, 8)]
 
 
-@pytest.mark.parametrize("kernel,lanes", KERNELS)
+@pytest.mark.parametrize("kernel,lanes,rows", KERNELS)
 @pytest.mark.parametrize("sizes", [(64, 64, 64), (192, 192, 192)])
-def test_mxv_ones_closed_form_bitexact(A, sizes, kernel, lanes):
+def test_mxv_ones_closed_form_bitexact(A, sizes, kernel, lanes, rows):
     M = A.Matrix.stencil27(*sizes)
-    M.set_kernel(kernel, lanes)
+    M.set_kernel(kernel, lanes, rows)
     n = M.nrows
     x = A.Vector(n, fill=1.0)
     y = A.Vector(n, fill=0.0)
@@ -122,13 +123,13 @@ def test_mxv_ones_closed_form_bitexact(A, sizes, kernel, lanes):
     assert np.array_equal(y.numpy(), closed_form_A1(*sizes))
 
 
-@pytest.mark.parametrize("kernel,lanes", KERNELS)
+@pytest.mark.parametrize("kernel,lanes,rows", KERNELS)
 @pytest.mark.parametrize("name,sizes", [("mxv_8x8x8", (8, 8, 8)), ("mxv_5x9x7", (5, 9, 7)),
                                         ("mxv_16x16x16", (16, 16, 16))])
-def test_mxv_matches_reference_fixture(A, golden, name, sizes, kernel, lanes):
+def test_mxv_matches_reference_fixture(A, golden, name, sizes, kernel, lanes, rows):
     g = golden(name)
     M = A.Matrix.stencil27(*sizes)
-    M.set_kernel(kernel, lanes)
+    M.set_kernel(kernel, lanes, rows)
     x = A.Vector.from_numpy(g["x"])
     y = A.Vector.from_numpy(g["y0"])
     assert A.mxv(y, M, x) == A.SUCCESS
@@ -158,14 +159,14 @@ def test_mxv_random_matrix_vs_oracle_all_kernels(A):
     yt0 = rng.standard_normal(m)
     wantT = O.spmv(refT, xt, yt0)
     M = A.Matrix.from_coo(n, m, I, J, V, flags=A.MAT_WITH_CCS)
-    for kernel, lanes in KERNELS:
-        M.set_kernel(kernel, lanes)
+    for kernel, lanes, rows in KERNELS:
+        M.set_kernel(kernel, lanes, rows)
         y = A.Vector.from_numpy(y0)
         assert A.mxv(y, M, A.Vector.from_numpy(x)) == A.SUCCESS
-        assert rel_inf(y.numpy(), want) <= FP64_RTOL, (kernel, lanes)
+        assert rel_inf(y.numpy(), want) <= FP64_RTOL, (kernel, lanes, rows)
         yt = A.Vector.from_numpy(yt0)
         assert A.vxm(yt, A.Vector.from_numpy(xt), M) == A.SUCCESS  # x A = A^T x: the CCS gather
-        assert rel_inf(yt.numpy(), wantT) <= FP64_RTOL, (kernel, lanes)
+        assert rel_inf(yt.numpy(), wantT) <= FP64_RTOL, (kernel, lanes, rows)
 
 
 def test_mxv_f32(A):
@@ -185,8 +186,8 @@ def test_lorland_masked_mxv_bitexact(A):
     I, J = O.symmetrize_dedup(n, s.astype(np.uint32), d.astype(np.uint32))
     ref = O.coo_to_compressed(n, n, I, J, which=0)
     M = A.Matrix.from_coo(n, n, I, J)
-    for kernel, lanes in KERNELS:
-        M.set_kernel(kernel, lanes)
+    for kernel, lanes, rows in KERNELS:
+        M.set_kernel(kernel, lanes, rows)
         x = (rng.random(n) < 0.05).astype(np.uint8)
         y0 = (rng.random(n) < 0.1).astype(np.uint8)
         mask = (rng.random(n) < 0.5).astype(np.uint8)
@@ -216,8 +217,8 @@ def test_pattern_row_sums(A):
 
 def test_autotune_picks_a_valid_kernel(A):
     M = A.Matrix.stencil27(64, 64, 64)
-    k, lanes = M.autotune()
-    assert k in (1, 2)
+    k, lanes, rows = M.autotune()
+    assert k in (1, 2) and rows in (1, 2, 4)
     x = A.Vector(M.nrows, fill=1.0)
     y = A.Vector(M.nrows, fill=0.0)
     A.mxv(y, M, x)
