@@ -104,6 +104,13 @@ SIGNATURES = {
     "alp_hip_convert_f64_to_f32": (_RC, [_S, _P, _P]),
     "alp_hip_convert_f32_to_f64": (_RC, [_S, _P, _P]),
     "alp_hip_cg": (_RC, [_P, _P, _P, _S, _D, _I, ctypes.POINTER(_S), ctypes.POINTER(_D)]),
+    "alp_hip_hpcg_create": (_RC, [ctypes.POINTER(_P), _S, _S, _S, _S, _S, _S, _S, _U]),
+    "alp_hip_hpcg_destroy": (_RC, [_P]),
+    "alp_hip_hpcg_info": (_RC, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S),
+                                 ctypes.POINTER(_I)]),
+    "alp_hip_hpcg_vcycle": (_RC, [_P, _P, _P]),
+    "alp_hip_hpcg_solve": (_RC, [_P, _P, _P, _S, _D, _I, _I, ctypes.POINTER(_S), ctypes.POINTER(_D),
+                                  ctypes.POINTER(_S)]),
 }
 
 _lib = None
@@ -315,3 +322,39 @@ def cg(A: Matrix, b: Vector, x: Vector, max_iter: int, tol: float, precision: in
     it, res = _S(), _D()
     rc = lib().alp_hip_cg(A.h, b.ptr, x.ptr, max_iter, tol, precision, ctypes.byref(it), ctypes.byref(res))
     return rc, it.value, res.value
+
+
+class Hpcg:
+    """The HPCG multigrid hierarchy on the device (build_hpcg_system + hpcg)."""
+
+    def __init__(self, nx, ny, nz, max_levels=3, colours=0, presmooth=1, postsmooth=1, f32=True):
+        h = _P()
+        check(lib().alp_hip_hpcg_create(ctypes.byref(h), nx, ny, nz, max_levels, colours, presmooth,
+                                        postsmooth, MAT_WITH_F32 if f32 else 0), "hpcg_create")
+        self.h = h.value
+        n, nnz, lv, pr = _S(), _S(), _S(), _I()
+        check(lib().alp_hip_hpcg_info(self.h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(lv),
+                                      ctypes.byref(pr)), "hpcg_info")
+        self.n, self.nnz, self.levels, self.proper = n.value, nnz.value, lv.value, bool(pr.value)
+
+    def vcycle(self, r: Vector, z: Vector) -> None:
+        check(lib().alp_hip_hpcg_vcycle(self.h, r.ptr, z.ptr), "hpcg_vcycle")
+
+    def solve(self, b: Vector, x: Vector, max_iter: int, tol: float, precision: int = 0,
+              with_precond: bool = True):
+        """Returns (iterations, ||r||, refinement steps)."""
+        it, res, outer = _S(), _D(), _S()
+        check(lib().alp_hip_hpcg_solve(self.h, b.ptr, x.ptr, max_iter, tol, precision, int(with_precond),
+                                       ctypes.byref(it), ctypes.byref(res), ctypes.byref(outer)), "hpcg_solve")
+        return it.value, res.value, outer.value
+
+    def free(self) -> None:
+        if getattr(self, "h", None):
+            lib().alp_hip_hpcg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

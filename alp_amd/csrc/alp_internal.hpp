@@ -145,10 +145,14 @@ struct Storage {
 	uint32_t *row_blocks_l = nullptr;
 	uint64_t *row_block_base_l = nullptr;
 	size_t n_row_blocks_l = 0;
+	// wavefront tiles (<= 512 nonzeros, <= 64 rows) of the CSR-wave kernel
+	uint32_t *row_blocks_w = nullptr;
+	uint64_t *row_block_base_w = nullptr;
+	size_t n_row_blocks_w = 0;
 	// statistics for kernel selection
 	double avg_len = 0;
 	size_t max_len = 0;
-	int kernel = 0;  // 0 auto, 1 vector, 2 stream
+	int kernel = 0;  // 0 auto, 1 vector, 2 stream, 3 wave
 	int lanes = 0;
 	int rows = 0;    // rows per lane group of the CSR-vector kernel
 };

@@ -1,0 +1,963 @@
+// HPCG on the device: grb::algorithms::hpcg (algorithms/hpcg/hpcg.hpp:97-223) with
+// its multigrid V-cycle (multigrid_v_cycle.hpp:67-246), red-black Gauss-Seidel
+// smoother (red_black_gauss_seidel.hpp:55-123) and system builder
+// (system_building_utils.hpp:84-150).
+//
+// Layout. Every level keeps its 27-point matrix in colour-permuted CRS: rows are
+// renumbered so that each colour's rows are contiguous (stable within a colour),
+// columns follow the same renumbering. A colour sweep then streams one
+// contiguous block of the matrix, and the whole solve runs in permuted space
+// (b is gathered in and x scattered out once per solve).
+//
+// Smoother. The reference's colour step is t = A[mask,:] z over a snapshot of z,
+// then z[i] = ((r[i] - t[i]) + z[i] d) / d for the masked rows. When the colouring
+// is proper (no two neighbours share a colour, checked on the device at build
+// time) no row of a colour reads another row of that colour, so the step runs as
+// one fused tile kernel; otherwise (the reference's index-mod-k masks) it runs as
+// the two phases the reference has.
+//
+// Grid transfers. The coarsener of build_ndims_coarsener_matrix has one 1.0 per
+// coarse row (injection at even coordinates, ndim_matrix_builders.hpp:520-530), so
+// restriction r_c = R (r - A z) only needs (A z) at the injected fine rows: one
+// fused kernel gathers those rows, and also zeroes z_c (multi_grid's set(z, 0)).
+// Prolongation z += R^T z_c scatters along the same map.
+//
+// Precision. precision = 0: everything FP64. precision = 1: the bulk (SpMV,
+// smoother, grid transfers, inner CG vectors) in FP32 inside an FP64 iterative
+// refinement whose residual b - A x and solution update stay FP64.
+#include "spmv_device.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <vector>
+
+namespace alp {
+
+constexpr int kJ = 8;                  // tile loads per thread
+constexpr int kTileNnz = kJ * kBlock;  // 2048 nonzeros per tile
+
+struct HpLevel {
+	size_t nx = 0, ny = 0, nz = 0, n = 0, nnz = 0;
+	uint64_t *off = nullptr;
+	uint32_t *idx = nullptr;
+	double *val = nullptr;
+	float *val32 = nullptr;
+	double *diag = nullptr;
+	float *diag32 = nullptr;
+	uint32_t *perm = nullptr;   // new -> old
+	uint32_t *iperm = nullptr;  // old -> new
+	uint8_t *colour = nullptr;  // colour of each new row
+	int ncolors = 0;
+	bool proper = false;
+	std::vector<size_t> color_off;       // rows of colour c: [color_off[c], color_off[c+1])
+	std::vector<size_t> color_tile_off;  // its tiles: [color_tile_off[c], color_tile_off[c+1])
+	uint32_t *tile_rb = nullptr;         // tile t covers rows [tile_rb[t], tile_rb[t+1]) ...
+	uint32_t *tile_re = nullptr;         // ... ending at tile_re[t] (tiles of a colour are contiguous)
+	uint64_t *tile_base = nullptr;       // off[tile_rb[t]]
+	uint64_t *tile_top = nullptr;        // off[tile_re[t]]
+	size_t ntiles = 0;
+	uint32_t *inj = nullptr;             // coarse level: coarse new row -> fine new row
+	double *z = nullptr, *r = nullptr, *t = nullptr;
+	float *z32 = nullptr, *r32 = nullptr, *t32 = nullptr;
+};
+
+struct HpState {
+	double rz, old_rz, beta, alpha, rr, norm, norm0, tol;
+	unsigned long long iter, max_iter;
+	int done, pad[3];
+};
+
+struct IrState {
+	double rnorm, norm0, inv, pad;
+	int done, pad2[3];
+};
+
+template <typename T> struct Vec;
+template <> struct Vec<double> {
+	static double *z(HpLevel &L) { return L.z; }
+	static double *r(HpLevel &L) { return L.r; }
+	static double *t(HpLevel &L) { return L.t; }
+	static const double *val(const HpLevel &L) { return L.val; }
+	static const double *diag(const HpLevel &L) { return L.diag; }
+};
+template <> struct Vec<float> {
+	static float *z(HpLevel &L) { return L.z32; }
+	static float *r(HpLevel &L) { return L.r32; }
+	static float *t(HpLevel &L) { return L.t32; }
+	static const float *val(const HpLevel &L) { return L.val32; }
+	static const float *diag(const HpLevel &L) { return L.diag32; }
+};
+
+// ---- epilogues of the tile fold ------------------------------------------------
+
+template <typename T>
+struct EpStore {  // y = A x (overwrite)
+	T *y;
+	__device__ void prefetch(uint32_t) {}
+	__device__ void finish(uint32_t r, T acc, bool empty) { y[r] = empty ? T(0) : acc; }
+};
+
+template <typename T>
+struct EpSgs {  // fused colour step: z_r = ((r_r - (A z)_r) + z_r d_r) / d_r
+	const T *rr;
+	const T *d;
+	T *z;
+	T ri, di, zi;
+	__device__ void prefetch(uint32_t r) {
+		ri = rr[r];
+		di = d[r];
+		zi = z[r];
+	}
+	__device__ void finish(uint32_t r, T acc, bool) { z[r] = ((ri - acc) + zi * di) / di; }
+};
+
+template <typename T>
+struct EpStoreDot {  // y = A p and the partial of p.y
+	T *y;
+	const T *p;
+	T pi;
+	double part;
+	__device__ void prefetch(uint32_t r) { pi = p[r]; }
+	__device__ void finish(uint32_t r, T acc, bool empty) {
+		const T v = empty ? T(0) : acc;
+		y[r] = v;
+		part += (double)v * (double)pi;
+	}
+};
+
+template <typename T>
+struct EpResid {  // r = b - A x and the partial of r.r (FP64 outer residual)
+	const T *b;
+	T *rout;
+	T bi;
+	double part;
+	__device__ void prefetch(uint32_t r) { bi = b[r]; }
+	__device__ void finish(uint32_t r, T acc, bool empty) {
+		const T v = bi - (empty ? T(0) : acc);
+		rout[r] = v;
+		part += (double)v * (double)v;
+	}
+};
+
+// ---- tile kernels ---------------------------------------------------------------
+
+struct Tiles {
+	const uint32_t *rb, *re;
+	const uint64_t *base, *top;
+	size_t first, count;
+};
+
+// One block per tile; the epilogue decides what the fold feeds.
+template <typename T, typename Ep>
+__global__ __launch_bounds__(kBlock) void k_tiles(Tiles tl, const uint64_t *__restrict__ off,
+                                                  const uint32_t *__restrict__ idx,
+                                                  const T *__restrict__ val,
+                                                  const T *__restrict__ x, Ep ep,
+                                                  const int *done) {
+	__shared__ T tile[kTileNnz];
+	__shared__ T red[kBlock / kWave];
+	if (done && *done) return;
+	for (size_t b = blockIdx.x; b < tl.count; b += gridDim.x) {
+		const size_t t = tl.first + b;
+		const uint32_t r0 = tl.rb[t], r1 = tl.re[t];
+		const uint64_t base = tl.base[t], top = tl.top[t];
+		if (top - base <= (uint64_t)kTileNnz)
+			tile_fold<PlusTimes<T>, kJ>(r0, r1, base, top, off, idx, val, x, tile, ep);
+		else
+			long_row_fold<PlusTimes<T>>(r0, base, top, idx, val, x, red, ep);
+	}
+}
+
+// The same with a deterministic reduction of ep.part: a persistent grid, one
+// partial per block, the last block folds the partials and calls Fin.
+template <typename T, typename Ep, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_tiles_reduce(Tiles tl, const uint64_t *__restrict__ off,
+                                                         const uint32_t *__restrict__ idx,
+                                                         const T *__restrict__ val,
+                                                         const T *__restrict__ x, Ep ep,
+                                                         double *partials, unsigned *ticket,
+                                                         Fin fin, const int *done) {
+	__shared__ T tile[kTileNnz];
+	__shared__ T red[kBlock / kWave];
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	if (done && *done) return;
+	ep.part = 0.0;
+	for (size_t b = blockIdx.x; b < tl.count; b += gridDim.x) {
+		const size_t t = tl.first + b;
+		const uint32_t r0 = tl.rb[t], r1 = tl.re[t];
+		const uint64_t base = tl.base[t], top = tl.top[t];
+		if (top - base <= (uint64_t)kTileNnz)
+			tile_fold<PlusTimes<T>, kJ>(r0, r1, base, top, off, idx, val, x, tile, ep);
+		else
+			long_row_fold<PlusTimes<T>>(r0, base, top, idx, val, x, red, ep);
+	}
+	const double p = block_sum(ep.part, lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = p;
+	if (last_block(ticket, &is_last)) {
+		const double s = block_sum_partials(partials, gridDim.x, lds);
+		if (threadIdx.x == 0) fin(s);
+	}
+}
+
+// phase 2 of a snapshot colour step: z = ((r - t) + z d) / d over rows [r0, r1)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sgs_update(size_t r0, size_t r1, const T *r, const T *t,
+                                                       const T *d, T *z, const int *done) {
+	if (done && *done) return;
+	for (size_t i = r0 + (size_t)blockIdx.x * kBlock + threadIdx.x; i < r1; i += (size_t)gridDim.x * kBlock) {
+		const T di = d[i];
+		z[i] = ((r[i] - t[i]) + z[i] * di) / di;
+	}
+}
+
+// restriction: r_c[i] = r_f[f] - (A_f z_f)[f] with f = inj[i], and z_c[i] = 0
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict(size_t nc, const uint32_t *__restrict__ inj,
+                                                     const uint64_t *__restrict__ off,
+                                                     const uint32_t *__restrict__ idx,
+                                                     const T *__restrict__ val,
+                                                     const T *__restrict__ zf,
+                                                     const T *__restrict__ rf, T *rc, T *zc,
+                                                     const int *done) {
+	constexpr int G = 32;
+	if (done && *done) return;
+	const size_t i = ((size_t)blockIdx.x * kBlock + threadIdx.x) / G;
+	const int lane = threadIdx.x % G;
+	if (i >= nc) return;
+	const uint32_t f = inj[i];
+	uint64_t beg[1] = {off[f]}, end[1] = {off[f + 1]};
+	T acc[1];
+	rows_gather<G, 1, PlusTimes<T>>(beg, end, lane, idx, val, zf, acc);
+	if (lane == 0) {
+		rc[i] = rf[f] - acc[0];
+		zc[i] = T(0);
+	}
+}
+
+// prolongation: z_f[inj[i]] += z_c[i]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_prolong(size_t nc, const uint32_t *inj, const T *zc,
+                                                    T *zf, const int *done) {
+	if (done && *done) return;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < nc; i += (size_t)gridDim.x * kBlock)
+		zf[inj[i]] += zc[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_done(size_t n, T *x, T v, const int *done) {
+	if (done && *done) return;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		x[i] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copy_done(size_t n, const T *s, T *d, const int *done) {
+	if (done && *done) return;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		d[i] = s[i];
+}
+
+// ---- CG part of hpcg (hpcg.hpp:148-221) -------------------------------------------
+
+// rz = r.z; beta = rz / old_rz (iteration 0: p = z)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hp_rz(size_t n, const T *r, const T *z, double *partials,
+                                                  unsigned *ticket, HpState *st) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	if (st->done) return;
+	double p = 0.0;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		p += (double)r[i] * (double)z[i];
+	p = block_sum(p, lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = p;
+	if (last_block(ticket, &is_last)) {
+		const double rz = block_sum_partials(partials, gridDim.x, lds);
+		if (threadIdx.x == 0) {
+			st->old_rz = st->rz;
+			st->rz = rz;
+			st->beta = st->iter == 0 ? 0.0 : rz / st->old_rz;
+		}
+	}
+}
+
+// p = beta p + z (iteration 0: p = z)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hp_p(size_t n, T *p, const T *z, const HpState *st) {
+	if (st->done) return;
+	const bool first = st->iter == 0;
+	const T beta = (T)st->beta;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		p[i] = first ? z[i] : beta * p[i] + z[i];
+}
+
+struct FinAlpha {
+	HpState *st;
+	__device__ void operator()(double pAp) const { st->alpha = st->rz / pAp; }
+};
+
+// x += alpha p; r += (-alpha) Ap; norm = sqrt(r.r); stop when !(iter < max && norm/norm0 > tol)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hp_update(size_t n, T *x, T *r, const T *p, const T *Ap,
+                                                      double *partials, unsigned *ticket, HpState *st) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	if (st->done) return;
+	const T alpha = (T)st->alpha;
+	double q = 0.0;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+		x[i] = alpha * p[i] + x[i];
+		const T ri = (-alpha) * Ap[i] + r[i];
+		r[i] = ri;
+		q += (double)ri * (double)ri;
+	}
+	q = block_sum(q, lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = q;
+	if (last_block(ticket, &is_last)) {
+		const double rr = block_sum_partials(partials, gridDim.x, lds);
+		if (threadIdx.x == 0) {
+			st->rr = rr;
+			st->norm = std::sqrt(rr);
+			st->iter += 1;
+			if (!(st->iter < st->max_iter && st->norm / st->norm0 > st->tol)) st->done = 1;
+		}
+	}
+}
+
+struct FinInit {  // prelude: norm0 = ||b - A x||
+	HpState *st;
+	double tol;
+	unsigned long long max_iter;
+	__device__ void operator()(double rr) const {
+		st->rr = rr;
+		st->norm = std::sqrt(rr);
+		st->norm0 = st->norm;
+		st->tol = tol;
+		st->iter = 0;
+		st->max_iter = max_iter;
+		st->rz = st->old_rz = st->beta = st->alpha = 0.0;
+		st->done = max_iter == 0 ? 1 : 0;
+	}
+};
+
+struct FinIr {  // FP64 outer residual of the refinement loop
+	IrState *ir;
+	double tol;
+	int first;
+	__device__ void operator()(double rr) const {
+		ir->rnorm = std::sqrt(rr);
+		if (first) ir->norm0 = ir->rnorm;
+		ir->inv = ir->rnorm > 0 ? 1.0 / ir->rnorm : 0.0;
+		ir->done = !(ir->rnorm / ir->norm0 > tol) ? 1 : 0;
+	}
+};
+
+__global__ __launch_bounds__(kBlock) void k_ir_down(size_t n, const double *r, float *r32, float *x32,
+                                                   const IrState *ir) {
+	const double s = ir->inv;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+		r32[i] = (float)(r[i] * s);
+		x32[i] = 0.0f;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_ir_up(size_t n, double *x, const float *d32, const IrState *ir) {
+	const double s = ir->rnorm;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		x[i] = s * (double)d32[i] + x[i];
+}
+
+// ---- permutations ------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather_perm(size_t n, const uint32_t *perm, const T *src, T *dst) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		dst[i] = src[perm[i]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scatter_perm(size_t n, const uint32_t *perm, const T *src, T *dst) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		dst[perm[i]] = src[i];
+}
+
+// ---- hierarchy build ------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_colour_keys(size_t nx, size_t ny, size_t nz, int colours,
+                                                       uint32_t *key, uint32_t *iota) {
+	const size_t n = nx * ny * nz;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+		uint32_t c;
+		if (colours == 0) {
+			const size_t x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+			c = (uint32_t)((x & 1) + 2 * (y & 1) + 4 * (z & 1));
+		} else {
+			c = (uint32_t)(i % (size_t)colours);
+		}
+		key[i] = c;
+		iota[i] = (uint32_t)i;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_inverse_perm(size_t n, const uint32_t *perm, uint32_t *iperm) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		iperm[perm[i]] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_perm_lengths(size_t n, const uint32_t *perm,
+                                                        const uint64_t *off, uint64_t *len) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+		const uint32_t o = perm[i];
+		len[i] = off[o + 1] - off[o];
+	}
+}
+
+// row i' of the permuted matrix = old row perm[i'], columns renumbered by iperm,
+// entries kept in their stored order; the diagonal is extracted on the way
+__global__ __launch_bounds__(kBlock) void k_perm_fill(size_t n, const uint32_t *perm,
+                                                     const uint32_t *iperm, const uint64_t *off,
+                                                     const uint32_t *idx, const double *val,
+                                                     const uint64_t *noff, uint32_t *nidx,
+                                                     double *nval, double *diag) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+		const uint32_t o = perm[i];
+		uint64_t w = noff[i];
+		double d = 0.0;
+		for (uint64_t k = off[o]; k < off[o + 1]; ++k, ++w) {
+			const uint32_t c = idx[k];
+			nidx[w] = iperm[c];
+			nval[w] = val[k];
+			if (c == o) d = val[k];
+		}
+		diag[i] = d;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_colour_of_new(size_t n, const uint32_t *sorted_key, uint8_t *col) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+		col[i] = (uint8_t)sorted_key[i];
+}
+
+// flags[0] = 1 if some row has an off-diagonal neighbour of its own colour
+__global__ __launch_bounds__(kBlock) void k_check_proper(size_t n, const uint64_t *off, const uint32_t *idx,
+                                                        const uint8_t *col, int *flag) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+		const uint8_t c = col[i];
+		for (uint64_t k = off[i]; k < off[i + 1]; ++k) {
+			const uint32_t j = idx[k];
+			if (j != i && col[j] == c) {
+				atomicOr(flag, 1);
+				break;
+			}
+		}
+	}
+}
+
+// coarse new row i -> fine new row at (2x, 2y, 2z) of its coarse coordinates
+__global__ __launch_bounds__(kBlock) void k_injection(size_t nc, size_t nxc, size_t nyc, size_t nxf,
+                                                     size_t nyf, const uint32_t *perm_c,
+                                                     const uint32_t *iperm_f, uint32_t *inj) {
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < nc; i += (size_t)gridDim.x * kBlock) {
+		const size_t o = perm_c[i];
+		const size_t x = o % nxc, y = (o / nxc) % nyc, z = o / (nxc * nyc);
+		const size_t f = 2 * x + nxf * (2 * y + nyf * 2 * z);
+		inj[i] = iperm_f[f];
+	}
+}
+
+static unsigned grid_for(size_t n) {
+	size_t g = (n + kBlock - 1) / kBlock;
+	const size_t cap = fill_blocks(8);
+	if (g > cap) g = cap;
+	if (g < 1) g = 1;
+	return (unsigned)g;
+}
+
+static unsigned reduce_grid(size_t n) {
+	size_t g = (n + kBlock - 1) / kBlock;
+	const size_t cap = fill_blocks(4);
+	if (g > cap) g = cap;
+	if (g > (size_t)kMaxPartials) g = kMaxPartials;
+	if (g < 1) g = 1;
+	return (unsigned)g;
+}
+
+}  // namespace alp
+
+using namespace alp;
+
+struct alp_hip_hpcg_s {
+	std::vector<HpLevel> L;
+	size_t presmooth = 1, postsmooth = 1;
+	bool have_f32 = false;
+	HpState *st = nullptr, *st32 = nullptr;
+	IrState *ir = nullptr;
+	double *p = nullptr, *Ap = nullptr, *b = nullptr, *x = nullptr;  // finest level, FP64
+	float *p32 = nullptr, *Ap32 = nullptr, *x32 = nullptr;           // finest level, FP32
+	int *never = nullptr;  // a device int kept 0: the "done" of kernels outside a CG
+};
+
+namespace alp {
+
+static void free_level(HpLevel &L) {
+	void *ps[] = {L.off, L.idx, L.val, L.val32, L.diag, L.diag32, L.perm, L.iperm, L.colour,
+	              L.tile_rb, L.tile_re, L.tile_base, L.tile_top, L.inj, L.z, L.r, L.t,
+	              L.z32, L.r32, L.t32};
+	for (void *p : ps)
+		if (p) hipFree(p);
+	L = HpLevel();
+}
+
+template <typename T>
+static alp_rc_t dmalloc(T **p, size_t count) {
+	ALP_HIP_TRY(hipMalloc(p, sizeof(T) * (count ? count : 1)));
+	return ALP_SUCCESS;
+}
+
+// Build one level: the stencil (device generator), its colouring, the colour
+// permutation, the permuted CRS, diagonal, properness, tiles per colour.
+static alp_rc_t build_level(HpLevel &L, size_t nx, size_t ny, size_t nz, int colours, bool f32) {
+	hipStream_t s = stream();
+	L.nx = nx;
+	L.ny = ny;
+	L.nz = nz;
+	L.n = nx * ny * nz;
+	const size_t n = L.n;
+	alp_hip_matrix_t A = nullptr;
+	ALP_TRY(alp_hip_matrix_stencil27(&A, nx, ny, nz, 26.0, -1.0, 0));
+	L.nnz = A->nnz;
+	const size_t nnz = L.nnz;
+	L.ncolors = colours == 0 ? 8 : colours;
+	alp_rc_t rc = ALP_SUCCESS;
+	uint32_t *key = nullptr, *iota = nullptr, *skey = nullptr;
+	uint64_t *len = nullptr;
+	void *tmp = nullptr;
+	do {
+		if ((rc = dmalloc(&key, n)) || (rc = dmalloc(&iota, n)) || (rc = dmalloc(&skey, n)) ||
+		    (rc = dmalloc(&L.perm, n)) || (rc = dmalloc(&L.iperm, n)) || (rc = dmalloc(&len, n + 1)) ||
+		    (rc = dmalloc(&L.off, n + 1)) || (rc = dmalloc(&L.idx, nnz)) || (rc = dmalloc(&L.val, nnz)) ||
+		    (rc = dmalloc(&L.diag, n)) || (rc = dmalloc(&L.colour, n)) || (rc = dmalloc(&L.z, n)) ||
+		    (rc = dmalloc(&L.r, n)) || (rc = dmalloc(&L.t, n)))
+			break;
+		hipLaunchKernelGGL(k_colour_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, nx, ny, nz, colours, key, iota);
+		int end_bit = 1;
+		while ((1 << end_bit) < L.ncolors) ++end_bit;
+		size_t tb = 0;
+		hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, skey, iota, L.perm, (int)n, 0, end_bit, s);
+		if ((rc = dmalloc((char **)&tmp, tb))) break;
+		if (hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, skey, iota, L.perm, (int)n, 0, end_bit, s) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		hipFree(tmp);
+		tmp = nullptr;
+		hipLaunchKernelGGL(k_inverse_perm, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const uint32_t *)L.perm, L.iperm);
+		hipLaunchKernelGGL(k_colour_of_new, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const uint32_t *)skey, L.colour);
+		hipLaunchKernelGGL(k_perm_lengths, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const uint32_t *)L.perm,
+		                   (const uint64_t *)A->crs.off, len);
+		hipMemsetAsync(len + n, 0, sizeof(uint64_t), s);
+		tb = 0;
+		hipcub::DeviceScan::ExclusiveSum(nullptr, tb, len, L.off, n + 1, s);
+		if ((rc = dmalloc((char **)&tmp, tb))) break;
+		if (hipcub::DeviceScan::ExclusiveSum(tmp, tb, len, L.off, n + 1, s) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		hipLaunchKernelGGL(k_perm_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const uint32_t *)L.perm,
+		                   (const uint32_t *)L.iperm, (const uint64_t *)A->crs.off, (const uint32_t *)A->crs.idx,
+		                   (const double *)A->crs.val, (const uint64_t *)L.off, L.idx, L.val, L.diag);
+		int *flag = nullptr;
+		if ((rc = dmalloc(&flag, 1))) break;
+		hipMemsetAsync(flag, 0, sizeof(int), s);
+		hipLaunchKernelGGL(k_check_proper, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const uint64_t *)L.off,
+		                   (const uint32_t *)L.idx, (const uint8_t *)L.colour, flag);
+		int hflag = 0;
+		if (hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+		    hipStreamSynchronize(s) != hipSuccess) {
+			hipFree(flag);
+			rc = ALP_PANIC;
+			break;
+		}
+		hipFree(flag);
+		L.proper = hflag == 0;
+		if (hipGetLastError() != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if (f32) {
+			if ((rc = dmalloc(&L.val32, nnz)) || (rc = dmalloc(&L.diag32, n)) || (rc = dmalloc(&L.z32, n)) ||
+			    (rc = dmalloc(&L.r32, n)) || (rc = dmalloc(&L.t32, n)))
+				break;
+			if ((rc = alp_hip_convert_f64_to_f32(nnz, L.val, L.val32))) break;
+			if ((rc = alp_hip_convert_f64_to_f32(n, L.diag, L.diag32))) break;
+		}
+		// colour ranges and tiles (host, from the permuted offsets)
+		std::vector<uint32_t> hkey(n);
+		std::vector<uint64_t> hoff(n + 1);
+		if (hipMemcpy(hkey.data(), skey, sizeof(uint32_t) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+		    hipMemcpy(hoff.data(), L.off, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		L.color_off.assign(L.ncolors + 1, 0);
+		for (size_t i = 0; i < n; ++i) L.color_off[hkey[i] + 1]++;
+		for (int c = 0; c < L.ncolors; ++c) L.color_off[c + 1] += L.color_off[c];
+		std::vector<uint32_t> rb, re;
+		std::vector<uint64_t> bb, tt;
+		L.color_tile_off.assign(L.ncolors + 1, 0);
+		for (int c = 0; c < L.ncolors; ++c) {
+			size_t r = L.color_off[c];
+			const size_t rend = L.color_off[c + 1];
+			while (r < rend) {
+				size_t e = r + 1;
+				while (e < rend && hoff[e + 1] - hoff[r] <= (uint64_t)kTileNnz && e - r < (size_t)kBlock) ++e;
+				rb.push_back((uint32_t)r);
+				re.push_back((uint32_t)e);
+				bb.push_back(hoff[r]);
+				tt.push_back(hoff[e]);
+				r = e;
+			}
+			L.color_tile_off[c + 1] = rb.size();
+		}
+		L.ntiles = rb.size();
+		if ((rc = dmalloc(&L.tile_rb, L.ntiles)) || (rc = dmalloc(&L.tile_re, L.ntiles)) ||
+		    (rc = dmalloc(&L.tile_base, L.ntiles)) || (rc = dmalloc(&L.tile_top, L.ntiles)))
+			break;
+		if (L.ntiles &&
+		    (hipMemcpy(L.tile_rb, rb.data(), 4 * L.ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+		     hipMemcpy(L.tile_re, re.data(), 4 * L.ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+		     hipMemcpy(L.tile_base, bb.data(), 8 * L.ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+		     hipMemcpy(L.tile_top, tt.data(), 8 * L.ntiles, hipMemcpyHostToDevice) != hipSuccess)) {
+			rc = ALP_PANIC;
+			break;
+		}
+	} while (0);
+	hipStreamSynchronize(s);
+	if (tmp) hipFree(tmp);
+	if (key) hipFree(key);
+	if (iota) hipFree(iota);
+	if (skey) hipFree(skey);
+	if (len) hipFree(len);
+	alp_hip_matrix_destroy(A);
+	return rc;
+}
+
+static Tiles all_tiles(const HpLevel &L) {
+	return Tiles{L.tile_rb, L.tile_re, L.tile_base, L.tile_top, 0, L.ntiles};
+}
+
+static Tiles colour_tiles(const HpLevel &L, int c) {
+	return Tiles{L.tile_rb, L.tile_re, L.tile_base, L.tile_top, L.color_tile_off[c],
+	             L.color_tile_off[c + 1] - L.color_tile_off[c]};
+}
+
+// ---- V-cycle enqueue ------------------------------------------------------------------
+
+template <typename T>
+static void colour_step(HpLevel &L, int c, hipStream_t s, const int *done) {
+	const Tiles tl = colour_tiles(L, c);
+	if (tl.count == 0) return;
+	T *z = Vec<T>::z(L);
+	if (L.proper) {
+		EpSgs<T> ep{Vec<T>::r(L), Vec<T>::diag(L), z, T(0), T(0), T(0)};
+		hipLaunchKernelGGL((k_tiles<T, EpSgs<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
+		                   (const uint64_t *)L.off, (const uint32_t *)L.idx, Vec<T>::val(L), (const T *)z, ep, done);
+	} else {
+		EpStore<T> ep{Vec<T>::t(L)};
+		hipLaunchKernelGGL((k_tiles<T, EpStore<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
+		                   (const uint64_t *)L.off, (const uint32_t *)L.idx, Vec<T>::val(L), (const T *)z, ep, done);
+		const size_t r0 = L.color_off[c], r1 = L.color_off[c + 1];
+		hipLaunchKernelGGL(k_sgs_update<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, s, r0, r1,
+		                   (const T *)Vec<T>::r(L), (const T *)Vec<T>::t(L), Vec<T>::diag(L), z, done);
+	}
+}
+
+template <typename T>
+static void rbgs(HpLevel &L, hipStream_t s, const int *done) {
+	for (int c = 0; c < L.ncolors; ++c) colour_step<T>(L, c, s, done);
+	for (int c = L.ncolors - 1; c >= 0; --c) colour_step<T>(L, c, s, done);
+}
+
+// multi_grid (multigrid_v_cycle.hpp:180-246) from level l; z_l must already be 0
+template <typename T>
+static void vcycle(alp_hip_hpcg_s *H, size_t l, hipStream_t s, const int *done) {
+	HpLevel &F = H->L[l];
+	if (l + 1 == H->L.size()) {
+		rbgs<T>(F, s, done);
+		return;
+	}
+	for (size_t i = 0; i < H->presmooth; ++i) rbgs<T>(F, s, done);
+	HpLevel &C = H->L[l + 1];
+	const size_t threads = C.n * 32;
+	hipLaunchKernelGGL(k_restrict<T>, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C.n,
+	                   (const uint32_t *)C.inj, (const uint64_t *)F.off, (const uint32_t *)F.idx,
+	                   Vec<T>::val(F), (const T *)Vec<T>::z(F), (const T *)Vec<T>::r(F), Vec<T>::r(C),
+	                   Vec<T>::z(C), done);
+	vcycle<T>(H, l + 1, s, done);
+	hipLaunchKernelGGL(k_prolong<T>, dim3(grid_for(C.n)), dim3(kBlock), 0, s, C.n, (const uint32_t *)C.inj,
+	                   (const T *)Vec<T>::z(C), Vec<T>::z(F), done);
+	for (size_t i = 0; i < H->postsmooth; ++i) rbgs<T>(F, s, done);
+}
+
+// one hpcg iteration (hpcg.hpp:148-221) on precision T, solution x, state st
+template <typename T>
+static void hpcg_iteration(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bool precond, hipStream_t s) {
+	HpLevel &F = H->L[0];
+	const size_t n = F.n;
+	const int *done = &st->done;
+	T *z = Vec<T>::z(F), *r = Vec<T>::r(F);
+	if (precond) {
+		hipLaunchKernelGGL(k_fill_done<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, T(0), done);
+		vcycle<T>(H, 0, s, done);
+	} else {
+		hipLaunchKernelGGL(k_copy_done<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const T *)r, z, done);
+	}
+	const unsigned rg = reduce_grid(n);
+	hipLaunchKernelGGL(k_hp_rz<T>, dim3(rg), dim3(kBlock), 0, s, n, (const T *)r, (const T *)z,
+	                   ctx().d_partials, ctx().d_tickets, st);
+	hipLaunchKernelGGL(k_hp_p<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, p, (const T *)z, (const HpState *)st);
+	const Tiles tl = all_tiles(F);
+	EpStoreDot<T> ep{Ap, p, T(0), 0.0};
+	size_t g = tl.count < (size_t)fill_blocks(8) ? tl.count : fill_blocks(8);
+	hipLaunchKernelGGL((k_tiles_reduce<T, EpStoreDot<T>, FinAlpha>), dim3(g), dim3(kBlock), 0, s, tl,
+	                   (const uint64_t *)F.off, (const uint32_t *)F.idx, Vec<T>::val(F), (const T *)p, ep,
+	                   ctx().d_partials, ctx().d_tickets, FinAlpha{st}, done);
+	hipLaunchKernelGGL(k_hp_update<T>, dim3(rg), dim3(kBlock), 0, s, n, x, r, (const T *)p, (const T *)Ap,
+	                   ctx().d_partials, ctx().d_tickets, st);
+}
+
+// prelude: r = b - A x, norm0 = ||r|| (hpcg.hpp:124-137)
+template <typename T>
+static void hpcg_prelude(alp_hip_hpcg_s *H, const T *b, const T *x, T *Ap, HpState *st, size_t max_iter,
+                         double tol, hipStream_t s) {
+	HpLevel &F = H->L[0];
+	const Tiles tl = all_tiles(F);
+	EpResid<T> ep{b, Vec<T>::r(F), T(0), 0.0};
+	size_t g = tl.count < (size_t)fill_blocks(8) ? tl.count : fill_blocks(8);
+	(void)Ap;
+	hipLaunchKernelGGL((k_tiles_reduce<T, EpResid<T>, FinInit>), dim3(g), dim3(kBlock), 0, s, tl,
+	                   (const uint64_t *)F.off, (const uint32_t *)F.idx, Vec<T>::val(F), x, ep,
+	                   ctx().d_partials, ctx().d_tickets, FinInit{st, tol, (unsigned long long)max_iter},
+	                   (const int *)nullptr);
+}
+
+// Run hpcg iterations until the state says done. One iteration is captured into a
+// hipGraph and replayed; the state is read back every `check` iterations.
+template <typename T>
+static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bool precond,
+                               HpState *h_st) {
+	hipStream_t s = stream();
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	bool use_graph = false;
+	if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+		hpcg_iteration<T>(H, x, p, Ap, st, precond, s);
+		if (hipStreamEndCapture(s, &graph) == hipSuccess &&
+		    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
+			use_graph = true;
+	}
+	(void)hipGetLastError();
+	const int check = 4;
+	alp_rc_t rc = ALP_SUCCESS;
+	for (;;) {
+		for (int i = 0; i < check; ++i) {
+			if (use_graph) {
+				if (hipGraphLaunch(exec, s) != hipSuccess) {
+					rc = ALP_PANIC;
+					break;
+				}
+			} else {
+				hpcg_iteration<T>(H, x, p, Ap, st, precond, s);
+			}
+		}
+		if (rc) break;
+		if (hipMemcpyAsync(h_st, st, sizeof(HpState), hipMemcpyDeviceToHost, s) != hipSuccess ||
+		    hipStreamSynchronize(s) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if (h_st->done) break;
+	}
+	if (exec) hipGraphExecDestroy(exec);
+	if (graph) hipGraphDestroy(graph);
+	if (hipGetLastError() != hipSuccess) rc = ALP_PANIC;
+	return rc;
+}
+
+}  // namespace alp
+
+// ---------------------------------------------------------------------------
+// exported C ABI
+// ---------------------------------------------------------------------------
+
+ALP_EXPORT alp_rc_t alp_hip_hpcg_destroy(alp_hip_hpcg_t H) {
+	if (!H) return ALP_SUCCESS;
+	hipStreamSynchronize(stream());
+	for (HpLevel &L : H->L) free_level(L);
+	void *ps[] = {H->st, H->st32, H->ir, H->p, H->Ap, H->b, H->x, H->p32, H->Ap32, H->x32, H->never};
+	for (void *p : ps)
+		if (p) hipFree(p);
+	delete H;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_hpcg_create(alp_hip_hpcg_t *out, size_t nx, size_t ny, size_t nz,
+                                        size_t max_levels, size_t colours, size_t presmooth,
+                                        size_t postsmooth, unsigned flags) {
+	ALP_TRY(ensure_init());
+	if (!out) return ALP_ILLEGAL;
+	*out = nullptr;
+	if (nx < 3 || ny < 3 || nz < 3) return ALP_ILLEGAL;
+	if (colours > 255 || colours == 1) return ALP_ILLEGAL;
+	if (colours > nx * ny * nz) return ALP_ILLEGAL;  // build_static_color_masks rejects this too
+	auto *H = new (std::nothrow) alp_hip_hpcg_s();
+	if (!H) return ALP_OUTOFMEM;
+	H->presmooth = presmooth;
+	H->postsmooth = postsmooth;
+	H->have_f32 = (flags & ALP_MAT_WITH_F32) != 0;
+	// level sizes as build_hpcg_system: halve while min side / 2^l >= 4 (coarsening
+	// step 2, minimum physical size 4) and fewer than max_levels coarse levels
+	std::vector<size_t> sx = {nx}, sy = {ny}, sz = {nz};
+	size_t minc = std::min(nx, std::min(ny, nz)) / 2;
+	while (minc >= 4 && sx.size() - 1 < max_levels) {
+		if (sx.back() % 2 || sy.back() % 2 || sz.back() % 2) break;
+		sx.push_back(sx.back() / 2);
+		sy.push_back(sy.back() / 2);
+		sz.push_back(sz.back() / 2);
+		minc /= 2;
+	}
+	H->L.resize(sx.size());
+	alp_rc_t rc = ALP_SUCCESS;
+	for (size_t l = 0; l < sx.size() && rc == ALP_SUCCESS; ++l) {
+		if (sx[l] < 3 || sy[l] < 3 || sz[l] < 3) {
+			rc = ALP_ILLEGAL;
+			break;
+		}
+		rc = build_level(H->L[l], sx[l], sy[l], sz[l], (int)colours, H->have_f32);
+	}
+	for (size_t l = 1; l < H->L.size() && rc == ALP_SUCCESS; ++l) {
+		HpLevel &C = H->L[l], &F = H->L[l - 1];
+		if ((rc = dmalloc(&C.inj, C.n))) break;
+		hipLaunchKernelGGL(k_injection, dim3(grid_for(C.n)), dim3(kBlock), 0, stream(), C.n, C.nx, C.ny, F.nx,
+		                   F.ny, (const uint32_t *)C.perm, (const uint32_t *)F.iperm, C.inj);
+	}
+	const size_t n = H->L.empty() ? 0 : H->L[0].n;
+	if (rc == ALP_SUCCESS) {
+		if ((rc = dmalloc(&H->st, 1)) || (rc = dmalloc(&H->st32, 1)) || (rc = dmalloc(&H->ir, 1)) ||
+		    (rc = dmalloc(&H->p, n)) || (rc = dmalloc(&H->Ap, n)) || (rc = dmalloc(&H->b, n)) ||
+		    (rc = dmalloc(&H->x, n)) || (rc = dmalloc(&H->never, 1))) {
+		} else if (H->have_f32 &&
+		           ((rc = dmalloc(&H->p32, n)) || (rc = dmalloc(&H->Ap32, n)) || (rc = dmalloc(&H->x32, n)))) {
+		} else if (hipMemsetAsync(H->never, 0, sizeof(int), stream()) != hipSuccess ||
+		           hipStreamSynchronize(stream()) != hipSuccess) {
+			rc = ALP_PANIC;
+		}
+	}
+	if (rc) {
+		alp_hip_hpcg_destroy(H);
+		return rc;
+	}
+	*out = H;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_hpcg_info(alp_hip_hpcg_t H, size_t *n, size_t *nnz, size_t *levels,
+                                      int *proper) {
+	if (!H || H->L.empty()) return ALP_ILLEGAL;
+	if (n) *n = H->L[0].n;
+	if (nnz) *nnz = H->L[0].nnz;
+	if (levels) *levels = H->L.size();
+	if (proper) {
+		int all = 1;
+		for (const HpLevel &L : H->L) all &= L.proper ? 1 : 0;
+		*proper = all;
+	}
+	return ALP_SUCCESS;
+}
+
+// One V-cycle z = M r (FP64, original numbering): the preconditioner application
+// of hpcg's multi_grid, exposed for parity tests.
+ALP_EXPORT alp_rc_t alp_hip_hpcg_vcycle(alp_hip_hpcg_t H, const double *d_r, double *d_z) {
+	ALP_TRY(ensure_init());
+	if (!H || !d_r || !d_z) return ALP_ILLEGAL;
+	hipStream_t s = stream();
+	HpLevel &F = H->L[0];
+	hipLaunchKernelGGL(k_gather_perm<double>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, F.n,
+	                   (const uint32_t *)F.perm, d_r, F.r);
+	hipLaunchKernelGGL(k_fill_done<double>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, F.n, F.z, 0.0,
+	                   (const int *)H->never);
+	vcycle<double>(H, 0, s, H->never);
+	hipLaunchKernelGGL(k_scatter_perm<double>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, F.n,
+	                   (const uint32_t *)F.perm, (const double *)F.z, d_z);
+	ALP_LAUNCH_CHECK();
+	ALP_HIP_TRY(hipStreamSynchronize(s));
+	return ALP_SUCCESS;
+}
+
+// hpcg (hpcg.hpp:97-223): solve A x = b from the given x; stop when
+// ||r|| / ||r0|| <= tol or after max_iter iterations. *residual = ||r|| (not
+// squared, as the reference). precision 1: FP32 bulk in FP64 iterative refinement;
+// then *iters counts inner iterations and *outer the refinement steps.
+ALP_EXPORT alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, double *d_x,
+                                       size_t max_iter, double tol, int precision, int with_precond,
+                                       size_t *iters, double *residual, size_t *outer) {
+	ALP_TRY(ensure_init());
+	if (!H || !d_b || !d_x || !iters || !residual) return ALP_ILLEGAL;
+	if (precision != 0 && precision != 1) return ALP_ILLEGAL;
+	if (precision == 1 && !H->have_f32) return ALP_ILLEGAL;
+	hipStream_t s = stream();
+	HpLevel &F = H->L[0];
+	const size_t n = F.n;
+	hipLaunchKernelGGL(k_gather_perm<double>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const uint32_t *)F.perm,
+	                   d_b, H->b);
+	hipLaunchKernelGGL(k_gather_perm<double>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const uint32_t *)F.perm,
+	                   (const double *)d_x, H->x);
+	ALP_LAUNCH_CHECK();
+	HpState h{};
+	size_t total = 0, nouter = 0;
+	double res = 0.0;
+	if (precision == 0) {
+		hpcg_prelude<double>(H, H->b, H->x, H->Ap, H->st, max_iter, tol, s);
+		ALP_TRY(run_iterations<double>(H, H->x, H->p, H->Ap, H->st, with_precond != 0, &h));
+		total = h.iter;
+		res = h.norm;
+	} else {
+		// FP64 refinement: r = b - A x (FP64 tiles), inner FP32 PCG on r/||r||
+		const double inner_tol = 1e-5;
+		IrState hir{};
+		for (int o = 0; o < 200; ++o) {
+			const Tiles tl = all_tiles(F);
+			EpResid<double> ep{H->b, F.r, 0.0, 0.0};
+			size_t g = tl.count < (size_t)fill_blocks(8) ? tl.count : fill_blocks(8);
+			hipLaunchKernelGGL((k_tiles_reduce<double, EpResid<double>, FinIr>), dim3(g), dim3(kBlock), 0, s, tl,
+			                   (const uint64_t *)F.off, (const uint32_t *)F.idx, (const double *)F.val,
+			                   (const double *)H->x, ep, ctx().d_partials, ctx().d_tickets,
+			                   FinIr{H->ir, tol, o == 0 ? 1 : 0}, (const int *)nullptr);
+			ALP_LAUNCH_CHECK();
+			ALP_HIP_TRY(hipMemcpyAsync(&hir, H->ir, sizeof(IrState), hipMemcpyDeviceToHost, s));
+			ALP_HIP_TRY(hipStreamSynchronize(s));
+			res = hir.rnorm;
+			if (hir.done || total >= max_iter) break;
+			++nouter;
+			// inner: b32 = r / ||r||, x32 = 0; its residual starts as b32 (A 0 = 0)
+			hipLaunchKernelGGL(k_ir_down, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const double *)F.r, H->Ap32,
+			                   H->x32, (const IrState *)H->ir);
+			hpcg_prelude<float>(H, H->Ap32, H->x32, H->Ap32, H->st32, max_iter - total, inner_tol, s);
+			ALP_TRY(run_iterations<float>(H, H->x32, H->p32, H->Ap32, H->st32, with_precond != 0, &h));
+			total += h.iter;
+			hipLaunchKernelGGL(k_ir_up, dim3(grid_for(n)), dim3(kBlock), 0, s, n, H->x, (const float *)H->x32,
+			                   (const IrState *)H->ir);
+			ALP_LAUNCH_CHECK();
+		}
+	}
+	hipLaunchKernelGGL(k_scatter_perm<double>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const uint32_t *)F.perm,
+	                   (const double *)H->x, d_x);
+	ALP_LAUNCH_CHECK();
+	ALP_HIP_TRY(hipStreamSynchronize(s));
+	*iters = total;
+	*residual = res;
+	if (outer) *outer = nouter;
+	return ALP_SUCCESS;
+}

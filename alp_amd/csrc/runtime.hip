@@ -149,13 +149,14 @@ __global__ __launch_bounds__(kBlock) void k_row_stats(size_t n, const uint64_t *
 // `tile` nonzeros (rows longer than a tile get a block of their own). Built on
 // the host from the downloaded offsets; O(nrows).
 static alp_rc_t upload_row_blocks(const std::vector<uint64_t> &off, size_t nrows, size_t tile,
-                                  uint32_t **d_rb, uint64_t **d_base, size_t *count) {
+                                  uint32_t **d_rb, uint64_t **d_base, size_t *count,
+                                  size_t max_rows = kBlock) {
 	std::vector<uint32_t> rb;
 	rb.push_back(0);
 	size_t r = 0;
 	while (r < nrows) {
 		size_t e = r + 1;
-		while (e < nrows && off[e + 1] - off[r] <= tile && e - r < kBlock) ++e;
+		while (e < nrows && off[e + 1] - off[r] <= tile && e - r < max_rows) ++e;
 		rb.push_back(static_cast<uint32_t>(e));
 		r = e;
 	}
@@ -180,6 +181,8 @@ static alp_rc_t build_row_blocks(Storage &S) {
 	std::vector<uint64_t> off(S.nmajor + 1);
 	ALP_HIP_TRY(hipMemcpy(off.data(), S.off, sizeof(uint64_t) * (S.nmajor + 1), hipMemcpyDeviceToHost));
 	ALP_TRY(upload_row_blocks(off, S.nmajor, 2048, &S.row_blocks, &S.row_block_base, &S.n_row_blocks));
+	ALP_TRY(upload_row_blocks(off, S.nmajor, 512, &S.row_blocks_w, &S.row_block_base_w,
+	                          &S.n_row_blocks_w, kWave));
 	return upload_row_blocks(off, S.nmajor, 4096, &S.row_blocks_l, &S.row_block_base_l,
 	                         &S.n_row_blocks_l);
 }
@@ -218,6 +221,8 @@ void storage_free(Storage &S) {
 	if (S.row_blocks) hipFree(S.row_blocks);
 	if (S.row_block_base) hipFree(S.row_block_base);
 	if (S.row_blocks_l) hipFree(S.row_blocks_l);
+	if (S.row_blocks_w) hipFree(S.row_blocks_w);
+	if (S.row_block_base_w) hipFree(S.row_block_base_w);
 	if (S.row_block_base_l) hipFree(S.row_block_base_l);
 	S = Storage();
 }

@@ -168,6 +168,91 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
 	}
 }
 
+// Within one wavefront LDS operations execute in issue order; the fences and the
+// wave barrier keep the compiler from moving LDS accesses across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// CSR-wave: like CSR-stream, but each wavefront owns a tile of whole rows (at
+// most 64 rows and 512 nonzeros) and its own LDS slice, so tiles proceed
+// independently with no block-wide barrier. One tile per wave, plain order.
+template <typename R, int MASK, bool OVERWRITE>
+__global__ __launch_bounds__(kBlock) void k_spmv_wave(
+    size_t ntiles, const uint32_t *__restrict__ rb, const uint64_t *__restrict__ rbase,
+    const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx,
+    const void *__restrict__ vval, const typename R::X *__restrict__ x,
+    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
+	using A = typename R::A;
+	using Y = typename R::Y;
+	using VT = typename Val<R>::T;
+	constexpr int J = 8;
+	constexpr int kTile = J * kWave;
+	__shared__ A tiles[kBlock / kWave][kTile];
+	const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+	A *tile = tiles[wave];
+	const auto *val = static_cast<const typename R::V *>(vval);
+	const size_t t = (size_t)blockIdx.x * (kBlock / kWave) + wave;
+	if (t >= ntiles) return;  // the whole wavefront leaves together
+	const uint32_t r0 = rb[t], r1 = rb[t + 1];
+	const uint64_t base = rbase[t], top = rbase[t + 1];
+	if (top - base <= (uint64_t)kTile) {
+		const unsigned nr = r1 - r0;
+		unsigned tpr = 1;
+		while (tpr < kWave && tpr * 2 * nr <= kWave) tpr *= 2;
+		const unsigned r = r0 + lane / tpr, sub = lane % tpr;
+		bool active = r < r1;
+		uint64_t rk = 0, re = 0;
+		A yv = R::zero();
+		if (active) {
+			rk = off[r];
+			re = off[r + 1];
+			if (MASK) active = (MASK == 1) == (mask[r] != 0);
+			if (!OVERWRITE && active && sub == 0) yv = static_cast<A>(y[r]);
+		}
+		uint32_t jj[J];
+		VT vv[J];
+#pragma unroll
+		for (int q = 0; q < J; ++q) {
+			const uint64_t k = base + lane + (uint64_t)q * kWave;
+			jj[q] = 0;
+			vv[q] = VT(0);
+			if (k < top) {
+				jj[q] = __builtin_nontemporal_load(idx + k);
+				vv[q] = Val<R>::load(val, k);
+			}
+		}
+#pragma unroll
+		for (int q = 0; q < J; ++q) {
+			const uint64_t k = base + lane + (uint64_t)q * kWave;
+			if (k < top) tile[k - base] = Val<R>::mul(vv[q], x[jj[q]]);
+		}
+		wave_lds_sync();
+		A acc = R::zero();
+		for (uint64_t q = rk + sub; q < re; q += tpr) acc = R::add(acc, tile[q - base]);
+		for (unsigned s = tpr / 2; s > 0; s >>= 1) acc = R::add(acc, shfl_xor(acc, s));
+		if (active && sub == 0) {
+			if (rk == re) {
+				if (OVERWRITE) y[r] = static_cast<Y>(R::zero());
+			} else {
+				y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(yv, acc));
+			}
+		}
+	} else {
+		// one row longer than a tile: the wavefront strides and folds it
+		A acc = R::zero();
+		for (uint64_t k = base + lane; k < top; k += kWave)
+			acc = R::add(acc, Val<R>::mul(Val<R>::load(val, k), x[__builtin_nontemporal_load(idx + k)]));
+		acc = group_fold<kWave, R>(acc);
+		bool active = true;
+		if (MASK) active = (MASK == 1) == (mask[r0] != 0);
+		if (lane == 0 && active)
+			y[r0] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(static_cast<A>(y[r0]), acc));
+	}
+}
+
 // ---- launch ------------------------------------------------------------------
 
 template <typename R, int MASK, bool OW, int RR>
@@ -234,6 +319,18 @@ static void launch_stream(int variant, int tile, const Storage &S, const void *v
 #undef ALP_STREAM
 }
 
+template <typename R, int MASK, bool OW>
+static void launch_wave(const Storage &S, const void *val, const void *x, void *y,
+                        const uint8_t *mask, hipStream_t s) {
+	const size_t per = kBlock / kWave;
+	const size_t grid = (S.n_row_blocks_w + per - 1) / per;
+	if (grid == 0) return;
+	hipLaunchKernelGGL((k_spmv_wave<R, MASK, OW>), dim3(grid), dim3(kBlock), 0, s, S.n_row_blocks_w,
+	                   (const uint32_t *)S.row_blocks_w, (const uint64_t *)S.row_block_base_w,
+	                   (const uint64_t *)S.off, (const uint32_t *)S.idx, val,
+	                   (const typename R::X *)x, (typename R::Y *)y, mask);
+}
+
 int default_lanes(const Storage &S) {
 	const double a = S.avg_len;
 	if (a <= 3) return 2;
@@ -253,8 +350,10 @@ static void dispatch(const Storage &S, const void *val, unsigned descr, void *y,
 	int RR = S.rows ? S.rows : 1;  // CSR-stream: the launch variant
 	if (kernel == 0) kernel = (S.avg_len < 4 && S.max_len > 64) ? 2 : 1;
 #define ALP_GO(MM, OWV)                                                 \
-	if (kernel == 2)                                                    \
-		launch_stream<R, MM, OWV>(RR, S.lanes, S, val, x, y, mask, s);               \
+	if (kernel == 3)                                                    \
+		launch_wave<R, MM, OWV>(S, val, x, y, mask, s);                 \
+	else if (kernel == 2)                                               \
+		launch_stream<R, MM, OWV>(RR, S.lanes, S, val, x, y, mask, s);  \
 	else                                                                \
 		launch_vector<R, MM, OWV>(G, RR, S, val, x, y, mask, s);
 	if (ow) {
@@ -338,7 +437,7 @@ ALP_EXPORT alp_rc_t alp_hip_vxm(alp_hip_matrix_t A, alp_semiring_t ring, unsigne
 }
 
 ALP_EXPORT alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes, int rows) {
-	if (!A || kernel < 0 || kernel > 2) return ALP_ILLEGAL;
+	if (!A || kernel < 0 || kernel > 3) return ALP_ILLEGAL;
 	if (lanes != 0 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 &&
 	    lanes != 64)
 		return ALP_ILLEGAL;
@@ -385,6 +484,7 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 		for (int r : {1, 2, 4}) cands.push_back({1, g, r});
 	for (int t : {8, 16})
 		for (int v : {1, 2, 4, 8}) cands.push_back({2, t, v});
+	cands.push_back({3, 0, 0});
 	float best = 1e30f;
 	Cand pick = {1, default_lanes(A->crs), 1};
 	for (const Cand &c : cands) {

@@ -154,6 +154,82 @@ __device__ __forceinline__ void rows_gather(const uint64_t (&beg)[RR], const uin
 	for (int i = 0; i < RR; ++i) acc[i] = group_fold<G, R>(acc[i]);
 }
 
+// ---- block tile fold (the CSR-stream pattern with a caller epilogue) ----------
+// Rows [r0, r1) of one tile whose nonzeros [base, top) number at most J*kBlock.
+// Every thread issues its J index/value loads, then the x gathers, products go
+// to LDS, and tpr threads per row fold the row's slice. Before the loads,
+// ep.prefetch(row) runs once per row on its folding thread (to issue the row's
+// own loads early); after the fold ep.finish(row, acc, empty) runs on that
+// thread. Two __syncthreads: callers must reach this uniformly per block.
+template <typename R, int J, typename Ep>
+__device__ __forceinline__ void tile_fold(uint32_t r0, uint32_t r1, uint64_t base, uint64_t top,
+                                          const uint64_t *__restrict__ off,
+                                          const uint32_t *__restrict__ idx,
+                                          const typename R::V *__restrict__ val,
+                                          const typename R::X *__restrict__ x,
+                                          typename R::A *tile, Ep &ep) {
+	using A = typename R::A;
+	using VT = typename Val<R>::T;
+	const unsigned nr = r1 - r0;
+	unsigned tpr = 1;
+	while (tpr < 64 && tpr * 2 * nr <= kBlock) tpr *= 2;
+	const unsigned r = r0 + threadIdx.x / tpr, sub = threadIdx.x % tpr;
+	const bool own = r < r1;
+	uint64_t rk = 0, re = 0;
+	if (own) {
+		rk = off[r];
+		re = off[r + 1];
+		if (sub == 0) ep.prefetch(r);
+	}
+	uint32_t jj[J];
+	VT vv[J];
+#pragma unroll
+	for (int q = 0; q < J; ++q) {
+		const uint64_t k = base + threadIdx.x + (uint64_t)q * kBlock;
+		jj[q] = 0;
+		vv[q] = VT(0);
+		if (k < top) {
+			jj[q] = __builtin_nontemporal_load(idx + k);
+			vv[q] = Val<R>::load(val, k);
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < J; ++q) {
+		const uint64_t k = base + threadIdx.x + (uint64_t)q * kBlock;
+		if (k < top) tile[k - base] = Val<R>::mul(vv[q], x[jj[q]]);
+	}
+	__syncthreads();
+	A acc = R::zero();
+	for (uint64_t q = rk + sub; q < re; q += tpr) acc = R::add(acc, tile[q - base]);
+	for (unsigned s = tpr / 2; s > 0; s >>= 1) acc = R::add(acc, shfl_xor(acc, s));
+	if (own && sub == 0) ep.finish(r, acc, rk == re);
+	__syncthreads();
+}
+
+// one row longer than a tile, folded by the whole block; ep as above (thread 0)
+template <typename R, typename Ep>
+__device__ __forceinline__ void long_row_fold(uint32_t r0, uint64_t base, uint64_t top,
+                                              const uint32_t *__restrict__ idx,
+                                              const typename R::V *__restrict__ val,
+                                              const typename R::X *__restrict__ x,
+                                              typename R::A *red, Ep &ep) {
+	using A = typename R::A;
+	if (threadIdx.x == 0) ep.prefetch(r0);
+	A acc = R::zero();
+	for (uint64_t k = base + threadIdx.x; k < top; k += kBlock)
+		acc = R::add(acc, Val<R>::mul(Val<R>::load(val, k), x[__builtin_nontemporal_load(idx + k)]));
+	acc = group_fold<kWave, R>(acc);
+	__syncthreads();
+	if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = acc;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		A t = red[0];
+		for (int i = 1; i < kBlock / kWave; ++i) t = R::add(t, red[i]);
+		ep.finish(r0, t, false);
+	}
+	__syncthreads();
+}
+
 // ---- persistent-grid work split -----------------------------------------------
 // `nitems` units (row chunks) are split into one contiguous region per XCD
 // (sized by that XCD's share of the grid); inside a region the XCD's blocks take

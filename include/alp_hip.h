@@ -119,7 +119,9 @@ alp_rc_t alp_hip_matrix_download(alp_hip_matrix_t A, int which, uint64_t *h_off,
                                  uint32_t *h_idx, void *h_val);
 /* select the SpMV kernel: 0 = auto, 1 = CSR-vector (`lanes` per row: 2..64, `rows`
  * per lane group: 1, 2 or 4; 0 = default), 2 = CSR-stream (`rows` picks the launch:
- * 1 = persistent grid 4 blocks/CU, 2 = persistent 8 blocks/CU, 4 = a block per tile) */
+ * 1 = persistent grid 4 blocks/CU, 2 = persistent 8 blocks/CU, 4 = a block per tile,
+ * 8 = a block per tile in plain block order; `lanes` 16 = 4096-nonzero tiles),
+ * 3 = CSR-wave (a 512-nonzero tile per wavefront) */
 alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes, int rows);
 alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes, int *rows);
 /* time each candidate SpMV kernel on this matrix and keep the fastest */
@@ -165,6 +167,32 @@ alp_rc_t alp_hip_convert_f32_to_f64(size_t n, const float *d_x, double *d_y);
  * refinement (residual b - A x and the solution update stay FP64). */
 alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *d_b, double *d_x, size_t max_iter,
                     double tol, int precision, size_t *iters, double *residual);
+
+/* ---- HPCG (algorithms/hpcg/hpcg.hpp:97-223, multigrid_v_cycle.hpp, red_black_gauss_seidel.hpp,
+ *      system_building_utils.hpp:84-150) ----
+ * A handle owns the whole multigrid hierarchy of the nx*ny*nz 27-point problem
+ * (diagonal 26, off-diagonals -1), built on the device as build_hpcg_system does:
+ * coarsening step 2, minimum side 4, at most max_levels coarse levels, injection
+ * coarseners. colours = k > 1 colours rows by index mod k (build_static_color_masks,
+ * matrix_building_utils.hpp:139-165); colours = 0 uses the 8-colour parity colouring
+ * (x, y, z mod 2). presmooth/postsmooth: symmetric Gauss-Seidel sweeps per level.
+ * flags: ALP_MAT_WITH_F32 also keeps FP32 copies for precision = 1. */
+typedef struct alp_hip_hpcg_s *alp_hip_hpcg_t;
+alp_rc_t alp_hip_hpcg_create(alp_hip_hpcg_t *H, size_t nx, size_t ny, size_t nz, size_t max_levels,
+                             size_t colours, size_t presmooth, size_t postsmooth, unsigned flags);
+alp_rc_t alp_hip_hpcg_destroy(alp_hip_hpcg_t H);
+/* n, nnz of the finest level, number of grids (finest included), and whether every
+ * level's colouring is proper (then colour steps run fused) */
+alp_rc_t alp_hip_hpcg_info(alp_hip_hpcg_t H, size_t *n, size_t *nnz, size_t *levels, int *proper);
+/* z = M r: one V-cycle of the preconditioner (FP64, original row numbering) */
+alp_rc_t alp_hip_hpcg_vcycle(alp_hip_hpcg_t H, const double *d_r, double *d_z);
+/* Solve A x = b from the given x: stop when ||r||/||r0|| <= tol or after max_iter
+ * iterations; *residual = ||r||; precision 0 = FP64, 1 = FP32 SpMV/smoother/transfers
+ * inside FP64 iterative refinement (*iters = inner iterations, *outer = refinement
+ * steps; outer may be NULL). with_precond = 0 runs hpcg's unpreconditioned branch. */
+alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, double *d_x, size_t max_iter,
+                            double tol, int precision, int with_precond, size_t *iters,
+                            double *residual, size_t *outer);
 
 #ifdef __cplusplus
 }

@@ -104,7 +104,7 @@ def closed_form_A1(nx, ny, nz):
 
 
 KERNELS = [(1, 2, 1), (1, 4, 2), (1, 8, 4), (1, 16, 1), (1, 16, 4), (1, 32, 2), (1, 64, 1), (2, 0, 1),
-           (2, 0, 2), (2, 0, 4), (2, 8, 8), (2, 16, 1), (2, 16, 4), (2, 16, 8)]
+           (2, 0, 2), (2, 0, 4), (2, 8, 8), (2, 16, 1), (2, 16, 4), (2, 16, 8), (3, 0, 0)]
 
 
 @pytest.mark.parametrize("kernel,lanes,rows", KERNELS)
