@@ -120,6 +120,21 @@ __device__ __forceinline__ bool last_block(unsigned *ticket, bool *lds_flag) {
 	return *lds_flag;
 }
 
+// Grid-stride loop over [0, n) with U independent elements per thread per step,
+// so each thread has U loads in flight (a plain grid-stride loop keeps one or two
+// and leaves streaming kernels latency-bound). The visiting order per thread is
+// fixed, so reductions built on it stay deterministic.
+template <int U, typename F>
+__device__ __forceinline__ void grid_for_each(size_t n, F &&f) {
+	const size_t stride = (size_t)gridDim.x * kBlock;
+	size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+	for (; i + (U - 1) * stride < n; i += U * stride) {
+#pragma unroll
+		for (int u = 0; u < U; ++u) f(i + u * stride);
+	}
+	for (; i < n; i += stride) f(i);
+}
+
 // fixed-order sum of n partials by one block (n <= kMaxPartials)
 template <typename T>
 __device__ __forceinline__ T block_sum_partials(const T *p, unsigned n, T *lds) {

@@ -110,8 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dot_alpha(CgArgs<T> a) {
 	__shared__ bool is_last;
 	if (a.st->done) return;
 	double part = 0.0;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (size_t)gridDim.x * kBlock)
-		part += (double)a.temp[i] * (double)a.u[i];
+	grid_for_each<4>(a.n, [&](size_t i) { part += (double)a.temp[i] * (double)a.u[i]; });
 	part = block_sum(part, lds);
 	if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
 	if (last_block(a.ticket, &is_last)) {
@@ -127,12 +126,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a) {
 	if (a.st->done) return;
 	const T alpha = (T)a.st->alpha;
 	double part = 0.0;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(a.n, [&](size_t i) {
 		a.x[i] = alpha * a.u[i] + a.x[i];
 		const T ri = a.r[i] - alpha * a.temp[i];
 		a.r[i] = ri;
 		part += (double)ri * (double)ri;
-	}
+	});
 	part = block_sum(part, lds);
 	if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
 	if (last_block(a.ticket, &is_last)) {
@@ -157,8 +156,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs<T> a) {
 	if (a.st->done) return;
 	const T ratio = (T)a.st->ratio;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (size_t)gridDim.x * kBlock)
-		a.u[i] = ratio * a.u[i] + a.r[i];
+	grid_for_each<4>(a.n, [&](size_t i) { a.u[i] = ratio * a.u[i] + a.r[i]; });
 }
 
 // prelude: r = b - temp (temp = A x), u = r, sigma = r.r, bnorm2 = bb.bb where bb
@@ -169,14 +167,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs<T> a, const double *b
 	__shared__ double lds[kBlock / kWave];
 	__shared__ bool is_last;
 	double prr = 0.0, pbb = 0.0;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(a.n, [&](size_t i) {
 		const T ri = a.b[i] - a.temp[i];
 		a.r[i] = ri;
 		a.u[i] = ri;
 		prr += (double)ri * (double)ri;
 		const double bi = bb_f64 ? bb_f64[i] : (double)bb_t[i];
 		pbb += bi * bi;
-	}
+	});
 	prr = block_sum(prr, lds);
 	pbb = block_sum(pbb, lds);
 	if (threadIdx.x == 0) {
@@ -220,12 +218,12 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(size_t n, const double *
 	__shared__ double lds[kBlock / kWave];
 	__shared__ bool is_last;
 	double p = 0.0, pb = 0.0;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(n, [&](size_t i) {
 		const double ri = b[i] - t[i];
 		r[i] = ri;
 		p += ri * ri;
 		if (first) pb += b[i] * b[i];
-	}
+	});
 	p = block_sum(p, lds);
 	pb = block_sum(pb, lds);
 	if (threadIdx.x == 0) {
@@ -247,17 +245,16 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(size_t n, const double *
 __global__ __launch_bounds__(kBlock) void k_ir_scale_down(size_t n, const double *r, float *r32,
                                                          float *x32, const IrState *ir) {
 	const double s = ir->inv_rnorm;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(n, [&](size_t i) {
 		r32[i] = (float)(r[i] * s);
 		x32[i] = 0.0f;
-	}
+	});
 }
 
 __global__ __launch_bounds__(kBlock) void k_ir_correct(size_t n, double *x, const float *d32,
                                                       const IrState *ir) {
 	const double s = ir->rnorm;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		x[i] = s * (double)d32[i] + x[i];
+	grid_for_each<4>(n, [&](size_t i) { x[i] = s * (double)d32[i] + x[i]; });
 }
 
 // ---- host driver ----------------------------------------------------------------

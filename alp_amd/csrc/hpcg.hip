@@ -30,12 +30,23 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace alp {
 
-constexpr int kJ = 8;                  // tile loads per thread
-constexpr int kTileNnz = kJ * kBlock;  // 2048 nonzeros per tile
+// Tile loads per thread: 8 for FP64 (2048-nonzero tiles), 16 for FP32 (4096),
+// so both precisions keep the same bytes in flight per block.
+template <typename T> struct TileJ { static constexpr int J = sizeof(T) == 8 ? 8 : 16; };
+
+struct TileSet {
+	std::vector<size_t> color_tile_off;  // tiles of colour c: [color_tile_off[c], color_tile_off[c+1])
+	uint32_t *rb = nullptr;              // tile t covers rows [rb[t], re[t])
+	uint32_t *re = nullptr;
+	uint64_t *base = nullptr;            // off[rb[t]]
+	uint64_t *top = nullptr;             // off[re[t]]
+	size_t count = 0;
+};
 
 struct HpLevel {
 	size_t nx = 0, ny = 0, nz = 0, n = 0, nnz = 0;
@@ -51,12 +62,7 @@ struct HpLevel {
 	int ncolors = 0;
 	bool proper = false;
 	std::vector<size_t> color_off;       // rows of colour c: [color_off[c], color_off[c+1])
-	std::vector<size_t> color_tile_off;  // its tiles: [color_tile_off[c], color_tile_off[c+1])
-	uint32_t *tile_rb = nullptr;         // tile t covers rows [tile_rb[t], tile_rb[t+1]) ...
-	uint32_t *tile_re = nullptr;         // ... ending at tile_re[t] (tiles of a colour are contiguous)
-	uint64_t *tile_base = nullptr;       // off[tile_rb[t]]
-	uint64_t *tile_top = nullptr;        // off[tile_re[t]]
-	size_t ntiles = 0;
+	TileSet ts[2];                       // tiles for FP64 ([0], 2048 nonzeros) and FP32 ([1], 4096)
 	uint32_t *inj = nullptr;             // coarse level: coarse new row -> fine new row
 	double *z = nullptr, *r = nullptr, *t = nullptr;
 	float *z32 = nullptr, *r32 = nullptr, *t32 = nullptr;
@@ -146,6 +152,7 @@ struct Tiles {
 	const uint32_t *rb, *re;
 	const uint64_t *base, *top;
 	size_t first, count;
+	int xcd;  // 1: each XCD walks one contiguous range of tiles (neighbour reuse in its L2)
 };
 
 // One block per tile; the epilogue decides what the fold feeds.
@@ -155,10 +162,12 @@ __global__ __launch_bounds__(kBlock) void k_tiles(Tiles tl, const uint64_t *__re
                                                   const T *__restrict__ val,
                                                   const T *__restrict__ x, Ep ep,
                                                   const int *done) {
+	constexpr int kJ = TileJ<T>::J, kTileNnz = kJ * kBlock;
 	__shared__ T tile[kTileNnz];
 	__shared__ T red[kBlock / kWave];
 	if (done && *done) return;
-	for (size_t b = blockIdx.x; b < tl.count; b += gridDim.x) {
+	const WorkSplit w = tl.xcd ? xcd_split(tl.count) : WorkSplit{blockIdx.x, tl.count, gridDim.x};
+	for (size_t b = w.first; b < w.last; b += w.step) {
 		const size_t t = tl.first + b;
 		const uint32_t r0 = tl.rb[t], r1 = tl.re[t];
 		const uint64_t base = tl.base[t], top = tl.top[t];
@@ -178,13 +187,15 @@ __global__ __launch_bounds__(kBlock) void k_tiles_reduce(Tiles tl, const uint64_
                                                          const T *__restrict__ x, Ep ep,
                                                          double *partials, unsigned *ticket,
                                                          Fin fin, const int *done) {
+	constexpr int kJ = TileJ<T>::J, kTileNnz = kJ * kBlock;
 	__shared__ T tile[kTileNnz];
 	__shared__ T red[kBlock / kWave];
 	__shared__ double lds[kBlock / kWave];
 	__shared__ bool is_last;
 	if (done && *done) return;
 	ep.part = 0.0;
-	for (size_t b = blockIdx.x; b < tl.count; b += gridDim.x) {
+	const WorkSplit w = tl.xcd ? xcd_split(tl.count) : WorkSplit{blockIdx.x, tl.count, gridDim.x};
+	for (size_t b = w.first; b < w.last; b += w.step) {
 		const size_t t = tl.first + b;
 		const uint32_t r0 = tl.rb[t], r1 = tl.re[t];
 		const uint64_t base = tl.base[t], top = tl.top[t];
@@ -210,6 +221,15 @@ __global__ __launch_bounds__(kBlock) void k_sgs_update(size_t r0, size_t r1, con
 		const T di = d[i];
 		z[i] = ((r[i] - t[i]) + z[i] * di) / di;
 	}
+}
+
+// the first colour step after z = 0: (A z) = 0 on every row, so the step is
+// z = ((r - 0) + 0 d) / d = r / d and needs no matrix read
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sgs_first(size_t r0, size_t r1, const T *r, const T *d, T *z,
+                                                      const int *done) {
+	if (done && *done) return;
+	grid_for_each<4>(r1 - r0, [&](size_t i) { z[r0 + i] = r[r0 + i] / d[r0 + i]; });
 }
 
 // restriction: r_c[i] = r_f[f] - (A_f z_f)[f] with f = inj[i], and z_c[i] = 0
@@ -241,22 +261,19 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong(size_t nc, const uint32_t *inj, const T *zc,
                                                     T *zf, const int *done) {
 	if (done && *done) return;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < nc; i += (size_t)gridDim.x * kBlock)
-		zf[inj[i]] += zc[i];
+	grid_for_each<4>(nc, [&](size_t i) { zf[inj[i]] += zc[i]; });
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill_done(size_t n, T *x, T v, const int *done) {
 	if (done && *done) return;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		x[i] = v;
+	grid_for_each<4>(n, [&](size_t i) { x[i] = v; });
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_copy_done(size_t n, const T *s, T *d, const int *done) {
 	if (done && *done) return;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		d[i] = s[i];
+	grid_for_each<4>(n, [&](size_t i) { d[i] = s[i]; });
 }
 
 // ---- CG part of hpcg (hpcg.hpp:148-221) -------------------------------------------
@@ -269,8 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_hp_rz(size_t n, const T *r, const T 
 	__shared__ bool is_last;
 	if (st->done) return;
 	double p = 0.0;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		p += (double)r[i] * (double)z[i];
+	grid_for_each<4>(n, [&](size_t i) { p += (double)r[i] * (double)z[i]; });
 	p = block_sum(p, lds);
 	if (threadIdx.x == 0) partials[blockIdx.x] = p;
 	if (last_block(ticket, &is_last)) {
@@ -289,8 +305,24 @@ __global__ __launch_bounds__(kBlock) void k_hp_p(size_t n, T *p, const T *z, con
 	if (st->done) return;
 	const bool first = st->iter == 0;
 	const T beta = (T)st->beta;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		p[i] = first ? z[i] : beta * p[i] + z[i];
+	grid_for_each<4>(n, [&](size_t i) { p[i] = first ? z[i] : beta * p[i] + z[i]; });
+}
+
+// alpha = rz / (p.Ap)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hp_alpha(size_t n, const T *p, const T *Ap, double *partials,
+                                                     unsigned *ticket, HpState *st) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	if (st->done) return;
+	double q = 0.0;
+	grid_for_each<4>(n, [&](size_t i) { q += (double)p[i] * (double)Ap[i]; });
+	q = block_sum(q, lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = q;
+	if (last_block(ticket, &is_last)) {
+		const double pAp = block_sum_partials(partials, gridDim.x, lds);
+		if (threadIdx.x == 0) st->alpha = st->rz / pAp;
+	}
 }
 
 struct FinAlpha {
@@ -307,12 +339,12 @@ __global__ __launch_bounds__(kBlock) void k_hp_update(size_t n, T *x, T *r, cons
 	if (st->done) return;
 	const T alpha = (T)st->alpha;
 	double q = 0.0;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(n, [&](size_t i) {
 		x[i] = alpha * p[i] + x[i];
 		const T ri = (-alpha) * Ap[i] + r[i];
 		r[i] = ri;
 		q += (double)ri * (double)ri;
-	}
+	});
 	q = block_sum(q, lds);
 	if (threadIdx.x == 0) partials[blockIdx.x] = q;
 	if (last_block(ticket, &is_last)) {
@@ -357,30 +389,27 @@ struct FinIr {  // FP64 outer residual of the refinement loop
 __global__ __launch_bounds__(kBlock) void k_ir_down(size_t n, const double *r, float *r32, float *x32,
                                                    const IrState *ir) {
 	const double s = ir->inv;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(n, [&](size_t i) {
 		r32[i] = (float)(r[i] * s);
 		x32[i] = 0.0f;
-	}
+	});
 }
 
 __global__ __launch_bounds__(kBlock) void k_ir_up(size_t n, double *x, const float *d32, const IrState *ir) {
 	const double s = ir->rnorm;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		x[i] = s * (double)d32[i] + x[i];
+	grid_for_each<4>(n, [&](size_t i) { x[i] = s * (double)d32[i] + x[i]; });
 }
 
 // ---- permutations ------------------------------------------------------------------
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gather_perm(size_t n, const uint32_t *perm, const T *src, T *dst) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		dst[i] = src[perm[i]];
+	grid_for_each<4>(n, [&](size_t i) { dst[i] = src[perm[i]]; });
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scatter_perm(size_t n, const uint32_t *perm, const T *src, T *dst) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		dst[perm[i]] = src[i];
+	grid_for_each<4>(n, [&](size_t i) { dst[perm[i]] = src[i]; });
 }
 
 // ---- hierarchy build ------------------------------------------------------------------
@@ -388,7 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_perm(size_t n, const uint32_
 __global__ __launch_bounds__(kBlock) void k_colour_keys(size_t nx, size_t ny, size_t nz, int colours,
                                                        uint32_t *key, uint32_t *iota) {
 	const size_t n = nx * ny * nz;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(n, [&](size_t i) {
 		uint32_t c;
 		if (colours == 0) {
 			const size_t x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
@@ -398,20 +427,19 @@ __global__ __launch_bounds__(kBlock) void k_colour_keys(size_t nx, size_t ny, si
 		}
 		key[i] = c;
 		iota[i] = (uint32_t)i;
-	}
+	});
 }
 
 __global__ __launch_bounds__(kBlock) void k_inverse_perm(size_t n, const uint32_t *perm, uint32_t *iperm) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		iperm[perm[i]] = (uint32_t)i;
+	grid_for_each<4>(n, [&](size_t i) { iperm[perm[i]] = (uint32_t)i; });
 }
 
 __global__ __launch_bounds__(kBlock) void k_perm_lengths(size_t n, const uint32_t *perm,
                                                         const uint64_t *off, uint64_t *len) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(n, [&](size_t i) {
 		const uint32_t o = perm[i];
 		len[i] = off[o + 1] - off[o];
-	}
+	});
 }
 
 // row i' of the permuted matrix = old row perm[i'], columns renumbered by iperm,
@@ -421,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_perm_fill(size_t n, const uint32_t *
                                                      const uint32_t *idx, const double *val,
                                                      const uint64_t *noff, uint32_t *nidx,
                                                      double *nval, double *diag) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(n, [&](size_t i) {
 		const uint32_t o = perm[i];
 		uint64_t w = noff[i];
 		double d = 0.0;
@@ -432,18 +460,17 @@ __global__ __launch_bounds__(kBlock) void k_perm_fill(size_t n, const uint32_t *
 			if (c == o) d = val[k];
 		}
 		diag[i] = d;
-	}
+	});
 }
 
 __global__ __launch_bounds__(kBlock) void k_colour_of_new(size_t n, const uint32_t *sorted_key, uint8_t *col) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		col[i] = (uint8_t)sorted_key[i];
+	grid_for_each<4>(n, [&](size_t i) { col[i] = (uint8_t)sorted_key[i]; });
 }
 
 // flags[0] = 1 if some row has an off-diagonal neighbour of its own colour
 __global__ __launch_bounds__(kBlock) void k_check_proper(size_t n, const uint64_t *off, const uint32_t *idx,
                                                         const uint8_t *col, int *flag) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(n, [&](size_t i) {
 		const uint8_t c = col[i];
 		for (uint64_t k = off[i]; k < off[i + 1]; ++k) {
 			const uint32_t j = idx[k];
@@ -452,19 +479,19 @@ __global__ __launch_bounds__(kBlock) void k_check_proper(size_t n, const uint64_
 				break;
 			}
 		}
-	}
+	});
 }
 
 // coarse new row i -> fine new row at (2x, 2y, 2z) of its coarse coordinates
 __global__ __launch_bounds__(kBlock) void k_injection(size_t nc, size_t nxc, size_t nyc, size_t nxf,
                                                      size_t nyf, const uint32_t *perm_c,
                                                      const uint32_t *iperm_f, uint32_t *inj) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < nc; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(nc, [&](size_t i) {
 		const size_t o = perm_c[i];
 		const size_t x = o % nxc, y = (o / nxc) % nyc, z = o / (nxc * nyc);
 		const size_t f = 2 * x + nxf * (2 * y + nyf * 2 * z);
 		inj[i] = iperm_f[f];
-	}
+	});
 }
 
 static unsigned grid_for(size_t n) {
@@ -503,7 +530,8 @@ namespace alp {
 
 static void free_level(HpLevel &L) {
 	void *ps[] = {L.off, L.idx, L.val, L.val32, L.diag, L.diag32, L.perm, L.iperm, L.colour,
-	              L.tile_rb, L.tile_re, L.tile_base, L.tile_top, L.inj, L.z, L.r, L.t,
+	              L.ts[0].rb, L.ts[0].re, L.ts[0].base, L.ts[0].top, L.ts[1].rb, L.ts[1].re,
+	              L.ts[1].base, L.ts[1].top, L.inj, L.z, L.r, L.t,
 	              L.z32, L.r32, L.t32};
 	for (void *p : ps)
 		if (p) hipFree(p);
@@ -604,34 +632,38 @@ static alp_rc_t build_level(HpLevel &L, size_t nx, size_t ny, size_t nz, int col
 		L.color_off.assign(L.ncolors + 1, 0);
 		for (size_t i = 0; i < n; ++i) L.color_off[hkey[i] + 1]++;
 		for (int c = 0; c < L.ncolors; ++c) L.color_off[c + 1] += L.color_off[c];
-		std::vector<uint32_t> rb, re;
-		std::vector<uint64_t> bb, tt;
-		L.color_tile_off.assign(L.ncolors + 1, 0);
-		for (int c = 0; c < L.ncolors; ++c) {
-			size_t r = L.color_off[c];
-			const size_t rend = L.color_off[c + 1];
-			while (r < rend) {
-				size_t e = r + 1;
-				while (e < rend && hoff[e + 1] - hoff[r] <= (uint64_t)kTileNnz && e - r < (size_t)kBlock) ++e;
-				rb.push_back((uint32_t)r);
-				re.push_back((uint32_t)e);
-				bb.push_back(hoff[r]);
-				tt.push_back(hoff[e]);
-				r = e;
+		for (int which = 0; which < 2 && rc == ALP_SUCCESS; ++which) {
+			TileSet &S = L.ts[which];
+			const uint64_t tile = which == 0 ? 2048 : 4096;
+			std::vector<uint32_t> rb, re;
+			std::vector<uint64_t> bb, tt;
+			S.color_tile_off.assign(L.ncolors + 1, 0);
+			for (int c = 0; c < L.ncolors; ++c) {
+				size_t r = L.color_off[c];
+				const size_t rend = L.color_off[c + 1];
+				while (r < rend) {
+					size_t e = r + 1;
+					while (e < rend && hoff[e + 1] - hoff[r] <= tile && e - r < (size_t)kBlock) ++e;
+					rb.push_back((uint32_t)r);
+					re.push_back((uint32_t)e);
+					bb.push_back(hoff[r]);
+					tt.push_back(hoff[e]);
+					r = e;
+				}
+				S.color_tile_off[c + 1] = rb.size();
 			}
-			L.color_tile_off[c + 1] = rb.size();
-		}
-		L.ntiles = rb.size();
-		if ((rc = dmalloc(&L.tile_rb, L.ntiles)) || (rc = dmalloc(&L.tile_re, L.ntiles)) ||
-		    (rc = dmalloc(&L.tile_base, L.ntiles)) || (rc = dmalloc(&L.tile_top, L.ntiles)))
-			break;
-		if (L.ntiles &&
-		    (hipMemcpy(L.tile_rb, rb.data(), 4 * L.ntiles, hipMemcpyHostToDevice) != hipSuccess ||
-		     hipMemcpy(L.tile_re, re.data(), 4 * L.ntiles, hipMemcpyHostToDevice) != hipSuccess ||
-		     hipMemcpy(L.tile_base, bb.data(), 8 * L.ntiles, hipMemcpyHostToDevice) != hipSuccess ||
-		     hipMemcpy(L.tile_top, tt.data(), 8 * L.ntiles, hipMemcpyHostToDevice) != hipSuccess)) {
-			rc = ALP_PANIC;
-			break;
+			S.count = rb.size();
+			if ((rc = dmalloc(&S.rb, S.count)) || (rc = dmalloc(&S.re, S.count)) ||
+			    (rc = dmalloc(&S.base, S.count)) || (rc = dmalloc(&S.top, S.count)))
+				break;
+			if (S.count &&
+			    (hipMemcpy(S.rb, rb.data(), 4 * S.count, hipMemcpyHostToDevice) != hipSuccess ||
+			     hipMemcpy(S.re, re.data(), 4 * S.count, hipMemcpyHostToDevice) != hipSuccess ||
+			     hipMemcpy(S.base, bb.data(), 8 * S.count, hipMemcpyHostToDevice) != hipSuccess ||
+			     hipMemcpy(S.top, tt.data(), 8 * S.count, hipMemcpyHostToDevice) != hipSuccess)) {
+				rc = ALP_PANIC;
+				break;
+			}
 		}
 	} while (0);
 	hipStreamSynchronize(s);
@@ -644,22 +676,41 @@ static alp_rc_t build_level(HpLevel &L, size_t nx, size_t ny, size_t nz, int col
 	return rc;
 }
 
-static Tiles all_tiles(const HpLevel &L) {
-	return Tiles{L.tile_rb, L.tile_re, L.tile_base, L.tile_top, 0, L.ntiles};
+static int g_tile_xcd = -1;
+static int tile_xcd() {
+	if (g_tile_xcd < 0) {
+		const char *e = std::getenv("ALP_HIP_HPCG_XCD");
+		g_tile_xcd = e ? std::atoi(e) : 1;
+	}
+	return g_tile_xcd;
 }
 
+template <typename T>
+static Tiles all_tiles(const HpLevel &L) {
+	const TileSet &S = L.ts[sizeof(T) == 8 ? 0 : 1];
+	return Tiles{S.rb, S.re, S.base, S.top, 0, S.count, tile_xcd()};
+}
+
+template <typename T>
 static Tiles colour_tiles(const HpLevel &L, int c) {
-	return Tiles{L.tile_rb, L.tile_re, L.tile_base, L.tile_top, L.color_tile_off[c],
-	             L.color_tile_off[c + 1] - L.color_tile_off[c]};
+	const TileSet &S = L.ts[sizeof(T) == 8 ? 0 : 1];
+	return Tiles{S.rb, S.re, S.base, S.top, S.color_tile_off[c],
+	             S.color_tile_off[c + 1] - S.color_tile_off[c], tile_xcd()};
 }
 
 // ---- V-cycle enqueue ------------------------------------------------------------------
 
 template <typename T>
-static void colour_step(HpLevel &L, int c, hipStream_t s, const int *done) {
-	const Tiles tl = colour_tiles(L, c);
+static void colour_step(HpLevel &L, int c, hipStream_t s, const int *done, bool z_is_zero = false) {
+	const Tiles tl = colour_tiles<T>(L, c);
 	if (tl.count == 0) return;
 	T *z = Vec<T>::z(L);
+	if (z_is_zero) {
+		const size_t r0 = L.color_off[c], r1 = L.color_off[c + 1];
+		hipLaunchKernelGGL(k_sgs_first<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, s, r0, r1,
+		                   (const T *)Vec<T>::r(L), Vec<T>::diag(L), z, done);
+		return;
+	}
 	if (L.proper) {
 		EpSgs<T> ep{Vec<T>::r(L), Vec<T>::diag(L), z, T(0), T(0), T(0)};
 		hipLaunchKernelGGL((k_tiles<T, EpSgs<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
@@ -674,9 +725,10 @@ static void colour_step(HpLevel &L, int c, hipStream_t s, const int *done) {
 	}
 }
 
+// z_is_zero: z was just set to 0 (the first sweep of a V-cycle level)
 template <typename T>
-static void rbgs(HpLevel &L, hipStream_t s, const int *done) {
-	for (int c = 0; c < L.ncolors; ++c) colour_step<T>(L, c, s, done);
+static void rbgs(HpLevel &L, hipStream_t s, const int *done, bool z_is_zero = false) {
+	for (int c = 0; c < L.ncolors; ++c) colour_step<T>(L, c, s, done, z_is_zero && c == 0);
 	for (int c = L.ncolors - 1; c >= 0; --c) colour_step<T>(L, c, s, done);
 }
 
@@ -685,10 +737,10 @@ template <typename T>
 static void vcycle(alp_hip_hpcg_s *H, size_t l, hipStream_t s, const int *done) {
 	HpLevel &F = H->L[l];
 	if (l + 1 == H->L.size()) {
-		rbgs<T>(F, s, done);
+		rbgs<T>(F, s, done, true);
 		return;
 	}
-	for (size_t i = 0; i < H->presmooth; ++i) rbgs<T>(F, s, done);
+	for (size_t i = 0; i < H->presmooth; ++i) rbgs<T>(F, s, done, i == 0);
 	HpLevel &C = H->L[l + 1];
 	const size_t threads = C.n * 32;
 	hipLaunchKernelGGL(k_restrict<T>, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C.n,
@@ -701,12 +753,11 @@ static void vcycle(alp_hip_hpcg_s *H, size_t l, hipStream_t s, const int *done) 
 	for (size_t i = 0; i < H->postsmooth; ++i) rbgs<T>(F, s, done);
 }
 
-// one hpcg iteration (hpcg.hpp:148-221) on precision T, solution x, state st
+// z = M r (V-cycle) or z = r, in precision T, on the finest level's vectors
 template <typename T>
-static void hpcg_iteration(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bool precond, hipStream_t s) {
+static void precondition(alp_hip_hpcg_s *H, bool precond, hipStream_t s, const int *done) {
 	HpLevel &F = H->L[0];
 	const size_t n = F.n;
-	const int *done = &st->done;
 	T *z = Vec<T>::z(F), *r = Vec<T>::r(F);
 	if (precond) {
 		hipLaunchKernelGGL(k_fill_done<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, T(0), done);
@@ -714,18 +765,59 @@ static void hpcg_iteration(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bo
 	} else {
 		hipLaunchKernelGGL(k_copy_done<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (const T *)r, z, done);
 	}
+}
+
+// the CG part of one hpcg iteration (hpcg.hpp:148-221) once z is known
+template <typename T>
+static void cg_step(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, hipStream_t s) {
+	HpLevel &F = H->L[0];
+	const size_t n = F.n;
+	const int *done = &st->done;
+	T *z = Vec<T>::z(F), *r = Vec<T>::r(F);
 	const unsigned rg = reduce_grid(n);
 	hipLaunchKernelGGL(k_hp_rz<T>, dim3(rg), dim3(kBlock), 0, s, n, (const T *)r, (const T *)z,
 	                   ctx().d_partials, ctx().d_tickets, st);
 	hipLaunchKernelGGL(k_hp_p<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, p, (const T *)z, (const HpState *)st);
-	const Tiles tl = all_tiles(F);
-	EpStoreDot<T> ep{Ap, p, T(0), 0.0};
-	size_t g = tl.count < (size_t)fill_blocks(8) ? tl.count : fill_blocks(8);
-	hipLaunchKernelGGL((k_tiles_reduce<T, EpStoreDot<T>, FinAlpha>), dim3(g), dim3(kBlock), 0, s, tl,
-	                   (const uint64_t *)F.off, (const uint32_t *)F.idx, Vec<T>::val(F), (const T *)p, ep,
-	                   ctx().d_partials, ctx().d_tickets, FinAlpha{st}, done);
+	const Tiles tl = all_tiles<T>(F);
+	// Ap = A p on a one-tile-per-block grid, then p.Ap -> alpha
+	EpStore<T> ep{Ap};
+	hipLaunchKernelGGL((k_tiles<T, EpStore<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
+	                   (const uint64_t *)F.off, (const uint32_t *)F.idx, Vec<T>::val(F), (const T *)p, ep, done);
+	hipLaunchKernelGGL(k_hp_alpha<T>, dim3(rg), dim3(kBlock), 0, s, n, (const T *)p, (const T *)Ap,
+	                   ctx().d_partials, ctx().d_tickets, st);
 	hipLaunchKernelGGL(k_hp_update<T>, dim3(rg), dim3(kBlock), 0, s, n, x, r, (const T *)p, (const T *)Ap,
 	                   ctx().d_partials, ctx().d_tickets, st);
+}
+
+__global__ __launch_bounds__(kBlock) void k_down32(size_t n, const double *r, float *r32, float *z32,
+                                                  const int *done) {
+	if (*done) return;
+	grid_for_each<4>(n, [&](size_t i) {
+		r32[i] = (float)r[i];
+		z32[i] = 0.0f;
+	});
+}
+
+__global__ __launch_bounds__(kBlock) void k_up64(size_t n, const float *z32, double *z, const int *done) {
+	if (*done) return;
+	grid_for_each<4>(n, [&](size_t i) { z[i] = (double)z32[i]; });
+}
+
+// one hpcg iteration on precision T; with FP32PC the (FP64) iteration's
+// preconditioner, the whole V-cycle, runs in FP32
+template <typename T, bool FP32PC>
+static void hpcg_iteration(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bool precond, hipStream_t s) {
+	const int *done = &st->done;
+	if (FP32PC && precond) {
+		HpLevel &F = H->L[0];
+		hipLaunchKernelGGL(k_down32, dim3(grid_for(F.n)), dim3(kBlock), 0, s, F.n, (const double *)F.r, F.r32,
+		                   F.z32, done);
+		vcycle<float>(H, 0, s, done);
+		hipLaunchKernelGGL(k_up64, dim3(grid_for(F.n)), dim3(kBlock), 0, s, F.n, (const float *)F.z32, F.z, done);
+	} else {
+		precondition<T>(H, precond, s, done);
+	}
+	cg_step<T>(H, x, p, Ap, st, s);
 }
 
 // prelude: r = b - A x, norm0 = ||r|| (hpcg.hpp:124-137)
@@ -733,7 +825,7 @@ template <typename T>
 static void hpcg_prelude(alp_hip_hpcg_s *H, const T *b, const T *x, T *Ap, HpState *st, size_t max_iter,
                          double tol, hipStream_t s) {
 	HpLevel &F = H->L[0];
-	const Tiles tl = all_tiles(F);
+	const Tiles tl = all_tiles<T>(F);
 	EpResid<T> ep{b, Vec<T>::r(F), T(0), 0.0};
 	size_t g = tl.count < (size_t)fill_blocks(8) ? tl.count : fill_blocks(8);
 	(void)Ap;
@@ -745,7 +837,7 @@ static void hpcg_prelude(alp_hip_hpcg_s *H, const T *b, const T *x, T *Ap, HpSta
 
 // Run hpcg iterations until the state says done. One iteration is captured into a
 // hipGraph and replayed; the state is read back every `check` iterations.
-template <typename T>
+template <typename T, bool FP32PC = false>
 static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bool precond,
                                HpState *h_st) {
 	hipStream_t s = stream();
@@ -753,7 +845,7 @@ static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st
 	hipGraphExec_t exec = nullptr;
 	bool use_graph = false;
 	if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-		hpcg_iteration<T>(H, x, p, Ap, st, precond, s);
+		hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, s);
 		if (hipStreamEndCapture(s, &graph) == hipSuccess &&
 		    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
 			use_graph = true;
@@ -769,7 +861,7 @@ static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st
 					break;
 				}
 			} else {
-				hpcg_iteration<T>(H, x, p, Ap, st, precond, s);
+				hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, s);
 			}
 		}
 		if (rc) break;
@@ -905,8 +997,8 @@ ALP_EXPORT alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, doub
                                        size_t *iters, double *residual, size_t *outer) {
 	ALP_TRY(ensure_init());
 	if (!H || !d_b || !d_x || !iters || !residual) return ALP_ILLEGAL;
-	if (precision != 0 && precision != 1) return ALP_ILLEGAL;
-	if (precision == 1 && !H->have_f32) return ALP_ILLEGAL;
+	if (precision < 0 || precision > 2) return ALP_ILLEGAL;
+	if (precision != 0 && !H->have_f32) return ALP_ILLEGAL;
 	hipStream_t s = stream();
 	HpLevel &F = H->L[0];
 	const size_t n = F.n;
@@ -918,9 +1010,12 @@ ALP_EXPORT alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, doub
 	HpState h{};
 	size_t total = 0, nouter = 0;
 	double res = 0.0;
-	if (precision == 0) {
+	if (precision == 0 || precision == 2) {
 		hpcg_prelude<double>(H, H->b, H->x, H->Ap, H->st, max_iter, tol, s);
-		ALP_TRY(run_iterations<double>(H, H->x, H->p, H->Ap, H->st, with_precond != 0, &h));
+		if (precision == 0)
+			ALP_TRY(run_iterations<double>(H, H->x, H->p, H->Ap, H->st, with_precond != 0, &h));
+		else
+			ALP_TRY((run_iterations<double, true>(H, H->x, H->p, H->Ap, H->st, with_precond != 0, &h)));
 		total = h.iter;
 		res = h.norm;
 	} else {
@@ -928,7 +1023,7 @@ ALP_EXPORT alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, doub
 		const double inner_tol = 1e-5;
 		IrState hir{};
 		for (int o = 0; o < 200; ++o) {
-			const Tiles tl = all_tiles(F);
+			const Tiles tl = all_tiles<double>(F);
 			EpResid<double> ep{H->b, F.r, 0.0, 0.0};
 			size_t g = tl.count < (size_t)fill_blocks(8) ? tl.count : fill_blocks(8);
 			hipLaunchKernelGGL((k_tiles_reduce<double, EpResid<double>, FinIr>), dim3(g), dim3(kBlock), 0, s, tl,

@@ -45,38 +45,33 @@ alp_rc_t read_scalars(double *h, const double *d, int count) {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(size_t n, T *x, T v) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		x[i] = v;
+	grid_for_each<4>(n, [&](size_t i) { x[i] = v; });
 }
 
 template <typename S, typename D>
 __global__ __launch_bounds__(kBlock) void k_convert(size_t n, const S *x, D *y) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		y[i] = static_cast<D>(x[i]);
+	grid_for_each<4>(n, [&](size_t i) { y[i] = static_cast<D>(x[i]); });
 }
 
 __global__ __launch_bounds__(kBlock) void k_axpy(size_t n, double a, const double *x, double *y) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		y[i] = a * x[i] + y[i];
+	grid_for_each<4>(n, [&](size_t i) { y[i] = a * x[i] + y[i]; });
 }
 
 __global__ __launch_bounds__(kBlock) void k_xpay(size_t n, double a, const double *x,
                                                  const double *y, double *z) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		z[i] = a * x[i] + y[i];
+	grid_for_each<4>(n, [&](size_t i) { z[i] = a * x[i] + y[i]; });
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale(size_t n, double a, double *x) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		x[i] = a * x[i];
+	grid_for_each<4>(n, [&](size_t i) { x[i] = a * x[i]; });
 }
 
 __global__ __launch_bounds__(kBlock) void k_ewise(size_t n, int op, const double *x,
                                                   const double *y, double *z) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(n, [&](size_t i) {
 		const double a = x[i], b = y[i];
 		z[i] = op == 0 ? a + b : op == 1 ? a - b : op == 2 ? a * b : a / b;
-	}
+	});
 }
 
 // Deterministic dot: a fixed grid (a function of n only), fixed per-thread strides
@@ -89,8 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_dot(size_t n, const T *__restrict__ 
 	__shared__ double lds[kBlock / kWave];
 	__shared__ bool is_last;
 	double acc = 0.0;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		acc += y ? (double)x[i] * (double)y[i] : (double)x[i];
+	grid_for_each<4>(n, [&](size_t i) { acc += y ? (double)x[i] * (double)y[i] : (double)x[i]; });
 	acc = block_sum(acc, lds);
 	if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 	if (last_block(ticket, &is_last)) {
@@ -134,10 +128,10 @@ static alp_rc_t dot_impl(size_t n, const T *x, const T *y, double *h_z) {
 __global__ __launch_bounds__(kBlock) void k_row_stats(size_t n, const uint64_t *off,
                                                       unsigned long long *maxlen) {
 	unsigned long long m = 0;
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+	grid_for_each<4>(n, [&](size_t i) {
 		const unsigned long long l = off[i + 1] - off[i];
 		m = l > m ? l : m;
-	}
+	});
 	for (int s = kWave / 2; s > 0; s >>= 1) {
 		const unsigned long long o = __shfl_xor(m, s, kWave);
 		m = o > m ? o : m;
@@ -259,8 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_place(size_t nnz, const uint32_t *so
 }
 
 __global__ __launch_bounds__(kBlock) void k_iota(size_t n, uint32_t *x) {
-	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-		x[i] = static_cast<uint32_t>(i);
+	grid_for_each<4>(n, [&](size_t i) { x[i] = static_cast<uint32_t>(i); });
 }
 
 // d_major/d_minor/d_vals are device arrays in input order

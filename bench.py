@@ -156,6 +156,34 @@ def main() -> int:
             log(f"[bench] CG {tag}: rc={rc} it={it} {dt * 1e3:.1f} ms err={err:.2e}")
         if "cg_fp64" in solves and "cg_mixed" in solves:
             solves["cg_speedup_mixed_vs_fp64"] = round(solves["cg_fp64"]["ms"] / solves["cg_mixed"]["ms"], 3)
+        # HPCG: 4 grids (192/96/48/24), 8-colour symmetric Gauss-Seidel, b = A*1, x0 = 0
+        log("[bench] building the HPCG hierarchy")
+        H = alp.Hpcg(side, side, side, max_levels=3, colours=0)
+        bnorm = np.sqrt(float(alp.dot(b, b)))
+        for precision, tag in ((0, "fp64"), (1, "mixed_ir"), (2, "mixed_pc")):
+            xs = alp.Vector(n, fill=0.0)
+            H.solve(b, xs, 2, 1e-10, precision)  # warm-up (graph capture, first touches)
+            times = []
+            for rep in range(3):
+                xs.set(0.0)
+                alp.sync()
+                t0 = time.perf_counter()
+                it, res, outer = H.solve(b, xs, 500, 1e-10, precision)
+                times.append(time.perf_counter() - t0)
+            r = alp.Vector(n)
+            L.alp_hip_memcpy_d2d(r.ptr, b.ptr, b.nbytes)
+            L.alp_hip_scale_f64(n, -1.0, r.ptr)
+            alp.mxv(r, A, xs)
+            true_rel = float(np.sqrt(alp.dot(r, r)) / bnorm)
+            err = float(np.max(np.abs(xs.numpy() - 1.0)))
+            ms = 1e3 * sorted(times)[1]
+            solves[f"hpcg_{tag}"] = {"ms": round(ms, 2), "iterations": it, "refinements": outer,
+                                     "rel_residual": true_rel, "max_err_vs_exact": err,
+                                     "levels": H.levels, "colours": 8}
+            log(f"[bench] HPCG {tag}: it={it} outer={outer} {ms:.1f} ms rel={true_rel:.2e} err={err:.2e}")
+        for tag in ("mixed_ir", "mixed_pc"):
+            solves[f"hpcg_speedup_{tag}_vs_fp64"] = round(solves["hpcg_fp64"]["ms"] / solves[f"hpcg_{tag}"]["ms"], 3)
+        H.free()
 
     out = {
         "metric": METRIC,

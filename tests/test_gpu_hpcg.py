@@ -108,7 +108,7 @@ def true_relres(A, M, b, x):
     return np.sqrt(A.dot(r, r)) / np.sqrt(A.dot(b, b))
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_hpcg_mixed_and_fp64_reach_1e10(A, precision):
     sizes = (64, 64, 64)
     H = A.Hpcg(*sizes, max_levels=3, colours=0)
@@ -126,7 +126,7 @@ def test_hpcg_192_four_levels(A):
     H = A.Hpcg(*sizes, max_levels=3, colours=0)
     assert H.levels == 4 and H.proper and H.nnz == 189_119_224
     M, b = rhs_ones(A, *sizes)
-    for precision in (0, 1):
+    for precision in (0, 1, 2):
         x = A.Vector(H.n, fill=0.0)
         it, res, outer = H.solve(b, x, 500, 1e-10, precision=precision)
         assert true_relres(A, M, b, x) <= 1.5e-10, precision
