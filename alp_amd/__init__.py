@@ -40,6 +40,7 @@ OVERWRITE = 1 << 20
 
 VAL_F64, VAL_F32, VAL_PATTERN = 0, 1, 2
 MAT_WITH_CCS, MAT_WITH_F32 = 1, 2
+GRAPH_SYMMETRIZE, GRAPH_DROP_SELF_LOOPS = 1, 2
 
 
 class AlpError(RuntimeError):
@@ -104,6 +105,22 @@ SIGNATURES = {
     "alp_hip_convert_f64_to_f32": (_RC, [_S, _P, _P]),
     "alp_hip_convert_f32_to_f64": (_RC, [_S, _P, _P]),
     "alp_hip_cg": (_RC, [_P, _P, _P, _S, _D, _I, ctypes.POINTER(_S), ctypes.POINTER(_D)]),
+    "alp_hip_cg_transposed": (_RC, [_P, _P, _P, _S, _D, _I, ctypes.POINTER(_S), ctypes.POINTER(_D)]),
+    "alp_hip_matrix_conductance27": (_RC, [ctypes.POINTER(_P), _S, _S, _S, _P, _D, _U]),
+    "alp_hip_grad_conductance27": (_RC, [_S, _S, _S, _P, _P, _P]),
+    "alp_hip_grad_matrix": (_RC, [_P, _P, _P, _P]),
+    "alp_hip_rmat_edges": (_RC, [_U, _S, _D, _D, _D, ctypes.c_uint64, _P, _P]),
+    "alp_hip_graph_from_edges": (_RC, [ctypes.POINTER(_P), _S, _S, _P, _P, _U, ctypes.POINTER(_S)]),
+    "alp_hip_max_degree_vertex": (_RC, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_S)]),
+    "alp_hip_bfs": (_RC, [_P, ctypes.c_uint32, _P, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
+    "alp_hip_pagerank": (_RC, [_P, _P, _D, _D, _S, ctypes.POINTER(_S), ctypes.POINTER(_D)]),
+    "alp_hip_khop": (_RC, [_P, ctypes.c_uint32, _S, _P]),
+    "alp_hip_batch_create_laplace7": (_RC, [ctypes.POINTER(_P), _S, _P, _P, _P, _P, _P, _P, _S]),
+    "alp_hip_batch_destroy": (_RC, [_P]),
+    "alp_hip_batch_info": (_RC, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S), _P]),
+    "alp_hip_batch_spmm": (_RC, [_P, _P, _P]),
+    "alp_hip_batch_autotune": (_RC, [_P, ctypes.POINTER(_I)]),
+    "alp_hip_batch_solve": (_RC, [_P, _P, _P, _D, _S, _I, _P, _P, ctypes.POINTER(_S)]),
     "alp_hip_hpcg_create": (_RC, [ctypes.POINTER(_P), _S, _S, _S, _S, _S, _S, _S, _U]),
     "alp_hip_hpcg_destroy": (_RC, [_P]),
     "alp_hip_hpcg_info": (_RC, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S),
@@ -227,6 +244,16 @@ class Matrix:
         return cls(h.value)
 
     @classmethod
+    def conductance27(cls, nx, ny, nz, theta, shift: float = 1e-2, flags: int = 0) -> "Matrix":
+        """A(theta) of the differentiable solve; theta: a Vector or a device
+        pointer to 13 nx ny nz doubles (read during the call only)."""
+        h = _P()
+        tp = theta.ptr if isinstance(theta, Vector) else theta
+        check(lib().alp_hip_matrix_conductance27(ctypes.byref(h), nx, ny, nz, tp, shift, flags),
+              "matrix_conductance27")
+        return cls(h.value)
+
+    @classmethod
     def from_crs(cls, nrows, ncols, off, idx, val=None, value_type=None, flags=0) -> "Matrix":
         off = np.ascontiguousarray(off, np.uint64)
         idx = np.ascontiguousarray(idx, np.uint32)
@@ -317,10 +344,14 @@ def dot(x: Vector, y: Vector, z: float = 0.0) -> float:
     return out.value
 
 
-def cg(A: Matrix, b: Vector, x: Vector, max_iter: int, tol: float, precision: int = 0):
-    """grb::algorithms::conjugate_gradient on the device; returns (rc, iterations, ||r||^2)."""
+def cg(A: Matrix, b, x, max_iter: int, tol: float, precision: int = 0, transpose: bool = False):
+    """grb::algorithms::conjugate_gradient on the device (on A^T with transpose);
+    b and x are Vectors or raw device pointers; returns (rc, iterations, ||r||^2)."""
     it, res = _S(), _D()
-    rc = lib().alp_hip_cg(A.h, b.ptr, x.ptr, max_iter, tol, precision, ctypes.byref(it), ctypes.byref(res))
+    f = lib().alp_hip_cg_transposed if transpose else lib().alp_hip_cg
+    bp = b.ptr if isinstance(b, Vector) else b
+    xp = x.ptr if isinstance(x, Vector) else x
+    rc = f(A.h, bp, xp, max_iter, tol, precision, ctypes.byref(it), ctypes.byref(res))
     return rc, it.value, res.value
 
 
@@ -351,6 +382,106 @@ class Hpcg:
     def free(self) -> None:
         if getattr(self, "h", None):
             lib().alp_hip_hpcg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# graphs
+# ---------------------------------------------------------------------------
+
+def rmat_edges(scale: int, nedges: int, a=0.57, b=0.19, c=0.19, seed=1):
+    """Device R-MAT edge lists (src, dst) as uint32 Vectors."""
+    src = Vector(nedges, np.uint32)
+    dst = Vector(nedges, np.uint32)
+    check(lib().alp_hip_rmat_edges(scale, nedges, a, b, c, seed, src.ptr, dst.ptr), "rmat_edges")
+    return src, dst
+
+
+def graph_from_edges(n: int, src: Vector, dst: Vector, flags=GRAPH_SYMMETRIZE | GRAPH_DROP_SELF_LOOPS) -> Matrix:
+    h = _P()
+    nnz = _S()
+    check(lib().alp_hip_graph_from_edges(ctypes.byref(h), n, src.n, src.ptr, dst.ptr, flags, ctypes.byref(nnz)),
+          "graph_from_edges")
+    return Matrix(h.value)
+
+
+def max_degree_vertex(A: Matrix):
+    v, d = ctypes.c_uint32(), _S()
+    check(lib().alp_hip_max_degree_vertex(A.h, ctypes.byref(v), ctypes.byref(d)), "max_degree_vertex")
+    return v.value, d.value
+
+
+def bfs(A: Matrix, source: int):
+    """(levels as an int32 Vector, number of levels, vertices reached)."""
+    lev = Vector(A.nrows, np.int32)
+    nl, reached = _S(), _S()
+    check(lib().alp_hip_bfs(A.h, source, lev.ptr, ctypes.byref(nl), ctypes.byref(reached)), "bfs")
+    return lev, nl.value, reached.value
+
+
+def pagerank(L: Matrix, alpha=0.85, conv=1e-7, max_iter=1000):
+    """(rc, ranks Vector, iterations, quality) as grb::algorithms::simple_pagerank."""
+    pr = Vector(L.nrows)
+    it, q = _S(), _D()
+    rc = lib().alp_hip_pagerank(L.h, pr.ptr, alpha, conv, max_iter, ctypes.byref(it), ctypes.byref(q))
+    return rc, pr, it.value, q.value
+
+
+def khop(A: Matrix, source: int, k: int) -> Vector:
+    out = Vector(A.nrows, np.uint8)
+    check(lib().alp_hip_khop(A.h, source, k, out.ptr), "khop")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# batched multi-RHS CG
+# ---------------------------------------------------------------------------
+
+class Batch:
+    """Block-diagonal batch of anisotropic 7-point Laplacians with nrhs right-hand sides."""
+
+    def __init__(self, nx, ny, nz, cx, cy, cz, nrhs=8):
+        self.nsys = len(nx)
+        arrs = [np.ascontiguousarray(a, np.uint32) for a in (nx, ny, nz)] + \
+               [np.ascontiguousarray(a, np.float64) for a in (cx, cy, cz)]
+        h = _P()
+        check(lib().alp_hip_batch_create_laplace7(ctypes.byref(h), self.nsys, *[a.ctypes.data for a in arrs], nrhs),
+              "batch_create_laplace7")
+        self.h = h.value
+        self.nrhs = nrhs
+        nr, nnz = _S(), _S()
+        self.row0 = np.zeros(self.nsys + 1, np.uint64)
+        check(lib().alp_hip_batch_info(self.h, ctypes.byref(nr), ctypes.byref(nnz), self.row0.ctypes.data),
+              "batch_info")
+        self.nrows, self.nnz = nr.value, nnz.value
+
+    def spmm(self, X: Vector, Y: Vector) -> None:
+        check(lib().alp_hip_batch_spmm(self.h, X.ptr, Y.ptr), "batch_spmm")
+
+    def autotune(self) -> int:
+        w = _I()
+        check(lib().alp_hip_batch_autotune(self.h, ctypes.byref(w)), "batch_autotune")
+        return w.value
+
+    def solve(self, B: Vector, X: Vector, tol: float, max_iter: int = 10000, precision: int = 0):
+        """(rc, iterations per instance, true relative residual per instance, refinement steps)."""
+        n = self.nsys * self.nrhs
+        its = np.zeros(n, np.uint64)
+        rel = np.zeros(n)
+        outer = _S()
+        rc = lib().alp_hip_batch_solve(self.h, B.ptr, X.ptr, tol, max_iter, precision, its.ctypes.data,
+                                       rel.ctypes.data, ctypes.byref(outer))
+        return rc, its.reshape(self.nsys, self.nrhs), rel.reshape(self.nsys, self.nrhs), outer.value
+
+    def free(self) -> None:
+        if getattr(self, "h", None):
+            lib().alp_hip_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

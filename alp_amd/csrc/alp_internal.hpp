@@ -191,6 +191,20 @@ alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A);
 alp_rc_t spmv(const Storage &S, int value_type, alp_semiring_t ring, unsigned descr, void *y,
               const void *x, const uint8_t *mask, hipStream_t s);
 
+// a host-side preconditioner z = M^-1 r: fn(z, r, data), non-zero return = failure
+template <typename T>
+struct HostPrecond {
+	int (*fn)(T *, const T *, void *) = nullptr;
+	void *data = nullptr;
+};
+
+// CG in T's arithmetic on A's CRS values (A's value type must be T's); x and b
+// are device vectors; returns SUCCESS, FAILED (no convergence, or the
+// preconditioner failed), ILLEGAL/MISMATCH or PANIC.
+template <typename T>
+alp_rc_t cg_native(alp_hip_matrix_s *A, const T *b, T *x, size_t max_iter, double tol, HostPrecond<T> pc,
+                   size_t *iters, double *residual);
+
 // device scalars shared with the host
 alp_rc_t read_scalars(double *h, const double *d, int count);
 

@@ -1,0 +1,787 @@
+// Batched multi-RHS CG: many independent SPD systems, several right-hand sides
+// each, solved together. No counterpart in the reference (ALP solves one system
+// per conjugate_gradient call, algorithms/conjugate_gradient.hpp:191-484); each
+// instance follows that CG's recurrences.
+//
+// Layout. The systems are stacked block-diagonally in one CRS (system s owns rows
+// [row0[s], row0[s+1])); block vectors are row-major with the nrhs values of a row
+// contiguous, so one pass over A serves all right-hand sides (SpMM). Rows are cut
+// into tiles of kRowsPerTile rows that never straddle two systems; every
+// per-instance dot product is a fixed-order fold of per-tile partials.
+//
+// One iteration for all (system, rhs) instances is five launches -- SpMM with the
+// p.Ap partials, per-instance alpha, x/r update with the r.r partials,
+// per-instance beta and convergence flags, direction update -- captured once in
+// a hipGraph and replayed; converged instances freeze (their scalars are 0 and
+// their flags stop updates), the host only reads the active count between
+// batches of iterations.
+//
+// precision 0: FP64 CG. precision 1: FP32 SpMM and CG iterations on each
+// instance's residual equation inside FP64 iterative refinement (B - A X and the
+// update of X in FP64), per right-hand side, to the requested relative residual.
+#include "spmv_device.hpp"
+
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+namespace alp {
+
+constexpr int kRowsPerTile = 256;
+constexpr int kMaxRhs = 16;
+
+struct BatchInst {
+	double rz, alpha, beta, rr, tol, bnorm, scale;
+	int active, converged;
+	unsigned long long iter;
+};
+
+// ---- generator: anisotropic 7-point Laplacian per system -------------------------------
+// a(i,i) = 2 (cx + cy + cz), a(i, i +- 1) = -cx, a(i, i +- nx) = -cy, a(i, i +- nx ny) = -cz,
+// homogeneous Dirichlet boundaries; columns ascending.
+
+struct SysDesc {
+	uint32_t nx, ny, nz, pad;
+	double cx, cy, cz;
+	uint64_t row0, nnz0;
+};
+
+__device__ __forceinline__ unsigned lap_count(uint32_t x, uint32_t y, uint32_t z, const SysDesc &d) {
+	return 1u + (x > 0) + (x + 1 < d.nx) + (y > 0) + (y + 1 < d.ny) + (z > 0) + (z + 1 < d.nz);
+}
+
+__global__ __launch_bounds__(kBlock) void k_lap_counts(size_t nsys, const SysDesc *sys, const uint32_t *row_sys,
+                                                       size_t nrows, uint64_t *counts) {
+	grid_for_each<1>(nrows, [&](size_t r) {
+		const SysDesc d = sys[row_sys[r]];
+		const uint64_t i = r - d.row0;
+		const uint32_t x = i % d.nx, y = (i / d.nx) % d.ny, z = i / ((uint64_t)d.nx * d.ny);
+		counts[r] = lap_count(x, y, z, d);
+	});
+}
+
+__global__ __launch_bounds__(kBlock) void k_lap_fill(size_t nrows, const SysDesc *sys, const uint32_t *row_sys,
+                                                     const uint64_t *off, uint32_t *idx, double *val) {
+	grid_for_each<1>(nrows, [&](size_t r) {
+		const SysDesc d = sys[row_sys[r]];
+		const uint64_t i = r - d.row0;
+		const uint64_t sxy = (uint64_t)d.nx * d.ny;
+		const uint32_t x = i % d.nx, y = (i / d.nx) % d.ny, z = i / sxy;
+		uint64_t k = off[r];
+		auto put = [&](uint64_t c, double v) {
+			idx[k] = (uint32_t)(d.row0 + c);
+			val[k] = v;
+			++k;
+		};
+		if (z > 0) put(i - sxy, -d.cz);
+		if (y > 0) put(i - d.nx, -d.cy);
+		if (x > 0) put(i - 1, -d.cx);
+		put(i, 2.0 * (d.cx + d.cy + d.cz));
+		if (x + 1 < d.nx) put(i + 1, -d.cx);
+		if (y + 1 < d.ny) put(i + d.nx, -d.cy);
+		if (z + 1 < d.nz) put(i + sxy, -d.cz);
+	});
+}
+
+// ---- SpMM with per-tile, per-rhs partial dots -----------------------------------------------
+// W lanes per row, each lane owns nrhs / W right-hand sides (RPL). With DOT, the
+// partial of sum_i p[i][r] * (A p)[i][r] over the tile goes to part[tile*nrhs + r].
+// RES computes out = B - A X instead of A X and partials of out.out.
+
+template <typename T, int W, int RPL, bool DOT, bool RES>
+__global__ __launch_bounds__(kBlock) void k_spmm(size_t ntiles, const uint32_t *tile_row0, const uint32_t *tile_row1,
+                                                 const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx,
+                                                 const T *__restrict__ val, const T *__restrict__ X,
+                                                 const T *__restrict__ Bm, T *__restrict__ Y, double *part,
+                                                 const int *inst_active, const uint32_t *tile_sys) {
+	constexpr int NR = W * RPL;
+	constexpr int kRows = kBlock / W;
+	__shared__ double red[kBlock / kWave][NR];
+	const int lane = threadIdx.x % W;
+	const unsigned grp = threadIdx.x / W;
+	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		const uint32_t r0 = tile_row0[t], r1 = tile_row1[t];
+		double acc_dot[RPL];
+#pragma unroll
+		for (int q = 0; q < RPL; ++q) acc_dot[q] = 0.0;
+		for (uint32_t rb = r0; rb < r1; rb += kRows) {
+			const uint32_t r = rb + grp;
+			if (r >= r1) continue;
+			T acc[RPL];
+#pragma unroll
+			for (int q = 0; q < RPL; ++q) acc[q] = T(0);
+			const uint64_t b = off[r], e = off[r + 1];
+			for (uint64_t k = b; k < e; ++k) {
+				const uint32_t j = idx[k];
+				const T a = val[k];
+				const T *xr = X + (size_t)j * NR + lane * RPL;
+#pragma unroll
+				for (int q = 0; q < RPL; ++q) acc[q] += a * xr[q];
+			}
+			T *yr = Y + (size_t)r * NR + lane * RPL;
+			if (RES) {
+				const T *br = Bm + (size_t)r * NR + lane * RPL;
+#pragma unroll
+				for (int q = 0; q < RPL; ++q) {
+					const T v = br[q] - acc[q];
+					yr[q] = v;
+					acc_dot[q] += (double)v * (double)v;
+				}
+			} else {
+#pragma unroll
+				for (int q = 0; q < RPL; ++q) yr[q] = acc[q];
+				if (DOT) {
+					const T *pr = X + (size_t)r * NR + lane * RPL;
+#pragma unroll
+					for (int q = 0; q < RPL; ++q) acc_dot[q] += (double)acc[q] * (double)pr[q];
+				}
+			}
+		}
+		if (DOT || RES) {
+			// fold over the rows of the tile: lanes with the same `lane` own the same rhs
+#pragma unroll
+			for (int q = 0; q < RPL; ++q) {
+				double v = acc_dot[q];
+				for (int s = kWave / 2; s >= W; s >>= 1) v += __shfl_xor(v, s, kWave);
+				acc_dot[q] = v;
+			}
+			__syncthreads();
+			if ((threadIdx.x % kWave) < W) {
+#pragma unroll
+				for (int q = 0; q < RPL; ++q) red[threadIdx.x / kWave][lane * RPL + q] = acc_dot[q];
+			}
+			__syncthreads();
+			if (threadIdx.x < NR) {
+				double s = 0.0;
+				for (int w = 0; w < kBlock / kWave; ++w) s += red[w][threadIdx.x];
+				part[t * NR + threadIdx.x] = s;
+			}
+		}
+	}
+	(void)inst_active;
+	(void)tile_sys;
+}
+
+// Per-system fold of the tile partials (tiles of system s: [tile0[s], tile0[s+1])),
+// one block per system: thread t folds tiles t, t + kBlock, ... for every
+// right-hand side, then a fixed tree over the block; sums[r] (LDS) gets the
+// instance (s, r) total. Deterministic.
+__device__ __forceinline__ void system_sums(const double *part, const uint32_t *tile0, int s, int nrhs,
+                                            double *sums /* LDS, kMaxRhs */) {
+	__shared__ double red[kBlock / kWave][kMaxRhs];
+	double acc[kMaxRhs];
+#pragma unroll
+	for (int r = 0; r < kMaxRhs; ++r) acc[r] = 0.0;
+	for (uint32_t t = tile0[s] + threadIdx.x; t < tile0[s + 1]; t += kBlock)
+		for (int r = 0; r < nrhs; ++r) acc[r] += part[(size_t)t * nrhs + r];
+#pragma unroll
+	for (int r = 0; r < kMaxRhs; ++r) {
+		double v = acc[r];
+		for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+		acc[r] = v;
+	}
+	__syncthreads();
+	if (threadIdx.x % kWave == 0)
+		for (int r = 0; r < kMaxRhs; ++r) red[threadIdx.x / kWave][r] = acc[r];
+	__syncthreads();
+	if (threadIdx.x < kMaxRhs) {
+		double v = 0.0;
+		for (int w = 0; w < kBlock / kWave; ++w) v += red[w][threadIdx.x];
+		sums[threadIdx.x] = v;
+	}
+	__syncthreads();
+}
+
+// alpha = rz / pAp for active instances (one block per system)
+__global__ __launch_bounds__(kBlock) void k_b_alpha(int nrhs, const double *part, const uint32_t *tile0,
+                                                    BatchInst *in) {
+	__shared__ double sums[kMaxRhs];
+	const int s = blockIdx.x;
+	system_sums(part, tile0, s, nrhs, sums);
+	if (threadIdx.x < nrhs) {
+		BatchInst &I = in[s * nrhs + threadIdx.x];
+		I.alpha = I.active ? I.rz / sums[threadIdx.x] : 0.0;
+	}
+}
+
+// X += alpha P; R -= alpha AP, with the partials of R.R
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_update(size_t ntiles, const uint32_t *tile_row0,
+                                                     const uint32_t *tile_row1, const uint32_t *tile_sys, int nrhs,
+                                                     T *X, T *R, const T *P, const T *AP, const BatchInst *in,
+                                                     double *part) {
+	__shared__ double red[kBlock];
+	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		const size_t base = (size_t)tile_row0[t] * nrhs;
+		const size_t cnt = (size_t)(tile_row1[t] - tile_row0[t]) * nrhs;
+		const int sys = tile_sys[t];
+		// thread k handles entries k, k + kBlock, ... of the tile; kBlock % nrhs == 0,
+		// so each thread keeps one right-hand side
+		const int r = threadIdx.x % nrhs;
+		const BatchInst &I = in[sys * nrhs + r];
+		const T alpha = (T)I.alpha;
+		double acc = 0.0;
+		for (size_t k = threadIdx.x; k < cnt; k += kBlock) {
+			const size_t e = base + k;
+			X[e] = alpha * P[e] + X[e];
+			const T rv = R[e] - alpha * AP[e];
+			R[e] = rv;
+			acc += (double)rv * (double)rv;
+		}
+		red[threadIdx.x] = acc;
+		__syncthreads();
+		if (threadIdx.x < nrhs) {
+			double s = 0.0;
+			for (int k = threadIdx.x; k < kBlock; k += nrhs) s += red[k];
+			part[t * nrhs + threadIdx.x] = s;
+		}
+		__syncthreads();
+	}
+}
+
+// beta = rr / rz; stop tests (||r|| < tol * ||b||, conjugate_gradient.hpp:437)
+__global__ __launch_bounds__(kBlock) void k_b_beta(int nrhs, const double *part, const uint32_t *tile0,
+                                                   BatchInst *in, unsigned long long max_iter, unsigned *nactive) {
+	__shared__ double sums[kMaxRhs];
+	const int s = blockIdx.x;
+	system_sums(part, tile0, s, nrhs, sums);
+	if (threadIdx.x >= nrhs) return;
+	BatchInst &I = in[s * nrhs + threadIdx.x];
+	if (!I.active) {
+		I.beta = 0.0;
+		return;
+	}
+	const double rr = sums[threadIdx.x];
+	I.rr = rr;
+	I.iter += 1;
+	const bool conv = std::sqrt(rr) < I.tol;
+	if (conv || I.iter >= max_iter) {
+		I.active = 0;
+		I.converged = conv ? 1 : 0;
+		I.beta = 0.0;
+	} else {
+		I.beta = rr / I.rz;
+		I.rz = rr;
+		atomicAdd(nactive, 1u);
+	}
+}
+
+// P = R + beta P for active instances
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_dir(size_t ntiles, const uint32_t *tile_row0, const uint32_t *tile_row1,
+                                                  const uint32_t *tile_sys, int nrhs, T *P, const T *R,
+                                                  const BatchInst *in) {
+	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		const size_t base = (size_t)tile_row0[t] * nrhs;
+		const size_t cnt = (size_t)(tile_row1[t] - tile_row0[t]) * nrhs;
+		const int sys = tile_sys[t];
+		const int r = threadIdx.x % nrhs;
+		const BatchInst &I = in[sys * nrhs + r];
+		if (!I.active) continue;
+		const T beta = (T)I.beta;
+		for (size_t k = threadIdx.x; k < cnt; k += kBlock) P[base + k] = beta * P[base + k] + R[base + k];
+	}
+}
+
+// start of a (sub)solve from X = 0: R = B_s (scaled), P = R, rz = ||R||^2 per instance
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_start_part(size_t ntiles, const uint32_t *tile_row0,
+                                                         const uint32_t *tile_row1, int nrhs, const T *R,
+                                                         double *part) {
+	__shared__ double red[kBlock];
+	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		const size_t base = (size_t)tile_row0[t] * nrhs;
+		const size_t cnt = (size_t)(tile_row1[t] - tile_row0[t]) * nrhs;
+		double acc = 0.0;
+		for (size_t k = threadIdx.x; k < cnt; k += kBlock) acc += (double)R[base + k] * (double)R[base + k];
+		red[threadIdx.x] = acc;
+		__syncthreads();
+		if (threadIdx.x < nrhs) {
+			double s = 0.0;
+			for (int k = threadIdx.x; k < kBlock; k += nrhs) s += red[k];
+			part[t * nrhs + threadIdx.x] = s;
+		}
+		__syncthreads();
+	}
+}
+
+// instance set-up for an inner solve: rz = ||r||^2, tol absolute = reltol * ||r||,
+// inactive when the outer loop already converged it
+__global__ __launch_bounds__(kBlock) void k_b_start(int nrhs, const double *part, const uint32_t *tile0,
+                                                    BatchInst *in, const int *outer_active, double reltol,
+                                                    unsigned *nactive) {
+	__shared__ double sums[kMaxRhs];
+	const int s = blockIdx.x;
+	system_sums(part, tile0, s, nrhs, sums);
+	if (threadIdx.x >= nrhs) return;
+	const int i = s * nrhs + threadIdx.x;
+	BatchInst &I = in[i];
+	const double rz = sums[threadIdx.x];
+	I.rz = rz;
+	I.rr = rz;
+	I.iter = 0;
+	I.tol = reltol * std::sqrt(rz);
+	I.alpha = I.beta = 0.0;
+	I.converged = 0;
+	I.active = (outer_active[i] && rz > 0.0) ? 1 : 0;
+	if (I.active) atomicAdd(nactive, 1u);
+}
+
+// outer (FP64) residual norms: rnorm = ||B - A X||, stop at tol * ||B||
+__global__ __launch_bounds__(kBlock) void k_b_outer(int nrhs, const double *part, const uint32_t *tile0,
+                                                    double *rnorm, const double *bnorm, int *outer_active,
+                                                    double tol, unsigned *nactive) {
+	__shared__ double sums[kMaxRhs];
+	const int s = blockIdx.x;
+	system_sums(part, tile0, s, nrhs, sums);
+	if (threadIdx.x >= nrhs) return;
+	const int i = s * nrhs + threadIdx.x;
+	const double rn = std::sqrt(sums[threadIdx.x]);
+	rnorm[i] = rn;
+	const bool done = !(rn > tol * bnorm[i]);
+	outer_active[i] = done ? 0 : 1;
+	if (!done) atomicAdd(nactive, 1u);
+}
+
+// R32 = R64 / rnorm, X32 = 0, P32 = R32 (row-major blocks)
+__global__ __launch_bounds__(kBlock) void k_b_down(size_t ntot, int nrhs, const double *R, const double *rnorm,
+                                                   const uint32_t *row_sys, float *R32, float *X32, float *P32) {
+	grid_for_each<4>(ntot, [&](size_t e) {
+		const size_t row = e / nrhs;
+		const int r = (int)(e % nrhs);
+		const double rn = rnorm[(size_t)row_sys[row] * nrhs + r];
+		const float v = rn > 0.0 ? (float)(R[e] / rn) : 0.0f;
+		R32[e] = v;
+		P32[e] = v;
+		X32[e] = 0.0f;
+	});
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_up(size_t ntot, int nrhs, double *X, const float *D,
+                                                 const double *rnorm, const int *outer_active,
+                                                 const uint32_t *row_sys) {
+	grid_for_each<4>(ntot, [&](size_t e) {
+		const size_t row = e / nrhs;
+		const int r = (int)(e % nrhs);
+		const size_t inst = (size_t)row_sys[row] * nrhs + r;
+		if (outer_active[inst]) X[e] = rnorm[inst] * (double)D[e] + X[e];
+	});
+}
+
+// ||b|| per instance; an instance with b = 0 is solved by x = 0
+__global__ __launch_bounds__(kBlock) void k_b_bnorm(int nrhs, const double *part, const uint32_t *tile0,
+                                                    double *bnorm, int *outer_active) {
+	__shared__ double sums[kMaxRhs];
+	const int s = blockIdx.x;
+	system_sums(part, tile0, s, nrhs, sums);
+	if (threadIdx.x >= nrhs) return;
+	const int i = s * nrhs + threadIdx.x;
+	const double bn = std::sqrt(sums[threadIdx.x]);
+	bnorm[i] = bn;
+	outer_active[i] = bn > 0.0 ? 1 : 0;
+}
+
+// a counter reset as a kernel, so that it can sit inside a captured graph
+__global__ void k_b_zero(unsigned *c) { *c = 0u; }
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_copy(size_t n, const T *s, T *d) {
+	grid_for_each<4>(n, [&](size_t i) { d[i] = s[i]; });
+}
+
+}  // namespace alp
+
+using namespace alp;
+
+struct alp_hip_batch_s {
+	size_t nsys = 0, nrhs = 0, nrows = 0, nnz = 0;
+	std::vector<uint64_t> row0;       // host: first row of each system (nsys + 1)
+	uint64_t *off = nullptr;
+	uint32_t *idx = nullptr;
+	double *val = nullptr;
+	float *val32 = nullptr;
+	uint32_t *row_sys = nullptr;      // system of each row
+	SysDesc *sys = nullptr;
+	size_t ntiles = 0;
+	uint32_t *tile_row0 = nullptr, *tile_row1 = nullptr, *tile_sys = nullptr;
+	uint32_t *tile0 = nullptr;        // first tile of each system (nsys + 1)
+	int W = 0;                        // lanes per row of the SpMM (autotuned)
+	// workspace
+	double *part = nullptr;
+	BatchInst *inst = nullptr;
+	unsigned *nactive = nullptr;
+	double *rnorm = nullptr, *bnorm = nullptr;
+	int *outer_active = nullptr;
+	double *R = nullptr, *P = nullptr, *AP = nullptr;
+	float *R32 = nullptr, *P32 = nullptr, *AP32 = nullptr, *X32 = nullptr;
+};
+
+namespace alp {
+
+template <typename T, bool DOT, bool RES>
+static void launch_spmm(alp_hip_batch_s *B, int W, const T *val, const T *X, const T *Bm, T *Y, hipStream_t s) {
+	const size_t grid = B->ntiles < (size_t)fill_blocks(8) ? B->ntiles : fill_blocks(8);
+	const int nr = (int)B->nrhs;
+#define ALP_SPMM(WW, RPL)                                                                                 \
+	hipLaunchKernelGGL((k_spmm<T, WW, RPL, DOT, RES>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,        \
+	                   (const uint32_t *)B->tile_row0, (const uint32_t *)B->tile_row1,                  \
+	                   (const uint64_t *)B->off, (const uint32_t *)B->idx, val, X, Bm, Y, B->part,       \
+	                   (const int *)nullptr, (const uint32_t *)B->tile_sys)
+	if (nr == 8) {
+		if (W == 8) ALP_SPMM(8, 1);
+		else if (W == 4) ALP_SPMM(4, 2);
+		else if (W == 2) ALP_SPMM(2, 4);
+		else ALP_SPMM(1, 8);
+	} else if (nr == 4) {
+		if (W >= 4) ALP_SPMM(4, 1);
+		else if (W == 2) ALP_SPMM(2, 2);
+		else ALP_SPMM(1, 4);
+	} else if (nr == 2) {
+		if (W >= 2) ALP_SPMM(2, 1);
+		else ALP_SPMM(1, 2);
+	} else if (nr == 1) {
+		ALP_SPMM(1, 1);
+	} else {  // 16
+		if (W >= 16) ALP_SPMM(16, 1);
+		else if (W == 8) ALP_SPMM(8, 2);
+		else if (W == 4) ALP_SPMM(4, 4);
+		else ALP_SPMM(2, 8);
+	}
+#undef ALP_SPMM
+}
+
+static void free_batch(alp_hip_batch_s *B) {
+	void *ps[] = {B->off, B->idx, B->val, B->val32, B->row_sys, B->sys, B->tile_row0, B->tile_row1, B->tile_sys,
+	              B->tile0, B->part, B->inst, B->nactive, B->rnorm, B->bnorm, B->outer_active, B->R, B->P, B->AP,
+	              B->R32, B->P32, B->AP32, B->X32};
+	for (void *p : ps)
+		if (p) hipFree(p);
+}
+
+template <typename T>
+static alp_rc_t bmalloc(T **p, size_t n) {
+	ALP_HIP_TRY(hipMalloc(p, sizeof(T) * (n ? n : 1)));
+	return ALP_SUCCESS;
+}
+
+// Iterate the inner CG on precision T until no instance is active.
+template <typename T>
+static alp_rc_t batch_iterate(alp_hip_batch_s *B, const T *val, T *X, T *R, T *P, T *AP, size_t max_iter,
+                              size_t *its) {
+	hipStream_t s = stream();
+	const unsigned nsys = (unsigned)B->nsys;
+	const int nr = (int)B->nrhs;
+	const size_t g = B->ntiles < (size_t)fill_blocks(8) ? B->ntiles : fill_blocks(8);
+	auto iteration = [&](hipStream_t st) {
+		launch_spmm<T, true, false>(B, B->W, val, P, nullptr, AP, st);
+		hipLaunchKernelGGL(k_b_alpha, dim3(nsys), dim3(kBlock), 0, st, nr, (const double *)B->part,
+		                   (const uint32_t *)B->tile0, B->inst);
+		hipLaunchKernelGGL(k_b_update<T>, dim3(g), dim3(kBlock), 0, st, B->ntiles, (const uint32_t *)B->tile_row0,
+		                   (const uint32_t *)B->tile_row1, (const uint32_t *)B->tile_sys, nr, X, R, (const T *)P,
+		                   (const T *)AP, (const BatchInst *)B->inst, B->part);
+		hipLaunchKernelGGL(k_b_beta, dim3(nsys), dim3(kBlock), 0, st, nr, (const double *)B->part,
+		                   (const uint32_t *)B->tile0, B->inst, (unsigned long long)max_iter, B->nactive);
+		hipLaunchKernelGGL(k_b_dir<T>, dim3(g), dim3(kBlock), 0, st, B->ntiles, (const uint32_t *)B->tile_row0,
+		                   (const uint32_t *)B->tile_row1, (const uint32_t *)B->tile_sys, nr, P, (const T *)R,
+		                   (const BatchInst *)B->inst);
+	};
+	// capture `batch` iterations once; the active count of the last one is read back
+	const int batch = 8;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	bool use_graph = false;
+	if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+		for (int i = 0; i < batch; ++i) {
+			// nactive counts the instances still active after each iteration
+			hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(1), 0, s, B->nactive);
+			iteration(s);
+		}
+		if (hipStreamEndCapture(s, &graph) == hipSuccess &&
+		    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
+			use_graph = true;
+	}
+	(void)hipGetLastError();
+	size_t done_its = 0;
+	alp_rc_t rc = ALP_SUCCESS;
+	for (;;) {
+		if (use_graph) {
+			if (hipGraphLaunch(exec, s) != hipSuccess) {
+				rc = ALP_PANIC;
+				break;
+			}
+		} else {
+			for (int i = 0; i < batch; ++i) {
+				hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(1), 0, s, B->nactive);
+				iteration(s);
+			}
+		}
+		done_its += batch;
+		unsigned act = 0;
+		if (hipMemcpyAsync(&act, B->nactive, sizeof(act), hipMemcpyDeviceToHost, s) != hipSuccess ||
+		    hipStreamSynchronize(s) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if (act == 0 || done_its >= max_iter + batch) break;
+	}
+	if (exec) hipGraphExecDestroy(exec);
+	if (graph) hipGraphDestroy(graph);
+	if (hipGetLastError() != hipSuccess) rc = ALP_PANIC;
+	*its = done_its;
+	return rc;
+}
+
+}  // namespace alp
+
+ALP_EXPORT alp_rc_t alp_hip_batch_destroy(alp_hip_batch_t B) {
+	if (!B) return ALP_SUCCESS;
+	hipStreamSynchronize(stream());
+	free_batch(B);
+	delete B;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *out, size_t nsys, const uint32_t *h_nx,
+                                                  const uint32_t *h_ny, const uint32_t *h_nz, const double *h_cx,
+                                                  const double *h_cy, const double *h_cz, size_t nrhs) {
+	ALP_TRY(ensure_init());
+	if (!out || nsys == 0 || !h_nx || !h_ny || !h_nz || !h_cx || !h_cy || !h_cz) return ALP_ILLEGAL;
+	*out = nullptr;
+	if (!(nrhs == 1 || nrhs == 2 || nrhs == 4 || nrhs == 8 || nrhs == 16)) return ALP_ILLEGAL;
+	auto *B = new (std::nothrow) alp_hip_batch_s();
+	if (!B) return ALP_OUTOFMEM;
+	B->nsys = nsys;
+	B->nrhs = nrhs;
+	B->row0.assign(nsys + 1, 0);
+	std::vector<SysDesc> desc(nsys);
+	std::vector<uint32_t> t0(nsys + 1, 0), trow0, trow1, tsys;
+	uint64_t nnz = 0;
+	alp_rc_t rc = ALP_SUCCESS;
+	for (size_t s = 0; s < nsys; ++s) {
+		if (h_nx[s] == 0 || h_ny[s] == 0 || h_nz[s] == 0 || !(h_cx[s] > 0 && h_cy[s] > 0 && h_cz[s] > 0)) {
+			rc = ALP_ILLEGAL;
+			break;
+		}
+		const uint64_t n = (uint64_t)h_nx[s] * h_ny[s] * h_nz[s];
+		desc[s] = SysDesc{h_nx[s], h_ny[s], h_nz[s], 0, h_cx[s], h_cy[s], h_cz[s], B->row0[s], nnz};
+		B->row0[s + 1] = B->row0[s] + n;
+		nnz += 7 * n - 2 * ((uint64_t)h_ny[s] * h_nz[s] + (uint64_t)h_nx[s] * h_nz[s] + (uint64_t)h_nx[s] * h_ny[s]);
+		for (uint64_t r = B->row0[s]; r < B->row0[s + 1]; r += kRowsPerTile) {
+			trow0.push_back((uint32_t)r);
+			trow1.push_back((uint32_t)std::min<uint64_t>(r + kRowsPerTile, B->row0[s + 1]));
+			tsys.push_back((uint32_t)s);
+		}
+		t0[s + 1] = (uint32_t)trow0.size();
+	}
+	if (rc == ALP_SUCCESS && B->row0[nsys] >= 0xFFFFFFFFull) rc = ALP_OVERFLW;
+	if (rc) {
+		delete B;
+		return rc;
+	}
+	B->nrows = B->row0[nsys];
+	B->nnz = nnz;
+	B->ntiles = trow0.size();
+	const size_t ntot = B->nrows * nrhs, ninst = nsys * nrhs;
+	hipStream_t st = stream();
+	do {
+		std::vector<uint32_t> rsys(B->nrows);
+		for (size_t s = 0; s < nsys; ++s)
+			for (uint64_t r = B->row0[s]; r < B->row0[s + 1]; ++r) rsys[r] = (uint32_t)s;
+		if ((rc = bmalloc(&B->off, B->nrows + 1)) || (rc = bmalloc(&B->idx, nnz)) || (rc = bmalloc(&B->val, nnz)) ||
+		    (rc = bmalloc(&B->val32, nnz)) || (rc = bmalloc(&B->row_sys, B->nrows)) ||
+		    (rc = bmalloc(&B->sys, nsys)) || (rc = bmalloc(&B->tile_row0, B->ntiles)) ||
+		    (rc = bmalloc(&B->tile_row1, B->ntiles)) || (rc = bmalloc(&B->tile_sys, B->ntiles)) ||
+		    (rc = bmalloc(&B->tile0, nsys + 1)) || (rc = bmalloc(&B->part, B->ntiles * nrhs)) ||
+		    (rc = bmalloc(&B->inst, ninst)) || (rc = bmalloc(&B->nactive, 1)) || (rc = bmalloc(&B->rnorm, ninst)) ||
+		    (rc = bmalloc(&B->bnorm, ninst)) || (rc = bmalloc(&B->outer_active, ninst)) ||
+		    (rc = bmalloc(&B->R, ntot)) || (rc = bmalloc(&B->P, ntot)) || (rc = bmalloc(&B->AP, ntot)) ||
+		    (rc = bmalloc(&B->R32, ntot)) || (rc = bmalloc(&B->P32, ntot)) || (rc = bmalloc(&B->AP32, ntot)) ||
+		    (rc = bmalloc(&B->X32, ntot)))
+			break;
+		if (hipMemcpy(B->row_sys, rsys.data(), 4 * B->nrows, hipMemcpyHostToDevice) != hipSuccess ||
+		    hipMemcpy(B->sys, desc.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice) != hipSuccess ||
+		    hipMemcpy(B->tile_row0, trow0.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+		    hipMemcpy(B->tile_row1, trow1.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+		    hipMemcpy(B->tile_sys, tsys.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+		    hipMemcpy(B->tile0, t0.data(), 4 * (nsys + 1), hipMemcpyHostToDevice) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		// offsets: per-row counts then an exclusive scan (host: the counts are cheap to recompute)
+		uint64_t *counts = nullptr;
+		if ((rc = bmalloc(&counts, B->nrows + 1))) break;
+		hipMemsetAsync(counts + B->nrows, 0, 8, st);
+		hipLaunchKernelGGL(k_lap_counts, dim3(fill_blocks(8)), dim3(kBlock), 0, st, nsys, (const SysDesc *)B->sys,
+		                   (const uint32_t *)B->row_sys, B->nrows, counts);
+		std::vector<uint64_t> hc(B->nrows + 1);
+		if (hipMemcpy(hc.data(), counts, 8 * (B->nrows + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+			hipFree(counts);
+			rc = ALP_PANIC;
+			break;
+		}
+		hipFree(counts);
+		uint64_t acc = 0;
+		for (size_t r = 0; r <= B->nrows; ++r) {
+			const uint64_t c = hc[r];
+			hc[r] = acc;
+			acc += c;
+		}
+		if (acc != nnz || hipMemcpy(B->off, hc.data(), 8 * (B->nrows + 1), hipMemcpyHostToDevice) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		hipLaunchKernelGGL(k_lap_fill, dim3(fill_blocks(8)), dim3(kBlock), 0, st, B->nrows, (const SysDesc *)B->sys,
+		                   (const uint32_t *)B->row_sys, (const uint64_t *)B->off, B->idx, B->val);
+		if ((rc = alp_hip_convert_f64_to_f32(nnz, B->val, B->val32))) break;
+		if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) rc = ALP_PANIC;
+	} while (0);
+	if (rc) {
+		alp_hip_batch_destroy(B);
+		return rc;
+	}
+	B->W = (int)std::min<size_t>(nrhs, 8);
+	*out = B;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_batch_info(alp_hip_batch_t B, size_t *nrows, size_t *nnz, uint64_t *h_row0) {
+	if (!B) return ALP_ILLEGAL;
+	if (nrows) *nrows = B->nrows;
+	if (nnz) *nnz = B->nnz;
+	if (h_row0) std::memcpy(h_row0, B->row0.data(), sizeof(uint64_t) * (B->nsys + 1));
+	return ALP_SUCCESS;
+}
+
+// Y = A X for all systems (row-major blocks, FP64)
+ALP_EXPORT alp_rc_t alp_hip_batch_spmm(alp_hip_batch_t B, const double *d_X, double *d_Y) {
+	ALP_TRY(ensure_init());
+	if (!B || !d_X || !d_Y) return ALP_ILLEGAL;
+	launch_spmm<double, false, false>(B, B->W, B->val, d_X, nullptr, d_Y, stream());
+	ALP_LAUNCH_CHECK();
+	ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	return ALP_SUCCESS;
+}
+
+// The RHS tile width: lanes per row of the SpMM, each lane owning nrhs / W
+// right-hand sides; the FP32 SpMM is timed for every width and the fastest kept.
+ALP_EXPORT alp_rc_t alp_hip_batch_autotune(alp_hip_batch_t B, int *width) {
+	ALP_TRY(ensure_init());
+	if (!B) return ALP_ILLEGAL;
+	hipStream_t s = stream();
+	hipEvent_t e0, e1;
+	ALP_HIP_TRY(hipEventCreate(&e0));
+	ALP_HIP_TRY(hipEventCreate(&e1));
+	float best = 1e30f;
+	int pick = B->W;
+	for (int W : {1, 2, 4, 8, 16}) {
+		if ((size_t)W > B->nrhs) continue;
+		for (int i = 0; i < 2; ++i) launch_spmm<float, true, false>(B, W, B->val32, B->P32, nullptr, B->AP32, s);
+		ALP_HIP_TRY(hipEventRecord(e0, s));
+		for (int i = 0; i < 5; ++i) launch_spmm<float, true, false>(B, W, B->val32, B->P32, nullptr, B->AP32, s);
+		ALP_HIP_TRY(hipEventRecord(e1, s));
+		ALP_HIP_TRY(hipEventSynchronize(e1));
+		float ms = 0;
+		ALP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+		if (ms < best) {
+			best = ms;
+			pick = W;
+		}
+	}
+	hipEventDestroy(e0);
+	hipEventDestroy(e1);
+	B->W = pick;
+	if (width) *width = pick;
+	return ALP_SUCCESS;
+}
+
+// Solve A_s X_s = B_s for every system s and right-hand side, from X = 0, to
+// ||b - A x|| <= tol ||b|| per right-hand side. Blocks are row-major (nrows x nrhs).
+// h_iters / h_relres (nsys * nrhs entries, may be NULL) get each instance's CG
+// iterations (summed over refinements) and final relative residual (FP64, true).
+ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, double *d_X, double tol,
+                                        size_t max_iter, int precision, size_t *h_iters, double *h_relres,
+                                        size_t *outer) {
+	ALP_TRY(ensure_init());
+	if (!B || !d_B || !d_X || !(tol > 0) || max_iter == 0) return ALP_ILLEGAL;
+	if (precision != 0 && precision != 1) return ALP_ILLEGAL;
+	hipStream_t s = stream();
+	const int nr = (int)B->nrhs;
+	const int ninst = (int)(B->nsys * B->nrhs);
+	const size_t ntot = B->nrows * B->nrhs;
+	const size_t g = B->ntiles < (size_t)fill_blocks(8) ? B->ntiles : fill_blocks(8);
+	std::vector<unsigned long long> its(ninst, 0);
+	ALP_HIP_TRY(hipMemsetAsync(d_X, 0, sizeof(double) * ntot, s));
+	// ||b|| per instance
+	hipLaunchKernelGGL(k_b_start_part<double>, dim3(g), dim3(kBlock), 0, s, B->ntiles, (const uint32_t *)B->tile_row0,
+	                   (const uint32_t *)B->tile_row1, nr, d_B, B->part);
+	hipLaunchKernelGGL(k_b_bnorm, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
+	                   (const uint32_t *)B->tile0, B->bnorm, B->outer_active);
+	size_t nouter = 0;
+	std::vector<BatchInst> hin(ninst);
+	if (precision == 0) {
+		// FP64 CG on all instances: R = B, P = B (X = 0)
+		hipLaunchKernelGGL(k_b_copy<double>, dim3(fill_blocks(8)), dim3(kBlock), 0, s, ntot, d_B, B->R);
+		hipLaunchKernelGGL(k_b_copy<double>, dim3(fill_blocks(8)), dim3(kBlock), 0, s, ntot, d_B, B->P);
+		hipLaunchKernelGGL(k_b_start_part<double>, dim3(g), dim3(kBlock), 0, s, B->ntiles,
+		                   (const uint32_t *)B->tile_row0, (const uint32_t *)B->tile_row1, nr, (const double *)B->R,
+		                   B->part);
+		ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
+		hipLaunchKernelGGL(k_b_start, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
+		                   (const uint32_t *)B->tile0, B->inst, (const int *)B->outer_active, tol, B->nactive);
+		ALP_LAUNCH_CHECK();
+		size_t it = 0;
+		ALP_TRY(batch_iterate<double>(B, B->val, d_X, B->R, B->P, B->AP, max_iter, &it));
+		ALP_HIP_TRY(hipMemcpy(hin.data(), B->inst, sizeof(BatchInst) * ninst, hipMemcpyDeviceToHost));
+		for (int i = 0; i < ninst; ++i) its[i] = hin[i].iter;
+	} else {
+		const double inner_tol = 1e-4;
+		for (int o = 0; o < 100; ++o) {
+			// FP64 residual R = B - A X and its norms; instances at tol leave
+			launch_spmm<double, false, true>(B, B->W, B->val, d_X, d_B, B->R, s);
+			ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
+			hipLaunchKernelGGL(k_b_outer, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
+			                   (const uint32_t *)B->tile0, B->rnorm, (const double *)B->bnorm, B->outer_active,
+			                   tol, B->nactive);
+			unsigned act = 0;
+			ALP_HIP_TRY(hipMemcpyAsync(&act, B->nactive, sizeof(act), hipMemcpyDeviceToHost, s));
+			ALP_HIP_TRY(hipStreamSynchronize(s));
+			if (act == 0) break;
+			++nouter;
+			hipLaunchKernelGGL(k_b_down, dim3(fill_blocks(8)), dim3(kBlock), 0, s, ntot, nr, (const double *)B->R,
+			                   (const double *)B->rnorm, (const uint32_t *)B->row_sys, B->R32, B->X32, B->P32);
+			hipLaunchKernelGGL(k_b_start_part<float>, dim3(g), dim3(kBlock), 0, s, B->ntiles,
+			                   (const uint32_t *)B->tile_row0, (const uint32_t *)B->tile_row1, nr,
+			                   (const float *)B->R32, B->part);
+			ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
+			hipLaunchKernelGGL(k_b_start, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
+			                   (const uint32_t *)B->tile0, B->inst, (const int *)B->outer_active, inner_tol,
+			                   B->nactive);
+			ALP_LAUNCH_CHECK();
+			size_t it = 0;
+			ALP_TRY(batch_iterate<float>(B, B->val32, B->X32, B->R32, B->P32, B->AP32, max_iter, &it));
+			ALP_HIP_TRY(hipMemcpy(hin.data(), B->inst, sizeof(BatchInst) * ninst, hipMemcpyDeviceToHost));
+			for (int i = 0; i < ninst; ++i) its[i] += hin[i].iter;
+			hipLaunchKernelGGL(k_b_up, dim3(fill_blocks(8)), dim3(kBlock), 0, s, ntot, nr, d_X, (const float *)B->X32,
+			                   (const double *)B->rnorm, (const int *)B->outer_active, (const uint32_t *)B->row_sys);
+			ALP_LAUNCH_CHECK();
+		}
+	}
+	// final true residuals in FP64
+	launch_spmm<double, false, true>(B, B->W, B->val, d_X, d_B, B->R, s);
+	ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
+	hipLaunchKernelGGL(k_b_outer, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
+	                   (const uint32_t *)B->tile0, B->rnorm, (const double *)B->bnorm, B->outer_active, tol,
+	                   B->nactive);
+	ALP_LAUNCH_CHECK();
+	std::vector<double> rn(ninst), bn(ninst);
+	unsigned act = 0;
+	ALP_HIP_TRY(hipMemcpy(rn.data(), B->rnorm, 8 * ninst, hipMemcpyDeviceToHost));
+	ALP_HIP_TRY(hipMemcpy(bn.data(), B->bnorm, 8 * ninst, hipMemcpyDeviceToHost));
+	ALP_HIP_TRY(hipMemcpy(&act, B->nactive, sizeof(act), hipMemcpyDeviceToHost));
+	for (int i = 0; i < ninst; ++i) {
+		if (h_iters) h_iters[i] = its[i];
+		if (h_relres) h_relres[i] = bn[i] > 0 ? rn[i] / bn[i] : 0.0;
+	}
+	if (outer) *outer = nouter;
+	return act == 0 ? ALP_SUCCESS : ALP_FAILED;
+}

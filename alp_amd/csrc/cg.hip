@@ -23,6 +23,7 @@
 #include "spmv_device.hpp"
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace alp {
@@ -40,7 +41,8 @@ struct CgState {
 	unsigned long long max_iter;
 	int done;
 	int converged;
-	int pad2[2];
+	int precond;       // preconditioned: sigma and ratio come from r.z (k_pcg_rz)
+	int pad2;
 };
 
 template <typename T>
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a) {
 			if (conv || st->iter >= st->max_iter) {
 				st->done = 1;
 				st->converged = conv ? 1 : 0;
-			} else {
+			} else if (!st->precond) {
 				st->ratio = rr / st->sigma;
 				st->sigma = rr;
 			}
@@ -197,8 +199,40 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs<T> a, const double *b
 			st->converged = 0;
 			st->ratio = 0.0;
 			st->alpha = 0.0;
+			st->precond = 0;
 		}
 	}
+}
+
+// Preconditioned CG (conjugate_gradient.hpp:342-352,441-471): after z = M^-1 r,
+// rz = r.z; on the first call sigma = rz and the direction restarts (u = z),
+// afterwards ratio = rz / sigma and sigma = rz.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_rz(CgArgs<T> a, const T *z, int first) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	if (a.st->done) return;
+	double part = 0.0;
+	grid_for_each<4>(a.n, [&](size_t i) { part += (double)a.r[i] * (double)z[i]; });
+	part = block_sum(part, lds);
+	if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
+	if (last_block(a.ticket, &is_last)) {
+		const double rz = block_sum_partials(a.partials, gridDim.x, lds);
+		if (threadIdx.x == 0) {
+			CgState *st = a.st;
+			st->precond = 1;
+			st->ratio = first ? 0.0 : rz / st->sigma;
+			st->sigma = rz;
+		}
+	}
+}
+
+// u = z + ratio u
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(CgArgs<T> a, const T *z) {
+	if (a.st->done) return;
+	const T ratio = (T)a.st->ratio;
+	grid_for_each<4>(a.n, [&](size_t i) { a.u[i] = ratio * a.u[i] + z[i]; });
 }
 
 // ---- iterative refinement helpers (FP64 outer loop) -----------------------------
@@ -274,11 +308,12 @@ template <typename T>
 struct CgRunner {
 	const Storage &S;
 	CgArgs<T> a;
+	int vt;  // the matrix's value type: FP64 (FP32 iterations use its FP32 copy) or FP32
 	int G, RR;
 	bool fused;
 	unsigned gspmv, gvec;
 
-	CgRunner(const Storage &S_, CgArgs<T> a_) : S(S_), a(a_) {
+	CgRunner(const Storage &S_, CgArgs<T> a_, int vt_ = ALP_VAL_F64) : S(S_), a(a_), vt(vt_) {
 		G = S.lanes ? S.lanes : default_lanes(S);
 		RR = S.rows ? S.rows : 1;
 		fused = S.kernel != 2 && G >= 2;
@@ -291,7 +326,8 @@ struct CgRunner {
 		gvec = vec_grid(a.n);
 	}
 
-	alp_rc_t iteration(hipStream_t s) {
+	// the part of an iteration up to the stopping test: temp = A u, alpha, x, r, r.r
+	alp_rc_t step(hipStream_t s) {
 		if (fused) {
 #define ALP_CG_CASE(GG)                                                                          \
 	case GG:                                                                                     \
@@ -313,13 +349,19 @@ struct CgRunner {
 			}
 #undef ALP_CG_CASE
 		} else {
-			// the matrix is FP64; the FP32 semiring selects its FP32 value copy
-			ALP_TRY(spmv(S, ALP_VAL_F64,
+			// for an FP64 matrix the FP32 semiring selects its FP32 value copy
+			ALP_TRY(spmv(S, vt,
 			             std::is_same<T, float>::value ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64,
 			             ALP_DESC_OVERWRITE, a.temp, a.u, nullptr, s));
 			hipLaunchKernelGGL(k_cg_dot_alpha<T>, dim3(gvec), dim3(kBlock), 0, s, a);
 		}
 		hipLaunchKernelGGL(k_cg_update<T>, dim3(gvec), dim3(kBlock), 0, s, a);
+		ALP_LAUNCH_CHECK();
+		return ALP_SUCCESS;
+	}
+
+	alp_rc_t iteration(hipStream_t s) {
+		ALP_TRY(step(s));
 		hipLaunchKernelGGL(k_cg_dir<T>, dim3(gvec), dim3(kBlock), 0, s, a);
 		ALP_LAUNCH_CHECK();
 		return ALP_SUCCESS;
@@ -375,7 +417,7 @@ struct DeviceBuffers {
 	}
 };
 
-static alp_rc_t cg_fp64(alp_hip_matrix_t A, const double *b, double *x, size_t max_iter,
+static alp_rc_t cg_fp64(alp_hip_matrix_t A, Storage &S, const double *b, double *x, size_t max_iter,
                         double tol, size_t *iters, double *residual) {
 	const size_t n = A->nrows;
 	hipStream_t s = stream();
@@ -386,14 +428,14 @@ static alp_rc_t cg_fp64(alp_hip_matrix_t A, const double *b, double *x, size_t m
 	ALP_TRY(buf.get(&u, n));
 	ALP_TRY(buf.get(&temp, n));
 	ALP_TRY(buf.get(&st, 1));
-	CgArgs<double> a{n, A->crs.off, A->crs.idx, (const double *)A->crs.val, x, r, u, temp, b,
+	CgArgs<double> a{n, S.off, S.idx, (const double *)S.val, x, r, u, temp, b,
 	                 ctx().d_partials, ctx().d_tickets, st};
 	// temp = A x; r = b - temp; u = r; sigma = r.r; tol *= sqrt(b.b) (:332-378)
-	ALP_TRY(spmv(A->crs, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, temp, x, nullptr, s));
+	ALP_TRY(spmv(S, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, temp, x, nullptr, s));
 	hipLaunchKernelGGL(k_cg_init<double>, dim3(vec_grid(n)), dim3(kBlock), 0, s, a,
 	                   (const double *)b, (const double *)nullptr, tol, max_iter);
 	ALP_LAUNCH_CHECK();
-	CgRunner<double> runner(A->crs, a);
+	CgRunner<double> runner(S, a);
 	CgState h;
 	ALP_TRY(runner.run(s, &h));
 	*iters = h.iter;
@@ -401,12 +443,12 @@ static alp_rc_t cg_fp64(alp_hip_matrix_t A, const double *b, double *x, size_t m
 	return h.converged ? ALP_SUCCESS : ALP_FAILED;
 }
 
-static alp_rc_t cg_mixed(alp_hip_matrix_t A, const double *b, double *x, size_t max_iter,
+static alp_rc_t cg_mixed(alp_hip_matrix_t A, Storage &S, const double *b, double *x, size_t max_iter,
                          double tol, size_t *iters, double *residual) {
 	const size_t n = A->nrows;
-	if (!A->crs.val32) {
-		ALP_HIP_TRY(hipMalloc(&A->crs.val32, sizeof(float) * (A->nnz ? A->nnz : 1)));
-		ALP_TRY(alp_hip_convert_f64_to_f32(A->nnz, (const double *)A->crs.val, A->crs.val32));
+	if (!S.val32) {
+		ALP_HIP_TRY(hipMalloc(&S.val32, sizeof(float) * (A->nnz ? A->nnz : 1)));
+		ALP_TRY(alp_hip_convert_f64_to_f32(A->nnz, (const double *)S.val, S.val32));
 	}
 	hipStream_t s = stream();
 	DeviceBuffers buf;
@@ -423,15 +465,15 @@ static alp_rc_t cg_mixed(alp_hip_matrix_t A, const double *b, double *x, size_t 
 	ALP_TRY(buf.get(&b32, n));
 	ALP_TRY(buf.get(&st, 1));
 	ALP_TRY(buf.get(&ir, 1));
-	CgArgs<float> a{n, A->crs.off, A->crs.idx, A->crs.val32, x32, r32, u32, temp32, b32,
+	CgArgs<float> a{n, S.off, S.idx, S.val32, x32, r32, u32, temp32, b32,
 	                ctx().d_partials, ctx().d_tickets, st};
-	CgRunner<float> runner(A->crs, a);
+	CgRunner<float> runner(S, a);
 	const double inner_tol = 1e-5;
 	size_t total = 0;
 	IrState hir{};
 	CgState h{};
 	for (int outer = 0; outer < 100; ++outer) {
-		ALP_TRY(spmv(A->crs, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, t64, x, nullptr, s));
+		ALP_TRY(spmv(S, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, t64, x, nullptr, s));
 		hipLaunchKernelGGL(k_ir_residual, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, b, t64, r64,
 		                   ctx().d_partials, ctx().d_tickets, ir, tol, outer == 0 ? 1 : 0);
 		ALP_LAUNCH_CHECK();
@@ -456,12 +498,87 @@ static alp_rc_t cg_mixed(alp_hip_matrix_t A, const double *b, double *x, size_t 
 	return hir.done ? ALP_SUCCESS : ALP_FAILED;
 }
 
+// CG in the arithmetic of T on A's own values: the transition solver's path
+// (src/transition/solver.cpp:248-273 -> conjugate_gradient.hpp:191-484). With a
+// preconditioner, z = M^-1 r is a host callback on host copies of r and z
+// (solver.cpp:118-135): each iteration stops after the stopping test, copies r
+// out, calls the callback and copies z back; z keeps the callback's previous
+// output between calls, as the reference's workspace vector does.
+template <typename T>
+alp_rc_t cg_native(alp_hip_matrix_s *A, const T *b, T *x, size_t max_iter, double tol, HostPrecond<T> pc,
+                   size_t *iters, double *residual) {
+	const size_t n = A->nrows;
+	const int vt = std::is_same<T, float>::value ? ALP_VAL_F32 : ALP_VAL_F64;
+	if (A->value_type != vt) return ALP_MISMATCH;
+	Storage &S = A->crs;
+	hipStream_t s = stream();
+	DeviceBuffers buf;
+	T *r, *u, *temp, *z = nullptr;
+	CgState *st;
+	ALP_TRY(buf.get(&r, n));
+	ALP_TRY(buf.get(&u, n));
+	ALP_TRY(buf.get(&temp, n));
+	ALP_TRY(buf.get(&st, 1));
+	CgArgs<T> a{n, S.off, S.idx, (const T *)S.val, x, r, u, temp, b, ctx().d_partials, ctx().d_tickets, st};
+	const alp_semiring_t ring = vt == ALP_VAL_F32 ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64;
+	ALP_TRY(spmv(S, vt, ring, ALP_DESC_OVERWRITE, temp, x, nullptr, s));
+	const double *bb64 = std::is_same<T, double>::value ? (const double *)b : nullptr;
+	hipLaunchKernelGGL(k_cg_init<T>, dim3(vec_grid(n)), dim3(kBlock), 0, s, a, bb64,
+	                   std::is_same<T, double>::value ? (const T *)nullptr : b, tol, max_iter);
+	ALP_LAUNCH_CHECK();
+	CgRunner<T> runner(S, a, vt);
+	CgState h{};
+	if (!pc.fn) {
+		ALP_TRY(runner.run(s, &h));
+	} else {
+		ALP_TRY(buf.get(&z, n));
+		struct Pinned {
+			T *p = nullptr;
+			~Pinned() {
+				if (p) hipHostFree(p);
+			}
+		} hr, hz;
+		ALP_HIP_TRY(hipHostMalloc(&hr.p, sizeof(T) * n, hipHostMallocDefault));
+		ALP_HIP_TRY(hipHostMalloc(&hz.p, sizeof(T) * n, hipHostMallocDefault));
+		std::memset(hz.p, 0, sizeof(T) * n);
+		const unsigned g = vec_grid(n);
+		for (size_t it = 0;; ++it) {
+			if (it > 0) {
+				ALP_TRY(runner.step(s));
+				ALP_HIP_TRY(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+				ALP_HIP_TRY(hipStreamSynchronize(s));
+				if (h.done) break;
+			}
+			ALP_HIP_TRY(hipMemcpyAsync(hr.p, r, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+			ALP_HIP_TRY(hipStreamSynchronize(s));
+			if (pc.fn(hz.p, hr.p, pc.data) != 0) {
+				fprintf(stderr, "alp_hip: user preconditioner returned a non-zero error code\n");
+				*iters = it;
+				*residual = it > 0 ? h.rr : std::numeric_limits<double>::infinity();
+				return ALP_FAILED;
+			}
+			ALP_HIP_TRY(hipMemcpyAsync(z, hz.p, sizeof(T) * n, hipMemcpyHostToDevice, s));
+			hipLaunchKernelGGL(k_pcg_rz<T>, dim3(g), dim3(kBlock), 0, s, a, (const T *)z, it == 0 ? 1 : 0);
+			hipLaunchKernelGGL(k_pcg_dir<T>, dim3(g), dim3(kBlock), 0, s, a, (const T *)z);
+			ALP_LAUNCH_CHECK();
+		}
+	}
+	*iters = h.iter;
+	*residual = h.rr;
+	return h.converged ? ALP_SUCCESS : ALP_FAILED;
+}
+
+template alp_rc_t cg_native<float>(alp_hip_matrix_s *, const float *, float *, size_t, double, HostPrecond<float>,
+                                   size_t *, double *);
+template alp_rc_t cg_native<double>(alp_hip_matrix_s *, const double *, double *, size_t, double,
+                                    HostPrecond<double>, size_t *, double *);
+
 }  // namespace alp
 
 using namespace alp;
 
-ALP_EXPORT alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *b, double *x, size_t max_iter,
-                               double tol, int precision, size_t *iters, double *residual) {
+static alp_rc_t cg_entry(alp_hip_matrix_t A, bool transpose, const double *b, double *x, size_t max_iter,
+                         double tol, int precision, size_t *iters, double *residual) {
 	ALP_TRY(ensure_init());
 	if (!A || !b || !x || !iters || !residual) return ALP_ILLEGAL;
 	if (A->nrows != A->ncols) return ALP_ILLEGAL;
@@ -472,7 +589,20 @@ ALP_EXPORT alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *b, double *x, s
 	*iters = 0;
 	*residual = std::numeric_limits<double>::infinity();
 	if (A->nrows == 0) return ALP_SUCCESS;
-	if (precision == 0) return cg_fp64(A, b, x, max_iter, tol, iters, residual);
-	if (precision == 1) return cg_mixed(A, b, x, max_iter, tol, iters, residual);
+	if (transpose && !A->ccs.off) ALP_TRY(build_ccs_from_crs(A));
+	Storage &S = transpose ? A->ccs : A->crs;
+	if (precision == 0) return cg_fp64(A, S, b, x, max_iter, tol, iters, residual);
+	if (precision == 1) return cg_mixed(A, S, b, x, max_iter, tol, iters, residual);
 	return ALP_ILLEGAL;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *b, double *x, size_t max_iter,
+                               double tol, int precision, size_t *iters, double *residual) {
+	return cg_entry(A, false, b, x, max_iter, tol, precision, iters, residual);
+}
+
+// CG on A^T (the CCS gathered as rows): the adjoint solve A^T lambda = g
+ALP_EXPORT alp_rc_t alp_hip_cg_transposed(alp_hip_matrix_t A, const double *b, double *x, size_t max_iter,
+                                          double tol, int precision, size_t *iters, double *residual) {
+	return cg_entry(A, true, b, x, max_iter, tol, precision, iters, residual);
 }

@@ -1,0 +1,571 @@
+// Graph workloads of the hip backend on pattern matrices (Matrix<void>):
+//   R-MAT edge generation and the symmetrise / deduplicate / drop-self-loop
+//   build, breadth-first search levels over the lor-land semiring, ALP's
+//   simple_pagerank (algorithms/simple_pagerank.hpp:146-533) and the k-hop
+//   neighbourhood of knn (algorithms/knn.hpp:82-140).
+//
+// BFS. ALP has no BFS driver; its building block is the masked lor-land vxm,
+// for which vxm_generic (reference/blas2.hpp:1108-1132, 1311-1334) picks the CRS
+// scatter (push from the frontier) when the input is sparse and the gather
+// (pull into unvisited rows, with the lor early exit of has_immutable_nonzeroes,
+// :354-360) when it is dense. The device BFS makes the same choice per level with
+// the frontier's edge count: push expands frontier edges in an edge-balanced
+// kernel (prefix sums over frontier degrees + binary search), claiming
+// unvisited vertices with an atomic compare-and-swap on the level array; pull
+// scans each unvisited vertex's row and stops at its first frontier neighbour.
+// Levels are integers and do not depend on the visiting order, so they are
+// bit-exact with a textbook queue BFS.
+#include "spmv_device.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <vector>
+
+namespace alp {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+	x += 0x9E3779B97F4A7C15ull;
+	x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+	x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+	return x ^ (x >> 31);
+}
+
+// Counter-based R-MAT (Graph500 recursive quadrant choice): edge e at level b
+// draws u = mix64(seed ^ mix64(64 e + b)) / 2^64 (53 bits) and picks the quadrant
+// by thresholds a, a+b, a+b+c; each edge is independent of the others.
+__global__ __launch_bounds__(kBlock) void k_rmat(unsigned scale, size_t nedges, double a, double ab,
+                                                 double abc, uint64_t seed, uint32_t *src, uint32_t *dst) {
+	grid_for_each<1>(nedges, [&](size_t e) {
+		uint32_t s = 0, d = 0;
+		for (unsigned lvl = 0; lvl < scale; ++lvl) {
+			const uint64_t h = mix64(seed ^ mix64((uint64_t)e * 64u + lvl));
+			const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+			const uint32_t bit = 1u << (scale - 1 - lvl);
+			if (u < a) {
+			} else if (u < ab) {
+				d |= bit;
+			} else if (u < abc) {
+				s |= bit;
+			} else {
+				s |= bit;
+				d |= bit;
+			}
+		}
+		src[e] = s;
+		dst[e] = d;
+	});
+}
+
+// keys (major << 32 | minor); self loops become ~0 and sort to the end
+__global__ __launch_bounds__(kBlock) void k_edge_keys(size_t nedges, const uint32_t *src, const uint32_t *dst,
+                                                      int symmetric, int drop_loops, int transpose,
+                                                      uint64_t *keys) {
+	grid_for_each<4>(nedges, [&](size_t e) {
+		const uint64_t s = src[e], d = dst[e];
+		const bool loop = drop_loops && s == d;
+		const uint64_t k0 = transpose ? (d << 32 | s) : (s << 32 | d);
+		keys[e] = loop ? ~0ull : k0;
+		if (symmetric) keys[nedges + e] = loop ? ~0ull : (transpose ? (s << 32 | d) : (d << 32 | s));
+	});
+}
+
+__global__ __launch_bounds__(kBlock) void k_keys_minor(size_t nnz, const uint64_t *keys, uint32_t *idx) {
+	grid_for_each<4>(nnz, [&](size_t k) { idx[k] = (uint32_t)(keys[k] & 0xffffffffu); });
+}
+
+// off[r] = first position whose key's major part is >= r (keys sorted ascending)
+__global__ __launch_bounds__(kBlock) void k_offsets_from_sorted(size_t nmajor, const uint64_t *keys, size_t nnz,
+                                                                uint64_t *off) {
+	grid_for_each<1>(nmajor + 1, [&](size_t r) {
+		const uint64_t target = (uint64_t)r << 32;
+		size_t lo = 0, hi = nnz;
+		while (lo < hi) {
+			const size_t mid = (lo + hi) / 2;
+			if (keys[mid] < target) lo = mid + 1;
+			else hi = mid;
+		}
+		off[r] = lo;
+	});
+}
+
+// ---- BFS ---------------------------------------------------------------------------
+
+struct BfsCounters {
+	unsigned long long next_vertices;  // vertices discovered in this level
+	unsigned long long next_edges;     // sum of their degrees
+};
+
+__global__ __launch_bounds__(kBlock) void k_bfs_init(size_t n, int32_t *level, uint32_t source) {
+	grid_for_each<4>(n, [&](size_t i) { level[i] = i == source ? 0 : -1; });
+}
+
+__global__ __launch_bounds__(kBlock) void k_frontier_degrees(size_t q, const uint32_t *queue, const uint64_t *off,
+                                                             uint64_t *deg) {
+	grid_for_each<4>(q, [&](size_t i) {
+		const uint32_t v = queue[i];
+		deg[i] = off[v + 1] - off[v];
+	});
+}
+
+// push: one thread per frontier edge; vertex found by binary search in the
+// exclusive scan of frontier degrees
+__global__ __launch_bounds__(kBlock) void k_bfs_push(size_t q, size_t fedges, const uint32_t *queue,
+                                                     const uint64_t *scan, const uint64_t *off,
+                                                     const uint32_t *idx, int32_t *level, int32_t next,
+                                                     uint32_t *nqueue, BfsCounters *cnt) {
+	grid_for_each<1>(fedges, [&](size_t e) {
+		size_t lo = 0, hi = q;  // largest i with scan[i] <= e
+		while (hi - lo > 1) {
+			const size_t mid = (lo + hi) / 2;
+			if (scan[mid] <= e) lo = mid;
+			else hi = mid;
+		}
+		const uint32_t v = queue[lo];
+		const uint32_t w = idx[off[v] + (e - scan[lo])];
+		if (level[w] < 0 && atomicCAS(level + w, -1, next) == -1) {
+			const unsigned long long pos = atomicAdd(&cnt->next_vertices, 1ull);
+			nqueue[pos] = w;
+			atomicAdd(&cnt->next_edges, (unsigned long long)(off[w + 1] - off[w]));
+		}
+	});
+}
+
+// pull: every unvisited vertex scans its row for a neighbour in the frontier
+// (level == cur) and stops at the first one (lor early exit)
+__global__ __launch_bounds__(kBlock) void k_bfs_pull(size_t n, const uint64_t *off, const uint32_t *idx,
+                                                     int32_t *level, int32_t cur, BfsCounters *cnt) {
+	unsigned long long nv = 0, ne = 0;
+	grid_for_each<1>(n, [&](size_t w) {
+		if (level[w] >= 0) return;
+		const uint64_t b = off[w], e = off[w + 1];
+		for (uint64_t k = b; k < e; ++k) {
+			if (level[idx[k]] == cur) {
+				level[w] = cur + 1;
+				nv += 1;
+				ne += e - b;
+				break;
+			}
+		}
+	});
+	// block totals, then one atomic per block (integer sums: order-free)
+	__shared__ unsigned long long s_nv[kBlock / kWave], s_ne[kBlock / kWave];
+	for (int s = kWave / 2; s > 0; s >>= 1) {
+		nv += __shfl_xor(nv, s, kWave);
+		ne += __shfl_xor(ne, s, kWave);
+	}
+	if (threadIdx.x % kWave == 0) {
+		s_nv[threadIdx.x / kWave] = nv;
+		s_ne[threadIdx.x / kWave] = ne;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		unsigned long long a = 0, b = 0;
+		for (int i = 0; i < kBlock / kWave; ++i) {
+			a += s_nv[i];
+			b += s_ne[i];
+		}
+		if (a) atomicAdd(&cnt->next_vertices, a);
+		if (b) atomicAdd(&cnt->next_edges, b);
+	}
+}
+
+// queue of the vertices at level `lev` (after a pull step, to switch back to push)
+__global__ __launch_bounds__(kBlock) void k_bfs_collect(size_t n, const int32_t *level, int32_t lev,
+                                                        uint32_t *queue, unsigned long long *count) {
+	grid_for_each<4>(n, [&](size_t i) {
+		if (level[i] == lev) queue[atomicAdd(count, 1ull)] = (uint32_t)i;
+	});
+}
+
+// ---- PageRank (simple_pagerank.hpp:146-533) -------------------------------------------
+
+struct PrState {
+	double dangling, residual, alpha, conv;
+	unsigned long long iter, max_iter;
+	int done, converged, pad[2];
+};
+
+// row_sum[i] = alpha / out-degree (0 for dangling rows), pr = 1/n (:206-297)
+__global__ __launch_bounds__(kBlock) void k_pr_init(size_t n, const uint64_t *off, double alpha, double *rs,
+                                                    double *pr) {
+	grid_for_each<4>(n, [&](size_t i) {
+		const double deg = (double)(off[i + 1] - off[i]);
+		rs[i] = deg > 0.0 ? alpha / deg : 0.0;
+		pr[i] = 1.0 / (double)n;
+	});
+}
+
+// pr_next = pr * row_sum (0 where dangling) and the dangling mass (:329-345, :385)
+__global__ __launch_bounds__(kBlock) void k_pr_scale(size_t n, const double *pr, const double *rs, double *pn,
+                                                     double *partials, unsigned *ticket, PrState *st) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	if (st->done) return;
+	double dang = 0.0;
+	grid_for_each<4>(n, [&](size_t i) {
+		const double r = rs[i];
+		if (r == 0.0) {
+			dang += pr[i];
+			pn[i] = 0.0;
+		} else {
+			pn[i] = pr[i] * r;
+		}
+	});
+	dang = block_sum(dang, lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = dang;
+	if (last_block(ticket, &is_last)) {
+		const double d = block_sum_partials(partials, gridDim.x, lds);
+		if (threadIdx.x == 0) st->dangling = (st->alpha * d + 1.0 - st->alpha) / (double)n;
+	}
+}
+
+// pr = pr_nextnext + dangling and the L1 residual (:441-455, :490)
+__global__ __launch_bounds__(kBlock) void k_pr_update(size_t n, double *pr, const double *pnn, double *partials,
+                                                      unsigned *ticket, PrState *st) {
+	__shared__ double lds[kBlock / kWave];
+	__shared__ bool is_last;
+	if (st->done) return;
+	const double dang = st->dangling;
+	double res = 0.0;
+	grid_for_each<4>(n, [&](size_t i) {
+		const double old = pr[i];
+		const double v = pnn[i] + dang;
+		pr[i] = v;
+		res += old > v ? old - v : v - old;
+	});
+	res = block_sum(res, lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = res;
+	if (last_block(ticket, &is_last)) {
+		const double r = block_sum_partials(partials, gridDim.x, lds);
+		if (threadIdx.x == 0) {
+			st->residual = r;
+			st->iter += 1;
+			const bool conv = st->conv != 0.0 && r <= st->conv;
+			if (conv || st->iter >= st->max_iter) {
+				st->done = 1;
+				st->converged = conv ? 1 : 0;
+			}
+		}
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_u8_single(size_t n, uint8_t *x, uint32_t at) {
+	grid_for_each<4>(n, [&](size_t i) { x[i] = i == at ? 1 : 0; });
+}
+
+static unsigned g_grid(size_t n) {
+	size_t g = (n + kBlock - 1) / kBlock;
+	const size_t cap = fill_blocks(8);
+	if (g > cap) g = cap;
+	if (g < 1) g = 1;
+	return (unsigned)g;
+}
+
+static unsigned g_rgrid(size_t n) {
+	size_t g = (n + kBlock - 1) / kBlock;
+	const size_t cap = fill_blocks(4);
+	if (g > cap) g = cap;
+	if (g < 1) g = 1;
+	return (unsigned)g;
+}
+
+template <typename T>
+struct DevBuf {
+	T *p = nullptr;
+	~DevBuf() {
+		if (p) hipFree(p);
+	}
+	alp_rc_t alloc(size_t count) {
+		ALP_HIP_TRY(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+		return ALP_SUCCESS;
+	}
+};
+
+// sorted unique keys (major << 32 | minor) -> compressed storage
+static alp_rc_t keys_to_storage(Storage &S, size_t nmajor, size_t nminor, const uint64_t *keys, size_t nnz) {
+	hipStream_t s = stream();
+	S.nmajor = nmajor;
+	S.nminor = nminor;
+	ALP_HIP_TRY(hipMalloc(&S.off, sizeof(uint64_t) * (nmajor + 1)));
+	ALP_HIP_TRY(hipMalloc(&S.idx, sizeof(uint32_t) * (nnz ? nnz : 1)));
+	if (nnz) hipLaunchKernelGGL(k_keys_minor, dim3(g_grid(nnz)), dim3(kBlock), 0, s, nnz, keys, S.idx);
+	hipLaunchKernelGGL(k_offsets_from_sorted, dim3(g_grid(nmajor + 1)), dim3(kBlock), 0, s, nmajor, keys, nnz, S.off);
+	ALP_LAUNCH_CHECK();
+	ALP_HIP_TRY(hipStreamSynchronize(s));
+	return storage_finalize(S, ALP_VAL_PATTERN, false);
+}
+
+// edges -> sorted unique keys of one orientation
+static alp_rc_t edges_to_keys(size_t nedges, const uint32_t *src, const uint32_t *dst, bool symmetric,
+                              bool drop_loops, bool transpose, DevBuf<uint64_t> &out, size_t *nout) {
+	hipStream_t s = stream();
+	const size_t m = symmetric ? 2 * nedges : nedges;
+	DevBuf<uint64_t> keys, sorted;
+	ALP_TRY(keys.alloc(m));
+	ALP_TRY(sorted.alloc(m));
+	ALP_TRY(out.alloc(m));
+	if (nedges)
+		hipLaunchKernelGGL(k_edge_keys, dim3(g_grid(nedges)), dim3(kBlock), 0, s, nedges, src, dst,
+		                   symmetric ? 1 : 0, drop_loops ? 1 : 0, transpose ? 1 : 0, keys.p);
+	ALP_LAUNCH_CHECK();
+	size_t tb = 0;
+	ALP_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys.p, sorted.p, (int)m, 0, 64, s));
+	DevBuf<char> tmp;
+	ALP_TRY(tmp.alloc(tb));
+	ALP_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, keys.p, sorted.p, (int)m, 0, 64, s));
+	DevBuf<unsigned long long> nsel;
+	ALP_TRY(nsel.alloc(1));
+	tb = 0;
+	ALP_HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tb, sorted.p, out.p, nsel.p, (int)m, s));
+	DevBuf<char> tmp2;
+	ALP_TRY(tmp2.alloc(tb));
+	ALP_HIP_TRY(hipcub::DeviceSelect::Unique(tmp2.p, tb, sorted.p, out.p, nsel.p, (int)m, s));
+	unsigned long long u = 0;
+	ALP_HIP_TRY(hipMemcpyAsync(&u, nsel.p, sizeof(u), hipMemcpyDeviceToHost, s));
+	ALP_HIP_TRY(hipStreamSynchronize(s));
+	// a trailing ~0 key (dropped self loops) is not an edge
+	if (u > 0) {
+		uint64_t last = 0;
+		ALP_HIP_TRY(hipMemcpy(&last, out.p + (u - 1), sizeof(last), hipMemcpyDeviceToHost));
+		if (last == ~0ull) --u;
+	}
+	*nout = u;
+	return ALP_SUCCESS;
+}
+
+}  // namespace alp
+
+using namespace alp;
+
+ALP_EXPORT alp_rc_t alp_hip_rmat_edges(unsigned scale, size_t nedges, double a, double b, double c,
+                                       uint64_t seed, uint32_t *d_src, uint32_t *d_dst) {
+	ALP_TRY(ensure_init());
+	if (scale == 0 || scale > 31 || !(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0)) return ALP_ILLEGAL;
+	if (nedges && (!d_src || !d_dst)) return ALP_ILLEGAL;
+	if (nedges == 0) return ALP_SUCCESS;
+	const double ab = a + b, abc = a + b + c;  // the same roundings as the host generator
+	hipLaunchKernelGGL(k_rmat, dim3(g_grid(nedges)), dim3(kBlock), 0, stream(), scale, nedges, a, ab, abc, seed,
+	                   d_src, d_dst);
+	ALP_LAUNCH_CHECK();
+	ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_graph_from_edges(alp_hip_matrix_t *A, size_t n, size_t nedges, const uint32_t *d_src,
+                                             const uint32_t *d_dst, unsigned flags, size_t *nnz_out) {
+	ALP_TRY(ensure_init());
+	if (!A) return ALP_ILLEGAL;
+	*A = nullptr;
+	if (n == 0 || n > 0xFFFFFFFFull) return ALP_ILLEGAL;
+	if (nedges && (!d_src || !d_dst)) return ALP_ILLEGAL;
+	if (nedges >= (1ull << 30)) return ALP_OVERFLW;
+	const bool sym = (flags & ALP_GRAPH_SYMMETRIZE) != 0;
+	const bool loops = (flags & ALP_GRAPH_DROP_SELF_LOOPS) != 0;
+	auto *M = new (std::nothrow) alp_hip_matrix_s();
+	if (!M) return ALP_OUTOFMEM;
+	M->nrows = M->ncols = n;
+	M->value_type = ALP_VAL_PATTERN;
+	M->flags = ALP_MAT_WITH_CCS;
+	alp_rc_t rc = ALP_SUCCESS;
+	do {
+		DevBuf<uint64_t> keys;
+		size_t nnz = 0;
+		if ((rc = edges_to_keys(nedges, d_src, d_dst, sym, loops, false, keys, &nnz))) break;
+		// every key must name vertices below n
+		uint64_t last = 0;
+		if (nnz && hipMemcpy(&last, keys.p + (nnz - 1), sizeof(last), hipMemcpyDeviceToHost) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if (nnz && (last >> 32) >= n) {
+			rc = ALP_MISMATCH;
+			break;
+		}
+		M->nnz = nnz;
+		if ((rc = keys_to_storage(M->crs, n, n, keys.p, nnz))) break;
+		DevBuf<uint64_t> tkeys;
+		size_t tnnz = 0;
+		if ((rc = edges_to_keys(nedges, d_src, d_dst, sym, loops, true, tkeys, &tnnz))) break;
+		if (tnnz != nnz) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if (tnnz && hipMemcpy(&last, tkeys.p + (tnnz - 1), sizeof(last), hipMemcpyDeviceToHost) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if (tnnz && (last >> 32) >= n) {
+			rc = ALP_MISMATCH;
+			break;
+		}
+		rc = keys_to_storage(M->ccs, n, n, tkeys.p, tnnz);
+	} while (0);
+	if (rc) {
+		alp_hip_matrix_destroy(M);
+		return rc;
+	}
+	if (nnz_out) *nnz_out = M->nnz;
+	*A = M;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_max_degree_vertex(alp_hip_matrix_t A, uint32_t *vertex, size_t *degree) {
+	ALP_TRY(ensure_init());
+	if (!A || !vertex) return ALP_ILLEGAL;
+	if (A->nrows == 0) return ALP_ILLEGAL;
+	std::vector<uint64_t> off(A->nrows + 1);
+	ALP_HIP_TRY(hipMemcpy(off.data(), A->crs.off, sizeof(uint64_t) * off.size(), hipMemcpyDeviceToHost));
+	size_t best = 0;
+	uint64_t bd = 0;
+	for (size_t i = 0; i < A->nrows; ++i) {
+		const uint64_t d = off[i + 1] - off[i];
+		if (d > bd) {  // first vertex of maximum degree
+			bd = d;
+			best = i;
+		}
+	}
+	*vertex = (uint32_t)best;
+	if (degree) *degree = bd;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_bfs(alp_hip_matrix_t A, uint32_t source, int32_t *d_level, size_t *nlevels,
+                                size_t *reached) {
+	ALP_TRY(ensure_init());
+	if (!A || !d_level) return ALP_ILLEGAL;
+	if (A->nrows != A->ncols) return ALP_ILLEGAL;
+	const size_t n = A->nrows;
+	if (source >= n) return ALP_ILLEGAL;
+	hipStream_t s = stream();
+	const Storage &S = A->crs;
+	DevBuf<uint32_t> q0, q1;
+	DevBuf<uint64_t> deg, scan;
+	DevBuf<BfsCounters> cnt;
+	ALP_TRY(q0.alloc(n));
+	ALP_TRY(q1.alloc(n));
+	ALP_TRY(deg.alloc(n + 1));
+	ALP_TRY(scan.alloc(n + 1));
+	ALP_TRY(cnt.alloc(1));
+	size_t scan_tb = 0;
+	ALP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tb, deg.p, scan.p, n + 1, s));
+	DevBuf<char> tmp;
+	ALP_TRY(tmp.alloc(scan_tb));
+	hipLaunchKernelGGL(k_bfs_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, d_level, source);
+	ALP_HIP_TRY(hipStreamSynchronize(s));
+	ALP_HIP_TRY(hipMemcpy(q0.p, &source, sizeof(uint32_t), hipMemcpyHostToDevice));
+	uint64_t src_deg[2];
+	ALP_HIP_TRY(hipMemcpy(src_deg, S.off + source, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+	unsigned long long fq = 1, fe = src_deg[1] - src_deg[0];
+	unsigned long long unexplored = A->nnz - fe;
+	bool queue_valid = true;  // q0 holds the frontier
+	size_t total = 1;
+	int32_t cur = 0;
+	const double alpha = 14.0, beta = 24.0;  // direction switching (Beamer et al.)
+	bool pull = false;
+	while (fq > 0) {
+		// ALP's choice between scatter and gather, by the frontier's edge count
+		if (!pull && (double)fe > (double)unexplored / alpha) pull = true;
+		else if (pull && (double)fq < (double)n / beta) pull = false;
+		ALP_HIP_TRY(hipMemsetAsync(cnt.p, 0, sizeof(BfsCounters), s));
+		if (pull) {
+			hipLaunchKernelGGL(k_bfs_pull, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint64_t *)S.off,
+			                   (const uint32_t *)S.idx, d_level, cur, cnt.p);
+			queue_valid = false;
+		} else {
+			if (!queue_valid) {
+				unsigned long long *c = reinterpret_cast<unsigned long long *>(scan.p);  // scratch counter
+				ALP_HIP_TRY(hipMemsetAsync(c, 0, sizeof(unsigned long long), s));
+				hipLaunchKernelGGL(k_bfs_collect, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const int32_t *)d_level, cur,
+				                   q0.p, c);
+				queue_valid = true;
+			}
+			hipLaunchKernelGGL(k_frontier_degrees, dim3(g_grid(fq)), dim3(kBlock), 0, s, (size_t)fq,
+			                   (const uint32_t *)q0.p, (const uint64_t *)S.off, deg.p);
+			ALP_HIP_TRY(hipMemsetAsync(deg.p + fq, 0, sizeof(uint64_t), s));
+			ALP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, scan_tb, deg.p, scan.p, (size_t)fq + 1, s));
+			if (fe > 0)
+				hipLaunchKernelGGL(k_bfs_push, dim3(g_grid(fe)), dim3(kBlock), 0, s, (size_t)fq, (size_t)fe,
+				                   (const uint32_t *)q0.p, (const uint64_t *)scan.p, (const uint64_t *)S.off,
+				                   (const uint32_t *)S.idx, d_level, cur + 1, q1.p, cnt.p);
+			std::swap(q0.p, q1.p);
+		}
+		ALP_LAUNCH_CHECK();
+		BfsCounters h{};
+		ALP_HIP_TRY(hipMemcpyAsync(&h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+		ALP_HIP_TRY(hipStreamSynchronize(s));
+		fq = h.next_vertices;
+		fe = h.next_edges;
+		unexplored = unexplored > fe ? unexplored - fe : 0;
+		total += fq;
+		if (fq > 0) ++cur;
+	}
+	if (nlevels) *nlevels = (size_t)cur + 1;
+	if (reached) *reached = total;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double alpha, double conv, size_t max_iter,
+                                     size_t *iters, double *quality) {
+	ALP_TRY(ensure_init());
+	if (!L || !d_pr) return ALP_ILLEGAL;
+	// the reference's argument checks (simple_pagerank.hpp:178-203)
+	if (L->nrows != L->ncols) return ALP_ILLEGAL;
+	if (!(alpha > 0.0 && alpha < 1.0)) return ALP_ILLEGAL;
+	if (max_iter == 0) return ALP_ILLEGAL;
+	const size_t n = L->nrows;
+	if (n == 0) return ALP_SUCCESS;
+	hipStream_t s = stream();
+	if (!L->ccs.off) ALP_TRY(build_ccs_from_crs(L));
+	DevBuf<double> rs, pn, pnn;
+	DevBuf<PrState> st;
+	ALP_TRY(rs.alloc(n));
+	ALP_TRY(pn.alloc(n));
+	ALP_TRY(pnn.alloc(n));
+	ALP_TRY(st.alloc(1));
+	PrState h{0.0, 0.0, alpha, conv, 0ull, (unsigned long long)max_iter, 0, 0, {0, 0}};
+	ALP_HIP_TRY(hipMemcpy(st.p, &h, sizeof(h), hipMemcpyHostToDevice));
+	hipLaunchKernelGGL(k_pr_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint64_t *)L->crs.off, alpha,
+	                   rs.p, d_pr);
+	ALP_LAUNCH_CHECK();
+	const unsigned rg = g_rgrid(n);
+	for (;;) {
+		for (int k = 0; k < 8; ++k) {
+			hipLaunchKernelGGL(k_pr_scale, dim3(rg), dim3(kBlock), 0, s, n, (const double *)d_pr, (const double *)rs.p,
+			                   pn.p, ctx().d_partials, ctx().d_tickets, st.p);
+			// pr_nextnext = pr_next L: the CCS gather of vxm (blas2.hpp:1432-1466)
+			ALP_TRY(spmv(L->ccs, L->value_type, ALP_PLUS_PATTERN_F64, ALP_DESC_OVERWRITE, pnn.p, pn.p, nullptr, s));
+			hipLaunchKernelGGL(k_pr_update, dim3(rg), dim3(kBlock), 0, s, n, d_pr, (const double *)pnn.p,
+			                   ctx().d_partials, ctx().d_tickets, st.p);
+		}
+		ALP_LAUNCH_CHECK();
+		ALP_HIP_TRY(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+		ALP_HIP_TRY(hipStreamSynchronize(s));
+		if (h.done) break;
+	}
+	if (iters) *iters = h.iter;
+	if (quality) *quality = h.residual;
+	return h.converged ? ALP_SUCCESS : ALP_FAILED;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_khop(alp_hip_matrix_t A, uint32_t source, size_t k, uint8_t *d_out) {
+	ALP_TRY(ensure_init());
+	if (!A || !d_out) return ALP_ILLEGAL;
+	if (A->nrows != A->ncols) return ALP_ILLEGAL;
+	const size_t n = A->nrows;
+	if (source >= n) return ALP_ILLEGAL;
+	hipStream_t s = stream();
+	DevBuf<uint8_t> cur;
+	ALP_TRY(cur.alloc(n));
+	hipLaunchKernelGGL(k_u8_single, dim3(g_grid(n)), dim3(kBlock), 0, s, n, cur.p, source);
+	if (!A->ccs.off) ALP_TRY(build_ccs_from_crs(A));
+	for (size_t h = 0; h < k; ++h) {
+		// next = cur (add_identity) | A cur over lor-land (mpv.hpp:94-170, knn.hpp:128-133)
+		ALP_HIP_TRY(hipMemcpyAsync(d_out, cur.p, n, hipMemcpyDeviceToDevice, s));
+		ALP_TRY(spmv(A->ccs, A->value_type, ALP_LOR_LAND_BOOL, 0, d_out, cur.p, nullptr, s));
+		ALP_HIP_TRY(hipMemcpyAsync(cur.p, d_out, n, hipMemcpyDeviceToDevice, s));
+	}
+	ALP_HIP_TRY(hipMemcpyAsync(d_out, cur.p, n, hipMemcpyDeviceToDevice, s));
+	ALP_HIP_TRY(hipStreamSynchronize(s));
+	return ALP_SUCCESS;
+}

@@ -331,6 +331,12 @@ static void launch_wave(const Storage &S, const void *val, const void *x, void *
 	                   (const typename R::X *)x, (typename R::Y *)y, mask);
 }
 
+// without autotuning: CSR-stream (whole-block folds of long rows) for skewed
+// row lengths, CSR-vector otherwise
+int auto_kernel(const Storage &S) {
+	return (S.max_len > 64 && (double)S.max_len > 8.0 * S.avg_len) ? 2 : 1;
+}
+
 int default_lanes(const Storage &S) {
 	const double a = S.avg_len;
 	if (a <= 3) return 2;
@@ -347,8 +353,9 @@ static void dispatch(const Storage &S, const void *val, unsigned descr, void *y,
 	const int mmode = mask ? ((descr & ALP_DESC_INVERT_MASK) ? 2 : 1) : 0;
 	int kernel = S.kernel;
 	int G = S.lanes ? S.lanes : default_lanes(S);
-	int RR = S.rows ? S.rows : 1;  // CSR-stream: the launch variant
-	if (kernel == 0) kernel = (S.avg_len < 4 && S.max_len > 64) ? 2 : 1;
+	int RR = S.rows;  // CSR-stream: the launch variant
+	if (kernel == 0) kernel = auto_kernel(S);
+	if (RR == 0) RR = kernel == 2 ? 8 : 1;
 #define ALP_GO(MM, OWV)                                                 \
 	if (kernel == 3)                                                    \
 		launch_wave<R, MM, OWV>(S, val, x, y, mask, s);                 \
@@ -452,10 +459,10 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, in
 ALP_EXPORT alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes, int *rows) {
 	if (!A) return ALP_ILLEGAL;
 	int k = A->crs.kernel, l = A->crs.lanes ? A->crs.lanes : default_lanes(A->crs);
-	if (k == 0) k = (A->crs.avg_len < 4 && A->crs.max_len > 64) ? 2 : 1;
+	if (k == 0) k = auto_kernel(A->crs);
 	if (kernel) *kernel = k;
 	if (lanes) *lanes = l;
-	if (rows) *rows = A->crs.rows ? A->crs.rows : 1;
+	if (rows) *rows = A->crs.rows ? A->crs.rows : (k == 2 ? 8 : 1);
 	return ALP_SUCCESS;
 }
 

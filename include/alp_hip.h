@@ -168,6 +168,26 @@ alp_rc_t alp_hip_convert_f32_to_f64(size_t n, const float *d_x, double *d_y);
 alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *d_b, double *d_x, size_t max_iter,
                     double tol, int precision, size_t *iters, double *residual);
 
+/* CG on A^T (gathering over the CCS): the adjoint solve A^T lambda = g; same
+ * arguments and conventions as alp_hip_cg */
+alp_rc_t alp_hip_cg_transposed(alp_hip_matrix_t A, const double *d_b, double *d_x, size_t max_iter,
+                               double tol, int precision, size_t *iters, double *residual);
+
+/* ---- differentiable solve (no counterpart in the reference) ----
+ * A(theta): the 27-point stencil graph's weighted Laplacian plus `shift` on the
+ * diagonal (A_ij = -theta_e for the edge e = (i, j), A_ii = sum of theta at i + shift).
+ * theta holds 13 conductances per vertex, slot d for the d-th positive neighbour
+ * offset in the order dz in {0,1}, dy in {-1,0,1}, dx in {-1,0,1} (positive: dz > 0,
+ * or dz = 0 and dy > 0, or dz = dy = 0 and dx > 0); out-of-grid slots are unused. */
+alp_rc_t alp_hip_matrix_conductance27(alp_hip_matrix_t *A, size_t nx, size_t ny, size_t nz,
+                                      const double *d_theta, double shift, unsigned flags);
+/* dL/dtheta (13 per vertex, 0 in unused slots) = -(lambda_i - lambda_j)(x_i - x_j) */
+alp_rc_t alp_hip_grad_conductance27(size_t nx, size_t ny, size_t nz, const double *d_lambda,
+                                    const double *d_x, double *d_dtheta);
+/* dL/dA = -lambda x^T on A's pattern, one value per stored CRS entry */
+alp_rc_t alp_hip_grad_matrix(alp_hip_matrix_t A, const double *d_lambda, const double *d_x,
+                             double *d_dA);
+
 /* ---- HPCG (algorithms/hpcg/hpcg.hpp:97-223, multigrid_v_cycle.hpp, red_black_gauss_seidel.hpp,
  *      system_building_utils.hpp:84-150) ----
  * A handle owns the whole multigrid hierarchy of the nx*ny*nz 27-point problem
@@ -196,6 +216,60 @@ alp_rc_t alp_hip_hpcg_vcycle(alp_hip_hpcg_t H, const double *d_r, double *d_z);
 alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, double *d_x, size_t max_iter,
                             double tol, int precision, int with_precond, size_t *iters,
                             double *residual, size_t *outer);
+
+/* ---- graphs (pattern matrices, Matrix<void>) ---- */
+#define ALP_GRAPH_SYMMETRIZE 1u       /* add (j, i) for every edge (i, j) */
+#define ALP_GRAPH_DROP_SELF_LOOPS 2u  /* drop edges (i, i) */
+/* Graph500-style R-MAT edges (no counterpart in ALP): edge e picks, at each of
+ * `scale` levels, a quadrant with probabilities a, b, c, 1-a-b-c from a counter-based
+ * generator seeded by `seed`; deterministic for any launch shape. */
+alp_rc_t alp_hip_rmat_edges(unsigned scale, size_t nedges, double a, double b, double c,
+                            uint64_t seed, uint32_t *d_src, uint32_t *d_dst);
+/* an n x n pattern matrix (CRS and CCS) from device edge lists; duplicates are
+ * removed (buildMatrixUnique's uniqueness, reference/io.hpp:2304, established on the
+ * device by sort + unique); rows list columns in ascending order */
+alp_rc_t alp_hip_graph_from_edges(alp_hip_matrix_t *A, size_t n, size_t nedges,
+                                  const uint32_t *d_src, const uint32_t *d_dst, unsigned flags,
+                                  size_t *nnz);
+/* the first vertex of maximum out-degree */
+alp_rc_t alp_hip_max_degree_vertex(alp_hip_matrix_t A, uint32_t *vertex, size_t *degree);
+/* breadth-first levels from `source` over the lor-land semiring: d_level[v] = hops
+ * from source, -1 if unreachable; push/pull switched per level like vxm_generic's
+ * scatter/gather choice (reference/blas2.hpp:1108-1132, 1311-1334) */
+alp_rc_t alp_hip_bfs(alp_hip_matrix_t A, uint32_t source, int32_t *d_level, size_t *nlevels,
+                     size_t *reached);
+/* simple_pagerank (algorithms/simple_pagerank.hpp:146-533): d_pr (n doubles) gets the
+ * ranks; returns ALP_FAILED when `conv` was not reached within max_iter */
+alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double alpha, double conv,
+                          size_t max_iter, size_t *iters, double *quality);
+/* knn (algorithms/knn.hpp:82-140): d_out[v] = 1 iff v is within k hops of source */
+alp_rc_t alp_hip_khop(alp_hip_matrix_t A, uint32_t source, size_t k, uint8_t *d_out);
+
+/* ---- batched multi-right-hand-side CG (no counterpart in the reference; each
+ *      instance follows conjugate_gradient.hpp:191-484) ----
+ * nsys anisotropic 7-point Laplacians on nx*ny*nz grids with per-axis coefficients
+ * (diagonal 2(cx+cy+cz), off-diagonals -cx, -cy, -cz; Dirichlet boundaries), stacked
+ * block-diagonally; nrhs in {1, 2, 4, 8, 16} right-hand sides per system. Blocks of
+ * vectors are row-major: entry (row, k) at row * nrhs + k. */
+typedef struct alp_hip_batch_s *alp_hip_batch_t;
+alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *B, size_t nsys, const uint32_t *h_nx,
+                                       const uint32_t *h_ny, const uint32_t *h_nz,
+                                       const double *h_cx, const double *h_cy, const double *h_cz,
+                                       size_t nrhs);
+alp_rc_t alp_hip_batch_destroy(alp_hip_batch_t B);
+/* total rows, nonzeros, and (h_row0, nsys + 1 entries, may be NULL) each system's first row */
+alp_rc_t alp_hip_batch_info(alp_hip_batch_t B, size_t *nrows, size_t *nnz, uint64_t *h_row0);
+/* Y = A X over all systems (FP64) */
+alp_rc_t alp_hip_batch_spmm(alp_hip_batch_t B, const double *d_X, double *d_Y);
+/* time the SpMM for each right-hand-side tile width (lanes per row) and keep the fastest */
+alp_rc_t alp_hip_batch_autotune(alp_hip_batch_t B, int *width);
+/* all systems from X = 0 to ||b - A x|| <= tol ||b|| per right-hand side; precision
+ * 0 = FP64 CG, 1 = FP32 SpMM/CG inside FP64 iterative refinement. h_iters and
+ * h_relres (nsys * nrhs entries each, may be NULL): CG iterations and final true
+ * relative residual per instance. ALP_FAILED if some instance missed tol. */
+alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, double *d_X, double tol,
+                             size_t max_iter, int precision, size_t *h_iters, double *h_relres,
+                             size_t *outer);
 
 #ifdef __cplusplus
 }

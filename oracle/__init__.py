@@ -73,6 +73,8 @@ def lib() -> ctypes.CDLL:
         L.orc_bfs_levels.argtypes = [_S, _P, _P, _U32, _P]
         L.orc_khop.argtypes = [_S, _P, _P, _U32, _S, _P]
         L.orc_rmat_edges.argtypes = [ctypes.c_uint, _S, _D, _D, _D, _U64, _P, _P]
+        L.orc_pcg_f64.argtypes = [_S, _P, _P, _P, _P, _P, _S, _D, ctypes.c_void_p, _P, _P, _P]
+        L.orc_pcg_f32.argtypes = [_S, _P, _P, _P, _P, _P, _S, ctypes.c_float, ctypes.c_void_p, _P, _P, _P]
         L.orc_mix64.restype = _U64
         L.orc_mix64.argtypes = [_U64]
         _lib = L
@@ -278,6 +280,50 @@ def symmetrize_dedup(n: int, src: np.ndarray, dst: np.ndarray):
     return (key // np.uint64(n)).astype(np.uint32), (key % np.uint64(n)).astype(np.uint32)
 
 
+PRECOND_F64 = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+PRECOND_F32 = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                               ctypes.POINTER(ctypes.c_float), ctypes.c_void_p)
+
+
+def precond_callback(fn, n: int, dtype=np.float64):
+    """Wraps fn(z, r) (numpy views of length n; writes z) as a C preconditioner;
+    an int returned by fn is the callback's status, anything else means 0."""
+    proto = PRECOND_F64 if np.dtype(dtype) == np.float64 else PRECOND_F32
+
+    def cb(z, r, data):
+        rv = fn(np.ctypeslib.as_array(z, (n,)), np.ctypeslib.as_array(r, (n,)))
+        return rv if isinstance(rv, int) else 0
+
+    return proto(cb)
+
+
+def pcg(off, idx, val, b, x0, max_iter: int, tol: float, precond=None):
+    """The transition solver's CG (preconditioned when precond is a callback
+    from precond_callback) in val's precision on a caller's CRS; returns
+    (status, x, iterations, residual), status 0 converged, 1 iteration limit,
+    2 preconditioner failure."""
+    dt = np.dtype(val.dtype)
+    off = np.ascontiguousarray(off, np.uint64)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    val = np.ascontiguousarray(val, dt)
+    b = np.ascontiguousarray(b, dt)
+    x = np.array(x0, dtype=dt)
+    it = _S()
+    if dt == np.float64:
+        res = _D()
+        f = lib().orc_pcg_f64
+    else:
+        res = ctypes.c_float()
+        f = lib().orc_pcg_f32
+    rc = f(len(off) - 1, _ptr(off), _ptr(idx), _ptr(val), _ptr(b), _ptr(x), max_iter, tol,
+           ctypes.cast(precond, ctypes.c_void_p) if precond is not None else None, None,
+           ctypes.byref(it), ctypes.byref(res))
+    if rc < 0:
+        raise MemoryError("orc_pcg")
+    return rc, x, it.value, res.value
+
+
 # ---------------------------------------------------------------------------
 # ALP's own reference backend (oracle/_ref), when it was built
 # ---------------------------------------------------------------------------
@@ -287,6 +333,16 @@ _refs: dict = {}
 
 def ref_available(flavour: str = "seq") -> bool:
     return os.path.exists(os.path.join(HERE, "_ref", f"libalpref_{flavour}.so"))
+
+
+def ref_solver() -> ctypes.CDLL:
+    """ALP's transition CG C ABI on the reference backend (oracle/_ref)."""
+    if "solver" not in _refs:
+        path = os.path.join(HERE, "_ref", "libalpref_solver.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        _refs["solver"] = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+    return _refs["solver"]
 
 
 def ref(flavour: str = "seq") -> ctypes.CDLL:

@@ -264,6 +264,108 @@ int orc_cg(size_t n, const uint64_t *off, const uint32_t *idx, const double *val
 	return sqrt(res) >= tol ? 1 : 0;
 }
 
+/*
+ * preconditioned_conjugate_gradient (algorithms/conjugate_gradient.hpp:191-484)
+ * in the arithmetic of T, as the transition solver calls it
+ * (src/transition/solver.cpp:248-273) on a caller's CRS in the caller's order.
+ * With pc == NULL it is the unpreconditioned form (z aliases r, :505-583); with
+ * a preconditioner: z = 0 then pc(z, r) in the prelude (:342-349), u = z,
+ * sigma = r.z (:356-364); per iteration after the stopping test pc(z, r),
+ * beta = r.z, alpha = beta / sigma, u = alpha u, u = z + u, sigma = beta
+ * (:441-471). dot follows dot_generic with a 64-byte block (8 doubles or 16
+ * floats; base/config.hpp:115-137). A non-zero pc return ends the solve with
+ * FAILED (solver.cpp:124-131). Returns 0 (converged), 1 (iteration limit,
+ * FAILED) or 2 (the preconditioner failed, FAILED).
+ */
+#define ORC_DEFINE_PCG(T, SFX, SQRT)                                                           \
+	static T orc_dot_##SFX(size_t n, const T *x, const T *y) {                                 \
+		const size_t blk = 64 / sizeof(T);                                                     \
+		if (n == 0) return (T)0;                                                               \
+		T reduced = x[n - 1] * y[n - 1];                                                       \
+		size_t i = 0;                                                                          \
+		while (i + blk < n - 1) {                                                              \
+			T zz[16];                                                                          \
+			for (size_t k = 0; k < blk; ++k) zz[k] = x[i + k] * y[i + k];                      \
+			for (size_t k = 0; k < blk; ++k) reduced = reduced + zz[k];                        \
+			i += blk;                                                                          \
+		}                                                                                      \
+		for (; i < n - 1; ++i) {                                                               \
+			const T t = x[i] * y[i];                                                           \
+			reduced = t + reduced;                                                             \
+		}                                                                                      \
+		return (T)0 + reduced;                                                                 \
+	}                                                                                          \
+	static void orc_mxv_##SFX(size_t n, const uint64_t *off, const uint32_t *idx, const T *val, \
+	                          const T *x, T *y) {                                              \
+		for (size_t i = 0; i < n; ++i) {                                                       \
+			if (off[i] == off[i + 1]) continue;                                                \
+			T output = (T)0;                                                                   \
+			for (uint64_t k = off[i]; k < off[i + 1]; ++k) {                                   \
+				const T result = val[k] * x[idx[k]];                                           \
+				output = result + output;                                                      \
+			}                                                                                  \
+			y[i] = y[i] + output;                                                              \
+		}                                                                                      \
+	}                                                                                          \
+	int orc_pcg_##SFX(size_t n, const uint64_t *off, const uint32_t *idx, const T *val,        \
+	                  const T *b, T *x, size_t max_iter, T tol, orc_precond_##SFX##_t pc,      \
+	                  void *pc_data, size_t *iters, T *residual) {                             \
+		T *r = calloc(n, sizeof(T)), *u = calloc(n, sizeof(T)), *temp = calloc(n, sizeof(T)); \
+		T *z = pc ? calloc(n, sizeof(T)) : r;                                                  \
+		if (!r || !u || !temp || !z) {                                                         \
+			free(r); free(u); free(temp);                                                      \
+			if (pc) free(z);                                                                   \
+			return -1;                                                                         \
+		}                                                                                      \
+		int ret = 0;                                                                           \
+		*iters = 0;                                                                            \
+		*residual = (T)INFINITY;                                                               \
+		orc_mxv_##SFX(n, off, idx, val, x, temp);                                              \
+		for (size_t i = 0; i < n; ++i) r[i] = (T)0 + b[i];                                     \
+		for (size_t i = 0; i < n; ++i) r[i] = r[i] - temp[i];                                  \
+		if (pc && pc(z, r, pc_data) != 0) ret = 2;                                             \
+		T sigma = 0, bnorm = 0, alpha, beta, res = (T)INFINITY;                                \
+		size_t iter = 0;                                                                       \
+		if (!ret) {                                                                            \
+			memcpy(u, z, n * sizeof(T));                                                       \
+			sigma = orc_dot_##SFX(n, r, z);                                                    \
+			bnorm = orc_dot_##SFX(n, b, b);                                                    \
+			tol *= SQRT(bnorm);                                                                \
+			for (;;) {                                                                         \
+				++iter;                                                                        \
+				memset(temp, 0, n * sizeof(T));                                                \
+				orc_mxv_##SFX(n, off, idx, val, u, temp);                                      \
+				beta = orc_dot_##SFX(n, temp, u);                                              \
+				alpha = sigma / beta;                                                          \
+				for (size_t i = 0; i < n; ++i) x[i] = alpha * u[i] + x[i];                     \
+				for (size_t i = 0; i < n; ++i) temp[i] = alpha * temp[i];                      \
+				for (size_t i = 0; i < n; ++i) r[i] = r[i] - temp[i];                          \
+				alpha = orc_dot_##SFX(n, r, r);                                                \
+				res = alpha;                                                                   \
+				if (SQRT(res) < tol || iter >= max_iter) break;                                \
+				if (pc) {                                                                      \
+					if (pc(z, r, pc_data) != 0) { ret = 2; break; }                            \
+					beta = orc_dot_##SFX(n, r, z);                                             \
+				} else {                                                                       \
+					beta = alpha;                                                              \
+				}                                                                              \
+				alpha = beta / sigma;                                                          \
+				for (size_t i = 0; i < n; ++i) u[i] = alpha * u[i];                            \
+				for (size_t i = 0; i < n; ++i) u[i] = z[i] + u[i];                             \
+				sigma = beta;                                                                  \
+			}                                                                                  \
+			*iters = iter;                                                                     \
+			*residual = res;                                                                   \
+			if (!ret && SQRT(res) >= tol) ret = 1;                                             \
+		}                                                                                      \
+		free(r); free(u); free(temp);                                                          \
+		if (pc) free(z);                                                                       \
+		return ret;                                                                            \
+	}
+
+ORC_DEFINE_PCG(double, f64, sqrt)
+ORC_DEFINE_PCG(float, f32, sqrtf)
+
 /* ------------------------------------------------------------------------- */
 /* HPCG                                                                        */
 /* ------------------------------------------------------------------------- */

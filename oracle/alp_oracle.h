@@ -50,6 +50,18 @@ int orc_cg(size_t n, const uint64_t *off, const uint32_t *idx, const double *val
            const double *b, double *x, size_t max_iter, double tol, size_t *iters,
            double *residual, double *res_hist);
 
+/* preconditioned CG in T's arithmetic on a caller's CRS (transition solver
+ * path); pc may be NULL. Returns 0 converged, 1 iteration limit, 2 the
+ * preconditioner returned non-zero, -1 out of memory. */
+typedef int (*orc_precond_f64_t)(double *, const double *, void *);
+typedef int (*orc_precond_f32_t)(float *, const float *, void *);
+int orc_pcg_f64(size_t n, const uint64_t *off, const uint32_t *idx, const double *val,
+                const double *b, double *x, size_t max_iter, double tol, orc_precond_f64_t pc,
+                void *pc_data, size_t *iters, double *residual);
+int orc_pcg_f32(size_t n, const uint64_t *off, const uint32_t *idx, const float *val,
+                const float *b, float *x, size_t max_iter, float tol, orc_precond_f32_t pc,
+                void *pc_data, size_t *iters, float *residual);
+
 typedef struct {
 	size_t levels;      /* number of grids including the finest */
 	size_t nx[8], ny[8], nz[8];

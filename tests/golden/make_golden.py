@@ -4,7 +4,10 @@ Run in a container that holds the reference sources (``make -C oracle ref``
 must have built ``oracle/_ref/libalpref_seq.so``, ALP's sequential
 ``reference`` backend behind ``oracle/ref_alp.cpp``):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py              # every fixture
+    python tests/golden/make_golden.py transition   # the transition CG group only
+
+(the transition group also needs ``oracle/_ref/libalpref_solver.so``).
 
 Every input is generated from a stated seed; outputs are what the reference
 returned. The fixtures are data only (inputs where they are not regenerable
@@ -40,9 +43,107 @@ def save(name, manifest, meta, **arrays):
                           arrays={k: list(np.asarray(v).shape) for k, v in arrays.items()})
 
 
-def main():
-    R = O.ref("seq")
+def scaled_stencil(nx, ny, nz, seed):
+    """D A D of the 27-point stencil (d ~ U[0.3, 3)) with every row's entries in
+    a random order: an SPD CRS whose Jacobi preconditioner matters and whose
+    within-row order is not the generator's."""
+    A = O.stencil(nx, ny, nz)
+    n = len(A.off) - 1
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(0.3, 3.0, n)
+    off = A.off.astype(np.int64)
+    idx = A.idx.astype(np.int64)
+    rows = np.repeat(np.arange(n), np.diff(off))
+    val = d[rows] * A.val * d[idx]
+    for i in range(n):
+        p = rng.permutation(off[i + 1] - off[i]) + off[i]
+        idx[off[i]:off[i + 1]] = idx[p]
+        val[off[i]:off[i + 1]] = val[p]
+    return off, idx, val, rng.standard_normal(n), rng.random(n)
+
+
+def transition_cg(manifest):
+    """ALP's transition CG C ABI (src/transition/solver.cpp on the reference
+    backend, oracle/_ref/libalpref_solver.so): the caller's CRS, x0 and b in,
+    x, the return code, iteration count and residual out."""
+    R = O.ref_solver()
+    Pv = ctypes.c_void_p
+    cases = [
+        # name, grid, seed, dtype, precond, tol, max_iter, fail_at (precond call that returns 7)
+        ("d_plain", (6, 5, 4), 3, np.float64, False, 1e-10, 1000, 0),
+        ("d_jacobi", (6, 5, 4), 3, np.float64, True, 1e-10, 1000, 0),
+        ("d_maxit", (6, 5, 4), 3, np.float64, False, 1e-10, 10, 0),
+        ("d_precond_fails", (6, 5, 4), 3, np.float64, True, 1e-10, 1000, 5),
+        ("d_jacobi_large", (12, 10, 9), 8, np.float64, True, 1e-12, 1000, 0),
+        ("s_plain", (6, 5, 4), 3, np.float32, False, 1e-5, 1000, 0),
+        ("s_jacobi", (6, 5, 4), 3, np.float32, True, 1e-5, 1000, 0),
+    ]
+    out = {}
+    for name, grid, seed, dt, pre, tol, max_iter, fail_at in cases:
+        off, idx, val, b, x0 = scaled_stencil(*grid, seed)
+        n = len(off) - 1
+        sfx = "dii" if dt == np.float64 else "sii"
+        ctype = ctypes.c_double if dt == np.float64 else ctypes.c_float
+        a, ja, ia = val.astype(dt), idx.astype(np.int32), off.astype(np.int32)
+        bb, x = b.astype(dt), x0.astype(dt)
+        h = Pv()
+        assert getattr(R, "sparse_cg_init_" + sfx)(ctypes.byref(h), ctypes.c_size_t(n), a.ctypes.data_as(Pv),
+                                                   ja.ctypes.data_as(Pv), ia.ctypes.data_as(Pv)) == 0
+        f = getattr(R, "sparse_cg_set_tolerance_" + sfx)
+        f.argtypes = [Pv, ctype]
+        f(h, tol)
+        f = getattr(R, "sparse_cg_set_max_iter_count_" + sfx)
+        f.argtypes = [Pv, ctypes.c_size_t]
+        f(h, max_iter)
+        cb = None
+        if pre:
+            diag = np.zeros(n, dt)
+            for i in range(n):
+                for k in range(ia[i], ia[i + 1]):
+                    if ja[k] == i:
+                        diag[i] = a[k]
+            calls = [0]
+
+            def jacobi(z, r, diag=diag, calls=calls, fail_at=fail_at):
+                calls[0] += 1
+                if fail_at and calls[0] == fail_at:
+                    return 7
+                np.divide(r, diag, out=z)
+                return 0
+
+            cb = O.precond_callback(jacobi, n, dt)
+            f = getattr(R, "sparse_cg_set_preconditioner_" + sfx)
+            f.argtypes = [Pv, Pv, Pv]
+            assert f(h, ctypes.cast(cb, Pv), None) == 0
+        rc = getattr(R, "sparse_cg_solve_" + sfx)(h, x.ctypes.data_as(Pv), bb.ctypes.data_as(Pv))
+        it, res = ctypes.c_size_t(), ctype()
+        getattr(R, "sparse_cg_get_iter_count_" + sfx)(h, ctypes.byref(it))
+        getattr(R, "sparse_cg_get_residual_" + sfx)(h, ctypes.byref(res))
+        getattr(R, "sparse_cg_destroy_" + sfx)(h)
+        meta = np.array([rc, it.value, max_iter, fail_at, 1 if pre else 0, grid[0], grid[1], grid[2], seed])
+        out.update({f"{name}_ia": ia, f"{name}_ja": ja, f"{name}_a": a, f"{name}_b": bb,
+                    f"{name}_x0": x0.astype(dt), f"{name}_x": x, f"{name}_meta": meta,
+                    f"{name}_tol": np.array([tol], dt), f"{name}_residual": np.array([res.value], dt)})
+    save("transition_cg", manifest,
+         dict(kind="transition_cg", matrix="D A D of the 27-pt stencil, d ~ U[0.3,3), rows shuffled; "
+              "numpy default_rng(seed): d, row permutations, b ~ N(0,1), x0 ~ U[0,1)",
+              preconditioner="Jacobi z = r / diag(A) (returns 7 on call fail_at when fail_at > 0)",
+              meta="rc, iterations, max_iter, fail_at, preconditioned, nx, ny, nz, seed",
+              cases=[c[0] for c in cases]), **out)
+
+
+def main(only=None):
     manifest = {}
+    mpath = os.path.join(HERE, "manifest.json")
+    if only:
+        with open(mpath) as f:
+            manifest = json.load(f)
+        {"transition": transition_cg}[only](manifest)
+        with open(mpath, "w") as f:
+            json.dump(manifest, f, indent=1, sort_keys=True)
+        print(f"updated {only}")
+        return
+    R = O.ref("seq")
 
     # -- storage: the reference's CRS and CCS of small stencils -------------------
     for (nx, ny, nz) in [(3, 3, 3), (4, 5, 6)]:
@@ -175,10 +276,13 @@ def main():
              I=I, J=J, pr=pr, rc=np.array([rc]), iterations=np.array([it.value]),
              quality=np.array([q.value]), source=np.array([src]), **knn)
 
-    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+    transition_cg(manifest)
+
+    with open(mpath, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
     print(f"wrote {len(manifest)} fixtures")
 
 
 if __name__ == "__main__":
-    main()
+    # `make_golden.py transition` regenerates that group only
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

@@ -20,7 +20,8 @@ def declared_functions():
         text = open(os.path.join(INCLUDE, h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
-        for m in re.finditer(r"^\s*[A-Za-z_][\w \*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, flags=re.M):
+        for m in re.finditer(r"^(?!\s*typedef)\s*[A-Za-z_][\w \*]*?\b([a-z_][a-z0-9_]*)\s*\(", text,
+                             flags=re.M):
             name = m.group(1)
             if name in ("if", "for", "while", "return", "sizeof", "defined"):
                 continue

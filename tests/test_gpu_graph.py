@@ -1,0 +1,117 @@
+"""Graph semirings on the device against the oracle: R-MAT generation and the
+graph build (bit-exact), BFS levels over lor-land (bit-exact with a textbook
+queue BFS), k-hop neighbourhoods (bit-exact with the reference's knn fixtures)
+and simple_pagerank (<= 1e-12 relative to the reference's fixtures)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def A():
+    import alp_amd
+    alp_amd.init(0)
+    return alp_amd
+
+
+def expected_crs(n, src, dst):
+    I, J = O.symmetrize_dedup(n, src, dst)  # sorted by (I, J)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum(np.bincount(I, minlength=n))
+    return off, J, I
+
+
+@pytest.mark.parametrize("scale,seed", [(6, 1), (10, 3), (14, 7)])
+def test_rmat_generator_and_build_bitexact(A, scale, seed):
+    n, m = 1 << scale, 16 << scale
+    s, d = A.rmat_edges(scale, m, seed=seed)
+    os_, od = O.rmat_edges(scale, m, seed=seed)
+    assert np.array_equal(s.numpy(), os_) and np.array_equal(d.numpy(), od)
+    G = A.graph_from_edges(n, s, d)
+    off, idx, I = expected_crs(n, os_, od)
+    goff, gidx, _ = G.download(0, with_values=False)
+    assert np.array_equal(goff, off) and np.array_equal(gidx, idx)
+    coff, cidx, _ = G.download(1, with_values=False)  # symmetric: CCS == CRS
+    assert np.array_equal(coff, off) and np.array_equal(cidx, idx)
+
+
+def test_build_directed_with_loops_and_duplicates(A):
+    src = np.array([0, 0, 1, 2, 2, 3, 3], np.uint32)
+    dst = np.array([1, 1, 1, 0, 3, 3, 0], np.uint32)
+    G = A.graph_from_edges(5, A.Vector.from_numpy(src), A.Vector.from_numpy(dst), flags=0)
+    off, idx, _ = G.download(0, with_values=False)
+    assert list(off) == [0, 1, 2, 4, 6, 6]
+    assert list(idx) == [1, 1, 0, 3, 0, 3]
+    coff, cidx, _ = G.download(1, with_values=False)
+    assert list(coff) == [0, 2, 4, 4, 6, 6] and list(cidx) == [2, 3, 0, 1, 2, 3]
+    G2 = A.graph_from_edges(5, A.Vector.from_numpy(src), A.Vector.from_numpy(dst), flags=A.GRAPH_DROP_SELF_LOOPS)
+    off2, idx2, _ = G2.download(0, with_values=False)
+    assert list(off2) == [0, 1, 1, 2, 3, 3] and list(idx2) == [1, 0, 3]
+
+
+@pytest.mark.parametrize("scale,seed", [(10, 1), (14, 5), (18, 2)])
+def test_bfs_levels_bitexact(A, scale, seed):
+    n, m = 1 << scale, 16 << scale
+    s, d = A.rmat_edges(scale, m, seed=seed)
+    G = A.graph_from_edges(n, s, d)
+    goff, gidx, _ = G.download(0, with_values=False)
+    ref = O.CSR(n, n, goff, gidx, None)
+    src, _ = A.max_degree_vertex(G)
+    assert src == int(np.argmax(np.diff(goff.astype(np.int64))))
+    rng = np.random.default_rng(seed)
+    for source in [src, int(rng.integers(0, n)), int(np.argmin(np.diff(goff.astype(np.int64))))]:
+        lev, nl, reached = A.bfs(G, source)
+        want = O.bfs_levels(ref, source)
+        assert np.array_equal(lev.numpy(), want), source
+        assert reached == int((want >= 0).sum()) and nl == int(want.max()) + 1
+
+
+def test_bfs_on_a_path_forces_many_levels(A):
+    n = 3000
+    src = np.arange(n - 1, dtype=np.uint32)
+    G = A.graph_from_edges(n, A.Vector.from_numpy(src), A.Vector.from_numpy(src + 1))
+    lev, nl, reached = A.bfs(G, 0)
+    assert np.array_equal(lev.numpy(), np.arange(n, dtype=np.int32)) and nl == n and reached == n
+
+
+@pytest.mark.parametrize("scale", [9, 11])
+def test_khop_and_pagerank_match_reference_fixtures(A, golden, scale):
+    g = golden(f"graph_rmat_s{scale}")
+    n = 1 << scale
+    G = A.graph_from_edges(n, A.Vector.from_numpy(g["I"]), A.Vector.from_numpy(g["J"]), flags=0)
+    src = int(g["source"][0])
+    for k in (1, 2, 3):
+        assert np.array_equal(A.khop(G, src, k).numpy(), g[f"knn_{k}"]), k
+    rc, pr, it, q = A.pagerank(G, 0.85, 1e-9, 1000)
+    assert rc == A.SUCCESS
+    assert abs(it - int(g["iterations"][0])) <= 1
+    want = g["pr"]
+    assert np.max(np.abs(pr.numpy() - want)) <= 1e-12 * np.max(want)
+
+
+def test_pagerank_smoke10(A, golden):
+    g = golden("pagerank_smoke10")
+    G = A.graph_from_edges(10, A.Vector.from_numpy(g["I"]), A.Vector.from_numpy(g["J"]), flags=0)
+    rc, pr, it, q = A.pagerank(G, 0.85, 1e-7, 1000)
+    assert rc == A.SUCCESS and abs(it - int(g["iterations"][0])) <= 1
+    assert np.max(np.abs(pr.numpy() - g["pr"])) <= 1e-12
+    assert A.pagerank(G, 0.85, 1e-30, 5)[0] == A.FAILED  # not converged: FAILED, as the reference
+
+
+def test_rmat_scale22_bfs_and_pagerank_invariants(A):
+    scale = 22
+    n, m = 1 << scale, 16 << scale
+    s, d = A.rmat_edges(scale, m, seed=1)
+    G = A.graph_from_edges(n, s, d)
+    src, deg = A.max_degree_vertex(G)
+    lev, nl, reached = A.bfs(G, src)
+    off, idx, _ = G.download(0, with_values=False)
+    want = O.bfs_levels(O.CSR(n, n, off, idx, None), src)
+    assert np.array_equal(lev.numpy(), want)
+    rc, pr, it, q = A.pagerank(G, 0.85, 1e-9, 1000)
+    p = pr.numpy()
+    assert rc == A.SUCCESS and q <= 1e-9
+    assert abs(p.sum() - 1.0) < 1e-8 and p.min() > 0

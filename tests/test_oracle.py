@@ -313,3 +313,66 @@ def test_rmat_generator_properties():
     assert np.allclose(f, [0.57, 0.19, 0.19, 0.05], atol=0.005)
     s3, _ = O.rmat_edges(10, 1000, seed=2)
     assert not np.array_equal(s3, s[:1000])
+
+
+# ---- transition CG C ABI (src/transition/solver.cpp) --------------------------
+
+TRANSITION_CASES = ["d_plain", "d_jacobi", "d_maxit", "d_precond_fails", "d_jacobi_large", "s_plain", "s_jacobi"]
+
+
+def transition_case(golden, name):
+    z = golden("transition_cg")
+    g = {k[len(name) + 1:]: z[k] for k in z.files if k.startswith(name + "_")}
+    meta = g["meta"]
+    return g, dict(rc=int(meta[0]), iters=int(meta[1]), max_iter=int(meta[2]), fail_at=int(meta[3]),
+                   precond=bool(meta[4]))
+
+
+def jacobi_for(g, fail_at, dtype):
+    ia, ja, a = g["ia"].astype(np.int64), g["ja"].astype(np.int64), g["a"]
+    n = len(ia) - 1
+    rows = np.repeat(np.arange(n), np.diff(ia))
+    diag = np.zeros(n, dtype)
+    diag[rows[ja == rows]] = a[ja == rows]
+    calls = [0]
+
+    def fn(z, r):
+        calls[0] += 1
+        if fail_at and calls[0] == fail_at:
+            return 7
+        np.divide(r, diag, out=z)
+        return 0
+
+    return fn, calls
+
+
+@pytest.mark.parametrize("name", TRANSITION_CASES)
+def test_oracle_pcg_matches_reference_transition_solver(golden, name):
+    """The oracle's (preconditioned) CG reproduces ALP's transition solver bit for
+    bit: same return code, iteration count, residual and x, in FP64 and FP32."""
+    g, m = transition_case(golden, name)
+    dt = g["a"].dtype
+    n = len(g["ia"]) - 1
+    cb = None
+    if m["precond"]:
+        fn, _ = jacobi_for(g, m["fail_at"], dt)
+        cb = O.precond_callback(fn, n, dt)
+    status, x, iters, res = O.pcg(g["ia"], g["ja"], g["a"], g["b"], g["x0"], m["max_iter"],
+                                  float(g["tol"][0]), cb)
+    assert (0 if status == 0 else 4) == m["rc"]  # NO_ERROR / FAILED
+    assert iters == m["iters"]
+    assert np.array_equal(x, g["x"])
+    assert res == g["residual"][0]
+
+
+def test_oracle_pcg_jacobi_on_constant_diagonal_is_plain_cg():
+    """Closed-form check: Jacobi with a constant diagonal only rescales z, so the
+    iterates equal the unpreconditioned ones up to rounding."""
+    A = O.stencil(5, 4, 3)
+    n = len(A.off) - 1
+    b = np.random.default_rng(2).standard_normal(n)
+    s0, x0, i0, r0 = O.pcg(A.off, A.idx, A.val, b, np.zeros(n), 1000, 1e-10)
+    cb = O.precond_callback(lambda z, r: np.divide(r, 26.0, out=z), n)
+    s1, x1, i1, r1 = O.pcg(A.off, A.idx, A.val, b, np.zeros(n), 1000, 1e-10, cb)
+    assert s0 == s1 == 0 and i0 == i1
+    assert np.max(np.abs(x0 - x1)) <= 1e-12 * np.max(np.abs(x0))
