@@ -133,8 +133,29 @@ SIGNATURES = {
 _lib = None
 
 
+def hip_runtimes() -> list:
+    """Paths of the HIP runtime libraries (libamdhip64) mapped into this process."""
+    paths = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                p = line.split()[-1]
+                if "libamdhip64" in os.path.basename(p):
+                    paths.add(os.path.realpath(p))
+    except OSError:
+        pass
+    return sorted(paths)
+
+
 def lib() -> ctypes.CDLL:
-    """The loaded libalp_hip.so (raises if it was not built: there is no fallback)."""
+    """The loaded libalp_hip.so (raises if it was not built: there is no fallback).
+
+    libalp_hip.so needs the HIP runtime by its soname (libamdhip64.so.7); when
+    PyTorch (ROCm) is imported first, the library binds to the runtime torch
+    loaded, so one process holds one runtime. Loading this library first and
+    torch afterwards would load a second runtime, which cannot use the device:
+    import torch (or alp_amd.autograd) before the first library call when both
+    are used."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):

@@ -14,6 +14,10 @@ import torch
 
 import alp_amd as alp
 
+if alp._lib is not None and len(alp.hip_runtimes()) > 1:
+    raise ImportError("libalp_hip.so was loaded before torch and bound to another HIP runtime: "
+                      f"{alp.hip_runtimes()}; import torch (or alp_amd.autograd) first")
+
 
 class ConductanceSolve(torch.autograd.Function):
     @staticmethod

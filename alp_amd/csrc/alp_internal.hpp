@@ -60,6 +60,15 @@ Context &ctx();
 alp_rc_t ensure_init();
 inline hipStream_t stream() { return ctx().stream; }
 
+// A blocking copy ordered after all work already enqueued on the library's
+// stream. That stream is created non-blocking (no implicit ordering with the
+// null stream, so a caller's or torch's null-stream work does not serialise
+// with it), which is why a plain hipMemcpy would not wait for our kernels.
+inline hipError_t copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+	const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, stream());
+	return e != hipSuccess ? e : hipStreamSynchronize(stream());
+}
+
 // number of blocks that fill the chip `per_cu` deep with kBlock-thread blocks
 inline unsigned fill_blocks(int per_cu) {
 	return static_cast<unsigned>(ctx().num_cus > 0 ? ctx().num_cus * per_cu : 256 * per_cu);

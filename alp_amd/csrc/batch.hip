@@ -598,12 +598,12 @@ ALP_EXPORT alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *out, size_t n
 		    (rc = bmalloc(&B->R32, ntot)) || (rc = bmalloc(&B->P32, ntot)) || (rc = bmalloc(&B->AP32, ntot)) ||
 		    (rc = bmalloc(&B->X32, ntot)))
 			break;
-		if (hipMemcpy(B->row_sys, rsys.data(), 4 * B->nrows, hipMemcpyHostToDevice) != hipSuccess ||
-		    hipMemcpy(B->sys, desc.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice) != hipSuccess ||
-		    hipMemcpy(B->tile_row0, trow0.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
-		    hipMemcpy(B->tile_row1, trow1.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
-		    hipMemcpy(B->tile_sys, tsys.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
-		    hipMemcpy(B->tile0, t0.data(), 4 * (nsys + 1), hipMemcpyHostToDevice) != hipSuccess) {
+		if (copy_sync(B->row_sys, rsys.data(), 4 * B->nrows, hipMemcpyHostToDevice) != hipSuccess ||
+		    copy_sync(B->sys, desc.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice) != hipSuccess ||
+		    copy_sync(B->tile_row0, trow0.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+		    copy_sync(B->tile_row1, trow1.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+		    copy_sync(B->tile_sys, tsys.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+		    copy_sync(B->tile0, t0.data(), 4 * (nsys + 1), hipMemcpyHostToDevice) != hipSuccess) {
 			rc = ALP_PANIC;
 			break;
 		}
@@ -614,7 +614,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *out, size_t n
 		hipLaunchKernelGGL(k_lap_counts, dim3(fill_blocks(8)), dim3(kBlock), 0, st, nsys, (const SysDesc *)B->sys,
 		                   (const uint32_t *)B->row_sys, B->nrows, counts);
 		std::vector<uint64_t> hc(B->nrows + 1);
-		if (hipMemcpy(hc.data(), counts, 8 * (B->nrows + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+		if (copy_sync(hc.data(), counts, 8 * (B->nrows + 1), hipMemcpyDeviceToHost) != hipSuccess) {
 			hipFree(counts);
 			rc = ALP_PANIC;
 			break;
@@ -626,7 +626,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *out, size_t n
 			hc[r] = acc;
 			acc += c;
 		}
-		if (acc != nnz || hipMemcpy(B->off, hc.data(), 8 * (B->nrows + 1), hipMemcpyHostToDevice) != hipSuccess) {
+		if (acc != nnz || copy_sync(B->off, hc.data(), 8 * (B->nrows + 1), hipMemcpyHostToDevice) != hipSuccess) {
 			rc = ALP_PANIC;
 			break;
 		}
@@ -731,7 +731,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 		ALP_LAUNCH_CHECK();
 		size_t it = 0;
 		ALP_TRY(batch_iterate<double>(B, B->val, d_X, B->R, B->P, B->AP, max_iter, &it));
-		ALP_HIP_TRY(hipMemcpy(hin.data(), B->inst, sizeof(BatchInst) * ninst, hipMemcpyDeviceToHost));
+		ALP_HIP_TRY(copy_sync(hin.data(), B->inst, sizeof(BatchInst) * ninst, hipMemcpyDeviceToHost));
 		for (int i = 0; i < ninst; ++i) its[i] = hin[i].iter;
 	} else {
 		const double inner_tol = 1e-4;
@@ -759,7 +759,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 			ALP_LAUNCH_CHECK();
 			size_t it = 0;
 			ALP_TRY(batch_iterate<float>(B, B->val32, B->X32, B->R32, B->P32, B->AP32, max_iter, &it));
-			ALP_HIP_TRY(hipMemcpy(hin.data(), B->inst, sizeof(BatchInst) * ninst, hipMemcpyDeviceToHost));
+			ALP_HIP_TRY(copy_sync(hin.data(), B->inst, sizeof(BatchInst) * ninst, hipMemcpyDeviceToHost));
 			for (int i = 0; i < ninst; ++i) its[i] += hin[i].iter;
 			hipLaunchKernelGGL(k_b_up, dim3(fill_blocks(8)), dim3(kBlock), 0, s, ntot, nr, d_X, (const float *)B->X32,
 			                   (const double *)B->rnorm, (const int *)B->outer_active, (const uint32_t *)B->row_sys);
@@ -775,9 +775,9 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 	ALP_LAUNCH_CHECK();
 	std::vector<double> rn(ninst), bn(ninst);
 	unsigned act = 0;
-	ALP_HIP_TRY(hipMemcpy(rn.data(), B->rnorm, 8 * ninst, hipMemcpyDeviceToHost));
-	ALP_HIP_TRY(hipMemcpy(bn.data(), B->bnorm, 8 * ninst, hipMemcpyDeviceToHost));
-	ALP_HIP_TRY(hipMemcpy(&act, B->nactive, sizeof(act), hipMemcpyDeviceToHost));
+	ALP_HIP_TRY(copy_sync(rn.data(), B->rnorm, 8 * ninst, hipMemcpyDeviceToHost));
+	ALP_HIP_TRY(copy_sync(bn.data(), B->bnorm, 8 * ninst, hipMemcpyDeviceToHost));
+	ALP_HIP_TRY(copy_sync(&act, B->nactive, sizeof(act), hipMemcpyDeviceToHost));
 	for (int i = 0; i < ninst; ++i) {
 		if (h_iters) h_iters[i] = its[i];
 		if (h_relres) h_relres[i] = bn[i] > 0 ? rn[i] / bn[i] : 0.0;

@@ -327,7 +327,7 @@ static alp_rc_t edges_to_keys(size_t nedges, const uint32_t *src, const uint32_t
 	// a trailing ~0 key (dropped self loops) is not an edge
 	if (u > 0) {
 		uint64_t last = 0;
-		ALP_HIP_TRY(hipMemcpy(&last, out.p + (u - 1), sizeof(last), hipMemcpyDeviceToHost));
+		ALP_HIP_TRY(copy_sync(&last, out.p + (u - 1), sizeof(last), hipMemcpyDeviceToHost));
 		if (last == ~0ull) --u;
 	}
 	*nout = u;
@@ -374,7 +374,7 @@ ALP_EXPORT alp_rc_t alp_hip_graph_from_edges(alp_hip_matrix_t *A, size_t n, size
 		if ((rc = edges_to_keys(nedges, d_src, d_dst, sym, loops, false, keys, &nnz))) break;
 		// every key must name vertices below n
 		uint64_t last = 0;
-		if (nnz && hipMemcpy(&last, keys.p + (nnz - 1), sizeof(last), hipMemcpyDeviceToHost) != hipSuccess) {
+		if (nnz && copy_sync(&last, keys.p + (nnz - 1), sizeof(last), hipMemcpyDeviceToHost) != hipSuccess) {
 			rc = ALP_PANIC;
 			break;
 		}
@@ -391,7 +391,7 @@ ALP_EXPORT alp_rc_t alp_hip_graph_from_edges(alp_hip_matrix_t *A, size_t n, size
 			rc = ALP_PANIC;
 			break;
 		}
-		if (tnnz && hipMemcpy(&last, tkeys.p + (tnnz - 1), sizeof(last), hipMemcpyDeviceToHost) != hipSuccess) {
+		if (tnnz && copy_sync(&last, tkeys.p + (tnnz - 1), sizeof(last), hipMemcpyDeviceToHost) != hipSuccess) {
 			rc = ALP_PANIC;
 			break;
 		}
@@ -415,7 +415,7 @@ ALP_EXPORT alp_rc_t alp_hip_max_degree_vertex(alp_hip_matrix_t A, uint32_t *vert
 	if (!A || !vertex) return ALP_ILLEGAL;
 	if (A->nrows == 0) return ALP_ILLEGAL;
 	std::vector<uint64_t> off(A->nrows + 1);
-	ALP_HIP_TRY(hipMemcpy(off.data(), A->crs.off, sizeof(uint64_t) * off.size(), hipMemcpyDeviceToHost));
+	ALP_HIP_TRY(copy_sync(off.data(), A->crs.off, sizeof(uint64_t) * off.size(), hipMemcpyDeviceToHost));
 	size_t best = 0;
 	uint64_t bd = 0;
 	for (size_t i = 0; i < A->nrows; ++i) {
@@ -453,9 +453,9 @@ ALP_EXPORT alp_rc_t alp_hip_bfs(alp_hip_matrix_t A, uint32_t source, int32_t *d_
 	ALP_TRY(tmp.alloc(scan_tb));
 	hipLaunchKernelGGL(k_bfs_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, d_level, source);
 	ALP_HIP_TRY(hipStreamSynchronize(s));
-	ALP_HIP_TRY(hipMemcpy(q0.p, &source, sizeof(uint32_t), hipMemcpyHostToDevice));
+	ALP_HIP_TRY(copy_sync(q0.p, &source, sizeof(uint32_t), hipMemcpyHostToDevice));
 	uint64_t src_deg[2];
-	ALP_HIP_TRY(hipMemcpy(src_deg, S.off + source, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+	ALP_HIP_TRY(copy_sync(src_deg, S.off + source, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
 	unsigned long long fq = 1, fe = src_deg[1] - src_deg[0];
 	unsigned long long unexplored = A->nnz - fe;
 	bool queue_valid = true;  // q0 holds the frontier
@@ -524,7 +524,7 @@ ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double al
 	ALP_TRY(pnn.alloc(n));
 	ALP_TRY(st.alloc(1));
 	PrState h{0.0, 0.0, alpha, conv, 0ull, (unsigned long long)max_iter, 0, 0, {0, 0}};
-	ALP_HIP_TRY(hipMemcpy(st.p, &h, sizeof(h), hipMemcpyHostToDevice));
+	ALP_HIP_TRY(copy_sync(st.p, &h, sizeof(h), hipMemcpyHostToDevice));
 	hipLaunchKernelGGL(k_pr_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint64_t *)L->crs.off, alpha,
 	                   rs.p, d_pr);
 	ALP_LAUNCH_CHECK();

@@ -624,8 +624,8 @@ static alp_rc_t build_level(HpLevel &L, size_t nx, size_t ny, size_t nz, int col
 		// colour ranges and tiles (host, from the permuted offsets)
 		std::vector<uint32_t> hkey(n);
 		std::vector<uint64_t> hoff(n + 1);
-		if (hipMemcpy(hkey.data(), skey, sizeof(uint32_t) * n, hipMemcpyDeviceToHost) != hipSuccess ||
-		    hipMemcpy(hoff.data(), L.off, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+		if (copy_sync(hkey.data(), skey, sizeof(uint32_t) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+		    copy_sync(hoff.data(), L.off, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost) != hipSuccess) {
 			rc = ALP_PANIC;
 			break;
 		}
@@ -657,10 +657,10 @@ static alp_rc_t build_level(HpLevel &L, size_t nx, size_t ny, size_t nz, int col
 			    (rc = dmalloc(&S.base, S.count)) || (rc = dmalloc(&S.top, S.count)))
 				break;
 			if (S.count &&
-			    (hipMemcpy(S.rb, rb.data(), 4 * S.count, hipMemcpyHostToDevice) != hipSuccess ||
-			     hipMemcpy(S.re, re.data(), 4 * S.count, hipMemcpyHostToDevice) != hipSuccess ||
-			     hipMemcpy(S.base, bb.data(), 8 * S.count, hipMemcpyHostToDevice) != hipSuccess ||
-			     hipMemcpy(S.top, tt.data(), 8 * S.count, hipMemcpyHostToDevice) != hipSuccess)) {
+			    (copy_sync(S.rb, rb.data(), 4 * S.count, hipMemcpyHostToDevice) != hipSuccess ||
+			     copy_sync(S.re, re.data(), 4 * S.count, hipMemcpyHostToDevice) != hipSuccess ||
+			     copy_sync(S.base, bb.data(), 8 * S.count, hipMemcpyHostToDevice) != hipSuccess ||
+			     copy_sync(S.top, tt.data(), 8 * S.count, hipMemcpyHostToDevice) != hipSuccess)) {
 				rc = ALP_PANIC;
 				break;
 			}

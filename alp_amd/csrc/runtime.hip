@@ -161,9 +161,9 @@ static alp_rc_t upload_row_blocks(const std::vector<uint64_t> &off, size_t nrows
 	*d_rb = nullptr;
 	*d_base = nullptr;
 	ALP_HIP_TRY(hipMalloc(d_rb, sizeof(uint32_t) * rb.size()));
-	ALP_HIP_TRY(hipMemcpy(*d_rb, rb.data(), sizeof(uint32_t) * rb.size(), hipMemcpyHostToDevice));
+	ALP_HIP_TRY(copy_sync(*d_rb, rb.data(), sizeof(uint32_t) * rb.size(), hipMemcpyHostToDevice));
 	ALP_HIP_TRY(hipMalloc(d_base, sizeof(uint64_t) * rbase.size()));
-	ALP_HIP_TRY(hipMemcpy(*d_base, rbase.data(), sizeof(uint64_t) * rbase.size(), hipMemcpyHostToDevice));
+	ALP_HIP_TRY(copy_sync(*d_base, rbase.data(), sizeof(uint64_t) * rbase.size(), hipMemcpyHostToDevice));
 	*count = rb.size() - 1;
 	return ALP_SUCCESS;
 }
@@ -173,7 +173,7 @@ static alp_rc_t upload_row_blocks(const std::vector<uint64_t> &off, size_t nrows
 // gets a tile of its own. Built on the host from the downloaded offsets; O(nrows).
 static alp_rc_t build_row_blocks(Storage &S) {
 	std::vector<uint64_t> off(S.nmajor + 1);
-	ALP_HIP_TRY(hipMemcpy(off.data(), S.off, sizeof(uint64_t) * (S.nmajor + 1), hipMemcpyDeviceToHost));
+	ALP_HIP_TRY(copy_sync(off.data(), S.off, sizeof(uint64_t) * (S.nmajor + 1), hipMemcpyDeviceToHost));
 	ALP_TRY(upload_row_blocks(off, S.nmajor, 2048, &S.row_blocks, &S.row_block_base, &S.n_row_blocks));
 	ALP_TRY(upload_row_blocks(off, S.nmajor, 512, &S.row_blocks_w, &S.row_block_base_w,
 	                          &S.n_row_blocks_w, kWave));
@@ -183,7 +183,7 @@ static alp_rc_t build_row_blocks(Storage &S) {
 
 alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32) {
 	uint64_t total = 0;
-	ALP_HIP_TRY(hipMemcpy(&total, S.off + S.nmajor, sizeof(uint64_t), hipMemcpyDeviceToHost));
+	ALP_HIP_TRY(copy_sync(&total, S.off + S.nmajor, sizeof(uint64_t), hipMemcpyDeviceToHost));
 	if (want_f32 && value_type == ALP_VAL_F64 && total > 0) {
 		ALP_HIP_TRY(hipMalloc(&S.val32, sizeof(float) * total));
 		hipLaunchKernelGGL((k_convert<double, float>), dim3(stream_grid(total)), dim3(kBlock), 0,
@@ -653,9 +653,9 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_create_crs(alp_hip_matrix_t *A, size_t nrows,
 			rc = ALP_OUTOFMEM;
 			break;
 		}
-		if (hipMemcpy(M->crs.off, h_off, sizeof(uint64_t) * (nrows + 1), hipMemcpyHostToDevice) ||
-		    (nnz && hipMemcpy(M->crs.idx, h_idx, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice)) ||
-		    (nnz && vs && hipMemcpy(M->crs.val, h_val, vs * nnz, hipMemcpyHostToDevice))) {
+		if (copy_sync(M->crs.off, h_off, sizeof(uint64_t) * (nrows + 1), hipMemcpyHostToDevice) ||
+		    (nnz && copy_sync(M->crs.idx, h_idx, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice)) ||
+		    (nnz && vs && copy_sync(M->crs.val, h_val, vs * nnz, hipMemcpyHostToDevice))) {
 			rc = ALP_PANIC;
 			break;
 		}
@@ -690,9 +690,9 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_create_coo(alp_hip_matrix_t *A, size_t nrows,
 			rc = ALP_OUTOFMEM;
 			break;
 		}
-		if (nnz && (hipMemcpy(dI, h_I, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice) ||
-		            hipMemcpy(dJ, h_J, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice) ||
-		            (vs && hipMemcpy(dV, h_V, vs * nnz, hipMemcpyHostToDevice)))) {
+		if (nnz && (copy_sync(dI, h_I, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice) ||
+		            copy_sync(dJ, h_J, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice) ||
+		            (vs && copy_sync(dV, h_V, vs * nnz, hipMemcpyHostToDevice)))) {
 			rc = ALP_PANIC;
 			break;
 		}
@@ -787,9 +787,9 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_download(alp_hip_matrix_t A, int which, uint6
 	Storage &S = which == 0 ? A->crs : A->ccs;
 	if (!S.off) return ALP_ILLEGAL;
 	ALP_HIP_TRY(hipStreamSynchronize(stream()));
-	if (h_off) ALP_HIP_TRY(hipMemcpy(h_off, S.off, sizeof(uint64_t) * (S.nmajor + 1), hipMemcpyDeviceToHost));
-	if (h_idx && A->nnz) ALP_HIP_TRY(hipMemcpy(h_idx, S.idx, sizeof(uint32_t) * A->nnz, hipMemcpyDeviceToHost));
+	if (h_off) ALP_HIP_TRY(copy_sync(h_off, S.off, sizeof(uint64_t) * (S.nmajor + 1), hipMemcpyDeviceToHost));
+	if (h_idx && A->nnz) ALP_HIP_TRY(copy_sync(h_idx, S.idx, sizeof(uint32_t) * A->nnz, hipMemcpyDeviceToHost));
 	const size_t vs = value_size(A->value_type);
-	if (h_val && vs && A->nnz) ALP_HIP_TRY(hipMemcpy(h_val, S.val, vs * A->nnz, hipMemcpyDeviceToHost));
+	if (h_val && vs && A->nnz) ALP_HIP_TRY(copy_sync(h_val, S.val, vs * A->nnz, hipMemcpyDeviceToHost));
 	return ALP_SUCCESS;
 }

@@ -130,6 +130,7 @@ def test_conductance_matrix_values(A):
 
 
 @pytest.mark.gpu
+@pytest.mark.torch
 @pytest.mark.parametrize("precision", [0, 1])
 def test_autograd_against_cpu_adjoint(A, precision):
     import torch
@@ -195,6 +196,7 @@ def test_transposed_cg_equals_cg_on_symmetric(A):
 
 
 @pytest.mark.gpu
+@pytest.mark.torch
 @pytest.mark.parametrize("precision", [0, 1])
 def test_reverse_mode_equals_forward_tangent_at_96_cubed(A, precision):
     """Size-independent check on a 96^3 grid (884736 unknowns, shift 1e-2): the

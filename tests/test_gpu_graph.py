@@ -49,7 +49,8 @@ def test_build_directed_with_loops_and_duplicates(A):
     assert list(coff) == [0, 2, 4, 4, 6, 6] and list(cidx) == [2, 3, 0, 1, 2, 3]
     G2 = A.graph_from_edges(5, A.Vector.from_numpy(src), A.Vector.from_numpy(dst), flags=A.GRAPH_DROP_SELF_LOOPS)
     off2, idx2, _ = G2.download(0, with_values=False)
-    assert list(off2) == [0, 1, 1, 2, 3, 3] and list(idx2) == [1, 0, 3]
+    # rows: 0 -> {1}, 1 -> {} (loop dropped), 2 -> {0, 3}, 3 -> {0} (loop dropped), 4 -> {}
+    assert list(off2) == [0, 1, 1, 3, 4, 4] and list(idx2) == [1, 0, 3, 0]
 
 
 @pytest.mark.parametrize("scale,seed", [(10, 1), (14, 5), (18, 2)])
