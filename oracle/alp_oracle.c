@@ -15,6 +15,12 @@
  *   orc_dot                reference dot fixtures (bit-exact); exact integer sums.
  *   orc_cg                 reference conjugate_gradient fixtures (bit-exact x,
  *                          iterations, residual); dense direct solve.
+ *   orc_pcg_f64/_f32       the reference's transition CG C ABI (src/transition/
+ *                          solver.cpp, built from source in oracle/_ref): bit-exact
+ *                          x, return code, iterations and residual, with and
+ *                          without a Jacobi preconditioner, at the iteration limit
+ *                          and on a failing preconditioner; a constant-diagonal
+ *                          Jacobi reproduces the plain CG.
  *   orc_hpcg               reference hpcg fixtures (bit-exact x, iterations,
  *                          residual) with mod-k colours; x -> 1 since b = A*1.
  *   orc_pagerank           reference simple_pagerank fixtures; the 10-node vector
