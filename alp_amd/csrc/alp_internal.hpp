@@ -55,6 +55,7 @@ struct Context {
 	double *h_pinned = nullptr;     // 64 pinned host doubles
 };
 constexpr int kMaxPartials = 4096;
+constexpr int kSellRows = 64;  // rows per sliced-ELL slice: one wavefront, one lane per row
 
 Context &ctx();
 alp_rc_t ensure_init();
@@ -173,10 +174,21 @@ struct Storage {
 	uint32_t *row_blocks_w = nullptr;
 	uint64_t *row_block_base_w = nullptr;
 	size_t n_row_blocks_w = 0;
+	// sliced ELLPACK copy for matrices with regular rows: slices of kSellRows
+	// consecutive rows; a slice of width w (its longest row) holds w * kSellRows
+	// entries stored q-major, so entry q of row r sits at
+	// sell_off[r / kSellRows] + q * kSellRows + r % kSellRows. Padding entries (q
+	// beyond a row's length) are never read by the kernels.
+	uint64_t *sell_off = nullptr;  // n_slices + 1
+	uint32_t *sell_idx = nullptr;
+	void *sell_val = nullptr;      // the matrix's value type, or null (pattern)
+	float *sell_val32 = nullptr;   // FP32 copy when val32 exists
+	size_t n_slices = 0;
+	size_t sell_entries = 0;
 	// statistics for kernel selection
 	double avg_len = 0;
 	size_t max_len = 0;
-	int kernel = 0;  // 0 auto, 1 vector, 2 stream, 3 wave
+	int kernel = 0;  // 0 auto, 1 vector, 2 stream, 3 wave, 4 sliced ELL
 	int lanes = 0;
 	int rows = 0;    // rows per lane group of the CSR-vector kernel
 };
@@ -188,6 +200,10 @@ struct alp_hip_matrix_s {
 	int value_type = ALP_VAL_F64;
 	unsigned flags = 0;
 	alp::Storage crs, ccs;
+	// solver state kept between calls on this matrix (work vectors and captured
+	// iteration graphs); released with the matrix
+	void *solver_cache = nullptr;
+	void (*free_solver_cache)(void *) = nullptr;
 };
 
 namespace alp {

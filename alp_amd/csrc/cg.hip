@@ -22,6 +22,7 @@
 // r/||r|| in FP32 to a relative 1e-5, then x += ||r|| d in FP64.
 #include "spmv_device.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -303,6 +304,65 @@ static unsigned vec_grid(size_t n) {
 }
 
 int default_lanes(const Storage &S);
+int auto_kernel(const Storage &S);
+
+// Work vectors and the captured iteration batch of one solver configuration,
+// kept on the matrix between calls: a repeated solve with the same storage,
+// vectors and kernel choice replays the graph without allocating or capturing.
+struct CgSlot {
+	size_t n = 0;
+	std::vector<void *> bufs;
+	std::vector<size_t> sizes;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	const void *key[5] = {};
+	int kcfg[3] = {-1, -1, -1};
+
+	void drop_graph() {
+		if (exec) hipGraphExecDestroy(exec);
+		if (graph) hipGraphDestroy(graph);
+		exec = nullptr;
+		graph = nullptr;
+	}
+	~CgSlot() {
+		drop_graph();
+		for (void *p : bufs) hipFree(p);
+	}
+	// buffers of the given byte sizes; reallocated (and the graph dropped) when they change
+	alp_rc_t buffers(std::initializer_list<size_t> want, void **out) {
+		const std::vector<size_t> w(want);
+		if (w != sizes) {
+			drop_graph();
+			for (void *p : bufs) hipFree(p);
+			bufs.clear();
+			sizes.clear();
+			for (size_t b : w) {
+				void *q = nullptr;
+				ALP_HIP_TRY(hipMalloc(&q, b ? b : 1));
+				bufs.push_back(q);
+				sizes.push_back(b);
+			}
+		}
+		for (size_t i = 0; i < bufs.size(); ++i) out[i] = bufs[i];
+		return ALP_SUCCESS;
+	}
+};
+
+enum CgSlotId { kSlotFp64 = 0, kSlotMixedInner, kSlotNativeF32, kSlotNativeF64, kSlotTransposed, kSlotMixedInnerT, kSlots };
+
+struct CgCache {
+	CgSlot slot[kSlots];
+};
+
+static void free_cg_cache(void *p) { delete static_cast<CgCache *>(p); }
+
+static CgSlot &cg_slot(alp_hip_matrix_s *A, int id) {
+	if (!A->solver_cache) {
+		A->solver_cache = new CgCache();
+		A->free_solver_cache = free_cg_cache;
+	}
+	return static_cast<CgCache *>(A->solver_cache)->slot[id];
+}
 
 template <typename T>
 struct CgRunner {
@@ -316,7 +376,10 @@ struct CgRunner {
 	CgRunner(const Storage &S_, CgArgs<T> a_, int vt_ = ALP_VAL_F64) : S(S_), a(a_), vt(vt_) {
 		G = S.lanes ? S.lanes : default_lanes(S);
 		RR = S.rows ? S.rows : 1;
-		fused = S.kernel != 2 && G >= 2;
+		// the fused SpMV + dot kernel is the CSR-vector gather; the other
+		// kernels run as a generic SpMV followed by the dot
+		const int k = S.kernel ? S.kernel : auto_kernel(S);
+		fused = k == 1 && G >= 2;
 		const size_t rows_per = (size_t)(kBlock / G) * RR;
 		const size_t nchunks = (a.n + rows_per - 1) / rows_per;
 		size_t g = fill_blocks(8);
@@ -367,19 +430,35 @@ struct CgRunner {
 		return ALP_SUCCESS;
 	}
 
-	// run until done; batches of `batch` iterations replayed from a captured graph
-	alp_rc_t run(hipStream_t s, CgState *h_state) {
+	// run until done; batches of `batch` iterations replayed from a captured graph,
+	// reused from `slot` when it was captured for the same storage, vectors and
+	// kernel choice (and kept there for the next call)
+	alp_rc_t run(hipStream_t s, CgState *h_state, CgSlot *slot = nullptr) {
 		const int batch = 16;
 		hipGraphExec_t exec = nullptr;
 		hipGraph_t graph = nullptr;
 		bool use_graph = false;
-		if (s != nullptr && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-			alp_rc_t rc = ALP_SUCCESS;
-			for (int i = 0; i < batch && rc == ALP_SUCCESS; ++i) rc = iteration(s);
-			if (hipStreamEndCapture(s, &graph) == hipSuccess && rc == ALP_SUCCESS &&
-			    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
-				use_graph = true;
-			(void)hipGetLastError();
+		const void *key[5] = {&S, a.x, a.b, a.val, a.st};
+		const int kcfg[3] = {S.kernel, S.lanes, S.rows};
+		if (slot && slot->exec && std::equal(key, key + 5, slot->key) && std::equal(kcfg, kcfg + 3, slot->kcfg)) {
+			exec = slot->exec;
+			use_graph = true;
+		} else {
+			if (slot) slot->drop_graph();
+			if (s != nullptr && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+				alp_rc_t rc = ALP_SUCCESS;
+				for (int i = 0; i < batch && rc == ALP_SUCCESS; ++i) rc = iteration(s);
+				if (hipStreamEndCapture(s, &graph) == hipSuccess && rc == ALP_SUCCESS &&
+				    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
+					use_graph = true;
+				(void)hipGetLastError();
+			}
+			if (slot && use_graph) {
+				slot->graph = graph;
+				slot->exec = exec;
+				std::copy(key, key + 5, slot->key);
+				std::copy(kcfg, kcfg + 3, slot->kcfg);
+			}
 		}
 		alp_rc_t rc = ALP_SUCCESS;
 		for (;;) {
@@ -396,24 +475,11 @@ struct CgRunner {
 			}
 			if (h_state->done) break;
 		}
-		if (exec) hipGraphExecDestroy(exec);
-		if (graph) hipGraphDestroy(graph);
+		if (!slot || slot->exec != exec) {
+			if (exec) hipGraphExecDestroy(exec);
+			if (graph) hipGraphDestroy(graph);
+		}
 		return rc;
-	}
-};
-
-struct DeviceBuffers {
-	std::vector<void *> ptrs;
-	~DeviceBuffers() {
-		for (void *p : ptrs) hipFree(p);
-	}
-	template <typename T>
-	alp_rc_t get(T **p, size_t count) {
-		void *q = nullptr;
-		ALP_HIP_TRY(hipMalloc(&q, sizeof(T) * (count ? count : 1)));
-		ptrs.push_back(q);
-		*p = static_cast<T *>(q);
-		return ALP_SUCCESS;
 	}
 };
 
@@ -421,13 +487,11 @@ static alp_rc_t cg_fp64(alp_hip_matrix_t A, Storage &S, const double *b, double 
                         double tol, size_t *iters, double *residual) {
 	const size_t n = A->nrows;
 	hipStream_t s = stream();
-	DeviceBuffers buf;
-	double *r, *u, *temp;
-	CgState *st;
-	ALP_TRY(buf.get(&r, n));
-	ALP_TRY(buf.get(&u, n));
-	ALP_TRY(buf.get(&temp, n));
-	ALP_TRY(buf.get(&st, 1));
+	CgSlot &slot = cg_slot(A, &S == &A->ccs ? kSlotTransposed : kSlotFp64);
+	void *bp[4];
+	ALP_TRY(slot.buffers({8 * n, 8 * n, 8 * n, sizeof(CgState)}, bp));
+	double *r = (double *)bp[0], *u = (double *)bp[1], *temp = (double *)bp[2];
+	CgState *st = (CgState *)bp[3];
 	CgArgs<double> a{n, S.off, S.idx, (const double *)S.val, x, r, u, temp, b,
 	                 ctx().d_partials, ctx().d_tickets, st};
 	// temp = A x; r = b - temp; u = r; sigma = r.r; tol *= sqrt(b.b) (:332-378)
@@ -437,7 +501,7 @@ static alp_rc_t cg_fp64(alp_hip_matrix_t A, Storage &S, const double *b, double 
 	ALP_LAUNCH_CHECK();
 	CgRunner<double> runner(S, a);
 	CgState h;
-	ALP_TRY(runner.run(s, &h));
+	ALP_TRY(runner.run(s, &h, &slot));
 	*iters = h.iter;
 	*residual = h.rr;
 	return h.converged ? ALP_SUCCESS : ALP_FAILED;
@@ -451,20 +515,14 @@ static alp_rc_t cg_mixed(alp_hip_matrix_t A, Storage &S, const double *b, double
 		ALP_TRY(alp_hip_convert_f64_to_f32(A->nnz, (const double *)S.val, S.val32));
 	}
 	hipStream_t s = stream();
-	DeviceBuffers buf;
-	double *r64, *t64;
-	float *x32, *r32, *u32, *temp32, *b32;
-	CgState *st;
-	IrState *ir;
-	ALP_TRY(buf.get(&r64, n));
-	ALP_TRY(buf.get(&t64, n));
-	ALP_TRY(buf.get(&x32, n));
-	ALP_TRY(buf.get(&r32, n));
-	ALP_TRY(buf.get(&u32, n));
-	ALP_TRY(buf.get(&temp32, n));
-	ALP_TRY(buf.get(&b32, n));
-	ALP_TRY(buf.get(&st, 1));
-	ALP_TRY(buf.get(&ir, 1));
+	CgSlot &slot = cg_slot(A, &S == &A->ccs ? kSlotMixedInnerT : kSlotMixedInner);
+	void *bp[9];
+	ALP_TRY(slot.buffers({8 * n, 8 * n, 4 * n, 4 * n, 4 * n, 4 * n, 4 * n, sizeof(CgState), sizeof(IrState)}, bp));
+	double *r64 = (double *)bp[0], *t64 = (double *)bp[1];
+	float *x32 = (float *)bp[2], *r32 = (float *)bp[3], *u32 = (float *)bp[4], *temp32 = (float *)bp[5],
+	      *b32 = (float *)bp[6];
+	CgState *st = (CgState *)bp[7];
+	IrState *ir = (IrState *)bp[8];
 	CgArgs<float> a{n, S.off, S.idx, S.val32, x32, r32, u32, temp32, b32,
 	                ctx().d_partials, ctx().d_tickets, st};
 	CgRunner<float> runner(S, a);
@@ -487,7 +545,7 @@ static alp_rc_t cg_mixed(alp_hip_matrix_t A, Storage &S, const double *b, double
 		hipLaunchKernelGGL(k_cg_init<float>, dim3(vec_grid(n)), dim3(kBlock), 0, s, a,
 		                   (const double *)nullptr, (const float *)b32, inner_tol, max_iter - total);
 		ALP_LAUNCH_CHECK();
-		ALP_TRY(runner.run(s, &h));
+		ALP_TRY(runner.run(s, &h, &slot));
 		total += h.iter;
 		hipLaunchKernelGGL(k_ir_correct, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, x,
 		                   (const float *)x32, (const IrState *)ir);
@@ -512,13 +570,11 @@ alp_rc_t cg_native(alp_hip_matrix_s *A, const T *b, T *x, size_t max_iter, doubl
 	if (A->value_type != vt) return ALP_MISMATCH;
 	Storage &S = A->crs;
 	hipStream_t s = stream();
-	DeviceBuffers buf;
-	T *r, *u, *temp, *z = nullptr;
-	CgState *st;
-	ALP_TRY(buf.get(&r, n));
-	ALP_TRY(buf.get(&u, n));
-	ALP_TRY(buf.get(&temp, n));
-	ALP_TRY(buf.get(&st, 1));
+	CgSlot &slot = cg_slot(A, std::is_same<T, float>::value ? kSlotNativeF32 : kSlotNativeF64);
+	void *bp[5];
+	ALP_TRY(slot.buffers({sizeof(T) * n, sizeof(T) * n, sizeof(T) * n, sizeof(CgState), sizeof(T) * n}, bp));
+	T *r = (T *)bp[0], *u = (T *)bp[1], *temp = (T *)bp[2], *z = (T *)bp[4];
+	CgState *st = (CgState *)bp[3];
 	CgArgs<T> a{n, S.off, S.idx, (const T *)S.val, x, r, u, temp, b, ctx().d_partials, ctx().d_tickets, st};
 	const alp_semiring_t ring = vt == ALP_VAL_F32 ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64;
 	ALP_TRY(spmv(S, vt, ring, ALP_DESC_OVERWRITE, temp, x, nullptr, s));
@@ -529,9 +585,8 @@ alp_rc_t cg_native(alp_hip_matrix_s *A, const T *b, T *x, size_t max_iter, doubl
 	CgRunner<T> runner(S, a, vt);
 	CgState h{};
 	if (!pc.fn) {
-		ALP_TRY(runner.run(s, &h));
+		ALP_TRY(runner.run(s, &h, &slot));
 	} else {
-		ALP_TRY(buf.get(&z, n));
 		struct Pinned {
 			T *p = nullptr;
 			~Pinned() {

@@ -181,6 +181,115 @@ static alp_rc_t build_row_blocks(Storage &S) {
 	                         &S.n_row_blocks_l);
 }
 
+// ---- sliced ELLPACK copy ------------------------------------------------------
+//
+// Built for matrices whose rows are regular enough that padding every 64-row
+// slice to its longest row costs little (the 27-point stencils, the HPCG levels,
+// the conductance matrices): one lane per row then reads the q-th nonzeros of
+// 64 consecutive rows as one contiguous segment, and for stencil-like matrices
+// the x gathers of a wavefront fall into a few contiguous runs.
+
+__global__ __launch_bounds__(kBlock) void k_sell_width(size_t nrows, size_t nslices, const uint64_t *off,
+                                                       uint64_t *size) {
+	grid_for_each<1>(nslices, [&](size_t sl) {
+		const size_t r0 = sl * kSellRows;
+		const size_t r1 = r0 + kSellRows < nrows ? r0 + kSellRows : nrows;
+		uint64_t w = 0;
+		for (size_t r = r0; r < r1; ++r) {
+			const uint64_t l = off[r + 1] - off[r];
+			w = l > w ? l : w;
+		}
+		size[sl] = w * kSellRows;
+	});
+	if (blockIdx.x == 0 && threadIdx.x == 0) size[nslices] = 0;
+}
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_sell_fill(size_t nrows, size_t nslices, const uint64_t *off,
+                                                      const uint32_t *idx, const V *val, const float *val32,
+                                                      const uint64_t *soff, uint32_t *sidx, V *sval,
+                                                      float *sval32) {
+	// one thread per (slice, lane); it writes its row's entries and padding
+	grid_for_each<1>(nslices * kSellRows, [&](size_t t) {
+		const size_t sl = t / kSellRows, lane = t % kSellRows;
+		const size_t r = sl * kSellRows + lane;
+		const uint64_t e0 = soff[sl], w = (soff[sl + 1] - e0) / kSellRows;
+		const uint64_t b = r < nrows ? off[r] : 0, len = r < nrows ? off[r + 1] - b : 0;
+		for (uint64_t q = 0; q < w; ++q) {
+			const uint64_t e = e0 + q * kSellRows + lane;
+			const bool in = q < len;
+			sidx[e] = in ? idx[b + q] : 0u;
+			if (sval) sval[e] = in ? val[b + q] : V(0);
+			if (sval32) sval32[e] = in ? val32[b + q] : 0.0f;
+		}
+	});
+}
+
+// Builds S's sliced-ELL copy when its padded size stays within 1.5 nnz and no
+// row is longer than 4096; otherwise leaves it absent. Values follow value_type.
+static alp_rc_t build_sell(Storage &S, int value_type, uint64_t nnz) {
+	if (S.nmajor == 0 || nnz == 0 || S.max_len > 4096) return ALP_SUCCESS;
+	const size_t ns = (S.nmajor + kSellRows - 1) / kSellRows;
+	uint64_t *soff = nullptr, *sizes = nullptr;
+	ALP_HIP_TRY(hipMalloc(&soff, sizeof(uint64_t) * (ns + 1)));
+	if (hipMalloc(&sizes, sizeof(uint64_t) * (ns + 1)) != hipSuccess) {
+		hipFree(soff);
+		return ALP_OUTOFMEM;
+	}
+	hipLaunchKernelGGL(k_sell_width, dim3(stream_grid(ns)), dim3(kBlock), 0, stream(), S.nmajor, ns,
+	                   (const uint64_t *)S.off, sizes);
+	size_t tmp_bytes = 0;
+	hipError_t e = hipGetLastError();
+	if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, sizes, soff, ns + 1, stream());
+	void *tmp = nullptr;
+	if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1);
+	if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, sizes, soff, ns + 1, stream());
+	uint64_t total = 0;
+	const hipError_t e2 = e == hipSuccess ? copy_sync(&total, soff + ns, sizeof(uint64_t), hipMemcpyDeviceToHost) : e;
+	if (tmp) hipFree(tmp);
+	hipFree(sizes);
+	if (e2 != hipSuccess) {
+		hipFree(soff);
+		ALP_HIP_TRY(e2);
+	}
+	if ((double)total > 1.5 * (double)nnz) {
+		hipFree(soff);
+		return ALP_SUCCESS;
+	}
+	const size_t vs = value_type == ALP_VAL_F64 ? 8 : value_type == ALP_VAL_F32 ? 4 : 0;
+	uint32_t *sidx = nullptr;
+	void *sval = nullptr;
+	float *sval32 = nullptr;
+	if (hipMalloc(&sidx, sizeof(uint32_t) * total) != hipSuccess ||
+	    (vs && hipMalloc(&sval, vs * total) != hipSuccess) ||
+	    (S.val32 && hipMalloc(&sval32, sizeof(float) * total) != hipSuccess)) {
+		// not enough memory for the copy: keep the CRS kernels only
+		(void)hipGetLastError();
+		if (sidx) hipFree(sidx);
+		if (sval) hipFree(sval);
+		if (sval32) hipFree(sval32);
+		hipFree(soff);
+		return ALP_SUCCESS;
+	}
+	const unsigned g = stream_grid(ns * kSellRows);
+	if (value_type == ALP_VAL_F32)
+		hipLaunchKernelGGL(k_sell_fill<float>, dim3(g), dim3(kBlock), 0, stream(), S.nmajor, ns,
+		                   (const uint64_t *)S.off, (const uint32_t *)S.idx, (const float *)S.val,
+		                   (const float *)S.val32, (const uint64_t *)soff, sidx, (float *)sval, sval32);
+	else
+		hipLaunchKernelGGL(k_sell_fill<double>, dim3(g), dim3(kBlock), 0, stream(), S.nmajor, ns,
+		                   (const uint64_t *)S.off, (const uint32_t *)S.idx, (const double *)S.val,
+		                   (const float *)S.val32, (const uint64_t *)soff, sidx, (double *)sval, sval32);
+	ALP_LAUNCH_CHECK();
+	S.sell_off = soff;
+	S.sell_idx = sidx;
+	S.sell_val = sval;
+	S.sell_val32 = sval32;
+	S.n_slices = ns;
+	S.sell_entries = total;
+	return ALP_SUCCESS;
+}
+
 alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32) {
 	uint64_t total = 0;
 	ALP_HIP_TRY(copy_sync(&total, S.off + S.nmajor, sizeof(uint64_t), hipMemcpyDeviceToHost));
@@ -204,6 +313,7 @@ alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32) {
 	hipFree(d_max);
 	S.max_len = mx;
 	S.avg_len = S.nmajor ? double(total) / double(S.nmajor) : 0.0;
+	ALP_TRY(build_sell(S, value_type, total));
 	return build_row_blocks(S);
 }
 
@@ -218,6 +328,10 @@ void storage_free(Storage &S) {
 	if (S.row_blocks_w) hipFree(S.row_blocks_w);
 	if (S.row_block_base_w) hipFree(S.row_block_base_w);
 	if (S.row_block_base_l) hipFree(S.row_block_base_l);
+	if (S.sell_off) hipFree(S.sell_off);
+	if (S.sell_idx) hipFree(S.sell_idx);
+	if (S.sell_val) hipFree(S.sell_val);
+	if (S.sell_val32) hipFree(S.sell_val32);
 	S = Storage();
 }
 
@@ -624,6 +738,7 @@ static alp_rc_t finish_matrix(alp_hip_matrix_t M) {
 ALP_EXPORT alp_rc_t alp_hip_matrix_destroy(alp_hip_matrix_t A) {
 	if (!A) return ALP_SUCCESS;
 	hipStreamSynchronize(stream());
+	if (A->free_solver_cache) A->free_solver_cache(A->solver_cache);
 	storage_free(A->crs);
 	storage_free(A->ccs);
 	delete A;

@@ -168,6 +168,202 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
 	}
 }
 
+// Sliced ELL: one wavefront per 64-row slice, one lane per row. A lane walks its
+// row in storage order and folds sequentially (output = product + output, then
+// y = y + output), the order of the reference's vxm_inner_kernel_gather
+// (reference/blas2.hpp:463-489); the products and sums are not contracted into
+// FMAs, so FP64 results equal the sequential reference's bit for bit. Each load
+// of the index and value streams is one contiguous 64-entry segment, and for
+// stencil-like matrices a gather instruction touches a few contiguous runs of x.
+// U entries per lane are in flight before the first multiply. XCD: each XCD
+// walks one contiguous range of slices (its L2 keeps the x window).
+template <typename R, int MASK, bool OVERWRITE, int U, bool XCD>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell(
+    size_t nrows, size_t nslices, const uint64_t *__restrict__ soff, const uint32_t *__restrict__ sidx,
+    const void *__restrict__ sval, const uint64_t *__restrict__ off, const typename R::X *__restrict__ x,
+    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
+	using A = typename R::A;
+	using Y = typename R::Y;
+	using X = typename R::X;
+	using VT = typename Val<R>::T;
+	const auto *val = static_cast<const typename R::V *>(sval);
+	const unsigned bl = XCD ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	const size_t sl = (size_t)bl * (kBlock / kSellRows) + threadIdx.x / kSellRows;
+	if (sl >= nslices) return;  // whole wavefronts leave together
+	const unsigned lane = threadIdx.x % kSellRows;
+	const size_t r = sl * kSellRows + lane;
+	const uint64_t e0 = soff[sl];
+	const unsigned w = (unsigned)((soff[sl + 1] - e0) / kSellRows);
+	bool active = r < nrows;
+	unsigned len = 0;
+	A yv = R::zero();
+	if (active) {
+		len = (unsigned)(off[r + 1] - off[r]);
+		if (MASK) active = (MASK == 1) == (mask[r] != 0);
+		if (!active) len = 0;
+		if (!OVERWRITE && active && len) yv = static_cast<A>(y[r]);
+	}
+	A acc = R::zero();
+	const uint32_t *ip = sidx + e0 + lane;
+	for (unsigned q0 = 0; q0 < w; q0 += U) {
+		uint32_t j[U];
+		VT v[U];
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			const unsigned q = q0 + u;
+			j[u] = 0;
+			v[u] = VT(0);
+			if (q < len) {
+				j[u] = __builtin_nontemporal_load(ip + (size_t)q * kSellRows);
+				v[u] = Val<R>::load(val, e0 + (size_t)q * kSellRows + lane);
+			}
+		}
+		X xv[U];
+#pragma unroll
+		for (int u = 0; u < U; ++u) xv[u] = q0 + u < len ? x[j[u]] : X(0);
+#pragma unroll
+		for (int u = 0; u < U; ++u)
+			if (q0 + u < len) acc = R::add(Val<R>::mul(v[u], xv[u]), acc);
+	}
+	if (!active) return;
+	if (len == 0) {
+		if (OVERWRITE) y[r] = static_cast<Y>(R::zero());
+	} else {
+		y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(yv, acc));
+	}
+}
+
+// CSR-stream, software-pipelined: while a block gathers and folds tile t, the
+// index and value loads of its next tile t' are already in flight (and the
+// row bounds of t'' = the tile after t'), so HBM streams continuously instead of
+// idling during the gather / LDS fold of every tile. Short tiles only carry
+// data in registers; a row longer than a tile is folded by the whole block as in
+// k_spmv_stream. Persistent grid, plain block order.
+template <typename R, int J>
+struct TileRegs {
+	uint32_t j[J];
+	typename Val<R>::T v[J];
+};
+
+template <typename R, int J>
+__device__ __forceinline__ void tile_issue(TileRegs<R, J> &t, uint64_t base, uint64_t top,
+                                           const uint32_t *__restrict__ idx,
+                                           const typename R::V *__restrict__ val) {
+#pragma unroll
+	for (int q = 0; q < J; ++q) {
+		const uint64_t k = base + threadIdx.x + (uint64_t)q * kBlock;
+		t.j[q] = 0;
+		t.v[q] = typename Val<R>::T(0);
+		if (k < top) {
+			t.j[q] = __builtin_nontemporal_load(idx + k);
+			t.v[q] = Val<R>::load(val, k);
+		}
+	}
+}
+
+template <typename R, int MASK, bool OVERWRITE, int J>
+__global__ __launch_bounds__(kBlock) void k_spmv_stream_pipe(
+    size_t nrb, const uint32_t *__restrict__ rb, const uint64_t *__restrict__ rbase,
+    const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx, const void *__restrict__ vval,
+    const typename R::X *__restrict__ x, typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
+	using A = typename R::A;
+	using Y = typename R::Y;
+	constexpr int kTile = J * kBlock;
+	__shared__ A tile[kTile];
+	__shared__ A red[kBlock / kWave];
+	const auto *val = static_cast<const typename R::V *>(vval);
+	const size_t step = gridDim.x;
+	size_t b = blockIdx.x;
+	if (b >= nrb) return;
+	// metadata of the current tile (c), the next (n) and the one after (m)
+	uint32_t c0 = rb[b], c1 = rb[b + 1];
+	uint64_t cb = rbase[b], ct = rbase[b + 1];
+	uint32_t n0 = 0, n1 = 0, m0 = 0, m1 = 0;
+	uint64_t nb = 0, nt = 0, mb = 0, mt = 0;
+	if (b + step < nrb) {
+		n0 = rb[b + step];
+		n1 = rb[b + step + 1];
+		nb = rbase[b + step];
+		nt = rbase[b + step + 1];
+	}
+	TileRegs<R, J> cur, nxt;
+	if (ct - cb <= (uint64_t)kTile) tile_issue<R, J>(cur, cb, ct, idx, val);
+	for (; b < nrb; b += step) {
+		const bool has_n = b + step < nrb;
+		const bool has_m = b + 2 * step < nrb;
+		if (has_m) {
+			m0 = rb[b + 2 * step];
+			m1 = rb[b + 2 * step + 1];
+			mb = rbase[b + 2 * step];
+			mt = rbase[b + 2 * step + 1];
+		}
+		const bool n_short = has_n && nt - nb <= (uint64_t)kTile;
+		if (ct - cb <= (uint64_t)kTile) {
+			const unsigned nr = c1 - c0;
+			unsigned tpr = 1;
+			while (tpr < 64 && tpr * 2 * nr <= kBlock) tpr *= 2;
+			const unsigned r = c0 + threadIdx.x / tpr, sub = threadIdx.x % tpr;
+			bool active = r < c1;
+			uint64_t rk = 0, re = 0;
+			A yv = R::zero();
+			if (active) {
+				rk = off[r];
+				re = off[r + 1];
+				if (MASK) active = (MASK == 1) == (mask[r] != 0);
+				if (!OVERWRITE && active && sub == 0) yv = static_cast<A>(y[r]);
+			}
+			typename R::X xg[J];
+#pragma unroll
+			for (int q = 0; q < J; ++q) xg[q] = x[cur.j[q]];
+			if (n_short) tile_issue<R, J>(nxt, nb, nt, idx, val);
+#pragma unroll
+			for (int q = 0; q < J; ++q) {
+				const uint64_t k = cb + threadIdx.x + (uint64_t)q * kBlock;
+				if (k < ct) tile[k - cb] = Val<R>::mul(cur.v[q], xg[q]);
+			}
+			__syncthreads();
+			A acc = R::zero();
+			for (uint64_t q = rk + sub; q < re; q += tpr) acc = R::add(acc, tile[q - cb]);
+			for (unsigned s = tpr / 2; s > 0; s >>= 1) acc = R::add(acc, shfl_xor(acc, s));
+			if (active && sub == 0) {
+				if (rk == re) {
+					if (OVERWRITE) y[r] = static_cast<Y>(R::zero());
+				} else {
+					y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(yv, acc));
+				}
+			}
+			__syncthreads();
+		} else {
+			if (n_short) tile_issue<R, J>(nxt, nb, nt, idx, val);
+			A acc = R::zero();
+			for (uint64_t k = cb + threadIdx.x; k < ct; k += kBlock)
+				acc = R::add(acc, Val<R>::mul(Val<R>::load(val, k), x[__builtin_nontemporal_load(idx + k)]));
+			acc = group_fold<kWave, R>(acc);
+			__syncthreads();
+			if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = acc;
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				A t = red[0];
+				for (int i = 1; i < kBlock / kWave; ++i) t = R::add(t, red[i]);
+				bool active = true;
+				if (MASK) active = (MASK == 1) == (mask[c0] != 0);
+				if (active) y[c0] = OVERWRITE ? static_cast<Y>(t) : static_cast<Y>(R::add(static_cast<A>(y[c0]), t));
+			}
+			__syncthreads();
+		}
+		// rotate: n -> c, m -> n
+		c0 = n0;
+		c1 = n1;
+		cb = nb;
+		ct = nt;
+		n0 = m0;
+		n1 = m1;
+		nb = mb;
+		nt = mt;
+		cur = nxt;
+	}
+}
+
 // Within one wavefront LDS operations execute in issue order; the fences and the
 // wave barrier keep the compiler from moving LDS accesses across this point.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -295,7 +491,8 @@ static void launch_vector(int G, int RR, const Storage &S, const void *val, cons
 }
 
 // variant (`rows`): 0/1 = persistent grid, 4 blocks per CU; 2 = persistent, 8 per
-// CU; 4 = one block per row tile; 8 = one block per tile in plain block order.
+// CU; 4 = one block per row tile; 8 = one block per tile in plain block order;
+// 16 / 32 = software-pipelined persistent grid with 4 / 2 blocks per CU.
 // tile (`lanes`): 0/8 = 2048-nonzero tiles, 16 = 4096-nonzero tiles.
 template <typename R, int MASK, bool OW>
 static void launch_stream(int variant, int tile, const Storage &S, const void *val, const void *x,
@@ -304,6 +501,18 @@ static void launch_stream(int variant, int tile, const Storage &S, const void *v
 	const size_t nrb = big ? S.n_row_blocks_l : S.n_row_blocks;
 	const uint32_t *rb = big ? S.row_blocks_l : S.row_blocks;
 	const uint64_t *rbase = big ? S.row_block_base_l : S.row_block_base;
+	if (variant == 16 || variant == 32) {
+		size_t grid = fill_blocks(variant == 16 ? 4 : 2);
+		if (grid > nrb) grid = nrb;
+		if (grid == 0) return;
+#define ALP_PIPE(JJ)                                                                                \
+	hipLaunchKernelGGL((k_spmv_stream_pipe<R, MASK, OW, JJ>), dim3(grid), dim3(kBlock), 0, s, nrb, rb, \
+	                   rbase, (const uint64_t *)S.off, (const uint32_t *)S.idx, val,                 \
+	                   (const typename R::X *)x, (typename R::Y *)y, mask)
+		if (big) ALP_PIPE(16); else ALP_PIPE(8);
+#undef ALP_PIPE
+		return;
+	}
 	size_t cap = variant >= 4 ? nrb : fill_blocks(variant == 2 ? 8 : 4);
 	size_t grid = nrb < cap ? nrb : cap;
 	if (grid == 0) return;
@@ -319,6 +528,44 @@ static void launch_stream(int variant, int tile, const Storage &S, const void *v
 #undef ALP_STREAM
 }
 
+// sliced ELL: U (`rows`) entries per lane in flight: 4, 8 or 16; `lanes` = 1 maps
+// slices to XCDs contiguously, 0 keeps plain block order
+template <typename R, int MASK, bool OW>
+static void launch_sell(int U, int xcd, const Storage &S, const void *sval, const void *x, void *y,
+                        const uint8_t *mask, hipStream_t s) {
+	const size_t per = kBlock / kSellRows;
+	const size_t grid = (S.n_slices + per - 1) / per;
+	if (grid == 0) return;
+#define ALP_SELL(UU, XX)                                                                           \
+	hipLaunchKernelGGL((k_spmv_sell<R, MASK, OW, UU, XX>), dim3(grid), dim3(kBlock), 0, s, S.nmajor, \
+	                   S.n_slices, (const uint64_t *)S.sell_off, (const uint32_t *)S.sell_idx, sval,   \
+	                   (const uint64_t *)S.off, (const typename R::X *)x, (typename R::Y *)y, mask)
+	if (xcd) {
+		if (U == 16) ALP_SELL(16, true); else if (U == 4) ALP_SELL(4, true); else ALP_SELL(8, true);
+	} else {
+		if (U == 16) ALP_SELL(16, false); else if (U == 4) ALP_SELL(4, false); else ALP_SELL(8, false);
+	}
+#undef ALP_SELL
+}
+
+// the sliced-ELL array that corresponds to a CRS value array (null: not built)
+static bool sell_values(const Storage &S, const void *val, const void **sval) {
+	if (!S.sell_off) return false;
+	if (val == nullptr) {
+		*sval = nullptr;
+		return true;
+	}
+	if (val == S.val && S.sell_val) {
+		*sval = S.sell_val;
+		return true;
+	}
+	if (val == S.val32 && S.sell_val32) {
+		*sval = S.sell_val32;
+		return true;
+	}
+	return false;
+}
+
 template <typename R, int MASK, bool OW>
 static void launch_wave(const Storage &S, const void *val, const void *x, void *y,
                         const uint8_t *mask, hipStream_t s) {
@@ -331,9 +578,10 @@ static void launch_wave(const Storage &S, const void *val, const void *x, void *
 	                   (const typename R::X *)x, (typename R::Y *)y, mask);
 }
 
-// without autotuning: CSR-stream (whole-block folds of long rows) for skewed
-// row lengths, CSR-vector otherwise
+// without autotuning: sliced ELL where it was built (regular rows), CSR-stream
+// (whole-block folds of long rows) for skewed row lengths, CSR-vector otherwise
 int auto_kernel(const Storage &S) {
+	if (S.sell_off) return 4;
 	return (S.max_len > 64 && (double)S.max_len > 8.0 * S.avg_len) ? 2 : 1;
 }
 
@@ -355,9 +603,13 @@ static void dispatch(const Storage &S, const void *val, unsigned descr, void *y,
 	int G = S.lanes ? S.lanes : default_lanes(S);
 	int RR = S.rows;  // CSR-stream: the launch variant
 	if (kernel == 0) kernel = auto_kernel(S);
-	if (RR == 0) RR = kernel == 2 ? 8 : 1;
+	const void *sval = nullptr;
+	if (kernel == 4 && !sell_values(S, val, &sval)) kernel = 2;  // no sliced-ELL copy of these values
+	if (RR == 0) RR = kernel == 2 ? 8 : kernel == 4 ? 8 : 1;
 #define ALP_GO(MM, OWV)                                                 \
-	if (kernel == 3)                                                    \
+	if (kernel == 4)                                                    \
+		launch_sell<R, MM, OWV>(RR, S.lanes, S, sval, x, y, mask, s);   \
+	else if (kernel == 3)                                               \
 		launch_wave<R, MM, OWV>(S, val, x, y, mask, s);                 \
 	else if (kernel == 2)                                               \
 		launch_stream<R, MM, OWV>(RR, S.lanes, S, val, x, y, mask, s);  \
@@ -444,11 +696,21 @@ ALP_EXPORT alp_rc_t alp_hip_vxm(alp_hip_matrix_t A, alp_semiring_t ring, unsigne
 }
 
 ALP_EXPORT alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes, int rows) {
-	if (!A || kernel < 0 || kernel > 3) return ALP_ILLEGAL;
+	if (!A || kernel < 0 || kernel > 4) return ALP_ILLEGAL;
+	if (kernel == 4) {
+		// sliced ELL: lanes 0/1 = plain / XCD-contiguous slice order, rows = entries in flight
+		if (!A->crs.sell_off || (lanes != 0 && lanes != 1) || (rows != 0 && rows != 4 && rows != 8 && rows != 16))
+			return ALP_ILLEGAL;
+		A->crs.kernel = A->ccs.kernel = kernel;
+		A->crs.lanes = A->ccs.lanes = lanes;
+		A->crs.rows = A->ccs.rows = rows;
+		return ALP_SUCCESS;
+	}
 	if (lanes != 0 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 &&
 	    lanes != 64)
 		return ALP_ILLEGAL;
-	if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && !(kernel == 2 && rows == 8))
+	if (rows != 0 && rows != 1 && rows != 2 && rows != 4 &&
+	    !(kernel == 2 && (rows == 8 || rows == 16 || rows == 32)))
 		return ALP_ILLEGAL;
 	A->crs.kernel = A->ccs.kernel = kernel;
 	A->crs.lanes = A->ccs.lanes = lanes;
@@ -458,11 +720,12 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, in
 
 ALP_EXPORT alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes, int *rows) {
 	if (!A) return ALP_ILLEGAL;
-	int k = A->crs.kernel, l = A->crs.lanes ? A->crs.lanes : default_lanes(A->crs);
+	int k = A->crs.kernel;
 	if (k == 0) k = auto_kernel(A->crs);
+	int l = k == 4 ? A->crs.lanes : (A->crs.lanes ? A->crs.lanes : default_lanes(A->crs));
 	if (kernel) *kernel = k;
 	if (lanes) *lanes = l;
-	if (rows) *rows = A->crs.rows ? A->crs.rows : (k == 2 ? 8 : 1);
+	if (rows) *rows = A->crs.rows ? A->crs.rows : (k == 2 || k == 4 ? 8 : 1);
 	return ALP_SUCCESS;
 }
 
@@ -490,8 +753,11 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 	for (int g : {2, 4, 8, 16, 32, 64})
 		for (int r : {1, 2, 4}) cands.push_back({1, g, r});
 	for (int t : {8, 16})
-		for (int v : {1, 2, 4, 8}) cands.push_back({2, t, v});
+		for (int v : {1, 2, 4, 8, 16, 32}) cands.push_back({2, t, v});
 	cands.push_back({3, 0, 0});
+	if (A->crs.sell_off)
+		for (int xcd : {0, 1})
+			for (int u : {4, 8, 16}) cands.push_back({4, xcd, u});
 	float best = 1e30f;
 	Cand pick = {1, default_lanes(A->crs), 1};
 	for (const Cand &c : cands) {

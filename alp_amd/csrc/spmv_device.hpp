@@ -6,6 +6,12 @@
 
 #include "alp_internal.hpp"
 
+// No contraction of a * b + c into FMAs in any kernel built on these helpers:
+// products and sums round as written, as in the reference backends (compiled
+// without contraction for the fixtures), so sequential per-row folds reproduce
+// them exactly. The kernels are bandwidth-bound; FMAs would buy nothing.
+#pragma clang fp contract(off)
+
 namespace alp {
 
 // ---- semirings -------------------------------------------------------------

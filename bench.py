@@ -65,6 +65,132 @@ def cpu_baseline(side: int, nnz: int, n: int) -> dict:
             "kind": "port", "sample": f"oracle orc_spmv_f64 on the {side}^3 stencil, median of {reps}"}
 
 
+def timed(fn, sync):
+    sync()
+    t0 = time.perf_counter()
+    out = fn()
+    sync()
+    return out, time.perf_counter() - t0
+
+
+def leg_c1(alp, L) -> dict:
+    """64^3 stencil, b = A 1, x0 ~ U[0,1) (numpy seed 42), 50 unpreconditioned
+    FP64 CG iterations; GB/s counts the SpMV overwrite form plus 72 B/row of
+    fused BLAS1 (update 48, direction 24) per iteration. The CPU reference runs
+    the same 50 iterations (from x0 = 0; the count, not x0, sets the work)."""
+    side = 64
+    A = alp.Matrix.stencil27(side, side, side)
+    n, nnz = A.nrows, A.nnz
+    b = alp.Vector(n, fill=0.0)
+    alp.mxv(b, A, alp.Vector(n, fill=1.0))
+    x0 = np.random.default_rng(42).random(n)
+    times = []
+    for _ in range(5):
+        x = alp.Vector.from_numpy(x0)
+        (rc, it, res), dt = timed(lambda: alp.cg(A, b, x, 50, 1e-300), alp.sync)
+        times.append(dt)
+    ms = 1e3 * sorted(times)[2]
+    per_it = spmv_bytes(n, nnz, accumulate=False) + 72 * n
+    out = {"ms": round(ms, 3), "iterations": it, "residual": res,
+           "gbs_per_iteration": round(per_it * it / (ms * 1e-3) / 1e9, 1)}
+    import oracle as O
+    if O.ref_available("omp"):
+        R = O.ref("omp")
+        it_ref = ctypes.c_size_t()
+        ref_ms = R.ref_time_stencil_cg(side, side, side, 50, 1e-300, ctypes.byref(it_ref))
+        out["cpu_reference_omp_ms"] = round(ref_ms, 2)
+        out["speedup_vs_cpu"] = round(ref_ms / ms, 1)
+    return out
+
+
+def leg_c2(alp, scale: int = 22) -> dict:
+    """R-MAT scale 22, edge factor 16, (a, b, c) = (.57, .19, .19), seed 1, symmetrised,
+    deduplicated, no self-loops; BFS from the highest-degree vertex; PageRank
+    (0.85, 1e-9)."""
+    n = 1 << scale
+    (src, dst), t_gen = timed(lambda: alp.rmat_edges(scale, 16 * n, seed=1), alp.sync)
+    G, t_build = timed(lambda: alp.graph_from_edges(n, src, dst), alp.sync)
+    del src, dst
+    nnz = G.nnz
+    s, deg = alp.max_degree_vertex(G)
+    alp.bfs(G, s)  # warm-up
+    times = []
+    for _ in range(3):
+        (lev, nl, reached), dt = timed(lambda: alp.bfs(G, s), alp.sync)
+        times.append(dt)
+    bfs_ms = 1e3 * sorted(times)[1]
+    (rc, pr, it, q), t_pr = timed(lambda: alp.pagerank(G, 0.85, 1e-9, 1000), alp.sync)
+    pr_bytes = 4 * nnz + 8 * (n + 1) + 32 * n
+    return {"n": n, "nnz": nnz, "source": s, "source_degree": deg, "gen_ms": round(1e3 * t_gen, 2),
+            "build_ms": round(1e3 * t_build, 2), "bfs_ms": round(bfs_ms, 3), "bfs_levels": nl,
+            "bfs_reached": reached, "bfs_gteps": round(nnz / 2 / (bfs_ms * 1e-3) / 1e9, 2),
+            "pagerank_rc": rc, "pagerank_iterations": it, "pagerank_ms": round(1e3 * t_pr, 2),
+            "pagerank_gbs": round(pr_bytes * it / t_pr / 1e9, 1)}
+
+
+def leg_c4(alp) -> dict:
+    """64 anisotropic 7-point Laplacians, sides U{16..96} per axis, coefficients
+    lognormal(0, 0.5) (numpy seed 2024), 8 N(0,1) right-hand sides each, to 1e-8."""
+    rng = np.random.default_rng(2024)
+    nx, ny, nz = (rng.integers(16, 97, 64) for _ in range(3))
+    cx, cy, cz = (rng.lognormal(0.0, 0.5, 64) for _ in range(3))
+    B = alp.Batch(nx, ny, nz, cx, cy, cz, 8)
+    w = B.autotune()
+    rhs = alp.Vector.from_numpy(rng.standard_normal(B.nrows * 8))
+    out = {"systems": 64, "rhs": 8, "rows": B.nrows, "nnz": B.nnz, "spmm_lanes": w}
+    for precision, tag in ((0, "fp64"), (1, "mixed")):
+        X = alp.Vector(B.nrows * 8, fill=0.0)
+        B.solve(rhs, X, 1e-8, 20000, precision)  # warm-up (graph capture)
+        (rc, its, rel, outer), dt = timed(lambda: B.solve(rhs, X, 1e-8, 20000, precision), alp.sync)
+        out[tag] = {"rc": rc, "ms": round(1e3 * dt, 2), "max_iterations": int(its.max()),
+                    "max_rel_residual": float(rel.max()), "refinements": outer}
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    B.free()
+    return out
+
+
+def leg_c5(alp, L, side: int = 128) -> dict:
+    """Differentiable solve on the 128^3 27-point graph: theta ~ lognormal(0, 1)
+    per edge (numpy seed 5), A(theta) = weighted Laplacian + 1e-2 I, b ~ N(0,1),
+    x* from a second theta (seed 6); loss ||x - x*||^2. Timed: forward CG, the
+    adjoint (transposed) solve and the dtheta scatter, to 1e-10."""
+    n = side ** 3
+    rng = np.random.default_rng(5)
+    theta = alp.Vector.from_numpy(rng.lognormal(0.0, 1.0, 13 * n))
+    theta_ref = alp.Vector.from_numpy(np.random.default_rng(6).lognormal(0.0, 1.0, 13 * n))
+    b = alp.Vector.from_numpy(rng.standard_normal(n))
+    Aref = alp.Matrix.conductance27(side, side, side, theta_ref, 1e-2)
+    xs = alp.Vector(n, fill=0.0)
+    alp.cg(Aref, b, xs, 100000, 1e-12)
+    Aref.free()
+    xstar = xs.numpy()
+    out = {"n": n, "edges_slots": 13 * n}
+    grads = {}
+    for precision, tag in ((0, "fp64"), (1, "mixed")):
+        A = alp.Matrix.conductance27(side, side, side, theta, 1e-2,
+                                     alp.MAT_WITH_CCS | (alp.MAT_WITH_F32 if precision else 0))
+        x = alp.Vector(n, fill=0.0)
+        lam = alp.Vector(n, fill=0.0)
+        dth = alp.Vector(13 * n)
+        alp.sync()
+        t0 = time.perf_counter()
+        rc1, it1, _ = alp.cg(A, b, x, 100000, 1e-10, precision)
+        t1 = time.perf_counter()
+        g = alp.Vector.from_numpy(2.0 * (x.numpy() - xstar))  # dL/dx (host, untimed below)
+        t2 = time.perf_counter()
+        rc2, it2, _ = alp.cg(A, g, lam, 100000, 1e-10, precision, transpose=True)
+        alp.check(L.alp_hip_grad_conductance27(side, side, side, lam.ptr, x.ptr, dth.ptr), "grad")
+        alp.sync()
+        t3 = time.perf_counter()
+        grads[tag] = dth.numpy()
+        out[tag] = {"rc": [rc1, rc2], "forward_iterations": it1, "adjoint_iterations": it2,
+                    "forward_ms": round(1e3 * (t1 - t0), 2), "adjoint_and_scatter_ms": round(1e3 * (t3 - t2), 2)}
+        A.free()
+    gm = np.max(np.abs(grads["fp64"]))
+    out["grad_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(grads["mixed"] - grads["fp64"])) / gm)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +199,8 @@ def main() -> int:
     ap.add_argument("--side", type=int, default=192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other workloads (64^3 CG, R-MAT graphs, batched CG, adjoint)")
     ap.add_argument("--kernel", type=int, default=0,
                     help="skip the autotuner: 1 = CSR-vector, 2 = CSR-stream (with --lanes/--rows)")
     ap.add_argument("--lanes", type=int, default=0)
@@ -185,6 +313,14 @@ def main() -> int:
             solves[f"hpcg_speedup_{tag}_vs_fp64"] = round(solves["hpcg_fp64"]["ms"] / solves[f"hpcg_{tag}"]["ms"], 3)
         H.free()
 
+    configs = {}
+    if not args.no_configs:
+        for name, fn in (("c1_cg50_64cubed", lambda: leg_c1(alp, L)), ("c2_rmat_s22", lambda: leg_c2(alp)),
+                         ("c4_batched_64x8", lambda: leg_c4(alp)), ("c5_adjoint_128cubed", lambda: leg_c5(alp, L))):
+            log(f"[bench] {name}")
+            configs[name] = fn()
+            log(f"[bench] {name}: {json.dumps(configs[name])}")
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -206,6 +342,7 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "mxv_overwrite": {"us_per_launch": round(ow_ms * 1e3, 2), "gbs": round(ow_gbs, 1)},
         "solves": solves,
+        "configs": configs,
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline")

@@ -104,10 +104,13 @@ def closed_form_A1(nx, ny, nz):
 
 
 KERNELS = [(1, 2, 1), (1, 4, 2), (1, 8, 4), (1, 16, 1), (1, 16, 4), (1, 32, 2), (1, 64, 1), (2, 0, 1),
-           (2, 0, 2), (2, 0, 4), (2, 8, 8), (2, 16, 1), (2, 16, 4), (2, 16, 8), (3, 0, 0)]
+           (2, 0, 2), (2, 0, 4), (2, 8, 8), (2, 16, 1), (2, 16, 4), (2, 16, 8), (2, 8, 16), (2, 16, 32),
+           (3, 0, 0)]
+# sliced ELL (kernel 4), built for matrices with regular rows: plain / XCD order, 4-16 in flight
+SELL = [(4, 0, 4), (4, 0, 8), (4, 0, 16), (4, 1, 8)]
 
 
-@pytest.mark.parametrize("kernel,lanes,rows", KERNELS)
+@pytest.mark.parametrize("kernel,lanes,rows", KERNELS + SELL)
 @pytest.mark.parametrize("sizes", [(64, 64, 64), (192, 192, 192)])
 def test_mxv_ones_closed_form_bitexact(A, sizes, kernel, lanes, rows):
     M = A.Matrix.stencil27(*sizes)
@@ -123,7 +126,7 @@ def test_mxv_ones_closed_form_bitexact(A, sizes, kernel, lanes, rows):
     assert np.array_equal(y.numpy(), closed_form_A1(*sizes))
 
 
-@pytest.mark.parametrize("kernel,lanes,rows", KERNELS)
+@pytest.mark.parametrize("kernel,lanes,rows", KERNELS + SELL)
 @pytest.mark.parametrize("name,sizes", [("mxv_8x8x8", (8, 8, 8)), ("mxv_5x9x7", (5, 9, 7)),
                                         ("mxv_16x16x16", (16, 16, 16))])
 def test_mxv_matches_reference_fixture(A, golden, name, sizes, kernel, lanes, rows):
@@ -137,6 +140,46 @@ def test_mxv_matches_reference_fixture(A, golden, name, sizes, kernel, lanes, ro
     yt = A.Vector.from_numpy(g["y0"])
     assert A.mxv(yt, M, x, descr=A.TRANSPOSE_MATRIX) == A.SUCCESS
     assert rel_inf(yt.numpy(), g["yt"]) <= FP64_RTOL
+
+
+@pytest.mark.parametrize("kernel,lanes,rows", SELL)
+@pytest.mark.parametrize("name,sizes", [("mxv_8x8x8", (8, 8, 8)), ("mxv_5x9x7", (5, 9, 7)),
+                                        ("mxv_16x16x16", (16, 16, 16))])
+def test_sell_mxv_bit_identical_to_reference(A, golden, name, sizes, kernel, lanes, rows):
+    """The sliced-ELL kernel folds each row sequentially in storage order without
+    FMA contraction: the reference's own order, so y is identical bit for bit."""
+    g = golden(name)
+    M = A.Matrix.stencil27(*sizes)
+    assert M.get_kernel()[0] == 4  # the default for a stencil
+    M.set_kernel(kernel, lanes, rows)
+    y = A.Vector.from_numpy(g["y0"])
+    assert A.mxv(y, M, A.Vector.from_numpy(g["x"])) == A.SUCCESS
+    assert np.array_equal(y.numpy(), g["y"])
+
+
+def test_sell_pattern_lorland_and_plus_pattern(A):
+    """Sliced ELL over a pattern matrix: lor-land with both mask modes, and the
+    plus/pattern row sums, against the oracle."""
+    st = O.stencil(20, 18, 16)
+    n = len(st.off) - 1
+    M = A.Matrix.from_crs(n, n, st.off, st.idx)  # pattern (no values)
+    assert M.get_kernel()[0] == 4
+    rng = np.random.default_rng(21)
+    for kernel, lanes, rows in SELL:
+        M.set_kernel(kernel, lanes, rows)
+        x = (rng.random(n) < 0.05).astype(np.uint8)
+        y0 = (rng.random(n) < 0.1).astype(np.uint8)
+        mask = (rng.random(n) < 0.5).astype(np.uint8)
+        for inv in (False, True):
+            y = A.Vector.from_numpy(y0)
+            assert A.mxv(y, M, A.Vector.from_numpy(x), ring=A.LOR_LAND_BOOL,
+                         descr=A.INVERT_MASK if inv else 0, mask=A.Vector.from_numpy(mask)) == A.SUCCESS
+            assert np.array_equal(y.numpy(), O.spmv_lorland(st, x, y0, mask=mask, invert_mask=inv))
+        xs = rng.random(n)
+        y = A.Vector(n, fill=0.0)
+        assert A.mxv(y, M, A.Vector.from_numpy(xs), ring=A.PLUS_PATTERN_F64) == A.SUCCESS
+        want = O.spmv(O.CSR(n, n, st.off, st.idx, np.ones(st.nnz)), xs)
+        assert np.max(np.abs(y.numpy() - want)) <= 1e-12 * np.max(np.abs(want))
 
 
 def test_mxv_random_matrix_vs_oracle_all_kernels(A):
