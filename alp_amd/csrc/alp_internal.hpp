@@ -55,7 +55,9 @@ struct Context {
 	double *h_pinned = nullptr;     // 64 pinned host doubles
 };
 constexpr int kMaxPartials = 4096;
-constexpr int kSellRows = 64;  // rows per sliced-ELL slice: one wavefront, one lane per row
+constexpr int kSellRows = 64;
+constexpr uint64_t kLongRow = 2048;  // CSR-stream: longer rows go through the chunk path
+constexpr uint64_t kChunk = 8192;    // nonzeros per chunk of a long row  // rows per sliced-ELL slice: one wavefront, one lane per row
 
 Context &ctx();
 alp_rc_t ensure_init();
@@ -185,6 +187,15 @@ struct Storage {
 	float *sell_val32 = nullptr;   // FP32 copy when val32 exists
 	size_t n_slices = 0;
 	size_t sell_entries = 0;
+	// rows longer than kLongRow nonzeros, for the CSR-stream path: split into
+	// chunks of kChunk nonzeros that separate blocks fold into partials, then one
+	// thread per long row folds its partials in chunk order (deterministic)
+	uint32_t *lr_rows = nullptr;   // n_long
+	uint64_t *lr_c0 = nullptr;     // n_long + 1: chunks of long row i are [lr_c0[i], lr_c0[i+1])
+	uint32_t *ch_row = nullptr;    // n_chunks: index (into lr_rows) of the chunk's row
+	uint64_t *ch_beg = nullptr;    // n_chunks: first nonzero of the chunk
+	void *ch_part = nullptr;       // n_chunks partial folds (8 bytes each)
+	size_t n_long = 0, n_chunks = 0;
 	// statistics for kernel selection
 	double avg_len = 0;
 	size_t max_len = 0;

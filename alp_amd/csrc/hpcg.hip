@@ -66,6 +66,16 @@ struct HpLevel {
 	uint32_t *inj = nullptr;             // coarse level: coarse new row -> fine new row
 	double *z = nullptr, *r = nullptr, *t = nullptr;
 	float *z32 = nullptr, *r32 = nullptr, *t32 = nullptr;
+	// sliced ELL of the permuted matrix: slices of at most kSellRows rows that never
+	// straddle a colour; slice s covers rows [srow[s], srow[s+1]) and its entries
+	// [soff[s], soff[s+1]) q-major (entry q of the slice's lane l at soff[s] + q*64 + l)
+	uint32_t *srow = nullptr;
+	uint64_t *soff = nullptr;
+	uint32_t *sidx = nullptr;
+	double *sval = nullptr;
+	float *sval32 = nullptr;
+	size_t nslices = 0;
+	std::vector<size_t> colour_slice_off;  // slices of colour c: [cso[c], cso[c+1])
 };
 
 struct HpState {
@@ -86,6 +96,7 @@ template <> struct Vec<double> {
 	static double *t(HpLevel &L) { return L.t; }
 	static const double *val(const HpLevel &L) { return L.val; }
 	static const double *diag(const HpLevel &L) { return L.diag; }
+	static const double *sval(const HpLevel &L) { return L.sval; }
 };
 template <> struct Vec<float> {
 	static float *z(HpLevel &L) { return L.z32; }
@@ -93,6 +104,7 @@ template <> struct Vec<float> {
 	static float *t(HpLevel &L) { return L.t32; }
 	static const float *val(const HpLevel &L) { return L.val32; }
 	static const float *diag(const HpLevel &L) { return L.diag32; }
+	static const float *sval(const HpLevel &L) { return L.sval32; }
 };
 
 // ---- epilogues of the tile fold ------------------------------------------------
@@ -210,6 +222,93 @@ __global__ __launch_bounds__(kBlock) void k_tiles_reduce(Tiles tl, const uint64_
 		const double s = block_sum_partials(partials, gridDim.x, lds);
 		if (threadIdx.x == 0) fin(s);
 	}
+}
+
+// Sliced-ELL row products over slices [s0, s1): one wavefront per slice, one
+// lane per row, the row folded sequentially in storage order (the order of the
+// reference's masked mxv, reference/blas2.hpp:463-489); ep.prefetch / ep.finish
+// as in tile_fold. The colour step of a proper colouring (EpSgs) reads z only at
+// other colours' rows and at the lane's own row, so the update needs no snapshot.
+template <typename T, int U, bool XCD, typename Ep>
+__global__ __launch_bounds__(kBlock) void k_sell_rows(size_t s0, size_t s1, const uint32_t *__restrict__ srow,
+                                                      const uint64_t *__restrict__ soff,
+                                                      const uint32_t *__restrict__ sidx,
+                                                      const T *__restrict__ sval,
+                                                      const uint64_t *__restrict__ off,
+                                                      const T *__restrict__ x, Ep ep, const int *done) {
+	if (done && *done) return;
+	const unsigned bl = XCD ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	const size_t sl = s0 + (size_t)bl * (kBlock / kSellRows) + threadIdx.x / kSellRows;
+	if (sl >= s1) return;  // whole wavefronts leave together
+	const unsigned lane = threadIdx.x % kSellRows;
+	const uint32_t row = srow[sl] + lane;
+	const bool act = row < srow[sl + 1];
+	const uint64_t e0 = soff[sl];
+	const unsigned w = (unsigned)((soff[sl + 1] - e0) / kSellRows);
+	unsigned len = 0;
+	if (act) {
+		len = (unsigned)(off[row + 1] - off[row]);
+		ep.prefetch(row);
+	}
+	T acc = T(0);
+	const uint32_t *ip = sidx + e0 + lane;
+	const T *vp = sval + e0 + lane;
+	for (unsigned q0 = 0; q0 < w; q0 += U) {
+		uint32_t j[U];
+		T v[U];
+#pragma unroll
+		for (int u = 0; u < U; ++u) {
+			const unsigned q = q0 + u;
+			j[u] = 0;
+			v[u] = T(0);
+			if (q < len) {
+				j[u] = __builtin_nontemporal_load(ip + (size_t)q * kSellRows);
+				v[u] = __builtin_nontemporal_load(vp + (size_t)q * kSellRows);
+			}
+		}
+		T xv[U];
+#pragma unroll
+		for (int u = 0; u < U; ++u) xv[u] = q0 + u < len ? x[j[u]] : T(0);
+#pragma unroll
+		for (int u = 0; u < U; ++u)
+			if (q0 + u < len) {
+				const T prod = v[u] * xv[u];
+				acc = prod + acc;
+			}
+	}
+	if (act) ep.finish(row, acc, len == 0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_hs_width(size_t ns, const uint32_t *srow, const uint64_t *off,
+                                                     uint64_t *sizes) {
+	grid_for_each<1>(ns, [&](size_t s) {
+		uint64_t w = 0;
+		for (uint32_t r = srow[s]; r < srow[s + 1]; ++r) {
+			const uint64_t l = off[r + 1] - off[r];
+			w = l > w ? l : w;
+		}
+		sizes[s] = w * kSellRows;
+	});
+	if (blockIdx.x == 0 && threadIdx.x == 0) sizes[ns] = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hs_fill(size_t ns, const uint32_t *srow, const uint64_t *off,
+                                                    const uint32_t *idx, const T *val, const uint64_t *soff,
+                                                    uint32_t *sidx, T *sval) {
+	grid_for_each<1>(ns * kSellRows, [&](size_t t) {
+		const size_t s = t / kSellRows, lane = t % kSellRows;
+		const uint32_t r = srow[s] + (uint32_t)lane;
+		const bool act = r < srow[s + 1];
+		const uint64_t e0 = soff[s], w = (soff[s + 1] - e0) / kSellRows;
+		const uint64_t b = act ? off[r] : 0, len = act ? off[r + 1] - b : 0;
+		for (uint64_t q = 0; q < w; ++q) {
+			const uint64_t e = e0 + q * kSellRows + lane;
+			const bool in = q < len;
+			if (sidx) sidx[e] = in ? idx[b + q] : 0u;
+			sval[e] = in ? val[b + q] : T(0);
+		}
+	});
 }
 
 // phase 2 of a snapshot colour step: z = ((r - t) + z d) / d over rows [r0, r1)
@@ -532,7 +631,7 @@ static void free_level(HpLevel &L) {
 	void *ps[] = {L.off, L.idx, L.val, L.val32, L.diag, L.diag32, L.perm, L.iperm, L.colour,
 	              L.ts[0].rb, L.ts[0].re, L.ts[0].base, L.ts[0].top, L.ts[1].rb, L.ts[1].re,
 	              L.ts[1].base, L.ts[1].top, L.inj, L.z, L.r, L.t,
-	              L.z32, L.r32, L.t32};
+	              L.z32, L.r32, L.t32, L.srow, L.soff, L.sidx, L.sval, L.sval32};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	L = HpLevel();
@@ -542,6 +641,62 @@ template <typename T>
 static alp_rc_t dmalloc(T **p, size_t count) {
 	ALP_HIP_TRY(hipMalloc(p, sizeof(T) * (count ? count : 1)));
 	return ALP_SUCCESS;
+}
+
+// The level's sliced-ELL copy (FP64, and FP32 when the level has FP32 values):
+// slices of kSellRows rows within each colour's row range.
+static alp_rc_t build_level_sell(HpLevel &L) {
+	hipStream_t s = stream();
+	std::vector<uint32_t> srow;
+	L.colour_slice_off.assign(L.ncolors + 1, 0);
+	for (int c = 0; c < L.ncolors; ++c) {
+		for (size_t r = L.color_off[c]; r < L.color_off[c + 1]; r += kSellRows) srow.push_back((uint32_t)r);
+		L.colour_slice_off[c + 1] = srow.size();
+	}
+	const size_t ns = srow.size();
+	srow.push_back((uint32_t)L.n);
+	L.nslices = ns;
+	uint64_t *sizes = nullptr;
+	void *tmp = nullptr;
+	alp_rc_t rc = ALP_SUCCESS;
+	do {
+		if ((rc = dmalloc(&L.srow, ns + 1)) || (rc = dmalloc(&L.soff, ns + 1)) || (rc = dmalloc(&sizes, ns + 1)))
+			break;
+		if (copy_sync(L.srow, srow.data(), 4 * (ns + 1), hipMemcpyHostToDevice) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if (ns == 0) break;
+		hipLaunchKernelGGL(k_hs_width, dim3(grid_for(ns)), dim3(kBlock), 0, s, ns, (const uint32_t *)L.srow,
+		                   (const uint64_t *)L.off, sizes);
+		size_t tb = 0;
+		if (hipcub::DeviceScan::ExclusiveSum(nullptr, tb, sizes, L.soff, ns + 1, s) != hipSuccess ||
+		    (rc = dmalloc((char **)&tmp, tb)) ||
+		    hipcub::DeviceScan::ExclusiveSum(tmp, tb, sizes, L.soff, ns + 1, s) != hipSuccess) {
+			if (!rc) rc = ALP_PANIC;
+			break;
+		}
+		uint64_t total = 0;
+		if (copy_sync(&total, L.soff + ns, sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if ((rc = dmalloc(&L.sidx, total)) || (rc = dmalloc(&L.sval, total))) break;
+		const unsigned g = grid_for(ns * kSellRows);
+		hipLaunchKernelGGL(k_hs_fill<double>, dim3(g), dim3(kBlock), 0, s, ns, (const uint32_t *)L.srow,
+		                   (const uint64_t *)L.off, (const uint32_t *)L.idx, (const double *)L.val,
+		                   (const uint64_t *)L.soff, L.sidx, L.sval);
+		if (L.val32) {
+			if ((rc = dmalloc(&L.sval32, total))) break;
+			hipLaunchKernelGGL(k_hs_fill<float>, dim3(g), dim3(kBlock), 0, s, ns, (const uint32_t *)L.srow,
+			                   (const uint64_t *)L.off, (const uint32_t *)L.idx, (const float *)L.val32,
+			                   (const uint64_t *)L.soff, (uint32_t *)nullptr, L.sval32);
+		}
+		if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) rc = ALP_PANIC;
+	} while (0);
+	if (tmp) hipFree(tmp);
+	if (sizes) hipFree(sizes);
+	return rc;
 }
 
 // Build one level: the stencil (device generator), its colouring, the colour
@@ -665,6 +820,8 @@ static alp_rc_t build_level(HpLevel &L, size_t nx, size_t ny, size_t nz, int col
 				break;
 			}
 		}
+		if (rc) break;
+		rc = build_level_sell(L);
 	} while (0);
 	hipStreamSynchronize(s);
 	if (tmp) hipFree(tmp);
@@ -691,6 +848,32 @@ static Tiles all_tiles(const HpLevel &L) {
 	return Tiles{S.rb, S.re, S.base, S.top, 0, S.count, tile_xcd()};
 }
 
+// Row products over slices [s0, s1) of the level's sliced-ELL copy in precision
+// T; false when the level has no copy in that precision (the tile path runs).
+static int g_use_sell = -1;
+template <typename T, typename Ep>
+static bool sell_rows(const HpLevel &L, size_t s0, size_t s1, const T *x, Ep ep, hipStream_t s,
+                      const int *done) {
+	if (g_use_sell < 0) {
+		const char *e = std::getenv("ALP_HIP_HPCG_SELL");
+		g_use_sell = e ? std::atoi(e) : 1;
+	}
+	const T *sval = Vec<T>::sval(L);
+	if (!g_use_sell || !sval || !L.soff) return false;
+	const size_t per = kBlock / kSellRows;
+	const size_t grid = (s1 - s0 + per - 1) / per;
+	if (grid == 0) return true;
+	if (tile_xcd())
+		hipLaunchKernelGGL((k_sell_rows<T, 8, true, Ep>), dim3(grid), dim3(kBlock), 0, s, s0, s1,
+		                   (const uint32_t *)L.srow, (const uint64_t *)L.soff, (const uint32_t *)L.sidx, sval,
+		                   (const uint64_t *)L.off, x, ep, done);
+	else
+		hipLaunchKernelGGL((k_sell_rows<T, 8, false, Ep>), dim3(grid), dim3(kBlock), 0, s, s0, s1,
+		                   (const uint32_t *)L.srow, (const uint64_t *)L.soff, (const uint32_t *)L.sidx, sval,
+		                   (const uint64_t *)L.off, x, ep, done);
+	return true;
+}
+
 template <typename T>
 static Tiles colour_tiles(const HpLevel &L, int c) {
 	const TileSet &S = L.ts[sizeof(T) == 8 ? 0 : 1];
@@ -711,14 +894,20 @@ static void colour_step(HpLevel &L, int c, hipStream_t s, const int *done, bool 
 		                   (const T *)Vec<T>::r(L), Vec<T>::diag(L), z, done);
 		return;
 	}
+	const size_t s0 = L.colour_slice_off.empty() ? 0 : L.colour_slice_off[c];
+	const size_t s1 = L.colour_slice_off.empty() ? 0 : L.colour_slice_off[c + 1];
 	if (L.proper) {
 		EpSgs<T> ep{Vec<T>::r(L), Vec<T>::diag(L), z, T(0), T(0), T(0)};
-		hipLaunchKernelGGL((k_tiles<T, EpSgs<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
-		                   (const uint64_t *)L.off, (const uint32_t *)L.idx, Vec<T>::val(L), (const T *)z, ep, done);
+		if (!sell_rows<T>(L, s0, s1, (const T *)z, ep, s, done))
+			hipLaunchKernelGGL((k_tiles<T, EpSgs<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
+			                   (const uint64_t *)L.off, (const uint32_t *)L.idx, Vec<T>::val(L), (const T *)z, ep,
+			                   done);
 	} else {
 		EpStore<T> ep{Vec<T>::t(L)};
-		hipLaunchKernelGGL((k_tiles<T, EpStore<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
-		                   (const uint64_t *)L.off, (const uint32_t *)L.idx, Vec<T>::val(L), (const T *)z, ep, done);
+		if (!sell_rows<T>(L, s0, s1, (const T *)z, ep, s, done))
+			hipLaunchKernelGGL((k_tiles<T, EpStore<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
+			                   (const uint64_t *)L.off, (const uint32_t *)L.idx, Vec<T>::val(L), (const T *)z, ep,
+			                   done);
 		const size_t r0 = L.color_off[c], r1 = L.color_off[c + 1];
 		hipLaunchKernelGGL(k_sgs_update<T>, dim3(grid_for(r1 - r0)), dim3(kBlock), 0, s, r0, r1,
 		                   (const T *)Vec<T>::r(L), (const T *)Vec<T>::t(L), Vec<T>::diag(L), z, done);
@@ -781,8 +970,9 @@ static void cg_step(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, hipStream
 	const Tiles tl = all_tiles<T>(F);
 	// Ap = A p on a one-tile-per-block grid, then p.Ap -> alpha
 	EpStore<T> ep{Ap};
-	hipLaunchKernelGGL((k_tiles<T, EpStore<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
-	                   (const uint64_t *)F.off, (const uint32_t *)F.idx, Vec<T>::val(F), (const T *)p, ep, done);
+	if (!sell_rows<T>(F, 0, F.nslices, (const T *)p, ep, s, done))
+		hipLaunchKernelGGL((k_tiles<T, EpStore<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
+		                   (const uint64_t *)F.off, (const uint32_t *)F.idx, Vec<T>::val(F), (const T *)p, ep, done);
 	hipLaunchKernelGGL(k_hp_alpha<T>, dim3(rg), dim3(kBlock), 0, s, n, (const T *)p, (const T *)Ap,
 	                   ctx().d_partials, ctx().d_tickets, st);
 	hipLaunchKernelGGL(k_hp_update<T>, dim3(rg), dim3(kBlock), 0, s, n, x, r, (const T *)p, (const T *)Ap,

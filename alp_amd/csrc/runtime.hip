@@ -171,9 +171,40 @@ static alp_rc_t upload_row_blocks(const std::vector<uint64_t> &off, size_t nrows
 // CSR-stream row tiles: greedy packing of whole rows into tiles of at most
 // 2048 (and, separately, 4096) nonzeros and 256 rows; a row longer than a tile
 // gets a tile of its own. Built on the host from the downloaded offsets; O(nrows).
+// long rows and their chunks (host, from the offsets)
+static alp_rc_t build_long_rows(Storage &S, const std::vector<uint64_t> &off) {
+	std::vector<uint32_t> rows, crow;
+	std::vector<uint64_t> c0, cbeg;
+	for (size_t r = 0; r < S.nmajor; ++r) {
+		const uint64_t len = off[r + 1] - off[r];
+		if (len <= kLongRow) continue;
+		c0.push_back(cbeg.size());
+		for (uint64_t k = off[r]; k < off[r + 1]; k += kChunk) {
+			crow.push_back((uint32_t)rows.size());
+			cbeg.push_back(k);
+		}
+		rows.push_back((uint32_t)r);
+	}
+	if (rows.empty()) return ALP_SUCCESS;
+	c0.push_back(cbeg.size());
+	S.n_long = rows.size();
+	S.n_chunks = cbeg.size();
+	ALP_HIP_TRY(hipMalloc(&S.lr_rows, 4 * S.n_long));
+	ALP_HIP_TRY(hipMalloc(&S.lr_c0, 8 * (S.n_long + 1)));
+	ALP_HIP_TRY(hipMalloc(&S.ch_row, 4 * S.n_chunks));
+	ALP_HIP_TRY(hipMalloc(&S.ch_beg, 8 * S.n_chunks));
+	ALP_HIP_TRY(hipMalloc(&S.ch_part, 8 * S.n_chunks));
+	ALP_HIP_TRY(copy_sync(S.lr_rows, rows.data(), 4 * S.n_long, hipMemcpyHostToDevice));
+	ALP_HIP_TRY(copy_sync(S.lr_c0, c0.data(), 8 * (S.n_long + 1), hipMemcpyHostToDevice));
+	ALP_HIP_TRY(copy_sync(S.ch_row, crow.data(), 4 * S.n_chunks, hipMemcpyHostToDevice));
+	ALP_HIP_TRY(copy_sync(S.ch_beg, cbeg.data(), 8 * S.n_chunks, hipMemcpyHostToDevice));
+	return ALP_SUCCESS;
+}
+
 static alp_rc_t build_row_blocks(Storage &S) {
 	std::vector<uint64_t> off(S.nmajor + 1);
 	ALP_HIP_TRY(copy_sync(off.data(), S.off, sizeof(uint64_t) * (S.nmajor + 1), hipMemcpyDeviceToHost));
+	ALP_TRY(build_long_rows(S, off));
 	ALP_TRY(upload_row_blocks(off, S.nmajor, 2048, &S.row_blocks, &S.row_block_base, &S.n_row_blocks));
 	ALP_TRY(upload_row_blocks(off, S.nmajor, 512, &S.row_blocks_w, &S.row_block_base_w,
 	                          &S.n_row_blocks_w, kWave));
@@ -332,6 +363,9 @@ void storage_free(Storage &S) {
 	if (S.sell_idx) hipFree(S.sell_idx);
 	if (S.sell_val) hipFree(S.sell_val);
 	if (S.sell_val32) hipFree(S.sell_val32);
+	void *lr[] = {S.lr_rows, S.lr_c0, S.ch_row, S.ch_beg, S.ch_part};
+	for (void *q : lr)
+		if (q) hipFree(q);
 	S = Storage();
 }
 

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
     const uint64_t *__restrict__ off,
     const uint32_t *__restrict__ idx, const void *__restrict__ vval,
     const typename R::X *__restrict__ x, typename R::Y *__restrict__ y,
-    const uint8_t *__restrict__ mask) {
+    const uint8_t *__restrict__ mask, int skip_long) {
 	using A = typename R::A;
 	using Y = typename R::Y;
 	using VT = typename Val<R>::T;
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
 			A acc = R::zero();
 			for (uint64_t q = rk + sub; q < re; q += tpr) acc = R::add(acc, tile[q - base]);
 			for (unsigned s = tpr / 2; s > 0; s >>= 1) acc = R::add(acc, shfl_xor(acc, s));
-			if (active && sub == 0) {
+			if (active && sub == 0 && !(skip_long && re - rk > kLongRow)) {
 				if (rk == re) {
 					if (OVERWRITE) y[r] = static_cast<Y>(R::zero());
 				} else {
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
 				}
 			}
 			__syncthreads();
-		} else {
+		} else if (!skip_long) {
 			// a single row longer than a tile: the whole block folds it
 			A acc = R::zero();
 			for (uint64_t k = base + threadIdx.x; k < top; k += kBlock)
@@ -490,6 +490,71 @@ static void launch_vector(int G, int RR, const Storage &S, const void *val, cons
 		launch_vector_rr<R, 0, OW, 4>(G, S, val, x, y, mask, s);
 }
 
+// Long rows (more than kLongRow nonzeros) on the CSR-stream path: block c folds
+// chunk c (at most kChunk nonzeros of one row) into part[c] with a fixed tree;
+// then thread i folds long row i's partials in chunk order and writes y. Many
+// blocks share a long row, so a power-law row no longer serialises one block.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_chunk_part(size_t nchunks, const uint32_t *__restrict__ ch_row,
+                                                       const uint64_t *__restrict__ ch_beg,
+                                                       const uint32_t *__restrict__ lr_rows,
+                                                       const uint64_t *__restrict__ off,
+                                                       const uint32_t *__restrict__ idx, const void *__restrict__ vval,
+                                                       const typename R::X *__restrict__ x,
+                                                       typename R::A *__restrict__ part) {
+	using A = typename R::A;
+	__shared__ A red[kBlock / kWave];
+	const auto *val = static_cast<const typename R::V *>(vval);
+	for (size_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+		const uint32_t r = lr_rows[ch_row[c]];
+		const uint64_t k0 = ch_beg[c], rend = off[r + 1];
+		const uint64_t k1 = k0 + kChunk < rend ? k0 + kChunk : rend;
+		A acc = R::zero();
+		for (uint64_t k = k0 + threadIdx.x; k < k1; k += kBlock)
+			acc = R::add(acc, Val<R>::mul(Val<R>::load(val, k), x[__builtin_nontemporal_load(idx + k)]));
+		acc = group_fold<kWave, R>(acc);
+		__syncthreads();
+		if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = acc;
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			A t = red[0];
+			for (int i = 1; i < kBlock / kWave; ++i) t = R::add(t, red[i]);
+			part[c] = t;
+		}
+	}
+}
+
+template <typename R, int MASK, bool OVERWRITE>
+__global__ __launch_bounds__(kBlock) void k_long_fold(size_t nlong, const uint32_t *__restrict__ lr_rows,
+                                                      const uint64_t *__restrict__ lr_c0,
+                                                      const typename R::A *__restrict__ part,
+                                                      typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
+	using A = typename R::A;
+	using Y = typename R::Y;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < nlong; i += (size_t)gridDim.x * kBlock) {
+		const uint32_t r = lr_rows[i];
+		if (MASK && (MASK == 1) != (mask[r] != 0)) continue;
+		A acc = R::zero();
+		for (uint64_t c = lr_c0[i]; c < lr_c0[i + 1]; ++c) acc = R::add(acc, part[c]);
+		y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(static_cast<A>(y[r]), acc));
+	}
+}
+
+template <typename R, int MASK, bool OW>
+static void launch_long_rows(const Storage &S, const void *val, const void *x, void *y, const uint8_t *mask,
+                             hipStream_t s) {
+	using A = typename R::A;
+	static_assert(sizeof(A) <= 8, "chunk partials hold 8 bytes");
+	size_t g = S.n_chunks < (size_t)fill_blocks(8) ? S.n_chunks : fill_blocks(8);
+	hipLaunchKernelGGL((k_chunk_part<R>), dim3(g), dim3(kBlock), 0, s, S.n_chunks, (const uint32_t *)S.ch_row,
+	                   (const uint64_t *)S.ch_beg, (const uint32_t *)S.lr_rows, (const uint64_t *)S.off,
+	                   (const uint32_t *)S.idx, val, (const typename R::X *)x, (A *)S.ch_part);
+	const size_t g2 = (S.n_long + kBlock - 1) / kBlock;
+	hipLaunchKernelGGL((k_long_fold<R, MASK, OW>), dim3(g2), dim3(kBlock), 0, s, S.n_long,
+	                   (const uint32_t *)S.lr_rows, (const uint64_t *)S.lr_c0, (const A *)S.ch_part,
+	                   (typename R::Y *)y, mask);
+}
+
 // variant (`rows`): 0/1 = persistent grid, 4 blocks per CU; 2 = persistent, 8 per
 // CU; 4 = one block per row tile; 8 = one block per tile in plain block order;
 // 16 / 32 = software-pipelined persistent grid with 4 / 2 blocks per CU.
@@ -516,16 +581,18 @@ static void launch_stream(int variant, int tile, const Storage &S, const void *v
 	size_t cap = variant >= 4 ? nrb : fill_blocks(variant == 2 ? 8 : 4);
 	size_t grid = nrb < cap ? nrb : cap;
 	if (grid == 0) return;
+	const int skip_long = S.n_long > 0 ? 1 : 0;
 #define ALP_STREAM(JJ, XX)                                                                      \
 	hipLaunchKernelGGL((k_spmv_stream<R, MASK, OW, JJ, XX>), dim3(grid), dim3(kBlock), 0, s, nrb, \
 	                   rb, rbase, (const uint64_t *)S.off, (const uint32_t *)S.idx, val,          \
-	                   (const typename R::X *)x, (typename R::Y *)y, mask)
+	                   (const typename R::X *)x, (typename R::Y *)y, mask, skip_long)
 	if (big) {
 		if (variant == 8) ALP_STREAM(16, false); else ALP_STREAM(16, true);
 	} else {
 		if (variant == 8) ALP_STREAM(8, false); else ALP_STREAM(8, true);
 	}
 #undef ALP_STREAM
+	if (skip_long) launch_long_rows<R, MASK, OW>(S, val, x, y, mask, s);
 }
 
 // sliced ELL: U (`rows`) entries per lane in flight: 4, 8 or 16; `lanes` = 1 maps
