@@ -127,6 +127,8 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_conductance27(alp_hip_matrix_t *A, size_t nx,
 	alp_rc_t rc = ALP_SUCCESS;
 	if (hipGetLastError() != hipSuccess) rc = ALP_PANIC;
 	if (!rc && M->crs.val32) rc = alp_hip_convert_f64_to_f32(M->nnz, (const double *)M->crs.val, M->crs.val32);
+	// the sliced-ELL copy was filled from the pattern's placeholder values
+	if (!rc) rc = storage_refresh_values(M->crs, M->value_type);
 	if (!rc && (flags & ALP_MAT_WITH_CCS)) rc = build_ccs_from_crs(M);
 	if (!rc && hipStreamSynchronize(stream()) != hipSuccess) rc = ALP_PANIC;
 	if (rc) {

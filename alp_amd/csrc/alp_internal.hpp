@@ -220,6 +220,7 @@ struct alp_hip_matrix_s {
 namespace alp {
 
 alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32);
+alp_rc_t storage_refresh_values(Storage &S, int value_type);
 void storage_free(Storage &S);
 alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A);
 

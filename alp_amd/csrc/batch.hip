@@ -93,13 +93,16 @@ __global__ __launch_bounds__(kBlock) void k_spmm(size_t ntiles, const uint32_t *
                                                  const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx,
                                                  const T *__restrict__ val, const T *__restrict__ X,
                                                  const T *__restrict__ Bm, T *__restrict__ Y, double *part,
-                                                 const int *inst_active, const uint32_t *tile_sys) {
+                                                 const int *sys_active, const uint32_t *tile_sys) {
 	constexpr int NR = W * RPL;
 	constexpr int kRows = kBlock / W;
 	__shared__ double red[kBlock / kWave][NR];
 	const int lane = threadIdx.x % W;
 	const unsigned grp = threadIdx.x / W;
 	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		// a system whose right-hand sides have all converged is frozen: skip its rows
+		// (the residual product RES always runs: the outer loop reads it for every system)
+		if (!RES && sys_active && !sys_active[tile_sys[t]]) continue;
 		const uint32_t r0 = tile_row0[t], r1 = tile_row1[t];
 		double acc_dot[RPL];
 #pragma unroll
@@ -158,8 +161,6 @@ __global__ __launch_bounds__(kBlock) void k_spmm(size_t ntiles, const uint32_t *
 			}
 		}
 	}
-	(void)inst_active;
-	(void)tile_sys;
 }
 
 // Per-system fold of the tile partials (tiles of system s: [tile0[s], tile0[s+1])),
@@ -209,9 +210,10 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_update(size_t ntiles, const uint32_t *tile_row0,
                                                      const uint32_t *tile_row1, const uint32_t *tile_sys, int nrhs,
                                                      T *X, T *R, const T *P, const T *AP, const BatchInst *in,
-                                                     double *part) {
+                                                     double *part, const int *sys_active) {
 	__shared__ double red[kBlock];
 	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		if (!sys_active[tile_sys[t]]) continue;  // uniform per block: the barriers below stay matched
 		const size_t base = (size_t)tile_row0[t] * nrhs;
 		const size_t cnt = (size_t)(tile_row1[t] - tile_row0[t]) * nrhs;
 		const int sys = tile_sys[t];
@@ -241,37 +243,45 @@ __global__ __launch_bounds__(kBlock) void k_b_update(size_t ntiles, const uint32
 
 // beta = rr / rz; stop tests (||r|| < tol * ||b||, conjugate_gradient.hpp:437)
 __global__ __launch_bounds__(kBlock) void k_b_beta(int nrhs, const double *part, const uint32_t *tile0,
-                                                   BatchInst *in, unsigned long long max_iter, unsigned *nactive) {
+                                                   BatchInst *in, unsigned long long max_iter, unsigned *nactive,
+                                                   int *sys_active) {
 	__shared__ double sums[kMaxRhs];
+	__shared__ int any;
 	const int s = blockIdx.x;
+	if (threadIdx.x == 0) any = 0;
 	system_sums(part, tile0, s, nrhs, sums);
-	if (threadIdx.x >= nrhs) return;
-	BatchInst &I = in[s * nrhs + threadIdx.x];
-	if (!I.active) {
-		I.beta = 0.0;
-		return;
+	if (threadIdx.x < nrhs) {
+		BatchInst &I = in[s * nrhs + threadIdx.x];
+		if (!I.active) {
+			I.beta = 0.0;
+		} else {
+			const double rr = sums[threadIdx.x];
+			I.rr = rr;
+			I.iter += 1;
+			const bool conv = std::sqrt(rr) < I.tol;
+			if (conv || I.iter >= max_iter) {
+				I.active = 0;
+				I.converged = conv ? 1 : 0;
+				I.beta = 0.0;
+			} else {
+				I.beta = rr / I.rz;
+				I.rz = rr;
+				atomicAdd(nactive, 1u);
+				atomicOr(&any, 1);
+			}
+		}
 	}
-	const double rr = sums[threadIdx.x];
-	I.rr = rr;
-	I.iter += 1;
-	const bool conv = std::sqrt(rr) < I.tol;
-	if (conv || I.iter >= max_iter) {
-		I.active = 0;
-		I.converged = conv ? 1 : 0;
-		I.beta = 0.0;
-	} else {
-		I.beta = rr / I.rz;
-		I.rz = rr;
-		atomicAdd(nactive, 1u);
-	}
+	__syncthreads();
+	if (threadIdx.x == 0) sys_active[s] = any;
 }
 
 // P = R + beta P for active instances
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_dir(size_t ntiles, const uint32_t *tile_row0, const uint32_t *tile_row1,
                                                   const uint32_t *tile_sys, int nrhs, T *P, const T *R,
-                                                  const BatchInst *in) {
+                                                  const BatchInst *in, const int *sys_active) {
 	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		if (!sys_active[tile_sys[t]]) continue;
 		const size_t base = (size_t)tile_row0[t] * nrhs;
 		const size_t cnt = (size_t)(tile_row1[t] - tile_row0[t]) * nrhs;
 		const int sys = tile_sys[t];
@@ -309,22 +319,30 @@ __global__ __launch_bounds__(kBlock) void k_b_start_part(size_t ntiles, const ui
 // inactive when the outer loop already converged it
 __global__ __launch_bounds__(kBlock) void k_b_start(int nrhs, const double *part, const uint32_t *tile0,
                                                     BatchInst *in, const int *outer_active, double reltol,
-                                                    unsigned *nactive) {
+                                                    unsigned *nactive, int *sys_active) {
 	__shared__ double sums[kMaxRhs];
+	__shared__ int any;
 	const int s = blockIdx.x;
+	if (threadIdx.x == 0) any = 0;
 	system_sums(part, tile0, s, nrhs, sums);
-	if (threadIdx.x >= nrhs) return;
-	const int i = s * nrhs + threadIdx.x;
-	BatchInst &I = in[i];
-	const double rz = sums[threadIdx.x];
-	I.rz = rz;
-	I.rr = rz;
-	I.iter = 0;
-	I.tol = reltol * std::sqrt(rz);
-	I.alpha = I.beta = 0.0;
-	I.converged = 0;
-	I.active = (outer_active[i] && rz > 0.0) ? 1 : 0;
-	if (I.active) atomicAdd(nactive, 1u);
+	if (threadIdx.x < nrhs) {
+		const int i = s * nrhs + threadIdx.x;
+		BatchInst &I = in[i];
+		const double rz = sums[threadIdx.x];
+		I.rz = rz;
+		I.rr = rz;
+		I.iter = 0;
+		I.tol = reltol * std::sqrt(rz);
+		I.alpha = I.beta = 0.0;
+		I.converged = 0;
+		I.active = (outer_active[i] && rz > 0.0) ? 1 : 0;
+		if (I.active) {
+			atomicAdd(nactive, 1u);
+			atomicOr(&any, 1);
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) sys_active[s] = any;
 }
 
 // outer (FP64) residual norms: rnorm = ||B - A X||, stop at tol * ||B||
@@ -412,6 +430,7 @@ struct alp_hip_batch_s {
 	unsigned *nactive = nullptr;
 	double *rnorm = nullptr, *bnorm = nullptr;
 	int *outer_active = nullptr;
+	int *sys_active = nullptr;        // 1 while any right-hand side of the system is active
 	double *R = nullptr, *P = nullptr, *AP = nullptr;
 	float *R32 = nullptr, *P32 = nullptr, *AP32 = nullptr, *X32 = nullptr;
 };
@@ -419,14 +438,15 @@ struct alp_hip_batch_s {
 namespace alp {
 
 template <typename T, bool DOT, bool RES>
-static void launch_spmm(alp_hip_batch_s *B, int W, const T *val, const T *X, const T *Bm, T *Y, hipStream_t s) {
+static void launch_spmm(alp_hip_batch_s *B, int W, const T *val, const T *X, const T *Bm, T *Y, hipStream_t s,
+                        const int *sys_active = nullptr) {
 	const size_t grid = B->ntiles < (size_t)fill_blocks(8) ? B->ntiles : fill_blocks(8);
 	const int nr = (int)B->nrhs;
 #define ALP_SPMM(WW, RPL)                                                                                 \
 	hipLaunchKernelGGL((k_spmm<T, WW, RPL, DOT, RES>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,        \
 	                   (const uint32_t *)B->tile_row0, (const uint32_t *)B->tile_row1,                  \
 	                   (const uint64_t *)B->off, (const uint32_t *)B->idx, val, X, Bm, Y, B->part,       \
-	                   (const int *)nullptr, (const uint32_t *)B->tile_sys)
+	                   sys_active, (const uint32_t *)B->tile_sys)
 	if (nr == 8) {
 		if (W == 8) ALP_SPMM(8, 1);
 		else if (W == 4) ALP_SPMM(4, 2);
@@ -452,7 +472,7 @@ static void launch_spmm(alp_hip_batch_s *B, int W, const T *val, const T *X, con
 
 static void free_batch(alp_hip_batch_s *B) {
 	void *ps[] = {B->off, B->idx, B->val, B->val32, B->row_sys, B->sys, B->tile_row0, B->tile_row1, B->tile_sys,
-	              B->tile0, B->part, B->inst, B->nactive, B->rnorm, B->bnorm, B->outer_active, B->R, B->P, B->AP,
+	              B->tile0, B->part, B->inst, B->nactive, B->rnorm, B->bnorm, B->outer_active, B->sys_active, B->R, B->P, B->AP,
 	              B->R32, B->P32, B->AP32, B->X32};
 	for (void *p : ps)
 		if (p) hipFree(p);
@@ -473,17 +493,18 @@ static alp_rc_t batch_iterate(alp_hip_batch_s *B, const T *val, T *X, T *R, T *P
 	const int nr = (int)B->nrhs;
 	const size_t g = B->ntiles < (size_t)fill_blocks(8) ? B->ntiles : fill_blocks(8);
 	auto iteration = [&](hipStream_t st) {
-		launch_spmm<T, true, false>(B, B->W, val, P, nullptr, AP, st);
+		launch_spmm<T, true, false>(B, B->W, val, P, nullptr, AP, st, B->sys_active);
 		hipLaunchKernelGGL(k_b_alpha, dim3(nsys), dim3(kBlock), 0, st, nr, (const double *)B->part,
 		                   (const uint32_t *)B->tile0, B->inst);
 		hipLaunchKernelGGL(k_b_update<T>, dim3(g), dim3(kBlock), 0, st, B->ntiles, (const uint32_t *)B->tile_row0,
 		                   (const uint32_t *)B->tile_row1, (const uint32_t *)B->tile_sys, nr, X, R, (const T *)P,
-		                   (const T *)AP, (const BatchInst *)B->inst, B->part);
+		                   (const T *)AP, (const BatchInst *)B->inst, B->part, (const int *)B->sys_active);
 		hipLaunchKernelGGL(k_b_beta, dim3(nsys), dim3(kBlock), 0, st, nr, (const double *)B->part,
-		                   (const uint32_t *)B->tile0, B->inst, (unsigned long long)max_iter, B->nactive);
+		                   (const uint32_t *)B->tile0, B->inst, (unsigned long long)max_iter, B->nactive,
+		                   B->sys_active);
 		hipLaunchKernelGGL(k_b_dir<T>, dim3(g), dim3(kBlock), 0, st, B->ntiles, (const uint32_t *)B->tile_row0,
 		                   (const uint32_t *)B->tile_row1, (const uint32_t *)B->tile_sys, nr, P, (const T *)R,
-		                   (const BatchInst *)B->inst);
+		                   (const BatchInst *)B->inst, (const int *)B->sys_active);
 	};
 	// capture `batch` iterations once; the active count of the last one is read back
 	const int batch = 8;
@@ -593,7 +614,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *out, size_t n
 		    (rc = bmalloc(&B->tile_row1, B->ntiles)) || (rc = bmalloc(&B->tile_sys, B->ntiles)) ||
 		    (rc = bmalloc(&B->tile0, nsys + 1)) || (rc = bmalloc(&B->part, B->ntiles * nrhs)) ||
 		    (rc = bmalloc(&B->inst, ninst)) || (rc = bmalloc(&B->nactive, 1)) || (rc = bmalloc(&B->rnorm, ninst)) ||
-		    (rc = bmalloc(&B->bnorm, ninst)) || (rc = bmalloc(&B->outer_active, ninst)) ||
+		    (rc = bmalloc(&B->bnorm, ninst)) || (rc = bmalloc(&B->outer_active, ninst)) || (rc = bmalloc(&B->sys_active, nsys)) ||
 		    (rc = bmalloc(&B->R, ntot)) || (rc = bmalloc(&B->P, ntot)) || (rc = bmalloc(&B->AP, ntot)) ||
 		    (rc = bmalloc(&B->R32, ntot)) || (rc = bmalloc(&B->P32, ntot)) || (rc = bmalloc(&B->AP32, ntot)) ||
 		    (rc = bmalloc(&B->X32, ntot)))
@@ -727,7 +748,8 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 		                   B->part);
 		ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
 		hipLaunchKernelGGL(k_b_start, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
-		                   (const uint32_t *)B->tile0, B->inst, (const int *)B->outer_active, tol, B->nactive);
+		                   (const uint32_t *)B->tile0, B->inst, (const int *)B->outer_active, tol, B->nactive,
+		                   B->sys_active);
 		ALP_LAUNCH_CHECK();
 		size_t it = 0;
 		ALP_TRY(batch_iterate<double>(B, B->val, d_X, B->R, B->P, B->AP, max_iter, &it));
@@ -755,7 +777,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 			ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
 			hipLaunchKernelGGL(k_b_start, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
 			                   (const uint32_t *)B->tile0, B->inst, (const int *)B->outer_active, inner_tol,
-			                   B->nactive);
+			                   B->nactive, B->sys_active);
 			ALP_LAUNCH_CHECK();
 			size_t it = 0;
 			ALP_TRY(batch_iterate<float>(B, B->val32, B->X32, B->R32, B->P32, B->AP32, max_iter, &it));

@@ -849,28 +849,19 @@ static Tiles all_tiles(const HpLevel &L) {
 }
 
 // Row products over slices [s0, s1) of the level's sliced-ELL copy in precision
-// T; false when the level has no copy in that precision (the tile path runs).
-static int g_use_sell = -1;
+// T (plain block order: a colour's slices are contiguous rows already); false
+// when the level has no copy in that precision (the tile path runs instead).
 template <typename T, typename Ep>
 static bool sell_rows(const HpLevel &L, size_t s0, size_t s1, const T *x, Ep ep, hipStream_t s,
                       const int *done) {
-	if (g_use_sell < 0) {
-		const char *e = std::getenv("ALP_HIP_HPCG_SELL");
-		g_use_sell = e ? std::atoi(e) : 1;
-	}
 	const T *sval = Vec<T>::sval(L);
-	if (!g_use_sell || !sval || !L.soff) return false;
+	if (!sval || !L.soff) return false;
 	const size_t per = kBlock / kSellRows;
 	const size_t grid = (s1 - s0 + per - 1) / per;
 	if (grid == 0) return true;
-	if (tile_xcd())
-		hipLaunchKernelGGL((k_sell_rows<T, 8, true, Ep>), dim3(grid), dim3(kBlock), 0, s, s0, s1,
-		                   (const uint32_t *)L.srow, (const uint64_t *)L.soff, (const uint32_t *)L.sidx, sval,
-		                   (const uint64_t *)L.off, x, ep, done);
-	else
-		hipLaunchKernelGGL((k_sell_rows<T, 8, false, Ep>), dim3(grid), dim3(kBlock), 0, s, s0, s1,
-		                   (const uint32_t *)L.srow, (const uint64_t *)L.soff, (const uint32_t *)L.sidx, sval,
-		                   (const uint64_t *)L.off, x, ep, done);
+	hipLaunchKernelGGL((k_sell_rows<T, 8, false, Ep>), dim3(grid), dim3(kBlock), 0, s, s0, s1,
+	                   (const uint32_t *)L.srow, (const uint64_t *)L.soff, (const uint32_t *)L.sidx, sval,
+	                   (const uint64_t *)L.off, x, ep, done);
 	return true;
 }
 

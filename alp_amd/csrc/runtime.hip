@@ -321,6 +321,25 @@ static alp_rc_t build_sell(Storage &S, int value_type, uint64_t nnz) {
 	return ALP_SUCCESS;
 }
 
+// Copies S's current CRS values (and FP32 copy) into its sliced-ELL copy, after a
+// caller rewrote the values in place; the index layout is unchanged.
+alp_rc_t storage_refresh_values(Storage &S, int value_type) {
+	if (!S.sell_off || S.n_slices == 0) return ALP_SUCCESS;
+	const unsigned g = stream_grid(S.n_slices * kSellRows);
+	if (value_type == ALP_VAL_F32)
+		hipLaunchKernelGGL(k_sell_fill<float>, dim3(g), dim3(kBlock), 0, stream(), S.nmajor, S.n_slices,
+		                   (const uint64_t *)S.off, (const uint32_t *)S.idx, (const float *)S.val,
+		                   (const float *)S.val32, (const uint64_t *)S.sell_off, S.sell_idx, (float *)S.sell_val,
+		                   S.sell_val32);
+	else if (value_type == ALP_VAL_F64)
+		hipLaunchKernelGGL(k_sell_fill<double>, dim3(g), dim3(kBlock), 0, stream(), S.nmajor, S.n_slices,
+		                   (const uint64_t *)S.off, (const uint32_t *)S.idx, (const double *)S.val,
+		                   (const float *)S.val32, (const uint64_t *)S.sell_off, S.sell_idx, (double *)S.sell_val,
+		                   S.sell_val32);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
 alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32) {
 	uint64_t total = 0;
 	ALP_HIP_TRY(copy_sync(&total, S.off + S.nmajor, sizeof(uint64_t), hipMemcpyDeviceToHost));

@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "CG solve time & SpMV GB/s (% HBM peak): mixed vs FP64 at 1e-10 rel residual"
 HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = {1: "csr-vector", 2: "csr-stream", 3: "csr-wave", 4: "sliced-ell"}
 
 
 def spmv_bytes(n: int, nnz: int, accumulate: bool = True) -> int:
@@ -63,6 +64,21 @@ def cpu_baseline(side: int, nnz: int, n: int) -> dict:
     ms = 1e3 * sorted(t)[reps // 2]
     return {"value": round(spmv_bytes(n, nnz) / (ms * 1e-3) / 1e9, 3), "unit": "GB/s", "cores": 1,
             "kind": "port", "sample": f"oracle orc_spmv_f64 on the {side}^3 stencil, median of {reps}"}
+
+
+def measured_traffic(kernel: int, lanes: int, rows: int):
+    """HBM bytes per launch of this SpMV configuration from the committed rocprofv3
+    FETCH_SIZE + WRITE_SIZE summaries (profiles/*_hbm_counters.md, calibrated on a
+    copy kernel of known size), or None when none covers it."""
+    import glob
+    import re
+    pat = re.compile(r"traffic_bytes_per_launch:\s*(\d+)\s*\(kernel=(\d+) lanes=(\d+) rows=(\d+), accumulate\)")
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_counters.md"))):
+        with open(path) as f:
+            for m in pat.finditer(f.read()):
+                if (int(m.group(2)), int(m.group(3)), int(m.group(4))) == (kernel, lanes, rows):
+                    return int(m.group(1)), os.path.relpath(path, ROOT)
+    return None
 
 
 def timed(fn, sync):
@@ -226,7 +242,7 @@ def main() -> int:
         kernel, lanes, rows = A.get_kernel()
     else:
         kernel, lanes, rows = A.autotune()
-    log(f"[bench] n={n} nnz={nnz} kernel={'csr-vector' if kernel == 1 else 'csr-stream'} lanes={lanes} rows={rows}")
+    log(f"[bench] n={n} nnz={nnz} kernel={KERNEL_NAMES.get(kernel, kernel)} lanes={lanes} rows={rows}")
     x = alp.Vector.from_numpy(np.random.default_rng(0).random(n))
     y = alp.Vector(n, fill=0.0)
 
@@ -336,14 +352,19 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"{side}^3 27-point stencil (n={n}, nnz={nnz}), FP64 plus-times "
                                f"grb::mxv y += A x on device CRS",
-                   "kernel": "csr-vector" if kernel == 1 else "csr-stream", "lanes": lanes, "rows": rows,
+                   "kernel": KERNEL_NAMES.get(kernel, str(kernel)), "lanes": lanes, "rows": rows,
                    "bytes_per_step": bytes_step},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "algorithmic_bytes_per_launch": bytes_step,
         "mxv_overwrite": {"us_per_launch": round(ow_ms * 1e3, 2), "gbs": round(ow_gbs, 1)},
         "solves": solves,
         "configs": configs,
     }
+    tr = measured_traffic(kernel, lanes, rows)
+    if tr is not None:
+        out["roofline"]["traffic"] = tr[0]
+        out["roofline"]["traffic_source"] = tr[1]
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline")
         out["cpu_baseline"] = cpu_baseline(side, nnz, n)

@@ -261,7 +261,11 @@ def test_pattern_row_sums(A):
 def test_autotune_picks_a_valid_kernel(A):
     M = A.Matrix.stencil27(64, 64, 64)
     k, lanes, rows = M.autotune()
-    assert k in (1, 2) and rows in (1, 2, 4)
+    valid = {1: lambda l, r: l in (2, 4, 8, 16, 32, 64) and r in (1, 2, 4),
+             2: lambda l, r: l in (8, 16) and r in (1, 2, 4, 8, 16, 32),
+             3: lambda l, r: True,
+             4: lambda l, r: l in (0, 1) and r in (4, 8, 16)}
+    assert k in valid and valid[k](lanes, rows), (k, lanes, rows)
     x = A.Vector(M.nrows, fill=1.0)
     y = A.Vector(M.nrows, fill=0.0)
     A.mxv(y, M, x)
@@ -356,3 +360,38 @@ def test_cg_to_1e10_fp64_and_mixed(A):
         r = bh - O.spmv(ref, xh)
         assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(bh), precision
         assert np.max(np.abs(xh - 1.0)) < 1e-7
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_cg_repeated_solves_reuse_cached_state(A, precision):
+    """The matrix keeps its CG work vectors and captured iteration graph between
+    solves: a repeated solve with the same vectors is bit-identical to the first;
+    new vectors, a new kernel choice and the transposed solve each give the right
+    answer (the cache is keyed on all of them)."""
+    M = A.Matrix.stencil27(24, 20, 16, flags=A.MAT_WITH_F32)
+    n = M.nrows
+    rng = np.random.default_rng(17)
+    bh = rng.standard_normal(n)
+    ref = O.stencil(24, 20, 16)
+    b = A.Vector.from_numpy(bh)
+    runs = []
+    for _ in range(3):
+        x = A.Vector(n, fill=0.0)
+        rc, it, res = A.cg(M, b, x, 5000, 1e-10, precision)
+        assert rc == A.SUCCESS
+        runs.append((x.numpy(), it, res))
+    # same b, fresh x each time: the x allocations may or may not reuse an address
+    for xh, it, res in runs[1:]:
+        assert np.array_equal(xh, runs[0][0]) and it == runs[0][1] and res == runs[0][2]
+    b2h = rng.standard_normal(n)
+    b2 = A.Vector.from_numpy(b2h)
+    x2 = A.Vector(n, fill=0.0)
+    assert A.cg(M, b2, x2, 5000, 1e-10, precision)[0] == A.SUCCESS
+    assert np.linalg.norm(b2h - O.spmv(ref, x2.numpy())) <= 1.05e-10 * np.linalg.norm(b2h)
+    M.set_kernel(2, 8, 8)
+    x3 = A.Vector(n, fill=0.0)
+    assert A.cg(M, b2, x3, 5000, 1e-10, precision)[0] == A.SUCCESS
+    assert np.linalg.norm(b2h - O.spmv(ref, x3.numpy())) <= 1.05e-10 * np.linalg.norm(b2h)
+    x4 = A.Vector(n, fill=0.0)
+    assert A.cg(M, b2, x4, 5000, 1e-10, precision, transpose=True)[0] == A.SUCCESS
+    assert np.linalg.norm(b2h - O.spmv(ref, x4.numpy())) <= 1.05e-10 * np.linalg.norm(b2h)
