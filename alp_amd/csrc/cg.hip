@@ -5,12 +5,11 @@
 // + foldl + dot + foldr + foldr (:387-471), each a full pass over n with a host
 // round trip for every scalar. Here one iteration is three kernels and no host
 // round trip; all scalars live in a device state block:
-//   k_cg_spmv_dot  temp = A u and the partial sums of temp.u in one pass; the
-//                  last block to finish forms alpha = sigma / (temp.u)
-//   k_cg_update    x += alpha u, r -= alpha temp and the partials of r.r; the last
-//                  block forms the residual, the stopping test (||r|| < tol ||b||,
-//                  :378,437) and beta = r.r / sigma
-//   k_cg_dir       u = r + beta u
+//   k_cg_spmv_dot  temp = A u and one partial sum of temp.u per block, in one pass
+//   k_cg_update    every block folds the temp.u partials into alpha = sigma / (temp.u);
+//                  x += alpha u, r -= alpha temp and one partial of r.r per block
+//   k_cg_dir       every block folds r.r, applies the stopping test (||r|| < tol ||b||,
+//                  :378,437) and, unless done, u = r + (r.r / sigma) u
 // Every reduction has a fixed tree (fixed grid, fixed per-thread strides, the
 // partials folded in index order), so results are run-to-run deterministic.
 // Iterations are enqueued in batches replayed from one captured hipGraph; the
@@ -29,21 +28,24 @@
 
 namespace alp {
 
-struct CgState {
-	double sigma;      // r.r of the current residual
-	double alpha;
-	double ratio;      // beta / sigma of the direction update
+// Per-iteration scalars. Iteration j reads slot j % 2 and only block 0 of its last
+// kernel writes slot (j + 1) % 2, so no kernel reads a slot that one of its own
+// blocks writes: reductions need no grid-wide synchronisation inside a kernel.
+struct CgScal {
+	double sigma;      // r.r (or r.z when preconditioned) of the current residual
 	double rr;         // last r.r (the reference's `residual`)
+	unsigned long long iter;
+	int done;
+	int converged;
+};
+
+struct CgState {
+	CgScal s[2];
 	double tol_abs;    // tol * sqrt(b.b)
 	double bnorm2;
 	double r0norm;     // sqrt of the first r.r (inner solves)
 	double pad;
-	unsigned long long iter;
 	unsigned long long max_iter;
-	int done;
-	int converged;
-	int precond;       // preconditioned: sigma and ratio come from r.z (k_pcg_rz)
-	int pad2;
 };
 
 template <typename T>
@@ -54,18 +56,28 @@ struct CgArgs {
 	const T *val;
 	T *x, *r, *u, *temp;
 	const T *b;
-	double *partials;
+	double *partials;  // kMaxPartials doubles: [0, half) p.Ap, [half, 2 half) r.r (or r.z)
 	unsigned *ticket;
 	CgState *st;
 };
 
+constexpr unsigned kHalfPartials = kMaxPartials / 2;
+
+// Reductions write one partial per block (a fixed tree inside the block); the
+// consumer kernel folds the partials in index order in every block. This costs
+// each consuming block one read of at most 16 KB (L2-resident) and avoids the
+// device-scope release/acquire a last-block ticket needs, which on this part
+// writes back and invalidates the L2 in every block.
+__device__ __forceinline__ double fold_partials(const double *p, unsigned n, double *lds) {
+	return block_sum_partials(p, n, lds);
+}
+
 // ---- kernels ------------------------------------------------------------------
 
 template <int G, int RR, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a) {
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a, const CgScal *in) {
 	__shared__ double lds[kBlock / kWave];
-	__shared__ bool is_last;
-	if (a.st->done) return;
+	if (in->done) return;
 	constexpr unsigned kGroups = kBlock / G;
 	constexpr unsigned kRows = kGroups * RR;
 	const size_t nchunks = (a.n + kRows - 1) / kRows;
@@ -100,34 +112,27 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a) {
 	}
 	part = block_sum(part, lds);
 	if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
-	if (last_block(a.ticket, &is_last)) {
-		const double pAp = block_sum_partials(a.partials, gridDim.x, lds);
-		if (threadIdx.x == 0) a.st->alpha = a.st->sigma / pAp;
-	}
 }
 
-// generic-SpMV path: only the dot (temp.u) and alpha
+// generic-SpMV path: the partials of temp.u
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_dot_alpha(CgArgs<T> a) {
+__global__ __launch_bounds__(kBlock) void k_cg_dot(CgArgs<T> a, const CgScal *in) {
 	__shared__ double lds[kBlock / kWave];
-	__shared__ bool is_last;
-	if (a.st->done) return;
+	if (in->done) return;
 	double part = 0.0;
 	grid_for_each<4>(a.n, [&](size_t i) { part += (double)a.temp[i] * (double)a.u[i]; });
 	part = block_sum(part, lds);
 	if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
-	if (last_block(a.ticket, &is_last)) {
-		const double pAp = block_sum_partials(a.partials, gridDim.x, lds);
-		if (threadIdx.x == 0) a.st->alpha = a.st->sigma / pAp;
-	}
 }
 
+// alpha = sigma / p.Ap (folded from npap partials); x += alpha u, r -= alpha temp,
+// and the partials of r.r
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a) {
+__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a, const CgScal *in, unsigned npap) {
 	__shared__ double lds[kBlock / kWave];
-	__shared__ bool is_last;
-	if (a.st->done) return;
-	const T alpha = (T)a.st->alpha;
+	if (in->done) return;
+	const double pAp = fold_partials(a.partials, npap, lds);
+	const T alpha = (T)(in->sigma / pAp);
 	double part = 0.0;
 	grid_for_each<4>(a.n, [&](size_t i) {
 		a.x[i] = alpha * a.u[i] + a.x[i];
@@ -136,39 +141,43 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a) {
 		part += (double)ri * (double)ri;
 	});
 	part = block_sum(part, lds);
-	if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
-	if (last_block(a.ticket, &is_last)) {
-		const double rr = block_sum_partials(a.partials, gridDim.x, lds);
-		if (threadIdx.x == 0) {
-			CgState *st = a.st;
-			st->rr = rr;
-			st->iter += 1;
-			const bool conv = std::sqrt(rr) < st->tol_abs;
-			if (conv || st->iter >= st->max_iter) {
-				st->done = 1;
-				st->converged = conv ? 1 : 0;
-			} else if (!st->precond) {
-				st->ratio = rr / st->sigma;
-				st->sigma = rr;
-			}
-		}
+	if (threadIdx.x == 0) a.partials[kHalfPartials + blockIdx.x] = part;
+}
+
+// r.r, the stopping test (sqrt(r.r) < tol ||b||, or the iteration limit;
+// conjugate_gradient.hpp:378,437) and, unless done, u = r + (r.r / sigma) u;
+// block 0 writes the next slot
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs<T> a, const CgScal *in, CgScal *out, unsigned nrr) {
+	__shared__ double lds[kBlock / kWave];
+	if (in->done) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) *out = *in;
+		return;
+	}
+	const double rr = fold_partials(a.partials + kHalfPartials, nrr, lds);
+	const unsigned long long iter = in->iter + 1;
+	const bool conv = std::sqrt(rr) < a.st->tol_abs;
+	const bool done = conv || iter >= a.st->max_iter;
+	if (!done) {
+		const T ratio = (T)(rr / in->sigma);
+		grid_for_each<4>(a.n, [&](size_t i) { a.u[i] = ratio * a.u[i] + a.r[i]; });
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		CgScal o;
+		o.sigma = done ? in->sigma : rr;
+		o.rr = rr;
+		o.iter = iter;
+		o.done = done ? 1 : 0;
+		o.converged = conv ? 1 : 0;
+		*out = o;
 	}
 }
 
+// prelude: r = b - temp (temp = A x), u = r, partials of r.r and of bb.bb where
+// bb is the right-hand side used for the stopping rule
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs<T> a) {
-	if (a.st->done) return;
-	const T ratio = (T)a.st->ratio;
-	grid_for_each<4>(a.n, [&](size_t i) { a.u[i] = ratio * a.u[i] + a.r[i]; });
-}
-
-// prelude: r = b - temp (temp = A x), u = r, sigma = r.r, bnorm2 = bb.bb where bb
-// is the right-hand side used for the stopping rule
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs<T> a, const double *bb_f64,
-                                                   const T *bb_t, double tol, size_t max_iter) {
+__global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs<T> a, const double *bb_f64, const T *bb_t) {
 	__shared__ double lds[kBlock / kWave];
-	__shared__ bool is_last;
 	double prr = 0.0, pbb = 0.0;
 	grid_for_each<4>(a.n, [&](size_t i) {
 		const T ri = a.b[i] - a.temp[i];
@@ -182,58 +191,72 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs<T> a, const double *b
 	pbb = block_sum(pbb, lds);
 	if (threadIdx.x == 0) {
 		a.partials[blockIdx.x] = prr;
-		a.partials[gridDim.x + blockIdx.x] = pbb;
-	}
-	if (last_block(a.ticket, &is_last)) {
-		const double rr = block_sum_partials(a.partials, gridDim.x, lds);
-		const double bn = block_sum_partials(a.partials + gridDim.x, gridDim.x, lds);
-		if (threadIdx.x == 0) {
-			CgState *st = a.st;
-			st->sigma = rr;
-			st->rr = rr;
-			st->bnorm2 = bn;
-			st->tol_abs = tol * std::sqrt(bn);
-			st->r0norm = std::sqrt(rr);
-			st->iter = 0;
-			st->max_iter = max_iter;
-			st->done = 0;
-			st->converged = 0;
-			st->ratio = 0.0;
-			st->alpha = 0.0;
-			st->precond = 0;
-		}
+		a.partials[kHalfPartials + blockIdx.x] = pbb;
 	}
 }
 
-// Preconditioned CG (conjugate_gradient.hpp:342-352,441-471): after z = M^-1 r,
-// rz = r.z; on the first call sigma = rz and the direction restarts (u = z),
-// afterwards ratio = rz / sigma and sigma = rz.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_rz(CgArgs<T> a, const T *z, int first) {
+// one block: slot 0 from the prelude's partials (sigma = r.r, tol *= sqrt(b.b))
+__global__ __launch_bounds__(kBlock) void k_cg_init_fin(CgState *st, const double *partials, unsigned n, double tol,
+                                                       unsigned long long max_iter) {
 	__shared__ double lds[kBlock / kWave];
-	__shared__ bool is_last;
-	if (a.st->done) return;
+	const double rr = fold_partials(partials, n, lds);
+	const double bn = fold_partials(partials + kHalfPartials, n, lds);
+	if (threadIdx.x == 0) {
+		st->s[0] = CgScal{rr, rr, 0ull, 0, 0};
+		st->s[1] = CgScal{rr, rr, 0ull, 0, 0};
+		st->bnorm2 = bn;
+		st->tol_abs = tol * std::sqrt(bn);
+		st->r0norm = std::sqrt(rr);
+		st->max_iter = max_iter;
+	}
+}
+
+// Preconditioned CG (conjugate_gradient.hpp:342-352,441-471). After the stopping
+// test (k_cg_check: the next slot gets r.r, the iteration count and done, sigma
+// unchanged) the host applies z = M^-1 r; k_pcg_rz forms the partials of r.z and
+// k_pcg_dir folds them: on the first call sigma = r.z and u = z, afterwards
+// u = z + (r.z / sigma) u and sigma = r.z.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_check(CgArgs<T> a, const CgScal *in, CgScal *out, unsigned nrr) {
+	__shared__ double lds[kBlock / kWave];
+	if (blockIdx.x != 0) return;
+	if (in->done) {
+		if (threadIdx.x == 0) *out = *in;
+		return;
+	}
+	const double rr = fold_partials(a.partials + kHalfPartials, nrr, lds);
+	const unsigned long long iter = in->iter + 1;
+	const bool conv = std::sqrt(rr) < a.st->tol_abs;
+	const bool done = conv || iter >= a.st->max_iter;
+	if (threadIdx.x == 0) *out = CgScal{in->sigma, rr, iter, done ? 1 : 0, conv ? 1 : 0};
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_rz(CgArgs<T> a, const T *z, const CgScal *in) {
+	__shared__ double lds[kBlock / kWave];
+	if (in->done) return;
 	double part = 0.0;
 	grid_for_each<4>(a.n, [&](size_t i) { part += (double)a.r[i] * (double)z[i]; });
 	part = block_sum(part, lds);
-	if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
-	if (last_block(a.ticket, &is_last)) {
-		const double rz = block_sum_partials(a.partials, gridDim.x, lds);
-		if (threadIdx.x == 0) {
-			CgState *st = a.st;
-			st->precond = 1;
-			st->ratio = first ? 0.0 : rz / st->sigma;
-			st->sigma = rz;
-		}
-	}
+	if (threadIdx.x == 0) a.partials[kHalfPartials + blockIdx.x] = part;
 }
 
-// u = z + ratio u
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir(CgArgs<T> a, const T *z) {
-	if (a.st->done) return;
-	const T ratio = (T)a.st->ratio;
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(CgArgs<T> a, const T *z, const CgScal *in, CgScal *out,
+                                                   unsigned nrz, int first) {
+	__shared__ double lds[kBlock / kWave];
+	if (in->done) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) *out = *in;
+		return;
+	}
+	const double rz = fold_partials(a.partials + kHalfPartials, nrz, lds);
+	const T ratio = first ? T(0) : (T)(rz / in->sigma);
 	grid_for_each<4>(a.n, [&](size_t i) { a.u[i] = ratio * a.u[i] + z[i]; });
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		CgScal o = *in;
+		o.sigma = rz;
+		*out = o;
+	}
 }
 
 // ---- iterative refinement helpers (FP64 outer loop) -----------------------------
@@ -364,6 +387,18 @@ static CgSlot &cg_slot(alp_hip_matrix_s *A, int id) {
 	return static_cast<CgCache *>(A->solver_cache)->slot[id];
 }
 
+// r = b - temp (temp = A x), u = r; slot 0 (and 1) from r.r; tol_abs = tol ||bb||
+template <typename T>
+static alp_rc_t cg_prelude(const CgArgs<T> &a, const double *bb64, const T *bbT, double tol, size_t max_iter,
+                           hipStream_t s) {
+	const unsigned g = vec_grid(a.n);
+	hipLaunchKernelGGL(k_cg_init<T>, dim3(g), dim3(kBlock), 0, s, a, bb64, bbT);
+	hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kBlock), 0, s, a.st, (const double *)a.partials, g, tol,
+	                   (unsigned long long)max_iter);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
 template <typename T>
 struct CgRunner {
 	const Storage &S;
@@ -384,22 +419,30 @@ struct CgRunner {
 		const size_t nchunks = (a.n + rows_per - 1) / rows_per;
 		size_t g = fill_blocks(8);
 		if (g > nchunks) g = nchunks;
+		if (g > kHalfPartials) g = kHalfPartials;
 		if (g < 1) g = 1;
 		gspmv = static_cast<unsigned>(g);
 		gvec = vec_grid(a.n);
 	}
 
-	// the part of an iteration up to the stopping test: temp = A u, alpha, x, r, r.r
-	alp_rc_t step(hipStream_t s) {
+	const CgScal *slot_in(int p) const { return &a.st->s[p]; }
+	CgScal *slot_out(int p) const { return &a.st->s[1 - p]; }
+
+	// the part of an iteration up to the stopping test, reading slot p: temp = A u,
+	// the partials of p.Ap, alpha, x, r and the partials of r.r
+	alp_rc_t step(hipStream_t s, int p) {
+		const CgScal *in = slot_in(p);
+		unsigned npap = gvec;
 		if (fused) {
-#define ALP_CG_CASE(GG)                                                                          \
-	case GG:                                                                                     \
-		if (RR == 4)                                                                             \
-			hipLaunchKernelGGL((k_cg_spmv_dot<GG, 4, T>), dim3(gspmv), dim3(kBlock), 0, s, a);   \
-		else if (RR == 2)                                                                        \
-			hipLaunchKernelGGL((k_cg_spmv_dot<GG, 2, T>), dim3(gspmv), dim3(kBlock), 0, s, a);   \
-		else                                                                                     \
-			hipLaunchKernelGGL((k_cg_spmv_dot<GG, 1, T>), dim3(gspmv), dim3(kBlock), 0, s, a);   \
+			npap = gspmv;
+#define ALP_CG_CASE(GG)                                                                               \
+	case GG:                                                                                          \
+		if (RR == 4)                                                                                  \
+			hipLaunchKernelGGL((k_cg_spmv_dot<GG, 4, T>), dim3(gspmv), dim3(kBlock), 0, s, a, in);    \
+		else if (RR == 2)                                                                             \
+			hipLaunchKernelGGL((k_cg_spmv_dot<GG, 2, T>), dim3(gspmv), dim3(kBlock), 0, s, a, in);    \
+		else                                                                                          \
+			hipLaunchKernelGGL((k_cg_spmv_dot<GG, 1, T>), dim3(gspmv), dim3(kBlock), 0, s, a, in);    \
 		break;
 			switch (G) {
 				ALP_CG_CASE(2)
@@ -416,25 +459,28 @@ struct CgRunner {
 			ALP_TRY(spmv(S, vt,
 			             std::is_same<T, float>::value ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64,
 			             ALP_DESC_OVERWRITE, a.temp, a.u, nullptr, s));
-			hipLaunchKernelGGL(k_cg_dot_alpha<T>, dim3(gvec), dim3(kBlock), 0, s, a);
+			hipLaunchKernelGGL(k_cg_dot<T>, dim3(gvec), dim3(kBlock), 0, s, a, in);
 		}
-		hipLaunchKernelGGL(k_cg_update<T>, dim3(gvec), dim3(kBlock), 0, s, a);
+		hipLaunchKernelGGL(k_cg_update<T>, dim3(gvec), dim3(kBlock), 0, s, a, in, npap);
 		ALP_LAUNCH_CHECK();
 		return ALP_SUCCESS;
 	}
 
-	alp_rc_t iteration(hipStream_t s) {
-		ALP_TRY(step(s));
-		hipLaunchKernelGGL(k_cg_dir<T>, dim3(gvec), dim3(kBlock), 0, s, a);
+	// one whole iteration from slot p into slot 1 - p
+	alp_rc_t iteration(hipStream_t s, int p) {
+		ALP_TRY(step(s, p));
+		hipLaunchKernelGGL(k_cg_dir<T>, dim3(gvec), dim3(kBlock), 0, s, a, slot_in(p), slot_out(p), gvec);
 		ALP_LAUNCH_CHECK();
 		return ALP_SUCCESS;
 	}
 
-	// run until done; batches of `batch` iterations replayed from a captured graph,
-	// reused from `slot` when it was captured for the same storage, vectors and
-	// kernel choice (and kept there for the next call)
+	// run until done, starting from slot 0; batches of `batch` (even) iterations
+	// replayed from a captured graph, so every batch starts and ends in slot 0.
+	// The graph is reused from `slot` when it was captured for the same storage,
+	// vectors and kernel choice (and kept there for the next call).
 	alp_rc_t run(hipStream_t s, CgState *h_state, CgSlot *slot = nullptr) {
 		const int batch = 16;
+		static_assert(batch % 2 == 0, "batches start and end in slot 0");
 		hipGraphExec_t exec = nullptr;
 		hipGraph_t graph = nullptr;
 		bool use_graph = false;
@@ -447,7 +493,7 @@ struct CgRunner {
 			if (slot) slot->drop_graph();
 			if (s != nullptr && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
 				alp_rc_t rc = ALP_SUCCESS;
-				for (int i = 0; i < batch && rc == ALP_SUCCESS; ++i) rc = iteration(s);
+				for (int i = 0; i < batch && rc == ALP_SUCCESS; ++i) rc = iteration(s, i % 2);
 				if (hipStreamEndCapture(s, &graph) == hipSuccess && rc == ALP_SUCCESS &&
 				    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
 					use_graph = true;
@@ -465,7 +511,7 @@ struct CgRunner {
 			if (use_graph) {
 				if (hipGraphLaunch(exec, s) != hipSuccess) { rc = ALP_PANIC; break; }
 			} else {
-				for (int i = 0; i < batch && rc == ALP_SUCCESS; ++i) rc = iteration(s);
+				for (int i = 0; i < batch && rc == ALP_SUCCESS; ++i) rc = iteration(s, i % 2);
 				if (rc) break;
 			}
 			if (hipMemcpyAsync(h_state, a.st, sizeof(CgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -473,7 +519,7 @@ struct CgRunner {
 				rc = ALP_PANIC;
 				break;
 			}
-			if (h_state->done) break;
+			if (h_state->s[0].done) break;
 		}
 		if (!slot || slot->exec != exec) {
 			if (exec) hipGraphExecDestroy(exec);
@@ -496,15 +542,13 @@ static alp_rc_t cg_fp64(alp_hip_matrix_t A, Storage &S, const double *b, double 
 	                 ctx().d_partials, ctx().d_tickets, st};
 	// temp = A x; r = b - temp; u = r; sigma = r.r; tol *= sqrt(b.b) (:332-378)
 	ALP_TRY(spmv(S, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, temp, x, nullptr, s));
-	hipLaunchKernelGGL(k_cg_init<double>, dim3(vec_grid(n)), dim3(kBlock), 0, s, a,
-	                   (const double *)b, (const double *)nullptr, tol, max_iter);
-	ALP_LAUNCH_CHECK();
+	ALP_TRY(cg_prelude(a, (const double *)b, (const double *)nullptr, tol, max_iter, s));
 	CgRunner<double> runner(S, a);
 	CgState h;
 	ALP_TRY(runner.run(s, &h, &slot));
-	*iters = h.iter;
-	*residual = h.rr;
-	return h.converged ? ALP_SUCCESS : ALP_FAILED;
+	*iters = h.s[0].iter;
+	*residual = h.s[0].rr;
+	return h.s[0].converged ? ALP_SUCCESS : ALP_FAILED;
 }
 
 static alp_rc_t cg_mixed(alp_hip_matrix_t A, Storage &S, const double *b, double *x, size_t max_iter,
@@ -542,11 +586,9 @@ static alp_rc_t cg_mixed(alp_hip_matrix_t A, Storage &S, const double *b, double
 		hipLaunchKernelGGL(k_ir_scale_down, dim3(vec_grid(n)), dim3(kBlock), 0, s, n,
 		                   (const double *)r64, b32, x32, (const IrState *)ir);
 		ALP_HIP_TRY(hipMemsetAsync(temp32, 0, sizeof(float) * n, s));
-		hipLaunchKernelGGL(k_cg_init<float>, dim3(vec_grid(n)), dim3(kBlock), 0, s, a,
-		                   (const double *)nullptr, (const float *)b32, inner_tol, max_iter - total);
-		ALP_LAUNCH_CHECK();
+		ALP_TRY(cg_prelude(a, (const double *)nullptr, (const float *)b32, inner_tol, max_iter - total, s));
 		ALP_TRY(runner.run(s, &h, &slot));
-		total += h.iter;
+		total += h.s[0].iter;
 		hipLaunchKernelGGL(k_ir_correct, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, x,
 		                   (const float *)x32, (const IrState *)ir);
 		ALP_LAUNCH_CHECK();
@@ -579,9 +621,7 @@ alp_rc_t cg_native(alp_hip_matrix_s *A, const T *b, T *x, size_t max_iter, doubl
 	const alp_semiring_t ring = vt == ALP_VAL_F32 ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64;
 	ALP_TRY(spmv(S, vt, ring, ALP_DESC_OVERWRITE, temp, x, nullptr, s));
 	const double *bb64 = std::is_same<T, double>::value ? (const double *)b : nullptr;
-	hipLaunchKernelGGL(k_cg_init<T>, dim3(vec_grid(n)), dim3(kBlock), 0, s, a, bb64,
-	                   std::is_same<T, double>::value ? (const T *)nullptr : b, tol, max_iter);
-	ALP_LAUNCH_CHECK();
+	ALP_TRY(cg_prelude(a, bb64, std::is_same<T, double>::value ? (const T *)nullptr : b, tol, max_iter, s));
 	CgRunner<T> runner(S, a, vt);
 	CgState h{};
 	if (!pc.fn) {
@@ -596,31 +636,37 @@ alp_rc_t cg_native(alp_hip_matrix_s *A, const T *b, T *x, size_t max_iter, doubl
 		ALP_HIP_TRY(hipHostMalloc(&hr.p, sizeof(T) * n, hipHostMallocDefault));
 		ALP_HIP_TRY(hipHostMalloc(&hz.p, sizeof(T) * n, hipHostMallocDefault));
 		std::memset(hz.p, 0, sizeof(T) * n);
+		// slot 0 holds the state after each stopping test, slot 1 the state the
+		// next iteration starts from (sigma = r.z)
 		const unsigned g = vec_grid(n);
 		for (size_t it = 0;; ++it) {
 			if (it > 0) {
-				ALP_TRY(runner.step(s));
+				ALP_TRY(runner.step(s, 1));
+				hipLaunchKernelGGL(k_cg_check<T>, dim3(1), dim3(kBlock), 0, s, a, runner.slot_in(1),
+				                   runner.slot_out(1), g);
+				ALP_LAUNCH_CHECK();
 				ALP_HIP_TRY(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
 				ALP_HIP_TRY(hipStreamSynchronize(s));
-				if (h.done) break;
+				if (h.s[0].done) break;
 			}
 			ALP_HIP_TRY(hipMemcpyAsync(hr.p, r, sizeof(T) * n, hipMemcpyDeviceToHost, s));
 			ALP_HIP_TRY(hipStreamSynchronize(s));
 			if (pc.fn(hz.p, hr.p, pc.data) != 0) {
 				fprintf(stderr, "alp_hip: user preconditioner returned a non-zero error code\n");
 				*iters = it;
-				*residual = it > 0 ? h.rr : std::numeric_limits<double>::infinity();
+				*residual = it > 0 ? h.s[0].rr : std::numeric_limits<double>::infinity();
 				return ALP_FAILED;
 			}
 			ALP_HIP_TRY(hipMemcpyAsync(z, hz.p, sizeof(T) * n, hipMemcpyHostToDevice, s));
-			hipLaunchKernelGGL(k_pcg_rz<T>, dim3(g), dim3(kBlock), 0, s, a, (const T *)z, it == 0 ? 1 : 0);
-			hipLaunchKernelGGL(k_pcg_dir<T>, dim3(g), dim3(kBlock), 0, s, a, (const T *)z);
+			hipLaunchKernelGGL(k_pcg_rz<T>, dim3(g), dim3(kBlock), 0, s, a, (const T *)z, runner.slot_in(0));
+			hipLaunchKernelGGL(k_pcg_dir<T>, dim3(g), dim3(kBlock), 0, s, a, (const T *)z, runner.slot_in(0),
+			                   runner.slot_out(0), g, it == 0 ? 1 : 0);
 			ALP_LAUNCH_CHECK();
 		}
 	}
-	*iters = h.iter;
-	*residual = h.rr;
-	return h.converged ? ALP_SUCCESS : ALP_FAILED;
+	*iters = h.s[0].iter;
+	*residual = h.s[0].rr;
+	return h.s[0].converged ? ALP_SUCCESS : ALP_FAILED;
 }
 
 template alp_rc_t cg_native<float>(alp_hip_matrix_s *, const float *, float *, size_t, double, HostPrecond<float>,
