@@ -157,6 +157,34 @@ __device__ __forceinline__ T block_sum_partials(const T *p, unsigned n, T *lds) 
 
 // ---- matrices ---------------------------------------------------------------
 
+// Diagonal form of a sliced ELL. A slice qualifies when every row's columns, in
+// storage order, are row + d_q for an increasing subsequence of one list of at
+// most 32 slot offsets d_0 .. d_{w-1} shared by the slice (any stencil, also after
+// a colour permutation, has this shape). Such a slice needs no column indices:
+// one offset per slot and one presence mask per row replace them, so a fold
+// streams values only and its x reads are unit-stride across the wavefront. The
+// fold still visits each row's entries in storage order, so results equal the
+// sliced-ELL (and CRS) fold bit for bit. Other slices keep the sliced-ELL arrays.
+struct DiaSlices {
+	uint8_t *flag = nullptr;   // per slice: 1 = diagonal form
+	int32_t *col = nullptr;    // per slot, at soff[s] / kSellRows + q: column - row
+	uint32_t *mask = nullptr;  // per (slice, lane): bit q set when the row has slot q
+	void *val = nullptr;       // values in slot order (sliced-ELL layout, 0 where absent)
+	float *val32 = nullptr;    // FP32 copy of them when the matrix has one
+	size_t n_dia = 0;          // slices in diagonal form
+};
+
+// Builds D for a sliced ELL of nslices slices with offsets soff; slice s covers
+// rows [srow[s], srow[s+1]) or, with srow null, [64 s, min(64 s + 64, nrows)).
+// Leaves D empty (and returns SUCCESS) when fewer than half the slices qualify.
+alp_rc_t dia_build(DiaSlices &D, size_t nslices, size_t nrows, const uint32_t *srow, const uint64_t *soff,
+                   size_t entries, const uint64_t *off, const uint32_t *idx, int value_type, const void *val,
+                   const float *val32);
+// rewrites D's values from the CRS values (after an in-place value update)
+alp_rc_t dia_refresh(DiaSlices &D, size_t nslices, size_t nrows, const uint32_t *srow, const uint64_t *soff,
+                     const uint64_t *off, int value_type, const void *val, const float *val32);
+void dia_free(DiaSlices &D);
+
 struct Storage {
 	uint64_t *off = nullptr;  // nmajor + 1
 	uint32_t *idx = nullptr;  // nnz
@@ -187,6 +215,7 @@ struct Storage {
 	float *sell_val32 = nullptr;   // FP32 copy when val32 exists
 	size_t n_slices = 0;
 	size_t sell_entries = 0;
+	DiaSlices dia;                 // diagonal form of the slices that have it
 	// rows longer than kLongRow nonzeros, for the CSR-stream path: split into
 	// chunks of kChunk nonzeros that separate blocks fold into partials, then one
 	// thread per long row folds its partials in chunk order (deterministic)
@@ -199,7 +228,7 @@ struct Storage {
 	// statistics for kernel selection
 	double avg_len = 0;
 	size_t max_len = 0;
-	int kernel = 0;  // 0 auto, 1 vector, 2 stream, 3 wave, 4 sliced ELL
+	int kernel = 0;  // 0 auto, 1 vector, 2 stream, 3 wave, 4 sliced ELL, 5 diagonal slices
 	int lanes = 0;
 	int rows = 0;    // rows per lane group of the CSR-vector kernel
 };

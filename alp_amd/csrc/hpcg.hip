@@ -76,13 +76,30 @@ struct HpLevel {
 	float *sval32 = nullptr;
 	size_t nslices = 0;
 	std::vector<size_t> colour_slice_off;  // slices of colour c: [cso[c], cso[c+1])
+	size_t sell_entries = 0;
+	DiaSlices dia;                         // diagonal form of the slices that have it
+};
+
+// Per-iteration scalars in two slots: iteration j reads slot j % 2 and only its
+// last kernel (one block) writes slot (j + 1) % 2, so no kernel reads a slot that
+// one of its own blocks writes and the reductions need no grid-wide fence.
+struct HpScal {
+	double rz;    // r.z of the iteration that wrote the slot (0 before the first)
+	double rr;    // r.r after it
+	double norm;  // sqrt(rr)
+	unsigned long long iter;
+	int done, pad;
 };
 
 struct HpState {
-	double rz, old_rz, beta, alpha, rr, norm, norm0, tol;
-	unsigned long long iter, max_iter;
-	int done, pad[3];
+	HpScal s[2];
+	double norm0, tol;
+	unsigned long long max_iter, pad;
 };
+
+// the shared partials buffer holds three regions of reduce_grid() partials each:
+// r.z, p.Ap and r.r of the current iteration
+constexpr unsigned kHpRegion = kMaxPartials / 4;
 
 struct IrState {
 	double rnorm, norm0, inv, pad;
@@ -97,6 +114,7 @@ template <> struct Vec<double> {
 	static const double *val(const HpLevel &L) { return L.val; }
 	static const double *diag(const HpLevel &L) { return L.diag; }
 	static const double *sval(const HpLevel &L) { return L.sval; }
+	static const double *dval(const HpLevel &L) { return (const double *)L.dia.val; }
 };
 template <> struct Vec<float> {
 	static float *z(HpLevel &L) { return L.z32; }
@@ -105,6 +123,7 @@ template <> struct Vec<float> {
 	static const float *val(const HpLevel &L) { return L.val32; }
 	static const float *diag(const HpLevel &L) { return L.diag32; }
 	static const float *sval(const HpLevel &L) { return L.sval32; }
+	static const float *dval(const HpLevel &L) { return L.dia.val32; }
 };
 
 // ---- epilogues of the tile fold ------------------------------------------------
@@ -229,52 +248,83 @@ __global__ __launch_bounds__(kBlock) void k_tiles_reduce(Tiles tl, const uint64_
 // reference's masked mxv, reference/blas2.hpp:463-489); ep.prefetch / ep.finish
 // as in tile_fold. The colour step of a proper colouring (EpSgs) reads z only at
 // other colours' rows and at the lane's own row, so the update needs no snapshot.
-template <typename T, int U, bool XCD, typename Ep>
+//
+// DIA: a slice in diagonal form reads no column indices: slot q of the lane's row,
+// when its mask bit is set, multiplies x[row + d_q] by the slot-ordered value; the
+// set bits come in storage order, so the fold is the same.
+template <typename T, int U, bool XCD, bool DIA, typename Ep>
 __global__ __launch_bounds__(kBlock) void k_sell_rows(size_t s0, size_t s1, const uint32_t *__restrict__ srow,
                                                       const uint64_t *__restrict__ soff,
                                                       const uint32_t *__restrict__ sidx,
                                                       const T *__restrict__ sval,
                                                       const uint64_t *__restrict__ off,
-                                                      const T *__restrict__ x, Ep ep, const int *done) {
+                                                      const T *__restrict__ x, Ep ep, const int *done,
+                                                      const uint8_t *__restrict__ dflag,
+                                                      const int32_t *__restrict__ dcol,
+                                                      const uint32_t *__restrict__ dmask,
+                                                      const T *__restrict__ dval) {
 	if (done && *done) return;
 	const unsigned bl = XCD ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-	const size_t sl = s0 + (size_t)bl * (kBlock / kSellRows) + threadIdx.x / kSellRows;
+	// uniform across the wavefront: slice bounds and slot offsets stay scalar
+	const size_t sl = s0 + (size_t)__builtin_amdgcn_readfirstlane(bl * (kBlock / kSellRows) + threadIdx.x / kSellRows);
 	if (sl >= s1) return;  // whole wavefronts leave together
 	const unsigned lane = threadIdx.x % kSellRows;
 	const uint32_t row = srow[sl] + lane;
 	const bool act = row < srow[sl + 1];
 	const uint64_t e0 = soff[sl];
 	const unsigned w = (unsigned)((soff[sl + 1] - e0) / kSellRows);
-	unsigned len = 0;
-	if (act) {
-		len = (unsigned)(off[row + 1] - off[row]);
-		ep.prefetch(row);
-	}
+	if (act) ep.prefetch(row);
 	T acc = T(0);
-	const uint32_t *ip = sidx + e0 + lane;
-	const T *vp = sval + e0 + lane;
-	for (unsigned q0 = 0; q0 < w; q0 += U) {
-		uint32_t j[U];
-		T v[U];
+	unsigned len = 0;
+	if (DIA && dflag[sl]) {
+		const uint32_t bits = act ? dmask[sl * kSellRows + lane] : 0u;
+		len = __builtin_popcount(bits);
+		const int dl = lane < w ? dcol[e0 / kSellRows + lane] : 0;  // lane q holds d_q (w <= 32)
+		const T *vp = dval + e0 + lane;
+		for (unsigned q0 = 0; q0 < w; q0 += U) {
+			T v[U], xv[U];
 #pragma unroll
-		for (int u = 0; u < U; ++u) {
-			const unsigned q = q0 + u;
-			j[u] = 0;
-			v[u] = T(0);
-			if (q < len) {
-				j[u] = __builtin_nontemporal_load(ip + (size_t)q * kSellRows);
-				v[u] = __builtin_nontemporal_load(vp + (size_t)q * kSellRows);
+			for (int u = 0; u < U; ++u) {
+				const unsigned q = q0 + u;  // < 32 + U
+				const bool in = q < w && ((bits >> q) & 1u);
+				const long long d = __builtin_amdgcn_readlane(dl, (int)q);
+				v[u] = in ? __builtin_nontemporal_load(vp + (size_t)q * kSellRows) : T(0);
+				xv[u] = in ? x[(long long)row + d] : T(0);
 			}
+#pragma unroll
+			for (int u = 0; u < U; ++u)
+				if (q0 + u < w && ((bits >> (q0 + u)) & 1u)) {
+					const T prod = v[u] * xv[u];
+					acc = prod + acc;
+				}
 		}
-		T xv[U];
+	} else {
+		if (act) len = (unsigned)(off[row + 1] - off[row]);
+		const uint32_t *ip = sidx + e0 + lane;
+		const T *vp = sval + e0 + lane;
+		for (unsigned q0 = 0; q0 < w; q0 += U) {
+			uint32_t j[U];
+			T v[U];
 #pragma unroll
-		for (int u = 0; u < U; ++u) xv[u] = q0 + u < len ? x[j[u]] : T(0);
-#pragma unroll
-		for (int u = 0; u < U; ++u)
-			if (q0 + u < len) {
-				const T prod = v[u] * xv[u];
-				acc = prod + acc;
+			for (int u = 0; u < U; ++u) {
+				const unsigned q = q0 + u;
+				j[u] = 0;
+				v[u] = T(0);
+				if (q < len) {
+					j[u] = __builtin_nontemporal_load(ip + (size_t)q * kSellRows);
+					v[u] = __builtin_nontemporal_load(vp + (size_t)q * kSellRows);
+				}
 			}
+			T xv[U];
+#pragma unroll
+			for (int u = 0; u < U; ++u) xv[u] = q0 + u < len ? x[j[u]] : T(0);
+#pragma unroll
+			for (int u = 0; u < U; ++u)
+				if (q0 + u < len) {
+					const T prod = v[u] * xv[u];
+					acc = prod + acc;
+				}
+		}
 	}
 	if (act) ep.finish(row, acc, len == 0);
 }
@@ -377,66 +427,51 @@ __global__ __launch_bounds__(kBlock) void k_copy_done(size_t n, const T *s, T *d
 
 // ---- CG part of hpcg (hpcg.hpp:148-221) -------------------------------------------
 
-// rz = r.z; beta = rz / old_rz (iteration 0: p = z)
+// partials of r.z (region 0)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_hp_rz(size_t n, const T *r, const T *z, double *partials,
-                                                  unsigned *ticket, HpState *st) {
+                                                  const HpScal *in) {
 	__shared__ double lds[kBlock / kWave];
-	__shared__ bool is_last;
-	if (st->done) return;
+	if (in->done) return;
 	double p = 0.0;
 	grid_for_each<4>(n, [&](size_t i) { p += (double)r[i] * (double)z[i]; });
 	p = block_sum(p, lds);
 	if (threadIdx.x == 0) partials[blockIdx.x] = p;
-	if (last_block(ticket, &is_last)) {
-		const double rz = block_sum_partials(partials, gridDim.x, lds);
-		if (threadIdx.x == 0) {
-			st->old_rz = st->rz;
-			st->rz = rz;
-			st->beta = st->iter == 0 ? 0.0 : rz / st->old_rz;
-		}
-	}
 }
 
-// p = beta p + z (iteration 0: p = z)
+// rz = r.z; p = z on the first iteration, else p = (rz / previous rz) p + z
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_hp_p(size_t n, T *p, const T *z, const HpState *st) {
-	if (st->done) return;
-	const bool first = st->iter == 0;
-	const T beta = (T)st->beta;
+__global__ __launch_bounds__(kBlock) void k_hp_p(size_t n, T *p, const T *z, const double *partials,
+                                                 unsigned nred, const HpScal *in) {
+	__shared__ double lds[kBlock / kWave];
+	if (in->done) return;
+	const bool first = in->iter == 0;
+	const double rz = first ? 0.0 : block_sum_partials(partials, nred, lds);
+	const T beta = first ? T(0) : (T)(rz / in->rz);
 	grid_for_each<4>(n, [&](size_t i) { p[i] = first ? z[i] : beta * p[i] + z[i]; });
 }
 
-// alpha = rz / (p.Ap)
+// partials of p.Ap (region 1)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_hp_alpha(size_t n, const T *p, const T *Ap, double *partials,
-                                                     unsigned *ticket, HpState *st) {
+__global__ __launch_bounds__(kBlock) void k_hp_pap(size_t n, const T *p, const T *Ap, double *partials,
+                                                   const HpScal *in) {
 	__shared__ double lds[kBlock / kWave];
-	__shared__ bool is_last;
-	if (st->done) return;
+	if (in->done) return;
 	double q = 0.0;
 	grid_for_each<4>(n, [&](size_t i) { q += (double)p[i] * (double)Ap[i]; });
 	q = block_sum(q, lds);
-	if (threadIdx.x == 0) partials[blockIdx.x] = q;
-	if (last_block(ticket, &is_last)) {
-		const double pAp = block_sum_partials(partials, gridDim.x, lds);
-		if (threadIdx.x == 0) st->alpha = st->rz / pAp;
-	}
+	if (threadIdx.x == 0) partials[kHpRegion + blockIdx.x] = q;
 }
 
-struct FinAlpha {
-	HpState *st;
-	__device__ void operator()(double pAp) const { st->alpha = st->rz / pAp; }
-};
-
-// x += alpha p; r += (-alpha) Ap; norm = sqrt(r.r); stop when !(iter < max && norm/norm0 > tol)
+// alpha = rz / p.Ap; x += alpha p; r += (-alpha) Ap; partials of r.r (region 2)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_hp_update(size_t n, T *x, T *r, const T *p, const T *Ap,
-                                                      double *partials, unsigned *ticket, HpState *st) {
+                                                      double *partials, unsigned nred, const HpScal *in) {
 	__shared__ double lds[kBlock / kWave];
-	__shared__ bool is_last;
-	if (st->done) return;
-	const T alpha = (T)st->alpha;
+	if (in->done) return;
+	const double rz = block_sum_partials(partials, nred, lds);
+	const double pAp = block_sum_partials(partials + kHpRegion, nred, lds);
+	const T alpha = (T)(rz / pAp);
 	double q = 0.0;
 	grid_for_each<4>(n, [&](size_t i) {
 		x[i] = alpha * p[i] + x[i];
@@ -445,15 +480,29 @@ __global__ __launch_bounds__(kBlock) void k_hp_update(size_t n, T *x, T *r, cons
 		q += (double)ri * (double)ri;
 	});
 	q = block_sum(q, lds);
-	if (threadIdx.x == 0) partials[blockIdx.x] = q;
-	if (last_block(ticket, &is_last)) {
-		const double rr = block_sum_partials(partials, gridDim.x, lds);
-		if (threadIdx.x == 0) {
-			st->rr = rr;
-			st->norm = std::sqrt(rr);
-			st->iter += 1;
-			if (!(st->iter < st->max_iter && st->norm / st->norm0 > st->tol)) st->done = 1;
-		}
+	if (threadIdx.x == 0) partials[2 * kHpRegion + blockIdx.x] = q;
+}
+
+// one block: norm = sqrt(r.r), iter + 1, stop when !(iter < max && norm/norm0 > tol)
+// (hpcg.hpp:148,213-221); writes the next slot
+__global__ __launch_bounds__(kBlock) void k_hp_fin(const double *partials, unsigned nred, const HpScal *in,
+                                                  HpScal *out, const HpState *st) {
+	__shared__ double lds[kBlock / kWave];
+	if (in->done) {
+		if (threadIdx.x == 0) *out = *in;
+		return;
+	}
+	const double rz = block_sum_partials(partials, nred, lds);
+	const double rr = block_sum_partials(partials + 2 * kHpRegion, nred, lds);
+	if (threadIdx.x == 0) {
+		HpScal o;
+		o.rz = rz;
+		o.rr = rr;
+		o.norm = std::sqrt(rr);
+		o.iter = in->iter + 1;
+		o.done = !(o.iter < st->max_iter && o.norm / st->norm0 > st->tol) ? 1 : 0;
+		o.pad = 0;
+		*out = o;
 	}
 }
 
@@ -462,14 +511,18 @@ struct FinInit {  // prelude: norm0 = ||b - A x||
 	double tol;
 	unsigned long long max_iter;
 	__device__ void operator()(double rr) const {
-		st->rr = rr;
-		st->norm = std::sqrt(rr);
-		st->norm0 = st->norm;
+		HpScal o;
+		o.rz = 0.0;
+		o.rr = rr;
+		o.norm = std::sqrt(rr);
+		o.iter = 0;
+		o.done = max_iter == 0 ? 1 : 0;
+		o.pad = 0;
+		st->s[0] = o;
+		st->s[1] = o;
+		st->norm0 = o.norm;
 		st->tol = tol;
-		st->iter = 0;
 		st->max_iter = max_iter;
-		st->rz = st->old_rz = st->beta = st->alpha = 0.0;
-		st->done = max_iter == 0 ? 1 : 0;
 	}
 };
 
@@ -605,7 +658,7 @@ static unsigned reduce_grid(size_t n) {
 	size_t g = (n + kBlock - 1) / kBlock;
 	const size_t cap = fill_blocks(4);
 	if (g > cap) g = cap;
-	if (g > (size_t)kMaxPartials) g = kMaxPartials;
+	if (g > (size_t)kHpRegion) g = kHpRegion;
 	if (g < 1) g = 1;
 	return (unsigned)g;
 }
@@ -634,6 +687,7 @@ static void free_level(HpLevel &L) {
 	              L.z32, L.r32, L.t32, L.srow, L.soff, L.sidx, L.sval, L.sval32};
 	for (void *p : ps)
 		if (p) hipFree(p);
+	dia_free(L.dia);
 	L = HpLevel();
 }
 
@@ -692,7 +746,12 @@ static alp_rc_t build_level_sell(HpLevel &L) {
 			                   (const uint64_t *)L.off, (const uint32_t *)L.idx, (const float *)L.val32,
 			                   (const uint64_t *)L.soff, (uint32_t *)nullptr, L.sval32);
 		}
-		if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) rc = ALP_PANIC;
+		if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		L.sell_entries = total;
+		rc = dia_build(L.dia, ns, L.n, L.srow, L.soff, total, L.off, L.idx, ALP_VAL_F64, L.val, L.val32);
 	} while (0);
 	if (tmp) hipFree(tmp);
 	if (sizes) hipFree(sizes);
@@ -833,6 +892,16 @@ static alp_rc_t build_level(HpLevel &L, size_t nx, size_t ny, size_t nz, int col
 	return rc;
 }
 
+// the diagonal slices are used where built; ALP_HIP_HPCG_DIA=0 keeps the sliced ELL
+static int g_dia = -1;
+static bool dia_enabled() {
+	if (g_dia < 0) {
+		const char *e = std::getenv("ALP_HIP_HPCG_DIA");
+		g_dia = e ? std::atoi(e) : 1;
+	}
+	return g_dia != 0;
+}
+
 static int g_tile_xcd = -1;
 static int tile_xcd() {
 	if (g_tile_xcd < 0) {
@@ -859,9 +928,17 @@ static bool sell_rows(const HpLevel &L, size_t s0, size_t s1, const T *x, Ep ep,
 	const size_t per = kBlock / kSellRows;
 	const size_t grid = (s1 - s0 + per - 1) / per;
 	if (grid == 0) return true;
-	hipLaunchKernelGGL((k_sell_rows<T, 8, false, Ep>), dim3(grid), dim3(kBlock), 0, s, s0, s1,
-	                   (const uint32_t *)L.srow, (const uint64_t *)L.soff, (const uint32_t *)L.sidx, sval,
-	                   (const uint64_t *)L.off, x, ep, done);
+	const T *dval = Vec<T>::dval(L);
+	if (dval && dia_enabled())
+		hipLaunchKernelGGL((k_sell_rows<T, 8, false, true, Ep>), dim3(grid), dim3(kBlock), 0, s, s0, s1,
+		                   (const uint32_t *)L.srow, (const uint64_t *)L.soff, (const uint32_t *)L.sidx, sval,
+		                   (const uint64_t *)L.off, x, ep, done, (const uint8_t *)L.dia.flag,
+		                   (const int32_t *)L.dia.col, (const uint32_t *)L.dia.mask, dval);
+	else
+		hipLaunchKernelGGL((k_sell_rows<T, 8, false, false, Ep>), dim3(grid), dim3(kBlock), 0, s, s0, s1,
+		                   (const uint32_t *)L.srow, (const uint64_t *)L.soff, (const uint32_t *)L.sidx, sval,
+		                   (const uint64_t *)L.off, x, ep, done, (const uint8_t *)nullptr, (const int32_t *)nullptr,
+		                   (const uint32_t *)nullptr, (const T *)nullptr);
 	return true;
 }
 
@@ -947,27 +1024,30 @@ static void precondition(alp_hip_hpcg_s *H, bool precond, hipStream_t s, const i
 	}
 }
 
-// the CG part of one hpcg iteration (hpcg.hpp:148-221) once z is known
+// the CG part of one hpcg iteration (hpcg.hpp:148-221) once z is known, reading
+// slot `par` of the state and writing slot 1 - par
 template <typename T>
-static void cg_step(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, hipStream_t s) {
+static void cg_step(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, int par, hipStream_t s) {
 	HpLevel &F = H->L[0];
 	const size_t n = F.n;
-	const int *done = &st->done;
+	const HpScal *in = &st->s[par];
+	const int *done = &in->done;
+	double *part = ctx().d_partials;
 	T *z = Vec<T>::z(F), *r = Vec<T>::r(F);
 	const unsigned rg = reduce_grid(n);
-	hipLaunchKernelGGL(k_hp_rz<T>, dim3(rg), dim3(kBlock), 0, s, n, (const T *)r, (const T *)z,
-	                   ctx().d_partials, ctx().d_tickets, st);
-	hipLaunchKernelGGL(k_hp_p<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, p, (const T *)z, (const HpState *)st);
+	hipLaunchKernelGGL(k_hp_rz<T>, dim3(rg), dim3(kBlock), 0, s, n, (const T *)r, (const T *)z, part, in);
+	hipLaunchKernelGGL(k_hp_p<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, p, (const T *)z,
+	                   (const double *)part, rg, in);
 	const Tiles tl = all_tiles<T>(F);
-	// Ap = A p on a one-tile-per-block grid, then p.Ap -> alpha
 	EpStore<T> ep{Ap};
 	if (!sell_rows<T>(F, 0, F.nslices, (const T *)p, ep, s, done))
 		hipLaunchKernelGGL((k_tiles<T, EpStore<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
 		                   (const uint64_t *)F.off, (const uint32_t *)F.idx, Vec<T>::val(F), (const T *)p, ep, done);
-	hipLaunchKernelGGL(k_hp_alpha<T>, dim3(rg), dim3(kBlock), 0, s, n, (const T *)p, (const T *)Ap,
-	                   ctx().d_partials, ctx().d_tickets, st);
-	hipLaunchKernelGGL(k_hp_update<T>, dim3(rg), dim3(kBlock), 0, s, n, x, r, (const T *)p, (const T *)Ap,
-	                   ctx().d_partials, ctx().d_tickets, st);
+	hipLaunchKernelGGL(k_hp_pap<T>, dim3(rg), dim3(kBlock), 0, s, n, (const T *)p, (const T *)Ap, part, in);
+	hipLaunchKernelGGL(k_hp_update<T>, dim3(rg), dim3(kBlock), 0, s, n, x, r, (const T *)p, (const T *)Ap, part,
+	                   rg, in);
+	hipLaunchKernelGGL(k_hp_fin, dim3(1), dim3(kBlock), 0, s, (const double *)part, rg, in, &st->s[1 - par],
+	                   (const HpState *)st);
 }
 
 __global__ __launch_bounds__(kBlock) void k_down32(size_t n, const double *r, float *r32, float *z32,
@@ -987,8 +1067,9 @@ __global__ __launch_bounds__(kBlock) void k_up64(size_t n, const float *z32, dou
 // one hpcg iteration on precision T; with FP32PC the (FP64) iteration's
 // preconditioner, the whole V-cycle, runs in FP32
 template <typename T, bool FP32PC>
-static void hpcg_iteration(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bool precond, hipStream_t s) {
-	const int *done = &st->done;
+static void hpcg_iteration(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bool precond, int par,
+                           hipStream_t s) {
+	const int *done = &st->s[par].done;
 	if (FP32PC && precond) {
 		HpLevel &F = H->L[0];
 		hipLaunchKernelGGL(k_down32, dim3(grid_for(F.n)), dim3(kBlock), 0, s, F.n, (const double *)F.r, F.r32,
@@ -998,7 +1079,7 @@ static void hpcg_iteration(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bo
 	} else {
 		precondition<T>(H, precond, s, done);
 	}
-	cg_step<T>(H, x, p, Ap, st, s);
+	cg_step<T>(H, x, p, Ap, st, par, s);
 }
 
 // prelude: r = b - A x, norm0 = ||r|| (hpcg.hpp:124-137)
@@ -1016,8 +1097,9 @@ static void hpcg_prelude(alp_hip_hpcg_s *H, const T *b, const T *x, T *Ap, HpSta
 	                   (const int *)nullptr);
 }
 
-// Run hpcg iterations until the state says done. One iteration is captured into a
-// hipGraph and replayed; the state is read back every `check` iterations.
+// Run hpcg iterations until the state says done. Two iterations (slot 0 -> 1 -> 0)
+// are captured into a hipGraph and replayed; the state is read back every `check`
+// iterations, always from slot 0.
 template <typename T, bool FP32PC = false>
 static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bool precond,
                                HpState *h_st) {
@@ -1026,23 +1108,26 @@ static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st
 	hipGraphExec_t exec = nullptr;
 	bool use_graph = false;
 	if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-		hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, s);
+		hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 0, s);
+		hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 1, s);
 		if (hipStreamEndCapture(s, &graph) == hipSuccess &&
 		    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
 			use_graph = true;
 	}
 	(void)hipGetLastError();
 	const int check = 4;
+	static_assert(check % 2 == 0, "checks read slot 0");
 	alp_rc_t rc = ALP_SUCCESS;
 	for (;;) {
-		for (int i = 0; i < check; ++i) {
+		for (int i = 0; i < check; i += 2) {
 			if (use_graph) {
 				if (hipGraphLaunch(exec, s) != hipSuccess) {
 					rc = ALP_PANIC;
 					break;
 				}
 			} else {
-				hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, s);
+				hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 0, s);
+				hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 1, s);
 			}
 		}
 		if (rc) break;
@@ -1051,7 +1136,7 @@ static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st
 			rc = ALP_PANIC;
 			break;
 		}
-		if (h_st->done) break;
+		if (h_st->s[0].done) break;
 	}
 	if (exec) hipGraphExecDestroy(exec);
 	if (graph) hipGraphDestroy(graph);
@@ -1197,8 +1282,8 @@ ALP_EXPORT alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, doub
 			ALP_TRY(run_iterations<double>(H, H->x, H->p, H->Ap, H->st, with_precond != 0, &h));
 		else
 			ALP_TRY((run_iterations<double, true>(H, H->x, H->p, H->Ap, H->st, with_precond != 0, &h)));
-		total = h.iter;
-		res = h.norm;
+		total = h.s[0].iter;
+		res = h.s[0].norm;
 	} else {
 		// FP64 refinement: r = b - A x (FP64 tiles), inner FP32 PCG on r/||r||
 		const double inner_tol = 1e-5;
@@ -1222,7 +1307,7 @@ ALP_EXPORT alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, doub
 			                   H->x32, (const IrState *)H->ir);
 			hpcg_prelude<float>(H, H->Ap32, H->x32, H->Ap32, H->st32, max_iter - total, inner_tol, s);
 			ALP_TRY(run_iterations<float>(H, H->x32, H->p32, H->Ap32, H->st32, with_precond != 0, &h));
-			total += h.iter;
+			total += h.s[0].iter;
 			hipLaunchKernelGGL(k_ir_up, dim3(grid_for(n)), dim3(kBlock), 0, s, n, H->x, (const float *)H->x32,
 			                   (const IrState *)H->ir);
 			ALP_LAUNCH_CHECK();

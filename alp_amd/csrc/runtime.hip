@@ -8,6 +8,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <mutex>
@@ -256,6 +257,164 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(size_t nrows, size_t nslic
 	});
 }
 
+// ---- diagonal slices ------------------------------------------------------------
+
+__device__ __forceinline__ void dia_rows(size_t sl, size_t nrows, const uint32_t *srow, size_t *r0, size_t *r1) {
+	if (srow) {
+		*r0 = srow[sl];
+		*r1 = srow[sl + 1];
+	} else {
+		*r0 = sl * kSellRows;
+		*r1 = *r0 + kSellRows < nrows ? *r0 + kSellRows : nrows;
+	}
+}
+
+// One wavefront per slice. The template is the slice's first row of maximal
+// length w (the slice width); its column offsets, in storage order, are the slot
+// offsets. Every row then matches its entries greedily against the slots in
+// order; the offsets of one row are distinct, so a row whose offset sequence is a
+// subsequence of the template's is always matched completely.
+__global__ __launch_bounds__(kBlock) void k_dia_detect(size_t nslices, size_t nrows, const uint32_t *srow,
+                                                      const uint64_t *soff, const uint64_t *off,
+                                                      const uint32_t *idx, uint8_t *flag, int32_t *col,
+                                                      uint32_t *mask, unsigned long long *count) {
+	const size_t sl = (size_t)blockIdx.x * (kBlock / kSellRows) + threadIdx.x / kSellRows;
+	if (sl >= nslices) return;  // whole wavefronts leave together
+	const unsigned lane = threadIdx.x % kSellRows;
+	size_t r0, r1;
+	dia_rows(sl, nrows, srow, &r0, &r1);
+	const size_t r = r0 + lane;
+	const bool act = r < r1;
+	const uint64_t e0 = soff[sl];
+	const unsigned w = (unsigned)((soff[sl + 1] - e0) / kSellRows);
+	const uint64_t b = act ? off[r] : 0;
+	const unsigned len = act ? (unsigned)(off[r + 1] - b) : 0u;
+	const unsigned long long tmpl = __ballot(act && len == w);
+	if (w == 0 || w > 32 || tmpl == 0) {
+		if (lane == 0) flag[sl] = 0;
+		mask[sl * kSellRows + lane] = 0;
+		return;
+	}
+	const int t = __ffsll(tmpl) - 1;
+	uint32_t bits = 0;
+	unsigned j = 0;
+	bool fits = true;
+	for (unsigned q = 0; q < w; ++q) {
+		const long long mine = (act && q < len) ? (long long)idx[b + q] - (long long)r : 0;
+		const long long d = __shfl(mine, t);
+		if (d < INT32_MIN || d > INT32_MAX) fits = false;
+		if (lane == 0) col[e0 / kSellRows + q] = (int32_t)d;
+		if (j < len && (long long)idx[b + j] - (long long)r == d) {
+			bits |= 1u << q;
+			++j;
+		}
+	}
+	const bool ok = fits && j == len;
+	const bool all = __ballot(!ok) == 0;
+	mask[sl * kSellRows + lane] = all ? bits : 0u;
+	if (lane == 0) {
+		flag[sl] = all ? 1 : 0;
+		if (all) atomicAdd(count, 1ull);
+	}
+}
+
+// values in slot order: slot q of a row holds its entry number popcount(mask
+// below q) when bit q is set, else 0; slices not in diagonal form are skipped
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_dia_fill(size_t nslices, size_t nrows, const uint32_t *srow,
+                                                    const uint64_t *soff, const uint64_t *off, const uint8_t *flag,
+                                                    const uint32_t *mask, const V *val, const float *val32,
+                                                    V *dval, float *dval32) {
+	grid_for_each<1>(nslices * kSellRows, [&](size_t t) {
+		const size_t sl = t / kSellRows, lane = t % kSellRows;
+		if (!flag[sl]) return;
+		size_t r0, r1;
+		dia_rows(sl, nrows, srow, &r0, &r1);
+		const size_t r = r0 + lane;
+		const uint64_t e0 = soff[sl], w = (soff[sl + 1] - e0) / kSellRows;
+		const uint32_t bits = r < r1 ? mask[t] : 0u;
+		const uint64_t b = r < r1 ? off[r] : 0;
+		uint64_t j = 0;
+		for (uint64_t q = 0; q < w; ++q) {
+			const uint64_t e = e0 + q * kSellRows + lane;
+			const bool in = (bits >> q) & 1u;
+			if (dval) dval[e] = in ? val[b + j] : V(0);
+			if (dval32) dval32[e] = in ? val32[b + j] : 0.0f;
+			j += in ? 1 : 0;
+		}
+	});
+}
+
+void dia_free(DiaSlices &D) {
+	void *ps[] = {D.flag, D.col, D.mask, D.val, D.val32};
+	for (void *p : ps)
+		if (p) hipFree(p);
+	D = DiaSlices();
+}
+
+alp_rc_t dia_refresh(DiaSlices &D, size_t nslices, size_t nrows, const uint32_t *srow, const uint64_t *soff,
+                     const uint64_t *off, int value_type, const void *val, const float *val32) {
+	if (!D.flag || nslices == 0) return ALP_SUCCESS;
+	const unsigned g = stream_grid(nslices * kSellRows);
+	if (value_type == ALP_VAL_F32)
+		hipLaunchKernelGGL(k_dia_fill<float>, dim3(g), dim3(kBlock), 0, stream(), nslices, nrows, srow, soff, off,
+		                   (const uint8_t *)D.flag, (const uint32_t *)D.mask, (const float *)val, val32,
+		                   (float *)D.val, D.val32);
+	else
+		hipLaunchKernelGGL(k_dia_fill<double>, dim3(g), dim3(kBlock), 0, stream(), nslices, nrows, srow, soff,
+		                   off, (const uint8_t *)D.flag, (const uint32_t *)D.mask, (const double *)val, val32,
+		                   (double *)D.val, D.val32);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
+alp_rc_t dia_build(DiaSlices &D, size_t nslices, size_t nrows, const uint32_t *srow, const uint64_t *soff,
+                   size_t entries, const uint64_t *off, const uint32_t *idx, int value_type, const void *val,
+                   const float *val32) {
+	dia_free(D);
+	if (nslices == 0 || entries == 0) return ALP_SUCCESS;
+	const size_t vs = value_type == ALP_VAL_F64 ? 8 : value_type == ALP_VAL_F32 ? 4 : 0;
+	unsigned long long *d_count = nullptr;
+	DiaSlices T;
+	if (hipMalloc(&T.flag, nslices) != hipSuccess || hipMalloc(&T.col, sizeof(int32_t) * (entries / kSellRows)) != hipSuccess ||
+	    hipMalloc(&T.mask, sizeof(uint32_t) * nslices * kSellRows) != hipSuccess ||
+	    hipMalloc(&d_count, sizeof(unsigned long long)) != hipSuccess) {
+		(void)hipGetLastError();
+		dia_free(T);
+		if (d_count) hipFree(d_count);
+		return ALP_SUCCESS;  // no memory for the diagonal form: sliced ELL only
+	}
+	alp_rc_t rc = ALP_SUCCESS;
+	unsigned long long cnt = 0;
+	if (hipMemsetAsync(d_count, 0, sizeof(unsigned long long), stream()) != hipSuccess) rc = ALP_PANIC;
+	if (rc == ALP_SUCCESS) {
+		const size_t per = kBlock / kSellRows;
+		hipLaunchKernelGGL(k_dia_detect, dim3((nslices + per - 1) / per), dim3(kBlock), 0, stream(), nslices, nrows,
+		                   srow, soff, off, idx, T.flag, T.col, T.mask, d_count);
+		if (hipGetLastError() != hipSuccess ||
+		    copy_sync(&cnt, d_count, sizeof(cnt), hipMemcpyDeviceToHost) != hipSuccess)
+			rc = ALP_PANIC;
+	}
+	hipFree(d_count);
+	if (rc != ALP_SUCCESS || 2 * cnt < nslices) {
+		dia_free(T);
+		return rc;
+	}
+	if ((vs && hipMalloc(&T.val, vs * entries) != hipSuccess) ||
+	    (val32 && hipMalloc(&T.val32, sizeof(float) * entries) != hipSuccess)) {
+		(void)hipGetLastError();
+		dia_free(T);
+		return ALP_SUCCESS;
+	}
+	T.n_dia = cnt;
+	if ((rc = dia_refresh(T, nslices, nrows, srow, soff, off, value_type, val, val32)) != ALP_SUCCESS) {
+		dia_free(T);
+		return rc;
+	}
+	D = T;
+	return ALP_SUCCESS;
+}
+
 // Builds S's sliced-ELL copy when its padded size stays within 1.5 nnz and no
 // row is longer than 4096; otherwise leaves it absent. Values follow value_type.
 static alp_rc_t build_sell(Storage &S, int value_type, uint64_t nnz) {
@@ -318,7 +477,7 @@ static alp_rc_t build_sell(Storage &S, int value_type, uint64_t nnz) {
 	S.sell_val32 = sval32;
 	S.n_slices = ns;
 	S.sell_entries = total;
-	return ALP_SUCCESS;
+	return dia_build(S.dia, ns, S.nmajor, nullptr, soff, total, S.off, S.idx, value_type, S.val, S.val32);
 }
 
 // Copies S's current CRS values (and FP32 copy) into its sliced-ELL copy, after a
@@ -337,7 +496,7 @@ alp_rc_t storage_refresh_values(Storage &S, int value_type) {
 		                   (const float *)S.val32, (const uint64_t *)S.sell_off, S.sell_idx, (double *)S.sell_val,
 		                   S.sell_val32);
 	ALP_LAUNCH_CHECK();
-	return ALP_SUCCESS;
+	return dia_refresh(S.dia, S.n_slices, S.nmajor, nullptr, S.sell_off, S.off, value_type, S.val, S.val32);
 }
 
 alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32) {
@@ -382,6 +541,7 @@ void storage_free(Storage &S) {
 	if (S.sell_idx) hipFree(S.sell_idx);
 	if (S.sell_val) hipFree(S.sell_val);
 	if (S.sell_val32) hipFree(S.sell_val32);
+	dia_free(S.dia);
 	void *lr[] = {S.lr_rows, S.lr_c0, S.ch_row, S.ch_beg, S.ch_part};
 	for (void *q : lr)
 		if (q) hipFree(q);

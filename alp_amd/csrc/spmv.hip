@@ -177,59 +177,85 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
 // stencil-like matrices a gather instruction touches a few contiguous runs of x.
 // U entries per lane are in flight before the first multiply. XCD: each XCD
 // walks one contiguous range of slices (its L2 keeps the x window).
-template <typename R, int MASK, bool OVERWRITE, int U, bool XCD>
+//
+// DIA: slices in diagonal form (the slice's flag set) read no column indices and
+// no row offsets: slot q of the lane's row, when its mask bit is set, multiplies
+// x[row + d_q] (d_q one scalar per slot) by the slot-ordered value. Set bits come
+// in the row's storage order, so the fold is the same sequence of operations.
+template <typename R, int MASK, bool OVERWRITE, int U, bool XCD, bool DIA>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     size_t nrows, size_t nslices, const uint64_t *__restrict__ soff, const uint32_t *__restrict__ sidx,
     const void *__restrict__ sval, const uint64_t *__restrict__ off, const typename R::X *__restrict__ x,
-    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
+    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask, const uint8_t *__restrict__ dflag,
+    const int32_t *__restrict__ dcol, const uint32_t *__restrict__ dmask, const void *__restrict__ dval) {
 	using A = typename R::A;
 	using Y = typename R::Y;
 	using X = typename R::X;
 	using VT = typename Val<R>::T;
 	const auto *val = static_cast<const typename R::V *>(sval);
 	const unsigned bl = XCD ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-	const size_t sl = (size_t)bl * (kBlock / kSellRows) + threadIdx.x / kSellRows;
+	// the slice index is uniform across the wavefront: keep it (and what is
+	// loaded from it) in scalar registers
+	const size_t sl = (size_t)__builtin_amdgcn_readfirstlane(bl * (kBlock / kSellRows) + threadIdx.x / kSellRows);
 	if (sl >= nslices) return;  // whole wavefronts leave together
 	const unsigned lane = threadIdx.x % kSellRows;
 	const size_t r = sl * kSellRows + lane;
 	const uint64_t e0 = soff[sl];
 	const unsigned w = (unsigned)((soff[sl + 1] - e0) / kSellRows);
 	bool active = r < nrows;
-	unsigned len = 0;
-	A yv = R::zero();
-	if (active) {
-		len = (unsigned)(off[r + 1] - off[r]);
-		if (MASK) active = (MASK == 1) == (mask[r] != 0);
-		if (!active) len = 0;
-		if (!OVERWRITE && active && len) yv = static_cast<A>(y[r]);
-	}
+	if (MASK && active) active = (MASK == 1) == (mask[r] != 0);
 	A acc = R::zero();
-	const uint32_t *ip = sidx + e0 + lane;
-	for (unsigned q0 = 0; q0 < w; q0 += U) {
-		uint32_t j[U];
-		VT v[U];
+	unsigned len = 0;
+	if (DIA && dflag[sl]) {
+		const uint32_t bits = active ? dmask[sl * kSellRows + lane] : 0u;
+		len = __builtin_popcount(bits);
+		// lane q holds slot offset d_q (w <= 32); readlane broadcasts it
+		const int dl = lane < w ? dcol[e0 / kSellRows + lane] : 0;
+		const auto *dv = static_cast<const typename R::V *>(dval);
+		for (unsigned q0 = 0; q0 < w; q0 += U) {
+			VT v[U];
+			X xv[U];
 #pragma unroll
-		for (int u = 0; u < U; ++u) {
-			const unsigned q = q0 + u;
-			j[u] = 0;
-			v[u] = VT(0);
-			if (q < len) {
-				j[u] = __builtin_nontemporal_load(ip + (size_t)q * kSellRows);
-				v[u] = Val<R>::load(val, e0 + (size_t)q * kSellRows + lane);
+			for (int u = 0; u < U; ++u) {
+				const unsigned q = q0 + u;  // < 32 + U <= 48
+				const bool in = q < w && ((bits >> q) & 1u);
+				const long long d = __builtin_amdgcn_readlane(dl, (int)q);
+				v[u] = in ? Val<R>::load(dv, e0 + (size_t)q * kSellRows + lane) : VT(0);
+				xv[u] = in ? x[(long long)r + d] : X(0);
 			}
+#pragma unroll
+			for (int u = 0; u < U; ++u)
+				if (q0 + u < w && ((bits >> (q0 + u)) & 1u)) acc = R::add(Val<R>::mul(v[u], xv[u]), acc);
 		}
-		X xv[U];
+	} else {
+		if (active) len = (unsigned)(off[r + 1] - off[r]);
+		const uint32_t *ip = sidx + e0 + lane;
+		for (unsigned q0 = 0; q0 < w; q0 += U) {
+			uint32_t j[U];
+			VT v[U];
 #pragma unroll
-		for (int u = 0; u < U; ++u) xv[u] = q0 + u < len ? x[j[u]] : X(0);
+			for (int u = 0; u < U; ++u) {
+				const unsigned q = q0 + u;
+				j[u] = 0;
+				v[u] = VT(0);
+				if (q < len) {
+					j[u] = __builtin_nontemporal_load(ip + (size_t)q * kSellRows);
+					v[u] = Val<R>::load(val, e0 + (size_t)q * kSellRows + lane);
+				}
+			}
+			X xv[U];
 #pragma unroll
-		for (int u = 0; u < U; ++u)
-			if (q0 + u < len) acc = R::add(Val<R>::mul(v[u], xv[u]), acc);
+			for (int u = 0; u < U; ++u) xv[u] = q0 + u < len ? x[j[u]] : X(0);
+#pragma unroll
+			for (int u = 0; u < U; ++u)
+				if (q0 + u < len) acc = R::add(Val<R>::mul(v[u], xv[u]), acc);
+		}
 	}
 	if (!active) return;
 	if (len == 0) {
 		if (OVERWRITE) y[r] = static_cast<Y>(R::zero());
 	} else {
-		y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(yv, acc));
+		y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(static_cast<A>(y[r]), acc));
 	}
 }
 
@@ -596,21 +622,30 @@ static void launch_stream(int variant, int tile, const Storage &S, const void *v
 }
 
 // sliced ELL: U (`rows`) entries per lane in flight: 4, 8 or 16; `lanes` = 1 maps
-// slices to XCDs contiguously, 0 keeps plain block order
+// slices to XCDs contiguously, 0 keeps plain block order. dia: the slices in
+// diagonal form take the index-free path with values dval (null for patterns).
 template <typename R, int MASK, bool OW>
-static void launch_sell(int U, int xcd, const Storage &S, const void *sval, const void *x, void *y,
-                        const uint8_t *mask, hipStream_t s) {
+static void launch_sell(int U, int xcd, const Storage &S, const void *sval, bool dia, const void *dval,
+                        const void *x, void *y, const uint8_t *mask, hipStream_t s) {
 	const size_t per = kBlock / kSellRows;
 	const size_t grid = (S.n_slices + per - 1) / per;
 	if (grid == 0) return;
-#define ALP_SELL(UU, XX)                                                                           \
-	hipLaunchKernelGGL((k_spmv_sell<R, MASK, OW, UU, XX>), dim3(grid), dim3(kBlock), 0, s, S.nmajor, \
-	                   S.n_slices, (const uint64_t *)S.sell_off, (const uint32_t *)S.sell_idx, sval,   \
-	                   (const uint64_t *)S.off, (const typename R::X *)x, (typename R::Y *)y, mask)
-	if (xcd) {
-		if (U == 16) ALP_SELL(16, true); else if (U == 4) ALP_SELL(4, true); else ALP_SELL(8, true);
+	const DiaSlices &D = S.dia;
+#define ALP_SELL(UU, XX, DD)                                                                            \
+	hipLaunchKernelGGL((k_spmv_sell<R, MASK, OW, UU, XX, DD>), dim3(grid), dim3(kBlock), 0, s, S.nmajor, \
+	                   S.n_slices, (const uint64_t *)S.sell_off, (const uint32_t *)S.sell_idx, sval,      \
+	                   (const uint64_t *)S.off, (const typename R::X *)x, (typename R::Y *)y, mask,       \
+	                   (const uint8_t *)D.flag, (const int32_t *)D.col, (const uint32_t *)D.mask, dval)
+	if (dia) {
+		if (xcd) {
+			if (U == 16) ALP_SELL(16, true, true); else if (U == 4) ALP_SELL(4, true, true); else ALP_SELL(8, true, true);
+		} else {
+			if (U == 16) ALP_SELL(16, false, true); else if (U == 4) ALP_SELL(4, false, true); else ALP_SELL(8, false, true);
+		}
+	} else if (xcd) {
+		if (U == 16) ALP_SELL(16, true, false); else if (U == 4) ALP_SELL(4, true, false); else ALP_SELL(8, true, false);
 	} else {
-		if (U == 16) ALP_SELL(16, false); else if (U == 4) ALP_SELL(4, false); else ALP_SELL(8, false);
+		if (U == 16) ALP_SELL(16, false, false); else if (U == 4) ALP_SELL(4, false, false); else ALP_SELL(8, false, false);
 	}
 #undef ALP_SELL
 }
@@ -633,6 +668,24 @@ static bool sell_values(const Storage &S, const void *val, const void **sval) {
 	return false;
 }
 
+// the same for the diagonal slices; pattern matrices (val null) need no values
+static bool dia_values(const Storage &S, const void *val, const void **dval) {
+	if (!S.dia.flag) return false;
+	if (val == nullptr) {
+		*dval = nullptr;
+		return true;
+	}
+	if (val == S.val && S.dia.val) {
+		*dval = S.dia.val;
+		return true;
+	}
+	if (val == S.val32 && S.dia.val32) {
+		*dval = S.dia.val32;
+		return true;
+	}
+	return false;
+}
+
 template <typename R, int MASK, bool OW>
 static void launch_wave(const Storage &S, const void *val, const void *x, void *y,
                         const uint8_t *mask, hipStream_t s) {
@@ -648,6 +701,7 @@ static void launch_wave(const Storage &S, const void *val, const void *x, void *
 // without autotuning: sliced ELL where it was built (regular rows), CSR-stream
 // (whole-block folds of long rows) for skewed row lengths, CSR-vector otherwise
 int auto_kernel(const Storage &S) {
+	if (S.dia.flag) return 5;
 	if (S.sell_off) return 4;
 	return (S.max_len > 64 && (double)S.max_len > 8.0 * S.avg_len) ? 2 : 1;
 }
@@ -670,12 +724,18 @@ static void dispatch(const Storage &S, const void *val, unsigned descr, void *y,
 	int G = S.lanes ? S.lanes : default_lanes(S);
 	int RR = S.rows;  // CSR-stream: the launch variant
 	if (kernel == 0) kernel = auto_kernel(S);
-	const void *sval = nullptr;
+	const void *sval = nullptr, *dval = nullptr;
+	bool dia = false;
+	if (kernel == 5) {
+		dia = dia_values(S, val, &dval);
+		kernel = 4;
+	}
 	if (kernel == 4 && !sell_values(S, val, &sval)) kernel = 2;  // no sliced-ELL copy of these values
+	if (kernel != 4) dia = false;
 	if (RR == 0) RR = kernel == 2 ? 8 : kernel == 4 ? 8 : 1;
 #define ALP_GO(MM, OWV)                                                 \
 	if (kernel == 4)                                                    \
-		launch_sell<R, MM, OWV>(RR, S.lanes, S, sval, x, y, mask, s);   \
+		launch_sell<R, MM, OWV>(RR, S.lanes, S, sval, dia, dval, x, y, mask, s); \
 	else if (kernel == 3)                                               \
 		launch_wave<R, MM, OWV>(S, val, x, y, mask, s);                 \
 	else if (kernel == 2)                                               \
@@ -763,11 +823,13 @@ ALP_EXPORT alp_rc_t alp_hip_vxm(alp_hip_matrix_t A, alp_semiring_t ring, unsigne
 }
 
 ALP_EXPORT alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes, int rows) {
-	if (!A || kernel < 0 || kernel > 4) return ALP_ILLEGAL;
-	if (kernel == 4) {
-		// sliced ELL: lanes 0/1 = plain / XCD-contiguous slice order, rows = entries in flight
+	if (!A || kernel < 0 || kernel > 5) return ALP_ILLEGAL;
+	if (kernel == 4 || kernel == 5) {
+		// sliced ELL (5: with its diagonal slices): lanes 0/1 = plain / XCD-contiguous
+		// slice order, rows = entries in flight
 		if (!A->crs.sell_off || (lanes != 0 && lanes != 1) || (rows != 0 && rows != 4 && rows != 8 && rows != 16))
 			return ALP_ILLEGAL;
+		if (kernel == 5 && !A->crs.dia.flag) return ALP_ILLEGAL;
 		A->crs.kernel = A->ccs.kernel = kernel;
 		A->crs.lanes = A->ccs.lanes = lanes;
 		A->crs.rows = A->ccs.rows = rows;
@@ -789,17 +851,18 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, i
 	if (!A) return ALP_ILLEGAL;
 	int k = A->crs.kernel;
 	if (k == 0) k = auto_kernel(A->crs);
-	int l = k == 4 ? A->crs.lanes : (A->crs.lanes ? A->crs.lanes : default_lanes(A->crs));
+	int l = k >= 4 ? A->crs.lanes : (A->crs.lanes ? A->crs.lanes : default_lanes(A->crs));
 	if (kernel) *kernel = k;
 	if (lanes) *lanes = l;
-	if (rows) *rows = A->crs.rows ? A->crs.rows : (k == 2 || k == 4 ? 8 : 1);
+	if (rows) *rows = A->crs.rows ? A->crs.rows : (k == 2 || k >= 4 ? 8 : 1);
 	return ALP_SUCCESS;
 }
 
 // Autotuner: times each candidate on this matrix's CRS with its natural semiring
 // (3 warm-up + 10 timed runs each, hipEvents on the library stream) and keeps
 // the fastest. Candidates: CSR-vector with G = 2..64 lanes per row and 1, 2 or 4
-// rows per lane group, and CSR-stream. The CCS gets the same choice.
+// rows per lane group, CSR-stream, CSR-wave, and where built the sliced ELL and
+// its diagonal slices. The CCS gets the same choice where it has the same form.
 ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 	ALP_TRY(ensure_init());
 	if (!A) return ALP_ILLEGAL;
@@ -825,6 +888,9 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 	if (A->crs.sell_off)
 		for (int xcd : {0, 1})
 			for (int u : {4, 8, 16}) cands.push_back({4, xcd, u});
+	if (A->crs.dia.flag)
+		for (int xcd : {0, 1})
+			for (int u : {4, 8, 16}) cands.push_back({5, xcd, u});
 	float best = 1e30f;
 	Cand pick = {1, default_lanes(A->crs), 1};
 	for (const Cand &c : cands) {

@@ -121,7 +121,11 @@ alp_rc_t alp_hip_matrix_download(alp_hip_matrix_t A, int which, uint64_t *h_off,
  * per lane group: 1, 2 or 4; 0 = default), 2 = CSR-stream (`rows` picks the launch:
  * 1 = persistent grid 4 blocks/CU, 2 = persistent 8 blocks/CU, 4 = a block per tile,
  * 8 = a block per tile in plain block order; `lanes` 16 = 4096-nonzero tiles),
- * 3 = CSR-wave (a 512-nonzero tile per wavefront) */
+ * 3 = CSR-wave (a 512-nonzero tile per wavefront), 4 = sliced ELL (64-row slices,
+ * one lane per row; `lanes` 0/1 = plain / XCD-contiguous slice order, `rows` 4, 8
+ * or 16 entries in flight), 5 = the sliced ELL with its index-free diagonal slices
+ * (same arguments). 4 and 5 exist only for matrices with regular rows, 5 only when
+ * most slices share one column-offset pattern per slice (stencils). */
 alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes, int rows);
 alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes, int *rows);
 /* time each candidate SpMV kernel on this matrix and keep the fastest */

@@ -108,9 +108,11 @@ KERNELS = [(1, 2, 1), (1, 4, 2), (1, 8, 4), (1, 16, 1), (1, 16, 4), (1, 32, 2), 
            (3, 0, 0)]
 # sliced ELL (kernel 4), built for matrices with regular rows: plain / XCD order, 4-16 in flight
 SELL = [(4, 0, 4), (4, 0, 8), (4, 0, 16), (4, 1, 8)]
+# the same with its index-free diagonal slices (kernel 5), built for stencils
+DIA = [(5, 0, 4), (5, 0, 8), (5, 0, 16), (5, 1, 8)]
 
 
-@pytest.mark.parametrize("kernel,lanes,rows", KERNELS + SELL)
+@pytest.mark.parametrize("kernel,lanes,rows", KERNELS + SELL + DIA)
 @pytest.mark.parametrize("sizes", [(64, 64, 64), (192, 192, 192)])
 def test_mxv_ones_closed_form_bitexact(A, sizes, kernel, lanes, rows):
     M = A.Matrix.stencil27(*sizes)
@@ -126,7 +128,7 @@ def test_mxv_ones_closed_form_bitexact(A, sizes, kernel, lanes, rows):
     assert np.array_equal(y.numpy(), closed_form_A1(*sizes))
 
 
-@pytest.mark.parametrize("kernel,lanes,rows", KERNELS + SELL)
+@pytest.mark.parametrize("kernel,lanes,rows", KERNELS + SELL + DIA)
 @pytest.mark.parametrize("name,sizes", [("mxv_8x8x8", (8, 8, 8)), ("mxv_5x9x7", (5, 9, 7)),
                                         ("mxv_16x16x16", (16, 16, 16))])
 def test_mxv_matches_reference_fixture(A, golden, name, sizes, kernel, lanes, rows):
@@ -142,15 +144,16 @@ def test_mxv_matches_reference_fixture(A, golden, name, sizes, kernel, lanes, ro
     assert rel_inf(yt.numpy(), g["yt"]) <= FP64_RTOL
 
 
-@pytest.mark.parametrize("kernel,lanes,rows", SELL)
+@pytest.mark.parametrize("kernel,lanes,rows", SELL + DIA)
 @pytest.mark.parametrize("name,sizes", [("mxv_8x8x8", (8, 8, 8)), ("mxv_5x9x7", (5, 9, 7)),
                                         ("mxv_16x16x16", (16, 16, 16))])
 def test_sell_mxv_bit_identical_to_reference(A, golden, name, sizes, kernel, lanes, rows):
-    """The sliced-ELL kernel folds each row sequentially in storage order without
-    FMA contraction: the reference's own order, so y is identical bit for bit."""
+    """The sliced-ELL kernel and its diagonal slices fold each row sequentially in
+    storage order without FMA contraction: the reference's own order, so y is
+    identical bit for bit."""
     g = golden(name)
     M = A.Matrix.stencil27(*sizes)
-    assert M.get_kernel()[0] == 4  # the default for a stencil
+    assert M.get_kernel()[0] == 5  # the default for a stencil
     M.set_kernel(kernel, lanes, rows)
     y = A.Vector.from_numpy(g["y0"])
     assert A.mxv(y, M, A.Vector.from_numpy(g["x"])) == A.SUCCESS
@@ -163,9 +166,9 @@ def test_sell_pattern_lorland_and_plus_pattern(A):
     st = O.stencil(20, 18, 16)
     n = len(st.off) - 1
     M = A.Matrix.from_crs(n, n, st.off, st.idx)  # pattern (no values)
-    assert M.get_kernel()[0] == 4
+    assert M.get_kernel()[0] == 5
     rng = np.random.default_rng(21)
-    for kernel, lanes, rows in SELL:
+    for kernel, lanes, rows in SELL + DIA:
         M.set_kernel(kernel, lanes, rows)
         x = (rng.random(n) < 0.05).astype(np.uint8)
         y0 = (rng.random(n) < 0.1).astype(np.uint8)
@@ -180,6 +183,49 @@ def test_sell_pattern_lorland_and_plus_pattern(A):
         assert A.mxv(y, M, A.Vector.from_numpy(xs), ring=A.PLUS_PATTERN_F64) == A.SUCCESS
         want = O.spmv(O.CSR(n, n, st.off, st.idx, np.ones(st.nnz)), xs)
         assert np.max(np.abs(y.numpy() - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+def test_dia_slices_mixed_with_sliced_ell(A):
+    """A stencil whose every third slice holds one row with its entries reversed:
+    those slices do not share one slot order and stay sliced ELL, the others take
+    the diagonal path. Kernel 5 must equal kernel 4 bit for bit (both fold each
+    row in storage order) and the oracle to FP64 round-off, in both accumulate
+    and overwrite mode, and under an output mask."""
+    st = O.stencil(24, 20, 18)
+    n = len(st.off) - 1
+    rng = np.random.default_rng(5)
+    off, idx = st.off.copy(), st.idx.copy()
+    val = rng.standard_normal(st.nnz)
+    for s in range(0, n // 64, 3):
+        r = s * 64 + 10
+        a, b = int(off[r]), int(off[r + 1])
+        idx[a:b] = idx[a:b][::-1].copy()
+        val[a:b] = val[a:b][::-1].copy()
+    M = A.Matrix.from_crs(n, n, off, idx, val)
+    assert M.get_kernel()[0] == 5
+    csr = O.CSR(n, n, off, idx, val)
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(n)
+    mask = (rng.random(n) < 0.5).astype(np.uint8)
+    want = O.spmv(csr, x, y0)
+    outs = {}
+    for kernel, lanes, rows in SELL + DIA:
+        M.set_kernel(kernel, lanes, rows)
+        y = A.Vector.from_numpy(y0)
+        assert A.mxv(y, M, A.Vector.from_numpy(x)) == A.SUCCESS
+        assert rel_inf(y.numpy(), want) <= FP64_RTOL, (kernel, lanes, rows)
+        yo = A.Vector(n, fill=7.0)
+        assert A.mxv(yo, M, A.Vector.from_numpy(x), descr=A.OVERWRITE) == A.SUCCESS
+        ym = A.Vector.from_numpy(y0)
+        assert A.mxv(ym, M, A.Vector.from_numpy(x), mask=A.Vector.from_numpy(mask)) == A.SUCCESS
+        outs[(kernel, lanes, rows)] = (y.numpy(), yo.numpy(), ym.numpy())
+    base = outs[SELL[0]]
+    for key, got in outs.items():
+        for a, b in zip(base, got):
+            assert np.array_equal(a, b), key
+    ym = base[2]
+    assert np.array_equal(ym[mask == 0], y0[mask == 0])
+    assert np.array_equal(ym[mask != 0], base[0][mask != 0])
 
 
 def test_mxv_random_matrix_vs_oracle_all_kernels(A):
@@ -264,7 +310,8 @@ def test_autotune_picks_a_valid_kernel(A):
     valid = {1: lambda l, r: l in (2, 4, 8, 16, 32, 64) and r in (1, 2, 4),
              2: lambda l, r: l in (8, 16) and r in (1, 2, 4, 8, 16, 32),
              3: lambda l, r: True,
-             4: lambda l, r: l in (0, 1) and r in (4, 8, 16)}
+             4: lambda l, r: l in (0, 1) and r in (4, 8, 16),
+             5: lambda l, r: l in (0, 1) and r in (4, 8, 16)}
     assert k in valid and valid[k](lanes, rows), (k, lanes, rows)
     x = A.Vector(M.nrows, fill=1.0)
     y = A.Vector(M.nrows, fill=0.0)
