@@ -250,12 +250,21 @@ namespace alp {
 
 alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32);
 alp_rc_t storage_refresh_values(Storage &S, int value_type);
+// gives an FP64 storage its FP32 value copy (and the sliced-ELL / diagonal-slice
+// copies of it) when it has none yet
+alp_rc_t storage_ensure_f32(Storage &S, size_t nnz);
 void storage_free(Storage &S);
 alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A);
 
 // SpMV entry used by mxv/vxm and by the solvers (one storage, one semiring).
+// skip (optional): a device flag; when it is set at launch time the product is not
+// computed (solver loops replayed past their stopping test)
 alp_rc_t spmv(const Storage &S, int value_type, alp_semiring_t ring, unsigned descr, void *y,
-              const void *x, const uint8_t *mask, hipStream_t s);
+              const void *x, const uint8_t *mask, hipStream_t s, const int *skip = nullptr);
+
+// y = A x over the FP32 value copy with FP32 vectors and an FP64 accumulator per
+// row (overwrite, no mask): the product of the mixed-precision solvers
+alp_rc_t spmv_f32_acc64(const Storage &S, float *y, const float *x, hipStream_t s, const int *skip = nullptr);
 
 // a host-side preconditioner z = M^-1 r: fn(z, r, data), non-zero return = failure
 template <typename T>

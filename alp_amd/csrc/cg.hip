@@ -41,10 +41,11 @@ struct CgScal {
 
 struct CgState {
 	CgScal s[2];
-	double tol_abs;    // tol * sqrt(b.b)
+	double tol_abs;    // the stopping threshold on sqrt(r.r): tol * sqrt(b.b), or a
+	                   // reliable-update segment's max(tol ||b||, delta ||r_true||)
 	double bnorm2;
 	double r0norm;     // sqrt of the first r.r (inner solves)
-	double pad;
+	double tol_true;   // reliable updates: tol * ||b|| for the FP64 residual
 	unsigned long long max_iter;
 };
 
@@ -59,6 +60,7 @@ struct CgArgs {
 	double *partials;  // kMaxPartials doubles: [0, half) p.Ap, [half, 2 half) r.r (or r.z)
 	unsigned *ticket;
 	CgState *st;
+	double *x64 = nullptr;  // set: the solution is FP64 (x unused), x64 += alpha u in FP64
 };
 
 constexpr unsigned kHalfPartials = kMaxPartials / 2;
@@ -132,14 +134,24 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a, const CgScal 
 	__shared__ double lds[kBlock / kWave];
 	if (in->done) return;
 	const double pAp = fold_partials(a.partials, npap, lds);
-	const T alpha = (T)(in->sigma / pAp);
+	const double alpha64 = in->sigma / pAp;
+	const T alpha = (T)alpha64;
 	double part = 0.0;
-	grid_for_each<4>(a.n, [&](size_t i) {
-		a.x[i] = alpha * a.u[i] + a.x[i];
-		const T ri = a.r[i] - alpha * a.temp[i];
-		a.r[i] = ri;
-		part += (double)ri * (double)ri;
-	});
+	if (a.x64) {
+		grid_for_each<4>(a.n, [&](size_t i) {
+			a.x64[i] = alpha64 * (double)a.u[i] + a.x64[i];
+			const T ri = a.r[i] - alpha * a.temp[i];
+			a.r[i] = ri;
+			part += (double)ri * (double)ri;
+		});
+	} else {
+		grid_for_each<4>(a.n, [&](size_t i) {
+			a.x[i] = alpha * a.u[i] + a.x[i];
+			const T ri = a.r[i] - alpha * a.temp[i];
+			a.r[i] = ri;
+			part += (double)ri * (double)ri;
+		});
+	}
 	part = block_sum(part, lds);
 	if (threadIdx.x == 0) a.partials[kHalfPartials + blockIdx.x] = part;
 }
@@ -251,11 +263,90 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(CgArgs<T> a, const T *z, con
 	}
 	const double rz = fold_partials(a.partials + kHalfPartials, nrz, lds);
 	const T ratio = first ? T(0) : (T)(rz / in->sigma);
-	grid_for_each<4>(a.n, [&](size_t i) { a.u[i] = ratio * a.u[i] + z[i]; });
+	if (first)
+		grid_for_each<4>(a.n, [&](size_t i) { a.u[i] = z[i]; });
+	else
+		grid_for_each<4>(a.n, [&](size_t i) { a.u[i] = ratio * a.u[i] + z[i]; });
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
 		CgScal o = *in;
 		o.sigma = rz;
 		*out = o;
+	}
+}
+
+// ---- reliable updates (mixed precision) -------------------------------------------
+//
+// CG iterations on the FP32 copy of A with FP32 vectors r, p, Ap; the product
+// accumulates each row in FP64 (one rounding per row) and x is kept and updated
+// in FP64 (x += alpha p). Both matter: FP32 row sums of a stencil cancel heavily
+// on smooth vectors, and an FP32 solution increment re-injects its rounding into
+// every true residual; with both in FP64 the iteration count stays at the FP64
+// method's. A segment of iterations ends when the recurrence residual drops below
+// max(tol ||b||, delta ||r_true||); then r_true = b - A x in FP64, r = (float)
+// r_true and p = r + (r.r / sigma) p, keeping the search direction (reliable
+// updated residuals: Sleijpen and van der Vorst, 1996; the double/single scheme
+// of Clark et al., 2010). Convergence is decided on the FP64 residual:
+// ||b - A x|| < tol ||b||.
+constexpr unsigned kQuarterPartials = kMaxPartials / 4;
+
+// r32 = (float)(b - t) (t = A x); partials of r_true.r_true, r32.r32 and (first) b.b
+__global__ __launch_bounds__(kBlock) void k_ru_resid(size_t n, const double *b, const double *t, float *r32,
+                                                    double *partials, int first) {
+	__shared__ double lds[kBlock / kWave];
+	double pt = 0.0, pf = 0.0, pb = 0.0;
+	grid_for_each<4>(n, [&](size_t i) {
+		const double ri = b[i] - t[i];
+		const float rf = (float)ri;
+		r32[i] = rf;
+		pt += ri * ri;
+		pf += (double)rf * (double)rf;
+		if (first) pb += b[i] * b[i];
+	});
+	pt = block_sum(pt, lds);
+	pf = block_sum(pf, lds);
+	pb = block_sum(pb, lds);
+	if (threadIdx.x == 0) {
+		partials[blockIdx.x] = pt;
+		partials[kQuarterPartials + blockIdx.x] = pf;
+		partials[2 * kQuarterPartials + blockIdx.x] = pb;
+	}
+}
+
+// p = r32 + (r32.r32 / sigma) p, or p = r32 on the first call (sigma from slot 0)
+__global__ __launch_bounds__(kBlock) void k_ru_dir(size_t n, const float *r32, float *p, const double *partials,
+                                                  unsigned np, const CgState *st, int first) {
+	__shared__ double lds[kBlock / kWave];
+	const double rr = block_sum_partials(partials + kQuarterPartials, np, lds);
+	if (first) {  // p holds no direction yet (and possibly no finite values)
+		grid_for_each<4>(n, [&](size_t i) { p[i] = r32[i]; });
+		return;
+	}
+	const float beta = (float)(rr / st->s[0].sigma);
+	grid_for_each<4>(n, [&](size_t i) { p[i] = beta * p[i] + r32[i]; });
+}
+
+// one block: the state after a reliable update (both slots), the FP64 stopping
+// test and the next segment's threshold
+__global__ __launch_bounds__(kBlock) void k_ru_fin(const double *partials, unsigned np, CgState *st, double tol,
+                                                  double delta, unsigned long long max_iter, int first) {
+	__shared__ double lds[kBlock / kWave];
+	const double rt = block_sum_partials(partials, np, lds);
+	const double rf = block_sum_partials(partials + kQuarterPartials, np, lds);
+	const double bb = first ? block_sum_partials(partials + 2 * kQuarterPartials, np, lds) : 0.0;
+	if (threadIdx.x == 0) {
+		if (first) {
+			st->bnorm2 = bb;
+			st->tol_true = tol * std::sqrt(bb);
+			st->max_iter = max_iter;
+		}
+		const unsigned long long iter = first ? 0ull : st->s[0].iter;
+		const bool conv = std::sqrt(rt) < st->tol_true;
+		const bool done = conv || iter >= st->max_iter;
+		const CgScal o{rf, rt, iter, done ? 1 : 0, conv ? 1 : 0};
+		st->s[0] = o;
+		st->s[1] = o;
+		st->r0norm = std::sqrt(rt);
+		st->tol_abs = std::fmax(st->tol_true, delta * std::sqrt(rt));
 	}
 }
 
@@ -338,7 +429,7 @@ struct CgSlot {
 	std::vector<size_t> sizes;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
-	const void *key[5] = {};
+	const void *key[6] = {};
 	int kcfg[3] = {-1, -1, -1};
 
 	void drop_graph() {
@@ -371,7 +462,17 @@ struct CgSlot {
 	}
 };
 
-enum CgSlotId { kSlotFp64 = 0, kSlotMixedInner, kSlotNativeF32, kSlotNativeF64, kSlotTransposed, kSlotMixedInnerT, kSlots };
+enum CgSlotId {
+	kSlotFp64 = 0,
+	kSlotMixedInner,
+	kSlotNativeF32,
+	kSlotNativeF64,
+	kSlotTransposed,
+	kSlotMixedInnerT,
+	kSlotReliable,
+	kSlotReliableT,
+	kSlots
+};
 
 struct CgCache {
 	CgSlot slot[kSlots];
@@ -408,13 +509,16 @@ struct CgRunner {
 	bool fused;
 	unsigned gspmv, gvec;
 
-	CgRunner(const Storage &S_, CgArgs<T> a_, int vt_ = ALP_VAL_F64) : S(S_), a(a_), vt(vt_) {
+	bool acc64 = false;  // T = float: the product accumulates rows in FP64 (spmv_f32_acc64)
+
+	CgRunner(const Storage &S_, CgArgs<T> a_, int vt_ = ALP_VAL_F64, bool acc64_ = false)
+	    : S(S_), a(a_), vt(vt_), acc64(acc64_) {
 		G = S.lanes ? S.lanes : default_lanes(S);
 		RR = S.rows ? S.rows : 1;
 		// the fused SpMV + dot kernel is the CSR-vector gather; the other
 		// kernels run as a generic SpMV followed by the dot
 		const int k = S.kernel ? S.kernel : auto_kernel(S);
-		fused = k == 1 && G >= 2;
+		fused = k == 1 && G >= 2 && !acc64;
 		const size_t rows_per = (size_t)(kBlock / G) * RR;
 		const size_t nchunks = (a.n + rows_per - 1) / rows_per;
 		size_t g = fill_blocks(8);
@@ -456,9 +560,13 @@ struct CgRunner {
 #undef ALP_CG_CASE
 		} else {
 			// for an FP64 matrix the FP32 semiring selects its FP32 value copy
-			ALP_TRY(spmv(S, vt,
-			             std::is_same<T, float>::value ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64,
-			             ALP_DESC_OVERWRITE, a.temp, a.u, nullptr, s));
+			// the product is skipped once the slot says done, as the other kernels are
+			if (acc64 && std::is_same<T, float>::value)
+				ALP_TRY(spmv_f32_acc64(S, (float *)a.temp, (const float *)a.u, s, &in->done));
+			else
+				ALP_TRY(spmv(S, vt,
+				             std::is_same<T, float>::value ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64,
+				             ALP_DESC_OVERWRITE, a.temp, a.u, nullptr, s, &in->done));
 			hipLaunchKernelGGL(k_cg_dot<T>, dim3(gvec), dim3(kBlock), 0, s, a, in);
 		}
 		hipLaunchKernelGGL(k_cg_update<T>, dim3(gvec), dim3(kBlock), 0, s, a, in, npap);
@@ -484,9 +592,9 @@ struct CgRunner {
 		hipGraphExec_t exec = nullptr;
 		hipGraph_t graph = nullptr;
 		bool use_graph = false;
-		const void *key[5] = {&S, a.x, a.b, a.val, a.st};
+		const void *key[6] = {&S, a.x, a.b, a.val, a.st, a.x64};
 		const int kcfg[3] = {S.kernel, S.lanes, S.rows};
-		if (slot && slot->exec && std::equal(key, key + 5, slot->key) && std::equal(kcfg, kcfg + 3, slot->kcfg)) {
+		if (slot && slot->exec && std::equal(key, key + 6, slot->key) && std::equal(kcfg, kcfg + 3, slot->kcfg)) {
 			exec = slot->exec;
 			use_graph = true;
 		} else {
@@ -502,18 +610,18 @@ struct CgRunner {
 			if (slot && use_graph) {
 				slot->graph = graph;
 				slot->exec = exec;
-				std::copy(key, key + 5, slot->key);
+				std::copy(key, key + 6, slot->key);
 				std::copy(kcfg, kcfg + 3, slot->kcfg);
 			}
 		}
+		auto launch = [&]() -> alp_rc_t {
+			if (use_graph) return hipGraphLaunch(exec, s) == hipSuccess ? ALP_SUCCESS : ALP_PANIC;
+			for (int i = 0; i < batch; ++i) ALP_TRY(iteration(s, i % 2));
+			return ALP_SUCCESS;
+		};
 		alp_rc_t rc = ALP_SUCCESS;
 		for (;;) {
-			if (use_graph) {
-				if (hipGraphLaunch(exec, s) != hipSuccess) { rc = ALP_PANIC; break; }
-			} else {
-				for (int i = 0; i < batch && rc == ALP_SUCCESS; ++i) rc = iteration(s, i % 2);
-				if (rc) break;
-			}
+			if ((rc = launch()) != ALP_SUCCESS) break;
 			if (hipMemcpyAsync(h_state, a.st, sizeof(CgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
 			    hipStreamSynchronize(s) != hipSuccess) {
 				rc = ALP_PANIC;
@@ -554,10 +662,7 @@ static alp_rc_t cg_fp64(alp_hip_matrix_t A, Storage &S, const double *b, double 
 static alp_rc_t cg_mixed(alp_hip_matrix_t A, Storage &S, const double *b, double *x, size_t max_iter,
                          double tol, size_t *iters, double *residual) {
 	const size_t n = A->nrows;
-	if (!S.val32) {
-		ALP_HIP_TRY(hipMalloc(&S.val32, sizeof(float) * (A->nnz ? A->nnz : 1)));
-		ALP_TRY(alp_hip_convert_f64_to_f32(A->nnz, (const double *)S.val, S.val32));
-	}
+	ALP_TRY(storage_ensure_f32(S, A->nnz));
 	hipStream_t s = stream();
 	CgSlot &slot = cg_slot(A, &S == &A->ccs ? kSlotMixedInnerT : kSlotMixedInner);
 	void *bp[9];
@@ -596,6 +701,49 @@ static alp_rc_t cg_mixed(alp_hip_matrix_t A, Storage &S, const double *b, double
 	*iters = total;
 	*residual = hir.rnorm * hir.rnorm;
 	return hir.done ? ALP_SUCCESS : ALP_FAILED;
+}
+
+// Mixed precision with reliable updates (see k_ru_resid): FP32 segments replayed
+// by the CG runner, FP64 residual updates between them. *iters counts the FP32
+// iterations; *residual = ||b - A x||^2 of the returned x.
+static alp_rc_t cg_reliable(alp_hip_matrix_t A, Storage &S, const double *b, double *x, size_t max_iter,
+                            double tol, size_t *iters, double *residual) {
+	const double delta = 0.1;
+	const size_t n = A->nrows;
+	ALP_TRY(storage_ensure_f32(S, A->nnz));
+	hipStream_t s = stream();
+	CgSlot &slot = cg_slot(A, &S == &A->ccs ? kSlotReliableT : kSlotReliable);
+	void *bp[5];
+	ALP_TRY(slot.buffers({8 * n, 4 * n, 4 * n, 4 * n, sizeof(CgState)}, bp));
+	double *t64 = (double *)bp[0];
+	float *r32 = (float *)bp[1], *p32 = (float *)bp[2], *ap32 = (float *)bp[3];
+	CgState *st = (CgState *)bp[4];
+	CgArgs<float> a{n, S.off, S.idx, S.val32, nullptr, r32, p32, ap32, nullptr, ctx().d_partials,
+	                ctx().d_tickets, st, x};
+	CgRunner<float> runner(S, a, ALP_VAL_F64, true);
+	unsigned g = vec_grid(n);
+	if (g > kQuarterPartials) g = kQuarterPartials;
+	double *part = ctx().d_partials;
+	CgState h{};
+	for (int first = 1;; first = 0) {
+		// r = b - A x in FP64; r32, p and the state from it
+		ALP_TRY(spmv(S, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, t64, x, nullptr, s));
+		hipLaunchKernelGGL(k_ru_resid, dim3(g), dim3(kBlock), 0, s, n, b, (const double *)t64, r32, part, first);
+		hipLaunchKernelGGL(k_ru_dir, dim3(g), dim3(kBlock), 0, s, n, (const float *)r32, p32, (const double *)part, g,
+		                   (const CgState *)st, first);
+		hipLaunchKernelGGL(k_ru_fin, dim3(1), dim3(kBlock), 0, s, (const double *)part, g, st, tol, delta,
+		                   (unsigned long long)max_iter, first);
+		ALP_LAUNCH_CHECK();
+		ALP_HIP_TRY(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+		ALP_HIP_TRY(hipStreamSynchronize(s));
+		if (h.s[0].done) break;
+		// FP32 iterations (x updated in FP64) until the recurrence residual reaches
+		// the segment's threshold
+		ALP_TRY(runner.run(s, &h, &slot));
+	}
+	*iters = h.s[0].iter;
+	*residual = h.s[0].rr;
+	return h.s[0].converged ? ALP_SUCCESS : ALP_FAILED;
 }
 
 // CG in the arithmetic of T on A's own values: the transition solver's path
@@ -693,7 +841,8 @@ static alp_rc_t cg_entry(alp_hip_matrix_t A, bool transpose, const double *b, do
 	if (transpose && !A->ccs.off) ALP_TRY(build_ccs_from_crs(A));
 	Storage &S = transpose ? A->ccs : A->crs;
 	if (precision == 0) return cg_fp64(A, S, b, x, max_iter, tol, iters, residual);
-	if (precision == 1) return cg_mixed(A, S, b, x, max_iter, tol, iters, residual);
+	if (precision == 1) return cg_reliable(A, S, b, x, max_iter, tol, iters, residual);
+	if (precision == 2) return cg_mixed(A, S, b, x, max_iter, tol, iters, residual);
 	return ALP_ILLEGAL;
 }
 

@@ -1117,20 +1117,20 @@ static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st
 	(void)hipGetLastError();
 	const int check = 4;
 	static_assert(check % 2 == 0, "checks read slot 0");
-	alp_rc_t rc = ALP_SUCCESS;
-	for (;;) {
+	auto launch = [&]() -> alp_rc_t {
 		for (int i = 0; i < check; i += 2) {
 			if (use_graph) {
-				if (hipGraphLaunch(exec, s) != hipSuccess) {
-					rc = ALP_PANIC;
-					break;
-				}
+				if (hipGraphLaunch(exec, s) != hipSuccess) return ALP_PANIC;
 			} else {
 				hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 0, s);
 				hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 1, s);
 			}
 		}
-		if (rc) break;
+		return ALP_SUCCESS;
+	};
+	alp_rc_t rc = ALP_SUCCESS;
+	for (;;) {
+		if ((rc = launch()) != ALP_SUCCESS) break;
 		if (hipMemcpyAsync(h_st, st, sizeof(HpState), hipMemcpyDeviceToHost, s) != hipSuccess ||
 		    hipStreamSynchronize(s) != hipSuccess) {
 			rc = ALP_PANIC;

@@ -499,6 +499,29 @@ alp_rc_t storage_refresh_values(Storage &S, int value_type) {
 	return dia_refresh(S.dia, S.n_slices, S.nmajor, nullptr, S.sell_off, S.off, value_type, S.val, S.val32);
 }
 
+alp_rc_t storage_ensure_f32(Storage &S, size_t nnz) {
+	if (S.val32) return ALP_SUCCESS;
+	if (!S.val) return ALP_MISMATCH;
+	ALP_HIP_TRY(hipMalloc(&S.val32, sizeof(float) * (nnz ? nnz : 1)));
+	if (nnz)
+		hipLaunchKernelGGL((k_convert<double, float>), dim3(stream_grid(nnz)), dim3(kBlock), 0, stream(), nnz,
+		                   (const double *)S.val, S.val32);
+	ALP_LAUNCH_CHECK();
+	if (S.sell_off && S.sell_entries) {
+		if (hipMalloc(&S.sell_val32, sizeof(float) * S.sell_entries) != hipSuccess) {
+			(void)hipGetLastError();
+			S.sell_val32 = nullptr;  // the FP32 products then use the CRS kernels
+		}
+	}
+	if (S.dia.flag && S.sell_entries) {
+		if (hipMalloc(&S.dia.val32, sizeof(float) * S.sell_entries) != hipSuccess) {
+			(void)hipGetLastError();
+			S.dia.val32 = nullptr;
+		}
+	}
+	return storage_refresh_values(S, ALP_VAL_F64);
+}
+
 alp_rc_t storage_finalize(Storage &S, int value_type, bool want_f32) {
 	uint64_t total = 0;
 	ALP_HIP_TRY(copy_sync(&total, S.off + S.nmajor, sizeof(uint64_t), hipMemcpyDeviceToHost));

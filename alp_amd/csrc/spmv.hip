@@ -23,7 +23,8 @@ template <int G, int RR, typename R, int MASK, bool OVERWRITE>
 __global__ __launch_bounds__(kBlock) void k_spmv_vector(
     size_t nrows, const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx,
     const void *__restrict__ vval, const typename R::X *__restrict__ x,
-    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
+    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask, const int *__restrict__ skip) {
+	if (skip && *skip) return;  // a solver loop that has stopped
 	using A = typename R::A;
 	using Y = typename R::Y;
 	constexpr unsigned kGroups = kBlock / G;
@@ -72,7 +73,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
     const uint64_t *__restrict__ off,
     const uint32_t *__restrict__ idx, const void *__restrict__ vval,
     const typename R::X *__restrict__ x, typename R::Y *__restrict__ y,
-    const uint8_t *__restrict__ mask, int skip_long) {
+    const uint8_t *__restrict__ mask, int skip_long, const int *__restrict__ skip) {
+	if (skip && *skip) return;  // a solver loop that has stopped
 	using A = typename R::A;
 	using Y = typename R::Y;
 	using VT = typename Val<R>::T;
@@ -187,7 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     size_t nrows, size_t nslices, const uint64_t *__restrict__ soff, const uint32_t *__restrict__ sidx,
     const void *__restrict__ sval, const uint64_t *__restrict__ off, const typename R::X *__restrict__ x,
     typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask, const uint8_t *__restrict__ dflag,
-    const int32_t *__restrict__ dcol, const uint32_t *__restrict__ dmask, const void *__restrict__ dval) {
+    const int32_t *__restrict__ dcol, const uint32_t *__restrict__ dmask, const void *__restrict__ dval, const int *__restrict__ skip) {
+	if (skip && *skip) return;  // a solver loop that has stopped
 	using A = typename R::A;
 	using Y = typename R::Y;
 	using X = typename R::X;
@@ -206,7 +209,63 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
 	if (MASK && active) active = (MASK == 1) == (mask[r] != 0);
 	A acc = R::zero();
 	unsigned len = 0;
-	if (DIA && dflag[sl]) {
+	// A ring whose product converts the loaded values (FP32 data, FP64 sums) takes
+	// the branch-free form: with a load inside a per-lane branch, the compiler
+	// places the conversion next to it and waits for each load in turn.
+	constexpr bool kConvert = !std::is_same<VT, A>::value && !R::kPattern;
+	// Loads are issued without per-lane branches (all U of a group in flight before
+	// the first use, also when the product converts them): the slot index is
+	// clamped to the slice width, whose entries all exist; an absent entry reads
+	// x[0] (the slices exist only when nnz > 0, so ncols >= 1) and is dropped by
+	// the select in the fold.
+	if (kConvert && DIA && dflag[sl]) {
+		const uint32_t bits = active ? dmask[sl * kSellRows + lane] : 0u;
+		len = __builtin_popcount(bits);
+		// lane q holds slot offset d_q (w <= 32); readlane broadcasts it
+		const int dl = lane < w ? dcol[e0 / kSellRows + lane] : 0;
+		const auto *dv = static_cast<const typename R::V *>(dval);
+		if (__builtin_amdgcn_ballot_w64(bits != 0) != 0) {
+			for (unsigned q0 = 0; q0 < w; q0 += U) {
+				VT v[U];
+				X xv[U];
+#pragma unroll
+				for (int u = 0; u < U; ++u) {
+					const unsigned q = q0 + u < w ? q0 + u : w - 1;
+					const long long d = __builtin_amdgcn_readlane(dl, (int)q);
+					const bool in = (bits >> q) & 1u;
+					v[u] = Val<R>::load(dv, e0 + (size_t)q * kSellRows + lane);
+					xv[u] = x[in ? (long long)r + d : 0ll];
+				}
+#pragma unroll
+				for (int u = 0; u < U; ++u) {
+					const unsigned q = q0 + u;
+					const A t = R::add(Val<R>::mul(v[u], xv[u]), acc);
+					acc = (q < w && ((bits >> q) & 1u)) ? t : acc;
+				}
+			}
+		}
+	} else if (kConvert) {
+		if (active) len = (unsigned)(off[r + 1] - off[r]);
+		const uint32_t *ip = sidx + e0 + lane;
+		for (unsigned q0 = 0; q0 < w; q0 += U) {
+			uint32_t j[U];
+			VT v[U];
+#pragma unroll
+			for (int u = 0; u < U; ++u) {
+				const unsigned q = q0 + u < w ? q0 + u : w - 1;
+				j[u] = __builtin_nontemporal_load(ip + (size_t)q * kSellRows);  // padding holds column 0
+				v[u] = Val<R>::load(val, e0 + (size_t)q * kSellRows + lane);
+			}
+			X xv[U];
+#pragma unroll
+			for (int u = 0; u < U; ++u) xv[u] = x[q0 + u < len ? j[u] : 0u];
+#pragma unroll
+			for (int u = 0; u < U; ++u) {
+				const A t = R::add(Val<R>::mul(v[u], xv[u]), acc);
+				acc = q0 + u < len ? t : acc;
+			}
+		}
+	} else if (DIA && dflag[sl]) {
 		const uint32_t bits = active ? dmask[sl * kSellRows + lane] : 0u;
 		len = __builtin_popcount(bits);
 		// lane q holds slot offset d_q (w <= 32); readlane broadcasts it
@@ -291,7 +350,8 @@ template <typename R, int MASK, bool OVERWRITE, int J>
 __global__ __launch_bounds__(kBlock) void k_spmv_stream_pipe(
     size_t nrb, const uint32_t *__restrict__ rb, const uint64_t *__restrict__ rbase,
     const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx, const void *__restrict__ vval,
-    const typename R::X *__restrict__ x, typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
+    const typename R::X *__restrict__ x, typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask, const int *__restrict__ skip) {
+	if (skip && *skip) return;  // a solver loop that has stopped
 	using A = typename R::A;
 	using Y = typename R::Y;
 	constexpr int kTile = J * kBlock;
@@ -406,7 +466,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_wave(
     size_t ntiles, const uint32_t *__restrict__ rb, const uint64_t *__restrict__ rbase,
     const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx,
     const void *__restrict__ vval, const typename R::X *__restrict__ x,
-    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
+    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask, const int *__restrict__ skip) {
+	if (skip && *skip) return;  // a solver loop that has stopped
 	using A = typename R::A;
 	using Y = typename R::Y;
 	using VT = typename Val<R>::T;
@@ -479,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_wave(
 
 template <typename R, int MASK, bool OW, int RR>
 static void launch_vector_rr(int G, const Storage &S, const void *val, const void *x, void *y,
-                             const uint8_t *mask, hipStream_t s) {
+                             const uint8_t *mask, hipStream_t s, const int *skip) {
 	const size_t rows_per = (kBlock / G) * RR;
 	const size_t nchunks = (S.nmajor + rows_per - 1) / rows_per;
 	size_t grid = nchunks < fill_blocks(8) ? nchunks : fill_blocks(8);
@@ -490,7 +551,7 @@ static void launch_vector_rr(int G, const Storage &S, const void *val, const voi
 	case GG:                                                                                         \
 		hipLaunchKernelGGL((k_spmv_vector<GG, RR, R, MASK, OW>), dim3(grid), dim3(kBlock), 0, s,   \
 		                   S.nmajor, (const uint64_t *)S.off, (const uint32_t *)S.idx, val,          \
-		                   (const X *)x, (Y *)y, mask);                                              \
+		                   (const X *)x, (Y *)y, mask, skip);                                        \
 		break;
 	switch (G) {
 		ALP_VEC_CASE(2)
@@ -506,14 +567,14 @@ static void launch_vector_rr(int G, const Storage &S, const void *val, const voi
 
 template <typename R, int MASK, bool OW>
 static void launch_vector(int G, int RR, const Storage &S, const void *val, const void *x, void *y,
-                          const uint8_t *mask, hipStream_t s) {
+                          const uint8_t *mask, hipStream_t s, const int *skip) {
 	// masked products keep one row per group (the mask already thins the rows)
 	if (MASK != 0 || RR <= 1)
-		launch_vector_rr<R, MASK, OW, 1>(G, S, val, x, y, mask, s);
+		launch_vector_rr<R, MASK, OW, 1>(G, S, val, x, y, mask, s, skip);
 	else if (RR == 2)
-		launch_vector_rr<R, 0, OW, 2>(G, S, val, x, y, mask, s);
+		launch_vector_rr<R, 0, OW, 2>(G, S, val, x, y, mask, s, skip);
 	else
-		launch_vector_rr<R, 0, OW, 4>(G, S, val, x, y, mask, s);
+		launch_vector_rr<R, 0, OW, 4>(G, S, val, x, y, mask, s, skip);
 }
 
 // Long rows (more than kLongRow nonzeros) on the CSR-stream path: block c folds
@@ -527,7 +588,8 @@ __global__ __launch_bounds__(kBlock) void k_chunk_part(size_t nchunks, const uin
                                                        const uint64_t *__restrict__ off,
                                                        const uint32_t *__restrict__ idx, const void *__restrict__ vval,
                                                        const typename R::X *__restrict__ x,
-                                                       typename R::A *__restrict__ part) {
+                                                       typename R::A *__restrict__ part, const int *__restrict__ skip) {
+	if (skip && *skip) return;  // a solver loop that has stopped
 	using A = typename R::A;
 	__shared__ A red[kBlock / kWave];
 	const auto *val = static_cast<const typename R::V *>(vval);
@@ -554,7 +616,8 @@ template <typename R, int MASK, bool OVERWRITE>
 __global__ __launch_bounds__(kBlock) void k_long_fold(size_t nlong, const uint32_t *__restrict__ lr_rows,
                                                       const uint64_t *__restrict__ lr_c0,
                                                       const typename R::A *__restrict__ part,
-                                                      typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask) {
+                                                      typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask, const int *__restrict__ skip) {
+	if (skip && *skip) return;  // a solver loop that has stopped
 	using A = typename R::A;
 	using Y = typename R::Y;
 	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < nlong; i += (size_t)gridDim.x * kBlock) {
@@ -568,17 +631,17 @@ __global__ __launch_bounds__(kBlock) void k_long_fold(size_t nlong, const uint32
 
 template <typename R, int MASK, bool OW>
 static void launch_long_rows(const Storage &S, const void *val, const void *x, void *y, const uint8_t *mask,
-                             hipStream_t s) {
+                             hipStream_t s, const int *skip) {
 	using A = typename R::A;
 	static_assert(sizeof(A) <= 8, "chunk partials hold 8 bytes");
 	size_t g = S.n_chunks < (size_t)fill_blocks(8) ? S.n_chunks : fill_blocks(8);
 	hipLaunchKernelGGL((k_chunk_part<R>), dim3(g), dim3(kBlock), 0, s, S.n_chunks, (const uint32_t *)S.ch_row,
 	                   (const uint64_t *)S.ch_beg, (const uint32_t *)S.lr_rows, (const uint64_t *)S.off,
-	                   (const uint32_t *)S.idx, val, (const typename R::X *)x, (A *)S.ch_part);
+	                   (const uint32_t *)S.idx, val, (const typename R::X *)x, (A *)S.ch_part, skip);
 	const size_t g2 = (S.n_long + kBlock - 1) / kBlock;
 	hipLaunchKernelGGL((k_long_fold<R, MASK, OW>), dim3(g2), dim3(kBlock), 0, s, S.n_long,
 	                   (const uint32_t *)S.lr_rows, (const uint64_t *)S.lr_c0, (const A *)S.ch_part,
-	                   (typename R::Y *)y, mask);
+	                   (typename R::Y *)y, mask, skip);
 }
 
 // variant (`rows`): 0/1 = persistent grid, 4 blocks per CU; 2 = persistent, 8 per
@@ -587,7 +650,7 @@ static void launch_long_rows(const Storage &S, const void *val, const void *x, v
 // tile (`lanes`): 0/8 = 2048-nonzero tiles, 16 = 4096-nonzero tiles.
 template <typename R, int MASK, bool OW>
 static void launch_stream(int variant, int tile, const Storage &S, const void *val, const void *x,
-                          void *y, const uint8_t *mask, hipStream_t s) {
+                          void *y, const uint8_t *mask, hipStream_t s, const int *skip) {
 	const bool big = tile == 16;
 	const size_t nrb = big ? S.n_row_blocks_l : S.n_row_blocks;
 	const uint32_t *rb = big ? S.row_blocks_l : S.row_blocks;
@@ -599,7 +662,7 @@ static void launch_stream(int variant, int tile, const Storage &S, const void *v
 #define ALP_PIPE(JJ)                                                                                \
 	hipLaunchKernelGGL((k_spmv_stream_pipe<R, MASK, OW, JJ>), dim3(grid), dim3(kBlock), 0, s, nrb, rb, \
 	                   rbase, (const uint64_t *)S.off, (const uint32_t *)S.idx, val,                 \
-	                   (const typename R::X *)x, (typename R::Y *)y, mask)
+	                   (const typename R::X *)x, (typename R::Y *)y, mask, skip)
 		if (big) ALP_PIPE(16); else ALP_PIPE(8);
 #undef ALP_PIPE
 		return;
@@ -611,14 +674,14 @@ static void launch_stream(int variant, int tile, const Storage &S, const void *v
 #define ALP_STREAM(JJ, XX)                                                                      \
 	hipLaunchKernelGGL((k_spmv_stream<R, MASK, OW, JJ, XX>), dim3(grid), dim3(kBlock), 0, s, nrb, \
 	                   rb, rbase, (const uint64_t *)S.off, (const uint32_t *)S.idx, val,          \
-	                   (const typename R::X *)x, (typename R::Y *)y, mask, skip_long)
+	                   (const typename R::X *)x, (typename R::Y *)y, mask, skip_long, skip)
 	if (big) {
 		if (variant == 8) ALP_STREAM(16, false); else ALP_STREAM(16, true);
 	} else {
 		if (variant == 8) ALP_STREAM(8, false); else ALP_STREAM(8, true);
 	}
 #undef ALP_STREAM
-	if (skip_long) launch_long_rows<R, MASK, OW>(S, val, x, y, mask, s);
+	if (skip_long) launch_long_rows<R, MASK, OW>(S, val, x, y, mask, s, skip);
 }
 
 // sliced ELL: U (`rows`) entries per lane in flight: 4, 8 or 16; `lanes` = 1 maps
@@ -626,7 +689,7 @@ static void launch_stream(int variant, int tile, const Storage &S, const void *v
 // diagonal form take the index-free path with values dval (null for patterns).
 template <typename R, int MASK, bool OW>
 static void launch_sell(int U, int xcd, const Storage &S, const void *sval, bool dia, const void *dval,
-                        const void *x, void *y, const uint8_t *mask, hipStream_t s) {
+                        const void *x, void *y, const uint8_t *mask, hipStream_t s, const int *skip) {
 	const size_t per = kBlock / kSellRows;
 	const size_t grid = (S.n_slices + per - 1) / per;
 	if (grid == 0) return;
@@ -635,7 +698,7 @@ static void launch_sell(int U, int xcd, const Storage &S, const void *sval, bool
 	hipLaunchKernelGGL((k_spmv_sell<R, MASK, OW, UU, XX, DD>), dim3(grid), dim3(kBlock), 0, s, S.nmajor, \
 	                   S.n_slices, (const uint64_t *)S.sell_off, (const uint32_t *)S.sell_idx, sval,      \
 	                   (const uint64_t *)S.off, (const typename R::X *)x, (typename R::Y *)y, mask,       \
-	                   (const uint8_t *)D.flag, (const int32_t *)D.col, (const uint32_t *)D.mask, dval)
+	                   (const uint8_t *)D.flag, (const int32_t *)D.col, (const uint32_t *)D.mask, dval, skip)
 	if (dia) {
 		if (xcd) {
 			if (U == 16) ALP_SELL(16, true, true); else if (U == 4) ALP_SELL(4, true, true); else ALP_SELL(8, true, true);
@@ -688,14 +751,14 @@ static bool dia_values(const Storage &S, const void *val, const void **dval) {
 
 template <typename R, int MASK, bool OW>
 static void launch_wave(const Storage &S, const void *val, const void *x, void *y,
-                        const uint8_t *mask, hipStream_t s) {
+                        const uint8_t *mask, hipStream_t s, const int *skip) {
 	const size_t per = kBlock / kWave;
 	const size_t grid = (S.n_row_blocks_w + per - 1) / per;
 	if (grid == 0) return;
 	hipLaunchKernelGGL((k_spmv_wave<R, MASK, OW>), dim3(grid), dim3(kBlock), 0, s, S.n_row_blocks_w,
 	                   (const uint32_t *)S.row_blocks_w, (const uint64_t *)S.row_block_base_w,
 	                   (const uint64_t *)S.off, (const uint32_t *)S.idx, val,
-	                   (const typename R::X *)x, (typename R::Y *)y, mask);
+	                   (const typename R::X *)x, (typename R::Y *)y, mask, skip);
 }
 
 // without autotuning: sliced ELL where it was built (regular rows), CSR-stream
@@ -717,7 +780,7 @@ int default_lanes(const Storage &S) {
 
 template <typename R>
 static void dispatch(const Storage &S, const void *val, unsigned descr, void *y, const void *x,
-                     const uint8_t *mask, hipStream_t s) {
+                     const uint8_t *mask, hipStream_t s, const int *skip = nullptr) {
 	const bool ow = (descr & ALP_DESC_OVERWRITE) != 0;
 	const int mmode = mask ? ((descr & ALP_DESC_INVERT_MASK) ? 2 : 1) : 0;
 	int kernel = S.kernel;
@@ -735,13 +798,13 @@ static void dispatch(const Storage &S, const void *val, unsigned descr, void *y,
 	if (RR == 0) RR = kernel == 2 ? 8 : kernel == 4 ? 8 : 1;
 #define ALP_GO(MM, OWV)                                                 \
 	if (kernel == 4)                                                    \
-		launch_sell<R, MM, OWV>(RR, S.lanes, S, sval, dia, dval, x, y, mask, s); \
+		launch_sell<R, MM, OWV>(RR, S.lanes, S, sval, dia, dval, x, y, mask, s, skip); \
 	else if (kernel == 3)                                               \
-		launch_wave<R, MM, OWV>(S, val, x, y, mask, s);                 \
+		launch_wave<R, MM, OWV>(S, val, x, y, mask, s, skip);           \
 	else if (kernel == 2)                                               \
-		launch_stream<R, MM, OWV>(RR, S.lanes, S, val, x, y, mask, s);  \
+		launch_stream<R, MM, OWV>(RR, S.lanes, S, val, x, y, mask, s, skip);  \
 	else                                                                \
-		launch_vector<R, MM, OWV>(G, RR, S, val, x, y, mask, s);
+		launch_vector<R, MM, OWV>(G, RR, S, val, x, y, mask, s, skip);
 	if (ow) {
 		if (mmode == 0) { ALP_GO(0, true) }
 		else if (mmode == 1) { ALP_GO(1, true) }
@@ -754,37 +817,45 @@ static void dispatch(const Storage &S, const void *val, unsigned descr, void *y,
 #undef ALP_GO
 }
 
+alp_rc_t spmv_f32_acc64(const Storage &S, float *y, const float *x, hipStream_t s, const int *skip) {
+	if (!S.off || !S.val32) return ALP_PANIC;
+	if (S.nmajor == 0) return ALP_SUCCESS;
+	dispatch<PlusTimesF32Acc64>(S, S.val32, ALP_DESC_OVERWRITE, y, x, nullptr, s, skip);
+	ALP_LAUNCH_CHECK();
+	return ALP_SUCCESS;
+}
+
 alp_rc_t spmv(const Storage &S, int value_type, alp_semiring_t ring, unsigned descr, void *y,
-              const void *x, const uint8_t *mask, hipStream_t s) {
+              const void *x, const uint8_t *mask, hipStream_t s, const int *skip) {
 	if (!S.off) return ALP_PANIC;
 	if (S.nmajor == 0) return ALP_SUCCESS;
 	if (!x || !y) return ALP_ILLEGAL;
 	switch (ring) {
 		case ALP_PLUS_TIMES_F64:
 			if (value_type == ALP_VAL_F64)
-				dispatch<PlusTimes<double>>(S, S.val, descr, y, x, mask, s);
+				dispatch<PlusTimes<double>>(S, S.val, descr, y, x, mask, s, skip);
 			else if (value_type == ALP_VAL_PATTERN)
-				dispatch<PlusPattern>(S, nullptr, descr, y, x, mask, s);
+				dispatch<PlusPattern>(S, nullptr, descr, y, x, mask, s, skip);
 			else
 				return ALP_MISMATCH;
 			break;
 		case ALP_PLUS_TIMES_F32:
 			if (value_type == ALP_VAL_F32)
-				dispatch<PlusTimes<float>>(S, S.val, descr, y, x, mask, s);
+				dispatch<PlusTimes<float>>(S, S.val, descr, y, x, mask, s, skip);
 			else if (value_type == ALP_VAL_F64 && S.val32)
-				dispatch<PlusTimes<float>>(S, S.val32, descr, y, x, mask, s);
+				dispatch<PlusTimes<float>>(S, S.val32, descr, y, x, mask, s, skip);
 			else
 				return ALP_MISMATCH;
 			break;
 		case ALP_LOR_LAND_BOOL:
-			dispatch<LorLand>(S, nullptr, descr, y, x, mask, s);
+			dispatch<LorLand>(S, nullptr, descr, y, x, mask, s, skip);
 			break;
 		case ALP_PLUS_PATTERN_F64:
-			dispatch<PlusPattern>(S, nullptr, descr, y, x, mask, s);
+			dispatch<PlusPattern>(S, nullptr, descr, y, x, mask, s, skip);
 			break;
 		case ALP_MIN_PLUS_F64:
 			if (value_type != ALP_VAL_F64) return ALP_MISMATCH;
-			dispatch<MinPlus>(S, S.val, descr, y, x, mask, s);
+			dispatch<MinPlus>(S, S.val, descr, y, x, mask, s, skip);
 			break;
 		default:
 			return ALP_ILLEGAL;

@@ -29,6 +29,17 @@ struct PlusTimes {
 	__device__ static A add(A a, A b) { return a + b; }
 };
 
+// FP32 values and vectors with an FP64 accumulator: the products of two floats
+// are exact in FP64 and the row is summed in FP64, so y has a single rounding to
+// FP32. The mixed-precision solvers use it (FP32 traffic, FP64-quality sums).
+struct PlusTimesF32Acc64 {
+	using V = float; using X = float; using Y = float; using A = double;
+	static constexpr bool kPattern = false;
+	__device__ static A zero() { return 0.0; }
+	__device__ static A mul(V a, X x) { return (double)a * (double)x; }
+	__device__ static A add(A a, A b) { return a + b; }
+};
+
 // lor-land over a pattern matrix: bytes 0/1 (monoid.hpp:139-147: lor has
 // immutable nonzeroes, so a row that is already true needs no work)
 struct LorLand {

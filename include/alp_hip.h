@@ -167,8 +167,14 @@ alp_rc_t alp_hip_convert_f32_to_f64(size_t n, const float *d_x, double *d_y);
  * Solves A x = b, stopping when ||r|| < tol*||b|| or after max_iter iterations;
  * *residual = ||r||^2 (the reference's convention), *iters = iterations done.
  * Returns ALP_FAILED when not converged, as the reference does.
- * precision: 0 = FP64 throughout; 1 = FP32 inner iterations inside FP64 iterative
- * refinement (residual b - A x and the solution update stay FP64). */
+ * precision: 0 = FP64 throughout; 1 = CG iterations on A's FP32 copy with FP32
+ * r, p, Ap (each product row summed in FP64, rounded once) and x updated in FP64,
+ * with reliable updates: whenever the FP32 recurrence residual has fallen by 10x
+ * since the last update, r = b - A x is formed in FP64 and CG continues from it
+ * with its search direction kept; convergence is decided on the FP64 residual,
+ * and *iters counts the FP32 iterations;
+ * 2 = restarted FP32 inner CG solves (each to 1e-5 relative) inside FP64
+ * iterative refinement. Modes 1 and 2 use A's FP32 copy (made on first use). */
 alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *d_b, double *d_x, size_t max_iter,
                     double tol, int precision, size_t *iters, double *residual);
 

@@ -399,17 +399,37 @@ def test_cg_to_1e10_fp64_and_mixed(A):
     A.mxv(b, M, A.Vector(n, fill=1.0))
     bh = b.numpy()
     ref = O.stencil(48, 48, 48)
-    for precision in (0, 1):
+    its = {}
+    for precision in (0, 1, 2):
         x = A.Vector(n, fill=0.0)
         rc, it, res = A.cg(M, b, x, 5000, 1e-10, precision)
         assert rc == A.SUCCESS, precision
         xh = x.numpy()
         r = bh - O.spmv(ref, xh)
         assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(bh), precision
+        # the reported residual is that of the returned x (FP64 for the mixed modes)
+        assert abs(res - np.dot(r, r)) <= 1e-3 * np.dot(r, r) + 1e-300, precision
         assert np.max(np.abs(xh - 1.0)) < 1e-7
+        its[precision] = it
+    # reliable updates keep the Krylov space: close to the FP64 iteration count,
+    # unlike restarted refinement
+    assert its[1] <= 1.3 * its[0], its
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+def test_cg_mixed_reports_failure_at_the_iteration_limit(A):
+    M = A.Matrix.stencil27(32, 32, 32, flags=A.MAT_WITH_F32)
+    n = M.nrows
+    b = A.Vector(n, fill=0.0)
+    A.mxv(b, M, A.Vector(n, fill=1.0))
+    for precision in (1, 2):
+        x = A.Vector(n, fill=0.0)
+        rc, it, res = A.cg(M, b, x, 7, 1e-10, precision)
+        assert rc == A.FAILED, precision
+        assert it == 7, (precision, it)
+        assert np.isfinite(res) and res > 0
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_cg_repeated_solves_reuse_cached_state(A, precision):
     """The matrix keeps its CG work vectors and captured iteration graph between
     solves: a repeated solve with the same vectors is bit-identical to the first;
