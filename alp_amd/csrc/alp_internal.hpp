@@ -165,9 +165,11 @@ __device__ __forceinline__ T block_sum_partials(const T *p, unsigned n, T *lds) 
 // streams values only and its x reads are unit-stride across the wavefront. The
 // fold still visits each row's entries in storage order, so results equal the
 // sliced-ELL (and CRS) fold bit for bit. Other slices keep the sliced-ELL arrays.
+constexpr unsigned kDiaSlots = 32;  // most slots of a slice in diagonal form
+
 struct DiaSlices {
 	uint8_t *flag = nullptr;   // per slice: 1 = diagonal form
-	int32_t *col = nullptr;    // per slot, at soff[s] / kSellRows + q: column - row
+	int32_t *col = nullptr;    // per slice kDiaSlots slot offsets, d_q at s * kDiaSlots + q: column - row
 	uint32_t *mask = nullptr;  // per (slice, lane): bit q set when the row has slot q
 	void *val = nullptr;       // values in slot order (sliced-ELL layout, 0 where absent)
 	float *val32 = nullptr;    // FP32 copy of them when the matrix has one

@@ -64,6 +64,11 @@ struct HpLevel {
 	std::vector<size_t> color_off;       // rows of colour c: [color_off[c], color_off[c+1])
 	TileSet ts[2];                       // tiles for FP64 ([0], 2048 nonzeros) and FP32 ([1], 4096)
 	uint32_t *inj = nullptr;             // coarse level: coarse new row -> fine new row
+	// coarse level whose injected fine rows are exactly one fine colour (the parity
+	// colouring's colour 0): fine row f of that colour -> coarse row f2c[f - f2c_base]
+	uint32_t *f2c = nullptr;
+	uint32_t f2c_base = 0;
+	int f2c_colour = -1;
 	double *z = nullptr, *r = nullptr, *t = nullptr;
 	float *z32 = nullptr, *r32 = nullptr, *t32 = nullptr;
 	// sliced ELL of the permuted matrix: slices of at most kSellRows rows that never
@@ -177,6 +182,21 @@ struct EpResid {  // r = b - A x and the partial of r.r (FP64 outer residual)
 	}
 };
 
+template <typename T>
+struct EpRestrict {  // coarse r = (r - A z) at the injected fine row, coarse z = 0
+	const T *rf;
+	const uint32_t *f2c;
+	uint32_t base;
+	T *rc, *zc;
+	T ri;
+	__device__ void prefetch(uint32_t r) { ri = rf[r]; }
+	__device__ void finish(uint32_t r, T acc, bool empty) {
+		const uint32_t c = f2c[r - base];
+		rc[c] = ri - (empty ? T(0) : acc);
+		zc[c] = T(0);
+	}
+};
+
 // ---- tile kernels ---------------------------------------------------------------
 
 struct Tiles {
@@ -263,23 +283,29 @@ __global__ __launch_bounds__(kBlock) void k_sell_rows(size_t s0, size_t s1, cons
                                                       const int32_t *__restrict__ dcol,
                                                       const uint32_t *__restrict__ dmask,
                                                       const T *__restrict__ dval) {
-	if (done && *done) return;
 	const unsigned bl = XCD ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
 	// uniform across the wavefront: slice bounds and slot offsets stay scalar
 	const size_t sl = s0 + (size_t)__builtin_amdgcn_readfirstlane(bl * (kBlock / kSellRows) + threadIdx.x / kSellRows);
 	if (sl >= s1) return;  // whole wavefronts leave together
 	const unsigned lane = threadIdx.x % kSellRows;
-	const uint32_t row = srow[sl] + lane;
-	const bool act = row < srow[sl + 1];
-	const uint64_t e0 = soff[sl];
-	const unsigned w = (unsigned)((soff[sl + 1] - e0) / kSellRows);
+	// everything that depends only on the slice index is loaded before the first
+	// wait (small levels are latency-bound: each dependent round trip counts)
+	const int stop = done ? *done : 0;
+	const uint32_t r0 = srow[sl], r1 = srow[sl + 1];
+	const uint64_t e0 = soff[sl], e1 = soff[sl + 1];
+	const bool dia = DIA && dflag[sl];
+	const uint32_t mbits = DIA ? dmask[sl * kSellRows + lane] : 0u;
+	const int dl = DIA && lane < kDiaSlots ? dcol[sl * kDiaSlots + lane] : 0;  // lane q holds d_q
+	if (stop) return;
+	const uint32_t row = r0 + lane;
+	const bool act = row < r1;
+	const unsigned w = (unsigned)((e1 - e0) / kSellRows);
 	if (act) ep.prefetch(row);
 	T acc = T(0);
 	unsigned len = 0;
-	if (DIA && dflag[sl]) {
-		const uint32_t bits = act ? dmask[sl * kSellRows + lane] : 0u;
+	if (dia) {
+		const uint32_t bits = act ? mbits : 0u;
 		len = __builtin_popcount(bits);
-		const int dl = lane < w ? dcol[e0 / kSellRows + lane] : 0;  // lane q holds d_q (w <= 32)
 		const T *vp = dval + e0 + lane;
 		for (unsigned q0 = 0; q0 < w; q0 += U) {
 			T v[U], xv[U];
@@ -566,6 +592,19 @@ __global__ __launch_bounds__(kBlock) void k_scatter_perm(size_t n, const uint32_
 
 // ---- hierarchy build ------------------------------------------------------------------
 
+__global__ __launch_bounds__(kBlock) void k_inj_check(size_t nc, const uint32_t *inj, uint32_t lo, uint32_t hi,
+                                                     int *bad) {
+	grid_for_each<4>(nc, [&](size_t i) {
+		const uint32_t f = inj[i];
+		if (f < lo || f >= hi) atomicOr(bad, 1);
+	});
+}
+
+__global__ __launch_bounds__(kBlock) void k_inj_inverse(size_t nc, const uint32_t *inj, uint32_t lo,
+                                                       uint32_t *f2c) {
+	grid_for_each<4>(nc, [&](size_t i) { f2c[inj[i] - lo] = (uint32_t)i; });
+}
+
 __global__ __launch_bounds__(kBlock) void k_colour_keys(size_t nx, size_t ny, size_t nz, int colours,
                                                        uint32_t *key, uint32_t *iota) {
 	const size_t n = nx * ny * nz;
@@ -684,7 +723,7 @@ static void free_level(HpLevel &L) {
 	void *ps[] = {L.off, L.idx, L.val, L.val32, L.diag, L.diag32, L.perm, L.iperm, L.colour,
 	              L.ts[0].rb, L.ts[0].re, L.ts[0].base, L.ts[0].top, L.ts[1].rb, L.ts[1].re,
 	              L.ts[1].base, L.ts[1].top, L.inj, L.z, L.r, L.t,
-	              L.z32, L.r32, L.t32, L.srow, L.soff, L.sidx, L.sval, L.sval32};
+	              L.z32, L.r32, L.t32, L.srow, L.soff, L.sidx, L.sval, L.sval32, L.f2c};
 	for (void *p : ps)
 		if (p) hipFree(p);
 	dia_free(L.dia);
@@ -999,11 +1038,20 @@ static void vcycle(alp_hip_hpcg_s *H, size_t l, hipStream_t s, const int *done) 
 	}
 	for (size_t i = 0; i < H->presmooth; ++i) rbgs<T>(F, s, done, i == 0);
 	HpLevel &C = H->L[l + 1];
-	const size_t threads = C.n * 32;
-	hipLaunchKernelGGL(k_restrict<T>, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C.n,
-	                   (const uint32_t *)C.inj, (const uint64_t *)F.off, (const uint32_t *)F.idx,
-	                   Vec<T>::val(F), (const T *)Vec<T>::z(F), (const T *)Vec<T>::r(F), Vec<T>::r(C),
-	                   Vec<T>::z(C), done);
+	bool swept = false;
+	if (C.f2c) {
+		const int c = C.f2c_colour;
+		EpRestrict<T> ep{Vec<T>::r(F), C.f2c, C.f2c_base, Vec<T>::r(C), Vec<T>::z(C), T(0)};
+		swept = sell_rows<T>(F, F.colour_slice_off[c], F.colour_slice_off[c + 1], (const T *)Vec<T>::z(F), ep, s,
+		                     done);
+	}
+	if (!swept) {
+		const size_t threads = C.n * 32;
+		hipLaunchKernelGGL(k_restrict<T>, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C.n,
+		                   (const uint32_t *)C.inj, (const uint64_t *)F.off, (const uint32_t *)F.idx,
+		                   Vec<T>::val(F), (const T *)Vec<T>::z(F), (const T *)Vec<T>::r(F), Vec<T>::r(C),
+		                   Vec<T>::z(C), done);
+	}
 	vcycle<T>(H, l + 1, s, done);
 	hipLaunchKernelGGL(k_prolong<T>, dim3(grid_for(C.n)), dim3(kBlock), 0, s, C.n, (const uint32_t *)C.inj,
 	                   (const T *)Vec<T>::z(C), Vec<T>::z(F), done);
@@ -1144,6 +1192,40 @@ static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st
 	return rc;
 }
 
+// When the injected fine rows of C are exactly one colour of F (a bijection
+// onto that colour's row range), the restriction runs as a sweep over that
+// colour's slices (EpRestrict) instead of one gather per coarse row.
+static alp_rc_t build_restrict_map(HpLevel &C, HpLevel &F) {
+	if (C.n == 0 || !F.soff || F.colour_slice_off.empty()) return ALP_SUCCESS;
+	uint32_t f0 = 0;
+	ALP_HIP_TRY(copy_sync(&f0, C.inj, sizeof(uint32_t), hipMemcpyDeviceToHost));
+	int c = -1;
+	for (int k = 0; k < F.ncolors; ++k)
+		if (f0 >= F.color_off[k] && f0 < F.color_off[k + 1]) c = k;
+	if (c < 0 || F.color_off[c + 1] - F.color_off[c] != C.n) return ALP_SUCCESS;
+	const uint32_t lo = (uint32_t)F.color_off[c], hi = (uint32_t)F.color_off[c + 1];
+	int *bad = nullptr;
+	ALP_TRY(dmalloc(&bad, 1));
+	int hbad = 1;
+	hipError_t e = hipMemsetAsync(bad, 0, sizeof(int), stream());
+	if (e == hipSuccess) {
+		hipLaunchKernelGGL(k_inj_check, dim3(grid_for(C.n)), dim3(kBlock), 0, stream(), C.n, (const uint32_t *)C.inj,
+		                   lo, hi, bad);
+		e = hipGetLastError();
+	}
+	if (e == hipSuccess) e = copy_sync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost);
+	hipFree(bad);
+	ALP_HIP_TRY(e);
+	if (hbad) return ALP_SUCCESS;  // inj values are distinct: in range and as many as the colour = a bijection
+	ALP_TRY(dmalloc(&C.f2c, C.n));
+	hipLaunchKernelGGL(k_inj_inverse, dim3(grid_for(C.n)), dim3(kBlock), 0, stream(), C.n, (const uint32_t *)C.inj, lo,
+	                   C.f2c);
+	ALP_LAUNCH_CHECK();
+	C.f2c_base = lo;
+	C.f2c_colour = c;
+	return ALP_SUCCESS;
+}
+
 }  // namespace alp
 
 // ---------------------------------------------------------------------------
@@ -1200,6 +1282,7 @@ ALP_EXPORT alp_rc_t alp_hip_hpcg_create(alp_hip_hpcg_t *out, size_t nx, size_t n
 		if ((rc = dmalloc(&C.inj, C.n))) break;
 		hipLaunchKernelGGL(k_injection, dim3(grid_for(C.n)), dim3(kBlock), 0, stream(), C.n, C.nx, C.ny, F.nx,
 		                   F.ny, (const uint32_t *)C.perm, (const uint32_t *)F.iperm, C.inj);
+		if ((rc = build_restrict_map(C, F))) break;
 	}
 	const size_t n = H->L.empty() ? 0 : H->L[0].n;
 	if (rc == ALP_SUCCESS) {

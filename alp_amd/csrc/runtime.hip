@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_dia_detect(size_t nslices, size_t nr
 	const uint64_t b = act ? off[r] : 0;
 	const unsigned len = act ? (unsigned)(off[r + 1] - b) : 0u;
 	const unsigned long long tmpl = __ballot(act && len == w);
-	if (w == 0 || w > 32 || tmpl == 0) {
+	if (w == 0 || w > kDiaSlots || tmpl == 0) {
 		if (lane == 0) flag[sl] = 0;
 		mask[sl * kSellRows + lane] = 0;
 		return;
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_dia_detect(size_t nslices, size_t nr
 		const long long mine = (act && q < len) ? (long long)idx[b + q] - (long long)r : 0;
 		const long long d = __shfl(mine, t);
 		if (d < INT32_MIN || d > INT32_MAX) fits = false;
-		if (lane == 0) col[e0 / kSellRows + q] = (int32_t)d;
+		if (lane == 0) col[sl * kDiaSlots + q] = (int32_t)d;
 		if (j < len && (long long)idx[b + j] - (long long)r == d) {
 			bits |= 1u << q;
 			++j;
@@ -376,7 +376,7 @@ alp_rc_t dia_build(DiaSlices &D, size_t nslices, size_t nrows, const uint32_t *s
 	const size_t vs = value_type == ALP_VAL_F64 ? 8 : value_type == ALP_VAL_F32 ? 4 : 0;
 	unsigned long long *d_count = nullptr;
 	DiaSlices T;
-	if (hipMalloc(&T.flag, nslices) != hipSuccess || hipMalloc(&T.col, sizeof(int32_t) * (entries / kSellRows)) != hipSuccess ||
+	if (hipMalloc(&T.flag, nslices) != hipSuccess || hipMalloc(&T.col, sizeof(int32_t) * kDiaSlots * nslices) != hipSuccess ||
 	    hipMalloc(&T.mask, sizeof(uint32_t) * nslices * kSellRows) != hipSuccess ||
 	    hipMalloc(&d_count, sizeof(unsigned long long)) != hipSuccess) {
 		(void)hipGetLastError();

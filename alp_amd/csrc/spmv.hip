@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
 		const uint32_t bits = active ? dmask[sl * kSellRows + lane] : 0u;
 		len = __builtin_popcount(bits);
 		// lane q holds slot offset d_q (w <= 32); readlane broadcasts it
-		const int dl = lane < w ? dcol[e0 / kSellRows + lane] : 0;
+		const int dl = lane < kDiaSlots ? dcol[sl * kDiaSlots + lane] : 0;
 		const auto *dv = static_cast<const typename R::V *>(dval);
 		if (__builtin_amdgcn_ballot_w64(bits != 0) != 0) {
 			for (unsigned q0 = 0; q0 < w; q0 += U) {
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
 		const uint32_t bits = active ? dmask[sl * kSellRows + lane] : 0u;
 		len = __builtin_popcount(bits);
 		// lane q holds slot offset d_q (w <= 32); readlane broadcasts it
-		const int dl = lane < w ? dcol[e0 / kSellRows + lane] : 0;
+		const int dl = lane < kDiaSlots ? dcol[sl * kDiaSlots + lane] : 0;
 		const auto *dv = static_cast<const typename R::V *>(dval);
 		for (unsigned q0 = 0; q0 < w; q0 += U) {
 			VT v[U];
