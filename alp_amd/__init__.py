@@ -90,6 +90,7 @@ SIGNATURES = {
     "alp_hip_matrix_set_kernel": (_RC, [_P, _I, _I, _I]),
     "alp_hip_matrix_get_kernel": (_RC, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "alp_hip_matrix_autotune": (_RC, [_P]),
+    "alp_hip_matrix_layout": (_RC, [_P, _I, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S)]),
     "alp_hip_mxv": (_RC, [_P, _I, _U, _P, _P, _P]),
     "alp_hip_vxm": (_RC, [_P, _I, _U, _P, _P, _P]),
     "alp_hip_dot_f64": (_RC, [_S, _P, _P, ctypes.POINTER(_D)]),
@@ -319,7 +320,8 @@ class Matrix:
         check(lib().alp_hip_matrix_set_kernel(self.h, kernel, lanes, rows), "matrix_set_kernel")
 
     def get_kernel(self):
-        """(kernel, lanes, rows): 1 = CSR-vector with `lanes` per row and `rows` per group, 2 = CSR-stream."""
+        """(kernel, lanes, rows): 1 = CSR-vector with `lanes` per row and `rows` per group, 2 = CSR-stream,
+        3 = CSR-wave, 4 = sliced ELL, 5 = sliced ELL with index-free diagonal slices."""
         k, l, r = _I(), _I(), _I()
         check(lib().alp_hip_matrix_get_kernel(self.h, ctypes.byref(k), ctypes.byref(l), ctypes.byref(r)),
               "matrix_get_kernel")
@@ -328,6 +330,13 @@ class Matrix:
     def autotune(self):
         check(lib().alp_hip_matrix_autotune(self.h), "matrix_autotune")
         return self.get_kernel()
+
+    def layout(self, which: int = 0):
+        """(slices, padded entries, slices in diagonal form) of the CRS (0) or CCS (1) sliced layout."""
+        a, b, c = _S(), _S(), _S()
+        check(lib().alp_hip_matrix_layout(self.h, which, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+              "matrix_layout")
+        return a.value, b.value, c.value
 
     def free(self) -> None:
         if getattr(self, "h", None):

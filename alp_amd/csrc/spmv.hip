@@ -929,6 +929,16 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, i
 	return ALP_SUCCESS;
 }
 
+ALP_EXPORT alp_rc_t alp_hip_matrix_layout(alp_hip_matrix_t A, int which, size_t *slices, size_t *entries,
+                                         size_t *dia_slices) {
+	if (!A || (which != 0 && which != 1)) return ALP_ILLEGAL;
+	const Storage &S = which == 0 ? A->crs : A->ccs;
+	if (slices) *slices = S.sell_off ? S.n_slices : 0;
+	if (entries) *entries = S.sell_off ? S.sell_entries : 0;
+	if (dia_slices) *dia_slices = S.dia.flag ? S.dia.n_dia : 0;
+	return ALP_SUCCESS;
+}
+
 // Autotuner: times each candidate on this matrix's CRS with its natural semiring
 // (3 warm-up + 10 timed runs each, hipEvents on the library stream) and keeps
 // the fastest. Candidates: CSR-vector with G = 2..64 lanes per row and 1, 2 or 4

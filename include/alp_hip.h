@@ -130,6 +130,12 @@ alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, int lanes, in
 alp_rc_t alp_hip_matrix_get_kernel(alp_hip_matrix_t A, int *kernel, int *lanes, int *rows);
 /* time each candidate SpMV kernel on this matrix and keep the fastest */
 alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A);
+/* the sliced layouts built for A's CRS (which = 0) or CCS (which = 1): *slices
+ * 64-row slices holding *entries padded entries (0, 0: none built), of which
+ * *dia_slices are in index-free diagonal form (no reference counterpart: the
+ * reference stores CRS/CCS only; this reports what the kernels stream) */
+alp_rc_t alp_hip_matrix_layout(alp_hip_matrix_t A, int which, size_t *slices, size_t *entries,
+                               size_t *dia_slices);
 
 /* ---- level-2: mxv / vxm (reference/blas2.hpp:870-2200) ----
  * y (+)= A (x) x  (mxv; ALP_DESC_TRANSPOSE_MATRIX gathers over the CCS instead)
