@@ -31,6 +31,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 namespace alp {
@@ -132,6 +133,23 @@ template <> struct Vec<float> {
 };
 
 // ---- epilogues of the tile fold ------------------------------------------------
+
+// The accumulator of a fold: Ep::Acc when the epilogue names one, else T.
+template <typename Ep, typename T, typename = void>
+struct AccOf {
+	using type = T;
+};
+template <typename Ep, typename T>
+struct AccOf<Ep, T, std::void_t<typename Ep::Acc>> {
+	using type = typename Ep::Acc;
+};
+
+struct EpStoreAcc64 {  // y = A x on FP32 data with FP64 row sums, rounded once
+	using Acc = double;
+	float *y;
+	__device__ void prefetch(uint32_t) {}
+	__device__ void finish(uint32_t r, double acc, bool empty) { y[r] = empty ? 0.0f : (float)acc; }
+};
 
 template <typename T>
 struct EpStore {  // y = A x (overwrite)
@@ -272,25 +290,19 @@ __global__ __launch_bounds__(kBlock) void k_tiles_reduce(Tiles tl, const uint64_
 // DIA: a slice in diagonal form reads no column indices: slot q of the lane's row,
 // when its mask bit is set, multiplies x[row + d_q] by the slot-ordered value; the
 // set bits come in storage order, so the fold is the same.
-template <typename T, int U, bool XCD, bool DIA, typename Ep>
-__global__ __launch_bounds__(kBlock) void k_sell_rows(size_t s0, size_t s1, const uint32_t *__restrict__ srow,
-                                                      const uint64_t *__restrict__ soff,
-                                                      const uint32_t *__restrict__ sidx,
-                                                      const T *__restrict__ sval,
-                                                      const uint64_t *__restrict__ off,
-                                                      const T *__restrict__ x, Ep ep, const int *done,
-                                                      const uint8_t *__restrict__ dflag,
-                                                      const int32_t *__restrict__ dcol,
-                                                      const uint32_t *__restrict__ dmask,
-                                                      const T *__restrict__ dval) {
-	const unsigned bl = XCD ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-	// uniform across the wavefront: slice bounds and slot offsets stay scalar
-	const size_t sl = s0 + (size_t)__builtin_amdgcn_readfirstlane(bl * (kBlock / kSellRows) + threadIdx.x / kSellRows);
-	if (sl >= s1) return;  // whole wavefronts leave together
-	const unsigned lane = threadIdx.x % kSellRows;
-	// everything that depends only on the slice index is loaded before the first
-	// wait (small levels are latency-bound: each dependent round trip counts)
-	const int stop = done ? *done : 0;
+
+// One slice of the sliced-ELL / diagonal-slice fold: one lane per row, the row
+// folded sequentially in storage order, ep.prefetch / ep.finish around it. Every
+// load that depends only on the slice index is issued before the first wait
+// (small levels are latency-bound: each dependent round trip counts); the slice
+// is skipped after those loads when stop is set.
+template <typename T, int U, bool DIA, typename Ep>
+__device__ __forceinline__ void slice_rows(size_t sl, unsigned lane, int stop, const uint32_t *__restrict__ srow,
+                                           const uint64_t *__restrict__ soff, const uint32_t *__restrict__ sidx,
+                                           const T *__restrict__ sval, const uint64_t *__restrict__ off,
+                                           const T *__restrict__ x, Ep &ep, const uint8_t *__restrict__ dflag,
+                                           const int32_t *__restrict__ dcol, const uint32_t *__restrict__ dmask,
+                                           const T *__restrict__ dval) {
 	const uint32_t r0 = srow[sl], r1 = srow[sl + 1];
 	const uint64_t e0 = soff[sl], e1 = soff[sl + 1];
 	const bool dia = DIA && dflag[sl];
@@ -301,9 +313,57 @@ __global__ __launch_bounds__(kBlock) void k_sell_rows(size_t s0, size_t s1, cons
 	const bool act = row < r1;
 	const unsigned w = (unsigned)((e1 - e0) / kSellRows);
 	if (act) ep.prefetch(row);
-	T acc = T(0);
+	using A = typename AccOf<Ep, T>::type;
+	A acc = A(0);
 	unsigned len = 0;
-	if (dia) {
+	if (!std::is_same<A, T>::value) {
+		// converting products: loads without per-lane branches, so that the
+		// conversions do not make the loads wait one by one (see k_spmv_sell);
+		// slot indices clamp to the slice width, absent entries read x[0]
+		if (dia) {
+			const uint32_t bits = act ? mbits : 0u;
+			len = __builtin_popcount(bits);
+			const T *vp = dval + e0 + lane;
+			for (unsigned q0 = 0; q0 < w; q0 += U) {
+				T v[U], xv[U];
+#pragma unroll
+				for (int u = 0; u < U; ++u) {
+					const unsigned q = q0 + u < w ? q0 + u : w - 1;
+					const long long d = __builtin_amdgcn_readlane(dl, (int)q);
+					v[u] = __builtin_nontemporal_load(vp + (size_t)q * kSellRows);
+					xv[u] = x[((bits >> q) & 1u) ? (long long)row + d : 0ll];
+				}
+#pragma unroll
+				for (int u = 0; u < U; ++u) {
+					const unsigned q = q0 + u;
+					const A t = (A)v[u] * (A)xv[u] + acc;
+					acc = (q < w && ((bits >> q) & 1u)) ? t : acc;
+				}
+			}
+		} else {
+			if (act) len = (unsigned)(off[row + 1] - off[row]);
+			const uint32_t *ip = sidx + e0 + lane;
+			const T *vp = sval + e0 + lane;
+			for (unsigned q0 = 0; q0 < w; q0 += U) {
+				uint32_t j[U];
+				T v[U];
+#pragma unroll
+				for (int u = 0; u < U; ++u) {
+					const unsigned q = q0 + u < w ? q0 + u : w - 1;
+					j[u] = __builtin_nontemporal_load(ip + (size_t)q * kSellRows);  // padding holds column 0
+					v[u] = __builtin_nontemporal_load(vp + (size_t)q * kSellRows);
+				}
+				T xv[U];
+#pragma unroll
+				for (int u = 0; u < U; ++u) xv[u] = x[q0 + u < len ? j[u] : 0u];
+#pragma unroll
+				for (int u = 0; u < U; ++u) {
+					const A t = (A)v[u] * (A)xv[u] + acc;
+					acc = q0 + u < len ? t : acc;
+				}
+			}
+		}
+	} else if (dia) {
 		const uint32_t bits = act ? mbits : 0u;
 		len = __builtin_popcount(bits);
 		const T *vp = dval + e0 + lane;
@@ -320,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_rows(size_t s0, size_t s1, cons
 #pragma unroll
 			for (int u = 0; u < U; ++u)
 				if (q0 + u < w && ((bits >> (q0 + u)) & 1u)) {
-					const T prod = v[u] * xv[u];
+					const A prod = v[u] * xv[u];
 					acc = prod + acc;
 				}
 		}
@@ -347,12 +407,32 @@ __global__ __launch_bounds__(kBlock) void k_sell_rows(size_t s0, size_t s1, cons
 #pragma unroll
 			for (int u = 0; u < U; ++u)
 				if (q0 + u < len) {
-					const T prod = v[u] * xv[u];
+					const A prod = v[u] * xv[u];
 					acc = prod + acc;
 				}
 		}
 	}
 	if (act) ep.finish(row, acc, len == 0);
+}
+
+template <typename T, int U, bool XCD, bool DIA, typename Ep>
+__global__ __launch_bounds__(kBlock) void k_sell_rows(size_t s0, size_t s1, const uint32_t *__restrict__ srow,
+                                                      const uint64_t *__restrict__ soff,
+                                                      const uint32_t *__restrict__ sidx,
+                                                      const T *__restrict__ sval,
+                                                      const uint64_t *__restrict__ off,
+                                                      const T *__restrict__ x, Ep ep, const int *done,
+                                                      const uint8_t *__restrict__ dflag,
+                                                      const int32_t *__restrict__ dcol,
+                                                      const uint32_t *__restrict__ dmask,
+                                                      const T *__restrict__ dval) {
+	const unsigned bl = XCD ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	// uniform across the wavefront: slice bounds and slot offsets stay scalar
+	const size_t sl = s0 + (size_t)__builtin_amdgcn_readfirstlane(bl * (kBlock / kSellRows) + threadIdx.x / kSellRows);
+	if (sl >= s1) return;  // whole wavefronts leave together
+	const int stop = done ? *done : 0;
+	slice_rows<T, U, DIA>(sl, threadIdx.x % kSellRows, stop, srow, soff, sidx, sval, off, x, ep, dflag, dcol, dmask,
+	                      dval);
 }
 
 __global__ __launch_bounds__(kBlock) void k_hs_width(size_t ns, const uint32_t *srow, const uint64_t *off,
@@ -492,19 +572,30 @@ __global__ __launch_bounds__(kBlock) void k_hp_pap(size_t n, const T *p, const T
 // alpha = rz / p.Ap; x += alpha p; r += (-alpha) Ap; partials of r.r (region 2)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_hp_update(size_t n, T *x, T *r, const T *p, const T *Ap,
-                                                      double *partials, unsigned nred, const HpScal *in) {
+                                                      double *partials, unsigned nred, const HpScal *in,
+                                                      double *x64) {
 	__shared__ double lds[kBlock / kWave];
 	if (in->done) return;
 	const double rz = block_sum_partials(partials, nred, lds);
 	const double pAp = block_sum_partials(partials + kHpRegion, nred, lds);
-	const T alpha = (T)(rz / pAp);
+	const double alpha64 = rz / pAp;
+	const T alpha = (T)alpha64;
 	double q = 0.0;
-	grid_for_each<4>(n, [&](size_t i) {
-		x[i] = alpha * p[i] + x[i];
-		const T ri = (-alpha) * Ap[i] + r[i];
-		r[i] = ri;
-		q += (double)ri * (double)ri;
-	});
+	if (x64) {  // the solution is FP64 (reliable updates): x += alpha p in FP64
+		grid_for_each<4>(n, [&](size_t i) {
+			x64[i] = alpha64 * (double)p[i] + x64[i];
+			const T ri = (-alpha) * Ap[i] + r[i];
+			r[i] = ri;
+			q += (double)ri * (double)ri;
+		});
+	} else {
+		grid_for_each<4>(n, [&](size_t i) {
+			x[i] = alpha * p[i] + x[i];
+			const T ri = (-alpha) * Ap[i] + r[i];
+			r[i] = ri;
+			q += (double)ri * (double)ri;
+		});
+	}
 	q = block_sum(q, lds);
 	if (threadIdx.x == 0) partials[2 * kHpRegion + blockIdx.x] = q;
 }
@@ -563,6 +654,56 @@ struct FinIr {  // FP64 outer residual of the refinement loop
 		ir->done = !(ir->rnorm / ir->norm0 > tol) ? 1 : 0;
 	}
 };
+
+// ---- reliable updates (mixed precision 1) -------------------------------------------
+//
+// FP32 PCG iterations (FP32 V-cycle, FP32 r, z, p, Ap with FP64 row sums in A p)
+// update an FP64 x. A segment ends when the FP32 recurrence residual has fallen
+// below max(tol, delta ||r_true|| / ||r0||) ||r0||; then r = b - A x is formed in
+// FP64 and replaces the FP32 residual. The next iteration's z = M r and
+// beta = r.z / previous r.z continue the same recurrence, so the search direction
+// is kept (reliable updated residuals, Sleijpen and van der Vorst, 1996). The
+// stopping test, ||r|| / ||r0|| <= tol (hpcg.hpp:148,213-221), is decided on the
+// FP64 residual.
+
+// r32 = (float) r and the partials of r.r
+__global__ __launch_bounds__(kBlock) void k_hp_ru_conv(size_t n, const double *r, float *r32, double *partials) {
+	__shared__ double lds[kBlock / kWave];
+	double q = 0.0;
+	grid_for_each<4>(n, [&](size_t i) {
+		const double ri = r[i];
+		r32[i] = (float)ri;
+		q += ri * ri;
+	});
+	q = block_sum(q, lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = q;
+}
+
+// one block: both slots after a reliable update, the FP64 stopping test and the
+// next segment's threshold (st->tol, relative to ||r0||)
+__global__ __launch_bounds__(kBlock) void k_hp_ru_fin(const double *partials, unsigned np, HpState *st, double tol,
+                                                     double delta, unsigned long long max_iter, int first) {
+	__shared__ double lds[kBlock / kWave];
+	const double rr = block_sum_partials(partials, np, lds);
+	if (threadIdx.x == 0) {
+		const double norm = std::sqrt(rr);
+		if (first) {
+			st->norm0 = norm;
+			st->max_iter = max_iter;
+		}
+		HpScal o;
+		o.rz = first ? 0.0 : st->s[0].rz;
+		o.rr = rr;
+		o.norm = norm;
+		o.iter = first ? 0ull : st->s[0].iter;
+		const bool conv = !(norm / st->norm0 > tol);
+		o.done = (conv || o.iter >= st->max_iter) ? 1 : 0;
+		o.pad = 0;
+		st->s[0] = o;
+		st->s[1] = o;
+		st->tol = std::fmax(tol, delta * norm / st->norm0);
+	}
+}
 
 __global__ __launch_bounds__(kBlock) void k_ir_down(size_t n, const double *r, float *r32, float *x32,
                                                    const IrState *ir) {
@@ -715,6 +856,10 @@ struct alp_hip_hpcg_s {
 	double *p = nullptr, *Ap = nullptr, *b = nullptr, *x = nullptr;  // finest level, FP64
 	float *p32 = nullptr, *Ap32 = nullptr, *x32 = nullptr;           // finest level, FP32
 	int *never = nullptr;  // a device int kept 0: the "done" of kernels outside a CG
+	// the reliable-update iteration graph (its vectors are fixed per hierarchy), one
+	// per preconditioning mode; captured on first use, kept until destroy
+	hipGraph_t ru_graph[2] = {nullptr, nullptr};
+	hipGraphExec_t ru_exec[2] = {nullptr, nullptr};
 };
 
 namespace alp {
@@ -1075,7 +1220,8 @@ static void precondition(alp_hip_hpcg_s *H, bool precond, hipStream_t s, const i
 // the CG part of one hpcg iteration (hpcg.hpp:148-221) once z is known, reading
 // slot `par` of the state and writing slot 1 - par
 template <typename T>
-static void cg_step(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, int par, hipStream_t s) {
+static void cg_step(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, int par, hipStream_t s,
+                    double *x64 = nullptr) {
 	HpLevel &F = H->L[0];
 	const size_t n = F.n;
 	const HpScal *in = &st->s[par];
@@ -1087,13 +1233,18 @@ static void cg_step(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, int par, 
 	hipLaunchKernelGGL(k_hp_p<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, p, (const T *)z,
 	                   (const double *)part, rg, in);
 	const Tiles tl = all_tiles<T>(F);
+	// reliable-update mode (FP64 x, FP32 data): A p with FP64 row sums
+	bool stored = false;
+	if constexpr (std::is_same<T, float>::value) {
+		if (x64) stored = sell_rows<T>(F, 0, F.nslices, (const T *)p, EpStoreAcc64{Ap}, s, done);
+	}
 	EpStore<T> ep{Ap};
-	if (!sell_rows<T>(F, 0, F.nslices, (const T *)p, ep, s, done))
+	if (!stored && !sell_rows<T>(F, 0, F.nslices, (const T *)p, ep, s, done))
 		hipLaunchKernelGGL((k_tiles<T, EpStore<T>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
 		                   (const uint64_t *)F.off, (const uint32_t *)F.idx, Vec<T>::val(F), (const T *)p, ep, done);
 	hipLaunchKernelGGL(k_hp_pap<T>, dim3(rg), dim3(kBlock), 0, s, n, (const T *)p, (const T *)Ap, part, in);
 	hipLaunchKernelGGL(k_hp_update<T>, dim3(rg), dim3(kBlock), 0, s, n, x, r, (const T *)p, (const T *)Ap, part,
-	                   rg, in);
+	                   rg, in, x64);
 	hipLaunchKernelGGL(k_hp_fin, dim3(1), dim3(kBlock), 0, s, (const double *)part, rg, in, &st->s[1 - par],
 	                   (const HpState *)st);
 }
@@ -1116,7 +1267,7 @@ __global__ __launch_bounds__(kBlock) void k_up64(size_t n, const float *z32, dou
 // preconditioner, the whole V-cycle, runs in FP32
 template <typename T, bool FP32PC>
 static void hpcg_iteration(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bool precond, int par,
-                           hipStream_t s) {
+                           hipStream_t s, double *x64 = nullptr) {
 	const int *done = &st->s[par].done;
 	if (FP32PC && precond) {
 		HpLevel &F = H->L[0];
@@ -1127,7 +1278,7 @@ static void hpcg_iteration(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bo
 	} else {
 		precondition<T>(H, precond, s, done);
 	}
-	cg_step<T>(H, x, p, Ap, st, par, s);
+	cg_step<T>(H, x, p, Ap, st, par, s, x64);
 }
 
 // prelude: r = b - A x, norm0 = ||r|| (hpcg.hpp:124-137)
@@ -1150,14 +1301,18 @@ static void hpcg_prelude(alp_hip_hpcg_s *H, const T *b, const T *x, T *Ap, HpSta
 // iterations, always from slot 0.
 template <typename T, bool FP32PC = false>
 static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st, bool precond,
-                               HpState *h_st) {
+                               HpState *h_st, double *x64 = nullptr, hipGraph_t *keep_graph = nullptr,
+                               hipGraphExec_t *keep_exec = nullptr) {
 	hipStream_t s = stream();
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	bool use_graph = false;
-	if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-		hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 0, s);
-		hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 1, s);
+	if (keep_exec && *keep_exec) {
+		exec = *keep_exec;
+		use_graph = true;
+	} else if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+		hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 0, s, x64);
+		hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 1, s, x64);
 		if (hipStreamEndCapture(s, &graph) == hipSuccess &&
 		    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
 			use_graph = true;
@@ -1170,8 +1325,8 @@ static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st
 			if (use_graph) {
 				if (hipGraphLaunch(exec, s) != hipSuccess) return ALP_PANIC;
 			} else {
-				hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 0, s);
-				hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 1, s);
+				hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 0, s, x64);
+				hpcg_iteration<T, FP32PC>(H, x, p, Ap, st, precond, 1, s, x64);
 			}
 		}
 		return ALP_SUCCESS;
@@ -1186,8 +1341,15 @@ static alp_rc_t run_iterations(alp_hip_hpcg_s *H, T *x, T *p, T *Ap, HpState *st
 		}
 		if (h_st->s[0].done) break;
 	}
-	if (exec) hipGraphExecDestroy(exec);
-	if (graph) hipGraphDestroy(graph);
+	if (keep_exec && use_graph) {
+		if (!*keep_exec) {
+			*keep_exec = exec;
+			*keep_graph = graph;
+		}
+	} else {
+		if (exec) hipGraphExecDestroy(exec);
+		if (graph) hipGraphDestroy(graph);
+	}
 	if (hipGetLastError() != hipSuccess) rc = ALP_PANIC;
 	return rc;
 }
@@ -1235,6 +1397,10 @@ static alp_rc_t build_restrict_map(HpLevel &C, HpLevel &F) {
 ALP_EXPORT alp_rc_t alp_hip_hpcg_destroy(alp_hip_hpcg_t H) {
 	if (!H) return ALP_SUCCESS;
 	hipStreamSynchronize(stream());
+	for (int i = 0; i < 2; ++i) {
+		if (H->ru_exec[i]) hipGraphExecDestroy(H->ru_exec[i]);
+		if (H->ru_graph[i]) hipGraphDestroy(H->ru_graph[i]);
+	}
 	for (HpLevel &L : H->L) free_level(L);
 	void *ps[] = {H->st, H->st32, H->ir, H->p, H->Ap, H->b, H->x, H->p32, H->Ap32, H->x32, H->never};
 	for (void *p : ps)
@@ -1339,14 +1505,16 @@ ALP_EXPORT alp_rc_t alp_hip_hpcg_vcycle(alp_hip_hpcg_t H, const double *d_r, dou
 
 // hpcg (hpcg.hpp:97-223): solve A x = b from the given x; stop when
 // ||r|| / ||r0|| <= tol or after max_iter iterations. *residual = ||r|| (not
-// squared, as the reference). precision 1: FP32 bulk in FP64 iterative refinement;
-// then *iters counts inner iterations and *outer the refinement steps.
+// squared, as the reference). precision 1: FP32 iterations with reliable FP64
+// residual updates (*outer = updates); 2: FP64 PCG with an FP32 V-cycle;
+// 3: FP32 bulk in restarted FP64 iterative refinement (*iters counts inner
+// iterations and *outer the refinement steps).
 ALP_EXPORT alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, double *d_x,
                                        size_t max_iter, double tol, int precision, int with_precond,
                                        size_t *iters, double *residual, size_t *outer) {
 	ALP_TRY(ensure_init());
 	if (!H || !d_b || !d_x || !iters || !residual) return ALP_ILLEGAL;
-	if (precision < 0 || precision > 2) return ALP_ILLEGAL;
+	if (precision < 0 || precision > 3) return ALP_ILLEGAL;
 	if (precision != 0 && !H->have_f32) return ALP_ILLEGAL;
 	hipStream_t s = stream();
 	HpLevel &F = H->L[0];
@@ -1365,6 +1533,33 @@ ALP_EXPORT alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, doub
 			ALP_TRY(run_iterations<double>(H, H->x, H->p, H->Ap, H->st, with_precond != 0, &h));
 		else
 			ALP_TRY((run_iterations<double, true>(H, H->x, H->p, H->Ap, H->st, with_precond != 0, &h)));
+		total = h.s[0].iter;
+		res = h.s[0].norm;
+	} else if (precision == 1) {
+		// reliable updates: FP32 PCG segments on an FP64 x, FP64 residual between them
+		const double delta = 0.1;
+		const unsigned g = reduce_grid(n);
+		double *part = ctx().d_partials;
+		for (int first = 1;; first = 0) {
+			if (!first) ++nouter;
+			EpResid<double> ep{H->b, F.r, 0.0, 0.0};
+			if (!sell_rows<double>(F, 0, F.nslices, (const double *)H->x, ep, s, (const int *)H->never)) {
+				const Tiles tl = all_tiles<double>(F);
+				hipLaunchKernelGGL((k_tiles<double, EpResid<double>>), dim3(tl.count), dim3(kBlock), 0, s, tl,
+				                   (const uint64_t *)F.off, (const uint32_t *)F.idx, (const double *)F.val,
+				                   (const double *)H->x, ep, (const int *)H->never);
+			}
+			hipLaunchKernelGGL(k_hp_ru_conv, dim3(g), dim3(kBlock), 0, s, n, (const double *)F.r, F.r32, part);
+			hipLaunchKernelGGL(k_hp_ru_fin, dim3(1), dim3(kBlock), 0, s, (const double *)part, g, H->st32, tol, delta,
+			                   (unsigned long long)max_iter, first);
+			ALP_LAUNCH_CHECK();
+			ALP_HIP_TRY(hipMemcpyAsync(&h, H->st32, sizeof(HpState), hipMemcpyDeviceToHost, s));
+			ALP_HIP_TRY(hipStreamSynchronize(s));
+			if (h.s[0].done) break;
+			const int pm = with_precond != 0 ? 1 : 0;
+			ALP_TRY(run_iterations<float>(H, H->x32, H->p32, H->Ap32, H->st32, pm != 0, &h, H->x, &H->ru_graph[pm],
+			                              &H->ru_exec[pm]));
+		}
 		total = h.s[0].iter;
 		res = h.s[0].norm;
 	} else {

@@ -223,11 +223,15 @@ alp_rc_t alp_hip_hpcg_info(alp_hip_hpcg_t H, size_t *n, size_t *nnz, size_t *lev
 /* z = M r: one V-cycle of the preconditioner (FP64, original row numbering) */
 alp_rc_t alp_hip_hpcg_vcycle(alp_hip_hpcg_t H, const double *d_r, double *d_z);
 /* Solve A x = b from the given x: stop when ||r||/||r0|| <= tol or after max_iter
- * iterations; *residual = ||r||. precision 0 = FP64 throughout; 1 = FP32 SpMV,
- * smoother and transfers in an inner FP32 PCG inside FP64 iterative refinement
- * (*iters = inner iterations, *outer = refinement steps); 2 = FP64 CG (SpMV, dots,
- * residual and solution updates in FP64) whose preconditioner, the whole V-cycle
- * (smoother, restriction SpMV, transfers), runs in FP32. outer may be NULL.
+ * iterations; *residual = ||r||. precision 0 = FP64 throughout; 1 = PCG iterations
+ * in FP32 (V-cycle, r, z, p, A p with FP64 row sums) on an FP64 x, with reliable
+ * updates: whenever the FP32 recurrence residual has fallen by 10x, r = b - A x is
+ * formed in FP64 and replaces it (*outer = updates; the stopping test and
+ * *residual are FP64); 2 = FP64 CG (SpMV, dots, residual and solution updates in
+ * FP64) whose preconditioner, the whole V-cycle (smoother, restriction SpMV,
+ * transfers), runs in FP32; 3 = FP32 SpMV, smoother and transfers in an inner FP32
+ * PCG restarted inside FP64 iterative refinement (*iters = inner iterations,
+ * *outer = refinement steps). outer may be NULL.
  * with_precond = 0 runs hpcg's unpreconditioned branch. */
 alp_rc_t alp_hip_hpcg_solve(alp_hip_hpcg_t H, const double *d_b, double *d_x, size_t max_iter,
                             double tol, int precision, int with_precond, size_t *iters,

@@ -108,8 +108,11 @@ def true_relres(A, M, b, x):
     return np.sqrt(A.dot(r, r)) / np.sqrt(A.dot(b, b))
 
 
-@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("precision", [0, 1, 2, 3])
 def test_hpcg_mixed_and_fp64_reach_1e10(A, precision):
+    """0: FP64; 1: FP32 iterations with reliable FP64 residual updates (the returned
+    residual and the stopping test are FP64); 2: FP64 PCG with an FP32 V-cycle;
+    3: restarted FP32 solves in FP64 refinement."""
     sizes = (64, 64, 64)
     H = A.Hpcg(*sizes, max_levels=3, colours=0)
     M, b = rhs_ones(A, *sizes)
@@ -117,8 +120,25 @@ def test_hpcg_mixed_and_fp64_reach_1e10(A, precision):
     it, res, outer = H.solve(b, x, 1000, 1e-10, precision=precision)
     assert true_relres(A, M, b, x) <= 1.5e-10
     assert np.max(np.abs(x.numpy() - 1.0)) < 1e-8
-    if precision == 1:
+    if precision in (1, 3):
         assert outer >= 2
+    if precision == 1:
+        # the returned ||r|| is the FP64 residual of the returned x
+        assert abs(res - true_relres(A, M, b, x) * np.sqrt(A.dot(b, b))) <= 1e-3 * res
+        x0 = A.Vector(H.n, fill=0.0)
+        it0, _, _ = H.solve(b, x0, 1000, 1e-10, precision=0)
+        assert it <= 1.3 * it0, (it, it0)
+
+
+def test_hpcg_reliable_updates_stop_at_the_iteration_limit(A):
+    sizes = (32, 32, 32)
+    H = A.Hpcg(*sizes, max_levels=2, colours=0)
+    M, b = rhs_ones(A, *sizes)
+    x = A.Vector(H.n, fill=0.0)
+    it, res, outer = H.solve(b, x, 5, 1e-10, precision=1)
+    assert it == 5
+    assert np.isfinite(res) and res > 0
+    assert abs(res - true_relres(A, M, b, x) * np.sqrt(A.dot(b, b))) <= 1e-6 * res
 
 
 def test_hpcg_192_four_levels(A):
@@ -126,7 +146,7 @@ def test_hpcg_192_four_levels(A):
     H = A.Hpcg(*sizes, max_levels=3, colours=0)
     assert H.levels == 4 and H.proper and H.nnz == 189_119_224
     M, b = rhs_ones(A, *sizes)
-    for precision in (0, 1, 2):
+    for precision in (0, 1, 2, 3):
         x = A.Vector(H.n, fill=0.0)
         it, res, outer = H.solve(b, x, 500, 1e-10, precision=precision)
         assert true_relres(A, M, b, x) <= 1.5e-10, precision
