@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_lap_fill(size_t nrows, const SysDesc
 // partial of sum_i p[i][r] * (A p)[i][r] over the tile goes to part[tile*nrhs + r].
 // RES computes out = B - A X instead of A X and partials of out.out.
 
-template <typename T, int W, int RPL, bool DOT, bool RES>
+template <typename T, int W, int RPL, bool DOT, bool RES, typename A = T>
 __global__ __launch_bounds__(kBlock) void k_spmm(size_t ntiles, const uint32_t *tile_row0, const uint32_t *tile_row1,
                                                  const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx,
                                                  const T *__restrict__ val, const T *__restrict__ X,
@@ -110,33 +110,37 @@ __global__ __launch_bounds__(kBlock) void k_spmm(size_t ntiles, const uint32_t *
 		for (uint32_t rb = r0; rb < r1; rb += kRows) {
 			const uint32_t r = rb + grp;
 			if (r >= r1) continue;
-			T acc[RPL];
+			A acc[RPL];  // A = double with FP32 data: exact products, FP64 row sums
 #pragma unroll
-			for (int q = 0; q < RPL; ++q) acc[q] = T(0);
+			for (int q = 0; q < RPL; ++q) acc[q] = A(0);
 			const uint64_t b = off[r], e = off[r + 1];
 			for (uint64_t k = b; k < e; ++k) {
 				const uint32_t j = idx[k];
-				const T a = val[k];
+				const A a = val[k];
 				const T *xr = X + (size_t)j * NR + lane * RPL;
 #pragma unroll
-				for (int q = 0; q < RPL; ++q) acc[q] += a * xr[q];
+				for (int q = 0; q < RPL; ++q) acc[q] += a * (A)xr[q];
 			}
 			T *yr = Y + (size_t)r * NR + lane * RPL;
 			if (RES) {
 				const T *br = Bm + (size_t)r * NR + lane * RPL;
 #pragma unroll
 				for (int q = 0; q < RPL; ++q) {
-					const T v = br[q] - acc[q];
+					const T v = (T)(br[q] - acc[q]);
 					yr[q] = v;
 					acc_dot[q] += (double)v * (double)v;
 				}
 			} else {
+				T yv[RPL];
 #pragma unroll
-				for (int q = 0; q < RPL; ++q) yr[q] = acc[q];
+				for (int q = 0; q < RPL; ++q) {
+					yv[q] = (T)acc[q];
+					yr[q] = yv[q];
+				}
 				if (DOT) {
 					const T *pr = X + (size_t)r * NR + lane * RPL;
 #pragma unroll
-					for (int q = 0; q < RPL; ++q) acc_dot[q] += (double)acc[q] * (double)pr[q];
+					for (int q = 0; q < RPL; ++q) acc_dot[q] += (double)yv[q] * (double)pr[q];
 				}
 			}
 		}
@@ -210,7 +214,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_update(size_t ntiles, const uint32_t *tile_row0,
                                                      const uint32_t *tile_row1, const uint32_t *tile_sys, int nrhs,
                                                      T *X, T *R, const T *P, const T *AP, const BatchInst *in,
-                                                     double *part, const int *sys_active) {
+                                                     double *part, const int *sys_active, double *X64) {
 	__shared__ double red[kBlock];
 	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
 		if (!sys_active[tile_sys[t]]) continue;  // uniform per block: the barriers below stay matched
@@ -225,7 +229,10 @@ __global__ __launch_bounds__(kBlock) void k_b_update(size_t ntiles, const uint32
 		double acc = 0.0;
 		for (size_t k = threadIdx.x; k < cnt; k += kBlock) {
 			const size_t e = base + k;
-			X[e] = alpha * P[e] + X[e];
+			if (X64)  // reliable updates: the solution is FP64
+				X64[e] = I.alpha * (double)P[e] + X64[e];
+			else
+				X[e] = alpha * P[e] + X[e];
 			const T rv = R[e] - alpha * AP[e];
 			R[e] = rv;
 			acc += (double)rv * (double)rv;
@@ -399,6 +406,77 @@ __global__ __launch_bounds__(kBlock) void k_b_bnorm(int nrhs, const double *part
 	outer_active[i] = bn > 0.0 ? 1 : 0;
 }
 
+// ---- reliable updates (precision 1) -----------------------------------------------
+//
+// FP32 CG iterations per instance (FP32 R, P, AP; FP64 row sums in the SpMM; FP64
+// X updated every iteration). An instance's segment ends when its FP32 recurrence
+// residual drops below max(tol ||b||, delta ||r_true||); then R = B - A X in FP64
+// for all instances, R32 = (float) R and P = R32 + (R32.R32 / rz) P, keeping the
+// search direction (reliable updated residuals, Sleijpen and van der Vorst, 1996).
+// Convergence is decided on the FP64 residual.
+
+// R32 = (float) R64 and the per-tile partials of R32.R32
+__global__ __launch_bounds__(kBlock) void k_b_ru_down(size_t ntiles, const uint32_t *tile_row0,
+                                                     const uint32_t *tile_row1, int nrhs, const double *R,
+                                                     float *R32, double *part) {
+	__shared__ double red[kBlock];
+	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		const size_t base = (size_t)tile_row0[t] * nrhs;
+		const size_t cnt = (size_t)(tile_row1[t] - tile_row0[t]) * nrhs;
+		double acc = 0.0;
+		for (size_t k = threadIdx.x; k < cnt; k += kBlock) {
+			const float v = (float)R[base + k];
+			R32[base + k] = v;
+			acc += (double)v * (double)v;
+		}
+		red[threadIdx.x] = acc;
+		__syncthreads();
+		if (threadIdx.x < nrhs) {
+			double s = 0.0;
+			for (int k = threadIdx.x; k < kBlock; k += nrhs) s += red[k];
+			part[t * nrhs + threadIdx.x] = s;
+		}
+		__syncthreads();
+	}
+}
+
+// per instance after a reliable update: rz = R32.R32, beta = rz / previous rz (0 on
+// the first), the segment threshold; active while the FP64 residual is above tol
+// and iterations remain
+__global__ __launch_bounds__(kBlock) void k_b_ru_start(int nrhs, const double *part, const uint32_t *tile0,
+                                                      BatchInst *in, const int *outer_active, const double *rnorm,
+                                                      const double *bnorm, double tol, double delta,
+                                                      unsigned long long max_iter, int first, unsigned *nactive,
+                                                      int *sys_active) {
+	__shared__ double sums[kMaxRhs];
+	__shared__ int any;
+	const int s = blockIdx.x;
+	if (threadIdx.x == 0) any = 0;
+	system_sums(part, tile0, s, nrhs, sums);
+	if (threadIdx.x < nrhs) {
+		const int i = s * nrhs + threadIdx.x;
+		BatchInst &I = in[i];
+		const double rz = sums[threadIdx.x];
+		if (first) {
+			I.iter = 0;
+			I.rz = 0.0;
+			I.converged = 0;
+		}
+		I.beta = (first || !(I.rz > 0.0)) ? 0.0 : rz / I.rz;
+		I.rz = rz;
+		I.rr = rz;
+		I.alpha = 0.0;
+		I.tol = std::fmax(tol * bnorm[i], delta * rnorm[i]);
+		I.active = (outer_active[i] && I.iter < max_iter && rz > 0.0) ? 1 : 0;
+		if (I.active) {
+			atomicAdd(nactive, 1u);
+			atomicOr(&any, 1);
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) sys_active[s] = any;
+}
+
 // a counter reset as a kernel, so that it can sit inside a captured graph
 __global__ void k_b_zero(unsigned *c) { *c = 0u; }
 
@@ -437,13 +515,13 @@ struct alp_hip_batch_s {
 
 namespace alp {
 
-template <typename T, bool DOT, bool RES>
+template <typename T, bool DOT, bool RES, typename A = T>
 static void launch_spmm(alp_hip_batch_s *B, int W, const T *val, const T *X, const T *Bm, T *Y, hipStream_t s,
                         const int *sys_active = nullptr) {
 	const size_t grid = B->ntiles < (size_t)fill_blocks(8) ? B->ntiles : fill_blocks(8);
 	const int nr = (int)B->nrhs;
 #define ALP_SPMM(WW, RPL)                                                                                 \
-	hipLaunchKernelGGL((k_spmm<T, WW, RPL, DOT, RES>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,        \
+	hipLaunchKernelGGL((k_spmm<T, WW, RPL, DOT, RES, A>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,     \
 	                   (const uint32_t *)B->tile_row0, (const uint32_t *)B->tile_row1,                  \
 	                   (const uint64_t *)B->off, (const uint32_t *)B->idx, val, X, Bm, Y, B->part,       \
 	                   sys_active, (const uint32_t *)B->tile_sys)
@@ -487,18 +565,21 @@ static alp_rc_t bmalloc(T **p, size_t n) {
 // Iterate the inner CG on precision T until no instance is active.
 template <typename T>
 static alp_rc_t batch_iterate(alp_hip_batch_s *B, const T *val, T *X, T *R, T *P, T *AP, size_t max_iter,
-                              size_t *its) {
+                              size_t *its, double *X64 = nullptr) {
 	hipStream_t s = stream();
 	const unsigned nsys = (unsigned)B->nsys;
 	const int nr = (int)B->nrhs;
 	const size_t g = B->ntiles < (size_t)fill_blocks(8) ? B->ntiles : fill_blocks(8);
 	auto iteration = [&](hipStream_t st) {
-		launch_spmm<T, true, false>(B, B->W, val, P, nullptr, AP, st, B->sys_active);
+		if (X64)  // reliable updates: FP64 row sums of the FP32 data
+			launch_spmm<T, true, false, double>(B, B->W, val, P, nullptr, AP, st, B->sys_active);
+		else
+			launch_spmm<T, true, false>(B, B->W, val, P, nullptr, AP, st, B->sys_active);
 		hipLaunchKernelGGL(k_b_alpha, dim3(nsys), dim3(kBlock), 0, st, nr, (const double *)B->part,
 		                   (const uint32_t *)B->tile0, B->inst);
 		hipLaunchKernelGGL(k_b_update<T>, dim3(g), dim3(kBlock), 0, st, B->ntiles, (const uint32_t *)B->tile_row0,
 		                   (const uint32_t *)B->tile_row1, (const uint32_t *)B->tile_sys, nr, X, R, (const T *)P,
-		                   (const T *)AP, (const BatchInst *)B->inst, B->part, (const int *)B->sys_active);
+		                   (const T *)AP, (const BatchInst *)B->inst, B->part, (const int *)B->sys_active, X64);
 		hipLaunchKernelGGL(k_b_beta, dim3(nsys), dim3(kBlock), 0, st, nr, (const double *)B->part,
 		                   (const uint32_t *)B->tile0, B->inst, (unsigned long long)max_iter, B->nactive,
 		                   B->sys_active);
@@ -724,7 +805,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
                                         size_t *outer) {
 	ALP_TRY(ensure_init());
 	if (!B || !d_B || !d_X || !(tol > 0) || max_iter == 0) return ALP_ILLEGAL;
-	if (precision != 0 && precision != 1) return ALP_ILLEGAL;
+	if (precision < 0 || precision > 2) return ALP_ILLEGAL;
 	hipStream_t s = stream();
 	const int nr = (int)B->nrhs;
 	const int ninst = (int)(B->nsys * B->nrhs);
@@ -753,6 +834,38 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 		ALP_LAUNCH_CHECK();
 		size_t it = 0;
 		ALP_TRY(batch_iterate<double>(B, B->val, d_X, B->R, B->P, B->AP, max_iter, &it));
+		ALP_HIP_TRY(copy_sync(hin.data(), B->inst, sizeof(BatchInst) * ninst, hipMemcpyDeviceToHost));
+		for (int i = 0; i < ninst; ++i) its[i] = hin[i].iter;
+	} else if (precision == 1) {
+		const double delta = 0.1;
+		ALP_HIP_TRY(hipMemsetAsync(B->P32, 0, sizeof(float) * ntot, s));
+		for (int first = 1;; first = 0) {
+			// FP64 residual R = B - A X for every instance, its norm, and who is done
+			launch_spmm<double, false, true>(B, B->W, B->val, d_X, d_B, B->R, s);
+			ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
+			hipLaunchKernelGGL(k_b_outer, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
+			                   (const uint32_t *)B->tile0, B->rnorm, (const double *)B->bnorm, B->outer_active,
+			                   tol, B->nactive);
+			hipLaunchKernelGGL(k_b_ru_down, dim3(g), dim3(kBlock), 0, s, B->ntiles, (const uint32_t *)B->tile_row0,
+			                   (const uint32_t *)B->tile_row1, nr, (const double *)B->R, B->R32, B->part);
+			ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
+			hipLaunchKernelGGL(k_b_ru_start, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
+			                   (const uint32_t *)B->tile0, B->inst, (const int *)B->outer_active,
+			                   (const double *)B->rnorm, (const double *)B->bnorm, tol, delta,
+			                   (unsigned long long)max_iter, first, B->nactive, B->sys_active);
+			ALP_LAUNCH_CHECK();
+			unsigned act = 0;
+			ALP_HIP_TRY(hipMemcpyAsync(&act, B->nactive, sizeof(act), hipMemcpyDeviceToHost, s));
+			ALP_HIP_TRY(hipStreamSynchronize(s));
+			if (act == 0) break;  // every instance converged or ran out of iterations
+			if (!first) ++nouter;
+			// P = R32 + beta P for the active instances (beta = 0 on the first segment)
+			hipLaunchKernelGGL(k_b_dir<float>, dim3(g), dim3(kBlock), 0, s, B->ntiles, (const uint32_t *)B->tile_row0,
+			                   (const uint32_t *)B->tile_row1, (const uint32_t *)B->tile_sys, nr, B->P32,
+			                   (const float *)B->R32, (const BatchInst *)B->inst, (const int *)B->sys_active);
+			size_t it = 0;
+			ALP_TRY(batch_iterate<float>(B, B->val32, B->X32, B->R32, B->P32, B->AP32, max_iter, &it, d_X));
+		}
 		ALP_HIP_TRY(copy_sync(hin.data(), B->inst, sizeof(BatchInst) * ninst, hipMemcpyDeviceToHost));
 		for (int i = 0; i < ninst; ++i) its[i] = hin[i].iter;
 	} else {

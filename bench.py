@@ -174,7 +174,7 @@ def leg_c4(alp) -> dict:
     w = B.autotune()
     rhs = alp.Vector.from_numpy(rng.standard_normal(B.nrows * 8))
     out = {"systems": 64, "rhs": 8, "rows": B.nrows, "nnz": B.nnz, "spmm_lanes": w}
-    for precision, tag in ((0, "fp64"), (1, "mixed")):
+    for precision, tag in ((0, "fp64"), (1, "mixed"), (2, "mixed_restarted")):
         X = alp.Vector(B.nrows * 8, fill=0.0)
         B.solve(rhs, X, 1e-8, 20000, precision)  # warm-up (graph capture)
         (rc, its, rel, outer), dt = timed(lambda: B.solve(rhs, X, 1e-8, 20000, precision), alp.sync)

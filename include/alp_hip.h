@@ -284,7 +284,9 @@ alp_rc_t alp_hip_batch_spmm(alp_hip_batch_t B, const double *d_X, double *d_Y);
 /* time the SpMM for each right-hand-side tile width (lanes per row) and keep the fastest */
 alp_rc_t alp_hip_batch_autotune(alp_hip_batch_t B, int *width);
 /* all systems from X = 0 to ||b - A x|| <= tol ||b|| per right-hand side; precision
- * 0 = FP64 CG, 1 = FP32 SpMM/CG inside FP64 iterative refinement. h_iters and
+ * 0 = FP64 CG, 1 = FP32 SpMM (FP64 row sums) and CG vectors on an FP64 X with
+ * reliable FP64 residual updates per instance, 2 = restarted FP32 SpMM/CG inside
+ * FP64 iterative refinement. h_iters and
  * h_relres (nsys * nrhs entries each, may be NULL): CG iterations and final true
  * relative residual per instance. ALP_FAILED if some instance missed tol. */
 alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, double *d_X, double tol,

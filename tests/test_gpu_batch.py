@@ -47,7 +47,7 @@ def test_spmm_matches_cpu(A):
     assert B.nnz == sum(laplace7(*(int(v[s]) for v in p[:3]), 1, 1, 1).nnz for s in range(B.nsys))
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_small_batch_against_direct_solves(A, precision):
     import scipy.sparse.linalg as spla
     rng = np.random.default_rng(7 + precision)
@@ -73,11 +73,13 @@ def test_c4_batch_64_systems(A):
     B, p = make_batch(A, rng, 64, 16, 96)
     B.autotune()
     rhs = rng.standard_normal(B.nrows * 8)
-    for precision in (0, 1):
+    maxits = {}
+    for precision in (0, 1, 2):
         X = A.Vector(B.nrows * 8, fill=0.0)
         rc, its, rel, outer = B.solve(A.Vector.from_numpy(rhs), X, 1e-8, 20000, precision)
         assert rc == A.SUCCESS, precision
         assert np.all(rel <= 1e-8) and its.min() > 0
+        maxits[precision] = int(its.max())
         # spot-check the device's own residual report on three systems on the CPU
         x = X.numpy().reshape(B.nrows, 8)
         r = rhs.reshape(B.nrows, 8)
@@ -87,3 +89,16 @@ def test_c4_batch_64_systems(A):
             res = np.linalg.norm(r[r0:r1] - M @ x[r0:r1], axis=0) / np.linalg.norm(r[r0:r1], axis=0)
             assert np.all(res <= 1.01e-8)
             assert np.allclose(res, rel[s], rtol=1e-3, atol=1e-12)
+    # reliable updates keep each instance's Krylov space: near the FP64 count
+    assert maxits[1] <= 1.3 * maxits[0], maxits
+
+
+def test_batch_reliable_updates_stop_at_the_iteration_limit(A):
+    rng = np.random.default_rng(11)
+    B, p = make_batch(A, rng, 5, 12, 20)
+    rhs = rng.standard_normal(B.nrows * 8)
+    X = A.Vector(B.nrows * 8, fill=0.0)
+    rc, its, rel, outer = B.solve(A.Vector.from_numpy(rhs), X, 1e-12, 6, 1)
+    assert rc == A.FAILED
+    assert its.max() == 6
+    assert np.all(np.isfinite(rel)) and np.all(rel > 1e-12)
