@@ -246,6 +246,9 @@ struct alp_hip_matrix_s {
 	// iteration graphs); released with the matrix
 	void *solver_cache = nullptr;
 	void (*free_solver_cache)(void *) = nullptr;
+	// graph-algorithm work buffers (BFS queues, degrees, scan space), kept likewise
+	void *graph_cache = nullptr;
+	void (*free_graph_cache)(void *) = nullptr;
 };
 
 namespace alp {

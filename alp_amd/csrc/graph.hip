@@ -180,11 +180,23 @@ __global__ __launch_bounds__(kBlock) void k_bfs_collect(size_t n, const int32_t 
 
 // ---- PageRank (simple_pagerank.hpp:146-533) -------------------------------------------
 
-struct PrState {
-	double dangling, residual, alpha, conv;
-	unsigned long long iter, max_iter;
-	int done, converged, pad[2];
+// Per-iteration scalars in two slots: iteration j reads slot j % 2 and only its
+// last kernel (one block) writes slot (j + 1) % 2, so the reductions need no
+// grid-wide fence: each block writes a partial, the consuming kernel folds them.
+struct PrScal {
+	double residual;
+	unsigned long long iter;
+	int done, converged;
 };
+
+struct PrState {
+	PrScal s[2];
+	double alpha, conv;
+	unsigned long long max_iter;
+};
+
+// the partials regions of one iteration: dangling mass, then the L1 residual
+constexpr unsigned kPrRegion = kMaxPartials / 2;
 
 // row_sum[i] = alpha / out-degree (0 for dangling rows), pr = 1/n (:206-297)
 __global__ __launch_bounds__(kBlock) void k_pr_init(size_t n, const uint64_t *off, double alpha, double *rs,
@@ -196,12 +208,12 @@ __global__ __launch_bounds__(kBlock) void k_pr_init(size_t n, const uint64_t *of
 	});
 }
 
-// pr_next = pr * row_sum (0 where dangling) and the dangling mass (:329-345, :385)
+// pr_next = pr * row_sum (0 where dangling) and the partials of the dangling
+// mass (:329-345, :385)
 __global__ __launch_bounds__(kBlock) void k_pr_scale(size_t n, const double *pr, const double *rs, double *pn,
-                                                     double *partials, unsigned *ticket, PrState *st) {
+                                                     double *partials, const PrScal *in) {
 	__shared__ double lds[kBlock / kWave];
-	__shared__ bool is_last;
-	if (st->done) return;
+	if (in->done) return;
 	double dang = 0.0;
 	grid_for_each<4>(n, [&](size_t i) {
 		const double r = rs[i];
@@ -214,19 +226,16 @@ __global__ __launch_bounds__(kBlock) void k_pr_scale(size_t n, const double *pr,
 	});
 	dang = block_sum(dang, lds);
 	if (threadIdx.x == 0) partials[blockIdx.x] = dang;
-	if (last_block(ticket, &is_last)) {
-		const double d = block_sum_partials(partials, gridDim.x, lds);
-		if (threadIdx.x == 0) st->dangling = (st->alpha * d + 1.0 - st->alpha) / (double)n;
-	}
 }
 
-// pr = pr_nextnext + dangling and the L1 residual (:441-455, :490)
+// pr = pr_nextnext + (alpha * dangling + 1 - alpha) / n and the partials of the L1
+// residual (:441-455, :490)
 __global__ __launch_bounds__(kBlock) void k_pr_update(size_t n, double *pr, const double *pnn, double *partials,
-                                                      unsigned *ticket, PrState *st) {
+                                                      unsigned np, const PrScal *in, const PrState *st) {
 	__shared__ double lds[kBlock / kWave];
-	__shared__ bool is_last;
-	if (st->done) return;
-	const double dang = st->dangling;
+	if (in->done) return;
+	const double d = block_sum_partials(partials, np, lds);
+	const double dang = (st->alpha * d + 1.0 - st->alpha) / (double)n;
 	double res = 0.0;
 	grid_for_each<4>(n, [&](size_t i) {
 		const double old = pr[i];
@@ -235,18 +244,27 @@ __global__ __launch_bounds__(kBlock) void k_pr_update(size_t n, double *pr, cons
 		res += old > v ? old - v : v - old;
 	});
 	res = block_sum(res, lds);
-	if (threadIdx.x == 0) partials[blockIdx.x] = res;
-	if (last_block(ticket, &is_last)) {
-		const double r = block_sum_partials(partials, gridDim.x, lds);
-		if (threadIdx.x == 0) {
-			st->residual = r;
-			st->iter += 1;
-			const bool conv = st->conv != 0.0 && r <= st->conv;
-			if (conv || st->iter >= st->max_iter) {
-				st->done = 1;
-				st->converged = conv ? 1 : 0;
-			}
-		}
+	if (threadIdx.x == 0) partials[kPrRegion + blockIdx.x] = res;
+}
+
+// one block: the residual, the iteration count and the stopping test into the
+// next slot
+__global__ __launch_bounds__(kBlock) void k_pr_fin(const double *partials, unsigned np, const PrScal *in, PrScal *out,
+                                                  const PrState *st) {
+	__shared__ double lds[kBlock / kWave];
+	if (in->done) {
+		if (threadIdx.x == 0) *out = *in;
+		return;
+	}
+	const double r = block_sum_partials(partials + kPrRegion, np, lds);
+	if (threadIdx.x == 0) {
+		PrScal o;
+		o.residual = r;
+		o.iter = in->iter + 1;
+		const bool conv = st->conv != 0.0 && r <= st->conv;
+		o.converged = conv ? 1 : 0;
+		o.done = (conv || o.iter >= st->max_iter) ? 1 : 0;
+		*out = o;
 	}
 }
 
@@ -430,6 +448,42 @@ ALP_EXPORT alp_rc_t alp_hip_max_degree_vertex(alp_hip_matrix_t A, uint32_t *vert
 	return ALP_SUCCESS;
 }
 
+// BFS work buffers, kept on the matrix between calls (a call allocates nothing
+// once they exist); released with the matrix
+struct BfsWork {
+	DevBuf<uint32_t> q0, q1;
+	DevBuf<uint64_t> deg, scan;
+	DevBuf<BfsCounters> cnt;
+	DevBuf<char> tmp;
+	size_t scan_tb = 0;
+};
+
+static void free_bfs_work(void *p) { delete static_cast<BfsWork *>(p); }
+
+static alp_rc_t bfs_work(alp_hip_matrix_s *A, BfsWork **out) {
+	if (!A->graph_cache) {
+		const size_t n = A->nrows;
+		auto *w = new (std::nothrow) BfsWork();
+		if (!w) return ALP_OUTOFMEM;
+		alp_rc_t rc = ALP_SUCCESS;
+		if ((rc = w->q0.alloc(n)) || (rc = w->q1.alloc(n)) || (rc = w->deg.alloc(n + 1)) ||
+		    (rc = w->scan.alloc(n + 1)) || (rc = w->cnt.alloc(1))) {
+			delete w;
+			return rc;
+		}
+		if (hipcub::DeviceScan::ExclusiveSum(nullptr, w->scan_tb, w->deg.p, w->scan.p, n + 1, stream()) !=
+		        hipSuccess ||
+		    (rc = w->tmp.alloc(w->scan_tb))) {
+			delete w;
+			return rc ? rc : ALP_PANIC;
+		}
+		A->graph_cache = w;
+		A->free_graph_cache = free_bfs_work;
+	}
+	*out = static_cast<BfsWork *>(A->graph_cache);
+	return ALP_SUCCESS;
+}
+
 ALP_EXPORT alp_rc_t alp_hip_bfs(alp_hip_matrix_t A, uint32_t source, int32_t *d_level, size_t *nlevels,
                                 size_t *reached) {
 	ALP_TRY(ensure_init());
@@ -439,18 +493,13 @@ ALP_EXPORT alp_rc_t alp_hip_bfs(alp_hip_matrix_t A, uint32_t source, int32_t *d_
 	if (source >= n) return ALP_ILLEGAL;
 	hipStream_t s = stream();
 	const Storage &S = A->crs;
-	DevBuf<uint32_t> q0, q1;
-	DevBuf<uint64_t> deg, scan;
-	DevBuf<BfsCounters> cnt;
-	ALP_TRY(q0.alloc(n));
-	ALP_TRY(q1.alloc(n));
-	ALP_TRY(deg.alloc(n + 1));
-	ALP_TRY(scan.alloc(n + 1));
-	ALP_TRY(cnt.alloc(1));
-	size_t scan_tb = 0;
-	ALP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tb, deg.p, scan.p, n + 1, s));
-	DevBuf<char> tmp;
-	ALP_TRY(tmp.alloc(scan_tb));
+	BfsWork *w = nullptr;
+	ALP_TRY(bfs_work(A, &w));
+	DevBuf<uint32_t> &q0 = w->q0, &q1 = w->q1;
+	DevBuf<uint64_t> &deg = w->deg, &scan = w->scan;
+	DevBuf<BfsCounters> &cnt = w->cnt;
+	DevBuf<char> &tmp = w->tmp;
+	size_t scan_tb = w->scan_tb;
 	hipLaunchKernelGGL(k_bfs_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, d_level, source);
 	ALP_HIP_TRY(hipStreamSynchronize(s));
 	ALP_HIP_TRY(copy_sync(q0.p, &source, sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -523,29 +572,39 @@ ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double al
 	ALP_TRY(pn.alloc(n));
 	ALP_TRY(pnn.alloc(n));
 	ALP_TRY(st.alloc(1));
-	PrState h{0.0, 0.0, alpha, conv, 0ull, (unsigned long long)max_iter, 0, 0, {0, 0}};
+	PrState h{};
+	h.s[0] = h.s[1] = PrScal{0.0, 0ull, 0, 0};
+	h.alpha = alpha;
+	h.conv = conv;
+	h.max_iter = (unsigned long long)max_iter;
 	ALP_HIP_TRY(copy_sync(st.p, &h, sizeof(h), hipMemcpyHostToDevice));
 	hipLaunchKernelGGL(k_pr_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint64_t *)L->crs.off, alpha,
 	                   rs.p, d_pr);
 	ALP_LAUNCH_CHECK();
-	const unsigned rg = g_rgrid(n);
+	unsigned rg = g_rgrid(n);
+	if (rg > kPrRegion) rg = kPrRegion;
+	double *part = ctx().d_partials;
 	for (;;) {
-		for (int k = 0; k < 8; ++k) {
+		for (int k = 0; k < 8; ++k) {  // even: every check reads slot 0
+			const PrScal *in = &st.p->s[k % 2];
 			hipLaunchKernelGGL(k_pr_scale, dim3(rg), dim3(kBlock), 0, s, n, (const double *)d_pr, (const double *)rs.p,
-			                   pn.p, ctx().d_partials, ctx().d_tickets, st.p);
+			                   pn.p, part, in);
 			// pr_nextnext = pr_next L: the CCS gather of vxm (blas2.hpp:1432-1466)
-			ALP_TRY(spmv(L->ccs, L->value_type, ALP_PLUS_PATTERN_F64, ALP_DESC_OVERWRITE, pnn.p, pn.p, nullptr, s));
-			hipLaunchKernelGGL(k_pr_update, dim3(rg), dim3(kBlock), 0, s, n, d_pr, (const double *)pnn.p,
-			                   ctx().d_partials, ctx().d_tickets, st.p);
+			ALP_TRY(spmv(L->ccs, L->value_type, ALP_PLUS_PATTERN_F64, ALP_DESC_OVERWRITE, pnn.p, pn.p, nullptr, s,
+			             &in->done));
+			hipLaunchKernelGGL(k_pr_update, dim3(rg), dim3(kBlock), 0, s, n, d_pr, (const double *)pnn.p, part, rg, in,
+			                   (const PrState *)st.p);
+			hipLaunchKernelGGL(k_pr_fin, dim3(1), dim3(kBlock), 0, s, (const double *)part, rg, in,
+			                   &st.p->s[1 - k % 2], (const PrState *)st.p);
 		}
 		ALP_LAUNCH_CHECK();
 		ALP_HIP_TRY(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
 		ALP_HIP_TRY(hipStreamSynchronize(s));
-		if (h.done) break;
+		if (h.s[0].done) break;
 	}
-	if (iters) *iters = h.iter;
-	if (quality) *quality = h.residual;
-	return h.converged ? ALP_SUCCESS : ALP_FAILED;
+	if (iters) *iters = h.s[0].iter;
+	if (quality) *quality = h.s[0].residual;
+	return h.s[0].converged ? ALP_SUCCESS : ALP_FAILED;
 }
 
 ALP_EXPORT alp_rc_t alp_hip_khop(alp_hip_matrix_t A, uint32_t source, size_t k, uint8_t *d_out) {

@@ -975,6 +975,7 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_destroy(alp_hip_matrix_t A) {
 	if (!A) return ALP_SUCCESS;
 	hipStreamSynchronize(stream());
 	if (A->free_solver_cache) A->free_solver_cache(A->solver_cache);
+	if (A->free_graph_cache) A->free_graph_cache(A->graph_cache);
 	storage_free(A->crs);
 	storage_free(A->ccs);
 	delete A;
