@@ -237,6 +237,8 @@ def main() -> int:
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other workloads (64^3 CG, R-MAT graphs, batched CG, adjoint)")
+    ap.add_argument("--skip-restarted", action="store_true",
+                    help="leave out the restarted-refinement solve modes (CG precision 2, HPCG precision 3)")
     ap.add_argument("--autotune", action="store_true",
                     help="time every SpMV kernel variant on the matrix and keep the fastest "
                          "(default: the library's own choice)")
@@ -316,7 +318,8 @@ def main() -> int:
     if not args.no_solve:
         b = alp.Vector(n, fill=0.0)
         alp.mxv(b, A, alp.Vector(n, fill=1.0))
-        for precision, tag in ((0, "fp64"), (1, "mixed"), (2, "mixed_restarted")):
+        cg_modes = ((0, "fp64"), (1, "mixed")) + (() if args.skip_restarted else ((2, "mixed_restarted"),))
+        for precision, tag in cg_modes:
             xs = alp.Vector(n, fill=0.0)
             alp.cg(A, b, xs, 2, 1e-10, precision)  # warm-up (work vectors, graph capture)
             times = []
@@ -341,7 +344,8 @@ def main() -> int:
         log("[bench] building the HPCG hierarchy")
         H = alp.Hpcg(side, side, side, max_levels=3, colours=0)
         bnorm = np.sqrt(float(alp.dot(b, b)))
-        for precision, tag in ((0, "fp64"), (1, "mixed"), (2, "mixed_pc"), (3, "mixed_ir")):
+        hpcg_modes = ((0, "fp64"), (1, "mixed"), (2, "mixed_pc")) + (() if args.skip_restarted else ((3, "mixed_ir"),))
+        for precision, tag in hpcg_modes:
             xs = alp.Vector(n, fill=0.0)
             H.solve(b, xs, 2, 1e-10, precision)  # warm-up (graph capture, first touches)
             times = []
@@ -362,7 +366,7 @@ def main() -> int:
                                      "rel_residual": true_rel, "max_err_vs_exact": err,
                                      "levels": H.levels, "colours": 8}
             log(f"[bench] HPCG {tag}: it={it} outer={outer} {ms:.1f} ms rel={true_rel:.2e} err={err:.2e}")
-        for tag in ("mixed", "mixed_pc", "mixed_ir"):
+        for _, tag in hpcg_modes[1:]:
             solves[f"hpcg_speedup_{tag}_vs_fp64"] = round(solves["hpcg_fp64"]["ms"] / solves[f"hpcg_{tag}"]["ms"], 3)
         solves["hpcg_mixed_schemes"] = ("mixed = FP32 PCG iterations (FP32 V-cycle, FP64 row sums in A p, FP64 x) "
                                         "with reliable FP64 residual updates; mixed_pc = FP64 PCG with an FP32 "
