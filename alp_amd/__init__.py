@@ -90,6 +90,9 @@ SIGNATURES = {
     "alp_hip_matrix_set_kernel": (_RC, [_P, _I, _I, _I]),
     "alp_hip_matrix_get_kernel": (_RC, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "alp_hip_matrix_autotune": (_RC, [_P]),
+    "alp_hip_mtx_read": (_RC, [ctypes.c_char_p, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S), _P, _P, _P,
+                               ctypes.POINTER(_I)]),
+    "alp_hip_matrix_read_mtx": (_RC, [ctypes.POINTER(_P), ctypes.c_char_p, _I, _U]),
     "alp_hip_matrix_layout": (_RC, [_P, _I, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S)]),
     "alp_hip_mxv": (_RC, [_P, _I, _U, _P, _P, _P]),
     "alp_hip_vxm": (_RC, [_P, _I, _U, _P, _P, _P]),
@@ -307,6 +310,14 @@ class Matrix:
               "matrix_create_coo")
         return cls(h.value)
 
+    @classmethod
+    def from_mtx(cls, path: str, value_type=VAL_F64, flags=0) -> "Matrix":
+        """A MatrixMarket coordinate file, built on the device (alp_hip_matrix_read_mtx)."""
+        h = _P()
+        check(lib().alp_hip_matrix_read_mtx(ctypes.byref(h), os.fsencode(path), value_type, flags),
+              "matrix_read_mtx")
+        return cls(h.value)
+
     def download(self, which=0, with_values=True, value_dtype=np.float64):
         nmaj = self.nrows if which == 0 else self.ncols
         off = np.zeros(nmaj + 1, np.uint64)
@@ -372,6 +383,23 @@ def dot(x: Vector, y: Vector, z: float = 0.0) -> float:
     f = lib().alp_hip_dot_f32 if x.dtype == np.float32 else lib().alp_hip_dot_f64
     check(f(x.n, x.ptr, y.ptr, ctypes.byref(out)), "dot")
     return out.value
+
+
+def mtx_read(path: str):
+    """Host parse of a MatrixMarket coordinate file (alp_hip_mtx_read): (nrows, ncols,
+    I, J, V, is_pattern), entries in the reference iterator's order (a symmetric
+    file's off-diagonal entry followed by its mirror), 0-based indices."""
+    r, c, z, pat = _S(), _S(), _S(), _I()
+    p = os.fsencode(path)
+    check(lib().alp_hip_mtx_read(p, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z), None, None, None,
+                                 ctypes.byref(pat)), "mtx_read")
+    nnz = z.value
+    I = np.zeros(max(nnz, 1), np.uint32)
+    J = np.zeros(max(nnz, 1), np.uint32)
+    V = np.zeros(max(nnz, 1), np.float64)
+    check(lib().alp_hip_mtx_read(p, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z), I.ctypes.data, J.ctypes.data,
+                                 V.ctypes.data, ctypes.byref(pat)), "mtx_read")
+    return r.value, c.value, I[:nnz], J[:nnz], V[:nnz], bool(pat.value)
 
 
 def cg(A: Matrix, b, x, max_iter: int, tol: float, precision: int = 0, transpose: bool = False):

@@ -108,6 +108,22 @@ alp_rc_t alp_hip_matrix_create_crs(alp_hip_matrix_t *A, size_t nrows, size_t nco
 alp_rc_t alp_hip_matrix_create_coo(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz,
                                    const uint32_t *h_I, const uint32_t *h_J, const void *h_V,
                                    int value_type, unsigned flags);
+/* MatrixMarket coordinate files (utils/parser/matrixFileReader.hpp,
+ * matrixFileIterator.hpp): real / double / integer / pattern fields, general or
+ * symmetric storage, 1-based indices. A symmetric file's off-diagonal entry is
+ * followed by its mirror (matrixFileIterator.hpp:411-436). Complex, hermitian,
+ * skew-symmetric and dense "array" files give ALP_UNSUPPORTED; a malformed file
+ * (bad banner, index out of range, too few or too many entries) ALP_ILLEGAL.
+ * alp_hip_mtx_read parses on the host: with h_I == NULL it reports the sizes (nnz
+ * after expansion) and whether the file is a pattern; with arrays of *nnz entries
+ * it fills them in that order (h_V may be NULL; pattern files give 1.0). */
+alp_rc_t alp_hip_mtx_read(const char *path, size_t *nrows, size_t *ncols, size_t *nnz, uint32_t *h_I,
+                          uint32_t *h_J, double *h_V, int *pattern);
+/* the file's matrix built as alp_hip_matrix_create_coo from those entries, in
+ * that order (buildMatrixUnique from the file iterator, reference/io.hpp:2304);
+ * value_type ALP_VAL_PATTERN keeps no values, F64 / F32 take the file's values
+ * (1 for a pattern file) */
+alp_rc_t alp_hip_matrix_read_mtx(alp_hip_matrix_t *A, const char *path, int value_type, unsigned flags);
 /* the 27-point stencil of build_ndims_system_matrix (algorithms/hpcg/
  * matrix_building_utils.hpp:60-73), generated on the device */
 alp_rc_t alp_hip_matrix_stencil27(alp_hip_matrix_t *A, size_t nx, size_t ny, size_t nz,
