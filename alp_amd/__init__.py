@@ -39,6 +39,7 @@ SAFE_OVERLAP = 1024
 OVERWRITE = 1 << 20
 
 VAL_F64, VAL_F32, VAL_PATTERN = 0, 1, 2
+MTX_PATTERN, MTX_SYMMETRIC = 1, 2
 MAT_WITH_CCS, MAT_WITH_F32 = 1, 2
 GRAPH_SYMMETRIZE, GRAPH_DROP_SELF_LOOPS = 1, 2
 
@@ -90,8 +91,8 @@ SIGNATURES = {
     "alp_hip_matrix_set_kernel": (_RC, [_P, _I, _I, _I]),
     "alp_hip_matrix_get_kernel": (_RC, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "alp_hip_matrix_autotune": (_RC, [_P]),
-    "alp_hip_mtx_read": (_RC, [ctypes.c_char_p, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S), _P, _P, _P,
-                               ctypes.POINTER(_I)]),
+    "alp_hip_mtx_read": (_RC, [ctypes.c_char_p, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S),
+                               ctypes.POINTER(_S), _P, _P, _P, ctypes.POINTER(_I)]),
     "alp_hip_matrix_read_mtx": (_RC, [ctypes.POINTER(_P), ctypes.c_char_p, _I, _U]),
     "alp_hip_matrix_layout": (_RC, [_P, _I, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S)]),
     "alp_hip_mxv": (_RC, [_P, _I, _U, _P, _P, _P]),
@@ -389,17 +390,17 @@ def mtx_read(path: str):
     """Host parse of a MatrixMarket coordinate file (alp_hip_mtx_read): (nrows, ncols,
     I, J, V, is_pattern), entries in the reference iterator's order (a symmetric
     file's off-diagonal entry followed by its mirror), 0-based indices."""
-    r, c, z, pat = _S(), _S(), _S(), _I()
+    r, c, z, e, kind = _S(), _S(), _S(), _S(), _I()
     p = os.fsencode(path)
-    check(lib().alp_hip_mtx_read(p, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z), None, None, None,
-                                 ctypes.byref(pat)), "mtx_read")
+    check(lib().alp_hip_mtx_read(p, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z), ctypes.byref(e), None, None,
+                                 None, ctypes.byref(kind)), "mtx_read")
     nnz = z.value
     I = np.zeros(max(nnz, 1), np.uint32)
     J = np.zeros(max(nnz, 1), np.uint32)
     V = np.zeros(max(nnz, 1), np.float64)
-    check(lib().alp_hip_mtx_read(p, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z), I.ctypes.data, J.ctypes.data,
-                                 V.ctypes.data, ctypes.byref(pat)), "mtx_read")
-    return r.value, c.value, I[:nnz], J[:nnz], V[:nnz], bool(pat.value)
+    check(lib().alp_hip_mtx_read(p, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z), ctypes.byref(e),
+                                 I.ctypes.data, J.ctypes.data, V.ctypes.data, ctypes.byref(kind)), "mtx_read")
+    return r.value, c.value, I[:nnz], J[:nnz], V[:nnz], bool(kind.value & MTX_PATTERN)
 
 
 def cg(A: Matrix, b, x, max_iter: int, tol: float, precision: int = 0, transpose: bool = False):

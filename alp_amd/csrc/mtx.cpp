@@ -23,6 +23,7 @@ namespace {
 
 struct MtxData {
 	size_t nrows = 0, ncols = 0, nnz = 0;  // nnz after symmetric expansion
+	size_t entries = 0;                    // entries in the file
 	bool pattern = false, symmetric = false;
 	std::vector<uint32_t> I, J;
 	std::vector<double> V;
@@ -115,16 +116,18 @@ alp_rc_t parse(const char *path, MtxData &m) {
 	}
 	if (next_line(f, line)) return ALP_ILLEGAL;  // more entries than announced
 	m.nnz = m.I.size();
+	m.entries = e;
 	return ALP_SUCCESS;
 }
 
 }  // namespace
 
 // Parses path. Called with h_I == NULL it only reports the sizes (nnz after the
-// symmetric expansion) and whether the file is a pattern; called again with
-// arrays of nnz entries (h_V may be NULL) it fills them in emission order.
-MTX_EXPORT alp_rc_t alp_hip_mtx_read(const char *path, size_t *nrows, size_t *ncols, size_t *nnz, uint32_t *h_I,
-                                     uint32_t *h_J, double *h_V, int *pattern) {
+// symmetric expansion, the file's entry count) and the kind (ALP_MTX_PATTERN,
+// ALP_MTX_SYMMETRIC); called again with arrays of nnz entries (h_V may be NULL)
+// it fills them in emission order.
+MTX_EXPORT alp_rc_t alp_hip_mtx_read(const char *path, size_t *nrows, size_t *ncols, size_t *nnz, size_t *entries,
+                                     uint32_t *h_I, uint32_t *h_J, double *h_V, int *kind) {
 	if (!path || !nrows || !ncols || !nnz) return ALP_ILLEGAL;
 	MtxData m;
 	const alp_rc_t rc = parse(path, m);
@@ -140,7 +143,8 @@ MTX_EXPORT alp_rc_t alp_hip_mtx_read(const char *path, size_t *nrows, size_t *nc
 	*nrows = m.nrows;
 	*ncols = m.ncols;
 	*nnz = m.nnz;
-	if (pattern) *pattern = m.pattern ? 1 : 0;
+	if (entries) *entries = m.entries;
+	if (kind) *kind = (m.pattern ? ALP_MTX_PATTERN : 0) | (m.symmetric ? ALP_MTX_SYMMETRIC : 0);
 	return ALP_SUCCESS;
 }
 

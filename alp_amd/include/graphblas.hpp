@@ -399,6 +399,32 @@ RC buildMatrixUnique(Matrix<void, b, R, C, N> &A, It1 I, It2 J, const size_t nz,
 	return internal::build(A, I, J, (const double *)nullptr, nz, false);
 }
 
+namespace internal {
+template <typename It, typename VT>
+void take_value(std::vector<VT> &v, const It &it, std::true_type) {
+	v.push_back(static_cast<VT>(it.v()));
+}
+template <typename It, typename VT>
+void take_value(std::vector<VT> &, const It &, std::false_type) {}
+}  // namespace internal
+
+// nonzero-iterator form (io.hpp:1587-1745): the iterator gives i(), j() and, for a
+// valued matrix, v(), e.g. utils::MatrixFileReader::begin(); entries are taken in
+// iteration order, which decides the within-row order as in the reference
+template <Descriptor descr = descriptors::no_operation, typename T, enum Backend b, typename R, typename C,
+          typename N, typename fwd_iterator>
+RC buildMatrixUnique(Matrix<T, b, R, C, N> &A, fwd_iterator start, const fwd_iterator end, const IOMode) {
+	typedef typename std::conditional<std::is_void<T>::value, double, T>::type VT;
+	std::vector<size_t> hi, hj;
+	std::vector<VT> hv;
+	for (; start != end; ++start) {
+		hi.push_back(static_cast<size_t>(start.i()));
+		hj.push_back(static_cast<size_t>(start.j()));
+		internal::take_value(hv, start, std::integral_constant<bool, !std::is_void<T>::value>());
+	}
+	return internal::build(A, hi.data(), hj.data(), hv.data(), hi.size(), !std::is_void<T>::value);
+}
+
 // ---- level 1 (base/blas1.hpp) -----------------------------------------------------------
 
 // z = z + sum_i x_i y_i over a plus-times semiring (blas1.hpp:4137), deterministic

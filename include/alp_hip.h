@@ -115,10 +115,13 @@ alp_rc_t alp_hip_matrix_create_coo(alp_hip_matrix_t *A, size_t nrows, size_t nco
  * skew-symmetric and dense "array" files give ALP_UNSUPPORTED; a malformed file
  * (bad banner, index out of range, too few or too many entries) ALP_ILLEGAL.
  * alp_hip_mtx_read parses on the host: with h_I == NULL it reports the sizes (nnz
- * after expansion) and whether the file is a pattern; with arrays of *nnz entries
- * it fills them in that order (h_V may be NULL; pattern files give 1.0). */
-alp_rc_t alp_hip_mtx_read(const char *path, size_t *nrows, size_t *ncols, size_t *nnz, uint32_t *h_I,
-                          uint32_t *h_J, double *h_V, int *pattern);
+ * after expansion, the file's entry count; entries and kind may be NULL) and the
+ * kind (bits ALP_MTX_PATTERN, ALP_MTX_SYMMETRIC); with arrays of *nnz entries it
+ * fills them in that order (h_V may be NULL; pattern files give 1.0). */
+#define ALP_MTX_PATTERN 1
+#define ALP_MTX_SYMMETRIC 2
+alp_rc_t alp_hip_mtx_read(const char *path, size_t *nrows, size_t *ncols, size_t *nnz, size_t *entries,
+                          uint32_t *h_I, uint32_t *h_J, double *h_V, int *kind);
 /* the file's matrix built as alp_hip_matrix_create_coo from those entries, in
  * that order (buildMatrixUnique from the file iterator, reference/io.hpp:2304);
  * value_type ALP_VAL_PATTERN keeps no values, F64 / F32 take the file's values

@@ -50,3 +50,42 @@ def test_grb_program(sizes):
     # vertex 0 is a corner: within 2 hops (Chebyshev) lie 3x3x3 vertices, 7 direct neighbours
     assert res["two_hop"] == 27
     assert res["next"] == 7
+
+
+MTX_PROG = os.path.join(ROOT, "tests", "cpp", "_build", "grb_mtx_program")
+
+
+def _write_mtx(path, n, pattern, seed):
+    rng = np.random.default_rng(seed)
+    cells = set()
+    while len(cells) < 4 * n:
+        i, j = sorted((int(v) for v in rng.integers(0, n, 2)), reverse=True)
+        cells.add((i, j))
+    with open(path, "w") as f:
+        f.write(f"%%MatrixMarket matrix coordinate {'pattern' if pattern else 'real'} symmetric\n")
+        f.write(f"{n} {n} {len(cells)}\n")
+        for i, j in sorted(cells):
+            f.write(f"{i + 1} {j + 1}\n" if pattern else f"{i + 1} {j + 1} {rng.standard_normal():.17g}\n")
+    return len(cells)
+
+
+@pytest.mark.parametrize("pattern", [0, 1])
+def test_grb_matrix_file_reader_program(tmp_path, pattern):
+    """utils::MatrixFileReader + buildMatrixUnique(A, begin, end) in an ALP-style
+    program: sizes as the reader reports them, y = A 1 against scipy's reading of
+    the same file (row sums; nonzero counts for a pattern file)."""
+    import scipy.io
+    if not os.path.exists(MTX_PROG):
+        pytest.fail(f"{MTX_PROG} was not built (make -C alp_amd)")
+    path = os.path.join(str(tmp_path), "m.mtx")
+    entries = _write_mtx(path, 150, pattern, 21 + pattern)
+    print("running", MTX_PROG, path, pattern, flush=True)
+    out = subprocess.run([MTX_PROG, path, str(pattern)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    S = scipy.io.mmread(path).tocsr()
+    assert res["rc"] == 0 and res["m"] == 150 and res["n"] == 150
+    assert res["entries"] == entries and res["symmetric"] == 1 and res["pattern"] == pattern
+    assert res["nz"] == S.nnz and res["nnz"] == S.nnz
+    want = np.diff(S.indptr).astype(np.float64) if pattern else np.asarray(S.sum(axis=1)).ravel()
+    assert np.max(np.abs(np.asarray(res["y"]) - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
