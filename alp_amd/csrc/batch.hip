@@ -837,7 +837,9 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 		ALP_HIP_TRY(copy_sync(hin.data(), B->inst, sizeof(BatchInst) * ninst, hipMemcpyDeviceToHost));
 		for (int i = 0; i < ninst; ++i) its[i] = hin[i].iter;
 	} else if (precision == 1) {
-		const double delta = 0.1;
+		// replacement after a 100x drop: at C4 (64 systems x 8 RHS) 3 FP64 residual
+		// passes instead of 7 for 435 rather than 410 iterations, 5% faster overall
+		const double delta = 0.01;
 		ALP_HIP_TRY(hipMemsetAsync(B->P32, 0, sizeof(float) * ntot, s));
 		for (int first = 1;; first = 0) {
 			// FP64 residual R = B - A X for every instance, its norm, and who is done
