@@ -194,7 +194,7 @@ alp_rc_t alp_hip_convert_f32_to_f64(size_t n, const float *d_x, double *d_y);
  * Returns ALP_FAILED when not converged, as the reference does.
  * precision: 0 = FP64 throughout; 1 = CG iterations on A's FP32 copy with FP32
  * r, p, Ap (each product row summed in FP64, rounded once) and x updated in FP64,
- * with reliable updates: whenever the FP32 recurrence residual has fallen by 10x
+ * with reliable updates: whenever the FP32 recurrence residual has fallen by 100x
  * since the last update, r = b - A x is formed in FP64 and CG continues from it
  * with its search direction kept; convergence is decided on the FP64 residual,
  * and *iters counts the FP32 iterations;
@@ -244,7 +244,7 @@ alp_rc_t alp_hip_hpcg_vcycle(alp_hip_hpcg_t H, const double *d_r, double *d_z);
 /* Solve A x = b from the given x: stop when ||r||/||r0|| <= tol or after max_iter
  * iterations; *residual = ||r||. precision 0 = FP64 throughout; 1 = PCG iterations
  * in FP32 (V-cycle, r, z, p, A p with FP64 row sums) on an FP64 x, with reliable
- * updates: whenever the FP32 recurrence residual has fallen by 10x, r = b - A x is
+ * updates: whenever the FP32 recurrence residual has fallen by 100x, r = b - A x is
  * formed in FP64 and replaces it (*outer = updates; the stopping test and
  * *residual are FP64); 2 = FP64 CG (SpMV, dots, residual and solution updates in
  * FP64) whose preconditioner, the whole V-cycle (smoother, restriction SpMV,
