@@ -271,6 +271,14 @@ alp_rc_t spmv(const Storage &S, int value_type, alp_semiring_t ring, unsigned de
 // row (overwrite, no mask): the product of the mixed-precision solvers
 alp_rc_t spmv_f32_acc64(const Storage &S, float *y, const float *x, hipStream_t s, const int *skip = nullptr);
 
+// y = A x on the sliced layout (kernel 4 or 5) of a square S with one partial of
+// y.x per block written to partials (*nparts gets the count): the fused product +
+// p.Ap of the CG loops. kind 0 = FP64, 1 = FP32 (the FP32 values of S), 2 = FP32
+// data with FP64 row sums. Returns false (nothing launched) when S has no such
+// layout for these values or needs more than max_partials blocks.
+bool spmv_dot(const Storage &S, int value_type, int kind, const void *x, void *y, double *partials,
+              unsigned max_partials, unsigned *nparts, const int *skip, hipStream_t s);
+
 // a host-side preconditioner z = M^-1 r: fn(z, r, data), non-zero return = failure
 template <typename T>
 struct HostPrecond {

@@ -6,6 +6,7 @@
 // round trip for every scalar. Here one iteration is three kernels and no host
 // round trip; all scalars live in a device state block:
 //   k_cg_spmv_dot  temp = A u and one partial sum of temp.u per block, in one pass
+//                  (CSR-vector matrices; on a sliced layout k_sell_dot does the same)
 //   k_cg_update    every block folds the temp.u partials into alpha = sigma / (temp.u);
 //                  x += alpha u, r -= alpha temp and one partial of r.r per block
 //   k_cg_dir       every block folds r.r, applies the stopping test (||r|| < tol ||b||,
@@ -559,15 +560,22 @@ struct CgRunner {
 			}
 #undef ALP_CG_CASE
 		} else {
-			// for an FP64 matrix the FP32 semiring selects its FP32 value copy
-			// the product is skipped once the slot says done, as the other kernels are
-			if (acc64 && std::is_same<T, float>::value)
-				ALP_TRY(spmv_f32_acc64(S, (float *)a.temp, (const float *)a.u, s, &in->done));
-			else
-				ALP_TRY(spmv(S, vt,
-				             std::is_same<T, float>::value ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64,
-				             ALP_DESC_OVERWRITE, a.temp, a.u, nullptr, s, &in->done));
-			hipLaunchKernelGGL(k_cg_dot<T>, dim3(gvec), dim3(kBlock), 0, s, a, in);
+			// the product is skipped once the slot says done, as the other kernels are;
+			// on a sliced layout the p.Ap partials come out of the product itself
+			const int kind = std::is_same<T, double>::value ? 0 : (acc64 ? 2 : 1);
+			unsigned np = 0;
+			if (spmv_dot(S, vt, kind, a.u, a.temp, a.partials, kHalfPartials, &np, &in->done, s)) {
+				npap = np;
+			} else {
+				// for an FP64 matrix the FP32 semiring selects its FP32 value copy
+				if (acc64 && std::is_same<T, float>::value)
+					ALP_TRY(spmv_f32_acc64(S, (float *)a.temp, (const float *)a.u, s, &in->done));
+				else
+					ALP_TRY(spmv(S, vt,
+					             std::is_same<T, float>::value ? ALP_PLUS_TIMES_F32 : ALP_PLUS_TIMES_F64,
+					             ALP_DESC_OVERWRITE, a.temp, a.u, nullptr, s, &in->done));
+				hipLaunchKernelGGL(k_cg_dot<T>, dim3(gvec), dim3(kBlock), 0, s, a, in);
+			}
 		}
 		hipLaunchKernelGGL(k_cg_update<T>, dim3(gvec), dim3(kBlock), 0, s, a, in, npap);
 		ALP_LAUNCH_CHECK();

@@ -184,31 +184,27 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
 // no row offsets: slot q of the lane's row, when its mask bit is set, multiplies
 // x[row + d_q] (d_q one scalar per slot) by the slot-ordered value. Set bits come
 // in the row's storage order, so the fold is the same sequence of operations.
-template <typename R, int MASK, bool OVERWRITE, int U, bool XCD, bool DIA>
-__global__ __launch_bounds__(kBlock) void k_spmv_sell(
-    size_t nrows, size_t nslices, const uint64_t *__restrict__ soff, const uint32_t *__restrict__ sidx,
-    const void *__restrict__ sval, const uint64_t *__restrict__ off, const typename R::X *__restrict__ x,
-    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask, const uint8_t *__restrict__ dflag,
-    const int32_t *__restrict__ dcol, const uint32_t *__restrict__ dmask, const void *__restrict__ dval, const int *__restrict__ skip) {
-	if (skip && *skip) return;  // a solver loop that has stopped
+// The fold of one slice (sliced ELL, or its diagonal form when the slice has one):
+// lane `lane` folds row r of slice sl in storage order; len = the row's entry
+// count (0 for an inactive lane). Shared by the product kernels below.
+template <typename R, int U, bool DIA>
+__device__ __forceinline__ typename R::A sell_fold(size_t sl, unsigned lane, size_t r, bool active,
+                                                   const uint64_t *__restrict__ soff,
+                                                   const uint32_t *__restrict__ sidx, const void *__restrict__ sval,
+                                                   const uint64_t *__restrict__ off,
+                                                   const typename R::X *__restrict__ x,
+                                                   const uint8_t *__restrict__ dflag,
+                                                   const int32_t *__restrict__ dcol,
+                                                   const uint32_t *__restrict__ dmask,
+                                                   const void *__restrict__ dval, unsigned &len) {
 	using A = typename R::A;
-	using Y = typename R::Y;
 	using X = typename R::X;
 	using VT = typename Val<R>::T;
 	const auto *val = static_cast<const typename R::V *>(sval);
-	const unsigned bl = XCD ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-	// the slice index is uniform across the wavefront: keep it (and what is
-	// loaded from it) in scalar registers
-	const size_t sl = (size_t)__builtin_amdgcn_readfirstlane(bl * (kBlock / kSellRows) + threadIdx.x / kSellRows);
-	if (sl >= nslices) return;  // whole wavefronts leave together
-	const unsigned lane = threadIdx.x % kSellRows;
-	const size_t r = sl * kSellRows + lane;
 	const uint64_t e0 = soff[sl];
 	const unsigned w = (unsigned)((soff[sl + 1] - e0) / kSellRows);
-	bool active = r < nrows;
-	if (MASK && active) active = (MASK == 1) == (mask[r] != 0);
 	A acc = R::zero();
-	unsigned len = 0;
+	len = 0;
 	// A ring whose product converts the loaded values (FP32 data, FP64 sums) takes
 	// the branch-free form: with a load inside a per-lane branch, the compiler
 	// places the conversion next to it and waits for each load in turn.
@@ -310,12 +306,70 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
 				if (q0 + u < len) acc = R::add(Val<R>::mul(v[u], xv[u]), acc);
 		}
 	}
+	return acc;
+}
+
+template <typename R, int MASK, bool OVERWRITE, int U, bool XCD, bool DIA>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell(
+    size_t nrows, size_t nslices, const uint64_t *__restrict__ soff, const uint32_t *__restrict__ sidx,
+    const void *__restrict__ sval, const uint64_t *__restrict__ off, const typename R::X *__restrict__ x,
+    typename R::Y *__restrict__ y, const uint8_t *__restrict__ mask, const uint8_t *__restrict__ dflag,
+    const int32_t *__restrict__ dcol, const uint32_t *__restrict__ dmask, const void *__restrict__ dval, const int *__restrict__ skip) {
+	if (skip && *skip) return;  // a solver loop that has stopped
+	using A = typename R::A;
+	using Y = typename R::Y;
+	const unsigned bl = XCD ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	// the slice index is uniform across the wavefront: keep it (and what is
+	// loaded from it) in scalar registers
+	const size_t sl = (size_t)__builtin_amdgcn_readfirstlane(bl * (kBlock / kSellRows) + threadIdx.x / kSellRows);
+	if (sl >= nslices) return;  // whole wavefronts leave together
+	const unsigned lane = threadIdx.x % kSellRows;
+	const size_t r = sl * kSellRows + lane;
+	bool active = r < nrows;
+	if (MASK && active) active = (MASK == 1) == (mask[r] != 0);
+	unsigned len = 0;
+	const A acc = sell_fold<R, U, DIA>(sl, lane, r, active, soff, sidx, sval, off, x, dflag, dcol, dmask, dval, len);
 	if (!active) return;
 	if (len == 0) {
 		if (OVERWRITE) y[r] = static_cast<Y>(R::zero());
 	} else {
 		y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(static_cast<A>(y[r]), acc));
 	}
+}
+
+// y = A x over the sliced layout and one partial of y.x per block (the p.Ap of
+// CG: x = p, y = A p); the rows and their folds are those of k_spmv_sell. The
+// slice loop strides by the grid; the launcher sizes the grid to one pass.
+template <typename R, int U, bool DIA>
+__global__ __launch_bounds__(kBlock) void k_sell_dot(
+    size_t nrows, size_t nslices, const uint64_t *__restrict__ soff, const uint32_t *__restrict__ sidx,
+    const void *__restrict__ sval, const uint64_t *__restrict__ off, const typename R::X *__restrict__ x,
+    typename R::Y *__restrict__ y, const uint8_t *__restrict__ dflag, const int32_t *__restrict__ dcol,
+    const uint32_t *__restrict__ dmask, const void *__restrict__ dval, double *__restrict__ partials,
+    const int *__restrict__ skip) {
+	__shared__ double lds[kBlock / kWave];
+	if (skip && *skip) return;
+	using A = typename R::A;
+	using Y = typename R::Y;
+	constexpr unsigned kPer = kBlock / kSellRows;
+	const unsigned lane = threadIdx.x % kSellRows;
+	double part = 0.0;
+	for (size_t base = (size_t)blockIdx.x * kPer; base < nslices; base += (size_t)gridDim.x * kPer) {
+		const size_t sl = (size_t)__builtin_amdgcn_readfirstlane(base + threadIdx.x / kSellRows);
+		if (sl >= nslices) continue;  // uniform across the wavefront
+		const size_t r = sl * kSellRows + lane;
+		const bool active = r < nrows;
+		unsigned len = 0;
+		const A acc = sell_fold<R, U, DIA>(sl, lane, r, active, soff, sidx, sval, off, x, dflag, dcol, dmask, dval,
+		                                   len);
+		if (active) {
+			const Y yv = len == 0 ? static_cast<Y>(R::zero()) : static_cast<Y>(acc);
+			y[r] = yv;
+			part += (double)yv * (double)x[r];
+		}
+	}
+	part = block_sum(part, lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = part;
 }
 
 // CSR-stream, software-pipelined: while a block gathers and folds tile t, the
@@ -815,6 +869,57 @@ static void dispatch(const Storage &S, const void *val, unsigned descr, void *y,
 		else { ALP_GO(2, false) }
 	}
 #undef ALP_GO
+}
+
+template <typename R>
+static bool sell_dot_launch(const Storage &S, const void *val, const void *x, void *y, double *partials,
+                            unsigned max_partials, unsigned *nparts, const int *skip, hipStream_t s) {
+	const int k = S.kernel ? S.kernel : auto_kernel(S);
+	if (k != 4 && k != 5) return false;
+	const void *sval = nullptr, *dval = nullptr;
+	if (!sell_values(S, val, &sval)) return false;
+	const bool dia = k == 5 && dia_values(S, val, &dval);
+	// one block per kBlock / kSellRows slices, as k_spmv_sell: measured, a grid
+	// stride over more slices streams worse than the separate product and dot
+	// (192^3: +2% per CG iteration), so larger matrices keep those
+	const size_t per = kBlock / kSellRows;
+	const size_t grid = (S.n_slices + per - 1) / per;
+	if (grid == 0 || grid > max_partials) return false;
+	const DiaSlices &D = S.dia;
+	const int U = S.rows == 4 || S.rows == 16 ? S.rows : 8;
+#define ALP_SDOT(UU, DD)                                                                                    \
+	hipLaunchKernelGGL((k_sell_dot<R, UU, DD>), dim3(grid), dim3(kBlock), 0, s, S.nmajor, S.n_slices,          \
+	                   (const uint64_t *)S.sell_off, (const uint32_t *)S.sell_idx, sval, (const uint64_t *)S.off, \
+	                   (const typename R::X *)x, (typename R::Y *)y, (const uint8_t *)D.flag,                   \
+	                   (const int32_t *)D.col, (const uint32_t *)D.mask, dval, partials, skip)
+	if (dia) {
+		if (U == 4) ALP_SDOT(4, true); else if (U == 16) ALP_SDOT(16, true); else ALP_SDOT(8, true);
+	} else {
+		if (U == 4) ALP_SDOT(4, false); else if (U == 16) ALP_SDOT(16, false); else ALP_SDOT(8, false);
+	}
+#undef ALP_SDOT
+	*nparts = (unsigned)grid;
+	return true;
+}
+
+bool spmv_dot(const Storage &S, int value_type, int kind, const void *x, void *y, double *partials,
+              unsigned max_partials, unsigned *nparts, const int *skip, hipStream_t s) {
+	if (!S.off || S.nmajor == 0 || S.nmajor != S.nminor) return false;
+	switch (kind) {
+		case 0:
+			if (value_type != ALP_VAL_F64) return false;
+			return sell_dot_launch<PlusTimes<double>>(S, S.val, x, y, partials, max_partials, nparts, skip, s);
+		case 1: {
+			const void *v = value_type == ALP_VAL_F32 ? S.val : S.val32;
+			if (!v) return false;
+			return sell_dot_launch<PlusTimes<float>>(S, v, x, y, partials, max_partials, nparts, skip, s);
+		}
+		case 2:
+			if (!S.val32) return false;
+			return sell_dot_launch<PlusTimesF32Acc64>(S, S.val32, x, y, partials, max_partials, nparts, skip, s);
+		default:
+			return false;
+	}
 }
 
 alp_rc_t spmv_f32_acc64(const Storage &S, float *y, const float *x, hipStream_t s, const int *skip) {
