@@ -166,15 +166,30 @@ __device__ __forceinline__ T block_sum_partials(const T *p, unsigned n, T *lds) 
 // fold still visits each row's entries in storage order, so results equal the
 // sliced-ELL (and CRS) fold bit for bit. Other slices keep the sliced-ELL arrays.
 constexpr unsigned kDiaSlots = 32;  // most slots of a slice in diagonal form
+// A slice of this many slots (the 27-point stencil's rows) takes a fold whose
+// width is a compile-time constant; other widths take the general loop.
+constexpr unsigned kDiaFullWidth = 27;
 
 struct DiaSlices {
 	uint8_t *flag = nullptr;   // per slice: 1 = diagonal form
 	int32_t *col = nullptr;    // per slice kDiaSlots slot offsets, d_q at s * kDiaSlots + q: column - row
 	uint32_t *mask = nullptr;  // per (slice, lane): bit q set when the row has slot q
-	void *val = nullptr;       // values in slot order (sliced-ELL layout, 0 where absent)
+	void *val = nullptr;       // values in slot order at dia_pos (0 where absent)
 	float *val32 = nullptr;    // FP32 copy of them when the matrix has one
 	size_t n_dia = 0;          // slices in diagonal form
 };
+
+// Where slot q of lane `lane` sits among the values of a diagonal slice of w
+// slots starting at entry e0 (a multiple of kSellRows). 8-byte values come in
+// lane-interleaved slot pairs, [q / 2][lane][q % 2], so that one 16-byte load
+// brings a row's two consecutive slots (half the load instructions of the
+// slot-major order, which the FP64 product is bound by); an odd last slot
+// follows slot-major. 4-byte values keep the slot-major order [q][lane].
+template <typename T>
+__host__ __device__ __forceinline__ uint64_t dia_pos(uint64_t e0, unsigned w, unsigned q, unsigned lane) {
+	if (sizeof(T) == 8 && q < (w & ~1u)) return e0 + (uint64_t)(q & ~1u) * kSellRows + 2u * lane + (q & 1u);
+	return e0 + (uint64_t)q * kSellRows + lane;
+}
 
 // Builds D for a sliced ELL of nslices slices with offsets soff; slice s covers
 // rows [srow[s], srow[s+1]) or, with srow null, [64 s, min(64 s + 64, nrows)).

@@ -323,14 +323,13 @@ __device__ __forceinline__ void slice_rows(size_t sl, unsigned lane, int stop, c
 		if (dia) {
 			const uint32_t bits = act ? mbits : 0u;
 			len = __builtin_popcount(bits);
-			const T *vp = dval + e0 + lane;
 			for (unsigned q0 = 0; q0 < w; q0 += U) {
 				T v[U], xv[U];
 #pragma unroll
 				for (int u = 0; u < U; ++u) {
 					const unsigned q = q0 + u < w ? q0 + u : w - 1;
 					const long long d = __builtin_amdgcn_readlane(dl, (int)q);
-					v[u] = __builtin_nontemporal_load(vp + (size_t)q * kSellRows);
+					v[u] = __builtin_nontemporal_load(dval + dia_pos<T>(e0, w, q, lane));
 					xv[u] = x[((bits >> q) & 1u) ? (long long)row + d : 0ll];
 				}
 #pragma unroll
@@ -366,23 +365,67 @@ __device__ __forceinline__ void slice_rows(size_t sl, unsigned lane, int stop, c
 	} else if (dia) {
 		const uint32_t bits = act ? mbits : 0u;
 		len = __builtin_popcount(bits);
-		const T *vp = dval + e0 + lane;
-		for (unsigned q0 = 0; q0 < w; q0 += U) {
-			T v[U], xv[U];
+		if constexpr (sizeof(T) == 8) {
+			// slot pairs (dia_pos): one 16-byte load per pair, then an odd last
+			// slot; the fold still runs q = 0, 1, ..., w - 1
+			typedef T T2 __attribute__((ext_vector_type(2)));
+			const T2 *vp = reinterpret_cast<const T2 *>(dval + e0) + lane;
+			const unsigned np = w / 2;
+			constexpr int P = U / 2 > 0 ? U / 2 : 1;
+			for (unsigned p0 = 0; p0 < np; p0 += P) {
+				T2 v[P];
+				T xv[2 * P];
 #pragma unroll
-			for (int u = 0; u < U; ++u) {
-				const unsigned q = q0 + u;  // < 32 + U
-				const bool in = q < w && ((bits >> q) & 1u);
-				const long long d = __builtin_amdgcn_readlane(dl, (int)q);
-				v[u] = in ? __builtin_nontemporal_load(vp + (size_t)q * kSellRows) : T(0);
-				xv[u] = in ? x[(long long)row + d] : T(0);
+				for (int u = 0; u < P; ++u) {
+					const unsigned p = p0 + u;  // q = 2p + h < 32 + U
+					const bool any = p < np && ((bits >> (2 * p)) & 3u);
+					v[u] = any ? __builtin_nontemporal_load(vp + (size_t)p * kSellRows) : T2{T(0), T(0)};
+#pragma unroll
+					for (int h = 0; h < 2; ++h) {
+						const unsigned q = 2 * p + h;
+						const bool in = p < np && ((bits >> q) & 1u);
+						const long long d = __builtin_amdgcn_readlane(dl, (int)q);
+						xv[2 * u + h] = in ? x[(long long)row + d] : T(0);
+					}
+				}
+#pragma unroll
+				for (int u = 0; u < P; ++u)
+#pragma unroll
+					for (int h = 0; h < 2; ++h) {
+						const unsigned q = 2 * (p0 + u) + h;
+						if (p0 + u < np && ((bits >> q) & 1u)) {
+							const A prod = (h ? v[u].y : v[u].x) * xv[2 * u + h];
+							acc = prod + acc;
+						}
+					}
 			}
-#pragma unroll
-			for (int u = 0; u < U; ++u)
-				if (q0 + u < w && ((bits >> (q0 + u)) & 1u)) {
-					const A prod = v[u] * xv[u];
+			if (w & 1u) {
+				const unsigned q = w - 1;
+				if ((bits >> q) & 1u) {
+					const long long d = __builtin_amdgcn_readlane(dl, (int)q);
+					const A prod = __builtin_nontemporal_load(dval + e0 + (size_t)q * kSellRows + lane) *
+					               x[(long long)row + d];
 					acc = prod + acc;
 				}
+			}
+		} else {
+			for (unsigned q0 = 0; q0 < w; q0 += U) {
+				T v[U], xv[U];
+#pragma unroll
+				for (int u = 0; u < U; ++u) {
+					const unsigned q = q0 + u;  // < 32 + U
+					const bool in = q < w && ((bits >> q) & 1u);
+					const long long d = __builtin_amdgcn_readlane(dl, (int)q);
+					v[u] = in ? __builtin_nontemporal_load(dval + dia_pos<T>(e0, w, q, lane)) : T(0);
+					xv[u] = in ? x[(long long)row + d] : T(0);
+				}
+#pragma unroll
+				for (int u = 0; u < U; ++u)
+					if (q0 + u < w && ((bits >> (q0 + u)) & 1u)) {
+						const A prod = v[u] * xv[u];
+						acc = prod + acc;
+					}
+			}
 		}
 	} else {
 		if (act) len = (unsigned)(off[row + 1] - off[row]);

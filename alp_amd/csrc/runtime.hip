@@ -319,7 +319,8 @@ __global__ __launch_bounds__(kBlock) void k_dia_detect(size_t nslices, size_t nr
 }
 
 // values in slot order: slot q of a row holds its entry number popcount(mask
-// below q) when bit q is set, else 0; slices not in diagonal form are skipped
+// below q) when bit q is set, else 0, at dia_pos; slices not in diagonal form
+// are skipped
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_dia_fill(size_t nslices, size_t nrows, const uint32_t *srow,
                                                     const uint64_t *soff, const uint64_t *off, const uint8_t *flag,
@@ -335,11 +336,10 @@ __global__ __launch_bounds__(kBlock) void k_dia_fill(size_t nslices, size_t nrow
 		const uint32_t bits = r < r1 ? mask[t] : 0u;
 		const uint64_t b = r < r1 ? off[r] : 0;
 		uint64_t j = 0;
-		for (uint64_t q = 0; q < w; ++q) {
-			const uint64_t e = e0 + q * kSellRows + lane;
+		for (unsigned q = 0; q < (unsigned)w; ++q) {
 			const bool in = (bits >> q) & 1u;
-			if (dval) dval[e] = in ? val[b + j] : V(0);
-			if (dval32) dval32[e] = in ? val32[b + j] : 0.0f;
+			if (dval) dval[dia_pos<V>(e0, (unsigned)w, q, (unsigned)lane)] = in ? val[b + j] : V(0);
+			if (dval32) dval32[dia_pos<float>(e0, (unsigned)w, q, (unsigned)lane)] = in ? val32[b + j] : 0.0f;
 			j += in ? 1 : 0;
 		}
 	});
