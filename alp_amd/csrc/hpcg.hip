@@ -323,21 +323,27 @@ __device__ __forceinline__ void slice_rows(size_t sl, unsigned lane, int stop, c
 		if (dia) {
 			const uint32_t bits = act ? mbits : 0u;
 			len = __builtin_popcount(bits);
-			for (unsigned q0 = 0; q0 < w; q0 += U) {
+			auto group = [&](unsigned q0, unsigned ww) {
 				T v[U], xv[U];
 #pragma unroll
 				for (int u = 0; u < U; ++u) {
-					const unsigned q = q0 + u < w ? q0 + u : w - 1;
+					const unsigned q = q0 + u < ww ? q0 + u : ww - 1;
 					const long long d = __builtin_amdgcn_readlane(dl, (int)q);
-					v[u] = __builtin_nontemporal_load(dval + dia_pos<T>(e0, w, q, lane));
+					v[u] = __builtin_nontemporal_load(dval + dia_pos<T>(e0, ww, q, lane));
 					xv[u] = x[((bits >> q) & 1u) ? (long long)row + d : 0ll];
 				}
 #pragma unroll
 				for (int u = 0; u < U; ++u) {
 					const unsigned q = q0 + u;
 					const A t = (A)v[u] * (A)xv[u] + acc;
-					acc = (q < w && ((bits >> q) & 1u)) ? t : acc;
+					acc = (q < ww && ((bits >> q) & 1u)) ? t : acc;
 				}
+			};
+			if (w == kDiaFullWidth) {  // compile-time width: constant slots and offsets
+#pragma unroll
+				for (unsigned q0 = 0; q0 < kDiaFullWidth; q0 += U) group(q0, kDiaFullWidth);
+			} else {
+				for (unsigned q0 = 0; q0 < w; q0 += U) group(q0, w);
 			}
 		} else {
 			if (act) len = (unsigned)(off[row + 1] - off[row]);
@@ -370,9 +376,10 @@ __device__ __forceinline__ void slice_rows(size_t sl, unsigned lane, int stop, c
 			// slot; the fold still runs q = 0, 1, ..., w - 1
 			typedef T T2 __attribute__((ext_vector_type(2)));
 			const T2 *vp = reinterpret_cast<const T2 *>(dval + e0) + lane;
-			const unsigned np = w / 2;
 			constexpr int P = U / 2 > 0 ? U / 2 : 1;
-			for (unsigned p0 = 0; p0 < np; p0 += P) {
+			// pairs [p0, p0 + P) of a row of ww slots
+			auto group = [&](unsigned p0, unsigned ww) {
+				const unsigned np = ww / 2;
 				T2 v[P];
 				T xv[2 * P];
 #pragma unroll
@@ -398,33 +405,49 @@ __device__ __forceinline__ void slice_rows(size_t sl, unsigned lane, int stop, c
 							acc = prod + acc;
 						}
 					}
-			}
-			if (w & 1u) {
-				const unsigned q = w - 1;
-				if ((bits >> q) & 1u) {
-					const long long d = __builtin_amdgcn_readlane(dl, (int)q);
-					const A prod = __builtin_nontemporal_load(dval + e0 + (size_t)q * kSellRows + lane) *
-					               x[(long long)row + d];
-					acc = prod + acc;
+			};
+			auto tail = [&](unsigned ww) {
+				if (ww & 1u) {
+					const unsigned q = ww - 1;
+					if ((bits >> q) & 1u) {
+						const long long d = __builtin_amdgcn_readlane(dl, (int)q);
+						const A prod = __builtin_nontemporal_load(dval + e0 + (size_t)q * kSellRows + lane) *
+						               x[(long long)row + d];
+						acc = prod + acc;
+					}
 				}
+			};
+			if (w == kDiaFullWidth) {
+#pragma unroll
+				for (unsigned p0 = 0; p0 < kDiaFullWidth / 2; p0 += P) group(p0, kDiaFullWidth);
+				tail(kDiaFullWidth);
+			} else {
+				for (unsigned p0 = 0; p0 < w / 2; p0 += P) group(p0, w);
+				tail(w);
 			}
 		} else {
-			for (unsigned q0 = 0; q0 < w; q0 += U) {
+			auto group = [&](unsigned q0, unsigned ww) {
 				T v[U], xv[U];
 #pragma unroll
 				for (int u = 0; u < U; ++u) {
 					const unsigned q = q0 + u;  // < 32 + U
-					const bool in = q < w && ((bits >> q) & 1u);
+					const bool in = q < ww && ((bits >> q) & 1u);
 					const long long d = __builtin_amdgcn_readlane(dl, (int)q);
-					v[u] = in ? __builtin_nontemporal_load(dval + dia_pos<T>(e0, w, q, lane)) : T(0);
+					v[u] = in ? __builtin_nontemporal_load(dval + dia_pos<T>(e0, ww, q, lane)) : T(0);
 					xv[u] = in ? x[(long long)row + d] : T(0);
 				}
 #pragma unroll
 				for (int u = 0; u < U; ++u)
-					if (q0 + u < w && ((bits >> (q0 + u)) & 1u)) {
+					if (q0 + u < ww && ((bits >> (q0 + u)) & 1u)) {
 						const A prod = v[u] * xv[u];
 						acc = prod + acc;
 					}
+			};
+			if (w == kDiaFullWidth) {
+#pragma unroll
+				for (unsigned q0 = 0; q0 < kDiaFullWidth; q0 += U) group(q0, kDiaFullWidth);
+			} else {
+				for (unsigned q0 = 0; q0 < w; q0 += U) group(q0, w);
 			}
 		}
 	} else {
