@@ -228,6 +228,96 @@ def test_dia_slices_mixed_with_sliced_ell(A):
     assert np.array_equal(ym[mask != 0], base[0][mask != 0])
 
 
+def banded_crs(n, offs, rng):
+    """Rows r with columns r + d for the sorted offsets d that land in [0, n),
+    in ascending order, random values: every slice shares one slot list."""
+    R = np.repeat(np.arange(n, dtype=np.int64), len(offs))
+    C = R + np.tile(np.asarray(offs, np.int64), n)
+    keep = (C >= 0) & (C < n)
+    R, C = R[keep], C[keep]
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(np.bincount(R, minlength=n), out=off[1:])
+    return off, C.astype(np.uint32), rng.standard_normal(C.size)
+
+
+@pytest.mark.parametrize("w", [1, 2, 5, 8, 13, 26, 27, 28, 32])
+def test_dia_every_slot_width(A, w):
+    """Diagonal slices of 1 to 32 slots, odd and even. FP64 values sit in slot
+    pairs with an odd last slot on its own, FP32 values slot-major; 27 slots take
+    the compile-time-width fold. For FP64 values, their FP32 copy and an FP32
+    matrix, kernel 5 must equal kernel 4 bit for bit (both fold each row in
+    storage order) and the oracle to round-off, in accumulate and overwrite mode."""
+    rng = np.random.default_rng(100 + w)
+    n = 3001
+    offs = np.sort(rng.choice(np.arange(-45, 46), size=w, replace=False))
+    off, idx, val = banded_crs(n, offs, rng)
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(n)
+    M = A.Matrix.from_crs(n, n, off, idx, val, flags=A.MAT_WITH_F32)
+    M32 = A.Matrix.from_crs(n, n, off, idx, val.astype(np.float32))
+    for mat in (M, M32):
+        slices, _, diag = mat.layout()
+        assert diag == slices, (w, slices, diag)
+    csr = O.CSR(n, n, off, idx, val)
+    want, want_ow = O.spmv(csr, x, y0), O.spmv(csr, x)
+    cases = [(M, A.PLUS_TIMES_F64, x, y0), (M, A.PLUS_TIMES_F32, x.astype(np.float32), y0.astype(np.float32)),
+             (M32, A.PLUS_TIMES_F32, x.astype(np.float32), y0.astype(np.float32))]
+    for mat, ring, xv, yv in cases:
+        outs = []
+        for kernel, lanes, rows in SELL + DIA:
+            mat.set_kernel(kernel, lanes, rows)
+            y = A.Vector.from_numpy(yv)
+            assert A.mxv(y, mat, A.Vector.from_numpy(xv), ring=ring) == A.SUCCESS
+            yo = A.Vector.from_numpy(np.full(n, 5.0, yv.dtype))
+            assert A.mxv(yo, mat, A.Vector.from_numpy(xv), ring=ring, descr=A.OVERWRITE) == A.SUCCESS
+            outs.append((y.numpy(), yo.numpy()))
+        for got in outs[1:]:
+            assert np.array_equal(got[0], outs[0][0]) and np.array_equal(got[1], outs[0][1]), (w, ring)
+        tol = FP64_RTOL if ring == A.PLUS_TIMES_F64 else 5e-5
+        assert rel_inf(outs[0][0].astype(np.float64), want) <= tol, (w, ring)
+        assert rel_inf(outs[0][1].astype(np.float64), want_ow) <= tol, (w, ring)
+
+
+def test_cg_mixed_on_seven_point_laplacian(A):
+    """Mixed CG whose FP32 product with FP64 row sums runs on diagonal slices of 7
+    slots (6 on the boundary planes), the general-width path, next to sliced-ELL
+    slices: all three precisions reach 1e-10 on an SPD 7-point operator."""
+    nx, ny, nz = 40, 36, 30
+    n = nx * ny * nz
+    offs = [-nx * ny, -nx, -1, 0, 1, nx, nx * ny]
+    R = np.repeat(np.arange(n, dtype=np.int64), 7)
+    C = R + np.tile(np.asarray(offs, np.int64), n)
+    xs, ys, zs = R % nx, (R // nx) % ny, R // (nx * ny)
+    d = C - R
+    keep = (C >= 0) & (C < n)
+    keep &= ~((d == -1) & (xs == 0)) & ~((d == 1) & (xs == nx - 1))
+    keep &= ~((d == -nx) & (ys == 0)) & ~((d == nx) & (ys == ny - 1))
+    keep &= ~((d == -nx * ny) & (zs == 0)) & ~((d == nx * ny) & (zs == nz - 1))
+    R, C, d = R[keep], C[keep], d[keep]
+    val = np.where(d == 0, 6.5, -1.0)
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(np.bincount(R, minlength=n), out=off[1:])
+    idx = C.astype(np.uint32)
+    M = A.Matrix.from_crs(n, n, off, idx, val, flags=A.MAT_WITH_F32)
+    assert M.get_kernel()[0] == 5
+    # a slice that straddles two z-planes mixes rows with -nx (the end of one
+    # plane) and rows without it but with -nx*ny (the start of the next): no one
+    # slot list covers both, so those slices keep the sliced-ELL path
+    slices, _, diag = M.layout()
+    assert 0.9 * slices <= diag < slices
+    csr = O.CSR(n, n, off, idx, val)
+    bh = np.random.default_rng(3).standard_normal(n)
+    b = A.Vector.from_numpy(bh)
+    its = {}
+    for precision in (0, 1, 2):
+        x = A.Vector(n, fill=0.0)
+        rc, it, _ = A.cg(M, b, x, 5000, 1e-10, precision)
+        assert rc == A.SUCCESS, precision
+        assert np.linalg.norm(bh - O.spmv(csr, x.numpy())) <= 1.05e-10 * np.linalg.norm(bh), precision
+        its[precision] = it
+    assert its[1] <= 1.3 * its[0], its
+
+
 def test_mxv_random_matrix_vs_oracle_all_kernels(A):
     rng = np.random.default_rng(7)
     n, m = 5000, 4000
