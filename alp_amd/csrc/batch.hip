@@ -22,6 +22,7 @@
 #include "spmv_device.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <vector>
 
@@ -501,7 +502,8 @@ struct alp_hip_batch_s {
 	size_t ntiles = 0;
 	uint32_t *tile_row0 = nullptr, *tile_row1 = nullptr, *tile_sys = nullptr;
 	uint32_t *tile0 = nullptr;        // first tile of each system (nsys + 1)
-	int W = 0;                        // lanes per row of the SpMM (autotuned)
+	int W = 0;                        // lanes per row of the FP32-data SpMM (autotuned)
+	int W64 = 0;                      // the same for the FP64 SpMM
 	// workspace
 	double *part = nullptr;
 	BatchInst *inst = nullptr;
@@ -574,7 +576,7 @@ static alp_rc_t batch_iterate(alp_hip_batch_s *B, const T *val, T *X, T *R, T *P
 		if (X64)  // reliable updates: FP64 row sums of the FP32 data
 			launch_spmm<T, true, false, double>(B, B->W, val, P, nullptr, AP, st, B->sys_active);
 		else
-			launch_spmm<T, true, false>(B, B->W, val, P, nullptr, AP, st, B->sys_active);
+			launch_spmm<T, true, false>(B, sizeof(T) == 8 ? B->W64 : B->W, val, P, nullptr, AP, st, B->sys_active);
 		hipLaunchKernelGGL(k_b_alpha, dim3(nsys), dim3(kBlock), 0, st, nr, (const double *)B->part,
 		                   (const uint32_t *)B->tile0, B->inst);
 		hipLaunchKernelGGL(k_b_update<T>, dim3(g), dim3(kBlock), 0, st, B->ntiles, (const uint32_t *)B->tile_row0,
@@ -741,7 +743,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *out, size_t n
 		alp_hip_batch_destroy(B);
 		return rc;
 	}
-	B->W = (int)std::min<size_t>(nrhs, 8);
+	B->W = B->W64 = (int)std::min<size_t>(nrhs, 8);
 	*out = B;
 	return ALP_SUCCESS;
 }
@@ -758,41 +760,57 @@ ALP_EXPORT alp_rc_t alp_hip_batch_info(alp_hip_batch_t B, size_t *nrows, size_t 
 ALP_EXPORT alp_rc_t alp_hip_batch_spmm(alp_hip_batch_t B, const double *d_X, double *d_Y) {
 	ALP_TRY(ensure_init());
 	if (!B || !d_X || !d_Y) return ALP_ILLEGAL;
-	launch_spmm<double, false, false>(B, B->W, B->val, d_X, nullptr, d_Y, stream());
+	launch_spmm<double, false, false>(B, B->W64, B->val, d_X, nullptr, d_Y, stream());
 	ALP_LAUNCH_CHECK();
 	ALP_HIP_TRY(hipStreamSynchronize(stream()));
 	return ALP_SUCCESS;
 }
 
 // The RHS tile width: lanes per row of the SpMM, each lane owning nrhs / W
-// right-hand sides; the FP32 SpMM is timed for every width and the fastest kept.
+// right-hand sides. The two products the solvers run are timed for every width
+// and the fastest kept for each: FP64 (precision 0 and the FP64 residuals) and
+// FP32 data with FP64 row sums (the mixed iterations). *width gets the FP64 one.
 ALP_EXPORT alp_rc_t alp_hip_batch_autotune(alp_hip_batch_t B, int *width) {
 	ALP_TRY(ensure_init());
 	if (!B) return ALP_ILLEGAL;
 	hipStream_t s = stream();
+	const size_t ntot = B->nrows * B->nrhs;
+	// defined operands (the work vectors hold whatever the last solve left)
+	ALP_HIP_TRY(hipMemsetAsync(B->P, 0, sizeof(double) * ntot, s));
+	ALP_HIP_TRY(hipMemsetAsync(B->P32, 0, sizeof(float) * ntot, s));
 	hipEvent_t e0, e1;
 	ALP_HIP_TRY(hipEventCreate(&e0));
 	ALP_HIP_TRY(hipEventCreate(&e1));
-	float best = 1e30f;
-	int pick = B->W;
-	for (int W : {1, 2, 4, 8, 16}) {
-		if ((size_t)W > B->nrhs) continue;
-		for (int i = 0; i < 2; ++i) launch_spmm<float, true, false>(B, W, B->val32, B->P32, nullptr, B->AP32, s);
-		ALP_HIP_TRY(hipEventRecord(e0, s));
-		for (int i = 0; i < 5; ++i) launch_spmm<float, true, false>(B, W, B->val32, B->P32, nullptr, B->AP32, s);
-		ALP_HIP_TRY(hipEventRecord(e1, s));
-		ALP_HIP_TRY(hipEventSynchronize(e1));
-		float ms = 0;
-		ALP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-		if (ms < best) {
-			best = ms;
-			pick = W;
+	const bool verbose = std::getenv("ALP_HIP_AUTOTUNE_VERBOSE") != nullptr;
+	for (int prec = 0; prec < 2; ++prec) {
+		float best = 1e30f;
+		int pick = prec ? B->W : B->W64;
+		for (int W : {1, 2, 4, 8, 16}) {
+			if ((size_t)W > B->nrhs) continue;
+			auto run = [&] {
+				if (prec)
+					launch_spmm<float, true, false, double>(B, W, B->val32, B->P32, nullptr, B->AP32, s);
+				else
+					launch_spmm<double, true, false>(B, W, B->val, B->P, nullptr, B->AP, s);
+			};
+			for (int i = 0; i < 2; ++i) run();
+			ALP_HIP_TRY(hipEventRecord(e0, s));
+			for (int i = 0; i < 5; ++i) run();
+			ALP_HIP_TRY(hipEventRecord(e1, s));
+			ALP_HIP_TRY(hipEventSynchronize(e1));
+			float ms = 0;
+			ALP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+			if (verbose) std::fprintf(stderr, "[alp_hip] batch spmm %s W=%d: %.1f us\n", prec ? "f32" : "f64", W, ms * 200.0f);
+			if (ms < best) {
+				best = ms;
+				pick = W;
+			}
 		}
+		(prec ? B->W : B->W64) = pick;
 	}
 	hipEventDestroy(e0);
 	hipEventDestroy(e1);
-	B->W = pick;
-	if (width) *width = pick;
+	if (width) *width = B->W64;
 	return ALP_SUCCESS;
 }
 
@@ -843,7 +861,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 		ALP_HIP_TRY(hipMemsetAsync(B->P32, 0, sizeof(float) * ntot, s));
 		for (int first = 1;; first = 0) {
 			// FP64 residual R = B - A X for every instance, its norm, and who is done
-			launch_spmm<double, false, true>(B, B->W, B->val, d_X, d_B, B->R, s);
+			launch_spmm<double, false, true>(B, B->W64, B->val, d_X, d_B, B->R, s);
 			ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
 			hipLaunchKernelGGL(k_b_outer, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
 			                   (const uint32_t *)B->tile0, B->rnorm, (const double *)B->bnorm, B->outer_active,
@@ -874,7 +892,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 		const double inner_tol = 1e-4;
 		for (int o = 0; o < 100; ++o) {
 			// FP64 residual R = B - A X and its norms; instances at tol leave
-			launch_spmm<double, false, true>(B, B->W, B->val, d_X, d_B, B->R, s);
+			launch_spmm<double, false, true>(B, B->W64, B->val, d_X, d_B, B->R, s);
 			ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
 			hipLaunchKernelGGL(k_b_outer, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
 			                   (const uint32_t *)B->tile0, B->rnorm, (const double *)B->bnorm, B->outer_active,
@@ -904,7 +922,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 		}
 	}
 	// final true residuals in FP64
-	launch_spmm<double, false, true>(B, B->W, B->val, d_X, d_B, B->R, s);
+	launch_spmm<double, false, true>(B, B->W64, B->val, d_X, d_B, B->R, s);
 	ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
 	hipLaunchKernelGGL(k_b_outer, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
 	                   (const uint32_t *)B->tile0, B->rnorm, (const double *)B->bnorm, B->outer_active, tol,

@@ -300,7 +300,9 @@ alp_rc_t alp_hip_batch_destroy(alp_hip_batch_t B);
 alp_rc_t alp_hip_batch_info(alp_hip_batch_t B, size_t *nrows, size_t *nnz, uint64_t *h_row0);
 /* Y = A X over all systems (FP64) */
 alp_rc_t alp_hip_batch_spmm(alp_hip_batch_t B, const double *d_X, double *d_Y);
-/* time the SpMM for each right-hand-side tile width (lanes per row) and keep the fastest */
+/* time the SpMM for each right-hand-side tile width (lanes per row) and keep the fastest,
+   separately for the FP64 product and the FP32-data product of the mixed iterations;
+   *width (may be NULL) gets the FP64 width */
 alp_rc_t alp_hip_batch_autotune(alp_hip_batch_t B, int *width);
 /* all systems from X = 0 to ||b - A x|| <= tol ||b|| per right-hand side; precision
  * 0 = FP64 CG, 1 = FP32 SpMM (FP64 row sums) and CG vectors on an FP64 X with
