@@ -123,6 +123,7 @@ SIGNATURES = {
     "alp_hip_batch_create_laplace7": (_RC, [ctypes.POINTER(_P), _S, _P, _P, _P, _P, _P, _P, _S]),
     "alp_hip_batch_destroy": (_RC, [_P]),
     "alp_hip_batch_info": (_RC, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S), _P]),
+    "alp_hip_batch_layout": (_RC, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
     "alp_hip_batch_spmm": (_RC, [_P, _P, _P]),
     "alp_hip_batch_autotune": (_RC, [_P, ctypes.POINTER(_I)]),
     "alp_hip_batch_solve": (_RC, [_P, _P, _P, _D, _S, _I, _P, _P, ctypes.POINTER(_S)]),
@@ -522,6 +523,12 @@ class Batch:
 
     def spmm(self, X: Vector, Y: Vector) -> None:
         check(lib().alp_hip_batch_spmm(self.h, X.ptr, Y.ptr), "batch_spmm")
+
+    def layout(self):
+        """(slices, slices in diagonal form) of the SpMM's diagonal layout (0 when not built)."""
+        a, b = _S(), _S()
+        check(lib().alp_hip_batch_layout(self.h, ctypes.byref(a), ctypes.byref(b)), "batch_layout")
+        return a.value, b.value
 
     def autotune(self) -> int:
         w = _I()

@@ -21,6 +21,7 @@
 // update of X in FP64), per right-hand side, to the requested relative residual.
 #include "spmv_device.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -163,6 +164,147 @@ __global__ __launch_bounds__(kBlock) void k_spmm(size_t ntiles, const uint32_t *
 				double s = 0.0;
 				for (int w = 0; w < kBlock / kWave; ++w) s += red[w][threadIdx.x];
 				part[t * NR + threadIdx.x] = s;
+			}
+		}
+	}
+}
+
+// The same product over the diagonal form of the systems' rows: slices of at
+// most 64 rows inside each tile (tile t's first slice is tile_sl0[t]). A slice in
+// diagonal form reads no column indices, so a row's value and X loads do not
+// wait for an index load: slot q of row r multiplies X row r + d_q. Slices not in
+// diagonal form take the CRS loop of k_spmm. The fold (acc += a_k x_k in storage
+// order) and the per-tile partials are those of k_spmm, bit for bit.
+template <typename T, int W, int RPL, bool DOT, typename A = T>
+__global__ __launch_bounds__(kBlock) void k_spmm_dia(size_t ntiles, const uint32_t *tile_row0,
+                                                     const uint32_t *tile_row1, const uint32_t *__restrict__ tile_sl0,
+                                                     const uint64_t *__restrict__ soff,
+                                                     const uint8_t *__restrict__ dflag,
+                                                     const int32_t *__restrict__ dcol,
+                                                     const uint32_t *__restrict__ dmask, const T *__restrict__ dval,
+                                                     const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx,
+                                                     const T *__restrict__ val, const T *__restrict__ X,
+                                                     T *__restrict__ Y, double *part, const int *sys_active,
+                                                     const uint32_t *tile_sys) {
+	constexpr int NR = W * RPL;
+	constexpr int kRows = kBlock / W;     // rows per pass of the block
+	constexpr int kWaveRows = kWave / W;  // rows per wavefront, all in one slice
+	constexpr int U = 4;                  // slots in flight
+	__shared__ double red[kBlock / kWave][NR];
+	const int lane = threadIdx.x % W;
+	const unsigned grp = threadIdx.x / W;
+	const unsigned wl = threadIdx.x % kWave;
+	const unsigned wave_row = (threadIdx.x / kWave) * kWaveRows;
+	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+		if (sys_active && !sys_active[tile_sys[t]]) continue;
+		const uint32_t r0 = tile_row0[t], r1 = tile_row1[t], sl0 = tile_sl0[t];
+		double acc_dot[RPL];
+#pragma unroll
+		for (int q = 0; q < RPL; ++q) acc_dot[q] = 0.0;
+		for (uint32_t rb = r0; rb < r1; rb += kRows) {
+			const uint32_t wr = rb + wave_row;  // the wavefront's first row
+			if (wr >= r1) continue;             // uniform across the wavefront
+			const uint32_t sl = (uint32_t)__builtin_amdgcn_readfirstlane(sl0 + (wr - r0) / kSellRows);
+			const uint32_t r = rb + grp;
+			const bool act = r < r1;
+			const unsigned ri = (r - r0) % kSellRows;  // row within its slice
+			A acc[RPL];
+#pragma unroll
+			for (int q = 0; q < RPL; ++q) acc[q] = A(0);
+			if (dflag[sl]) {
+				const uint64_t e0 = soff[sl];
+				const unsigned w = (unsigned)((soff[sl + 1] - e0) / kSellRows);
+				const uint32_t bits = act ? dmask[(size_t)sl * kSellRows + ri] : 0u;
+				// lane q holds slot offset d_q; readlane broadcasts it
+				const int dl = wl < kDiaSlots ? dcol[(size_t)sl * kDiaSlots + wl] : 0;
+				for (unsigned q0 = 0; q0 < w; q0 += U) {
+					T a[U], xv[U][RPL];
+					if constexpr (!std::is_same<T, A>::value || sizeof(T) == 4) {
+						// FP32 data (with FP64 or FP32 sums): loads without per-lane
+						// branches (measured: the branched form waits for each load in
+						// turn); the slot clamps to the width, an absent entry reads X
+						// row 0 and the select drops it
+#pragma unroll
+						for (int u = 0; u < U; ++u) {
+							const unsigned q = q0 + u < w ? q0 + u : w - 1;
+							const bool in = (bits >> q) & 1u;
+							const long long d = __builtin_amdgcn_readlane(dl, (int)q);
+							a[u] = dval[dia_pos<T>(e0, w, q, ri)];
+							const T *xr = X + (size_t)(in ? (long long)r + d : 0ll) * NR + lane * RPL;
+#pragma unroll
+							for (int j = 0; j < RPL; ++j) xv[u][j] = xr[j];
+						}
+#pragma unroll
+						for (int u = 0; u < U; ++u) {
+							const bool in = q0 + u < w && ((bits >> (q0 + u)) & 1u);
+							const A av = (A)a[u];
+#pragma unroll
+							for (int j = 0; j < RPL; ++j) {
+								const A t = acc[j] + av * (A)xv[u][j];
+								acc[j] = in ? t : acc[j];
+							}
+						}
+					} else {
+#pragma unroll
+						for (int u = 0; u < U; ++u) {
+							const unsigned q = q0 + u;  // < 32 + U
+							const bool in = q < w && ((bits >> q) & 1u);
+							const long long d = __builtin_amdgcn_readlane(dl, (int)q);
+							a[u] = in ? dval[dia_pos<T>(e0, w, q, ri)] : T(0);
+							const T *xr = X + (size_t)((long long)r + d) * NR + lane * RPL;
+#pragma unroll
+							for (int j = 0; j < RPL; ++j) xv[u][j] = in ? xr[j] : T(0);
+						}
+#pragma unroll
+						for (int u = 0; u < U; ++u)
+							if (q0 + u < w && ((bits >> (q0 + u)) & 1u)) {
+								const A av = (A)a[u];
+#pragma unroll
+								for (int j = 0; j < RPL; ++j) acc[j] += av * (A)xv[u][j];
+							}
+					}
+				}
+			} else if (act) {
+				const uint64_t b = off[r], e = off[r + 1];
+				for (uint64_t k = b; k < e; ++k) {
+					const A av = val[k];
+					const T *xr = X + (size_t)idx[k] * NR + lane * RPL;
+#pragma unroll
+					for (int j = 0; j < RPL; ++j) acc[j] += av * (A)xr[j];
+				}
+			}
+			if (act) {
+				T *yr = Y + (size_t)r * NR + lane * RPL;
+				T yv[RPL];
+#pragma unroll
+				for (int q = 0; q < RPL; ++q) {
+					yv[q] = (T)acc[q];
+					yr[q] = yv[q];
+				}
+				if (DOT) {
+					const T *pr = X + (size_t)r * NR + lane * RPL;
+#pragma unroll
+					for (int q = 0; q < RPL; ++q) acc_dot[q] += (double)yv[q] * (double)pr[q];
+				}
+			}
+		}
+		if (DOT) {
+#pragma unroll
+			for (int q = 0; q < RPL; ++q) {
+				double v = acc_dot[q];
+				for (int sh = kWave / 2; sh >= W; sh >>= 1) v += __shfl_xor(v, sh, kWave);
+				acc_dot[q] = v;
+			}
+			__syncthreads();
+			if ((threadIdx.x % kWave) < W) {
+#pragma unroll
+				for (int q = 0; q < RPL; ++q) red[threadIdx.x / kWave][lane * RPL + q] = acc_dot[q];
+			}
+			__syncthreads();
+			if (threadIdx.x < NR) {
+				double sum = 0.0;
+				for (int wv = 0; wv < kBlock / kWave; ++wv) sum += red[wv][threadIdx.x];
+				part[t * NR + threadIdx.x] = sum;
 			}
 		}
 	}
@@ -504,6 +646,13 @@ struct alp_hip_batch_s {
 	uint32_t *tile0 = nullptr;        // first tile of each system (nsys + 1)
 	int W = 0;                        // lanes per row of the FP32-data SpMM (autotuned)
 	int W64 = 0;                      // the same for the FP64 SpMM
+	// diagonal form of the rows (slices of <= 64 rows inside each tile); empty when
+	// fewer than half the slices qualify
+	DiaSlices dia;
+	size_t nslices = 0;
+	uint32_t *srow = nullptr;         // first row of each slice (nslices + 1)
+	uint64_t *soff = nullptr;         // first entry of each slice's slot block (nslices + 1)
+	uint32_t *tile_sl0 = nullptr;     // first slice of each tile
 	// workspace
 	double *part = nullptr;
 	BatchInst *inst = nullptr;
@@ -522,11 +671,26 @@ static void launch_spmm(alp_hip_batch_s *B, int W, const T *val, const T *X, con
                         const int *sys_active = nullptr) {
 	const size_t grid = B->ntiles < (size_t)fill_blocks(8) ? B->ntiles : fill_blocks(8);
 	const int nr = (int)B->nrhs;
-#define ALP_SPMM(WW, RPL)                                                                                 \
-	hipLaunchKernelGGL((k_spmm<T, WW, RPL, DOT, RES, A>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,     \
-	                   (const uint32_t *)B->tile_row0, (const uint32_t *)B->tile_row1,                  \
-	                   (const uint64_t *)B->off, (const uint32_t *)B->idx, val, X, Bm, Y, B->part,       \
-	                   sys_active, (const uint32_t *)B->tile_sys)
+	// products (not residuals) of the matrix's own values take the diagonal form
+	// when it was built
+	const T *dval = nullptr;
+	if (!RES && B->dia.flag) {
+		if (sizeof(T) == 8 && (const void *)val == (const void *)B->val) dval = (const T *)B->dia.val;
+		if (sizeof(T) == 4 && (const void *)val == (const void *)B->val32) dval = (const T *)B->dia.val32;
+	}
+#define ALP_SPMM(WW, RPL)                                                                                        \
+	if (dval)                                                                                                   \
+		hipLaunchKernelGGL((k_spmm_dia<T, WW, RPL, DOT, A>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,         \
+		                   (const uint32_t *)B->tile_row0, (const uint32_t *)B->tile_row1,                      \
+		                   (const uint32_t *)B->tile_sl0, (const uint64_t *)B->soff,                            \
+		                   (const uint8_t *)B->dia.flag, (const int32_t *)B->dia.col,                           \
+		                   (const uint32_t *)B->dia.mask, dval, (const uint64_t *)B->off,                       \
+		                   (const uint32_t *)B->idx, val, X, Y, B->part, sys_active, (const uint32_t *)B->tile_sys); \
+	else                                                                                                        \
+		hipLaunchKernelGGL((k_spmm<T, WW, RPL, DOT, RES, A>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,        \
+		                   (const uint32_t *)B->tile_row0, (const uint32_t *)B->tile_row1,                      \
+		                   (const uint64_t *)B->off, (const uint32_t *)B->idx, val, X, Bm, Y, B->part,          \
+		                   sys_active, (const uint32_t *)B->tile_sys)
 	if (nr == 8) {
 		if (W == 8) ALP_SPMM(8, 1);
 		else if (W == 4) ALP_SPMM(4, 2);
@@ -551,7 +715,9 @@ static void launch_spmm(alp_hip_batch_s *B, int W, const T *val, const T *X, con
 }
 
 static void free_batch(alp_hip_batch_s *B) {
-	void *ps[] = {B->off, B->idx, B->val, B->val32, B->row_sys, B->sys, B->tile_row0, B->tile_row1, B->tile_sys,
+	dia_free(B->dia);
+	void *ps[] = {B->srow, B->soff, B->tile_sl0,
+	              B->off, B->idx, B->val, B->val32, B->row_sys, B->sys, B->tile_row0, B->tile_row1, B->tile_sys,
 	              B->tile0, B->part, B->inst, B->nactive, B->rnorm, B->bnorm, B->outer_active, B->sys_active, B->R, B->P, B->AP,
 	              B->R32, B->P32, B->AP32, B->X32};
 	for (void *p : ps)
@@ -737,6 +903,35 @@ ALP_EXPORT alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *out, size_t n
 		hipLaunchKernelGGL(k_lap_fill, dim3(fill_blocks(8)), dim3(kBlock), 0, st, B->nrows, (const SysDesc *)B->sys,
 		                   (const uint32_t *)B->row_sys, (const uint64_t *)B->off, B->idx, B->val);
 		if ((rc = alp_hip_convert_f64_to_f32(nnz, B->val, B->val32))) break;
+		// diagonal form: slices of <= 64 rows inside each tile, slot block width =
+		// the slice's longest row (the rows' offsets are at hand on the host)
+		std::vector<uint32_t> srow, tsl0(B->ntiles);
+		for (size_t t = 0; t < B->ntiles; ++t) {
+			tsl0[t] = (uint32_t)srow.size();
+			for (uint64_t r = trow0[t]; r < trow1[t]; r += kSellRows) srow.push_back((uint32_t)r);
+		}
+		B->nslices = srow.size();
+		srow.push_back((uint32_t)B->nrows);
+		std::vector<uint64_t> soff(B->nslices + 1, 0);
+		for (size_t sl = 0; sl < B->nslices; ++sl) {
+			uint64_t w = 0;
+			for (uint64_t r = srow[sl]; r < srow[sl + 1]; ++r) w = std::max<uint64_t>(w, hc[r + 1] - hc[r]);
+			soff[sl + 1] = soff[sl] + w * kSellRows;
+		}
+		if ((rc = bmalloc(&B->srow, B->nslices + 1)) || (rc = bmalloc(&B->soff, B->nslices + 1)) ||
+		    (rc = bmalloc(&B->tile_sl0, B->ntiles)))
+			break;
+		if (copy_sync(B->srow, srow.data(), 4 * (B->nslices + 1), hipMemcpyHostToDevice) != hipSuccess ||
+		    copy_sync(B->soff, soff.data(), 8 * (B->nslices + 1), hipMemcpyHostToDevice) != hipSuccess ||
+		    copy_sync(B->tile_sl0, tsl0.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		const char *env = std::getenv("ALP_HIP_BATCH_DIA");  // "0": CRS products only (diagnostics)
+		if (!(env && env[0] == '0') &&
+		    (rc = dia_build(B->dia, B->nslices, B->nrows, B->srow, B->soff, soff[B->nslices], B->off, B->idx,
+		                    ALP_VAL_F64, B->val, B->val32)))
+			break;
 		if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) rc = ALP_PANIC;
 	} while (0);
 	if (rc) {
@@ -753,6 +948,15 @@ ALP_EXPORT alp_rc_t alp_hip_batch_info(alp_hip_batch_t B, size_t *nrows, size_t 
 	if (nrows) *nrows = B->nrows;
 	if (nnz) *nnz = B->nnz;
 	if (h_row0) std::memcpy(h_row0, B->row0.data(), sizeof(uint64_t) * (B->nsys + 1));
+	return ALP_SUCCESS;
+}
+
+// slices of the diagonal form and how many of them qualified (0 when the form
+// was not built: fewer than half qualified, or ALP_HIP_BATCH_DIA=0)
+ALP_EXPORT alp_rc_t alp_hip_batch_layout(alp_hip_batch_t B, size_t *nslices, size_t *ndiagonal) {
+	if (!B || !nslices || !ndiagonal) return ALP_ILLEGAL;
+	*nslices = B->nslices;
+	*ndiagonal = B->dia.flag ? B->dia.n_dia : 0;
 	return ALP_SUCCESS;
 }
 

@@ -298,6 +298,10 @@ alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *B, size_t nsys, const ui
 alp_rc_t alp_hip_batch_destroy(alp_hip_batch_t B);
 /* total rows, nonzeros, and (h_row0, nsys + 1 entries, may be NULL) each system's first row */
 alp_rc_t alp_hip_batch_info(alp_hip_batch_t B, size_t *nrows, size_t *nnz, uint64_t *h_row0);
+/* slices of the batch's diagonal form (at most 64 rows each, inside the SpMM tiles)
+   and how many qualified; *ndiagonal is 0 when the form was not built. No
+   counterpart in the reference (the batch itself has none). */
+alp_rc_t alp_hip_batch_layout(alp_hip_batch_t B, size_t *nslices, size_t *ndiagonal);
 /* Y = A X over all systems (FP64) */
 alp_rc_t alp_hip_batch_spmm(alp_hip_batch_t B, const double *d_X, double *d_Y);
 /* time the SpMM for each right-hand-side tile width (lanes per row) and keep the fastest,

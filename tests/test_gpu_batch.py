@@ -47,6 +47,42 @@ def test_spmm_matches_cpu(A):
     assert B.nnz == sum(laplace7(*(int(v[s]) for v in p[:3]), 1, 1, 1).nnz for s in range(B.nsys))
 
 
+def test_diagonal_form_equals_crs_bit_for_bit(A, monkeypatch):
+    """The SpMM over the diagonal form (slices of <= 64 rows inside each tile; the
+    slices that straddle a z-plane keep the CRS loop) folds each row in storage
+    order and keeps the per-tile partials: the same batch built without the form
+    (ALP_HIP_BATCH_DIA=0) gives bit-identical products and solves."""
+    rng = np.random.default_rng(31)
+    nx, ny, nz = (rng.integers(16, 41, 6) for _ in range(3))
+    cx, cy, cz = (rng.lognormal(0.0, 0.5, 6) for _ in range(3))
+    D = A.Batch(nx, ny, nz, cx, cy, cz, 8)
+    nsl, ndia = D.layout()
+    assert nsl > 0 and 0.5 * nsl <= ndia < nsl, (nsl, ndia)
+    monkeypatch.setenv("ALP_HIP_BATCH_DIA", "0")
+    C = A.Batch(nx, ny, nz, cx, cy, cz, 8)
+    assert C.layout()[1] == 0
+    monkeypatch.delenv("ALP_HIP_BATCH_DIA")
+    X = A.Vector.from_numpy(rng.standard_normal(D.nrows * 8))
+    Yd, Yc = A.Vector(D.nrows * 8), A.Vector(D.nrows * 8)
+    D.spmm(X, Yd)
+    C.spmm(X, Yc)
+    assert np.array_equal(Yd.numpy(), Yc.numpy())
+    for s in range(D.nsys):
+        r0, r1 = int(D.row0[s]), int(D.row0[s + 1])
+        M = laplace7(int(nx[s]), int(ny[s]), int(nz[s]), float(cx[s]), float(cy[s]), float(cz[s]))
+        want = M @ X.numpy().reshape(-1, 8)[r0:r1]
+        assert np.max(np.abs(Yd.numpy().reshape(-1, 8)[r0:r1] - want)) <= 1e-12 * np.max(np.abs(want))
+    # both at the default RHS width (no autotune): the same partial-sum trees
+    rhs = A.Vector.from_numpy(rng.standard_normal(D.nrows * 8))
+    for precision in (0, 1):
+        Xd, Xc = A.Vector(D.nrows * 8, fill=0.0), A.Vector(D.nrows * 8, fill=0.0)
+        rd = D.solve(rhs, Xd, 1e-8, 5000, precision)
+        rc = C.solve(rhs, Xc, 1e-8, 5000, precision)
+        assert rd[0] == A.SUCCESS and rc[0] == A.SUCCESS
+        assert np.array_equal(rd[1], rc[1]) and np.array_equal(Xd.numpy(), Xc.numpy()), precision
+        assert np.all(rd[2] <= 1e-8)
+
+
 @pytest.mark.parametrize("precision", [0, 1, 2])
 def test_small_batch_against_direct_solves(A, precision):
     import scipy.sparse.linalg as spla
