@@ -186,6 +186,39 @@ def leg_c4(alp) -> dict:
     return out
 
 
+def profile_solve(alp, args) -> int:
+    """One solve of exactly args.iters iterations (tolerance 1e-30) at
+    args.precision, for kernel traces: the per-kernel times of an iteration
+    without the timing loops of the full run."""
+    what, prec, its = args.profile_solve, args.precision, args.iters
+    if what == "batch":
+        rng = np.random.default_rng(2024)
+        nx, ny, nz = (rng.integers(16, 97, 64) for _ in range(3))
+        cx, cy, cz = (rng.lognormal(0.0, 0.5, 64) for _ in range(3))
+        B = alp.Batch(nx, ny, nz, cx, cy, cz, 8)
+        w = B.autotune()
+        rhs = alp.Vector.from_numpy(rng.standard_normal(B.nrows * 8))
+        X = alp.Vector(B.nrows * 8, fill=0.0)
+        rc = B.solve(rhs, X, 1e-30, its, prec)[0]
+        alp.sync()
+        log(f"[bench] profile-solve batch precision={prec} iterations={its} lanes={w} rc={rc}")
+        return 0
+    side = args.side
+    A = alp.Matrix.stencil27(side, side, side, flags=alp.MAT_WITH_F32)
+    b = alp.Vector(A.nrows, fill=0.0)
+    alp.check(alp.mxv(b, A, alp.Vector(A.nrows, fill=1.0)), "mxv")
+    x = alp.Vector(A.nrows, fill=0.0)
+    if what == "cg":
+        rc, it, _ = alp.cg(A, b, x, its, 1e-30, prec)
+    else:
+        H = alp.Hpcg(side, side, side, max_levels=3, colours=0)
+        it, _, _ = H.solve(b, x, its, 1e-30, prec)
+        rc = 0
+    alp.sync()
+    log(f"[bench] profile-solve {what} {side}^3 precision={prec} iterations={it} rc={rc}")
+    return 0
+
+
 def leg_c5(alp, L, side: int = 128) -> dict:
     """Differentiable solve on the 128^3 27-point graph: theta ~ lognormal(0, 1)
     per edge (numpy seed 5), A(theta) = weighted Laplacian + 1e-2 I, b ~ N(0,1),
@@ -250,6 +283,12 @@ def main() -> int:
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain streaming kernel of known size (f64 -> f32 conversion of "
                          "2^28 elements: 2 GiB read, 1 GiB written) for HBM counter calibration")
+    ap.add_argument("--profile-solve", choices=("cg", "hpcg", "batch"), default=None,
+                    help="run one short solve of --iters iterations (tolerance 1e-30, so every "
+                         "iteration runs) at --precision and exit: a workload for kernel traces "
+                         "(cg, hpcg: the 192^3 stencil; batch: the 64-system batch)")
+    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--iters", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -260,6 +299,8 @@ def main() -> int:
     alp.init(0)
     L = alp.lib()
     side = args.side
+    if args.profile_solve:
+        return profile_solve(alp, args)
     log(f"[bench] building {side}^3 27-point stencil on the device")
     A = alp.Matrix.stencil27(side, side, side, flags=alp.MAT_WITH_F32)
     n, nnz = A.nrows, A.nnz
