@@ -87,6 +87,35 @@ def cpu_baseline(side: int, nnz: int, n: int) -> dict:
             "kind": "port", "sample": f"oracle orc_spmv_f64 on the {side}^3 stencil, median of {reps}"}
 
 
+def cpu_solve_reference(side: int, cg_iterations: int, hpcg_iterations: int):
+    """The reference's own solvers (oracle/_ref, reference_omp) on this host, on a
+    bounded sample: 10 CG iterations and 2 HPCG iterations at side^3 (system
+    builds not timed). The full-solve estimates scale the per-iteration time by
+    the iteration counts of the device FP64 solves, which match the reference's
+    (the CG and HPCG fixture tests)."""
+    import oracle as O
+    if not O.ref_available("omp"):
+        return None
+    R = O.ref("omp")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    it = ctypes.c_size_t()
+    out = {"kind": "reference", "cores": threads}
+    ms = R.ref_time_stencil_cg(side, side, side, 10, 1e-300, ctypes.byref(it))
+    if ms > 0 and it.value:
+        per = ms / it.value
+        out["cg_fp64"] = {"iterations_timed": it.value, "ms_per_iteration": round(per, 2),
+                          "est_ms_full_solve": round(per * cg_iterations, 1), "iterations": cg_iterations}
+    ms = R.ref_time_hpcg(side, side, side, 3, 8, 2, ctypes.byref(it))
+    if ms > 0 and it.value:
+        per = ms / it.value
+        out["hpcg_fp64"] = {"iterations_timed": it.value, "ms_per_iteration": round(per, 2),
+                            "est_ms_full_solve": round(per * hpcg_iterations, 1), "iterations": hpcg_iterations}
+    out["sample"] = (f"ALP reference_omp conjugate_gradient (10 iterations) and hpcg (2 iterations, 4 levels, "
+                     f"8 colours) on the {side}^3 stencil; est_ms_full_solve = ms_per_iteration x the FP64 "
+                     f"iteration count")
+    return out
+
+
 def measured_traffic(kernel: int, lanes: int, rows: int):
     """HBM bytes per launch of this SpMV configuration from the committed rocprofv3
     FETCH_SIZE + WRITE_SIZE summaries (profiles/*_hbm_counters.md, calibrated on a
@@ -462,6 +491,11 @@ def main() -> int:
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline")
         out["cpu_baseline"] = cpu_baseline(side, nnz, n)
+        if solves.get("cg_fp64") and solves.get("hpcg_fp64"):
+            log("[bench] CPU reference solves (bounded sample)")
+            ref = cpu_solve_reference(side, solves["cg_fp64"]["iterations"], solves["hpcg_fp64"]["iterations"])
+            if ref:
+                out["cpu_baseline"]["solves"] = ref
     print(json.dumps(out), flush=True)
     return 0
 

@@ -365,5 +365,7 @@ def ref(flavour: str = "seq") -> ctypes.CDLL:
         R.ref_time_stencil_mxv.argtypes = [_S, _S, _S, _S]
         R.ref_time_stencil_cg.restype = _D
         R.ref_time_stencil_cg.argtypes = [_S, _S, _S, _S, _D, _P]
+        R.ref_time_hpcg.restype = _D
+        R.ref_time_hpcg.argtypes = [_S, _S, _S, _S, _S, _S, _P]
         _refs[flavour] = R
     return _refs[flavour]

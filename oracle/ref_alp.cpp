@@ -285,6 +285,38 @@ REF_API int ref_knn( size_t n, size_t nedges, const uint32_t * src, const uint32
 }
 
 /**
+ * CPU baseline timing: the reference's HPCG (hpcg.hpp:97) on the nx*ny*nz system
+ * of build_hpcg_system, b = A*1, x0 = 0, for exactly `max_iter` iterations
+ * (tolerance 0); returns wall milliseconds of the hpcg call (the system build is
+ * not timed) and its iteration count.
+ */
+REF_API double ref_time_hpcg( size_t nx, size_t ny, size_t nz, size_t max_levels, size_t colors,
+	size_t max_iter, size_t * iters
+) {
+	if( ensure_init() ) { return -1; }
+	const std::array< size_t, 3 > sizes = { nx, ny, nz };
+	grb::algorithms::hpcg_system_params< 3, double > params {
+		sizes, 1, colors, 26.0, -1.0, 4, max_levels, 2
+	};
+	std::unique_ptr< grb::algorithms::hpcg_data< double, double, double > > holder;
+	if( grb::algorithms::build_hpcg_system< 3, double >( holder, params ) != grb::SUCCESS ) { return -1; }
+	auto &A = holder->A;
+	auto &x = holder->x;
+	auto &b = holder->b;
+	grb::set( x, 1.0 );
+	grb::set( b, 0.0 );
+	grb::mxv( b, A, x, RealRing() );
+	grb::set( x, 0.0 );
+	size_t it = 0;
+	double res = 0;
+	const double t0 = now_ms();
+	grb::algorithms::hpcg( *holder, true, 1, 1, max_iter, 0.0, it, res );
+	const double dt = now_ms() - t0;
+	*iters = it;
+	return dt;
+}
+
+/**
  * CPU baseline timing: builds the nx*ny*nz stencil, then times `reps`
  * grb::set(y,0) + grb::mxv<dense>(y, A, x) pairs after one warm-up; returns the
  * median milliseconds of one mxv (the set is timed apart and not included).

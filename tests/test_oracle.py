@@ -376,3 +376,15 @@ def test_oracle_pcg_jacobi_on_constant_diagonal_is_plain_cg():
     s1, x1, i1, r1 = O.pcg(A.off, A.idx, A.val, b, np.zeros(n), 1000, 1e-10, cb)
     assert s0 == s1 == 0 and i0 == i1
     assert np.max(np.abs(x0 - x1)) <= 1e-12 * np.max(np.abs(x0))
+
+
+def test_reference_timing_entry_points():
+    """The bench's CPU reference legs: the compiled reference's CG and HPCG run
+    the requested number of iterations and report a positive time."""
+    import ctypes
+    if not O.ref_available("omp"):
+        pytest.skip("oracle/_ref (the compiled reference) is not built here")
+    R = O.ref("omp")
+    it = ctypes.c_size_t()
+    assert R.ref_time_stencil_cg(16, 16, 16, 5, 1e-300, ctypes.byref(it)) > 0 and it.value == 5
+    assert R.ref_time_hpcg(16, 16, 16, 2, 8, 3, ctypes.byref(it)) > 0 and it.value == 3
