@@ -172,7 +172,7 @@ def leg_c1(alp, L) -> dict:
 def leg_c2(alp, scale: int = 22) -> dict:
     """R-MAT scale 22, edge factor 16, (a, b, c) = (.57, .19, .19), seed 1, symmetrised,
     deduplicated, no self-loops; BFS from the highest-degree vertex; PageRank
-    (0.85, 1e-9)."""
+    (0.85, 1e-9); knn (k-hop reachability) 1 and 3 hops from the same vertex."""
     n = 1 << scale
     (src, dst), t_gen = timed(lambda: alp.rmat_edges(scale, 16 * n, seed=1), alp.sync)
     G, t_build = timed(lambda: alp.graph_from_edges(n, src, dst), alp.sync)
@@ -185,13 +185,23 @@ def leg_c2(alp, scale: int = 22) -> dict:
         (lev, nl, reached), dt = timed(lambda: alp.bfs(G, s), alp.sync)
         times.append(dt)
     bfs_ms = 1e3 * sorted(times)[1]
+    # warm-up: the first call builds the column (CCS) layout the pull product reads,
+    # which the reference builds with the matrix (buildMatrixUnique fills CRS and CCS)
+    alp.pagerank(G, 0.85, 1e-9, 1000)
     (rc, pr, it, q), t_pr = timed(lambda: alp.pagerank(G, 0.85, 1e-9, 1000), alp.sync)
     pr_bytes = 4 * nnz + 8 * (n + 1) + 32 * n
+    # knn (k-hop reachability) from the same source, 1 and 3 hops
+    kh = {}
+    for k in (1, 3):
+        alp.khop(G, s, k)  # warm-up
+        out, t_kh = timed(lambda: alp.khop(G, s, k), alp.sync)
+        kh[k] = (round(1e3 * t_kh, 3), int(np.count_nonzero(out.numpy())))
     return {"n": n, "nnz": nnz, "source": s, "source_degree": deg, "gen_ms": round(1e3 * t_gen, 2),
             "build_ms": round(1e3 * t_build, 2), "bfs_ms": round(bfs_ms, 3), "bfs_levels": nl,
             "bfs_reached": reached, "bfs_gteps": round(nnz / 2 / (bfs_ms * 1e-3) / 1e9, 2),
             "pagerank_rc": rc, "pagerank_iterations": it, "pagerank_ms": round(1e3 * t_pr, 2),
-            "pagerank_gbs": round(pr_bytes * it / t_pr / 1e9, 1)}
+            "pagerank_gbs": round(pr_bytes * it / t_pr / 1e9, 1),
+            "khop1_ms": kh[1][0], "khop1_reached": kh[1][1], "khop3_ms": kh[3][0], "khop3_reached": kh[3][1]}
 
 
 def leg_c4(alp) -> dict:
