@@ -330,8 +330,8 @@ def main() -> int:
     ap.add_argument("--iters", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path; run with --gpus 1",
+              file=sys.stderr)
         return 2
 
     import alp_amd as alp
