@@ -16,9 +16,10 @@
 // their flags stop updates), the host only reads the active count between
 // batches of iterations.
 //
-// precision 0: FP64 CG. precision 1: FP32 SpMM and CG iterations on each
-// instance's residual equation inside FP64 iterative refinement (B - A X and the
-// update of X in FP64), per right-hand side, to the requested relative residual.
+// precision 0: FP64 CG. precision 1: FP32 SpMM (FP64 row sums) and CG iterations
+// with reliable FP64 residual updates (X kept in FP64). precision 2: FP32 SpMM and
+// CG iterations on each instance's residual equation inside FP64 iterative
+// refinement. Per right-hand side, to the requested relative residual.
 #include "spmv_device.hpp"
 
 #include <algorithm>
@@ -357,7 +358,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_update(size_t ntiles, const uint32_t *tile_row0,
                                                      const uint32_t *tile_row1, const uint32_t *tile_sys, int nrhs,
                                                      T *X, T *R, const T *P, const T *AP, const BatchInst *in,
-                                                     double *part, const int *sys_active, double *X64) {
+                                                     double *part, const int *sys_active) {
 	__shared__ double red[kBlock];
 	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
 		if (!sys_active[tile_sys[t]]) continue;  // uniform per block: the barriers below stay matched
@@ -372,10 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_b_update(size_t ntiles, const uint32
 		double acc = 0.0;
 		for (size_t k = threadIdx.x; k < cnt; k += kBlock) {
 			const size_t e = base + k;
-			if (X64)  // reliable updates: the solution is FP64
-				X64[e] = I.alpha * (double)P[e] + X64[e];
-			else
-				X[e] = alpha * P[e] + X[e];
+			X[e] = alpha * P[e] + X[e];
 			const T rv = R[e] - alpha * AP[e];
 			R[e] = rv;
 			acc += (double)rv * (double)rv;
@@ -511,6 +509,14 @@ __global__ __launch_bounds__(kBlock) void k_b_outer(int nrhs, const double *part
 	if (!done) atomicAdd(nactive, 1u);
 }
 
+// group-wise update of the FP64 solution: X += X32, X32 = 0 (first: X32 = 0 only)
+__global__ __launch_bounds__(kBlock) void k_b_fold(size_t n, double *X, float *X32, int first) {
+	grid_for_each<4>(n, [&](size_t i) {
+		if (!first) X[i] += (double)X32[i];
+		X32[i] = 0.0f;
+	});
+}
+
 // R32 = R64 / rnorm, X32 = 0, P32 = R32 (row-major blocks)
 __global__ __launch_bounds__(kBlock) void k_b_down(size_t ntot, int nrhs, const double *R, const double *rnorm,
                                                    const uint32_t *row_sys, float *R32, float *X32, float *P32) {
@@ -551,8 +557,9 @@ __global__ __launch_bounds__(kBlock) void k_b_bnorm(int nrhs, const double *part
 
 // ---- reliable updates (precision 1) -----------------------------------------------
 //
-// FP32 CG iterations per instance (FP32 R, P, AP; FP64 row sums in the SpMM; FP64
-// X updated every iteration). An instance's segment ends when its FP32 recurrence
+// FP32 CG iterations per instance (FP32 R, P, AP; FP64 row sums in the SpMM; X
+// kept in FP64, a segment's increments accumulated in X32 and added to X before
+// each FP64 residual). An instance's segment ends when its FP32 recurrence
 // residual drops below max(tol ||b||, delta ||r_true||); then R = B - A X in FP64
 // for all instances, R32 = (float) R and P = R32 + (R32.R32 / rz) P, keeping the
 // search direction (reliable updated residuals, Sleijpen and van der Vorst, 1996).
@@ -733,13 +740,13 @@ static alp_rc_t bmalloc(T **p, size_t n) {
 // Iterate the inner CG on precision T until no instance is active.
 template <typename T>
 static alp_rc_t batch_iterate(alp_hip_batch_s *B, const T *val, T *X, T *R, T *P, T *AP, size_t max_iter,
-                              size_t *its, double *X64 = nullptr) {
+                              size_t *its, bool acc64 = false) {
 	hipStream_t s = stream();
 	const unsigned nsys = (unsigned)B->nsys;
 	const int nr = (int)B->nrhs;
 	const size_t g = B->ntiles < (size_t)fill_blocks(8) ? B->ntiles : fill_blocks(8);
 	auto iteration = [&](hipStream_t st) {
-		if (X64)  // reliable updates: FP64 row sums of the FP32 data
+		if (acc64)  // reliable updates: FP64 row sums of the FP32 data
 			launch_spmm<T, true, false, double>(B, B->W, val, P, nullptr, AP, st, B->sys_active);
 		else
 			launch_spmm<T, true, false>(B, sizeof(T) == 8 ? B->W64 : B->W, val, P, nullptr, AP, st, B->sys_active);
@@ -747,7 +754,7 @@ static alp_rc_t batch_iterate(alp_hip_batch_s *B, const T *val, T *X, T *R, T *P
 		                   (const uint32_t *)B->tile0, B->inst);
 		hipLaunchKernelGGL(k_b_update<T>, dim3(g), dim3(kBlock), 0, st, B->ntiles, (const uint32_t *)B->tile_row0,
 		                   (const uint32_t *)B->tile_row1, (const uint32_t *)B->tile_sys, nr, X, R, (const T *)P,
-		                   (const T *)AP, (const BatchInst *)B->inst, B->part, (const int *)B->sys_active, X64);
+		                   (const T *)AP, (const BatchInst *)B->inst, B->part, (const int *)B->sys_active);
 		hipLaunchKernelGGL(k_b_beta, dim3(nsys), dim3(kBlock), 0, st, nr, (const double *)B->part,
 		                   (const uint32_t *)B->tile0, B->inst, (unsigned long long)max_iter, B->nactive,
 		                   B->sys_active);
@@ -1064,7 +1071,10 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 		const double delta = 0.01;
 		ALP_HIP_TRY(hipMemsetAsync(B->P32, 0, sizeof(float) * ntot, s));
 		for (int first = 1;; first = 0) {
-			// FP64 residual R = B - A X for every instance, its norm, and who is done
+			// X += X32 (the last segment's update, accumulated in FP32: a group-wise
+			// update of the FP64 solution), X32 = 0; then the FP64 residual R = B - A X
+			// for every instance, its norm, and who is done
+			hipLaunchKernelGGL(k_b_fold, dim3(fill_blocks(8)), dim3(kBlock), 0, s, ntot, d_X, B->X32, first);
 			launch_spmm<double, false, true>(B, B->W64, B->val, d_X, d_B, B->R, s);
 			ALP_HIP_TRY(hipMemsetAsync(B->nactive, 0, sizeof(unsigned), s));
 			hipLaunchKernelGGL(k_b_outer, dim3(B->nsys), dim3(kBlock), 0, s, nr, (const double *)B->part,
@@ -1088,7 +1098,7 @@ ALP_EXPORT alp_rc_t alp_hip_batch_solve(alp_hip_batch_t B, const double *d_B, do
 			                   (const uint32_t *)B->tile_row1, (const uint32_t *)B->tile_sys, nr, B->P32,
 			                   (const float *)B->R32, (const BatchInst *)B->inst, (const int *)B->sys_active);
 			size_t it = 0;
-			ALP_TRY(batch_iterate<float>(B, B->val32, B->X32, B->R32, B->P32, B->AP32, max_iter, &it, d_X));
+			ALP_TRY(batch_iterate<float>(B, B->val32, B->X32, B->R32, B->P32, B->AP32, max_iter, &it, true));
 		}
 		ALP_HIP_TRY(copy_sync(hin.data(), B->inst, sizeof(BatchInst) * ninst, hipMemcpyDeviceToHost));
 		for (int i = 0; i < ninst; ++i) its[i] = hin[i].iter;

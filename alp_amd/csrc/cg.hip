@@ -278,17 +278,25 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(CgArgs<T> a, const T *z, con
 // ---- reliable updates (mixed precision) -------------------------------------------
 //
 // CG iterations on the FP32 copy of A with FP32 vectors r, p, Ap; the product
-// accumulates each row in FP64 (one rounding per row) and x is kept and updated
-// in FP64 (x += alpha p). Both matter: FP32 row sums of a stencil cancel heavily
-// on smooth vectors, and an FP32 solution increment re-injects its rounding into
-// every true residual; with both in FP64 the iteration count stays at the FP64
-// method's. A segment of iterations ends when the recurrence residual drops below
-// max(tol ||b||, delta ||r_true||); then r_true = b - A x in FP64, r = (float)
-// r_true and p = r + (r.r / sigma) p, keeping the search direction (reliable
+// accumulates each row in FP64 (one rounding per row): FP32 row sums of a stencil
+// cancel heavily on smooth vectors. x is kept in FP64; a segment's increments
+// (x32 += alpha p) accumulate in FP32 and are added to x before each FP64
+// residual (the group-wise update). A segment of iterations ends when the
+// recurrence residual drops below max(tol ||b||, delta ||r_true||); then
+// r_true = b - A x in FP64, r = (float) r_true and p = r + (r.r / sigma) p,
+// keeping the search direction (reliable
 // updated residuals: Sleijpen and van der Vorst, 1996; the double/single scheme
 // of Clark et al., 2010). Convergence is decided on the FP64 residual:
 // ||b - A x|| < tol ||b||.
 constexpr unsigned kQuarterPartials = kMaxPartials / 4;
+
+// group-wise update of the FP64 solution: x += x32, x32 = 0 (first: x32 = 0 only)
+__global__ __launch_bounds__(kBlock) void k_ru_fold(size_t n, double *x, float *x32, int first) {
+	grid_for_each<4>(n, [&](size_t i) {
+		if (!first) x[i] += (double)x32[i];
+		x32[i] = 0.0f;
+	});
+}
 
 // r32 = (float)(b - t) (t = A x); partials of r_true.r_true, r32.r32 and (first) b.b
 __global__ __launch_bounds__(kBlock) void k_ru_resid(size_t n, const double *b, const double *t, float *r32,
@@ -723,20 +731,24 @@ static alp_rc_t cg_reliable(alp_hip_matrix_t A, Storage &S, const double *b, dou
 	ALP_TRY(storage_ensure_f32(S, A->nnz));
 	hipStream_t s = stream();
 	CgSlot &slot = cg_slot(A, &S == &A->ccs ? kSlotReliableT : kSlotReliable);
-	void *bp[5];
-	ALP_TRY(slot.buffers({8 * n, 4 * n, 4 * n, 4 * n, sizeof(CgState)}, bp));
+	void *bp[6];
+	ALP_TRY(slot.buffers({8 * n, 4 * n, 4 * n, 4 * n, sizeof(CgState), 4 * n}, bp));
 	double *t64 = (double *)bp[0];
-	float *r32 = (float *)bp[1], *p32 = (float *)bp[2], *ap32 = (float *)bp[3];
+	float *r32 = (float *)bp[1], *p32 = (float *)bp[2], *ap32 = (float *)bp[3], *x32 = (float *)bp[5];
 	CgState *st = (CgState *)bp[4];
-	CgArgs<float> a{n, S.off, S.idx, S.val32, nullptr, r32, p32, ap32, nullptr, ctx().d_partials,
-	                ctx().d_tickets, st, x};
+	// a segment's iterations accumulate their update of x in FP32 (x32); it is
+	// folded into the FP64 x before each FP64 residual (group-wise update)
+	CgArgs<float> a{n, S.off, S.idx, S.val32, x32, r32, p32, ap32, nullptr, ctx().d_partials, ctx().d_tickets, st};
 	CgRunner<float> runner(S, a, ALP_VAL_F64, true);
 	unsigned g = vec_grid(n);
 	if (g > kQuarterPartials) g = kQuarterPartials;
 	double *part = ctx().d_partials;
 	CgState h{};
+	const unsigned gf = (unsigned)std::min<size_t>(fill_blocks(8), (n + kBlock - 1) / kBlock);
 	for (int first = 1;; first = 0) {
-		// r = b - A x in FP64; r32, p and the state from it
+		// x += x32 (the last segment's update), x32 = 0; then r = b - A x in FP64;
+		// r32, p and the state from it
+		hipLaunchKernelGGL(k_ru_fold, dim3(gf ? gf : 1), dim3(kBlock), 0, s, n, x, x32, first);
 		ALP_TRY(spmv(S, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, t64, x, nullptr, s));
 		hipLaunchKernelGGL(k_ru_resid, dim3(g), dim3(kBlock), 0, s, n, b, (const double *)t64, r32, part, first);
 		hipLaunchKernelGGL(k_ru_dir, dim3(g), dim3(kBlock), 0, s, n, (const float *)r32, p32, (const double *)part, g,
@@ -747,8 +759,8 @@ static alp_rc_t cg_reliable(alp_hip_matrix_t A, Storage &S, const double *b, dou
 		ALP_HIP_TRY(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
 		ALP_HIP_TRY(hipStreamSynchronize(s));
 		if (h.s[0].done) break;
-		// FP32 iterations (x updated in FP64) until the recurrence residual reaches
-		// the segment's threshold
+		// FP32 iterations (the update of x accumulated in x32) until the recurrence
+		// residual reaches the segment's threshold
 		ALP_TRY(runner.run(s, &h, &slot));
 	}
 	*iters = h.s[0].iter;

@@ -193,7 +193,8 @@ alp_rc_t alp_hip_convert_f32_to_f64(size_t n, const float *d_x, double *d_y);
  * *residual = ||r||^2 (the reference's convention), *iters = iterations done.
  * Returns ALP_FAILED when not converged, as the reference does.
  * precision: 0 = FP64 throughout; 1 = CG iterations on A's FP32 copy with FP32
- * r, p, Ap (each product row summed in FP64, rounded once) and x updated in FP64,
+ * r, p, Ap (each product row summed in FP64, rounded once) and x kept in FP64
+ * (a segment's increments accumulated in FP32 and added to it at each update),
  * with reliable updates: whenever the FP32 recurrence residual has fallen by 100x
  * since the last update, r = b - A x is formed in FP64 and CG continues from it
  * with its search direction kept; convergence is decided on the FP64 residual,
@@ -309,7 +310,8 @@ alp_rc_t alp_hip_batch_spmm(alp_hip_batch_t B, const double *d_X, double *d_Y);
    *width (may be NULL) gets the FP64 width */
 alp_rc_t alp_hip_batch_autotune(alp_hip_batch_t B, int *width);
 /* all systems from X = 0 to ||b - A x|| <= tol ||b|| per right-hand side; precision
- * 0 = FP64 CG, 1 = FP32 SpMM (FP64 row sums) and CG vectors on an FP64 X with
+ * 0 = FP64 CG, 1 = FP32 SpMM (FP64 row sums) and CG vectors on an FP64 X (a
+ * segment's increments accumulated in FP32 and added at each update) with
  * reliable FP64 residual updates per instance, 2 = restarted FP32 SpMM/CG inside
  * FP64 iterative refinement. h_iters and
  * h_relres (nsys * nrhs entries each, may be NULL): CG iterations and final true
