@@ -418,9 +418,9 @@ def main() -> int:
             log(f"[bench] CG {tag}: rc={rc} it={it} {dt * 1e3:.1f} ms err={err:.2e}")
         if "cg_fp64" in solves and "cg_mixed" in solves:
             solves["cg_speedup_mixed_vs_fp64"] = round(solves["cg_fp64"]["ms"] / solves["cg_mixed"]["ms"], 3)
-            solves["cg_mixed_scheme"] = ("FP32 iterations (FP32 A, r, p, Ap; FP64 row sums; FP64 x) with reliable "
-                                         "FP64 residual updates; mixed_restarted = FP32 inner solves in FP64 "
-                                         "iterative refinement")
+            solves["cg_mixed_scheme"] = ("FP32 iterations (FP32 A, r, p, Ap; FP64 row sums; FP64 x, its increments "
+                                         "accumulated in FP32 between updates) with reliable FP64 residual updates; "
+                                         "mixed_restarted = FP32 inner solves in FP64 iterative refinement")
         # HPCG: 4 grids (192/96/48/24), 8-colour symmetric Gauss-Seidel, b = A*1, x0 = 0
         log("[bench] building the HPCG hierarchy")
         H = alp.Hpcg(side, side, side, max_levels=3, colours=0)
