@@ -321,8 +321,27 @@ __device__ __forceinline__ void system_sums(const double *part, const uint32_t *
 	double acc[kMaxRhs];
 #pragma unroll
 	for (int r = 0; r < kMaxRhs; ++r) acc[r] = 0.0;
-	for (uint32_t t = tile0[s] + threadIdx.x; t < tile0[s + 1]; t += kBlock)
-		for (int r = 0; r < nrhs; ++r) acc[r] += part[(size_t)t * nrhs + r];
+	// four tiles' partials in flight per step (a one-tile loop waits on every
+	// load); each thread still adds its tiles in increasing order
+	const uint32_t t1 = tile0[s + 1];
+	uint32_t t = tile0[s] + threadIdx.x;
+	for (; t + 3 * kBlock < t1; t += 4 * kBlock) {
+		double v[4][kMaxRhs];
+#pragma unroll
+		for (int u = 0; u < 4; ++u)
+#pragma unroll
+			for (int r = 0; r < kMaxRhs; ++r)
+				v[u][r] = r < nrhs ? part[(size_t)(t + u * kBlock) * nrhs + r] : 0.0;
+#pragma unroll
+		for (int u = 0; u < 4; ++u)
+#pragma unroll
+			for (int r = 0; r < kMaxRhs; ++r)
+				if (r < nrhs) acc[r] += v[u][r];
+	}
+	for (; t < t1; t += kBlock)
+#pragma unroll
+		for (int r = 0; r < kMaxRhs; ++r)
+			if (r < nrhs) acc[r] += part[(size_t)t * nrhs + r];
 #pragma unroll
 	for (int r = 0; r < kMaxRhs; ++r) {
 		double v = acc[r];
