@@ -147,11 +147,21 @@ __device__ __forceinline__ void grid_for_each(size_t n, F &&f) {
 	for (; i < n; i += stride) f(i);
 }
 
-// fixed-order sum of n partials by one block (n <= kMaxPartials)
+// fixed-order sum of n partials by one block (n <= kMaxPartials); each thread
+// adds its partials i, i + kBlock, ... in order, four loads in flight per step
 template <typename T>
 __device__ __forceinline__ T block_sum_partials(const T *p, unsigned n, T *lds) {
 	T v = T(0);
-	for (unsigned i = threadIdx.x; i < n; i += kBlock) v += __builtin_nontemporal_load(p + i);
+	unsigned i = threadIdx.x;
+	for (; i + 3 * kBlock < n; i += 4 * kBlock) {
+		const T a0 = __builtin_nontemporal_load(p + i), a1 = __builtin_nontemporal_load(p + i + kBlock);
+		const T a2 = __builtin_nontemporal_load(p + i + 2 * kBlock), a3 = __builtin_nontemporal_load(p + i + 3 * kBlock);
+		v += a0;
+		v += a1;
+		v += a2;
+		v += a3;
+	}
+	for (; i < n; i += kBlock) v += __builtin_nontemporal_load(p + i);
 	return block_sum(v, lds);
 }
 
