@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -726,7 +727,12 @@ static alp_rc_t cg_reliable(alp_hip_matrix_t A, Storage &S, const double *b, dou
                             double tol, size_t *iters, double *residual) {
 	// residual replacement after a 100x drop of the FP32 recurrence: at 192^3 the
 	// iteration count equals that of a 10x interval, with half the FP64 updates
-	const double delta = 0.01;
+	// (ALP_HIP_CG_DELTA overrides the factor, for diagnostics)
+	double delta = 0.01;
+	if (const char *e = std::getenv("ALP_HIP_CG_DELTA")) {
+		const double v = std::atof(e);
+		if (v > 0.0 && v < 1.0) delta = v;
+	}
 	const size_t n = A->nrows;
 	ALP_TRY(storage_ensure_f32(S, A->nnz));
 	hipStream_t s = stream();
