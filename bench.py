@@ -15,10 +15,16 @@ fewer bytes than CRS. roofline.achieved / roofline.frac count these CRS
 algorithmic bytes over the kernel's event time; the bytes of the layout actually
 streamed (format_bytes_per_launch) give format_achieved_gbs / format_frac, and
 roofline.traffic is the measured HBM traffic per launch. The solve legs (FP64 and mixed-precision CG / HPCG to
-1e-10 relative residual) are reported beside it. Inputs are resident in HBM
-before the timed region.
+1e-10 relative residual) are reported beside it (`solves`), and `configs` holds
+the other workloads: 50 CG iterations at 64^3, BFS / PageRank / k-hop on an R-MAT
+graph of scale 22, 64 batched systems x 8 right-hand sides, and the
+differentiable solve at 128^3. `cpu_baseline` times ALP's own reference_omp on
+the host: the same SpMV, and (`cpu_baseline.solves`) bounded samples of its CG
+and HPCG. Inputs are resident in HBM before the timed region.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py --profile-solve {cg,hpcg,batch} --precision P --iters N   (a short
+        solve for kernel traces)
 """
 from __future__ import annotations
 
