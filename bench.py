@@ -486,7 +486,7 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{side}^3 27-point stencil (n={n}, nnz={nnz}), FP64 plus-times "
-                               f"grb::mxv y += A x on device CRS",
+                               f"grb::mxv y += A x on device-resident storage (CRS and the diagonal-slice copy the kernel streams)",
                    "kernel": KERNEL_NAMES.get(kernel, str(kernel)), "lanes": lanes, "rows": rows,
                    "bytes_per_step": bytes_step},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
