@@ -274,6 +274,7 @@ struct alp_hip_matrix_s {
 	// graph-algorithm work buffers (BFS queues, degrees, scan space), kept likewise
 	void *graph_cache = nullptr;
 	void (*free_graph_cache)(void *) = nullptr;
+	bool symmetric_pattern = false;  // built symmetrised (alp_hip_graph_from_edges)
 };
 
 namespace alp {

@@ -424,6 +424,7 @@ ALP_EXPORT alp_rc_t alp_hip_graph_from_edges(alp_hip_matrix_t *A, size_t n, size
 		return rc;
 	}
 	if (nnz_out) *nnz_out = M->nnz;
+	M->symmetric_pattern = sym;
 	*A = M;
 	return ALP_SUCCESS;
 }
@@ -484,13 +485,11 @@ static alp_rc_t bfs_work(alp_hip_matrix_s *A, BfsWork **out) {
 	return ALP_SUCCESS;
 }
 
-ALP_EXPORT alp_rc_t alp_hip_bfs(alp_hip_matrix_t A, uint32_t source, int32_t *d_level, size_t *nlevels,
-                                size_t *reached) {
-	ALP_TRY(ensure_init());
-	if (!A || !d_level) return ALP_ILLEGAL;
-	if (A->nrows != A->ncols) return ALP_ILLEGAL;
+// BFS levels from source, at most max_depth levels deep (deeper vertices keep -1);
+// shared by alp_hip_bfs and, on symmetric patterns, alp_hip_khop
+static alp_rc_t bfs_run(alp_hip_matrix_s *A, uint32_t source, int32_t *d_level, int32_t max_depth, size_t *nlevels,
+                        size_t *reached) {
 	const size_t n = A->nrows;
-	if (source >= n) return ALP_ILLEGAL;
 	hipStream_t s = stream();
 	const Storage &S = A->crs;
 	BfsWork *w = nullptr;
@@ -512,7 +511,7 @@ ALP_EXPORT alp_rc_t alp_hip_bfs(alp_hip_matrix_t A, uint32_t source, int32_t *d_
 	int32_t cur = 0;
 	const double alpha = 14.0, beta = 24.0;  // direction switching (Beamer et al.)
 	bool pull = false;
-	while (fq > 0) {
+	while (fq > 0 && cur < max_depth) {
 		// ALP's choice between scatter and gather, by the frontier's edge count
 		if (!pull && (double)fe > (double)unexplored / alpha) pull = true;
 		else if (pull && (double)fq < (double)n / beta) pull = false;
@@ -552,6 +551,22 @@ ALP_EXPORT alp_rc_t alp_hip_bfs(alp_hip_matrix_t A, uint32_t source, int32_t *d_
 	if (nlevels) *nlevels = (size_t)cur + 1;
 	if (reached) *reached = total;
 	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_bfs(alp_hip_matrix_t A, uint32_t source, int32_t *d_level, size_t *nlevels,
+                                size_t *reached) {
+	ALP_TRY(ensure_init());
+	if (!A || !d_level) return ALP_ILLEGAL;
+	if (A->nrows != A->ncols) return ALP_ILLEGAL;
+	if (source >= A->nrows) return ALP_ILLEGAL;
+	return bfs_run(A, source, d_level, std::numeric_limits<int32_t>::max(), nlevels, reached);
+}
+
+__global__ __launch_bounds__(kBlock) void k_within(size_t n, const int32_t *level, int32_t k, uint8_t *out) {
+	grid_for_each<4>(n, [&](size_t i) {
+		const int32_t l = level[i];
+		out[i] = (l >= 0 && l <= k) ? 1 : 0;
+	});
 }
 
 ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double alpha, double conv, size_t max_iter,
@@ -614,6 +629,19 @@ ALP_EXPORT alp_rc_t alp_hip_khop(alp_hip_matrix_t A, uint32_t source, size_t k, 
 	const size_t n = A->nrows;
 	if (source >= n) return ALP_ILLEGAL;
 	hipStream_t s = stream();
+	if (A->symmetric_pattern) {
+		// on a symmetric pattern the k-hop set is the BFS ball of radius k: a
+		// frontier BFS cut at depth k instead of k full lor-land products
+		DevBuf<int32_t> lev;
+		ALP_TRY(lev.alloc(n));
+		const int32_t depth = k < (size_t)std::numeric_limits<int32_t>::max() ? (int32_t)k
+		                                                                        : std::numeric_limits<int32_t>::max();
+		ALP_TRY(bfs_run(A, source, lev.p, depth, nullptr, nullptr));
+		hipLaunchKernelGGL(k_within, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const int32_t *)lev.p, depth, d_out);
+		ALP_LAUNCH_CHECK();
+		ALP_HIP_TRY(hipStreamSynchronize(s));
+		return ALP_SUCCESS;
+	}
 	DevBuf<uint8_t> cur;
 	ALP_TRY(cur.alloc(n));
 	hipLaunchKernelGGL(k_u8_single, dim3(g_grid(n)), dim3(kBlock), 0, s, n, cur.p, source);

@@ -236,6 +236,19 @@ def profile_solve(alp, args) -> int:
     args.precision, for kernel traces: the per-kernel times of an iteration
     without the timing loops of the full run."""
     what, prec, its = args.profile_solve, args.precision, args.iters
+    if what == "graph":
+        n = 1 << 22
+        src, dst = alp.rmat_edges(22, 16 * n, seed=1)
+        G = alp.graph_from_edges(n, src, dst)
+        del src, dst
+        s, _ = alp.max_degree_vertex(G)
+        alp.pagerank(G, 0.85, 1e-9, 1000)  # builds the CCS
+        lev, nl, reached = alp.bfs(G, s)
+        rc, pr, it, q = alp.pagerank(G, 0.85, 1e-9, 1000)
+        alp.khop(G, s, 3)
+        alp.sync()
+        log(f"[bench] profile-solve graph bfs_levels={nl} pagerank_iterations={it} rc={rc}")
+        return 0
     if what == "batch":
         rng = np.random.default_rng(2024)
         nx, ny, nz = (rng.integers(16, 97, 64) for _ in range(3))
@@ -328,10 +341,11 @@ def main() -> int:
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain streaming kernel of known size (f64 -> f32 conversion of "
                          "2^28 elements: 2 GiB read, 1 GiB written) for HBM counter calibration")
-    ap.add_argument("--profile-solve", choices=("cg", "hpcg", "batch"), default=None,
+    ap.add_argument("--profile-solve", choices=("cg", "hpcg", "batch", "graph"), default=None,
                     help="run one short solve of --iters iterations (tolerance 1e-30, so every "
                          "iteration runs) at --precision and exit: a workload for kernel traces "
-                         "(cg, hpcg: the 192^3 stencil; batch: the 64-system batch)")
+                         "(cg, hpcg: the 192^3 stencil; batch: the 64-system batch; graph: BFS, "
+                         "PageRank and 3-hop k-hop on the scale-22 R-MAT graph)")
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--iters", type=int, default=48)
     args = ap.parse_args()

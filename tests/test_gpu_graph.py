@@ -116,3 +116,28 @@ def test_rmat_scale22_bfs_and_pagerank_invariants(A):
     p = pr.numpy()
     assert rc == A.SUCCESS and q <= 1e-9
     assert abs(p.sum() - 1.0) < 1e-8 and p.min() > 0
+
+
+@pytest.mark.parametrize("scale,seed", [(10, 3), (16, 7)])
+def test_khop_on_symmetric_graph_matches_product_path(A, scale, seed):
+    """A symmetrised graph answers k-hop with a BFS cut at depth k; the same
+    pattern loaded as a plain CRS matrix takes the k boolean products.  Both
+    must equal the oracle for every k, including 0 and a k past the diameter."""
+    n, m = 1 << scale, 8 << scale
+    s, d = A.rmat_edges(scale, m, seed=seed)
+    G = A.graph_from_edges(n, s, d)
+    goff, gidx, _ = G.download(0, with_values=False)
+    P = A.Matrix.from_crs(n, n, goff, gidx[:int(goff[-1])])
+    ref = O.CSR(n, n, goff, gidx, None)
+    src, _ = A.max_degree_vertex(G)
+    rng = np.random.default_rng(seed)
+    for source in (src, int(rng.integers(0, n))):
+        for k in (0, 1, 2, 3, 5, 1 << 40):
+            if k <= 5:
+                want = O.khop(ref, source, k)
+            else:  # past the diameter: every vertex BFS reaches
+                want = (O.bfs_levels(ref, source) >= 0).astype(np.uint8)
+            got = A.khop(G, source, k).numpy()
+            assert np.array_equal(got, want), (source, k)
+            if k <= 5:
+                assert np.array_equal(A.khop(P, source, k).numpy(), want), (source, k)
