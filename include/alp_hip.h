@@ -282,7 +282,10 @@ alp_rc_t alp_hip_bfs(alp_hip_matrix_t A, uint32_t source, int32_t *d_level, size
  * ranks; returns ALP_FAILED when `conv` was not reached within max_iter */
 alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double alpha, double conv,
                           size_t max_iter, size_t *iters, double *quality);
-/* knn (algorithms/knn.hpp:82-140): d_out[v] = 1 iff v is within k hops of source */
+/* knn (algorithms/knn.hpp:82-140): d_out[v] = 1 iff v is within k hops of source.
+ * A matrix built by alp_hip_graph_from_edges with ALP_GRAPH_SYMMETRIZE answers by a
+ * breadth-first search stopped at depth k; any other matrix by k lor-land products,
+ * as the reference does (same result on a symmetric pattern) */
 alp_rc_t alp_hip_khop(alp_hip_matrix_t A, uint32_t source, size_t k, uint8_t *d_out);
 
 /* ---- batched multi-right-hand-side CG (no counterpart in the reference; each
