@@ -319,6 +319,10 @@ static alp_rc_t edges_to_keys(size_t nedges, const uint32_t *src, const uint32_t
                               bool drop_loops, bool transpose, DevBuf<uint64_t> &out, size_t *nout) {
 	hipStream_t s = stream();
 	const size_t m = symmetric ? 2 * nedges : nedges;
+	if (m == 0) {  // no edges: an empty pattern, nothing to sort
+		*nout = 0;
+		return ALP_SUCCESS;
+	}
 	DevBuf<uint64_t> keys, sorted;
 	ALP_TRY(keys.alloc(m));
 	ALP_TRY(sorted.alloc(m));

@@ -141,3 +141,26 @@ def test_khop_on_symmetric_graph_matches_product_path(A, scale, seed):
             assert np.array_equal(got, want), (source, k)
             if k <= 5:
                 assert np.array_equal(A.khop(P, source, k).numpy(), want), (source, k)
+
+
+def test_edgeless_and_loop_only_graphs(A):
+    """No edges, or only self loops that the build drops: every search stays at
+    its source, and a source outside the graph is rejected."""
+    n = 1000
+    none = np.zeros(0, np.uint32)
+    loops = np.arange(0, n, 7, dtype=np.uint32)
+    for s, d in ((none, none), (loops, loops)):
+        G = A.graph_from_edges(n, A.Vector.from_numpy(s), A.Vector.from_numpy(d))
+        assert G.nnz == 0
+        lev, nl, reached = A.bfs(G, 5)
+        want = np.full(n, -1, np.int32)
+        want[5] = 0
+        assert np.array_equal(lev.numpy(), want) and nl == 1 and reached == 1
+        one = np.zeros(n, np.uint8)
+        one[5] = 1
+        for k in (0, 1, 4):
+            assert np.array_equal(A.khop(G, 5, k).numpy(), one), k
+        with pytest.raises(A.AlpError):
+            A.khop(G, n, 1)
+        with pytest.raises(A.AlpError):
+            A.bfs(G, n)
