@@ -164,3 +164,9 @@ def test_edgeless_and_loop_only_graphs(A):
             A.khop(G, n, 1)
         with pytest.raises(A.AlpError):
             A.bfs(G, n)
+        # every vertex dangling: the teleport term alone, uniform ranks
+        rc, pr, it, q = A.pagerank(G, 0.85, 1e-9, 1000)
+        z = O.CSR(n, n, np.zeros(n + 1, np.uint64), np.zeros(1, np.uint32), None)
+        orc, opr, oit, _ = O.pagerank(z, z, 0.85, 1e-9, 1000)
+        assert rc == orc == A.SUCCESS and abs(it - oit) <= 1
+        assert np.max(np.abs(pr.numpy() - 1.0 / n)) <= 1e-15 and np.max(np.abs(opr - 1.0 / n)) <= 1e-15
