@@ -236,6 +236,21 @@ def profile_solve(alp, args) -> int:
     args.precision, for kernel traces: the per-kernel times of an iteration
     without the timing loops of the full run."""
     what, prec, its = args.profile_solve, args.precision, args.iters
+    if what == "adjoint":
+        side = 128
+        n = side ** 3
+        rng = np.random.default_rng(5)
+        theta = alp.Vector.from_numpy(rng.lognormal(0.0, 1.0, 13 * n))
+        b = alp.Vector.from_numpy(rng.standard_normal(n))
+        A = alp.Matrix.conductance27(side, side, side, theta, 1e-2,
+                                     alp.MAT_WITH_CCS | (alp.MAT_WITH_F32 if prec else 0))
+        x, lam, dth = alp.Vector(n, fill=0.0), alp.Vector(n, fill=0.0), alp.Vector(13 * n)
+        rc1, it1, _ = alp.cg(A, b, x, its, 1e-30, prec)
+        rc2, it2, _ = alp.cg(A, b, lam, its, 1e-30, prec, transpose=True)
+        alp.check(alp.lib().alp_hip_grad_conductance27(side, side, side, lam.ptr, x.ptr, dth.ptr), "grad")
+        alp.sync()
+        log(f"[bench] profile-solve adjoint precision={prec} forward_iterations={it1} adjoint_iterations={it2}")
+        return 0
     if what == "graph":
         n = 1 << 22
         src, dst = alp.rmat_edges(22, 16 * n, seed=1)
@@ -341,11 +356,12 @@ def main() -> int:
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain streaming kernel of known size (f64 -> f32 conversion of "
                          "2^28 elements: 2 GiB read, 1 GiB written) for HBM counter calibration")
-    ap.add_argument("--profile-solve", choices=("cg", "hpcg", "batch", "graph"), default=None,
+    ap.add_argument("--profile-solve", choices=("cg", "hpcg", "batch", "graph", "adjoint"), default=None,
                     help="run one short solve of --iters iterations (tolerance 1e-30, so every "
                          "iteration runs) at --precision and exit: a workload for kernel traces "
                          "(cg, hpcg: the 192^3 stencil; batch: the 64-system batch; graph: BFS, "
-                         "PageRank and 3-hop k-hop on the scale-22 R-MAT graph)")
+                         "PageRank and 3-hop k-hop on the scale-22 R-MAT graph; adjoint: forward and transposed "
+                         "solves on the 128^3 conductance matrix, then the dtheta scatter)")
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--iters", type=int, default=48)
     args = ap.parse_args()
