@@ -460,6 +460,7 @@ struct BfsWork {
 	DevBuf<uint64_t> deg, scan;
 	DevBuf<BfsCounters> cnt;
 	DevBuf<char> tmp;
+	DevBuf<int32_t> lev;  // k-hop's levels (alp_hip_bfs writes the caller's array)
 	size_t scan_tb = 0;
 };
 
@@ -472,7 +473,7 @@ static alp_rc_t bfs_work(alp_hip_matrix_s *A, BfsWork **out) {
 		if (!w) return ALP_OUTOFMEM;
 		alp_rc_t rc = ALP_SUCCESS;
 		if ((rc = w->q0.alloc(n)) || (rc = w->q1.alloc(n)) || (rc = w->deg.alloc(n + 1)) ||
-		    (rc = w->scan.alloc(n + 1)) || (rc = w->cnt.alloc(1))) {
+		    (rc = w->scan.alloc(n + 1)) || (rc = w->cnt.alloc(1)) || (rc = w->lev.alloc(n))) {
 			delete w;
 			return rc;
 		}
@@ -636,12 +637,12 @@ ALP_EXPORT alp_rc_t alp_hip_khop(alp_hip_matrix_t A, uint32_t source, size_t k, 
 	if (A->symmetric_pattern) {
 		// on a symmetric pattern the k-hop set is the BFS ball of radius k: a
 		// frontier BFS cut at depth k instead of k full lor-land products
-		DevBuf<int32_t> lev;
-		ALP_TRY(lev.alloc(n));
+		BfsWork *w = nullptr;
+		ALP_TRY(bfs_work(A, &w));
 		const int32_t depth = k < (size_t)std::numeric_limits<int32_t>::max() ? (int32_t)k
 		                                                                        : std::numeric_limits<int32_t>::max();
-		ALP_TRY(bfs_run(A, source, lev.p, depth, nullptr, nullptr));
-		hipLaunchKernelGGL(k_within, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const int32_t *)lev.p, depth, d_out);
+		ALP_TRY(bfs_run(A, source, w->lev.p, depth, nullptr, nullptr));
+		hipLaunchKernelGGL(k_within, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const int32_t *)w->lev.p, depth, d_out);
 		ALP_LAUNCH_CHECK();
 		ALP_HIP_TRY(hipStreamSynchronize(s));
 		return ALP_SUCCESS;
