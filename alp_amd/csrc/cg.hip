@@ -722,9 +722,11 @@ static alp_rc_t cg_mixed(alp_hip_matrix_t A, Storage &S, const double *b, double
 
 // Mixed precision with reliable updates (see k_ru_resid): FP32 segments replayed
 // by the CG runner, FP64 residual updates between them. *iters counts the FP32
-// iterations; *residual = ||b - A x||^2 of the returned x.
+// iterations; *residual = ||b - A x||^2 of the returned x. x_fp64: every
+// iteration adds alpha p to the FP64 x in FP64; otherwise a segment's increments
+// accumulate in an FP32 x32 that is added to x before each FP64 residual.
 static alp_rc_t cg_reliable(alp_hip_matrix_t A, Storage &S, const double *b, double *x, size_t max_iter,
-                            double tol, size_t *iters, double *residual) {
+                            double tol, size_t *iters, double *residual, bool x_fp64) {
 	// residual replacement after a 100x drop of the FP32 recurrence: at 192^3 the
 	// iteration count equals that of a 10x interval, with half the FP64 updates
 	// (ALP_HIP_CG_DELTA overrides the factor, for diagnostics)
@@ -745,6 +747,7 @@ static alp_rc_t cg_reliable(alp_hip_matrix_t A, Storage &S, const double *b, dou
 	// a segment's iterations accumulate their update of x in FP32 (x32); it is
 	// folded into the FP64 x before each FP64 residual (group-wise update)
 	CgArgs<float> a{n, S.off, S.idx, S.val32, x32, r32, p32, ap32, nullptr, ctx().d_partials, ctx().d_tickets, st};
+	if (x_fp64) a.x64 = x;  // x32 then stays zero and the fold below adds nothing
 	CgRunner<float> runner(S, a, ALP_VAL_F64, true);
 	unsigned g = vec_grid(n);
 	if (g > kQuarterPartials) g = kQuarterPartials;
@@ -754,7 +757,8 @@ static alp_rc_t cg_reliable(alp_hip_matrix_t A, Storage &S, const double *b, dou
 	for (int first = 1;; first = 0) {
 		// x += x32 (the last segment's update), x32 = 0; then r = b - A x in FP64;
 		// r32, p and the state from it
-		hipLaunchKernelGGL(k_ru_fold, dim3(gf ? gf : 1), dim3(kBlock), 0, s, n, x, x32, first);
+		if (!x_fp64 || first)
+			hipLaunchKernelGGL(k_ru_fold, dim3(gf ? gf : 1), dim3(kBlock), 0, s, n, x, x32, first);
 		ALP_TRY(spmv(S, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, t64, x, nullptr, s));
 		hipLaunchKernelGGL(k_ru_resid, dim3(g), dim3(kBlock), 0, s, n, b, (const double *)t64, r32, part, first);
 		hipLaunchKernelGGL(k_ru_dir, dim3(g), dim3(kBlock), 0, s, n, (const float *)r32, p32, (const double *)part, g,
@@ -869,8 +873,9 @@ static alp_rc_t cg_entry(alp_hip_matrix_t A, bool transpose, const double *b, do
 	if (transpose && !A->ccs.off) ALP_TRY(build_ccs_from_crs(A));
 	Storage &S = transpose ? A->ccs : A->crs;
 	if (precision == 0) return cg_fp64(A, S, b, x, max_iter, tol, iters, residual);
-	if (precision == 1) return cg_reliable(A, S, b, x, max_iter, tol, iters, residual);
+	if (precision == 1) return cg_reliable(A, S, b, x, max_iter, tol, iters, residual, true);
 	if (precision == 2) return cg_mixed(A, S, b, x, max_iter, tol, iters, residual);
+	if (precision == 3) return cg_reliable(A, S, b, x, max_iter, tol, iters, residual, false);
 	return ALP_ILLEGAL;
 }
 

@@ -131,10 +131,13 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push(size_t q, size_t fedges, co
 	});
 }
 
-// pull: every unvisited vertex scans its row for a neighbour in the frontier
-// (level == cur) and stops at the first one (lor early exit)
+// pull: every unvisited vertex scans its in-neighbours (the CCS column of w; the
+// CRS row when the pattern is symmetric) for one in the frontier (level == cur)
+// and stops at the first one (lor early exit). The edge counter adds w's
+// out-degree (out_off), which is what a following push step expands.
 __global__ __launch_bounds__(kBlock) void k_bfs_pull(size_t n, const uint64_t *off, const uint32_t *idx,
-                                                     int32_t *level, int32_t cur, BfsCounters *cnt) {
+                                                     const uint64_t *out_off, int32_t *level, int32_t cur,
+                                                     BfsCounters *cnt) {
 	unsigned long long nv = 0, ne = 0;
 	grid_for_each<1>(n, [&](size_t w) {
 		if (level[w] >= 0) return;
@@ -143,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_pull(size_t n, const uint64_t *o
 			if (level[idx[k]] == cur) {
 				level[w] = cur + 1;
 				nv += 1;
-				ne += e - b;
+				ne += out_off[w + 1] - out_off[w];
 				break;
 			}
 		}
@@ -497,6 +500,10 @@ static alp_rc_t bfs_run(alp_hip_matrix_s *A, uint32_t source, int32_t *d_level, 
 	const size_t n = A->nrows;
 	hipStream_t s = stream();
 	const Storage &S = A->crs;
+	// a pull step walks in-neighbours: the CCS, or the CRS itself when the
+	// pattern was built symmetric
+	if (!A->symmetric_pattern && !A->ccs.off) ALP_TRY(build_ccs_from_crs(A));
+	const Storage &P = A->symmetric_pattern ? A->crs : A->ccs;
 	BfsWork *w = nullptr;
 	ALP_TRY(bfs_work(A, &w));
 	DevBuf<uint32_t> &q0 = w->q0, &q1 = w->q1;
@@ -522,8 +529,8 @@ static alp_rc_t bfs_run(alp_hip_matrix_s *A, uint32_t source, int32_t *d_level, 
 		else if (pull && (double)fq < (double)n / beta) pull = false;
 		ALP_HIP_TRY(hipMemsetAsync(cnt.p, 0, sizeof(BfsCounters), s));
 		if (pull) {
-			hipLaunchKernelGGL(k_bfs_pull, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint64_t *)S.off,
-			                   (const uint32_t *)S.idx, d_level, cur, cnt.p);
+			hipLaunchKernelGGL(k_bfs_pull, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint64_t *)P.off,
+			                   (const uint32_t *)P.idx, (const uint64_t *)S.off, d_level, cur, cnt.p);
 			queue_valid = false;
 		} else {
 			if (!queue_valid) {

@@ -14,6 +14,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -53,7 +56,7 @@ bool next_line(FILE *f, std::string &line) {
 	}
 }
 
-alp_rc_t parse(const char *path, MtxData &m) {
+alp_rc_t parse_file(const char *path, MtxData &m) {
 	FILE *f = std::fopen(path, "r");
 	if (!f) return ALP_ILLEGAL;
 	struct Closer {
@@ -85,9 +88,16 @@ alp_rc_t parse(const char *path, MtxData &m) {
 	if (m.symmetric && r != c) return ALP_ILLEGAL;
 	m.nrows = r;
 	m.ncols = c;
-	m.I.reserve(m.symmetric ? 2 * e : e);
-	m.J.reserve(m.symmetric ? 2 * e : e);
-	if (!m.pattern) m.V.reserve(m.symmetric ? 2 * e : e);
+	// a coordinate file lists each position at most once: more entries than the
+	// r x c positions is a malformed header (and too many for 32-bit indices)
+	if (r && c && e / r > c) return ALP_ILLEGAL;
+	if (e > (1ull << 40)) return ALP_OVERFLW;
+	// the header's count is not trusted for the reservation: at most 2^24
+	// entries up front, the vectors grow past that as entries are read
+	const size_t cap = (size_t)std::min<unsigned long long>(m.symmetric ? 2 * e : e, 1ull << 24);
+	m.I.reserve(cap);
+	m.J.reserve(cap);
+	if (!m.pattern) m.V.reserve(cap);
 	for (unsigned long long k = 0; k < e; ++k) {
 		if (!next_line(f, line)) return ALP_ILLEGAL;  // fewer entries than announced
 		const char *p = line.c_str();
@@ -118,6 +128,18 @@ alp_rc_t parse(const char *path, MtxData &m) {
 	m.nnz = m.I.size();
 	m.entries = e;
 	return ALP_SUCCESS;
+}
+
+// parse_file, with an allocation failure (a file too large for host memory)
+// reported as ALP_OUTOFMEM instead of an exception leaving the C ABI
+alp_rc_t parse(const char *path, MtxData &m) {
+	try {
+		return parse_file(path, m);
+	} catch (const std::bad_alloc &) {
+		return ALP_OUTOFMEM;
+	} catch (const std::exception &) {
+		return ALP_PANIC;
+	}
 }
 
 }  // namespace
@@ -161,13 +183,17 @@ MTX_EXPORT alp_rc_t alp_hip_matrix_read_mtx(alp_hip_matrix_t *A, const char *pat
 	const void *vals = nullptr;
 	std::vector<double> v64;
 	std::vector<float> v32;
-	if (value_type == ALP_VAL_F64) {
-		if (m.pattern) v64.assign(m.nnz, 1.0); else v64.swap(m.V);
-		vals = v64.data();
-	} else if (value_type == ALP_VAL_F32) {
-		v32.resize(m.nnz);
-		for (size_t k = 0; k < m.nnz; ++k) v32[k] = m.pattern ? 1.0f : (float)m.V[k];
-		vals = v32.data();
+	try {
+		if (value_type == ALP_VAL_F64) {
+			if (m.pattern) v64.assign(m.nnz, 1.0); else v64.swap(m.V);
+			vals = v64.data();
+		} else if (value_type == ALP_VAL_F32) {
+			v32.resize(m.nnz);
+			for (size_t k = 0; k < m.nnz; ++k) v32[k] = m.pattern ? 1.0f : (float)m.V[k];
+			vals = v32.data();
+		}
+	} catch (const std::bad_alloc &) {
+		return ALP_OUTOFMEM;
 	}
 	return alp_hip_matrix_create_coo(A, m.nrows, m.ncols, m.nnz, m.I.data(), m.J.data(), vals, value_type, flags);
 }

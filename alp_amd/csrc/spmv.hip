@@ -1120,13 +1120,33 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 	                            : A->value_type == ALP_VAL_F32 ? ALP_PLUS_TIMES_F32
 	                                                           : ALP_PLUS_PATTERN_F64;
 	const size_t es = A->value_type == ALP_VAL_F32 ? 4 : 8;
-	void *x = nullptr, *y = nullptr;
-	ALP_HIP_TRY(hipMalloc(&x, es * A->ncols));
-	ALP_HIP_TRY(hipMalloc(&y, es * A->nrows));
+	// every exit path releases the probe vectors and events and, unless a choice
+	// was made, puts back the kernel configuration the matrix had
+	struct Probe {
+		alp_hip_matrix_s *A;
+		int cfg[3];
+		void *x = nullptr, *y = nullptr;
+		hipEvent_t e0 = nullptr, e1 = nullptr;
+		bool keep = false;
+		~Probe() {
+			if (!keep) {
+				A->crs.kernel = A->ccs.kernel = cfg[0];
+				A->crs.lanes = A->ccs.lanes = cfg[1];
+				A->crs.rows = A->ccs.rows = cfg[2];
+			}
+			if (e0) hipEventDestroy(e0);
+			if (e1) hipEventDestroy(e1);
+			if (x) hipFree(x);
+			if (y) hipFree(y);
+		}
+	} pr{A, {A->crs.kernel, A->crs.lanes, A->crs.rows}};
+	ALP_HIP_TRY(hipMalloc(&pr.x, es * A->ncols));
+	ALP_HIP_TRY(hipMalloc(&pr.y, es * A->nrows));
+	void *x = pr.x, *y = pr.y;
 	ALP_HIP_TRY(hipMemsetAsync(x, 0, es * A->ncols, stream()));
-	hipEvent_t e0, e1;
-	ALP_HIP_TRY(hipEventCreate(&e0));
-	ALP_HIP_TRY(hipEventCreate(&e1));
+	ALP_HIP_TRY(hipEventCreate(&pr.e0));
+	ALP_HIP_TRY(hipEventCreate(&pr.e1));
+	hipEvent_t e0 = pr.e0, e1 = pr.e1;
 	struct Cand { int kernel, lanes, rows; };
 	std::vector<Cand> cands;
 	for (int g : {2, 4, 8, 16, 32, 64})
@@ -1166,9 +1186,7 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 	A->crs.kernel = A->ccs.kernel = pick.kernel;
 	A->crs.lanes = A->ccs.lanes = pick.lanes;
 	A->crs.rows = A->ccs.rows = pick.rows;
-	hipEventDestroy(e0);
-	hipEventDestroy(e1);
-	hipFree(x);
-	hipFree(y);
+	pr.keep = true;
+	ALP_HIP_TRY(hipStreamSynchronize(stream()));  // the probe vectors are freed after the last launch
 	return ALP_SUCCESS;
 }

@@ -6,21 +6,25 @@ diagonal 26, off-diagonals -1), stored as device CRS. One step is one
 grb::mxv<dense>(y, A, x) under the FP64 plus-times semiring, i.e. y = y + A x
 (ALP's accumulate semantics, reference/blas2.hpp:1946).
 
-value = SpMV throughput in GB/s of the CRS algorithmic bytes per step (the
-metric's definition, independent of the storage the kernel streams):
-    12 nnz (8 B value + 4 B column index) + 8 (n+1) (row offsets)
-    + 8 n (x, read once) + 16 n (y read + written)  = 2,495,923,112 B.
-The default kernel streams the matrix as index-free diagonal slices, which need
-fewer bytes than CRS. roofline.achieved / roofline.frac count these CRS
-algorithmic bytes over the kernel's event time; the bytes of the layout actually
-streamed (format_bytes_per_launch) give format_achieved_gbs / format_frac, and
-roofline.traffic is the measured HBM traffic per launch. The solve legs (FP64 and mixed-precision CG / HPCG to
-1e-10 relative residual) are reported beside it (`solves`), and `configs` holds
-the other workloads: 50 CG iterations at 64^3, BFS / PageRank / k-hop on an R-MAT
-graph of scale 22, 64 batched systems x 8 right-hand sides, and the
-differentiable solve at 128^3. `cpu_baseline` times ALP's own reference_omp on
-the host: the same SpMV, and (`cpu_baseline.solves`) bounded samples of its CG
-and HPCG. Inputs are resident in HBM before the timed region.
+value = SpMV throughput in GB/s of the algorithmic bytes of the layout the timed
+kernel streams (format_bytes_per_launch) over the wall time per step. The default
+kernel streams the matrix as index-free diagonal slices:
+    8 E (values, E padded slot entries) + 393 S (per slice: a 4 B presence mask per
+    row, 32 slot offsets, a flag, the entry offset) + 8 n (x) + 16 n (y r+w)
+  = 1,731,557,376 B at 192^3.
+roofline.achieved / roofline.frac are the same bytes over the kernel's HIP-event
+time per launch, so frac <= 1 is a fraction of the 8 TB/s HBM peak. The metric's
+CRS algorithmic bytes (12 nnz + 8 (n+1) + 8 n + 16 n = 2,495,923,112 B) over the
+same time are reported separately as crs_equivalent_gbs (they exceed what the
+hardware moves). crs_path times the CSR-stream kernel (kernel 2) forced on the same
+matrix, in CRS bytes. roofline.traffic is the measured HBM traffic per launch. The
+solve legs (FP64 and mixed-precision CG / HPCG to 1e-10 relative residual) are
+reported beside it (`solves`), and `configs` holds the other workloads: 50 CG
+iterations at 64^3, BFS / PageRank / k-hop on an R-MAT graph of scale 22, 64
+batched systems x 8 right-hand sides, and the differentiable solve at 128^3.
+`cpu_baseline` times ALP's own reference_omp on the host: the same SpMV, and
+(`cpu_baseline.solves`) bounded samples of its CG and HPCG. Inputs are resident in
+HBM before the timed region.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
     python bench.py --profile-solve {cg,hpcg,batch} --precision P --iters N   (a short
@@ -408,14 +412,36 @@ def main() -> int:
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / args.steps
     kern_ms = ev.value / args.steps
-    bytes_step = spmv_bytes(n, nnz)
+    crs_bytes = spmv_bytes(n, nnz)
     fbytes = format_bytes(n, nnz, kernel, layout)
-    achieved = bytes_step / (kern_ms * 1e-3) / 1e9
-    format_achieved = fbytes / (kern_ms * 1e-3) / 1e9
-    value = bytes_step / (ms_step * 1e-3) / 1e9
+    achieved = fbytes / (kern_ms * 1e-3) / 1e9
+    crs_equiv = crs_bytes / (kern_ms * 1e-3) / 1e9
+    value = fbytes / (ms_step * 1e-3) / 1e9
     log(f"[bench] mxv: {kern_ms * 1e3:.1f} us/launch (events), {ms_step * 1e3:.1f} us/step (wall), "
-        f"{achieved:.0f} GB/s of CRS algorithmic bytes, {format_achieved:.0f} GB/s of the streamed layout "
-        f"({fbytes} B; slices={layout[0]} entries={layout[1]} diagonal={layout[2]})")
+        f"{achieved:.0f} GB/s of the streamed layout ({fbytes} B; slices={layout[0]} entries={layout[1]} "
+        f"diagonal={layout[2]}), {crs_equiv:.0f} GB/s CRS-equivalent")
+
+    # the CRS kernel any non-stencil matrix gets (CSR-stream), forced on the same matrix
+    crs_path = None
+    if kernel in (4, 5) and not args.kernel:
+        A.set_kernel(2, 0, 0)
+        for _ in range(max(2, args.warmup // 4)):
+            L.alp_hip_mxv(A.h, alp.PLUS_TIMES_F64, alp.DENSE, y.ptr, x.ptr, None)
+        alp.check(L.alp_hip_timer_start(), "timer_start")
+        crs_steps = max(10, args.steps // 4)
+        for _ in range(crs_steps):
+            L.alp_hip_mxv(A.h, alp.PLUS_TIMES_F64, alp.DENSE, y.ptr, x.ptr, None)
+        alp.check(L.alp_hip_timer_stop(ctypes.byref(ev)), "timer_stop")
+        c_ms = ev.value / crs_steps
+        c_gbs = crs_bytes / (c_ms * 1e-3) / 1e9
+        crs_path = {"kernel": "csr-stream", "launches": crs_steps, "us_per_launch": round(c_ms * 1e3, 2),
+                    "bytes_per_launch": crs_bytes, "achieved_gbs": round(c_gbs, 1),
+                    "frac": round(c_gbs / HBM_PEAK_GBS, 4)}
+        tr2 = measured_traffic(2, 0, 0)
+        if tr2 is not None:
+            crs_path["traffic"], crs_path["traffic_source"] = tr2
+        A.set_kernel(kernel, lanes, rows)
+        log(f"[bench] mxv csr-stream forced: {c_ms * 1e3:.1f} us/launch, {c_gbs:.0f} GB/s")
 
     # the overwrite form (set(y,0) folded into the product: y write-only)
     alp.sync()
@@ -439,7 +465,8 @@ def main() -> int:
     if not args.no_solve:
         b = alp.Vector(n, fill=0.0)
         alp.mxv(b, A, alp.Vector(n, fill=1.0))
-        cg_modes = ((0, "fp64"), (1, "mixed")) + (() if args.skip_restarted else ((2, "mixed_restarted"),))
+        cg_modes = ((0, "fp64"), (1, "mixed"), (3, "mixed_groupwise")) + \
+            (() if args.skip_restarted else ((2, "mixed_restarted"),))
         for precision, tag in cg_modes:
             xs = alp.Vector(n, fill=0.0)
             alp.cg(A, b, xs, 2, 1e-10, precision)  # warm-up (work vectors, graph capture)
@@ -458,9 +485,14 @@ def main() -> int:
             log(f"[bench] CG {tag}: rc={rc} it={it} {dt * 1e3:.1f} ms err={err:.2e}")
         if "cg_fp64" in solves and "cg_mixed" in solves:
             solves["cg_speedup_mixed_vs_fp64"] = round(solves["cg_fp64"]["ms"] / solves["cg_mixed"]["ms"], 3)
-            solves["cg_mixed_scheme"] = ("FP32 iterations (FP32 A, r, p, Ap; FP64 row sums; FP64 x, its increments "
-                                         "accumulated in FP32 between updates) with reliable FP64 residual updates; "
+            solves["cg_mixed_scheme"] = ("FP32 iterations (FP32 A, r, p, Ap; FP64 row sums) with the solution update "
+                                         "x += alpha p in FP64 every iteration and reliable FP64 residual updates; "
+                                         "mixed_groupwise = the same with a segment's increments of x accumulated "
+                                         "in FP32 and added to the FP64 x before each FP64 residual; "
                                          "mixed_restarted = FP32 inner solves in FP64 iterative refinement")
+            if "cg_mixed_groupwise" in solves:
+                solves["cg_speedup_mixed_groupwise_vs_fp64"] = round(
+                    solves["cg_fp64"]["ms"] / solves["cg_mixed_groupwise"]["ms"], 3)
         # HPCG: 4 grids (192/96/48/24), 8-colour symmetric Gauss-Seidel, b = A*1, x0 = 0
         log("[bench] building the HPCG hierarchy")
         H = alp.Hpcg(side, side, side, max_levels=3, colours=0)
@@ -518,17 +550,18 @@ def main() -> int:
         "config": {"workload": f"{side}^3 27-point stencil (n={n}, nnz={nnz}), FP64 plus-times "
                                f"grb::mxv y += A x on device-resident storage (CRS and the diagonal-slice copy the kernel streams)",
                    "kernel": KERNEL_NAMES.get(kernel, str(kernel)), "lanes": lanes, "rows": rows,
-                   "bytes_per_step": bytes_step},
+                   "bytes_per_step": fbytes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "note": "achieved counts the CRS algorithmic bytes (12 nnz + 8 (n+1) + 8 n + 16 n); the "
-                             "diagonal-slice layout streams fewer (no column indices or row offsets), so frac "
-                             "can exceed 1; format_frac is the fraction of peak in the layout's own bytes, and "
-                             "traffic the measured HBM bytes"},
-        "algorithmic_bytes_per_launch": bytes_step,
+                     "note": "achieved = the streamed layout's algorithmic bytes per launch "
+                             "(format_bytes_per_launch) / the kernel's HIP-event time per launch; traffic = the "
+                             "measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated)"},
         "format_bytes_per_launch": fbytes,
-        "format_achieved_gbs": round(format_achieved, 1),
-        "format_frac": round(format_achieved / HBM_PEAK_GBS, 4),
+        "crs_bytes_per_launch": crs_bytes,
+        "crs_equivalent_gbs": round(crs_equiv, 1),
+        "crs_equivalent_note": "the metric's CRS bytes (12 nnz + 8 (n+1) + 8 n + 16 n) over the same kernel time; "
+                               "not a fraction of peak, since the layout streams no column indices or row offsets",
+        "crs_path": crs_path,
         "layout": {"slices": layout[0], "padded_entries": layout[1], "diagonal_slices": layout[2]},
         "build_ms": {"stencil_crs_sell_diagonal_f32": round(build_ms, 1),
                      "what": "generator, CRS, sliced ELL, diagonal slices and the FP32 copies, on the device"},

@@ -193,14 +193,17 @@ alp_rc_t alp_hip_convert_f32_to_f64(size_t n, const float *d_x, double *d_y);
  * *residual = ||r||^2 (the reference's convention), *iters = iterations done.
  * Returns ALP_FAILED when not converged, as the reference does.
  * precision: 0 = FP64 throughout; 1 = CG iterations on A's FP32 copy with FP32
- * r, p, Ap (each product row summed in FP64, rounded once) and x kept in FP64
- * (a segment's increments accumulated in FP32 and added to it at each update),
- * with reliable updates: whenever the FP32 recurrence residual has fallen by 100x
- * since the last update, r = b - A x is formed in FP64 and CG continues from it
- * with its search direction kept; convergence is decided on the FP64 residual,
- * and *iters counts the FP32 iterations;
+ * r, p, Ap (each product row summed in FP64, rounded once) and the solution
+ * update x += alpha p done in FP64 on the FP64 x every iteration, with reliable
+ * updates: whenever the FP32 recurrence residual has fallen by 100x since the
+ * last update, r = b - A x is formed in FP64 and CG continues from it with its
+ * search direction kept; convergence is decided on the FP64 residual, and
+ * *iters counts the FP32 iterations;
  * 2 = restarted FP32 inner CG solves (each to 1e-5 relative) inside FP64
- * iterative refinement. Modes 1 and 2 use A's FP32 copy (made on first use). */
+ * iterative refinement;
+ * 3 = as 1, but a segment's increments of x accumulate in FP32 and are added to
+ * the FP64 x before each FP64 residual (a group-wise update: 8 fewer bytes per
+ * row and iteration). Modes 1-3 use A's FP32 copy (made on first use). */
 alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *d_b, double *d_x, size_t max_iter,
                     double tol, int precision, size_t *iters, double *residual);
 

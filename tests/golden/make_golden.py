@@ -6,6 +6,7 @@ must have built ``oracle/_ref/libalpref_seq.so``, ALP's sequential
 
     python tests/golden/make_golden.py              # every fixture
     python tests/golden/make_golden.py transition   # the transition CG group only
+    python tests/golden/make_golden.py cg_converge  # CG to 1e-10 at 64^3 and 192^3 only
 
 (the transition group also needs ``oracle/_ref/libalpref_solver.so``).
 
@@ -132,13 +133,34 @@ def transition_cg(manifest):
               cases=[c[0] for c in cases]), **out)
 
 
+def cg_converge(manifest):
+    """ALP's conjugate_gradient to a 1e-10 relative residual at the workloads'
+    own sizes (x0 = 0, b = A*1, the 27-point stencil): the iteration count, the
+    residual and a probe of x (summary only; x is 2 MB at 64^3)."""
+    R = O.ref("seq")
+    for side in (64, 192):
+        n = side ** 3
+        x0 = np.zeros(n)
+        x = np.zeros(n)
+        it, res = S(), D()
+        rc = R.ref_stencil_cg(side, side, side, P(x0), None, 1000, 1e-10, P(x), ctypes.byref(it),
+                              ctypes.byref(res))
+        probe = np.random.default_rng(99).integers(0, n, 64)
+        save(f"cg_{side}x{side}x{side}_tol1e-10", manifest,
+             dict(kind="cg_summary", sizes=[side] * 3, x0="0", b="A*1", max_iter=1000, tol=1e-10,
+                  probe_rng="numpy default_rng(99).integers(0,n,64)"),
+             rc=np.array([rc]), iterations=np.array([it.value]), residual=np.array([res.value]),
+             x_probe_idx=probe, x_probe=x[probe], x_max_err=np.array([np.max(np.abs(x - 1.0))]))
+        print(f"cg {side}^3: rc={rc} iterations={it.value} residual={res.value:.3e}", flush=True)
+
+
 def main(only=None):
     manifest = {}
     mpath = os.path.join(HERE, "manifest.json")
     if only:
         with open(mpath) as f:
             manifest = json.load(f)
-        {"transition": transition_cg}[only](manifest)
+        {"transition": transition_cg, "cg_converge": cg_converge}[only](manifest)
         with open(mpath, "w") as f:
             json.dump(manifest, f, indent=1, sort_keys=True)
         print(f"updated {only}")
@@ -277,6 +299,7 @@ def main(only=None):
              quality=np.array([q.value]), source=np.array([src]), **knn)
 
     transition_cg(manifest)
+    cg_converge(manifest)
 
     with open(mpath, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)

@@ -8,6 +8,7 @@ Pinned on the CPU in FP64, independently of the device formulas:
     scipy direct solves (every edge of a tiny grid, 32 random edges of a larger);
   * at larger sizes, a directional finite difference through two device solves.
 """
+import ctypes
 import itertools
 
 import numpy as np
@@ -235,3 +236,139 @@ def test_reverse_mode_equals_forward_tangent_at_96_cubed(A, precision):
     A.sync()
     forward = float(((w + xd) * dx).sum())
     assert abs(reverse - forward) <= (1e-8 if precision == 0 else 1e-7) * abs(forward)
+
+
+# ---------------------------------------------------------------------------
+# The differentiable-solve workload's own configuration: 128^3, theta ~
+# lognormal(0, 1), diagonal shift 1e-2, L = ||x - x*||^2 with x* solved from a
+# second seeded theta.
+# ---------------------------------------------------------------------------
+
+C5_SIDE = 128
+C5_SHIFT = 1e-2
+
+
+def _valid_slots(grid, rng, count):
+    """`count` distinct edge slots (vertex * 13 + d) whose neighbour lies inside
+    the grid, drawn uniformly."""
+    nx, ny, nz = grid
+    offs = positive_offsets()
+    out = set()
+    while len(out) < count:
+        i = int(rng.integers(0, nx * ny * nz))
+        d = int(rng.integers(0, 13))
+        x, y, z = i % nx, (i // nx) % ny, i // (nx * ny)
+        dx, dy, dz = offs[d]
+        if 0 <= x + dx < nx and 0 <= y + dy < ny and 0 <= z + dz < nz:
+            out.add(i * 13 + d)
+    return sorted(out)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_c5_config_gradient_against_central_differences(A, precision):
+    """dL/dtheta from the device adjoint (forward CG, transposed CG, dtheta
+    scatter) against central differences with h = 1e-6 on 32 random edges, at
+    the workload's size and coefficient law. Each difference is formed through
+    device FP64 solves without cancellation: x(theta +- h) = x0 + d+-, where
+    A(theta +- h) d+- = b - A(theta +- h) x0 is solved from 0, and
+    L+ - L- = (d+ - d-) . (2 (x0 - x*) + d+ + d-). Bar: 1e-6 relative per edge
+    (each edge's |dL/dtheta_e| floored at 1e-3 of the largest sampled one)."""
+    side, shift, h = C5_SIDE, C5_SHIFT, 1e-6
+    grid = (side, side, side)
+    n = side ** 3
+    L = A.lib()
+    rng = np.random.default_rng(5)
+    theta_h = rng.lognormal(0.0, 1.0, 13 * n)
+    b_h = rng.standard_normal(n)
+    theta = A.Vector.from_numpy(theta_h)
+    b = A.Vector.from_numpy(b_h)
+    tol = 1e-13 if precision == 0 else 1e-12
+    # x* from the reference theta (seed 6), FP64
+    tr = A.Vector.from_numpy(np.random.default_rng(6).lognormal(0.0, 1.0, 13 * n))
+    Aref = A.Matrix.conductance27(*grid, tr, shift)
+    xs = A.Vector(n, fill=0.0)
+    assert A.cg(Aref, b, xs, 100000, 1e-13)[0] == A.SUCCESS
+    Aref.free()
+    tr.free()
+    xstar = xs.numpy()
+    # forward and adjoint at the requested precision
+    M = A.Matrix.conductance27(*grid, theta, shift, A.MAT_WITH_CCS | (A.MAT_WITH_F32 if precision else 0))
+    x = A.Vector(n, fill=0.0)
+    rc, it, _ = A.cg(M, b, x, 100000, tol, precision)
+    assert rc == A.SUCCESS
+    x0 = x.numpy()
+    a = x0 - xstar
+    g = A.Vector.from_numpy(2.0 * a)
+    lam = A.Vector(n, fill=0.0)
+    rc, it2, _ = A.cg(M, g, lam, 100000, tol, precision, transpose=True)
+    assert rc == A.SUCCESS
+    dth = A.Vector(13 * n)
+    A.check(L.alp_hip_grad_conductance27(side, side, side, lam.ptr, x.ptr, dth.ptr), "grad")
+    grad = dth.numpy()
+    M.free()
+    # central differences through FP64 device solves
+    slots = _valid_slots(grid, np.random.default_rng(2025), 32)
+    fd = []
+    for s in slots:
+        d = {}
+        for sign in (1.0, -1.0):
+            v = ctypes.c_double(theta_h[s] + sign * h)
+            A.check(L.alp_hip_memcpy_h2d(theta.ptr + 8 * s, ctypes.byref(v), 8), "patch theta")
+            Mp = A.Matrix.conductance27(*grid, theta, shift)
+            t = A.Vector(n, fill=0.0)
+            A.check(A.mxv(t, Mp, x), "mxv")
+            rhs = A.Vector.from_numpy(b_h - t.numpy())
+            dv = A.Vector(n, fill=0.0)
+            assert A.cg(Mp, rhs, dv, 100000, 1e-13)[0] == A.SUCCESS
+            d[sign] = dv.numpy()
+            Mp.free()
+        v = ctypes.c_double(theta_h[s])
+        A.check(L.alp_hip_memcpy_h2d(theta.ptr + 8 * s, ctypes.byref(v), 8), "restore theta")
+        dp, dm = d[1.0], d[-1.0]
+        fd.append(float(np.dot(dp - dm, 2.0 * a + dp + dm)) / (2.0 * h))
+    fd = np.array(fd)
+    got = grad[slots]
+    floor = 1e-3 * np.max(np.abs(fd))
+    rel = np.abs(got - fd) / np.maximum(np.abs(fd), floor)
+    print(f"C5 precision {precision}: forward {it} / adjoint {it2} iterations; max relative "
+          f"gradient-vs-FD error {rel.max():.2e} over {len(slots)} edges", flush=True)
+    assert np.all(rel <= 1e-6), rel
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_c5_law_gradient_against_scipy_fp64(A, precision):
+    """The workload's coefficient law (theta ~ lognormal(0, 1), shift 1e-2,
+    L = ||x - x*||^2) on a grid scipy solves directly (24^3): the device
+    gradient of every edge against the CPU FP64 adjoint (spsolve forward and
+    transposed)."""
+    import scipy.sparse.linalg as spla
+    grid = (24, 24, 24)
+    n = int(np.prod(grid))
+    E = edge_list(*grid)
+    rng = np.random.default_rng(8)
+    theta_h = rng.lognormal(0.0, 1.0, 13 * n)
+    b_h = rng.standard_normal(n)
+    tref = np.random.default_rng(9).lognormal(0.0, 1.0, 13 * n)
+    K = assemble(theta_h, grid, E, C5_SHIFT).tocsc()
+    xstar = spla.spsolve(assemble(tref, grid, E, C5_SHIFT).tocsc(), b_h)
+    x_cpu = spla.spsolve(K, b_h)
+    lam_cpu = spla.spsolve(K.T.tocsc(), 2.0 * (x_cpu - xstar))
+    g_cpu = np.zeros_like(theta_h)
+    g_cpu[E[:, 0]] = -(lam_cpu[E[:, 1]] - lam_cpu[E[:, 2]]) * (x_cpu[E[:, 1]] - x_cpu[E[:, 2]])
+
+    theta = A.Vector.from_numpy(theta_h)
+    M = A.Matrix.conductance27(*grid, theta, C5_SHIFT, A.MAT_WITH_CCS | (A.MAT_WITH_F32 if precision else 0))
+    b = A.Vector.from_numpy(b_h)
+    x = A.Vector(n, fill=0.0)
+    assert A.cg(M, b, x, 100000, 1e-13, precision)[0] == A.SUCCESS
+    gx = A.Vector.from_numpy(2.0 * (x.numpy() - xstar))
+    lam = A.Vector(n, fill=0.0)
+    assert A.cg(M, gx, lam, 100000, 1e-13, precision, transpose=True)[0] == A.SUCCESS
+    dth = A.Vector(13 * n)
+    A.check(A.lib().alp_hip_grad_conductance27(*grid, lam.ptr, x.ptr, dth.ptr), "grad")
+    g = dth.numpy()
+    err = np.max(np.abs(g - g_cpu)) / np.max(np.abs(g_cpu))
+    print(f"24^3 lognormal theta, precision {precision}: max gradient error {err:.2e}", flush=True)
+    assert err <= 1e-8

@@ -482,6 +482,38 @@ def test_cg_64_fifty_iterations(A, golden):
     assert np.max(np.abs(xp - g["x_probe"])) <= 1e-9
 
 
+@pytest.mark.parametrize("side", [64, 192])
+def test_cg_to_1e10_matches_reference_iteration_count(A, golden, side):
+    """CG to a 1e-10 relative residual at the workloads' sizes (64^3 and the
+    192^3 headline grid; x0 = 0, b = A*1): the FP64 iteration count within 1 of
+    ALP's conjugate_gradient (conjugate_gradient.hpp:378,437) on the same
+    system, the residual below the stopping bound, and x at the probed entries
+    as close to the reference's as the tolerance allows. The mixed mode with
+    reliable updates reaches the same true FP64 residual."""
+    g = golden(f"cg_{side}x{side}x{side}_tol1e-10")
+    M = A.Matrix.stencil27(side, side, side, flags=A.MAT_WITH_F32)
+    n = M.nrows
+    b = A.Vector(n, fill=0.0)
+    A.mxv(b, M, A.Vector(n, fill=1.0))
+    bb = float(A.dot(b, b))
+    x = A.Vector(n, fill=0.0)
+    rc, it, res = A.cg(M, b, x, 1000, 1e-10)
+    want = int(g["iterations"][0])
+    print(f"cg {side}^3: {it} iterations (reference {want})", flush=True)
+    assert rc == A.SUCCESS and int(g["rc"][0]) == 0
+    assert abs(it - want) <= 1
+    assert res < (1e-10 ** 2) * bb
+    xp = x.numpy()[g["x_probe_idx"]]
+    assert np.max(np.abs(xp - g["x_probe"])) <= 1e-7
+    xm = A.Vector(n, fill=0.0)
+    rc, itm, resm = A.cg(M, b, xm, 5000, 1e-10, 1)
+    assert rc == A.SUCCESS
+    r = A.Vector.from_numpy(-b.numpy())
+    A.mxv(r, M, xm)  # r = A x - b
+    assert float(A.dot(r, r)) <= (1.05e-10 ** 2) * bb
+    assert itm <= 1.3 * want, (itm, want)
+
+
 def test_cg_to_1e10_fp64_and_mixed(A):
     M = A.Matrix.stencil27(48, 48, 48, flags=A.MAT_WITH_F32)
     n = M.nrows
@@ -490,7 +522,7 @@ def test_cg_to_1e10_fp64_and_mixed(A):
     bh = b.numpy()
     ref = O.stencil(48, 48, 48)
     its = {}
-    for precision in (0, 1, 2):
+    for precision in (0, 1, 2, 3):
         x = A.Vector(n, fill=0.0)
         rc, it, res = A.cg(M, b, x, 5000, 1e-10, precision)
         assert rc == A.SUCCESS, precision
@@ -503,7 +535,7 @@ def test_cg_to_1e10_fp64_and_mixed(A):
         its[precision] = it
     # reliable updates keep the Krylov space: close to the FP64 iteration count,
     # unlike restarted refinement
-    assert its[1] <= 1.3 * its[0], its
+    assert its[1] <= 1.3 * its[0] and its[3] <= 1.3 * its[0], its
 
 
 def test_cg_mixed_reports_failure_at_the_iteration_limit(A):
@@ -511,7 +543,7 @@ def test_cg_mixed_reports_failure_at_the_iteration_limit(A):
     n = M.nrows
     b = A.Vector(n, fill=0.0)
     A.mxv(b, M, A.Vector(n, fill=1.0))
-    for precision in (1, 2):
+    for precision in (1, 2, 3):
         x = A.Vector(n, fill=0.0)
         rc, it, res = A.cg(M, b, x, 7, 1e-10, precision)
         assert rc == A.FAILED, precision
@@ -519,7 +551,7 @@ def test_cg_mixed_reports_failure_at_the_iteration_limit(A):
         assert np.isfinite(res) and res > 0
 
 
-@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("precision", [0, 1, 2, 3])
 def test_cg_repeated_solves_reuse_cached_state(A, precision):
     """The matrix keeps its CG work vectors and captured iteration graph between
     solves: a repeated solve with the same vectors is bit-identical to the first;

@@ -102,20 +102,62 @@ def test_pagerank_smoke10(A, golden):
     assert A.pagerank(G, 0.85, 1e-30, 5)[0] == A.FAILED  # not converged: FAILED, as the reference
 
 
-def test_rmat_scale22_bfs_and_pagerank_invariants(A):
+def test_rmat_scale22_bfs_and_pagerank_against_oracle(A):
+    """The graph workload's own size: R-MAT scale 22, edge factor 16, seed 1,
+    symmetrised. BFS levels bit-exact with the oracle's queue BFS; PageRank
+    (0.85, 1e-9) within 1e-12 relative of the oracle's restatement of
+    simple_pagerank.hpp:146-533 on the same CRS/CCS, iterations within 1."""
     scale = 22
     n, m = 1 << scale, 16 << scale
     s, d = A.rmat_edges(scale, m, seed=1)
     G = A.graph_from_edges(n, s, d)
+    del s, d
     src, deg = A.max_degree_vertex(G)
     lev, nl, reached = A.bfs(G, src)
     off, idx, _ = G.download(0, with_values=False)
-    want = O.bfs_levels(O.CSR(n, n, off, idx, None), src)
+    crs = O.CSR(n, n, off, idx, None)
+    want = O.bfs_levels(crs, src)
     assert np.array_equal(lev.numpy(), want)
     rc, pr, it, q = A.pagerank(G, 0.85, 1e-9, 1000)
-    p = pr.numpy()
-    assert rc == A.SUCCESS and q <= 1e-9
-    assert abs(p.sum() - 1.0) < 1e-8 and p.min() > 0
+    coff, cidx, _ = G.download(1, with_values=False)
+    ccs = O.CSR(n, n, coff, cidx, None)
+    orc, opr, oit, oq = O.pagerank(crs, ccs, 0.85, 1e-9, 1000)
+    print(f"scale 22: nnz={G.nnz} pagerank it={it} (oracle {oit}) q={q:.3e} (oracle {oq:.3e})", flush=True)
+    assert rc == orc == A.SUCCESS
+    assert abs(it - oit) <= 1
+    assert np.max(np.abs(pr.numpy() - opr)) <= 1e-12 * np.max(opr)
+
+
+def test_bfs_directed_pull_walks_in_neighbours(A):
+    """Directed pattern (no symmetrisation): a pull step must look at each
+    vertex's in-neighbours. Edges 0->1 and 2->0 from source 0: the first step
+    already pulls (the frontier's edges exceed the unexplored ones / 14), and
+    only vertex 1 is one hop away; 2 is unreachable."""
+    src = np.array([0, 2], np.uint32)
+    dst = np.array([1, 0], np.uint32)
+    G = A.graph_from_edges(3, A.Vector.from_numpy(src), A.Vector.from_numpy(dst), flags=0)
+    lev, nl, reached = A.bfs(G, 0)
+    assert list(lev.numpy()) == [0, 1, -1] and nl == 2 and reached == 2
+
+
+@pytest.mark.parametrize("scale,seed", [(12, 4), (16, 9)])
+def test_bfs_directed_rmat_bitexact(A, scale, seed):
+    """Directed R-MAT graphs (self loops dropped, no symmetrisation): levels
+    follow out-edges, bit-exact with the oracle's queue BFS over the CRS, from
+    the highest out-degree vertex (whose frontier forces the pull switch) and
+    from a random one."""
+    n, m = 1 << scale, 16 << scale
+    s, d = A.rmat_edges(scale, m, seed=seed)
+    G = A.graph_from_edges(n, s, d, flags=A.GRAPH_DROP_SELF_LOOPS)
+    goff, gidx, _ = G.download(0, with_values=False)
+    ref = O.CSR(n, n, goff, gidx, None)
+    top, _ = A.max_degree_vertex(G)
+    rng = np.random.default_rng(seed)
+    for source in (top, int(rng.integers(0, n))):
+        lev, nl, reached = A.bfs(G, source)
+        want = O.bfs_levels(ref, source)
+        assert np.array_equal(lev.numpy(), want), source
+        assert reached == int((want >= 0).sum()) and nl == int(want.max()) + 1
 
 
 @pytest.mark.parametrize("scale,seed", [(10, 3), (16, 7)])

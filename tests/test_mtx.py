@@ -136,10 +136,19 @@ def test_unsupported_kinds(tmp_path, banner):
     "%%MatrixMarket matrix coordinate real general\n2 2 1\n1 1 1.0\n2 2 1\n",  # too many
     "%%MatrixMarket matrix coordinate real general\n2 2 1\n1 1\n",          # missing value
     "%%MatrixMarket matrix coordinate real symmetric\n2 3 1\n1 1 1.0\n",    # symmetric, not square
+    "%%MatrixMarket matrix coordinate real general\n2 2 5\n1 1 1.0\n",      # more entries than positions
 ])
 def test_malformed_files_are_rejected(tmp_path, text):
     with pytest.raises(RuntimeError, match="ILLEGAL"):
         A.mtx_read(write(tmp_path, "bad.mtx", text))
+
+
+def test_huge_entry_count_is_rejected_before_reserving(tmp_path):
+    """A header announcing 2^60 entries is refused from the header alone (no
+    reservation of that size, no exception out of the C ABI)."""
+    text = "%%MatrixMarket matrix coordinate real general\n4294967295 4294967295 1152921504606846976\n1 1 1.0\n"
+    with pytest.raises(RuntimeError, match="OVERFLW"):
+        A.mtx_read(write(tmp_path, "huge.mtx", text))
 
 
 def test_missing_file(tmp_path):

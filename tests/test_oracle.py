@@ -178,6 +178,21 @@ def test_cg_64_50_iterations_matches_reference(golden):
     assert hashlib.sha256(x.tobytes()).digest() == bytes(g["x_sha256"])
 
 
+def test_cg_64_to_1e10_matches_reference(golden):
+    """The oracle's CG to a 1e-10 relative residual at 64^3 (x0 = 0, b = A*1):
+    the reference's iteration count, return code and residual exactly, and x at
+    the probed entries bit for bit (the fixture's 192^3 count is checked on the
+    device only)."""
+    g = golden("cg_64x64x64_tol1e-10")
+    A = O.stencil(64, 64, 64)
+    b = O.spmv(A, np.ones(A.nrows))
+    rc, x, it, res, _ = O.cg(A, b, np.zeros(A.nrows), 1000, 1e-10)
+    assert rc == 0 == int(g["rc"][0])
+    assert it == int(g["iterations"][0])
+    assert res == g["residual"][0]
+    assert np.array_equal(x[g["x_probe_idx"]], g["x_probe"])
+
+
 def test_cg_against_direct_solve():
     A = O.stencil(5, 6, 7)
     rng = np.random.default_rng(5)
