@@ -36,6 +36,8 @@ STRUCTURAL = 8
 DENSE = 16
 ADD_IDENTITY = 32
 SAFE_OVERLAP = 1024
+TRANSPOSE_LEFT = 2048
+TRANSPOSE_RIGHT = 4096
 OVERWRITE = 1 << 20
 
 VAL_F64, VAL_F32, VAL_PATTERN = 0, 1, 2
@@ -95,6 +97,12 @@ SIGNATURES = {
                                ctypes.POINTER(_S), _P, _P, _P, ctypes.POINTER(_I)]),
     "alp_hip_matrix_read_mtx": (_RC, [ctypes.POINTER(_P), ctypes.c_char_p, _I, _U]),
     "alp_hip_matrix_layout": (_RC, [_P, _I, ctypes.POINTER(_S), ctypes.POINTER(_S), ctypes.POINTER(_S)]),
+    "alp_hip_get_stream": (_RC, [ctypes.POINTER(_P)]),
+    "alp_hip_matrix_storage": (_RC, [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "alp_hip_matrix_device_bytes": (_RC, [_P, ctypes.POINTER(_S)]),
+    "alp_hip_matrix_create_coo_map": (_RC, [ctypes.POINTER(_P), _S, _S, _S, _P, _P, _U, _P, _P]),
+    "alp_hip_matrix_create_device": (_RC, [ctypes.POINTER(_P), _S, _S, _S, _P, _P, _P, _I, _U, _P]),
+    "alp_hip_mxm": (_RC, [ctypes.POINTER(_P), _P, _P, _I, _U]),
     "alp_hip_mxv": (_RC, [_P, _I, _U, _P, _P, _P]),
     "alp_hip_vxm": (_RC, [_P, _I, _U, _P, _P, _P]),
     "alp_hip_dot_f64": (_RC, [_S, _P, _P, ctypes.POINTER(_D)]),
@@ -351,6 +359,12 @@ class Matrix:
               "matrix_layout")
         return a.value, b.value, c.value
 
+    def device_bytes(self) -> int:
+        """Device memory the matrix holds (CRS, CCS and every derived layout)."""
+        b = _S()
+        check(lib().alp_hip_matrix_device_bytes(self.h, ctypes.byref(b)), "matrix_device_bytes")
+        return b.value
+
     def free(self) -> None:
         if getattr(self, "h", None):
             lib().alp_hip_matrix_destroy(self.h)
@@ -377,6 +391,21 @@ def vxm(y: Vector, x: Vector, A: Matrix, ring: int = PLUS_TIMES_F64, descr: int 
         mask: Vector | None = None) -> int:
     """y (+)= x (x) A (grb::vxm, reference/blas2.hpp:1540-1694); returns the grb::RC code."""
     return lib().alp_hip_vxm(A.h, ring, descr, y.ptr, x.ptr, mask.ptr if mask is not None else None)
+
+
+def mxm(A: Matrix, B: Matrix, ring: int = PLUS_TIMES_F64, descr: int = 0, C: Matrix | None = None) -> Matrix:
+    """C (+)= op(A) (x) op(B) (grb::mxm, reference/blas3.hpp:1148-1725): a new matrix,
+    or, when C is given and holds entries, C updated in place (C's handle is
+    replaced by the result)."""
+    h = _P(C.h if C is not None else None)
+    check(lib().alp_hip_mxm(ctypes.byref(h), A.h, B.h, ring, descr), "mxm")
+    if C is not None:
+        C.h = h.value
+        r, c, z = _S(), _S(), _S()
+        check(lib().alp_hip_matrix_info(C.h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(z)), "matrix_info")
+        C.nrows, C.ncols, C.nnz = r.value, c.value, z.value
+        return C
+    return Matrix(h.value)
 
 
 def dot(x: Vector, y: Vector, z: float = 0.0) -> float:

@@ -275,6 +275,7 @@ struct alp_hip_matrix_s {
 	void *graph_cache = nullptr;
 	void (*free_graph_cache)(void *) = nullptr;
 	bool symmetric_pattern = false;  // built symmetrised (alp_hip_graph_from_edges)
+	bool ccs_ascending = false;      // CCS columns list rows ascending (mxm outputs)
 };
 
 namespace alp {
@@ -286,6 +287,11 @@ alp_rc_t storage_refresh_values(Storage &S, int value_type);
 alp_rc_t storage_ensure_f32(Storage &S, size_t nnz);
 void storage_free(Storage &S);
 alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A);
+alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A, uint64_t *src);
+// a new matrix handle with its sizes set and no storage yet
+alp_rc_t new_matrix(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz, int value_type, unsigned flags);
+// bytes of device memory a storage holds (all its layouts)
+size_t storage_bytes(const Storage &S, int value_type, size_t nnz);
 
 // SpMV entry used by mxv/vxm and by the solvers (one storage, one semiring).
 // skip (optional): a device flag; when it is set at launch time the product is not

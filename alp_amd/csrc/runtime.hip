@@ -588,17 +588,22 @@ __global__ __launch_bounds__(kBlock) void k_histogram(size_t nnz, const uint32_t
 		atomicAdd(reinterpret_cast<unsigned long long *>(counts + major[k]), 1ull);
 }
 
+// Places the stably sorted entries: within a major index in reverse input order
+// (the reference's counting sort, matrix.hpp:1922,1929) or, with forward, in
+// input order; src (optional) gets each position's input index.
 template <typename V>
 __global__ __launch_bounds__(kBlock) void k_place(size_t nnz, const uint32_t *sorted_major,
                                                   const uint32_t *sorted_pos,
                                                   const uint64_t *off, const uint32_t *minor,
-                                                  const V *vals, uint32_t *idx, V *out_vals) {
+                                                  const V *vals, uint32_t *idx, V *out_vals, int forward,
+                                                  uint64_t *src) {
 	for (size_t p = (size_t)blockIdx.x * kBlock + threadIdx.x; p < nnz; p += (size_t)gridDim.x * kBlock) {
 		const uint32_t r = sorted_major[p];
-		const uint64_t dest = off[r] + off[r + 1] - 1 - p;
+		const uint64_t dest = forward ? p : off[r] + off[r + 1] - 1 - p;
 		const uint32_t k = sorted_pos[p];
 		idx[dest] = minor[k];
 		if (vals) out_vals[dest] = vals[k];
+		if (src) src[dest] = k;
 	}
 }
 
@@ -606,10 +611,13 @@ __global__ __launch_bounds__(kBlock) void k_iota(size_t n, uint32_t *x) {
 	grid_for_each<4>(n, [&](size_t i) { x[i] = static_cast<uint32_t>(i); });
 }
 
-// d_major/d_minor/d_vals are device arrays in input order
+// d_major/d_minor/d_vals are device arrays in input order; forward keeps input
+// order inside each major index (else reversed, as the reference's build); src
+// (optional, nnz entries) gets the input index stored at each position
 static alp_rc_t compress_device(Storage &S, size_t nmajor, size_t nminor, size_t nnz,
                                 const uint32_t *d_major, const uint32_t *d_minor,
-                                const void *d_vals, int value_type) {
+                                const void *d_vals, int value_type, bool forward = false,
+                                uint64_t *src = nullptr) {
 	const size_t vs = value_size(value_type);
 	S.nmajor = nmajor;
 	S.nminor = nminor;
@@ -650,18 +658,19 @@ static alp_rc_t compress_device(Storage &S, size_t nmajor, size_t nminor, size_t
 		ALP_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, d_major, sorted_major, pos,
 		                                               sorted_pos, nnz, 0, end_bit, stream()));
 		hipFree(tmp);
+		const int fw = forward ? 1 : 0;
 		if (value_type == ALP_VAL_F64)
 			hipLaunchKernelGGL(k_place<double>, dim3(stream_grid(nnz)), dim3(kBlock), 0, stream(),
 			                   nnz, sorted_major, sorted_pos, S.off, d_minor,
-			                   (const double *)d_vals, S.idx, (double *)S.val);
+			                   (const double *)d_vals, S.idx, (double *)S.val, fw, src);
 		else if (value_type == ALP_VAL_F32)
 			hipLaunchKernelGGL(k_place<float>, dim3(stream_grid(nnz)), dim3(kBlock), 0, stream(),
 			                   nnz, sorted_major, sorted_pos, S.off, d_minor,
-			                   (const float *)d_vals, S.idx, (float *)S.val);
+			                   (const float *)d_vals, S.idx, (float *)S.val, fw, src);
 		else
 			hipLaunchKernelGGL(k_place<double>, dim3(stream_grid(nnz)), dim3(kBlock), 0, stream(),
 			                   nnz, sorted_major, sorted_pos, S.off, d_minor,
-			                   (const double *)nullptr, S.idx, (double *)nullptr);
+			                   (const double *)nullptr, S.idx, (double *)nullptr, fw, src);
 		ALP_LAUNCH_CHECK();
 		ALP_HIP_TRY(hipStreamSynchronize(stream()));
 		hipFree(pos);
@@ -679,7 +688,11 @@ __global__ __launch_bounds__(kBlock) void k_expand_rows(size_t nrows, const uint
 	for (uint64_t k = off[r]; k < off[r + 1]; ++k) rows[k] = static_cast<uint32_t>(r);
 }
 
-alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A) {
+// The CCS from the CRS: each column lists its rows in reverse CRS order (the
+// reference's counting sort fed in CRS order) or, for matrices flagged
+// ccs_ascending (mxm outputs), in ascending row order as mxm_generic fills it.
+// src (optional) gets each CCS entry's CRS position.
+alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A, uint64_t *src) {
 	Storage &C = A->crs;
 	uint32_t *rows = nullptr;
 	ALP_HIP_TRY(hipMalloc(&rows, sizeof(uint32_t) * (A->nnz ? A->nnz : 1)));
@@ -689,11 +702,13 @@ alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A) {
 		ALP_LAUNCH_CHECK();
 	}
 	alp_rc_t rc = compress_device(A->ccs, A->ncols, A->nrows, A->nnz, C.idx, rows, C.val,
-	                              A->value_type);
+	                              A->value_type, A->ccs_ascending, src);
 	hipFree(rows);
 	if (rc) return rc;
 	return storage_finalize(A->ccs, A->value_type, (A->flags & ALP_MAT_WITH_F32) != 0);
 }
+
+alp_rc_t build_ccs_from_crs(alp_hip_matrix_s *A) { return build_ccs_from_crs(A, nullptr); }
 
 // ---- 27-point stencil on the device ----------------------------------------
 //
@@ -946,8 +961,30 @@ ALP_EXPORT alp_rc_t alp_hip_convert_f32_to_f64(size_t n, const float *x, double 
 
 // ---- matrices ----
 
-static alp_rc_t new_matrix(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz,
-                           int value_type, unsigned flags) {
+namespace alp {
+
+size_t storage_bytes(const Storage &S, int value_type, size_t nnz) {
+	if (!S.off) return 0;
+	const size_t vs = value_size(value_type);
+	size_t b = sizeof(uint64_t) * (S.nmajor + 1) + sizeof(uint32_t) * nnz;
+	if (S.val) b += vs * nnz;
+	if (S.val32) b += sizeof(float) * nnz;
+	if (S.row_blocks) b += sizeof(uint32_t) * (S.n_row_blocks + 1) + sizeof(uint64_t) * (S.n_row_blocks + 1);
+	if (S.row_blocks_l) b += sizeof(uint32_t) * (S.n_row_blocks_l + 1) + sizeof(uint64_t) * (S.n_row_blocks_l + 1);
+	if (S.row_blocks_w) b += sizeof(uint32_t) * (S.n_row_blocks_w + 1) + sizeof(uint64_t) * (S.n_row_blocks_w + 1);
+	if (S.sell_off) b += sizeof(uint64_t) * (S.n_slices + 1);
+	if (S.sell_idx) b += sizeof(uint32_t) * S.sell_entries;
+	if (S.sell_val) b += vs * S.sell_entries;
+	if (S.sell_val32) b += sizeof(float) * S.sell_entries;
+	if (S.dia.flag) b += S.n_slices + sizeof(int32_t) * kDiaSlots * S.n_slices + sizeof(uint32_t) * S.n_slices * kSellRows;
+	if (S.dia.val) b += vs * S.sell_entries;
+	if (S.dia.val32) b += sizeof(float) * S.sell_entries;
+	b += S.n_long * 4 + (S.n_long + 1) * 8 + S.n_chunks * (4 + 8 + 8);
+	return b;
+}
+
+alp_rc_t new_matrix(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz,
+                    int value_type, unsigned flags) {
 	if (!A) return ALP_ILLEGAL;
 	if (value_type < 0 || value_type > 2) return ALP_ILLEGAL;
 	if (nrows > 0xFFFFFFFFull || ncols > 0xFFFFFFFFull) return ALP_OVERFLW;
@@ -963,6 +1000,8 @@ static alp_rc_t new_matrix(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size
 	*A = M;
 	return ALP_SUCCESS;
 }
+
+}  // namespace alp
 
 static alp_rc_t finish_matrix(alp_hip_matrix_t M) {
 	ALP_TRY(storage_finalize(M->crs, M->value_type, (M->flags & ALP_MAT_WITH_F32) != 0));
@@ -1144,4 +1183,120 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_download(alp_hip_matrix_t A, int which, uint6
 	const size_t vs = value_size(A->value_type);
 	if (h_val && vs && A->nnz) ALP_HIP_TRY(copy_sync(h_val, S.val, vs * A->nnz, hipMemcpyDeviceToHost));
 	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_get_stream(void **s) {
+	ALP_TRY(ensure_init());
+	if (!s) return ALP_ILLEGAL;
+	*s = (void *)stream();
+	return ALP_SUCCESS;
+}
+
+// internal::getCRS / getCCS (reference/matrix.hpp:72-90): the device arrays of
+// one orientation; the CCS is built on first request
+ALP_EXPORT alp_rc_t alp_hip_matrix_storage(alp_hip_matrix_t A, int which, const uint64_t **d_off,
+                                           const uint32_t **d_idx, const void **d_val) {
+	ALP_TRY(ensure_init());
+	if (!A || (which != 0 && which != 1)) return ALP_ILLEGAL;
+	if (which == 1 && !A->ccs.off) {
+		ALP_TRY(build_ccs_from_crs(A));
+		ALP_HIP_TRY(hipStreamSynchronize(stream()));
+	}
+	const Storage &S = which == 0 ? A->crs : A->ccs;
+	if (d_off) *d_off = S.off;
+	if (d_idx) *d_idx = S.idx;
+	if (d_val) *d_val = S.val;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_matrix_device_bytes(alp_hip_matrix_t A, size_t *bytes) {
+	if (!A || !bytes) return ALP_ILLEGAL;
+	*bytes = storage_bytes(A->crs, A->value_type, A->nnz) + storage_bytes(A->ccs, A->value_type, A->nnz);
+	return ALP_SUCCESS;
+}
+
+// buildMatrixUnique of a pattern, with each stored position's input index
+// (reference/matrix.hpp:1854-1951 places the same entries in the same order)
+ALP_EXPORT alp_rc_t alp_hip_matrix_create_coo_map(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz,
+                                                  const uint32_t *h_I, const uint32_t *h_J, unsigned flags,
+                                                  uint64_t *d_crs_src, uint64_t *d_ccs_src) {
+	ALP_TRY(ensure_init());
+	if (!A) return ALP_ILLEGAL;
+	*A = nullptr;
+	if (nnz && (!h_I || !h_J)) return ALP_ILLEGAL;
+	if (nnz >= 0xFFFFFFFFull) return ALP_OVERFLW;
+	for (size_t k = 0; k < nnz; ++k)
+		if (h_I[k] >= nrows || h_J[k] >= ncols) return ALP_MISMATCH;
+	ALP_TRY(new_matrix(A, nrows, ncols, nnz, ALP_VAL_PATTERN, flags & ALP_MAT_WITH_CCS));
+	alp_hip_matrix_t M = *A;
+	uint32_t *dI = nullptr, *dJ = nullptr;
+	alp_rc_t rc = ALP_SUCCESS;
+	do {
+		if (hipMalloc(&dI, sizeof(uint32_t) * (nnz ? nnz : 1)) || hipMalloc(&dJ, sizeof(uint32_t) * (nnz ? nnz : 1))) {
+			rc = ALP_OUTOFMEM;
+			break;
+		}
+		if (nnz && (copy_sync(dI, h_I, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice) ||
+		            copy_sync(dJ, h_J, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice))) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if ((rc = compress_device(M->crs, nrows, ncols, nnz, dI, dJ, nullptr, ALP_VAL_PATTERN, false, d_crs_src)))
+			break;
+		if ((rc = storage_finalize(M->crs, ALP_VAL_PATTERN, false))) break;
+		if (flags & ALP_MAT_WITH_CCS) {
+			if ((rc = compress_device(M->ccs, ncols, nrows, nnz, dJ, dI, nullptr, ALP_VAL_PATTERN, false, d_ccs_src)))
+				break;
+			rc = storage_finalize(M->ccs, ALP_VAL_PATTERN, false);
+		}
+	} while (0);
+	hipStreamSynchronize(stream());
+	if (dI) hipFree(dI);
+	if (dJ) hipFree(dJ);
+	if (rc) {
+		alp_hip_matrix_destroy(M);
+		*A = nullptr;
+	}
+	return rc;
+}
+
+// A matrix holding a copy of device CRS arrays (an mxm result); its CCS lists
+// each column's rows ascending, as mxm_generic fills it (reference/blas3.hpp:
+// 1622-1636), and d_ccs_src gets each CCS entry's CRS position
+ALP_EXPORT alp_rc_t alp_hip_matrix_create_device(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz,
+                                                 const uint64_t *d_off, const uint32_t *d_idx, const void *d_val,
+                                                 int value_type, unsigned flags, uint64_t *d_ccs_src) {
+	ALP_TRY(ensure_init());
+	if (!A) return ALP_ILLEGAL;
+	*A = nullptr;
+	if (!d_off || (nnz && !d_idx) || (nnz && value_type != ALP_VAL_PATTERN && !d_val)) return ALP_ILLEGAL;
+	if (nnz >= 0xFFFFFFFFull) return ALP_OVERFLW;
+	ALP_TRY(new_matrix(A, nrows, ncols, nnz, value_type, flags));
+	alp_hip_matrix_t M = *A;
+	M->ccs_ascending = true;
+	const size_t vs = value_size(value_type);
+	alp_rc_t rc = ALP_SUCCESS;
+	do {
+		if (hipMalloc(&M->crs.off, sizeof(uint64_t) * (nrows + 1)) != hipSuccess ||
+		    hipMalloc(&M->crs.idx, sizeof(uint32_t) * (nnz ? nnz : 1)) != hipSuccess ||
+		    (vs && hipMalloc(&M->crs.val, vs * (nnz ? nnz : 1)) != hipSuccess)) {
+			rc = ALP_OUTOFMEM;
+			break;
+		}
+		hipStream_t s = stream();
+		if (hipMemcpyAsync(M->crs.off, d_off, sizeof(uint64_t) * (nrows + 1), hipMemcpyDeviceToDevice, s) ||
+		    (nnz && hipMemcpyAsync(M->crs.idx, d_idx, sizeof(uint32_t) * nnz, hipMemcpyDeviceToDevice, s)) ||
+		    (nnz && vs && hipMemcpyAsync(M->crs.val, d_val, vs * nnz, hipMemcpyDeviceToDevice, s))) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if ((rc = storage_finalize(M->crs, value_type, (flags & ALP_MAT_WITH_F32) != 0))) break;
+		if (flags & ALP_MAT_WITH_CCS) rc = build_ccs_from_crs(M, d_ccs_src);
+	} while (0);
+	hipStreamSynchronize(stream());
+	if (rc) {
+		alp_hip_matrix_destroy(M);
+		*A = nullptr;
+	}
+	return rc;
 }

@@ -60,6 +60,8 @@ typedef enum {
 #define ALP_DESC_DENSE 16u
 #define ALP_DESC_ADD_IDENTITY 32u
 #define ALP_DESC_SAFE_OVERLAP 1024u
+#define ALP_DESC_TRANSPOSE_LEFT 2048u
+#define ALP_DESC_TRANSPOSE_RIGHT 4096u
 /* library-only bit: the output is known to be all zero (a grb::set(y,0) folded into
  * the product), so y is written without being read */
 #define ALP_DESC_OVERWRITE (1u << 20)
@@ -155,6 +157,48 @@ alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A);
  * reference stores CRS/CCS only; this reports what the kernels stream) */
 alp_rc_t alp_hip_matrix_layout(alp_hip_matrix_t A, int which, size_t *slices, size_t *entries,
                                size_t *dia_slices);
+
+/* ---- device storage, for callers that run kernels of their own on it ----
+ * (the grb:: header layer instantiates its generic semiring kernels this way) */
+/* the library's stream (work enqueued there is ordered with every library call) */
+alp_rc_t alp_hip_get_stream(void **hip_stream);
+/* device arrays of A's CRS (which = 0) or CCS (which = 1, built on first request):
+ * internal::getCRS / getCCS (reference/matrix.hpp:72-90); *d_val is NULL for a pattern
+ * matrix. The pointers stay valid until A is destroyed. */
+alp_rc_t alp_hip_matrix_storage(alp_hip_matrix_t A, int which, const uint64_t **d_off,
+                                const uint32_t **d_idx, const void **d_val);
+/* bytes of device memory A holds: CRS, CCS and every derived layout (sliced ELL,
+ * diagonal slices, FP32 copies, tile metadata) */
+alp_rc_t alp_hip_matrix_device_bytes(alp_hip_matrix_t A, size_t *bytes);
+/* buildMatrixUnique of a pattern matrix from host coordinates (as
+ * alp_hip_matrix_create_coo, reference/matrix.hpp:1854-1951) that also reports where
+ * each input entry went: d_crs_src[p] = the input index stored at CRS position p, and,
+ * with ALP_MAT_WITH_CCS, d_ccs_src[p] likewise for the CCS (device arrays of nnz
+ * entries; either may be NULL). A caller keeps values of any type in that order. */
+alp_rc_t alp_hip_matrix_create_coo_map(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz,
+                                       const uint32_t *h_I, const uint32_t *h_J, unsigned flags,
+                                       uint64_t *d_crs_src, uint64_t *d_ccs_src);
+/* a matrix holding a copy of device CRS arrays (d_val NULL for a pattern); with
+ * ALP_MAT_WITH_CCS its CCS lists each column's rows in ascending order, as
+ * mxm_generic fills an output (reference/blas3.hpp:1622-1636), and d_ccs_src (may be
+ * NULL) gets each CCS entry's CRS position */
+alp_rc_t alp_hip_matrix_create_device(alp_hip_matrix_t *A, size_t nrows, size_t ncols, size_t nnz,
+                                      const uint64_t *d_off, const uint32_t *d_idx, const void *d_val,
+                                      int value_type, unsigned flags, uint64_t *d_ccs_src);
+
+/* ---- level-3: mxm (reference/blas3.hpp:1148 mxm_generic, :1744 mxm) ----
+ * *C = *C (+) op(A) (x) op(B), op = transpose with ALP_DESC_TRANSPOSE_LEFT / _RIGHT.
+ * When *C is NULL or holds no entries the product is written to a new matrix; when
+ * it holds entries they are updated in place (their positions come first in each
+ * row, products fold into them), and *C is replaced by the result. Rows list their
+ * columns in the order the reference's SPA first reaches them; each entry folds its
+ * products in the reference's order, so results are the reference's bit for bit.
+ * Semirings: ALP_PLUS_TIMES_F64 and ALP_MIN_PLUS_F64 (double or pattern operands,
+ * a pattern entry being the multiplicative identity), ALP_PLUS_TIMES_F32 (float or
+ * pattern), ALP_LOR_LAND_BOOL (pattern operands, pattern result). The result has
+ * a CCS when A or B had one (ALP_MAT_WITH_CCS). */
+alp_rc_t alp_hip_mxm(alp_hip_matrix_t *C, alp_hip_matrix_t A, alp_hip_matrix_t B, alp_semiring_t ring,
+                     unsigned descr);
 
 /* ---- level-2: mxv / vxm (reference/blas2.hpp:870-2200) ----
  * y (+)= A (x) x  (mxv; ALP_DESC_TRANSPOSE_MATRIX gathers over the CCS instead)

@@ -75,6 +75,7 @@ def lib() -> ctypes.CDLL:
         L.orc_rmat_edges.argtypes = [ctypes.c_uint, _S, _D, _D, _D, _U64, _P, _P]
         L.orc_pcg_f64.argtypes = [_S, _P, _P, _P, _P, _P, _S, _D, ctypes.c_void_p, _P, _P, _P]
         L.orc_pcg_f32.argtypes = [_S, _P, _P, _P, _P, _P, _S, ctypes.c_float, ctypes.c_void_p, _P, _P, _P]
+        L.orc_mxm.argtypes = [_S, _S, _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int, _P, _P, _P, _S, _P]
         L.orc_mix64.restype = _U64
         L.orc_mix64.argtypes = [_U64]
         _lib = L
@@ -248,6 +249,33 @@ def pagerank(crs: CSR, ccs: CSR, alpha=0.85, conv=1e-7, max_iter=1000):
     return rc, pr, it.value, q.value
 
 
+def mxm(A: CSR, B: CSR, ring: int = 0, C: CSR | None = None) -> CSR:
+    """C (+)= A (x) B in mxm_generic's order (reference/blas3.hpp:1148-1725): ring 0
+    plus-times, 1 min-plus, 2 lor-land (pattern result). A None value array is a
+    pattern operand (entries = the multiplicative identity). Pass transposes as
+    their CCS (the rows of A^T)."""
+    assert A.ncols == B.nrows
+    m, n = A.nrows, B.ncols
+    ca = C is not None and C.nnz > 0
+    # products per row bound the output
+    bl = np.diff(B.off.astype(np.int64))
+    cs = np.concatenate([[0], np.cumsum(bl[A.idx[:A.nnz].astype(np.int64)])])
+    cap = int(cs[-1]) + (C.nnz if ca else 0) + 1
+    off = np.zeros(m + 1, np.uint64)
+    idx = np.zeros(cap, np.uint32)
+    val = np.zeros(cap, np.float64)
+    nz = _S()
+    L = lib()
+    rc = L.orc_mxm(m, n, _ptr(A.off), _ptr(A.idx), None if A.val is None else _ptr(A.val),
+                   _ptr(B.off), _ptr(B.idx), None if B.val is None else _ptr(B.val),
+                   _ptr(C.off) if ca else None, _ptr(C.idx) if ca else None,
+                   _ptr(C.val) if ca and C.val is not None else None, ring,
+                   _ptr(off), _ptr(idx), _ptr(val), cap, ctypes.byref(nz))
+    assert rc == 0, rc
+    k = nz.value
+    return CSR(m, n, off, idx[:k], None if ring == 2 else val[:k])
+
+
 def bfs_levels(A: CSR, source: int) -> np.ndarray:
     lev = np.zeros(A.nrows, np.int32)
     lib().orc_bfs_levels(A.nrows, _ptr(A.off), _ptr(A.idx), source, _ptr(lev))
@@ -361,6 +389,8 @@ def ref(flavour: str = "seq") -> ctypes.CDLL:
         R.ref_hpcg_levels.argtypes = [_S, _S, _S, _S, _P]
         R.ref_pagerank.argtypes = [_S, _S, _P, _P, _D, _D, _S, _P, _P, _P]
         R.ref_knn.argtypes = [_S, _S, _P, _P, _S, _S, _P]
+        R.ref_mxm.argtypes = [_S, _S, _S, _S, _P, _P, _P, _S, _P, _P, _P, _S, _P, _P, _P, ctypes.c_int,
+                              ctypes.c_int, _P, _P, _P, _P, _P, _P, _S, _P]
         R.ref_time_stencil_mxv.restype = _D
         R.ref_time_stencil_mxv.argtypes = [_S, _S, _S, _S]
         R.ref_time_stencil_cg.restype = _D

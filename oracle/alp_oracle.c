@@ -27,6 +27,10 @@
  *                          of tests/smoke/smoketests.sh:154.
  *   orc_khop               reference knn fixtures.
  *   orc_bfs_levels         brute-force level sets by repeated boolean products.
+ *   orc_mxm                reference grb::mxm fixtures (plus-times, min-plus,
+ *                          pattern operands, in-place updates, transposes): CRS
+ *                          order and values bit-exact; a dense brute-force
+ *                          product; the shifted identities of tests/unit/mxm.cpp.
  *   orc_rmat_edges         no reference counterpart (ALP has no R-MAT generator):
  *                          checked for range, determinism and quadrant frequencies.
  */
@@ -748,4 +752,75 @@ void orc_rmat_edges(unsigned scale, size_t nedges, double a, double b, double c,
 		src[e] = s;
 		dst[e] = d;
 	}
+}
+
+/*
+ * mxm_generic (reference/blas3.hpp:1148-1725), sequential: row i starts its SPA
+ * with C_in's row i (in-place update, :1512-1546: the existing entries first, in
+ * their order, with their values), then for A(i,k) in storage order and B(k,j)
+ * in storage order: the first product reaching column j sets c_ij = a (x) b,
+ * later ones fold c_ij = c_ij (+) (a (x) b) (:1582-1604); the row is written in
+ * the SPA's stack order (:1615-1640). ring: 0 plus-times, 1 min-plus (values),
+ * 2 lor-land (patterns: aval, bval, cval_in may be NULL, cval is not written).
+ * A NULL aval / bval means a pattern operand whose entries are the
+ * multiplicative identity (1 for plus-times, 0 for min-plus). Returns 0, or -1
+ * when cap is too small or memory runs out; *nnz_out = entries of C.
+ */
+int orc_mxm(size_t m, size_t n, const uint64_t *aoff, const uint32_t *aidx, const double *aval,
+            const uint64_t *boff, const uint32_t *bidx, const double *bval, const uint64_t *cin_off,
+            const uint32_t *cin_idx, const double *cin_val, int ring, uint64_t *coff, uint32_t *cidx,
+            double *cval, size_t cap, size_t *nnz_out) {
+	unsigned char *assigned = calloc(n ? n : 1, 1);
+	uint32_t *stack = malloc(sizeof(uint32_t) * (n ? n : 1));
+	double *buf = malloc(sizeof(double) * (n ? n : 1));
+	if (!assigned || !stack || !buf) { free(assigned); free(stack); free(buf); return -1; }
+	const double one = ring == 1 ? 0.0 : 1.0;
+	size_t nz = 0;
+	int rc = 0;
+	coff[0] = 0;
+	for (size_t i = 0; i < m && rc == 0; ++i) {
+		size_t top = 0;
+		if (cin_off) {
+			for (uint64_t q = cin_off[i]; q < cin_off[i + 1]; ++q) {
+				const uint32_t j = cin_idx[q];
+				if (!assigned[j]) {
+					assigned[j] = 1;
+					stack[top++] = j;
+					buf[j] = cin_val ? cin_val[q] : 1.0;
+				}
+			}
+		}
+		for (uint64_t q = aoff[i]; q < aoff[i + 1]; ++q) {
+			const uint32_t k = aidx[q];
+			const double a = aval ? aval[q] : one;
+			for (uint64_t l = boff[k]; l < boff[k + 1]; ++l) {
+				const uint32_t j = bidx[l];
+				const double b = bval ? bval[l] : one;
+				const double p = ring == 1 ? a + b : a * b;
+				if (!assigned[j]) {
+					assigned[j] = 1;
+					stack[top++] = j;
+					buf[j] = p;
+				} else if (ring == 0) {
+					buf[j] = buf[j] + p;
+				} else if (ring == 1) {
+					if (p < buf[j]) buf[j] = p;
+				}
+			}
+		}
+		if (nz + top > cap) { rc = -1; break; }
+		for (size_t t = 0; t < top; ++t) {
+			const uint32_t j = stack[t];
+			cidx[nz] = j;
+			if (cval && ring != 2) cval[nz] = buf[j];
+			assigned[j] = 0;
+			++nz;
+		}
+		coff[i + 1] = nz;
+	}
+	free(assigned);
+	free(stack);
+	free(buf);
+	*nnz_out = nz;
+	return rc;
 }

@@ -85,6 +85,15 @@ void orc_bfs_levels(size_t n, const uint64_t *off, const uint32_t *idx, uint32_t
 void orc_khop(size_t n, const uint64_t *off, const uint32_t *idx, uint32_t source, size_t k,
               uint8_t *out);
 
+/* ---- level 3 ---------------------------------------------------------------- */
+/* C = C_in (+) A (x) B in mxm_generic's order (ring: 0 plus-times, 1 min-plus,
+ * 2 lor-land on patterns); C_in may be absent (cin_off NULL); returns -1 when
+ * cap entries do not suffice */
+int orc_mxm(size_t m, size_t n, const uint64_t *aoff, const uint32_t *aidx, const double *aval,
+            const uint64_t *boff, const uint32_t *bidx, const double *bval, const uint64_t *cin_off,
+            const uint32_t *cin_idx, const double *cin_val, int ring, uint64_t *coff, uint32_t *cidx,
+            double *cval, size_t cap, size_t *nnz_out);
+
 /* ---- generators with no counterpart in the reference ------------------------ */
 void orc_rmat_edges(unsigned scale, size_t nedges, double a, double b, double c,
                     uint64_t seed, uint32_t *src, uint32_t *dst);

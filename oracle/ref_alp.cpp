@@ -19,6 +19,7 @@
  *   - hpcg                          include/graphblas/algorithms/hpcg/hpcg.hpp:97
  *   - simple_pagerank               include/graphblas/algorithms/simple_pagerank.hpp:146
  *   - knn                           include/graphblas/algorithms/knn.hpp:82
+ *   - grb::mxm                      include/graphblas/reference/blas3.hpp:1744
  */
 #include <graphblas.hpp>
 #include <graphblas/algorithms/conjugate_gradient.hpp>
@@ -282,6 +283,115 @@ REF_API int ref_knn( size_t n, size_t nedges, const uint32_t * src, const uint32
 	std::fill( out, out + n, 0 );
 	for( const auto &pair : u ) { out[ pair.first ] = pair.second ? 1 : 2; }
 	return rc;
+}
+
+namespace {
+
+template< typename T >
+int build_coo( grb::Matrix< T > &M, size_t nz, const uint32_t * I, const uint32_t * J, const double * V ) {
+	std::vector< size_t > i( I, I + nz ), j( J, J + nz );
+	std::vector< T > v( V, V + nz );
+	return static_cast< int >( grb::buildMatrixUnique( M, i.data(), j.data(), v.data(), nz, grb::SEQUENTIAL ) );
+}
+
+int build_coo( grb::Matrix< void > &M, size_t nz, const uint32_t * I, const uint32_t * J, const double * ) {
+	std::vector< size_t > i( I, I + nz ), j( J, J + nz );
+	return static_cast< int >( grb::buildMatrixUnique( M, i.data(), j.data(), nz, grb::SEQUENTIAL ) );
+}
+
+template< typename T >
+void dump_storage( const grb::Matrix< T > &C, size_t m, size_t n, uint64_t * crs_off, uint32_t * crs_idx,
+	double * crs_val, uint64_t * ccs_off, uint32_t * ccs_idx, double * ccs_val
+) {
+	const size_t nz = grb::nnz( C );
+	const auto &R = grb::internal::getCRS( C );
+	const auto &K = grb::internal::getCCS( C );
+	for( size_t i = 0; i <= m; ++i ) { crs_off[ i ] = R.col_start[ i ]; }
+	for( size_t j = 0; j <= n; ++j ) { ccs_off[ j ] = K.col_start[ j ]; }
+	for( size_t k = 0; k < nz; ++k ) {
+		crs_idx[ k ] = R.row_index[ k ];
+		ccs_idx[ k ] = K.row_index[ k ];
+		crs_val[ k ] = R.getValue( k, 1.0 );
+		ccs_val[ k ] = K.getValue( k, 1.0 );
+	}
+}
+
+template< typename T, class Ring >
+int run_mxm( size_t m, size_t k, size_t n, size_t nA, const uint32_t * IA, const uint32_t * JA,
+	const double * VA, size_t nB, const uint32_t * IB, const uint32_t * JB, const double * VB, size_t nC,
+	const uint32_t * IC, const uint32_t * JC, const double * VC, int transpose,
+	uint64_t * crs_off, uint32_t * crs_idx, double * crs_val, uint64_t * ccs_off, uint32_t * ccs_idx,
+	double * ccs_val, size_t cap, size_t * nnz_out
+) {
+	const size_t ar = ( transpose & 1 ) ? k : m, ac = ( transpose & 1 ) ? m : k;
+	const size_t br = ( transpose & 2 ) ? n : k, bc = ( transpose & 2 ) ? k : n;
+	grb::Matrix< T > A( ar, ac ), B( br, bc ), C( m, n );
+	int rc = build_coo( A, nA, IA, JA, VA );
+	rc = rc ? rc : build_coo( B, nB, IB, JB, VB );
+	if( !rc && nC ) { rc = build_coo( C, nC, IC, JC, VC ); }
+	if( rc ) { return rc; }
+	Ring ring;
+	grb::RC grc;
+	if( transpose == 0 ) {
+		grc = grb::mxm( C, A, B, ring, grb::RESIZE );
+		grc = grc ? grc : grb::mxm( C, A, B, ring );
+	} else if( transpose == 1 ) {
+		grc = grb::mxm< grb::descriptors::transpose_left >( C, A, B, ring, grb::RESIZE );
+		grc = grc ? grc : grb::mxm< grb::descriptors::transpose_left >( C, A, B, ring );
+	} else {
+		grc = grb::mxm< grb::descriptors::transpose_right >( C, A, B, ring, grb::RESIZE );
+		grc = grc ? grc : grb::mxm< grb::descriptors::transpose_right >( C, A, B, ring );
+	}
+	if( grc ) { return static_cast< int >( grc ); }
+	*nnz_out = grb::nnz( C );
+	if( *nnz_out > cap ) { return -2; }
+	dump_storage( C, m, n, crs_off, crs_idx, crs_val, ccs_off, ccs_idx, ccs_val );
+	return 0;
+}
+
+} // namespace
+
+/**
+ * grb::mxm (reference/blas3.hpp:1744) of matrices built by buildMatrixUnique from
+ * the given coordinates (C_in too when nC > 0: the in-place update), after a
+ * RESIZE call; dumps C's CRS and CCS. ring 0: plus-times over double, 1: min-plus
+ * over double, 2: plus-times over pattern (void) operands. transpose: 0 none,
+ * 1 transpose_left, 2 transpose_right (A and B are given in stored orientation).
+ */
+REF_API int ref_mxm( size_t m, size_t k, size_t n, size_t nA, const uint32_t * IA, const uint32_t * JA,
+	const double * VA, size_t nB, const uint32_t * IB, const uint32_t * JB, const double * VB, size_t nC,
+	const uint32_t * IC, const uint32_t * JC, const double * VC, int ring, int transpose,
+	uint64_t * crs_off, uint32_t * crs_idx, double * crs_val, uint64_t * ccs_off, uint32_t * ccs_idx,
+	double * ccs_val, size_t cap, size_t * nnz_out
+) {
+	if( ensure_init() ) { return -1; }
+	typedef grb::Semiring< grb::operators::min< double >, grb::operators::add< double >,
+		grb::identities::infinity, grb::identities::zero > MinPlus;
+	if( ring == 0 ) {
+		return run_mxm< double, RealRing >( m, k, n, nA, IA, JA, VA, nB, IB, JB, VB, nC, IC, JC, VC,
+			transpose, crs_off, crs_idx, crs_val, ccs_off, ccs_idx, ccs_val, cap, nnz_out );
+	}
+	if( ring == 1 ) {
+		return run_mxm< double, MinPlus >( m, k, n, nA, IA, JA, VA, nB, IB, JB, VB, nC, IC, JC, VC,
+			transpose, crs_off, crs_idx, crs_val, ccs_off, ccs_idx, ccs_val, cap, nnz_out );
+	}
+	// pattern operands: A and B void, C double (each product is 1 * 1)
+	const size_t ar = ( transpose & 1 ) ? k : m, ac = ( transpose & 1 ) ? m : k;
+	const size_t br = ( transpose & 2 ) ? n : k, bc = ( transpose & 2 ) ? k : n;
+	grb::Matrix< void > A( ar, ac ), B( br, bc );
+	grb::Matrix< double > C( m, n );
+	int rc = build_coo( A, nA, IA, JA, nullptr );
+	rc = rc ? rc : build_coo( B, nB, IB, JB, nullptr );
+	if( !rc && nC ) { rc = build_coo( C, nC, IC, JC, VC ); }
+	if( rc ) { return rc; }
+	RealRing rr;
+	grb::RC grc = grb::mxm( C, A, B, rr, grb::RESIZE );
+	grc = grc ? grc : grb::mxm( C, A, B, rr );
+	if( grc ) { return static_cast< int >( grc ); }
+	*nnz_out = grb::nnz( C );
+	if( *nnz_out > cap ) { return -2; }
+	dump_storage( C, m, n, crs_off, crs_idx, crs_val, ccs_off, ccs_idx, ccs_val );
+	return 0;
 }
 
 /**

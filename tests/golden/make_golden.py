@@ -7,6 +7,7 @@ must have built ``oracle/_ref/libalpref_seq.so``, ALP's sequential
     python tests/golden/make_golden.py              # every fixture
     python tests/golden/make_golden.py transition   # the transition CG group only
     python tests/golden/make_golden.py cg_converge  # CG to 1e-10 at 64^3 and 192^3 only
+    python tests/golden/make_golden.py mxm          # grb::mxm cases only
 
 (the transition group also needs ``oracle/_ref/libalpref_solver.so``).
 
@@ -154,13 +155,84 @@ def cg_converge(manifest):
         print(f"cg {side}^3: rc={rc} iterations={it.value} residual={res.value:.3e}", flush=True)
 
 
+def random_coo(rows, cols, per_row, rng, values=True):
+    """Distinct random coordinates (about per_row per row, some rows empty) in a
+    random input order, with N(0,1) values."""
+    cells = set()
+    for i in range(rows):
+        if rng.random() < 0.1:
+            continue
+        for j in rng.integers(0, cols, int(rng.integers(1, 2 * per_row))):
+            cells.add((i, int(j)))
+    cells = sorted(cells)
+    perm = rng.permutation(len(cells))
+    I = np.array([cells[p][0] for p in perm], np.uint32)
+    J = np.array([cells[p][1] for p in perm], np.uint32)
+    V = rng.standard_normal(len(cells)) if values else np.ones(len(cells))
+    return I, J, V
+
+
+def mxm_cases(manifest):
+    """ALP's grb::mxm (reference/blas3.hpp:1744, after a RESIZE call) on matrices
+    built by buildMatrixUnique from random coordinates: plus-times, min-plus and
+    pattern operands, in-place updates of a non-empty C, transpose_left /
+    transpose_right. Inputs (coordinates in input order) and C's CRS and CCS."""
+    R = O.ref("seq")
+    cases = [
+        # name, m, k, n, per_row, seed, ring, transpose, in_place
+        ("plus_times", 40, 35, 50, 3, 1, 0, 0, False),
+        ("plus_times_inplace", 40, 35, 50, 3, 2, 0, 0, True),
+        ("min_plus", 30, 30, 30, 4, 3, 1, 0, False),
+        ("min_plus_inplace", 30, 30, 30, 4, 4, 1, 0, True),
+        ("pattern", 60, 45, 55, 3, 5, 2, 0, False),
+        ("transpose_left", 33, 27, 41, 3, 6, 0, 1, False),
+        ("transpose_right", 29, 37, 31, 3, 7, 0, 2, False),
+        ("wide", 120, 200, 180, 6, 8, 0, 0, False),
+    ]
+    out = {}
+    for name, m, k, n, per, seed, ring, tr, inplace in cases:
+        rng = np.random.default_rng(seed)
+        ar, ac = (k, m) if tr == 1 else (m, k)
+        br, bc = (n, k) if tr == 2 else (k, n)
+        IA, JA, VA = random_coo(ar, ac, per, rng, ring != 2)
+        IB, JB, VB = random_coo(br, bc, per, rng, ring != 2)
+        if inplace:
+            IC, JC, VC = random_coo(m, n, 2, rng)
+        else:
+            IC, JC, VC = (np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0))
+        cap = m * n
+        crs_off, ccs_off = np.zeros(m + 1, np.uint64), np.zeros(n + 1, np.uint64)
+        crs_idx, ccs_idx = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+        crs_val, ccs_val = np.zeros(cap), np.zeros(cap)
+        nz = S()
+        rc = R.ref_mxm(m, k, n, len(IA), P(IA), P(JA), P(VA), len(IB), P(IB), P(JB), P(VB), len(IC), P(IC), P(JC),
+                       P(VC), ring, tr, P(crs_off), P(crs_idx), P(crs_val), P(ccs_off), P(ccs_idx), P(ccs_val),
+                       cap, ctypes.byref(nz))
+        assert rc == 0, (name, rc)
+        z = nz.value
+        meta = np.array([m, k, n, ring, tr, 1 if inplace else 0, seed])
+        out.update({f"{name}_meta": meta, f"{name}_IA": IA, f"{name}_JA": JA, f"{name}_VA": VA,
+                    f"{name}_IB": IB, f"{name}_JB": JB, f"{name}_VB": VB, f"{name}_IC": IC, f"{name}_JC": JC,
+                    f"{name}_VC": VC, f"{name}_crs_off": crs_off, f"{name}_crs_idx": crs_idx[:z],
+                    f"{name}_crs_val": crs_val[:z], f"{name}_ccs_off": ccs_off, f"{name}_ccs_idx": ccs_idx[:z],
+                    f"{name}_ccs_val": ccs_val[:z]})
+        print(f"mxm {name}: {z} entries", flush=True)
+    save("mxm", manifest,
+         dict(kind="mxm", cases=[c[0] for c in cases],
+              meta="m, k, n, ring (0 plus-times f64, 1 min-plus f64, 2 plus-times over void operands), "
+                   "transpose (0, 1 transpose_left, 2 transpose_right), in_place, seed",
+              inputs="coordinates in input order (buildMatrixUnique), A and B in stored orientation; "
+                     "random_coo(rows, cols, per_row, numpy default_rng(seed)) in make_golden.py"),
+         **out)
+
+
 def main(only=None):
     manifest = {}
     mpath = os.path.join(HERE, "manifest.json")
     if only:
         with open(mpath) as f:
             manifest = json.load(f)
-        {"transition": transition_cg, "cg_converge": cg_converge}[only](manifest)
+        {"transition": transition_cg, "cg_converge": cg_converge, "mxm": mxm_cases}[only](manifest)
         with open(mpath, "w") as f:
             json.dump(manifest, f, indent=1, sort_keys=True)
         print(f"updated {only}")
@@ -300,6 +372,7 @@ def main(only=None):
 
     transition_cg(manifest)
     cg_converge(manifest)
+    mxm_cases(manifest)
 
     with open(mpath, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)

@@ -403,3 +403,95 @@ def test_reference_timing_entry_points():
     it = ctypes.c_size_t()
     assert R.ref_time_stencil_cg(16, 16, 16, 5, 1e-300, ctypes.byref(it)) > 0 and it.value == 5
     assert R.ref_time_hpcg(16, 16, 16, 2, 8, 3, ctypes.byref(it)) > 0 and it.value == 3
+
+
+# ------------------------------------------------------------------- mxm ----
+
+def mxm_case(g, name):
+    """The fixture's operands as the reference stored them (buildMatrixUnique
+    order; a transposed operand by its CCS, the rows of its transpose)."""
+    m, k, n, ring, tr, inplace, seed = (int(v) for v in g[f"{name}_meta"])
+    vals = ring != 2
+    ar, ac = (k, m) if tr == 1 else (m, k)
+    br, bc = (n, k) if tr == 2 else (k, n)
+    A = O.coo_to_compressed(ar, ac, g[f"{name}_IA"], g[f"{name}_JA"], g[f"{name}_VA"] if vals else None,
+                            which=1 if tr == 1 else 0)
+    B = O.coo_to_compressed(br, bc, g[f"{name}_IB"], g[f"{name}_JB"], g[f"{name}_VB"] if vals else None,
+                            which=1 if tr == 2 else 0)
+    C = O.coo_to_compressed(m, n, g[f"{name}_IC"], g[f"{name}_JC"], g[f"{name}_VC"]) if inplace else None
+    return A, B, C, (0 if ring == 2 else ring)
+
+
+def ccs_ascending(C):
+    """The CCS of a CRS with each column's rows ascending (how mxm_generic fills
+    an output's CCS, reference/blas3.hpp:1622-1636)."""
+    rows = np.repeat(np.arange(C.nrows), np.diff(C.off.astype(np.int64)))
+    order = np.argsort(C.idx, kind="stable")
+    off = np.zeros(C.ncols + 1, np.uint64)
+    off[1:] = np.cumsum(np.bincount(C.idx.astype(np.int64), minlength=C.ncols))
+    return off, rows[order].astype(np.uint32), (None if C.val is None else C.val[order])
+
+
+MXM_CASES = ["plus_times", "plus_times_inplace", "min_plus", "min_plus_inplace", "pattern",
+             "transpose_left", "transpose_right", "wide"]
+
+
+@pytest.mark.parametrize("name", MXM_CASES)
+def test_mxm_matches_reference(golden, name):
+    """orc_mxm against ALP's grb::mxm: the CRS (offsets, column order, values) bit
+    for bit, and the reference's CCS is the ascending-row transpose of it."""
+    g = golden("mxm")
+    A, B, C, ring = mxm_case(g, name)
+    got = O.mxm(A, B, ring, C)
+    assert np.array_equal(got.off, g[f"{name}_crs_off"])
+    assert np.array_equal(got.idx, g[f"{name}_crs_idx"])
+    assert np.array_equal(got.val, g[f"{name}_crs_val"])
+    coff, cidx, cval = ccs_ascending(got)
+    assert np.array_equal(coff, g[f"{name}_ccs_off"]) and np.array_equal(cidx, g[f"{name}_ccs_idx"])
+    assert np.array_equal(cval, g[f"{name}_ccs_val"])
+
+
+@pytest.mark.parametrize("ring", [0, 1])
+def test_mxm_dense_bruteforce(ring):
+    """Values against a dense product independent of any order: plus-times within
+    rounding (1e-13 relative), min-plus exactly; the pattern is that of the dense
+    product's reachable entries."""
+    rng = np.random.default_rng(11 + ring)
+    m, k, n = 23, 19, 27
+    def rand(r, c):
+        D = np.where(rng.random((r, c)) < 0.2, rng.standard_normal((r, c)), 0.0)
+        I, J = np.nonzero(D)
+        return O.coo_to_compressed(r, c, I, J, D[I, J]), (D != 0)
+    A, Ma = rand(m, k)
+    B, Mb = rand(k, n)
+    C = O.mxm(A, B, ring)
+    Ad, Bd = A.dense(), B.dense()
+    reach = (Ma.astype(int) @ Mb.astype(int)) > 0
+    got = np.zeros((m, n))
+    has = np.zeros((m, n), bool)
+    for i in range(m):
+        for q in range(int(C.off[i]), int(C.off[i + 1])):
+            got[i, C.idx[q]] = C.val[q]
+            has[i, C.idx[q]] = True
+    assert np.array_equal(has, reach)
+    if ring == 0:
+        want = Ad @ Bd
+        assert np.max(np.abs(got - want)) <= 1e-13 * max(1.0, np.max(np.abs(want)))
+    else:
+        S = np.where(Ma[:, :, None] & Mb[None, :, :], Ad[:, :, None] + Bd[None, :, :], np.inf).min(axis=1)
+        assert np.array_equal(got[reach], S[reach])
+
+
+def test_mxm_shifted_identities_closed_form():
+    """The reference's unit test (tests/unit/mxm.cpp): eye(n, n, 1, 1) times
+    eye(n, n, 2, 2) is eye(n, n, 2, 3); updating that result in place by the same
+    product doubles it."""
+    n = 10
+    def eye(v, o):
+        I = np.arange(n - o, dtype=np.uint32)
+        return O.coo_to_compressed(n, n, I, I + o, np.full(n - o, float(v)))
+    C = O.mxm(eye(1, 1), eye(2, 2))
+    E = eye(2, 3)
+    assert np.array_equal(C.off, E.off) and np.array_equal(C.idx, E.idx) and np.array_equal(C.val, E.val)
+    C2 = O.mxm(eye(1, 1), eye(2, 2), 0, C)
+    assert np.array_equal(C2.idx, E.idx) and np.array_equal(C2.val, 2 * E.val)
