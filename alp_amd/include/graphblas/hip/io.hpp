@@ -1,0 +1,352 @@
+/*
+ * io.hpp -- grb::set, setElement, clear, buildVector and buildMatrixUnique of the
+ * grb:: header layer (ALP's base/io.hpp; reference/io.hpp:386,664,893,1186 and
+ * reference/matrix.hpp:1854-1951 for the matrix build).
+ */
+#ifndef ALP_HIP_GRB_IO_HPP
+#define ALP_HIP_GRB_IO_HPP
+
+#include "containers.hpp"
+
+namespace grb {
+
+namespace internal {
+namespace hip {
+
+// the mask predicate of utils::interpretMask (utils.hpp:318-341)
+template <Descriptor descr, typename M>
+__device__ __forceinline__ bool mask_at(const M *mval, const uint8_t *masg, const size_t i) {
+	bool ret;
+	if (descr & descriptors::structural)
+		ret = masg ? masg[i] != 0 : true;
+	else
+		ret = (masg ? masg[i] != 0 : true) && static_cast<bool>(mval[i]);
+	return (descr & descriptors::invert_mask) ? !ret : ret;
+}
+
+template <typename T, typename V>
+__global__ void k_set_scalar(size_t n, T *x, uint8_t *xa, V v) {
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		x[i] = static_cast<T>(v);
+		xa[i] = 1;
+	}
+}
+
+// x_i = v where the mask holds, x_i removed elsewhere; counts the result
+template <Descriptor descr, typename T, typename M, typename V>
+__global__ void k_set_masked(size_t n, T *x, uint8_t *xa, const M *mval, const uint8_t *masg, V v,
+                             unsigned long long *count) {
+	unsigned long long c = 0;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		const bool on = mask_at<descr>(mval, masg, i);
+		if (on) x[i] = static_cast<T>(v);
+		xa[i] = on ? 1 : 0;
+		c += on ? 1 : 0;
+	}
+	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
+	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
+// x = y (values cast) with y's sparsity, optionally restricted to a mask
+template <Descriptor descr, bool MASKED, typename T, typename U, typename M>
+__global__ void k_copy(size_t n, T *x, uint8_t *xa, const U *y, const uint8_t *ya, const M *mval,
+                       const uint8_t *masg, unsigned long long *count) {
+	unsigned long long c = 0;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		bool on = ya[i] != 0;
+		if (MASKED && on) on = mask_at<descr>(mval, masg, i);
+		if (on) x[i] = static_cast<T>(y[i]);
+		xa[i] = on ? 1 : 0;
+		c += on ? 1 : 0;
+	}
+	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
+	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
+// sparse build: entry k sets x[idx[k]]; a repeated index folds with Dup in input
+// order (one thread walks the entries in order: the fold order is the input's)
+template <class Dup, typename T>
+__global__ void k_scatter_build(size_t nz, const size_t *idx, const T *vals, T *x, uint8_t *xa) {
+	if (blockIdx.x != 0 || threadIdx.x != 0) return;
+	for (size_t k = 0; k < nz; ++k) {
+		const size_t i = idx[k];
+		if (xa[i]) {
+			foldl_op<Dup>(x[i], vals[k]);
+		} else {
+			x[i] = vals[k];
+			xa[i] = 1;
+		}
+	}
+}
+
+}  // namespace hip
+}  // namespace internal
+
+// ---- vectors ------------------------------------------------------------------
+
+template <typename T, enum Backend b, typename C>
+RC clear(Vector<T, b, C> &x) {
+	if (x.size())
+		hipLaunchKernelGGL(internal::hip::k_fill<uint8_t>, dim3(internal::hip::grid(x.size())),
+		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.assigned(), (uint8_t)0);
+	x.set_nnz(0);
+	return internal::hip::sync();
+}
+
+// x_i = val for every i (reference/io.hpp:893): x becomes dense
+template <Descriptor descr = descriptors::no_operation, typename T, enum Backend b, typename C, typename V>
+RC set(Vector<T, b, C> &x, const V val, const Phase phase = EXECUTE,
+       typename std::enable_if<std::is_convertible<V, T>::value>::type * = nullptr) {
+	if (phase == RESIZE) return SUCCESS;
+	if (x.size())
+		hipLaunchKernelGGL((internal::hip::k_set_scalar<T, T>), dim3(internal::hip::grid(x.size())),
+		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
+		                   static_cast<T>(val));
+	x.set_nnz(x.size());
+	return internal::hip::sync();
+}
+
+// x_i = val where the mask holds; x holds nothing else afterwards (io.hpp:977)
+template <Descriptor descr = descriptors::no_operation, typename T, enum Backend b, typename C, typename M, typename V>
+RC set(Vector<T, b, C> &x, const Vector<M, b, C> &mask, const V val, const Phase phase = EXECUTE,
+       typename std::enable_if<std::is_convertible<V, T>::value>::type * = nullptr) {
+	if (mask.size() == 0) return set<descr>(x, val, phase);
+	if (mask.size() != x.size()) return MISMATCH;
+	if ((descr & descriptors::dense) && (!x.dense() || !mask.dense())) return ILLEGAL;
+	if (phase == RESIZE) return SUCCESS;
+	// x is emptied first (reference/io.hpp:703-712); a mask that is x itself is
+	// read after that, as the reference reads it
+	if ((const void *)&x == (const void *)&mask) {
+		RC c = clear(x);
+		if (c) return c;
+	}
+	internal::hip::Counter cnt;
+	RC r = cnt.reset();
+	if (r) return r;
+	if (x.size())
+		hipLaunchKernelGGL((internal::hip::k_set_masked<descr, T, M, T>), dim3(internal::hip::grid(x.size())),
+		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
+		                   mask.raw(), mask.dense() ? nullptr : mask.assigned(), static_cast<T>(val), cnt.c.p);
+	unsigned long long z = 0;
+	r = cnt.read(z);
+	x.set_nnz((size_t)z);
+	return r;
+}
+
+// x = y (io.hpp:1186), with y's sparsity
+template <Descriptor descr = descriptors::no_operation, typename T, typename U, enum Backend b, typename C>
+RC set(Vector<T, b, C> &x, const Vector<U, b, C> &y, const Phase phase = EXECUTE) {
+	if (x.size() != y.size()) return MISMATCH;
+	if ((const void *)&x == (const void *)&y) return ILLEGAL;
+	if (phase == RESIZE) return SUCCESS;
+	internal::hip::Counter cnt;
+	RC r = cnt.reset();
+	if (r) return r;
+	if (x.size())
+		hipLaunchKernelGGL((internal::hip::k_copy<descr, false, T, U, uint8_t>), dim3(internal::hip::grid(x.size())),
+		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
+		                   y.raw(), y.assigned(), (const uint8_t *)nullptr, (const uint8_t *)nullptr, cnt.c.p);
+	unsigned long long z = 0;
+	r = cnt.read(z);
+	x.set_nnz((size_t)z);
+	return r;
+}
+
+// x = y where the mask holds (io.hpp:1298)
+template <Descriptor descr = descriptors::no_operation, typename T, typename M, typename U, enum Backend b,
+          typename C>
+RC set(Vector<T, b, C> &x, const Vector<M, b, C> &mask, const Vector<U, b, C> &y, const Phase phase = EXECUTE) {
+	if (mask.size() == 0) return set<descr>(x, y, phase);
+	if (x.size() != y.size() || mask.size() != x.size()) return MISMATCH;
+	if ((const void *)&x == (const void *)&y) return ILLEGAL;
+	if (phase == RESIZE) return SUCCESS;
+	internal::hip::Counter cnt;
+	RC r = cnt.reset();
+	if (r) return r;
+	if (x.size())
+		hipLaunchKernelGGL((internal::hip::k_copy<descr, true, T, U, M>), dim3(internal::hip::grid(x.size())),
+		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
+		                   y.raw(), y.assigned(), mask.raw(), mask.dense() ? nullptr : mask.assigned(), cnt.c.p);
+	unsigned long long z = 0;
+	r = cnt.read(z);
+	x.set_nnz((size_t)z);
+	return r;
+}
+
+// x_i = val (io.hpp setElement)
+template <Descriptor descr = descriptors::no_operation, typename T, enum Backend b, typename C, typename V>
+RC setElement(Vector<T, b, C> &x, const V val, const size_t i, const Phase phase = EXECUTE) {
+	if (i >= x.size()) return MISMATCH;
+	if (phase == RESIZE) return SUCCESS;
+	uint8_t was = 0;
+	if (alp_hip_memcpy_d2h(&was, x.assigned() + i, 1)) return PANIC;
+	const T v = static_cast<T>(val);
+	const uint8_t one = 1;
+	if (alp_hip_memcpy_h2d(x.raw() + i, &v, sizeof(T)) || alp_hip_memcpy_h2d(x.assigned() + i, &one, 1))
+		return PANIC;
+	if (!was) x.set_nnz(x.nnz() + 1);
+	return SUCCESS;
+}
+
+// dense build from a range of size(x) values (io.hpp buildVector)
+template <Descriptor descr = descriptors::no_operation, typename T, enum Backend b, typename C, typename It>
+RC buildVector(Vector<T, b, C> &x, It begin, const It end, const IOMode) {
+	std::vector<T> h;
+	for (; begin != end; ++begin) h.push_back(static_cast<T>(*begin));
+	if (h.size() != x.size()) return MISMATCH;
+	if (!h.empty()) {
+		if (alp_hip_memcpy_h2d(x.raw(), h.data(), sizeof(T) * h.size())) return PANIC;
+		hipLaunchKernelGGL(internal::hip::k_fill<uint8_t>, dim3(internal::hip::grid(x.size())),
+		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.assigned(), (uint8_t)1);
+	}
+	x.set_nnz(x.size());
+	return internal::hip::sync();
+}
+
+// sparse build from index and value ranges; repeated indices fold with dup in
+// input order (io.hpp buildVector with a Dup operator, default: keep the last)
+template <Descriptor descr = descriptors::no_operation, typename T, enum Backend b, typename C, typename ItI,
+          typename ItV, class Dup = operators::right_assign<T>>
+RC buildVector(Vector<T, b, C> &x, ItI ind, const ItI ind_end, ItV val, const ItV val_end, const IOMode,
+               const Dup & = Dup()) {
+	std::vector<size_t> hi;
+	std::vector<T> hv;
+	for (; ind != ind_end && val != val_end; ++ind, ++val) {
+		if ((size_t)*ind >= x.size()) return MISMATCH;
+		hi.push_back((size_t)*ind);
+		hv.push_back(static_cast<T>(*val));
+	}
+	if (ind != ind_end || val != val_end) return MISMATCH;
+	if (hi.empty()) return SUCCESS;
+	internal::hip::DevArray<size_t> di(hi.size());
+	internal::hip::DevArray<T> dv(hv.size());
+	if (alp_hip_memcpy_h2d(di.p, hi.data(), sizeof(size_t) * hi.size()) ||
+	    alp_hip_memcpy_h2d(dv.p, hv.data(), sizeof(T) * hv.size()))
+		return PANIC;
+	hipLaunchKernelGGL((internal::hip::k_scatter_build<Dup, T>), dim3(1), dim3(64), 0, internal::hip::stream(),
+	                   hi.size(), (const size_t *)di.p, (const T *)dv.p, x.raw(), x.assigned());
+	return x.recount();
+}
+
+// ---- matrices -------------------------------------------------------------------
+
+namespace internal {
+
+template <typename T, enum Backend b, typename R, typename C, typename N, typename VT>
+RC build_matrix(Matrix<T, b, R, C, N> &A, std::vector<uint32_t> &hi, std::vector<uint32_t> &hj,
+                const std::vector<VT> &hv) {
+	const size_t nz = hi.size();
+	alp_hip_matrix_t h = nullptr;
+	// the reference's buildMatrixUnique fills CRS and CCS (matrix.hpp:1854-1951)
+	if (std::is_void<T>::value || hip::lib_holds_values<T>::value) {
+		const int vt = hip::lib_type<T>::value;
+		RC r = rc(alp_hip_matrix_create_coo(&h, A.rows(), A.cols(), nz, hi.data(), hj.data(),
+		                                    vt == ALP_VAL_PATTERN ? nullptr : (const void *)hv.data(), vt,
+		                                    ALP_MAT_WITH_CCS));
+		if (r == SUCCESS) A.reset(h);
+		return r;
+	}
+	// values of other types: the library keeps the pattern, the values are placed
+	// here by where each input entry went
+	typedef typename std::conditional<std::is_void<T>::value, char, T>::type E;
+	hip::DevArray<uint64_t> crs_src(nz), ccs_src(nz);
+	RC r = rc(alp_hip_matrix_create_coo_map(&h, A.rows(), A.cols(), nz, hi.data(), hj.data(), ALP_MAT_WITH_CCS,
+	                                        crs_src.p, ccs_src.p));
+	if (r) return r;
+	hip::DevArray<E> in(nz), crs(nz), ccs(nz);
+	std::vector<E> hvt(hv.begin(), hv.end());
+	if (nz && alp_hip_memcpy_h2d(in.p, hvt.data(), sizeof(E) * nz)) {
+		alp_hip_matrix_destroy(h);
+		return PANIC;
+	}
+	if (nz) {
+		hipLaunchKernelGGL(hip::k_gather_values<E>, dim3(hip::grid(nz)), dim3(hip::kBlock), 0, hip::stream(), nz,
+		                   (const uint64_t *)crs_src.p, (const E *)in.p, crs.p);
+		hipLaunchKernelGGL(hip::k_gather_values<E>, dim3(hip::grid(nz)), dim3(hip::kBlock), 0, hip::stream(), nz,
+		                   (const uint64_t *)ccs_src.p, (const E *)in.p, ccs.p);
+	}
+	r = hip::sync();
+	if (r) {
+		alp_hip_matrix_destroy(h);
+		return r;
+	}
+	A.reset(h, std::move(crs), std::move(ccs));
+	return SUCCESS;
+}
+
+template <typename It, typename VT>
+void take_value(std::vector<VT> &v, const It &it, std::true_type) {
+	v.push_back(static_cast<VT>(it.v()));
+}
+template <typename It, typename VT>
+void take_value(std::vector<VT> &, const It &, std::false_type) {}
+
+}  // namespace internal
+
+// buildMatrixUnique (io.hpp:1587-1745): (I, J, V, nz) and iterator-range forms;
+// entries are stored in the reference backend's order (matrix.hpp:1854-1951)
+template <Descriptor descr = descriptors::no_operation, typename T, enum Backend b, typename R, typename C,
+          typename N, typename It1, typename It2, typename It3>
+RC buildMatrixUnique(Matrix<T, b, R, C, N> &A, It1 I, It2 J, It3 V, const size_t nz, const IOMode) {
+	typedef typename std::conditional<std::is_void<T>::value, double, T>::type VT;
+	std::vector<uint32_t> hi(nz), hj(nz);
+	std::vector<VT> hv;
+	hv.reserve(std::is_void<T>::value ? 0 : nz);
+	for (size_t k = 0; k < nz; ++k, ++I, ++J) {
+		if ((uint64_t)*I >= A.rows() || (uint64_t)*J >= A.cols()) return MISMATCH;
+		hi[k] = static_cast<uint32_t>(*I);
+		hj[k] = static_cast<uint32_t>(*J);
+		if (!std::is_void<T>::value) {
+			hv.push_back(static_cast<VT>(*V));
+			++V;
+		}
+	}
+	return internal::build_matrix(A, hi, hj, hv);
+}
+template <Descriptor descr = descriptors::no_operation, typename T, enum Backend b, typename R, typename C,
+          typename N, typename It1, typename It2, typename It3>
+RC buildMatrixUnique(Matrix<T, b, R, C, N> &A, It1 I, const It1 I_end, It2 J, const It2 J_end, It3 V,
+                     const It3 V_end, const IOMode mode) {
+	const size_t nz = static_cast<size_t>(I_end - I);
+	if ((size_t)(J_end - J) != nz || (size_t)(V_end - V) != nz) return MISMATCH;
+	return buildMatrixUnique<descr>(A, I, J, V, nz, mode);
+}
+// pattern matrices: (I, J, nz)
+template <Descriptor descr = descriptors::no_operation, enum Backend b, typename R, typename C, typename N,
+          typename It1, typename It2>
+RC buildMatrixUnique(Matrix<void, b, R, C, N> &A, It1 I, It2 J, const size_t nz, const IOMode) {
+	std::vector<uint32_t> hi(nz), hj(nz);
+	for (size_t k = 0; k < nz; ++k, ++I, ++J) {
+		if ((uint64_t)*I >= A.rows() || (uint64_t)*J >= A.cols()) return MISMATCH;
+		hi[k] = static_cast<uint32_t>(*I);
+		hj[k] = static_cast<uint32_t>(*J);
+	}
+	return internal::build_matrix(A, hi, hj, std::vector<double>());
+}
+// nonzero-iterator form (io.hpp:1587-1745): the iterator gives i(), j() and, for a
+// valued matrix, v(), e.g. utils::MatrixFileReader::begin(); entries are taken in
+// iteration order, which decides the within-row order as in the reference
+template <Descriptor descr = descriptors::no_operation, typename T, enum Backend b, typename R, typename C,
+          typename N, typename fwd_iterator>
+RC buildMatrixUnique(Matrix<T, b, R, C, N> &A, fwd_iterator start, const fwd_iterator end, const IOMode) {
+	typedef typename std::conditional<std::is_void<T>::value, double, T>::type VT;
+	std::vector<uint32_t> hi, hj;
+	std::vector<VT> hv;
+	for (; start != end; ++start) {
+		if ((uint64_t)start.i() >= A.rows() || (uint64_t)start.j() >= A.cols()) return MISMATCH;
+		hi.push_back(static_cast<uint32_t>(start.i()));
+		hj.push_back(static_cast<uint32_t>(start.j()));
+		internal::take_value(hv, start, std::integral_constant<bool, !std::is_void<T>::value>());
+	}
+	return internal::build_matrix(A, hi, hj, hv);
+}
+
+template <typename T, enum Backend b, typename R, typename C, typename N>
+RC clear(Matrix<T, b, R, C, N> &A) {
+	A.reset(nullptr);
+	return SUCCESS;
+}
+
+}  // namespace grb
+
+#endif

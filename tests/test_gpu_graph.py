@@ -105,8 +105,13 @@ def test_pagerank_smoke10(A, golden):
 def test_rmat_scale22_bfs_and_pagerank_against_oracle(A):
     """The graph workload's own size: R-MAT scale 22, edge factor 16, seed 1,
     symmetrised. BFS levels bit-exact with the oracle's queue BFS; PageRank
-    (0.85, 1e-9) within 1e-12 relative of the oracle's restatement of
-    simple_pagerank.hpp:146-533 on the same CRS/CCS, iterations within 1."""
+    (0.85, 1e-9) against the oracle's restatement of simple_pagerank.hpp:146-533
+    on the same CRS/CCS: iterations within 1, ranks within 1e-11 of the largest.
+    The device folds a hub column (over 10^5 entries here) in chunks, the
+    reference sequentially; recursive summation of L positive terms in two
+    orders can differ by up to (L - 1) eps relative (1e-11 at L = 10^5), and
+    45 iterations carry it. (The scale-9 and scale-11 graphs, whose columns are
+    short, agree with the reference's fixtures to 1e-12.)"""
     scale = 22
     n, m = 1 << scale, 16 << scale
     s, d = A.rmat_edges(scale, m, seed=1)
@@ -122,10 +127,12 @@ def test_rmat_scale22_bfs_and_pagerank_against_oracle(A):
     coff, cidx, _ = G.download(1, with_values=False)
     ccs = O.CSR(n, n, coff, cidx, None)
     orc, opr, oit, oq = O.pagerank(crs, ccs, 0.85, 1e-9, 1000)
-    print(f"scale 22: nnz={G.nnz} pagerank it={it} (oracle {oit}) q={q:.3e} (oracle {oq:.3e})", flush=True)
+    err = np.max(np.abs(pr.numpy() - opr)) / np.max(opr)
+    print(f"scale 22: nnz={G.nnz} pagerank it={it} (oracle {oit}) q={q:.3e} (oracle {oq:.3e}) "
+          f"max difference {err:.2e} of the largest rank", flush=True)
     assert rc == orc == A.SUCCESS
     assert abs(it - oit) <= 1
-    assert np.max(np.abs(pr.numpy() - opr)) <= 1e-12 * np.max(opr)
+    assert err <= 1e-11
 
 
 def test_bfs_directed_pull_walks_in_neighbours(A):

@@ -89,3 +89,162 @@ def test_grb_matrix_file_reader_program(tmp_path, pattern):
     assert res["nz"] == S.nnz and res["nnz"] == S.nnz
     want = np.diff(S.indptr).astype(np.float64) if pattern else np.asarray(S.sum(axis=1)).ravel()
     assert np.max(np.abs(np.asarray(res["y"]) - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
+
+
+SEM_PROG = os.path.join(ROOT, "tests", "cpp", "_build", "grb_semantics_program")
+HPCG_PROG = os.path.join(ROOT, "tests", "cpp", "_build", "grb_hpcg_program")
+
+
+def _run(prog, *args):
+    if not os.path.exists(prog):
+        pytest.fail(f"{prog} was not built (make -C alp_amd)")
+    print("running", prog, *args, flush=True)
+    out = subprocess.run([prog, *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def _vec(res, name):
+    return dict(zip((int(i) for i in res[name + "_idx"]), res[name + "_val"]))
+
+
+def _rows(res, name="A"):
+    """The matrix's entries per row, in storage order."""
+    rows = {}
+    for i, j, v in zip(res[name + "_I"], res[name + "_J"], res[name + "_V"]):
+        rows.setdefault(int(i), []).append((int(j), v))
+    return rows
+
+
+def _gather(rows, m, x, mul, add, ident, y0=None, mask=None, add_identity=False):
+    """vxm_inner_kernel_gather (reference/blas2.hpp:295-568) row by row, in
+    storage order: out = p (+) out; folded into an existing output entry."""
+    y = dict(y0 or {})
+    for i in range(m):
+        if mask is not None and not mask(i):
+            continue
+        out, hit = ident, False
+        if add_identity and i in x:
+            out, hit = x[i], True
+        for j, a in rows.get(i, []):
+            if j in x:
+                out, hit = add(mul(a, x[j]), out), True
+        if hit:
+            y[i] = add(y[i], out) if i in y else out
+    return y
+
+
+def _assert_vec(res, name, want, exact=True):
+    got = _vec(res, name)
+    assert int(res[name + "_nnz"]) == len(want), (name, res[name + "_nnz"], len(want))
+    assert set(got) == set(want), (name, sorted(set(got) ^ set(want)))
+    for k, v in want.items():
+        if exact:
+            assert got[k] == v, (name, k, got[k], v)
+        else:
+            assert abs(got[k] - v) <= 1e-12 * max(1.0, abs(v)), (name, k, got[k], v)
+
+
+@pytest.mark.parametrize("seed", [1, 7])
+def test_grb_generic_semirings_and_sparse_semantics(seed):
+    """The grb:: layer on arbitrary algebras and sparse vectors against brute
+    force from the program's own printed inputs (values are small multiples of
+    1/4, so every FP64 sum is exact and order-free): mxv / vxm over plus-times,
+    min-plus (fresh and accumulated output), max-times, plus-min over int, a
+    program-defined operator; lor-land nonzero counts, the pushed sparse input,
+    masks and add_identity; OVERLAP, ILLEGAL and MISMATCH; element-wise
+    operations, dots and folds under sparsity; a device eWiseLambda; mxm over
+    plus-min on int (the generic SpGEMM) and plus-times; masked set."""
+    r = _run(SEM_PROG, seed)
+    assert r["rc"] == 0
+    m, n = int(r["m"]), int(r["n"])
+    rows = _rows(r)
+    ival = {(int(i), int(j)): int(v) for i, j, v in zip(r["I"], r["J"], r["Vi"])}
+    irows = {i: [(j, ival[(i, j)]) for j, _ in lst] for i, lst in rows.items()}
+    x, xi, xb = _vec(r, "x"), {k: int(v) for k, v in _vec(r, "xi").items()}, _vec(r, "xb")
+    inf = float("inf")
+    mul, add = (lambda a, b: a * b), (lambda a, b: a + b)
+    # 1-3: double semirings
+    y1 = _gather(rows, m, x, mul, add, 0.0)
+    _assert_vec(r, "y_plus_times", y1)
+    _assert_vec(r, "y_min_plus", _gather(rows, m, x, add, min, inf))
+    _assert_vec(r, "y_min_plus_acc", _gather(rows, m, x, add, min, inf, y0={i: 1.5 for i in range(m)}))
+    _assert_vec(r, "y_max_times", _gather(rows, m, x, mul, max, -inf))
+    # 4: plus-min over int
+    imax = 2 ** 31 - 1
+    _assert_vec(r, "y_plus_min_int", _gather(irows, m, xi, min, add, 0))
+    # 5: (add mod 7, times) by vxm: gather over columns
+    xm = {k: int(v) for k, v in _vec(r, "xm").items()}
+    cols = {}
+    for i, lst in irows.items():
+        for j, a in lst:
+            cols.setdefault(j, []).append((i, a))
+    _assert_vec(r, "y_mod7_vxm", _gather(cols, n, xm, lambda a, s: s * a, lambda p, o: (p + o) % 7, 0))
+    # 6: lor-land
+    lor, land = (lambda a, b: bool(a) or bool(b)), (lambda a, b: bool(a) and bool(b))
+    prow = {i: [(j, 1.0) for j, _ in lst] for i, lst in rows.items()}
+    _assert_vec(r, "y_lor", _gather(prow, m, xb, land, lor, False))
+    _assert_vec(r, "y_lor_push", _gather(prow, m, {3: True}, land, lor, False))
+    mask = set(_vec(r, "mask"))
+    _assert_vec(r, "y_lor_masked", _gather(prow, m, xb, land, lor, False, mask=lambda i: i not in mask))
+    qrow = {i: [(j, 1.0) for j, _ in lst] for i, lst in rows.items() if i < n}
+    _assert_vec(r, "y_lor_add_identity", _gather(qrow, n, xb, land, lor, False, add_identity=True))
+    # 7: error codes (grb::RC values)
+    assert r["rc_overlap"] == 4 and r["rc_dense_sparse"] == 7 and r["rc_mismatch"] == 3
+    assert r["rc_safe_overlap_ok"] == 1
+    _assert_vec(r, "y_plus_times_twice", {k: 2 * v for k, v in y1.items()})
+    # 8: level 1
+    u, w = _vec(r, "u"), _vec(r, "w")
+    _assert_vec(r, "ewise_op_sub", {k: u[k] - w[k] for k in u if k in w})
+    _assert_vec(r, "ewise_monoid_max", {k: max(u.get(k, -inf), w.get(k, -inf)) for k in set(u) | set(w)})
+    z3 = dict(u)
+    for k in set(x) & set(w):
+        z3[k] = z3[k] + x[k] * w[k] if k in z3 else x[k] * w[k]
+    _assert_vec(r, "ewise_mul_into_u", z3)
+    z4 = dict(u)
+    for k in set(x) | set(w):
+        t = x[k] + w[k] if (k in x and k in w) else x.get(k, w.get(k))
+        z4[k] = z4[k] + t if k in z4 else t
+    _assert_vec(r, "ewise_add_into_u", z4)
+    _assert_vec(r, "foldl_op_mul", {k: (v * w[k] if k in w else v) for k, v in u.items()})
+    both = set(u) & set(w)
+    assert r["dot_max_times"] == max([-inf] + [u[k] * w[k] for k in both])
+    assert r["dot_plus_times"] == sum(u[k] * w[k] for k in both)
+    assert r["fold_sum_int"] == sum(xi.values())
+    assert r["fold_max"] == max([-inf] + list(u.values()))
+    # 9: eWiseLambda on u's nonzeroes
+    _assert_vec(r, "lambda_u", {k: 2 * v + w.get(k, 100.0) for k, v in u.items()})
+    # 10: mxm, A times A^T, as maps (the library path's storage order is pinned by
+    # the mxm fixture tests)
+    def mxm_map(rws, mulf, addf):
+        out = {}
+        for i, ri in rws.items():
+            di = dict(ri)
+            for k, rk in rws.items():
+                acc = None
+                for j, b in rk:
+                    if j in di:
+                        p = mulf(di[j], b)
+                        acc = p if acc is None else addf(acc, p)
+                if acc is not None:
+                    out[(i, k)] = acc
+        return out
+    for name, want in (("mxm_plus_min_int", mxm_map(irows, min, add)),
+                       ("mxm_plus_times", mxm_map(rows, mul, add))):
+        got = {(int(i), int(j)): v for i, j, v in zip(r[name + "_I"], r[name + "_J"], r[name + "_V"])}
+        assert got == want, name
+    _assert_vec(r, "masked_set", {i: 4.0 for i in mask})
+
+
+def test_grb_hpcg_program_as_the_reference_smoke_test(golden):
+    """build_hpcg_system + hpcg under a grb::Launcher, as tests/smoke/hpcg.cpp
+    runs it: 16^3, one coarse level, 27 index-mod colours, 56 iterations; the
+    smoke test's bar ||x - 1|| < 4e-14 and the reference's iteration count and
+    residual (fixture hpcg_16x16x16_l1_c27)."""
+    g = golden("hpcg_16x16x16_l1_c27")
+    r = _run(HPCG_PROG, 16, 16, 16, 1, 27, 56)
+    assert r["launcher_rc"] == 0 and r["rc"] == 0 and r["nprocs"] == 1 and r["pid"] == 0
+    assert r["iterations"] == int(g["iterations"][0]) == 56
+    assert abs(r["residual"] - g["residual"][0]) <= 1e-6 * g["residual"][0]
+    assert r["square_norm_diff"] < (4e-14) ** 2
+    assert r["pinned_nonzeroes"] == 16 ** 3
