@@ -294,8 +294,13 @@ static unsigned g_rgrid(size_t n) {
 template <typename T>
 struct DevBuf {
 	T *p = nullptr;
-	~DevBuf() {
+	DevBuf() = default;
+	DevBuf(const DevBuf &) = delete;
+	DevBuf &operator=(const DevBuf &) = delete;
+	~DevBuf() { release(); }
+	void release() {
 		if (p) hipFree(p);
+		p = nullptr;
 	}
 	alp_rc_t alloc(size_t count) {
 		ALP_HIP_TRY(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
@@ -456,8 +461,10 @@ ALP_EXPORT alp_rc_t alp_hip_max_degree_vertex(alp_hip_matrix_t A, uint32_t *vert
 	return ALP_SUCCESS;
 }
 
-// BFS work buffers, kept on the matrix between calls (a call allocates nothing
-// once they exist); released with the matrix
+// Graph work buffers, kept on the matrix between calls (a call allocates nothing
+// once they exist) and released with it: the BFS queues, degrees and scan space,
+// made on the first BFS or k-hop, and PageRank's vectors and state, made on its
+// first call
 struct BfsWork {
 	DevBuf<uint32_t> q0, q1;
 	DevBuf<uint64_t> deg, scan;
@@ -465,31 +472,47 @@ struct BfsWork {
 	DevBuf<char> tmp;
 	DevBuf<int32_t> lev;  // k-hop's levels (alp_hip_bfs writes the caller's array)
 	size_t scan_tb = 0;
+	DevBuf<double> rs, pn, pnn;  // PageRank: alpha / out-degree, pr * rs, the product
+	DevBuf<PrState> pr_state;
 };
 
 static void free_bfs_work(void *p) { delete static_cast<BfsWork *>(p); }
 
-static alp_rc_t bfs_work(alp_hip_matrix_s *A, BfsWork **out) {
+static alp_rc_t graph_work(alp_hip_matrix_s *A, BfsWork **out) {
 	if (!A->graph_cache) {
-		const size_t n = A->nrows;
 		auto *w = new (std::nothrow) BfsWork();
 		if (!w) return ALP_OUTOFMEM;
-		alp_rc_t rc = ALP_SUCCESS;
-		if ((rc = w->q0.alloc(n)) || (rc = w->q1.alloc(n)) || (rc = w->deg.alloc(n + 1)) ||
-		    (rc = w->scan.alloc(n + 1)) || (rc = w->cnt.alloc(1)) || (rc = w->lev.alloc(n))) {
-			delete w;
-			return rc;
-		}
-		if (hipcub::DeviceScan::ExclusiveSum(nullptr, w->scan_tb, w->deg.p, w->scan.p, n + 1, stream()) !=
-		        hipSuccess ||
-		    (rc = w->tmp.alloc(w->scan_tb))) {
-			delete w;
-			return rc ? rc : ALP_PANIC;
-		}
 		A->graph_cache = w;
 		A->free_graph_cache = free_bfs_work;
 	}
 	*out = static_cast<BfsWork *>(A->graph_cache);
+	return ALP_SUCCESS;
+}
+
+static alp_rc_t bfs_work(alp_hip_matrix_s *A, BfsWork **out) {
+	BfsWork *w = nullptr;
+	ALP_TRY(graph_work(A, &w));
+	if (!w->q0.p) {
+		const size_t n = A->nrows;
+		alp_rc_t rc = ALP_SUCCESS;
+		size_t tb = 0;
+		if ((rc = w->q0.alloc(n)) || (rc = w->q1.alloc(n)) || (rc = w->deg.alloc(n + 1)) ||
+		    (rc = w->scan.alloc(n + 1)) || (rc = w->cnt.alloc(1)) || (rc = w->lev.alloc(n)) ||
+		    (hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w->deg.p, w->scan.p, n + 1, stream()) != hipSuccess
+		         ? (rc = ALP_PANIC)
+		         : (rc = w->tmp.alloc(tb)))) {
+			w->q0.release();
+			w->q1.release();
+			w->deg.release();
+			w->scan.release();
+			w->cnt.release();
+			w->lev.release();
+			w->tmp.release();
+			return rc;
+		}
+		w->scan_tb = tb;
+	}
+	*out = w;
 	return ALP_SUCCESS;
 }
 
@@ -593,12 +616,20 @@ ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double al
 	if (n == 0) return ALP_SUCCESS;
 	hipStream_t s = stream();
 	if (!L->ccs.off) ALP_TRY(build_ccs_from_crs(L));
-	DevBuf<double> rs, pn, pnn;
-	DevBuf<PrState> st;
-	ALP_TRY(rs.alloc(n));
-	ALP_TRY(pn.alloc(n));
-	ALP_TRY(pnn.alloc(n));
-	ALP_TRY(st.alloc(1));
+	BfsWork *w = nullptr;
+	ALP_TRY(graph_work(L, &w));
+	if (!w->pr_state.p) {
+		alp_rc_t rc = ALP_SUCCESS;
+		if ((rc = w->rs.alloc(n)) || (rc = w->pn.alloc(n)) || (rc = w->pnn.alloc(n)) || (rc = w->pr_state.alloc(1))) {
+			w->rs.release();
+			w->pn.release();
+			w->pnn.release();
+			w->pr_state.release();
+			return rc;
+		}
+	}
+	DevBuf<double> &rs = w->rs, &pn = w->pn, &pnn = w->pnn;
+	DevBuf<PrState> &st = w->pr_state;
 	PrState h{};
 	h.s[0] = h.s[1] = PrScal{0.0, 0ull, 0, 0};
 	h.alpha = alpha;
