@@ -129,6 +129,7 @@ SIGNATURES = {
     "alp_hip_pagerank": (_RC, [_P, _P, _D, _D, _S, ctypes.POINTER(_S), ctypes.POINTER(_D)]),
     "alp_hip_khop": (_RC, [_P, ctypes.c_uint32, _S, _P]),
     "alp_hip_batch_create_laplace7": (_RC, [ctypes.POINTER(_P), _S, _P, _P, _P, _P, _P, _P, _S]),
+    "alp_hip_batch_create_crs": (_RC, [ctypes.POINTER(_P), _S, _P, _P, _P, _P, _S]),
     "alp_hip_batch_destroy": (_RC, [_P]),
     "alp_hip_batch_info": (_RC, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S), _P]),
     "alp_hip_batch_layout": (_RC, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
@@ -533,22 +534,49 @@ def khop(A: Matrix, source: int, k: int) -> Vector:
 # ---------------------------------------------------------------------------
 
 class Batch:
-    """Block-diagonal batch of anisotropic 7-point Laplacians with nrhs right-hand sides."""
+    """Block-diagonal batch of SPD systems with nrhs right-hand sides: anisotropic
+    7-point Laplacians (Batch(nx, ny, nz, cx, cy, cz)) or caller-supplied CRS
+    systems (Batch.from_crs)."""
 
-    def __init__(self, nx, ny, nz, cx, cy, cz, nrhs=8):
-        self.nsys = len(nx)
-        arrs = [np.ascontiguousarray(a, np.uint32) for a in (nx, ny, nz)] + \
-               [np.ascontiguousarray(a, np.float64) for a in (cx, cy, cz)]
+    def __init__(self, nx=None, ny=None, nz=None, cx=None, cy=None, cz=None, nrhs=8, _handle=None, _nsys=0):
         h = _P()
-        check(lib().alp_hip_batch_create_laplace7(ctypes.byref(h), self.nsys, *[a.ctypes.data for a in arrs], nrhs),
-              "batch_create_laplace7")
-        self.h = h.value
+        if _handle is None:
+            self.nsys = len(nx)
+            arrs = [np.ascontiguousarray(a, np.uint32) for a in (nx, ny, nz)] + \
+                   [np.ascontiguousarray(a, np.float64) for a in (cx, cy, cz)]
+            check(lib().alp_hip_batch_create_laplace7(ctypes.byref(h), self.nsys, *[a.ctypes.data for a in arrs],
+                                                      nrhs), "batch_create_laplace7")
+            self.h = h.value
+        else:
+            self.nsys, self.h = _nsys, _handle
         self.nrhs = nrhs
         nr, nnz = _S(), _S()
         self.row0 = np.zeros(self.nsys + 1, np.uint64)
         check(lib().alp_hip_batch_info(self.h, ctypes.byref(nr), ctypes.byref(nnz), self.row0.ctypes.data),
               "batch_info")
         self.nrows, self.nnz = nr.value, nnz.value
+
+    @classmethod
+    def from_crs(cls, systems, nrhs=8) -> "Batch":
+        """systems: a list of (off, idx, val) CRS arrays, one SPD matrix each
+        (columns local to the system); stacked into one batch."""
+        row0 = np.zeros(len(systems) + 1, np.uint64)
+        offs, idxs, vals = [np.zeros(1, np.uint64)], [], []
+        base = 0
+        for s, (off, idx, val) in enumerate(systems):
+            off = np.asarray(off, np.uint64)
+            row0[s + 1] = row0[s] + (off.size - 1)
+            offs.append(off[1:] + np.uint64(base))
+            idxs.append(np.asarray(idx, np.uint32))
+            vals.append(np.asarray(val, np.float64))
+            base += int(off[-1])
+        off = np.ascontiguousarray(np.concatenate(offs))
+        idx = np.ascontiguousarray(np.concatenate(idxs)) if idxs else np.zeros(0, np.uint32)
+        val = np.ascontiguousarray(np.concatenate(vals)) if vals else np.zeros(0)
+        h = _P()
+        check(lib().alp_hip_batch_create_crs(ctypes.byref(h), len(systems), row0.ctypes.data, off.ctypes.data,
+                                             idx.ctypes.data, val.ctypes.data, nrhs), "batch_create_crs")
+        return cls(nrhs=nrhs, _handle=h.value, _nsys=len(systems))
 
     def spmm(self, X: Vector, Y: Vector) -> None:
         check(lib().alp_hip_batch_spmm(self.h, X.ptr, Y.ptr), "batch_spmm")

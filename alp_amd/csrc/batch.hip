@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_lap_fill(size_t nrows, const SysDesc
 // partial of sum_i p[i][r] * (A p)[i][r] over the tile goes to part[tile*nrhs + r].
 // RES computes out = B - A X instead of A X and partials of out.out.
 
-template <typename T, int W, int RPL, bool DOT, bool RES, typename A = T>
+template <typename T, int W, int RPL, bool DOT, bool RES, typename A = T, bool XCD = false>
 __global__ __launch_bounds__(kBlock) void k_spmm(size_t ntiles, const uint32_t *tile_row0, const uint32_t *tile_row1,
                                                  const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx,
                                                  const T *__restrict__ val, const T *__restrict__ X,
@@ -102,7 +102,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm(size_t ntiles, const uint32_t *
 	__shared__ double red[kBlock / kWave][NR];
 	const int lane = threadIdx.x % W;
 	const unsigned grp = threadIdx.x / W;
-	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+	// XCD: each XCD walks one contiguous range of tiles, so the X rows that
+	// neighbouring tiles of a system share stay in that XCD's L2
+	const WorkSplit ws = XCD ? xcd_split(ntiles) : WorkSplit{blockIdx.x, ntiles, gridDim.x};
+	for (size_t t = ws.first; t < ws.last; t += ws.step) {
 		// a system whose right-hand sides have all converged is frozen: skip its rows
 		// (the residual product RES always runs: the outer loop reads it for every system)
 		if (!RES && sys_active && !sys_active[tile_sys[t]]) continue;
@@ -176,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm(size_t ntiles, const uint32_t *
 // wait for an index load: slot q of row r multiplies X row r + d_q. Slices not in
 // diagonal form take the CRS loop of k_spmm. The fold (acc += a_k x_k in storage
 // order) and the per-tile partials are those of k_spmm, bit for bit.
-template <typename T, int W, int RPL, bool DOT, typename A = T>
+template <typename T, int W, int RPL, bool DOT, typename A = T, bool XCD = false>
 __global__ __launch_bounds__(kBlock) void k_spmm_dia(size_t ntiles, const uint32_t *tile_row0,
                                                      const uint32_t *tile_row1, const uint32_t *__restrict__ tile_sl0,
                                                      const uint64_t *__restrict__ soff,
@@ -196,7 +199,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_dia(size_t ntiles, const uint32
 	const unsigned grp = threadIdx.x / W;
 	const unsigned wl = threadIdx.x % kWave;
 	const unsigned wave_row = (threadIdx.x / kWave) * kWaveRows;
-	for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+	const WorkSplit ws = XCD ? xcd_split(ntiles) : WorkSplit{blockIdx.x, ntiles, gridDim.x};
+	for (size_t t = ws.first; t < ws.last; t += ws.step) {
 		if (sys_active && !sys_active[tile_sys[t]]) continue;
 		const uint32_t r0 = tile_row0[t], r1 = tile_row1[t], sl0 = tile_sl0[t];
 		double acc_dot[RPL];
@@ -670,7 +674,7 @@ struct alp_hip_batch_s {
 	size_t ntiles = 0;
 	uint32_t *tile_row0 = nullptr, *tile_row1 = nullptr, *tile_sys = nullptr;
 	uint32_t *tile0 = nullptr;        // first tile of each system (nsys + 1)
-	int W = 0;                        // lanes per row of the FP32-data SpMM (autotuned)
+	int W = 0;                        // lanes per row of the FP32-data SpMM, plus kSpmmXcd (autotuned)
 	int W64 = 0;                      // the same for the FP64 SpMM
 	// diagonal form of the rows (slices of <= 64 rows inside each tile); empty when
 	// fewer than half the slices qualify
@@ -692,11 +696,17 @@ struct alp_hip_batch_s {
 
 namespace alp {
 
+// cfg: the lanes per row W (1..16) of the right-hand-side tile, plus kSpmmXcd
+// for the XCD-contiguous tile order
+constexpr int kSpmmXcd = 256;
+
 template <typename T, bool DOT, bool RES, typename A = T>
-static void launch_spmm(alp_hip_batch_s *B, int W, const T *val, const T *X, const T *Bm, T *Y, hipStream_t s,
+static void launch_spmm(alp_hip_batch_s *B, int cfg, const T *val, const T *X, const T *Bm, T *Y, hipStream_t s,
                         const int *sys_active = nullptr) {
 	const size_t grid = B->ntiles < (size_t)fill_blocks(8) ? B->ntiles : fill_blocks(8);
 	const int nr = (int)B->nrhs;
+	const int W = cfg & (kSpmmXcd - 1);
+	const bool xcd = (cfg & kSpmmXcd) != 0;
 	// products (not residuals) of the matrix's own values take the diagonal form
 	// when it was built
 	const T *dval = nullptr;
@@ -704,19 +714,27 @@ static void launch_spmm(alp_hip_batch_s *B, int W, const T *val, const T *X, con
 		if (sizeof(T) == 8 && (const void *)val == (const void *)B->val) dval = (const T *)B->dia.val;
 		if (sizeof(T) == 4 && (const void *)val == (const void *)B->val32) dval = (const T *)B->dia.val32;
 	}
-#define ALP_SPMM(WW, RPL)                                                                                        \
+#define ALP_SPMM_X(WW, RPL, XX)                                                                                  \
 	if (dval)                                                                                                   \
-		hipLaunchKernelGGL((k_spmm_dia<T, WW, RPL, DOT, A>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,         \
+		hipLaunchKernelGGL((k_spmm_dia<T, WW, RPL, DOT, A, XX>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,     \
 		                   (const uint32_t *)B->tile_row0, (const uint32_t *)B->tile_row1,                      \
 		                   (const uint32_t *)B->tile_sl0, (const uint64_t *)B->soff,                            \
 		                   (const uint8_t *)B->dia.flag, (const int32_t *)B->dia.col,                           \
 		                   (const uint32_t *)B->dia.mask, dval, (const uint64_t *)B->off,                       \
 		                   (const uint32_t *)B->idx, val, X, Y, B->part, sys_active, (const uint32_t *)B->tile_sys); \
 	else                                                                                                        \
-		hipLaunchKernelGGL((k_spmm<T, WW, RPL, DOT, RES, A>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,        \
+		hipLaunchKernelGGL((k_spmm<T, WW, RPL, DOT, RES, A, XX>), dim3(grid), dim3(kBlock), 0, s, B->ntiles,    \
 		                   (const uint32_t *)B->tile_row0, (const uint32_t *)B->tile_row1,                      \
 		                   (const uint64_t *)B->off, (const uint32_t *)B->idx, val, X, Bm, Y, B->part,          \
 		                   sys_active, (const uint32_t *)B->tile_sys)
+#define ALP_SPMM(WW, RPL)              \
+	do {                               \
+		if (xcd) {                     \
+			ALP_SPMM_X(WW, RPL, true); \
+		} else {                       \
+			ALP_SPMM_X(WW, RPL, false);\
+		}                              \
+	} while (0)
 	if (nr == 8) {
 		if (W == 8) ALP_SPMM(8, 1);
 		else if (W == 4) ALP_SPMM(4, 2);
@@ -738,6 +756,7 @@ static void launch_spmm(alp_hip_batch_s *B, int W, const T *val, const T *X, con
 		else ALP_SPMM(2, 8);
 	}
 #undef ALP_SPMM
+#undef ALP_SPMM_X
 }
 
 static void free_batch(alp_hip_batch_s *B) {
@@ -837,20 +856,103 @@ ALP_EXPORT alp_rc_t alp_hip_batch_destroy(alp_hip_batch_t B) {
 	return ALP_SUCCESS;
 }
 
+namespace alp {
+
+// The part of a batch's construction that does not depend on where its rows come
+// from: the tiles (kRowsPerTile rows, never straddling two systems), the device
+// arrays and workspace, the FP32 value copy and the diagonal form. B->nsys,
+// nrhs, row0 and nnz are set; fill(st, hc) writes B->off, B->idx (global
+// columns) and B->val on the device and leaves the row offsets in hc (host,
+// nrows + 1 entries).
+template <typename Fill>
+static alp_rc_t batch_assemble(alp_hip_batch_s *B, Fill &&fill) {
+	const size_t nsys = B->nsys, nrhs = B->nrhs, nnz = B->nnz;
+	std::vector<uint32_t> t0(nsys + 1, 0), trow0, trow1, tsys;
+	for (size_t s = 0; s < nsys; ++s) {
+		for (uint64_t r = B->row0[s]; r < B->row0[s + 1]; r += kRowsPerTile) {
+			trow0.push_back((uint32_t)r);
+			trow1.push_back((uint32_t)std::min<uint64_t>(r + kRowsPerTile, B->row0[s + 1]));
+			tsys.push_back((uint32_t)s);
+		}
+		t0[s + 1] = (uint32_t)trow0.size();
+	}
+	B->nrows = B->row0[nsys];
+	B->ntiles = trow0.size();
+	const size_t ntot = B->nrows * nrhs, ninst = nsys * nrhs;
+	hipStream_t st = stream();
+	std::vector<uint32_t> rsys(B->nrows);
+	for (size_t s = 0; s < nsys; ++s)
+		for (uint64_t r = B->row0[s]; r < B->row0[s + 1]; ++r) rsys[r] = (uint32_t)s;
+	alp_rc_t rc = ALP_SUCCESS;
+	if ((rc = bmalloc(&B->off, B->nrows + 1)) || (rc = bmalloc(&B->idx, nnz)) || (rc = bmalloc(&B->val, nnz)) ||
+	    (rc = bmalloc(&B->val32, nnz)) || (rc = bmalloc(&B->row_sys, B->nrows)) ||
+	    (rc = bmalloc(&B->tile_row0, B->ntiles)) || (rc = bmalloc(&B->tile_row1, B->ntiles)) ||
+	    (rc = bmalloc(&B->tile_sys, B->ntiles)) || (rc = bmalloc(&B->tile0, nsys + 1)) ||
+	    (rc = bmalloc(&B->part, B->ntiles * nrhs)) || (rc = bmalloc(&B->inst, ninst)) ||
+	    (rc = bmalloc(&B->nactive, 1)) || (rc = bmalloc(&B->rnorm, ninst)) || (rc = bmalloc(&B->bnorm, ninst)) ||
+	    (rc = bmalloc(&B->outer_active, ninst)) || (rc = bmalloc(&B->sys_active, nsys)) ||
+	    (rc = bmalloc(&B->R, ntot)) || (rc = bmalloc(&B->P, ntot)) || (rc = bmalloc(&B->AP, ntot)) ||
+	    (rc = bmalloc(&B->R32, ntot)) || (rc = bmalloc(&B->P32, ntot)) || (rc = bmalloc(&B->AP32, ntot)) ||
+	    (rc = bmalloc(&B->X32, ntot)))
+		return rc;
+	if (copy_sync(B->row_sys, rsys.data(), 4 * B->nrows, hipMemcpyHostToDevice) != hipSuccess ||
+	    copy_sync(B->tile_row0, trow0.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+	    copy_sync(B->tile_row1, trow1.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+	    copy_sync(B->tile_sys, tsys.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
+	    copy_sync(B->tile0, t0.data(), 4 * (nsys + 1), hipMemcpyHostToDevice) != hipSuccess)
+		return ALP_PANIC;
+	std::vector<uint64_t> hc;
+	if ((rc = fill(st, hc))) return rc;
+	if ((rc = alp_hip_convert_f64_to_f32(nnz, B->val, B->val32))) return rc;
+	// diagonal form: slices of <= 64 rows inside each tile, slot block width =
+	// the slice's longest row (the rows' offsets are at hand on the host)
+	std::vector<uint32_t> srow, tsl0(B->ntiles);
+	for (size_t t = 0; t < B->ntiles; ++t) {
+		tsl0[t] = (uint32_t)srow.size();
+		for (uint64_t r = trow0[t]; r < trow1[t]; r += kSellRows) srow.push_back((uint32_t)r);
+	}
+	B->nslices = srow.size();
+	srow.push_back((uint32_t)B->nrows);
+	std::vector<uint64_t> soff(B->nslices + 1, 0);
+	for (size_t sl = 0; sl < B->nslices; ++sl) {
+		uint64_t w = 0;
+		for (uint64_t r = srow[sl]; r < srow[sl + 1]; ++r) w = std::max<uint64_t>(w, hc[r + 1] - hc[r]);
+		soff[sl + 1] = soff[sl] + w * kSellRows;
+	}
+	if ((rc = bmalloc(&B->srow, B->nslices + 1)) || (rc = bmalloc(&B->soff, B->nslices + 1)) ||
+	    (rc = bmalloc(&B->tile_sl0, B->ntiles)))
+		return rc;
+	if (copy_sync(B->srow, srow.data(), 4 * (B->nslices + 1), hipMemcpyHostToDevice) != hipSuccess ||
+	    copy_sync(B->soff, soff.data(), 8 * (B->nslices + 1), hipMemcpyHostToDevice) != hipSuccess ||
+	    copy_sync(B->tile_sl0, tsl0.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess)
+		return ALP_PANIC;
+	const char *env = std::getenv("ALP_HIP_BATCH_DIA");  // "0": CRS products only (diagnostics)
+	if (!(env && env[0] == '0') &&
+	    (rc = dia_build(B->dia, B->nslices, B->nrows, B->srow, B->soff, soff[B->nslices], B->off, B->idx,
+	                    ALP_VAL_F64, B->val, B->val32)))
+		return rc;
+	if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) return ALP_PANIC;
+	B->W = B->W64 = (int)std::min<size_t>(nrhs, 8);
+	return ALP_SUCCESS;
+}
+
+static bool valid_nrhs(size_t nrhs) { return nrhs == 1 || nrhs == 2 || nrhs == 4 || nrhs == 8 || nrhs == 16; }
+
+}  // namespace alp
+
 ALP_EXPORT alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *out, size_t nsys, const uint32_t *h_nx,
                                                   const uint32_t *h_ny, const uint32_t *h_nz, const double *h_cx,
                                                   const double *h_cy, const double *h_cz, size_t nrhs) {
 	ALP_TRY(ensure_init());
 	if (!out || nsys == 0 || !h_nx || !h_ny || !h_nz || !h_cx || !h_cy || !h_cz) return ALP_ILLEGAL;
 	*out = nullptr;
-	if (!(nrhs == 1 || nrhs == 2 || nrhs == 4 || nrhs == 8 || nrhs == 16)) return ALP_ILLEGAL;
+	if (!valid_nrhs(nrhs)) return ALP_ILLEGAL;
 	auto *B = new (std::nothrow) alp_hip_batch_s();
 	if (!B) return ALP_OUTOFMEM;
 	B->nsys = nsys;
 	B->nrhs = nrhs;
 	B->row0.assign(nsys + 1, 0);
 	std::vector<SysDesc> desc(nsys);
-	std::vector<uint32_t> t0(nsys + 1, 0), trow0, trow1, tsys;
 	uint64_t nnz = 0;
 	alp_rc_t rc = ALP_SUCCESS;
 	for (size_t s = 0; s < nsys; ++s) {
@@ -862,109 +964,95 @@ ALP_EXPORT alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *out, size_t n
 		desc[s] = SysDesc{h_nx[s], h_ny[s], h_nz[s], 0, h_cx[s], h_cy[s], h_cz[s], B->row0[s], nnz};
 		B->row0[s + 1] = B->row0[s] + n;
 		nnz += 7 * n - 2 * ((uint64_t)h_ny[s] * h_nz[s] + (uint64_t)h_nx[s] * h_nz[s] + (uint64_t)h_nx[s] * h_ny[s]);
-		for (uint64_t r = B->row0[s]; r < B->row0[s + 1]; r += kRowsPerTile) {
-			trow0.push_back((uint32_t)r);
-			trow1.push_back((uint32_t)std::min<uint64_t>(r + kRowsPerTile, B->row0[s + 1]));
-			tsys.push_back((uint32_t)s);
-		}
-		t0[s + 1] = (uint32_t)trow0.size();
 	}
 	if (rc == ALP_SUCCESS && B->row0[nsys] >= 0xFFFFFFFFull) rc = ALP_OVERFLW;
 	if (rc) {
 		delete B;
 		return rc;
 	}
-	B->nrows = B->row0[nsys];
 	B->nnz = nnz;
-	B->ntiles = trow0.size();
-	const size_t ntot = B->nrows * nrhs, ninst = nsys * nrhs;
-	hipStream_t st = stream();
-	do {
-		std::vector<uint32_t> rsys(B->nrows);
-		for (size_t s = 0; s < nsys; ++s)
-			for (uint64_t r = B->row0[s]; r < B->row0[s + 1]; ++r) rsys[r] = (uint32_t)s;
-		if ((rc = bmalloc(&B->off, B->nrows + 1)) || (rc = bmalloc(&B->idx, nnz)) || (rc = bmalloc(&B->val, nnz)) ||
-		    (rc = bmalloc(&B->val32, nnz)) || (rc = bmalloc(&B->row_sys, B->nrows)) ||
-		    (rc = bmalloc(&B->sys, nsys)) || (rc = bmalloc(&B->tile_row0, B->ntiles)) ||
-		    (rc = bmalloc(&B->tile_row1, B->ntiles)) || (rc = bmalloc(&B->tile_sys, B->ntiles)) ||
-		    (rc = bmalloc(&B->tile0, nsys + 1)) || (rc = bmalloc(&B->part, B->ntiles * nrhs)) ||
-		    (rc = bmalloc(&B->inst, ninst)) || (rc = bmalloc(&B->nactive, 1)) || (rc = bmalloc(&B->rnorm, ninst)) ||
-		    (rc = bmalloc(&B->bnorm, ninst)) || (rc = bmalloc(&B->outer_active, ninst)) || (rc = bmalloc(&B->sys_active, nsys)) ||
-		    (rc = bmalloc(&B->R, ntot)) || (rc = bmalloc(&B->P, ntot)) || (rc = bmalloc(&B->AP, ntot)) ||
-		    (rc = bmalloc(&B->R32, ntot)) || (rc = bmalloc(&B->P32, ntot)) || (rc = bmalloc(&B->AP32, ntot)) ||
-		    (rc = bmalloc(&B->X32, ntot)))
-			break;
-		if (copy_sync(B->row_sys, rsys.data(), 4 * B->nrows, hipMemcpyHostToDevice) != hipSuccess ||
-		    copy_sync(B->sys, desc.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice) != hipSuccess ||
-		    copy_sync(B->tile_row0, trow0.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
-		    copy_sync(B->tile_row1, trow1.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
-		    copy_sync(B->tile_sys, tsys.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess ||
-		    copy_sync(B->tile0, t0.data(), 4 * (nsys + 1), hipMemcpyHostToDevice) != hipSuccess) {
-			rc = ALP_PANIC;
-			break;
-		}
+	rc = batch_assemble(B, [&](hipStream_t st, std::vector<uint64_t> &hc) -> alp_rc_t {
+		alp_rc_t r = ALP_SUCCESS;
+		if ((r = bmalloc(&B->sys, nsys))) return r;
+		if (copy_sync(B->sys, desc.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice) != hipSuccess)
+			return ALP_PANIC;
 		// offsets: per-row counts then an exclusive scan (host: the counts are cheap to recompute)
 		uint64_t *counts = nullptr;
-		if ((rc = bmalloc(&counts, B->nrows + 1))) break;
+		if ((r = bmalloc(&counts, B->nrows + 1))) return r;
 		hipMemsetAsync(counts + B->nrows, 0, 8, st);
 		hipLaunchKernelGGL(k_lap_counts, dim3(fill_blocks(8)), dim3(kBlock), 0, st, nsys, (const SysDesc *)B->sys,
 		                   (const uint32_t *)B->row_sys, B->nrows, counts);
-		std::vector<uint64_t> hc(B->nrows + 1);
+		hc.assign(B->nrows + 1, 0);
 		if (copy_sync(hc.data(), counts, 8 * (B->nrows + 1), hipMemcpyDeviceToHost) != hipSuccess) {
 			hipFree(counts);
-			rc = ALP_PANIC;
-			break;
+			return ALP_PANIC;
 		}
 		hipFree(counts);
 		uint64_t acc = 0;
-		for (size_t r = 0; r <= B->nrows; ++r) {
-			const uint64_t c = hc[r];
-			hc[r] = acc;
+		for (size_t i = 0; i <= B->nrows; ++i) {
+			const uint64_t c = hc[i];
+			hc[i] = acc;
 			acc += c;
 		}
-		if (acc != nnz || copy_sync(B->off, hc.data(), 8 * (B->nrows + 1), hipMemcpyHostToDevice) != hipSuccess) {
-			rc = ALP_PANIC;
-			break;
-		}
+		if (acc != nnz || copy_sync(B->off, hc.data(), 8 * (B->nrows + 1), hipMemcpyHostToDevice) != hipSuccess)
+			return ALP_PANIC;
 		hipLaunchKernelGGL(k_lap_fill, dim3(fill_blocks(8)), dim3(kBlock), 0, st, B->nrows, (const SysDesc *)B->sys,
 		                   (const uint32_t *)B->row_sys, (const uint64_t *)B->off, B->idx, B->val);
-		if ((rc = alp_hip_convert_f64_to_f32(nnz, B->val, B->val32))) break;
-		// diagonal form: slices of <= 64 rows inside each tile, slot block width =
-		// the slice's longest row (the rows' offsets are at hand on the host)
-		std::vector<uint32_t> srow, tsl0(B->ntiles);
-		for (size_t t = 0; t < B->ntiles; ++t) {
-			tsl0[t] = (uint32_t)srow.size();
-			for (uint64_t r = trow0[t]; r < trow1[t]; r += kSellRows) srow.push_back((uint32_t)r);
-		}
-		B->nslices = srow.size();
-		srow.push_back((uint32_t)B->nrows);
-		std::vector<uint64_t> soff(B->nslices + 1, 0);
-		for (size_t sl = 0; sl < B->nslices; ++sl) {
-			uint64_t w = 0;
-			for (uint64_t r = srow[sl]; r < srow[sl + 1]; ++r) w = std::max<uint64_t>(w, hc[r + 1] - hc[r]);
-			soff[sl + 1] = soff[sl] + w * kSellRows;
-		}
-		if ((rc = bmalloc(&B->srow, B->nslices + 1)) || (rc = bmalloc(&B->soff, B->nslices + 1)) ||
-		    (rc = bmalloc(&B->tile_sl0, B->ntiles)))
-			break;
-		if (copy_sync(B->srow, srow.data(), 4 * (B->nslices + 1), hipMemcpyHostToDevice) != hipSuccess ||
-		    copy_sync(B->soff, soff.data(), 8 * (B->nslices + 1), hipMemcpyHostToDevice) != hipSuccess ||
-		    copy_sync(B->tile_sl0, tsl0.data(), 4 * B->ntiles, hipMemcpyHostToDevice) != hipSuccess) {
-			rc = ALP_PANIC;
-			break;
-		}
-		const char *env = std::getenv("ALP_HIP_BATCH_DIA");  // "0": CRS products only (diagnostics)
-		if (!(env && env[0] == '0') &&
-		    (rc = dia_build(B->dia, B->nslices, B->nrows, B->srow, B->soff, soff[B->nslices], B->off, B->idx,
-		                    ALP_VAL_F64, B->val, B->val32)))
-			break;
-		if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) rc = ALP_PANIC;
-	} while (0);
+		return hipGetLastError() == hipSuccess ? ALP_SUCCESS : ALP_PANIC;
+	});
 	if (rc) {
 		alp_hip_batch_destroy(B);
 		return rc;
 	}
-	B->W = B->W64 = (int)std::min<size_t>(nrhs, 8);
+	*out = B;
+	return ALP_SUCCESS;
+}
+
+ALP_EXPORT alp_rc_t alp_hip_batch_create_crs(alp_hip_batch_t *out, size_t nsys, const uint64_t *h_row0,
+                                             const uint64_t *h_off, const uint32_t *h_idx, const double *h_val,
+                                             size_t nrhs) {
+	ALP_TRY(ensure_init());
+	if (!out || nsys == 0 || !h_row0 || !h_off) return ALP_ILLEGAL;
+	*out = nullptr;
+	if (!valid_nrhs(nrhs)) return ALP_ILLEGAL;
+	// the stacked CRS: rows ascending by system, offsets monotone from 0, each
+	// column inside its row's own system
+	if (h_row0[0] != 0) return ALP_ILLEGAL;
+	for (size_t s = 0; s < nsys; ++s)
+		if (h_row0[s + 1] <= h_row0[s]) return ALP_ILLEGAL;  // every system has rows
+	const uint64_t nrows = h_row0[nsys];
+	if (nrows >= 0xFFFFFFFFull) return ALP_OVERFLW;
+	if (h_off[0] != 0) return ALP_ILLEGAL;
+	for (uint64_t r = 0; r < nrows; ++r)
+		if (h_off[r + 1] < h_off[r]) return ALP_ILLEGAL;
+	const uint64_t nnz = h_off[nrows];
+	if (nnz && (!h_idx || !h_val)) return ALP_ILLEGAL;
+	std::vector<uint32_t> gidx(nnz);
+	for (size_t s = 0; s < nsys; ++s) {
+		const uint64_t n = h_row0[s + 1] - h_row0[s];
+		for (uint64_t k = h_off[h_row0[s]]; k < h_off[h_row0[s + 1]]; ++k) {
+			if (h_idx[k] >= n) return ALP_MISMATCH;
+			gidx[k] = (uint32_t)(h_row0[s] + h_idx[k]);
+		}
+	}
+	auto *B = new (std::nothrow) alp_hip_batch_s();
+	if (!B) return ALP_OUTOFMEM;
+	B->nsys = nsys;
+	B->nrhs = nrhs;
+	B->row0.assign(h_row0, h_row0 + nsys + 1);
+	B->nnz = nnz;
+	alp_rc_t rc = batch_assemble(B, [&](hipStream_t, std::vector<uint64_t> &hc) -> alp_rc_t {
+		hc.assign(h_off, h_off + nrows + 1);
+		if (copy_sync(B->off, h_off, 8 * (nrows + 1), hipMemcpyHostToDevice) != hipSuccess ||
+		    (nnz && copy_sync(B->idx, gidx.data(), 4 * nnz, hipMemcpyHostToDevice) != hipSuccess) ||
+		    (nnz && copy_sync(B->val, h_val, 8 * nnz, hipMemcpyHostToDevice) != hipSuccess))
+			return ALP_PANIC;
+		return ALP_SUCCESS;
+	});
+	if (rc) {
+		alp_hip_batch_destroy(B);
+		return rc;
+	}
 	*out = B;
 	return ALP_SUCCESS;
 }
@@ -1008,39 +1096,48 @@ ALP_EXPORT alp_rc_t alp_hip_batch_autotune(alp_hip_batch_t B, int *width) {
 	// defined operands (the work vectors hold whatever the last solve left)
 	ALP_HIP_TRY(hipMemsetAsync(B->P, 0, sizeof(double) * ntot, s));
 	ALP_HIP_TRY(hipMemsetAsync(B->P32, 0, sizeof(float) * ntot, s));
-	hipEvent_t e0, e1;
-	ALP_HIP_TRY(hipEventCreate(&e0));
-	ALP_HIP_TRY(hipEventCreate(&e1));
+	struct Events {
+		hipEvent_t e0 = nullptr, e1 = nullptr;
+		~Events() {
+			if (e0) hipEventDestroy(e0);
+			if (e1) hipEventDestroy(e1);
+		}
+	} ev;
+	ALP_HIP_TRY(hipEventCreate(&ev.e0));
+	ALP_HIP_TRY(hipEventCreate(&ev.e1));
 	const bool verbose = std::getenv("ALP_HIP_AUTOTUNE_VERBOSE") != nullptr;
+	int picks[2] = {B->W64, B->W};
 	for (int prec = 0; prec < 2; ++prec) {
 		float best = 1e30f;
-		int pick = prec ? B->W : B->W64;
-		for (int W : {1, 2, 4, 8, 16}) {
-			if ((size_t)W > B->nrhs) continue;
-			auto run = [&] {
-				if (prec)
-					launch_spmm<float, true, false, double>(B, W, B->val32, B->P32, nullptr, B->AP32, s);
-				else
-					launch_spmm<double, true, false>(B, W, B->val, B->P, nullptr, B->AP, s);
-			};
-			for (int i = 0; i < 2; ++i) run();
-			ALP_HIP_TRY(hipEventRecord(e0, s));
-			for (int i = 0; i < 5; ++i) run();
-			ALP_HIP_TRY(hipEventRecord(e1, s));
-			ALP_HIP_TRY(hipEventSynchronize(e1));
-			float ms = 0;
-			ALP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-			if (verbose) std::fprintf(stderr, "[alp_hip] batch spmm %s W=%d: %.1f us\n", prec ? "f32" : "f64", W, ms * 200.0f);
-			if (ms < best) {
-				best = ms;
-				pick = W;
+		for (int xcd : {0, kSpmmXcd})
+			for (int W : {1, 2, 4, 8, 16}) {
+				if ((size_t)W > B->nrhs) continue;
+				const int cfg = W | xcd;
+				auto run = [&] {
+					if (prec)
+						launch_spmm<float, true, false, double>(B, cfg, B->val32, B->P32, nullptr, B->AP32, s);
+					else
+						launch_spmm<double, true, false>(B, cfg, B->val, B->P, nullptr, B->AP, s);
+				};
+				for (int i = 0; i < 2; ++i) run();
+				ALP_HIP_TRY(hipEventRecord(ev.e0, s));
+				for (int i = 0; i < 5; ++i) run();
+				ALP_HIP_TRY(hipEventRecord(ev.e1, s));
+				ALP_HIP_TRY(hipEventSynchronize(ev.e1));
+				float ms = 0;
+				ALP_HIP_TRY(hipEventElapsedTime(&ms, ev.e0, ev.e1));
+				if (verbose)
+					std::fprintf(stderr, "[alp_hip] batch spmm %s W=%d%s: %.1f us\n", prec ? "f32" : "f64", W,
+					             xcd ? " xcd" : "", ms * 200.0f);
+				if (ms < best) {
+					best = ms;
+					picks[prec] = cfg;
+				}
 			}
-		}
-		(prec ? B->W : B->W64) = pick;
 	}
-	hipEventDestroy(e0);
-	hipEventDestroy(e1);
-	if (width) *width = B->W64;
+	B->W64 = picks[0];
+	B->W = picks[1];
+	if (width) *width = B->W64 & (kSpmmXcd - 1);
 	return ALP_SUCCESS;
 }
 

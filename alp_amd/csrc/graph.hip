@@ -20,6 +20,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace alp {
@@ -201,11 +202,13 @@ struct PrState {
 // the partials regions of one iteration: dangling mass, then the L1 residual
 constexpr unsigned kPrRegion = kMaxPartials / 2;
 
-// row_sum[i] = alpha / out-degree (0 for dangling rows), pr = 1/n (:206-297)
-__global__ __launch_bounds__(kBlock) void k_pr_init(size_t n, const uint64_t *off, double alpha, double *rs,
-                                                    double *pr) {
+// row_sum[i] = alpha / out-degree (0 for dangling rows), pr = 1/n (:206-297);
+// entry i is vertex perm[i] (perm null: vertex i)
+__global__ __launch_bounds__(kBlock) void k_pr_init(size_t n, const uint64_t *off, const uint32_t *perm,
+                                                    double alpha, double *rs, double *pr) {
 	grid_for_each<4>(n, [&](size_t i) {
-		const double deg = (double)(off[i + 1] - off[i]);
+		const size_t v = perm ? perm[i] : i;
+		const double deg = (double)(off[v + 1] - off[v]);
 		rs[i] = deg > 0.0 ? alpha / deg : 0.0;
 		pr[i] = 1.0 / (double)n;
 	});
@@ -269,6 +272,58 @@ __global__ __launch_bounds__(kBlock) void k_pr_fin(const double *partials, unsig
 		o.done = (conv || o.iter >= st->max_iter) ? 1 : 0;
 		*out = o;
 	}
+}
+
+// ---- PageRank's gather in degree order -------------------------------------------------
+//
+// Each nonzero of the product gathers the scaled rank of one vertex, and vertex
+// j is gathered once per out-edge. On power-law graphs a few percent of the
+// vertices take most of the gathers, but with their original numbers they are
+// spread over the whole 8n-byte vector, one cache line each. Numbering the
+// vertices by out-degree, highest first, packs that hot set into the front of
+// the vector, where it stays in each XCD's L2. Row i' of the relabelled pattern
+// is the gather row of vertex perm[i'] with every entry j renamed inv[j] and
+// kept in its storage position, so each row folds the same values in the same
+// order as before; only the cross-vertex reductions (dangling mass, residual)
+// see another order.
+
+// keys for a stable ascending sort that lists vertices by out-degree, highest
+// first, ties by vertex number
+__global__ __launch_bounds__(kBlock) void k_degree_keys(size_t n, const uint64_t *off, uint32_t *key,
+                                                        uint32_t *id) {
+	grid_for_each<4>(n, [&](size_t i) {
+		const uint64_t d = off[i + 1] - off[i];
+		key[i] = 0xFFFFFFFFu - (uint32_t)(d < 0xFFFFFFFFull ? d : 0xFFFFFFFFull);
+		id[i] = (uint32_t)i;
+	});
+}
+
+__global__ __launch_bounds__(kBlock) void k_perm_inverse(size_t n, const uint32_t *perm, uint32_t *inv,
+                                                         const uint64_t *src_off, uint64_t *len) {
+	grid_for_each<4>(n, [&](size_t i) {
+		const uint32_t v = perm[i];
+		inv[v] = (uint32_t)i;
+		len[i] = src_off[v + 1] - src_off[v];
+	});
+	if (blockIdx.x == 0 && threadIdx.x == 0) len[n] = 0;
+}
+
+// one wavefront per relabelled row copies the source row, renaming its entries
+__global__ __launch_bounds__(kBlock) void k_perm_fill(size_t n, const uint32_t *perm, const uint32_t *inv,
+                                                      const uint64_t *src_off, const uint32_t *src_idx,
+                                                      const uint64_t *off, uint32_t *idx) {
+	const size_t waves = (size_t)gridDim.x * (kBlock / kWave);
+	const unsigned lane = threadIdx.x % kWave;
+	for (size_t r = (size_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; r < n; r += waves) {
+		const uint64_t s0 = src_off[perm[r]], len = off[r + 1] - off[r], d0 = off[r];
+		for (uint64_t t = lane; t < len; t += kWave) idx[d0 + t] = inv[src_idx[s0 + t]];
+	}
+}
+
+// rank vector back to the caller's numbering
+__global__ __launch_bounds__(kBlock) void k_pr_unpermute(size_t n, const uint32_t *perm, const double *prp,
+                                                         double *pr) {
+	grid_for_each<4>(n, [&](size_t i) { pr[perm[i]] = prp[i]; });
 }
 
 __global__ __launch_bounds__(kBlock) void k_u8_single(size_t n, uint8_t *x, uint32_t at) {
@@ -414,6 +469,13 @@ ALP_EXPORT alp_rc_t alp_hip_graph_from_edges(alp_hip_matrix_t *A, size_t n, size
 		}
 		M->nnz = nnz;
 		if ((rc = keys_to_storage(M->crs, n, n, keys.p, nnz))) break;
+		if (sym) {
+			// a symmetrised pattern's column j lists the rows of row j's columns,
+			// ascending as the row lists them: the CCS is the CRS
+			M->ccs = M->crs;
+			M->ccs_alias = true;
+			break;
+		}
 		DevBuf<uint64_t> tkeys;
 		size_t tnnz = 0;
 		if ((rc = edges_to_keys(nedges, d_src, d_dst, sym, loops, true, tkeys, &tnnz))) break;
@@ -474,6 +536,12 @@ struct BfsWork {
 	size_t scan_tb = 0;
 	DevBuf<double> rs, pn, pnn;  // PageRank: alpha / out-degree, pr * rs, the product
 	DevBuf<PrState> pr_state;
+	// PageRank's gather pattern in degree order, its order (perm[i'] = vertex)
+	// and the ranks in that order; made on the first call
+	Storage pr_pat;
+	DevBuf<uint32_t> pr_perm;
+	DevBuf<double> pr_x;
+	~BfsWork() { storage_free(pr_pat); }
 };
 
 static void free_bfs_work(void *p) { delete static_cast<BfsWork *>(p); }
@@ -604,6 +672,79 @@ __global__ __launch_bounds__(kBlock) void k_within(size_t n, const int32_t *leve
 	});
 }
 
+// The gather pattern of the product pnn = pn L (the CCS, or the CRS of a
+// symmetrised pattern) renumbered by out-degree, highest first. Returns
+// SUCCESS with nothing built (perm stays null) when n is beyond the sort's
+// 31-bit item count; the product then gathers in the original numbering.
+static alp_rc_t pagerank_layout(alp_hip_matrix_s *L, BfsWork *w) {
+	const size_t n = L->nrows;
+	if (w->pr_perm.p || n > 0x7FFFFFFFull) return ALP_SUCCESS;
+	const char *env = std::getenv("ALP_HIP_PR_ORDER");  // "0": gather in the original numbering
+	if (env && env[0] == '0') return ALP_SUCCESS;
+	hipStream_t s = stream();
+	const Storage &G = L->ccs_alias || L->symmetric_pattern ? L->crs : L->ccs;
+	DevBuf<uint32_t> key, skey, id, perm, inv;
+	ALP_TRY(key.alloc(n));
+	ALP_TRY(skey.alloc(n));
+	ALP_TRY(id.alloc(n));
+	ALP_TRY(perm.alloc(n));
+	ALP_TRY(inv.alloc(n));
+	hipLaunchKernelGGL(k_degree_keys, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint64_t *)L->crs.off, key.p,
+	                   id.p);
+	ALP_LAUNCH_CHECK();
+	size_t tb = 0;
+	ALP_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key.p, skey.p, id.p, perm.p, (int)n, 0, 32, s));
+	DevBuf<char> tmp;
+	ALP_TRY(tmp.alloc(tb));
+	ALP_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key.p, skey.p, id.p, perm.p, (int)n, 0, 32, s));
+	DevBuf<uint64_t> len;
+	ALP_TRY(len.alloc(n + 1));
+	hipLaunchKernelGGL(k_perm_inverse, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint32_t *)perm.p, inv.p,
+	                   (const uint64_t *)G.off, len.p);
+	ALP_LAUNCH_CHECK();
+	Storage P;
+	P.nmajor = P.nminor = n;
+	alp_rc_t rc = ALP_SUCCESS;
+	do {
+		if (hipMalloc(&P.off, sizeof(uint64_t) * (n + 1)) != hipSuccess ||
+		    hipMalloc(&P.idx, sizeof(uint32_t) * (L->nnz ? L->nnz : 1)) != hipSuccess) {
+			rc = ALP_OUTOFMEM;
+			break;
+		}
+		size_t sb = 0;
+		if (hipcub::DeviceScan::ExclusiveSum(nullptr, sb, len.p, P.off, n + 1, s) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		DevBuf<char> stmp;
+		if ((rc = stmp.alloc(sb))) break;
+		if (hipcub::DeviceScan::ExclusiveSum(stmp.p, sb, len.p, P.off, n + 1, s) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		if (L->nnz)
+			hipLaunchKernelGGL(k_perm_fill, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint32_t *)perm.p,
+			                   (const uint32_t *)inv.p, (const uint64_t *)G.off, (const uint32_t *)G.idx,
+			                   (const uint64_t *)P.off, P.idx);
+		if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+			rc = ALP_PANIC;
+			break;
+		}
+		rc = storage_finalize(P, ALP_VAL_PATTERN, false);
+	} while (0);
+	if (rc) {
+		storage_free(P);
+		return rc;
+	}
+	if ((rc = w->pr_x.alloc(n))) {
+		storage_free(P);
+		return rc;
+	}
+	w->pr_pat = P;
+	std::swap(w->pr_perm.p, perm.p);
+	return ALP_SUCCESS;
+}
+
 ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double alpha, double conv, size_t max_iter,
                                      size_t *iters, double *quality) {
 	ALP_TRY(ensure_init());
@@ -628,6 +769,12 @@ ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double al
 			return rc;
 		}
 	}
+	ALP_TRY(pagerank_layout(L, w));
+	// the iteration runs in degree order when the layout exists: the ranks in
+	// pr_x, vertex perm[i] at i; the caller's vector gets them back at the end
+	const uint32_t *perm = w->pr_perm.p;
+	const Storage &P = perm ? w->pr_pat : L->ccs;
+	double *pr = perm ? w->pr_x.p : d_pr;
 	DevBuf<double> &rs = w->rs, &pn = w->pn, &pnn = w->pnn;
 	DevBuf<PrState> &st = w->pr_state;
 	PrState h{};
@@ -636,8 +783,8 @@ ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double al
 	h.conv = conv;
 	h.max_iter = (unsigned long long)max_iter;
 	ALP_HIP_TRY(copy_sync(st.p, &h, sizeof(h), hipMemcpyHostToDevice));
-	hipLaunchKernelGGL(k_pr_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint64_t *)L->crs.off, alpha,
-	                   rs.p, d_pr);
+	hipLaunchKernelGGL(k_pr_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint64_t *)L->crs.off, perm, alpha,
+	                   rs.p, pr);
 	ALP_LAUNCH_CHECK();
 	unsigned rg = g_rgrid(n);
 	if (rg > kPrRegion) rg = kPrRegion;
@@ -645,12 +792,12 @@ ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double al
 	for (;;) {
 		for (int k = 0; k < 8; ++k) {  // even: every check reads slot 0
 			const PrScal *in = &st.p->s[k % 2];
-			hipLaunchKernelGGL(k_pr_scale, dim3(rg), dim3(kBlock), 0, s, n, (const double *)d_pr, (const double *)rs.p,
+			hipLaunchKernelGGL(k_pr_scale, dim3(rg), dim3(kBlock), 0, s, n, (const double *)pr, (const double *)rs.p,
 			                   pn.p, part, in);
 			// pr_nextnext = pr_next L: the CCS gather of vxm (blas2.hpp:1432-1466)
-			ALP_TRY(spmv(L->ccs, L->value_type, ALP_PLUS_PATTERN_F64, ALP_DESC_OVERWRITE, pnn.p, pn.p, nullptr, s,
+			ALP_TRY(spmv(P, L->value_type, ALP_PLUS_PATTERN_F64, ALP_DESC_OVERWRITE, pnn.p, pn.p, nullptr, s,
 			             &in->done));
-			hipLaunchKernelGGL(k_pr_update, dim3(rg), dim3(kBlock), 0, s, n, d_pr, (const double *)pnn.p, part, rg, in,
+			hipLaunchKernelGGL(k_pr_update, dim3(rg), dim3(kBlock), 0, s, n, pr, (const double *)pnn.p, part, rg, in,
 			                   (const PrState *)st.p);
 			hipLaunchKernelGGL(k_pr_fin, dim3(1), dim3(kBlock), 0, s, (const double *)part, rg, in,
 			                   &st.p->s[1 - k % 2], (const PrState *)st.p);
@@ -659,6 +806,11 @@ ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double al
 		ALP_HIP_TRY(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
 		ALP_HIP_TRY(hipStreamSynchronize(s));
 		if (h.s[0].done) break;
+	}
+	if (perm) {
+		hipLaunchKernelGGL(k_pr_unpermute, dim3(g_grid(n)), dim3(kBlock), 0, s, n, perm, (const double *)pr, d_pr);
+		ALP_LAUNCH_CHECK();
+		ALP_HIP_TRY(hipStreamSynchronize(s));
 	}
 	if (iters) *iters = h.s[0].iter;
 	if (quality) *quality = h.s[0].residual;

@@ -477,7 +477,20 @@ static alp_rc_t build_sell(Storage &S, int value_type, uint64_t nnz) {
 	S.sell_val32 = sval32;
 	S.n_slices = ns;
 	S.sell_entries = total;
-	return dia_build(S.dia, ns, S.nmajor, nullptr, soff, total, S.off, S.idx, value_type, S.val, S.val32);
+	ALP_TRY(dia_build(S.dia, ns, S.nmajor, nullptr, soff, total, S.off, S.idx, value_type, S.val, S.val32));
+	const char *keep = std::getenv("ALP_HIP_KEEP_SELL");  // "1": keep them (tests of the sliced-ELL path)
+	if (S.dia.flag && S.dia.n_dia == ns && !(keep && keep[0] == '1')) {
+		// every slice has the diagonal form, which reads neither the column
+		// indices nor the sliced-ELL values: keep only the slice offsets
+		ALP_HIP_TRY(hipStreamSynchronize(stream()));
+		hipFree(S.sell_idx);
+		if (S.sell_val) hipFree(S.sell_val);
+		if (S.sell_val32) hipFree(S.sell_val32);
+		S.sell_idx = nullptr;
+		S.sell_val = nullptr;
+		S.sell_val32 = nullptr;
+	}
+	return ALP_SUCCESS;
 }
 
 // Copies S's current CRS values (and FP32 copy) into its sliced-ELL copy, after a
@@ -485,7 +498,9 @@ static alp_rc_t build_sell(Storage &S, int value_type, uint64_t nnz) {
 alp_rc_t storage_refresh_values(Storage &S, int value_type) {
 	if (!S.sell_off || S.n_slices == 0) return ALP_SUCCESS;
 	const unsigned g = stream_grid(S.n_slices * kSellRows);
-	if (value_type == ALP_VAL_F32)
+	if (!S.sell_idx) {
+		// diagonal slices only (no sliced-ELL arrays)
+	} else if (value_type == ALP_VAL_F32)
 		hipLaunchKernelGGL(k_sell_fill<float>, dim3(g), dim3(kBlock), 0, stream(), S.nmajor, S.n_slices,
 		                   (const uint64_t *)S.off, (const uint32_t *)S.idx, (const float *)S.val,
 		                   (const float *)S.val32, (const uint64_t *)S.sell_off, S.sell_idx, (float *)S.sell_val,
@@ -507,7 +522,7 @@ alp_rc_t storage_ensure_f32(Storage &S, size_t nnz) {
 		hipLaunchKernelGGL((k_convert<double, float>), dim3(stream_grid(nnz)), dim3(kBlock), 0, stream(), nnz,
 		                   (const double *)S.val, S.val32);
 	ALP_LAUNCH_CHECK();
-	if (S.sell_off && S.sell_entries) {
+	if (S.sell_off && S.sell_idx && S.sell_entries) {
 		if (hipMalloc(&S.sell_val32, sizeof(float) * S.sell_entries) != hipSuccess) {
 			(void)hipGetLastError();
 			S.sell_val32 = nullptr;  // the FP32 products then use the CRS kernels
@@ -1015,8 +1030,8 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_destroy(alp_hip_matrix_t A) {
 	hipStreamSynchronize(stream());
 	if (A->free_solver_cache) A->free_solver_cache(A->solver_cache);
 	if (A->free_graph_cache) A->free_graph_cache(A->graph_cache);
+	if (!A->ccs_alias) storage_free(A->ccs);
 	storage_free(A->crs);
-	storage_free(A->ccs);
 	delete A;
 	return ALP_SUCCESS;
 }
@@ -1211,7 +1226,8 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_storage(alp_hip_matrix_t A, int which, const 
 
 ALP_EXPORT alp_rc_t alp_hip_matrix_device_bytes(alp_hip_matrix_t A, size_t *bytes) {
 	if (!A || !bytes) return ALP_ILLEGAL;
-	*bytes = storage_bytes(A->crs, A->value_type, A->nnz) + storage_bytes(A->ccs, A->value_type, A->nnz);
+	*bytes = storage_bytes(A->crs, A->value_type, A->nnz) +
+	         (A->ccs_alias ? 0 : storage_bytes(A->ccs, A->value_type, A->nnz));
 	return ALP_SUCCESS;
 }
 

@@ -910,7 +910,16 @@ static void dispatch(const Storage &S, const void *val, unsigned descr, void *y,
 		dia = dia_values(S, val, &dval);
 		kernel = 4;
 	}
-	if (kernel == 4 && !sell_values(S, val, &sval)) kernel = 2;  // no sliced-ELL copy of these values
+	if (kernel == 4) {
+		if (S.sell_off && !S.sell_idx) {
+			// every slice is diagonal and no sliced-ELL arrays are kept: the slice
+			// kernel runs on the diagonal form (the same fold order)
+			dia = dia_values(S, val, &dval);
+			if (!dia) kernel = 2;
+		} else if (!sell_values(S, val, &sval)) {
+			kernel = 2;  // no sliced-ELL copy of these values: the CRS kernels
+		}
+	}
 	if (kernel != 4) dia = false;
 	if (RR == 0) RR = kernel == 2 ? 8 : kernel == 4 ? 8 : 1;
 #define ALP_GO(MM, OWV)                                                 \
@@ -940,8 +949,13 @@ static bool sell_dot_launch(const Storage &S, const void *val, const void *x, vo
 	const int k = S.kernel ? S.kernel : auto_kernel(S);
 	if (k != 4 && k != 5) return false;
 	const void *sval = nullptr, *dval = nullptr;
-	if (!sell_values(S, val, &sval)) return false;
-	const bool dia = k == 5 && dia_values(S, val, &dval);
+	bool dia = k == 5 && dia_values(S, val, &dval);
+	if (S.sell_off && !S.sell_idx) {  // diagonal slices only
+		dia = dia_values(S, val, &dval);
+		if (!dia) return false;
+	} else if (!sell_values(S, val, &sval)) {
+		return false;
+	}
 	// one block per kBlock / kSellRows slices, as k_spmv_sell: measured, a grid
 	// stride over more slices streams worse than the separate product and dot
 	// (192^3: +2% per CG iteration), so larger matrices keep those
@@ -1154,7 +1168,7 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 	for (int t : {8, 16})
 		for (int v : {1, 2, 4, 8, 16, 32}) cands.push_back({2, t, v});
 	cands.push_back({3, 0, 0});
-	if (A->crs.sell_off)
+	if (A->crs.sell_idx)
 		for (int xcd : {0, 1})
 			for (int u : {4, 8, 16}) cands.push_back({4, xcd, u});
 	if (A->crs.dia.flag)

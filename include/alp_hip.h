@@ -346,6 +346,17 @@ alp_rc_t alp_hip_batch_create_laplace7(alp_hip_batch_t *B, size_t nsys, const ui
                                        const uint32_t *h_ny, const uint32_t *h_nz,
                                        const double *h_cx, const double *h_cy, const double *h_cz,
                                        size_t nrhs);
+/* nsys caller-supplied systems as one stacked CRS (host arrays): system s owns
+ * rows [h_row0[s], h_row0[s+1]) (h_row0[0] = 0, every system non-empty);
+ * h_off (h_row0[nsys] + 1 offsets) indexes the stacked rows, and h_idx[k] is the
+ * column of entry k inside its own system (< the system's row count), h_val its
+ * value. Each system must be symmetric positive definite (CG's requirement; not
+ * checked). ALP_MISMATCH for a column outside its system, ALP_ILLEGAL for
+ * malformed offsets. Each system is what one grb::Matrix<double> would hold for
+ * a conjugate_gradient call (conjugate_gradient.hpp:191-484). */
+alp_rc_t alp_hip_batch_create_crs(alp_hip_batch_t *B, size_t nsys, const uint64_t *h_row0,
+                                  const uint64_t *h_off, const uint32_t *h_idx, const double *h_val,
+                                  size_t nrhs);
 alp_rc_t alp_hip_batch_destroy(alp_hip_batch_t B);
 /* total rows, nonzeros, and (h_row0, nsys + 1 entries, may be NULL) each system's first row */
 alp_rc_t alp_hip_batch_info(alp_hip_batch_t B, size_t *nrows, size_t *nnz, uint64_t *h_row0);

@@ -138,3 +138,96 @@ def test_batch_reliable_updates_stop_at_the_iteration_limit(A):
     assert rc == A.FAILED
     assert its.max() == 6
     assert np.all(np.isfinite(rel)) and np.all(rel > 1e-12)
+
+
+def random_spd(n, deg, rng):
+    """Uneven SPD system: a random symmetric pattern with weights in (-1, 0] and a
+    diagonal of the row's absolute sum plus a random shift in [0.01, 1) (strictly
+    diagonally dominant), columns ascending in each row."""
+    import scipy.sparse as sp
+    i = rng.integers(0, n, n * deg)
+    j = rng.integers(0, n, n * deg)
+    keep = i != j
+    w = -rng.random(keep.sum())
+    M = sp.coo_matrix((w, (i[keep], j[keep])), shape=(n, n)).tocsr()
+    M = M + M.T
+    M = M + sp.diags(-np.asarray(M.sum(axis=1)).ravel() + rng.uniform(0.01, 1.0, n))
+    M = M.tocsr()
+    M.sum_duplicates()
+    M.sort_indices()
+    return M
+
+
+def test_crs_batch_equals_generated_laplacians_bit_for_bit(A):
+    """The same anisotropic Laplacians given as caller CRS (columns ascending, as
+    the generator stores them) give the generator's batch: identical products and
+    solves, bit for bit, in both precisions."""
+    rng = np.random.default_rng(5)
+    G, p = make_batch(A, rng, 6, 8, 24)
+    mats = []
+    for s in range(G.nsys):
+        M = laplace7(*(int(v[s]) for v in p[:3]), *(float(v[s]) for v in p[3:]))
+        M.sort_indices()
+        mats.append((M.indptr, M.indices, M.data))
+    C = A.Batch.from_crs(mats, 8)
+    assert C.nrows == G.nrows and C.nnz == G.nnz and np.array_equal(C.row0, G.row0)
+    assert C.layout() == G.layout()
+    X = A.Vector.from_numpy(rng.standard_normal(G.nrows * 8))
+    Yg, Yc = A.Vector(G.nrows * 8), A.Vector(G.nrows * 8)
+    G.spmm(X, Yg)
+    C.spmm(X, Yc)
+    assert np.array_equal(Yg.numpy(), Yc.numpy())
+    rhs = A.Vector.from_numpy(rng.standard_normal(G.nrows * 8))
+    for precision in (0, 1):
+        Xg, Xc = A.Vector(G.nrows * 8, fill=0.0), A.Vector(G.nrows * 8, fill=0.0)
+        rg = G.solve(rhs, Xg, 1e-8, 5000, precision)
+        rc = C.solve(rhs, Xc, 1e-8, 5000, precision)
+        assert rg[0] == rc[0] == A.SUCCESS
+        assert np.array_equal(rg[1], rc[1]) and np.array_equal(Xg.numpy(), Xc.numpy()), precision
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_uneven_crs_systems_against_direct_solves(A, precision):
+    """Caller-supplied systems of uneven size and row length (random SPD patterns,
+    40 to 3000 rows, 1 to 40 entries per row): every instance reaches the FP64
+    relative residual 1e-8 and agrees with scipy's direct solve."""
+    import scipy.sparse.linalg as spla
+    rng = np.random.default_rng(100 + precision)
+    sizes = [40, 3000, 517, 1200, 64, 2049, 255, 900]
+    mats = [random_spd(n, int(rng.integers(1, 20)), rng) for n in sizes]
+    B = A.Batch.from_crs([(M.indptr, M.indices, M.data) for M in mats], 8)
+    assert B.nsys == len(sizes) and B.nrows == sum(sizes)
+    assert B.autotune() in (1, 2, 4, 8)
+    rhs = rng.standard_normal((B.nrows, 8))
+    X = A.Vector(B.nrows * 8, fill=0.0)
+    rc, its, rel, outer = B.solve(A.Vector.from_numpy(rhs.ravel()), X, 1e-8, 20000, precision)
+    assert rc == A.SUCCESS
+    assert np.all(rel <= 1e-8)
+    x = X.numpy().reshape(B.nrows, 8)
+    for s, M in enumerate(mats):
+        r0, r1 = int(B.row0[s]), int(B.row0[s + 1])
+        res = np.linalg.norm(rhs[r0:r1] - M @ x[r0:r1], axis=0) / np.linalg.norm(rhs[r0:r1], axis=0)
+        assert np.all(res <= 1.01e-8)
+        xs = spla.spsolve(M.tocsc(), rhs[r0:r1])
+        assert np.max(np.abs(x[r0:r1] - xs)) <= 1e-6 * np.max(np.abs(xs))
+
+
+def test_crs_batch_rejects_malformed_input(A):
+    import ctypes
+    lib = A.lib()
+    h = ctypes.c_void_p()
+    row0 = np.array([0, 2], np.uint64)
+    off = np.array([0, 1, 2], np.uint64)
+    val = np.array([1.0, 1.0])
+    bad_col = np.array([0, 2], np.uint32)  # column 2 outside a 2-row system
+    assert lib.alp_hip_batch_create_crs(ctypes.byref(h), 1, row0.ctypes.data, off.ctypes.data,
+                                        bad_col.ctypes.data, val.ctypes.data, 8) == A.MISMATCH
+    ok = np.array([0, 1], np.uint32)
+    bad_off = np.array([0, 2, 1], np.uint64)
+    assert lib.alp_hip_batch_create_crs(ctypes.byref(h), 1, row0.ctypes.data, bad_off.ctypes.data,
+                                        ok.ctypes.data, val.ctypes.data, 8) == A.ILLEGAL
+    assert lib.alp_hip_batch_create_crs(ctypes.byref(h), 1, row0.ctypes.data, off.ctypes.data,
+                                        ok.ctypes.data, val.ctypes.data, 3) == A.ILLEGAL  # nrhs
+    empty = np.array([0, 0], np.uint64)
+    assert lib.alp_hip_batch_create_crs(ctypes.byref(h), 1, empty.ctypes.data, off.ctypes.data,
+                                        ok.ctypes.data, val.ctypes.data, 8) == A.ILLEGAL

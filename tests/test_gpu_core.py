@@ -147,11 +147,13 @@ def test_mxv_matches_reference_fixture(A, golden, name, sizes, kernel, lanes, ro
 @pytest.mark.parametrize("kernel,lanes,rows", SELL + DIA)
 @pytest.mark.parametrize("name,sizes", [("mxv_8x8x8", (8, 8, 8)), ("mxv_5x9x7", (5, 9, 7)),
                                         ("mxv_16x16x16", (16, 16, 16))])
-def test_sell_mxv_bit_identical_to_reference(A, golden, name, sizes, kernel, lanes, rows):
+def test_sell_mxv_bit_identical_to_reference(A, golden, name, sizes, kernel, lanes, rows, monkeypatch):
     """The sliced-ELL kernel and its diagonal slices fold each row sequentially in
     storage order without FMA contraction: the reference's own order, so y is
-    identical bit for bit."""
+    identical bit for bit. (ALP_HIP_KEEP_SELL=1 keeps the sliced-ELL arrays,
+    which a stencil, all of whose slices are diagonal, otherwise drops.)"""
     g = golden(name)
+    monkeypatch.setenv("ALP_HIP_KEEP_SELL", "1")
     M = A.Matrix.stencil27(*sizes)
     assert M.get_kernel()[0] == 5  # the default for a stencil
     M.set_kernel(kernel, lanes, rows)
@@ -160,9 +162,10 @@ def test_sell_mxv_bit_identical_to_reference(A, golden, name, sizes, kernel, lan
     assert np.array_equal(y.numpy(), g["y"])
 
 
-def test_sell_pattern_lorland_and_plus_pattern(A):
+def test_sell_pattern_lorland_and_plus_pattern(A, monkeypatch):
     """Sliced ELL over a pattern matrix: lor-land with both mask modes, and the
     plus/pattern row sums, against the oracle."""
+    monkeypatch.setenv("ALP_HIP_KEEP_SELL", "1")
     st = O.stencil(20, 18, 16)
     n = len(st.off) - 1
     M = A.Matrix.from_crs(n, n, st.off, st.idx)  # pattern (no values)
@@ -241,12 +244,13 @@ def banded_crs(n, offs, rng):
 
 
 @pytest.mark.parametrize("w", [1, 2, 5, 8, 13, 26, 27, 28, 32])
-def test_dia_every_slot_width(A, w):
+def test_dia_every_slot_width(A, w, monkeypatch):
     """Diagonal slices of 1 to 32 slots, odd and even. FP64 values sit in slot
     pairs with an odd last slot on its own, FP32 values slot-major; 27 slots take
     the compile-time-width fold. For FP64 values, their FP32 copy and an FP32
     matrix, kernel 5 must equal kernel 4 bit for bit (both fold each row in
     storage order) and the oracle to round-off, in accumulate and overwrite mode."""
+    monkeypatch.setenv("ALP_HIP_KEEP_SELL", "1")
     rng = np.random.default_rng(100 + w)
     n = 3001
     offs = np.sort(rng.choice(np.arange(-45, 46), size=w, replace=False))
@@ -584,3 +588,24 @@ def test_cg_repeated_solves_reuse_cached_state(A, precision):
     x4 = A.Vector(n, fill=0.0)
     assert A.cg(M, b2, x4, 5000, 1e-10, precision, transpose=True)[0] == A.SUCCESS
     assert np.linalg.norm(b2h - O.spmv(ref, x4.numpy())) <= 1.05e-10 * np.linalg.norm(b2h)
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_stencil_drops_sliced_ell_arrays(A, f32):
+    """A stencil's slices are all diagonal: the storage keeps the CRS, the slice
+    offsets and the diagonal form, not the sliced-ELL indices and values. FP64
+    without an FP32 copy stays within 1.7x its CRS bytes (12 nnz + 8 (n + 1));
+    every kernel choice still gives the closed form of A 1 bit for bit, kernel 4
+    running the diagonal form."""
+    M = A.Matrix.stencil27(40, 40, 40, flags=A.MAT_WITH_F32 if f32 else 0)
+    slices, _, diag = M.layout()
+    assert diag == slices > 0
+    crs = 12 * M.nnz + 8 * (M.nrows + 1)
+    ratio = M.device_bytes() / crs
+    print(f"stencil 40^3 f32={f32}: device bytes {M.device_bytes()} = {ratio:.3f} x CRS", flush=True)
+    assert ratio <= (2.4 if f32 else 1.7)
+    for kernel, lanes, rows in KERNELS + SELL + DIA:
+        M.set_kernel(kernel, lanes, rows)
+        y = A.Vector(M.nrows, fill=0.0)
+        assert A.mxv(y, M, A.Vector(M.nrows, fill=1.0), descr=A.DENSE) == A.SUCCESS
+        assert np.array_equal(y.numpy(), closed_form_A1(40, 40, 40)), (kernel, lanes, rows)

@@ -239,12 +239,14 @@ def test_grb_generic_semirings_and_sparse_semantics(seed):
 def test_grb_hpcg_program_as_the_reference_smoke_test(golden):
     """build_hpcg_system + hpcg under a grb::Launcher, as tests/smoke/hpcg.cpp
     runs it: 16^3, one coarse level, 27 index-mod colours, 56 iterations; the
-    smoke test's bar ||x - 1|| < 4e-14 and the reference's iteration count and
-    residual (fixture hpcg_16x16x16_l1_c27)."""
+    smoke test's bar ||x - 1|| < 4e-14 and the reference's iteration count
+    (fixture hpcg_16x16x16_l1_c27). After 56 iterations both residuals are
+    round-off (||r||^2 ~ 1e-39 against ||b||^2 ~ 1e4): their values are noise in
+    either implementation, so the check is that both sit at that level."""
     g = golden("hpcg_16x16x16_l1_c27")
     r = _run(HPCG_PROG, 16, 16, 16, 1, 27, 56)
     assert r["launcher_rc"] == 0 and r["rc"] == 0 and r["nprocs"] == 1 and r["pid"] == 0
     assert r["iterations"] == int(g["iterations"][0]) == 56
-    assert abs(r["residual"] - g["residual"][0]) <= 1e-6 * g["residual"][0]
+    assert r["residual"] <= 1e-30 and g["residual"][0] <= 1e-30
     assert r["square_norm_diff"] < (4e-14) ** 2
     assert r["pinned_nonzeroes"] == 16 ** 3
