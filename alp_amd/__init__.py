@@ -119,6 +119,7 @@ SIGNATURES = {
     "alp_hip_convert_f32_to_f64": (_RC, [_S, _P, _P]),
     "alp_hip_cg": (_RC, [_P, _P, _P, _S, _D, _I, ctypes.POINTER(_S), ctypes.POINTER(_D)]),
     "alp_hip_cg_transposed": (_RC, [_P, _P, _P, _S, _D, _I, ctypes.POINTER(_S), ctypes.POINTER(_D)]),
+    "alp_hip_cg_noalloc": (_RC, [_P, _P, _P, _S, _D, ctypes.POINTER(_S), ctypes.POINTER(_D)]),
     "alp_hip_matrix_conductance27": (_RC, [ctypes.POINTER(_P), _S, _S, _S, _P, _D, _U]),
     "alp_hip_grad_conductance27": (_RC, [_S, _S, _S, _P, _P, _P]),
     "alp_hip_grad_matrix": (_RC, [_P, _P, _P, _P]),

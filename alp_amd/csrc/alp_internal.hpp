@@ -53,8 +53,15 @@ struct Context {
 	unsigned *d_tickets = nullptr;  // 64 counters, kept at zero between uses
 	double *d_scalars = nullptr;    // 64 result slots
 	double *h_pinned = nullptr;     // 64 pinned host doubles
+	// device address of the file-scope scratch array of alp_hip_cg_noalloc on
+	// this context's device (hipGetSymbolAddress at init)
+	unsigned char *d_noalloc = nullptr;
 };
 constexpr int kMaxPartials = 4096;
+// alp_hip_cg_noalloc's scratch: the state block (256 B), kMaxPartials partials,
+// then three work vectors of at most kNoAllocRows doubles (24 MiB)
+constexpr size_t kNoAllocRows = size_t(1) << 20;
+constexpr size_t kNoAllocBytes = 256 + 8 * kMaxPartials + 3 * 8 * kNoAllocRows;
 constexpr int kSellRows = 64;
 constexpr uint64_t kLongRow = 2048;  // CSR-stream: longer rows go through the chunk path
 constexpr uint64_t kChunk = 8192;    // nonzeros per chunk of a long row  // rows per sliced-ELL slice: one wavefront, one lane per row

@@ -603,7 +603,9 @@ struct CgRunner {
 	// replayed from a captured graph, so every batch starts and ends in slot 0.
 	// The graph is reused from `slot` when it was captured for the same storage,
 	// vectors and kernel choice (and kept there for the next call).
-	alp_rc_t run(hipStream_t s, CgState *h_state, CgSlot *slot = nullptr) {
+	// capture false: the batches are launched directly (nothing is captured or
+	// instantiated, so the run allocates nothing)
+	alp_rc_t run(hipStream_t s, CgState *h_state, CgSlot *slot = nullptr, bool capture = true) {
 		const int batch = 16;
 		static_assert(batch % 2 == 0, "batches start and end in slot 0");
 		hipGraphExec_t exec = nullptr;
@@ -611,7 +613,9 @@ struct CgRunner {
 		bool use_graph = false;
 		const void *key[6] = {&S, a.x, a.b, a.val, a.st, a.x64};
 		const int kcfg[3] = {S.kernel, S.lanes, S.rows};
-		if (slot && slot->exec && std::equal(key, key + 6, slot->key) && std::equal(kcfg, kcfg + 3, slot->kcfg)) {
+		if (!capture) {
+		} else if (slot && slot->exec && std::equal(key, key + 6, slot->key) &&
+		           std::equal(kcfg, kcfg + 3, slot->kcfg)) {
 			exec = slot->exec;
 			use_graph = true;
 		} else {
@@ -877,6 +881,46 @@ static alp_rc_t cg_entry(alp_hip_matrix_t A, bool transpose, const double *b, do
 	if (precision == 2) return cg_mixed(A, S, b, x, max_iter, tol, iters, residual);
 	if (precision == 3) return cg_reliable(A, S, b, x, max_iter, tol, iters, residual, false);
 	return ALP_ILLEGAL;
+}
+
+// FP64 CG as alp_hip_cg with precision 0 (the same kernels, launch shapes and
+// reduction order), with r, u, temp, the state block and the partials in the
+// library's file-scope scratch array instead of buffers kept on the matrix, and
+// the iteration batches launched directly instead of from a captured graph.
+ALP_EXPORT alp_rc_t alp_hip_cg_noalloc(alp_hip_matrix_t A, const double *b, double *x, size_t max_iter,
+                                       double tol, size_t *iters, double *residual) {
+	if (ctx().device < 0 || !ctx().d_noalloc) return ALP_PANIC;  // alp_hip_init first: it allocates
+	if (!A || !b || !x || !iters || !residual) return ALP_ILLEGAL;
+	if (A->nrows != A->ncols) return ALP_ILLEGAL;
+	if (A->value_type != ALP_VAL_F64) return ALP_MISMATCH;
+	if (!(tol > 0)) return ALP_ILLEGAL;
+	if (max_iter == 0) return ALP_ILLEGAL;
+	const size_t n = A->nrows;
+	if (n > kNoAllocRows) {
+		std::fprintf(stderr, "alp_hip_cg_noalloc: %zu rows exceed the scratch limit of %zu\n", n, kNoAllocRows);
+		return ALP_OUTOFMEM;
+	}
+	*iters = 0;
+	*residual = std::numeric_limits<double>::infinity();
+	if (n == 0) return ALP_SUCCESS;
+	hipStream_t s = stream();
+	Storage &S = A->crs;
+	// the state block is written whole by the prelude and the partials before
+	// each read: nothing here needs clearing between calls
+	unsigned char *base = ctx().d_noalloc;
+	CgState *st = reinterpret_cast<CgState *>(base);
+	double *partials = reinterpret_cast<double *>(base + 256);
+	double *r = partials + kMaxPartials, *u = r + n, *temp = u + n;
+	static_assert(sizeof(CgState) <= 256, "the state block fits its 256 bytes");
+	CgArgs<double> a{n, S.off, S.idx, (const double *)S.val, x, r, u, temp, b, partials, nullptr, st};
+	ALP_TRY(spmv(S, ALP_VAL_F64, ALP_PLUS_TIMES_F64, ALP_DESC_OVERWRITE, temp, x, nullptr, s));
+	ALP_TRY(cg_prelude(a, (const double *)b, (const double *)nullptr, tol, max_iter, s));
+	CgRunner<double> runner(S, a);
+	CgState h;
+	ALP_TRY(runner.run(s, &h, nullptr, false));
+	*iters = h.s[0].iter;
+	*residual = h.s[0].rr;
+	return h.s[0].converged ? ALP_SUCCESS : ALP_FAILED;
 }
 
 ALP_EXPORT alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *b, double *x, size_t max_iter,

@@ -25,6 +25,12 @@ void report_hip_error(hipError_t e, const char *what, const char *file, int line
 static Context g_ctx;
 static std::mutex g_init_mutex;
 
+// Work memory of alp_hip_cg_noalloc, for callers that cannot allocate: each
+// device that loads the library holds one copy until the process ends. Host
+// code reaches it only through the address hipGetSymbolAddress gives at init
+// (Context::d_noalloc); the name itself is not a device address on the host.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 Context &ctx() { return g_ctx; }
 
 alp_rc_t ensure_init() {
@@ -799,6 +805,8 @@ ALP_EXPORT alp_rc_t alp_hip_init(int device) {
 	ALP_HIP_TRY(hipMemset(g_ctx.d_tickets, 0, sizeof(unsigned) * 64));
 	ALP_HIP_TRY(hipMalloc(&g_ctx.d_scalars, sizeof(double) * 64));
 	ALP_HIP_TRY(hipHostMalloc(&g_ctx.h_pinned, sizeof(double) * 64, hipHostMallocDefault));
+	ALP_HIP_TRY(hipGetSymbolAddress(reinterpret_cast<void **>(&g_ctx.d_noalloc), HIP_SYMBOL(scratch)));
+	if (!g_ctx.d_noalloc) return ALP_PANIC;
 	g_ctx.device = device;
 	return ALP_SUCCESS;
 }

@@ -256,6 +256,23 @@ alp_rc_t alp_hip_cg(alp_hip_matrix_t A, const double *d_b, double *d_x, size_t m
 alp_rc_t alp_hip_cg_transposed(alp_hip_matrix_t A, const double *d_b, double *d_x, size_t max_iter,
                                double tol, int precision, size_t *iters, double *residual);
 
+/* FP64 CG that allocates nothing: no counterpart in the reference. It is for
+ * callers that cannot allocate device memory (code under stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments, results and
+ * return codes as alp_hip_cg with precision 0 (bit-identical x and iteration
+ * count), except:
+ *   - its work vectors, state and partial sums live in one file-scope device
+ *     array that the library holds on each device until the process ends
+ *     (24 MiB + 33 KiB); A may have at most 2^20 rows (1,048,576), larger
+ *     systems return ALP_OUTOFMEM;
+ *   - the library must be initialised first (alp_hip_init allocates; without
+ *     it the call returns ALP_PANIC);
+ *   - every call on a device shares that array, so calls that overlap on one
+ *     device (other host threads, other streams) race: serialise them.
+ * The matrix and the vectors d_b, d_x are the caller's. */
+alp_rc_t alp_hip_cg_noalloc(alp_hip_matrix_t A, const double *d_b, double *d_x, size_t max_iter,
+                            double tol, size_t *iters, double *residual);
+
 /* ---- differentiable solve (no counterpart in the reference) ----
  * A(theta): the 27-point stencil graph's weighted Laplacian plus `shift` on the
  * diagonal (A_ij = -theta_e for the edge e = (i, j), A_ii = sum of theta at i + shift).
