@@ -773,6 +773,11 @@ ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double al
 	// the iteration runs in degree order when the layout exists: the ranks in
 	// pr_x, vertex perm[i] at i; the caller's vector gets them back at the end
 	const uint32_t *perm = w->pr_perm.p;
+	if (perm) {  // the product follows the matrix's kernel choice (alp_hip_matrix_set_kernel)
+		w->pr_pat.kernel = L->ccs.kernel;
+		w->pr_pat.lanes = L->ccs.lanes;
+		w->pr_pat.rows = L->ccs.rows;
+	}
 	const Storage &P = perm ? w->pr_pat : L->ccs;
 	double *pr = perm ? w->pr_x.p : d_pr;
 	DevBuf<double> &rs = w->rs, &pn = w->pn, &pnn = w->pnn;

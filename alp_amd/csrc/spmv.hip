@@ -170,6 +170,118 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(
 	}
 }
 
+// CSR with LDS-staged row segments: a block stages its tile's column indices and
+// values in LDS with fully coalesced loads, then tpr threads per row walk the
+// row's slice of the tile and gather x themselves, U gathers in flight. Lanes
+// take consecutive rows, so on banded and stencil matrices the lanes of one
+// gather instruction read neighbouring entries of x; k_spmv_stream gathers in
+// storage order, where one instruction spans a few rows' whole column ranges.
+// Each thread adds the same products in the same order as k_spmv_stream's fold
+// and the tpr partials meet in the same shuffle tree: the two are bit-identical.
+template <typename R, int MASK, bool OVERWRITE, int J>
+__global__ __launch_bounds__(kBlock) void k_spmv_seg(
+    size_t nrb, const uint32_t *__restrict__ rb, const uint64_t *__restrict__ rbase,
+    const uint64_t *__restrict__ off, const uint32_t *__restrict__ idx, const void *__restrict__ vval,
+    const typename R::X *__restrict__ x, typename R::Y *__restrict__ y,
+    const uint8_t *__restrict__ mask, int skip_long, const int *__restrict__ skip) {
+	if (skip && *skip) return;  // a solver loop that has stopped
+	using A = typename R::A;
+	using X = typename R::X;
+	using Y = typename R::Y;
+	using VT = typename Val<R>::T;
+	constexpr int kTile = J * kBlock;
+	constexpr int U = 4;
+	__shared__ uint32_t sidx[kTile];
+	__shared__ VT sval[R::kPattern ? 1 : kTile];
+	__shared__ A red[kBlock / kWave];
+	const auto *val = static_cast<const typename R::V *>(vval);
+	for (size_t b = blockIdx.x; b < nrb; b += gridDim.x) {
+		const uint32_t r0 = rb[b], r1 = rb[b + 1];
+		const uint64_t base = rbase[b], top = rbase[b + 1];
+		if (top - base <= (uint64_t)kTile) {
+			const unsigned nr = r1 - r0;
+			unsigned tpr = 1;
+			while (tpr < 64 && tpr * 2 * nr <= kBlock) tpr *= 2;
+			const unsigned r = r0 + threadIdx.x / tpr, sub = threadIdx.x % tpr;
+			bool active = r < r1;
+			uint64_t rk = 0, re = 0;
+			A yv = R::zero();
+			if (active) {
+				rk = off[r];
+				re = off[r + 1];
+				if (MASK) active = (MASK == 1) == (mask[r] != 0);
+				if (!OVERWRITE && active && sub == 0) yv = static_cast<A>(y[r]);
+			}
+			// stage the tile: all J loads of a thread in flight, then the LDS stores
+			uint32_t jj[J];
+			VT vv[J];
+#pragma unroll
+			for (int q = 0; q < J; ++q) {
+				const uint64_t k = base + threadIdx.x + (uint64_t)q * kBlock;
+				jj[q] = 0;
+				vv[q] = VT(0);
+				if (k < top) {
+					jj[q] = __builtin_nontemporal_load(idx + k);
+					vv[q] = Val<R>::load(val, k);
+				}
+			}
+#pragma unroll
+			for (int q = 0; q < J; ++q) {
+				const uint64_t k = base + threadIdx.x + (uint64_t)q * kBlock;
+				if (k < top) {
+					sidx[k - base] = jj[q];
+					if (!R::kPattern) sval[k - base] = vv[q];
+				}
+			}
+			__syncthreads();
+			A acc = R::zero();
+			uint64_t q = rk + sub;
+			for (; q + (uint64_t)(U - 1) * tpr < re; q += (uint64_t)U * tpr) {
+				X xv[U];
+				VT av[U];
+#pragma unroll
+				for (int u = 0; u < U; ++u) {
+					const unsigned t = (unsigned)(q + (uint64_t)u * tpr - base);
+					xv[u] = x[sidx[t]];
+					av[u] = R::kPattern ? VT(0) : sval[R::kPattern ? 0 : t];
+				}
+#pragma unroll
+				for (int u = 0; u < U; ++u) acc = R::add(acc, Val<R>::mul(av[u], xv[u]));
+			}
+			for (; q < re; q += tpr) {
+				const unsigned t = (unsigned)(q - base);
+				acc = R::add(acc, Val<R>::mul(R::kPattern ? VT(0) : sval[R::kPattern ? 0 : t], x[sidx[t]]));
+			}
+			for (unsigned sh = tpr / 2; sh > 0; sh >>= 1) acc = R::add(acc, shfl_xor(acc, sh));
+			if (active && sub == 0 && !(skip_long && re - rk > kLongRow)) {
+				if (rk == re) {
+					if (OVERWRITE) y[r] = static_cast<Y>(R::zero());
+				} else {
+					y[r] = OVERWRITE ? static_cast<Y>(acc) : static_cast<Y>(R::add(yv, acc));
+				}
+			}
+			__syncthreads();
+		} else if (!skip_long) {
+			// a single row longer than a tile: the whole block folds it
+			A acc = R::zero();
+			for (uint64_t k = base + threadIdx.x; k < top; k += kBlock)
+				acc = R::add(acc, Val<R>::mul(Val<R>::load(val, k), x[__builtin_nontemporal_load(idx + k)]));
+			acc = group_fold<kWave, R>(acc);
+			__syncthreads();
+			if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = acc;
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				A t = red[0];
+				for (int i = 1; i < kBlock / kWave; ++i) t = R::add(t, red[i]);
+				bool active = true;
+				if (MASK) active = (MASK == 1) == (mask[r0] != 0);
+				if (active) y[r0] = OVERWRITE ? static_cast<Y>(t) : static_cast<Y>(R::add(static_cast<A>(y[r0]), t));
+			}
+			__syncthreads();
+		}
+	}
+}
+
 // Sliced ELL: one wavefront per 64-row slice, one lane per row. A lane walks its
 // row in storage order and folds sequentially (output = product + output, then
 // y = y + output), the order of the reference's vxm_inner_kernel_gather
@@ -763,7 +875,8 @@ static void launch_long_rows(const Storage &S, const void *val, const void *x, v
 
 // variant (`rows`): 0/1 = persistent grid, 4 blocks per CU; 2 = persistent, 8 per
 // CU; 4 = one block per row tile; 8 = one block per tile in plain block order;
-// 16 / 32 = software-pipelined persistent grid with 4 / 2 blocks per CU.
+// 16 / 32 = software-pipelined persistent grid with 4 / 2 blocks per CU; 64 =
+// LDS-staged row segments (k_spmv_seg), one block per tile.
 // tile (`lanes`): 0/8 = 2048-nonzero tiles, 16 = 4096-nonzero tiles.
 template <typename R, int MASK, bool OW>
 static void launch_stream(int variant, int tile, const Storage &S, const void *val, const void *x,
@@ -784,10 +897,21 @@ static void launch_stream(int variant, int tile, const Storage &S, const void *v
 #undef ALP_PIPE
 		return;
 	}
+	const int skip_long = S.n_long > 0 ? 1 : 0;
+	if (variant == 64) {  // LDS-staged row segments, a block per tile
+		if (nrb == 0) return;
+#define ALP_SEG(JJ)                                                                                 \
+	hipLaunchKernelGGL((k_spmv_seg<R, MASK, OW, JJ>), dim3(nrb), dim3(kBlock), 0, s, nrb, rb, rbase, \
+	                   (const uint64_t *)S.off, (const uint32_t *)S.idx, val, (const typename R::X *)x, \
+	                   (typename R::Y *)y, mask, skip_long, skip)
+		if (big) ALP_SEG(16); else ALP_SEG(8);
+#undef ALP_SEG
+		if (skip_long) launch_long_rows<R, MASK, OW>(S, val, x, y, mask, s, skip);
+		return;
+	}
 	size_t cap = variant >= 4 ? nrb : fill_blocks(variant == 2 ? 8 : 4);
 	size_t grid = nrb < cap ? nrb : cap;
 	if (grid == 0) return;
-	const int skip_long = S.n_long > 0 ? 1 : 0;
 #define ALP_STREAM(JJ, XX)                                                                      \
 	hipLaunchKernelGGL((k_spmv_stream<R, MASK, OW, JJ, XX>), dim3(grid), dim3(kBlock), 0, s, nrb, \
 	                   rb, rbase, (const uint64_t *)S.off, (const uint32_t *)S.idx, val,          \
@@ -1092,7 +1216,7 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_set_kernel(alp_hip_matrix_t A, int kernel, in
 	    lanes != 64)
 		return ALP_ILLEGAL;
 	if (rows != 0 && rows != 1 && rows != 2 && rows != 4 &&
-	    !(kernel == 2 && (rows == 8 || rows == 16 || rows == 32)))
+	    !(kernel == 2 && (rows == 8 || rows == 16 || rows == 32 || rows == 64)))
 		return ALP_ILLEGAL;
 	A->crs.kernel = A->ccs.kernel = kernel;
 	A->crs.lanes = A->ccs.lanes = lanes;
@@ -1166,7 +1290,7 @@ ALP_EXPORT alp_rc_t alp_hip_matrix_autotune(alp_hip_matrix_t A) {
 	for (int g : {2, 4, 8, 16, 32, 64})
 		for (int r : {1, 2, 4}) cands.push_back({1, g, r});
 	for (int t : {8, 16})
-		for (int v : {1, 2, 4, 8, 16, 32}) cands.push_back({2, t, v});
+		for (int v : {1, 2, 4, 8, 16, 32, 64}) cands.push_back({2, t, v});
 	cands.push_back({3, 0, 0});
 	if (A->crs.sell_idx)
 		for (int xcd : {0, 1})

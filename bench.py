@@ -98,11 +98,12 @@ def cpu_baseline(side: int, nnz: int, n: int) -> dict:
 
 
 def cpu_solve_reference(side: int, cg_iterations: int, hpcg_iterations: int):
-    """The reference's own solvers (oracle/_ref, reference_omp) on this host, on a
-    bounded sample: 10 CG iterations and 2 HPCG iterations at side^3 (system
-    builds not timed). The full-solve estimates scale the per-iteration time by
-    the iteration counts of the device FP64 solves, which match the reference's
-    (the CG and HPCG fixture tests)."""
+    """The reference's own solvers (oracle/_ref, reference_omp) on this host: the
+    whole FP64 CG solve to 1e-10 at side^3 (x0 = 0, b = A 1, system build not
+    timed; about 5 s on 16 threads), so its iteration count checks the device's
+    and its time is measured, not estimated; and a bounded HPCG sample (2
+    iterations, 4 levels, 8 colours; a whole solve would take about 40 s) scaled
+    by the device's FP64 iteration count."""
     import oracle as O
     if not O.ref_available("omp"):
         return None
@@ -110,19 +111,18 @@ def cpu_solve_reference(side: int, cg_iterations: int, hpcg_iterations: int):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     it = ctypes.c_size_t()
     out = {"kind": "reference", "cores": threads}
-    ms = R.ref_time_stencil_cg(side, side, side, 10, 1e-300, ctypes.byref(it))
+    ms = R.ref_time_stencil_cg(side, side, side, 1000, 1e-10, ctypes.byref(it))
     if ms > 0 and it.value:
-        per = ms / it.value
-        out["cg_fp64"] = {"iterations_timed": it.value, "ms_per_iteration": round(per, 2),
-                          "est_ms_full_solve": round(per * cg_iterations, 1), "iterations": cg_iterations}
+        out["cg_fp64"] = {"ms": round(ms, 1), "iterations": it.value, "device_iterations": cg_iterations,
+                          "ms_per_iteration": round(ms / it.value, 2), "timed": "whole solve to 1e-10"}
     ms = R.ref_time_hpcg(side, side, side, 3, 8, 2, ctypes.byref(it))
     if ms > 0 and it.value:
         per = ms / it.value
         out["hpcg_fp64"] = {"iterations_timed": it.value, "ms_per_iteration": round(per, 2),
                             "est_ms_full_solve": round(per * hpcg_iterations, 1), "iterations": hpcg_iterations}
-    out["sample"] = (f"ALP reference_omp conjugate_gradient (10 iterations) and hpcg (2 iterations, 4 levels, "
-                     f"8 colours) on the {side}^3 stencil; est_ms_full_solve = ms_per_iteration x the FP64 "
-                     f"iteration count")
+    out["sample"] = (f"ALP reference_omp conjugate_gradient to 1e-10 (whole solve, timed) and hpcg (2 iterations, "
+                     f"4 levels, 8 colours; est_ms_full_solve = ms_per_iteration x the device's FP64 iteration "
+                     f"count) on the {side}^3 stencil")
     return out
 
 
@@ -195,10 +195,17 @@ def leg_c2(alp, scale: int = 22) -> dict:
         (lev, nl, reached), dt = timed(lambda: alp.bfs(G, s), alp.sync)
         times.append(dt)
     bfs_ms = 1e3 * sorted(times)[1]
-    # warm-up: the first call builds the column (CCS) layout the pull product reads,
-    # which the reference builds with the matrix (buildMatrixUnique fills CRS and CCS)
-    alp.pagerank(G, 0.85, 1e-9, 1000)
+    # warm-up: the first call builds the gather pattern in out-degree order, once
+    # per graph (as the reference builds its CCS with the matrix)
+    _, t_pr_first = timed(lambda: alp.pagerank(G, 0.85, 1e-9, 1000), alp.sync)
     (rc, pr, it, q), t_pr = timed(lambda: alp.pagerank(G, 0.85, 1e-9, 1000), alp.sync)
+    # the same with the row-segment variant of the CSR-stream product
+    kcfg = G.get_kernel()
+    G.set_kernel(2, 8, 64)
+    alp.pagerank(G, 0.85, 1e-9, 1000)
+    (rc2, pr2, it2, _), t_pr2 = timed(lambda: alp.pagerank(G, 0.85, 1e-9, 1000), alp.sync)
+    G.set_kernel(*kcfg)
+    rowseg_diff = float(np.max(np.abs(pr2.numpy() - pr.numpy())))
     pr_bytes = 4 * nnz + 8 * (n + 1) + 32 * n
     # knn (k-hop reachability) from the same source, 1 and 3 hops
     kh = {}
@@ -210,7 +217,9 @@ def leg_c2(alp, scale: int = 22) -> dict:
             "build_ms": round(1e3 * t_build, 2), "bfs_ms": round(bfs_ms, 3), "bfs_levels": nl,
             "bfs_reached": reached, "bfs_gteps": round(nnz / 2 / (bfs_ms * 1e-3) / 1e9, 2),
             "pagerank_rc": rc, "pagerank_iterations": it, "pagerank_ms": round(1e3 * t_pr, 2),
-            "pagerank_gbs": round(pr_bytes * it / t_pr / 1e9, 1),
+            "pagerank_gbs": round(pr_bytes * it / t_pr / 1e9, 1), "pagerank_first_call_ms": round(1e3 * t_pr_first, 2),
+            "pagerank_rowseg_ms": round(1e3 * t_pr2, 2), "pagerank_rowseg_rc_iterations": [rc2, it2],
+            "pagerank_rowseg_max_abs_diff": rowseg_diff,
             "khop1_ms": kh[1][0], "khop1_reached": kh[1][1], "khop3_ms": kh[3][0], "khop3_reached": kh[3][1]}
 
 
@@ -577,9 +586,14 @@ def main() -> int:
         log("[bench] CPU baseline")
         out["cpu_baseline"] = cpu_baseline(side, nnz, n)
         if solves.get("cg_fp64") and solves.get("hpcg_fp64"):
-            log("[bench] CPU reference solves (bounded sample)")
+            log("[bench] CPU reference solves (CG whole, HPCG sampled)")
             ref = cpu_solve_reference(side, solves["cg_fp64"]["iterations"], solves["hpcg_fp64"]["iterations"])
             if ref:
+                if ref.get("cg_fp64"):
+                    c = ref["cg_fp64"]["ms"]
+                    ref["cg_fp64"]["device_fp64_speedup"] = round(c / solves["cg_fp64"]["ms"], 1)
+                    if solves.get("cg_mixed"):
+                        ref["cg_fp64"]["device_mixed_speedup"] = round(c / solves["cg_mixed"]["ms"], 1)
                 out["cpu_baseline"]["solves"] = ref
     print(json.dumps(out), flush=True)
     return 0

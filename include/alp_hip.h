@@ -141,7 +141,9 @@ alp_rc_t alp_hip_matrix_download(alp_hip_matrix_t A, int which, uint64_t *h_off,
 /* select the SpMV kernel: 0 = auto, 1 = CSR-vector (`lanes` per row: 2..64, `rows`
  * per lane group: 1, 2 or 4; 0 = default), 2 = CSR-stream (`rows` picks the launch:
  * 1 = persistent grid 4 blocks/CU, 2 = persistent 8 blocks/CU, 4 = a block per tile,
- * 8 = a block per tile in plain block order; `lanes` 16 = 4096-nonzero tiles),
+ * 8 = a block per tile in plain block order, 16 / 32 = software-pipelined persistent
+ * grid with 4 / 2 blocks per CU, 64 = the tile's indices and values staged in LDS and
+ * the rows gathered lane by lane; `lanes` 16 = 4096-nonzero tiles),
  * 3 = CSR-wave (a 512-nonzero tile per wavefront), 4 = sliced ELL (64-row slices,
  * one lane per row; `lanes` 0/1 = plain / XCD-contiguous slice order, `rows` 4, 8
  * or 16 entries in flight), 5 = the sliced ELL with its index-free diagonal slices

@@ -105,7 +105,7 @@ def closed_form_A1(nx, ny, nz):
 
 KERNELS = [(1, 2, 1), (1, 4, 2), (1, 8, 4), (1, 16, 1), (1, 16, 4), (1, 32, 2), (1, 64, 1), (2, 0, 1),
            (2, 0, 2), (2, 0, 4), (2, 8, 8), (2, 16, 1), (2, 16, 4), (2, 16, 8), (2, 8, 16), (2, 16, 32),
-           (3, 0, 0)]
+           (2, 0, 64), (2, 16, 64), (3, 0, 0)]
 # sliced ELL (kernel 4), built for matrices with regular rows: plain / XCD order, 4-16 in flight
 SELL = [(4, 0, 4), (4, 0, 8), (4, 0, 16), (4, 1, 8)]
 # the same with its index-free diagonal slices (kernel 5), built for stencils
@@ -609,3 +609,45 @@ def test_stencil_drops_sliced_ell_arrays(A, f32):
         y = A.Vector(M.nrows, fill=0.0)
         assert A.mxv(y, M, A.Vector(M.nrows, fill=1.0), descr=A.DENSE) == A.SUCCESS
         assert np.array_equal(y.numpy(), closed_form_A1(40, 40, 40)), (kernel, lanes, rows)
+
+
+@pytest.mark.parametrize("tile", [8, 16])
+def test_row_segment_kernel_equals_stream_bit_for_bit(A, tile):
+    """CSR-stream variant 64 stages a tile's indices and values in LDS and gathers
+    row by row; each thread adds the same products in the same order as the
+    storage-order variant 8, so y is identical bit for bit: skewed rows (long rows
+    through the chunk path), accumulate, overwrite, both mask modes, FP64, FP32,
+    min-plus and the pattern row sums."""
+    rng = np.random.default_rng(17)
+    n, m = 6000, 5000
+    lens = np.minimum(rng.zipf(1.5, n), 9000)
+    lens[rng.random(n) < 0.1] = 0
+    I = np.repeat(np.arange(n), lens).astype(np.uint32)
+    J = rng.integers(0, m, I.size).astype(np.uint32)
+    key = np.unique(I.astype(np.uint64) * m + J)
+    I, J = (key // m).astype(np.uint32), (key % m).astype(np.uint32)
+    V = rng.standard_normal(I.size)
+    M = A.Matrix.from_coo(n, m, I, J, V, flags=A.MAT_WITH_F32)
+    P = A.Matrix.from_coo(n, m, I, J)
+    x = rng.standard_normal(m)
+    y0 = rng.standard_normal(n)
+    mask = A.Vector.from_numpy((rng.random(n) < 0.5).astype(np.uint8))
+    cases = [(M, A.PLUS_TIMES_F64, x, y0), (M, A.PLUS_TIMES_F32, x.astype(np.float32), y0.astype(np.float32)),
+             (M, A.MIN_PLUS_F64, x, y0), (P, A.PLUS_PATTERN_F64, x, y0)]
+    for mat, ring, xv, yv in cases:
+        outs = []
+        for rows in (8, 64):
+            mat.set_kernel(2, tile, rows)
+            got = []
+            for descr, msk in ((0, None), (A.OVERWRITE, None), (0, mask), (A.INVERT_MASK, mask)):
+                y = A.Vector.from_numpy(yv)
+                assert A.mxv(y, mat, A.Vector.from_numpy(xv), ring=ring, descr=descr, mask=msk) == A.SUCCESS
+                got.append(y.numpy())
+            outs.append(got)
+        for a, b in zip(*outs):
+            assert np.array_equal(a, b), ring
+    ref = O.coo_to_compressed(n, m, I, J, V, which=0)
+    M.set_kernel(2, tile, 64)
+    y = A.Vector.from_numpy(y0)
+    assert A.mxv(y, M, A.Vector.from_numpy(x)) == A.SUCCESS
+    assert rel_inf(y.numpy(), O.spmv(ref, x, y0)) <= FP64_RTOL
