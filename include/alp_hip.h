@@ -383,7 +383,10 @@ alp_rc_t alp_hip_batch_info(alp_hip_batch_t B, size_t *nrows, size_t *nnz, uint6
    and how many qualified; *ndiagonal is 0 when the form was not built. No
    counterpart in the reference (the batch itself has none). */
 alp_rc_t alp_hip_batch_layout(alp_hip_batch_t B, size_t *nslices, size_t *ndiagonal);
-/* Y = A X over all systems (FP64) */
+/* Y = A X over all systems (FP64). Block vectors (d_X, d_Y here, d_B and d_X of
+   alp_hip_batch_solve) must be 16-byte aligned, as every hipMalloc pointer is:
+   the products move each row's right-hand sides in 16-byte chunks; others give
+   ALP_ILLEGAL. */
 alp_rc_t alp_hip_batch_spmm(alp_hip_batch_t B, const double *d_X, double *d_Y);
 /* time the SpMM for each right-hand-side tile width (lanes per row) and keep the fastest,
    separately for the FP64 product and the FP32-data product of the mixed iterations;

@@ -231,3 +231,16 @@ def test_crs_batch_rejects_malformed_input(A):
     empty = np.array([0, 0], np.uint64)
     assert lib.alp_hip_batch_create_crs(ctypes.byref(h), 1, empty.ctypes.data, off.ctypes.data,
                                         ok.ctypes.data, val.ctypes.data, 8) == A.ILLEGAL
+
+
+def test_batch_rejects_unaligned_block_vectors(A):
+    """The SpMM moves each row's right-hand sides in 16-byte chunks: block vectors
+    must be 16-byte aligned (every allocation is); an offset pointer is refused."""
+    rng = np.random.default_rng(3)
+    B, _ = make_batch(A, rng, 2, 4, 6)
+    X = A.Vector(B.nrows * 8 + 2, fill=1.0)
+    Y = A.Vector(B.nrows * 8 + 2)
+    lib = A.lib()
+    assert lib.alp_hip_batch_spmm(B.h, X.ptr + 8, Y.ptr) == A.ILLEGAL
+    assert lib.alp_hip_batch_spmm(B.h, X.ptr, Y.ptr + 8) == A.ILLEGAL
+    assert lib.alp_hip_batch_spmm(B.h, X.ptr + 16, Y.ptr + 16) == A.SUCCESS
