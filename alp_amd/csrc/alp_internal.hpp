@@ -64,7 +64,7 @@ constexpr size_t kNoAllocRows = size_t(1) << 20;
 constexpr size_t kNoAllocBytes = 256 + 8 * kMaxPartials + 3 * 8 * kNoAllocRows;
 constexpr int kSellRows = 64;
 constexpr uint64_t kLongRow = 2048;  // CSR-stream: longer rows go through the chunk path
-constexpr uint64_t kChunk = 8192;    // nonzeros per chunk of a long row  // rows per sliced-ELL slice: one wavefront, one lane per row
+constexpr uint64_t kChunk = 4096;    // nonzeros per chunk of a long row
 
 Context &ctx();
 alp_rc_t ensure_init();
@@ -259,6 +259,7 @@ struct Storage {
 	uint64_t *ch_beg = nullptr;    // n_chunks: first nonzero of the chunk
 	void *ch_part = nullptr;       // n_chunks partial folds (8 bytes each)
 	size_t n_long = 0, n_chunks = 0;
+	uint32_t chunk = 0;            // nonzeros per chunk (kChunk unless overridden at build)
 	// statistics for kernel selection
 	double avg_len = 0;
 	size_t max_len = 0;

@@ -182,11 +182,17 @@ static alp_rc_t upload_row_blocks(const std::vector<uint64_t> &off, size_t nrows
 static alp_rc_t build_long_rows(Storage &S, const std::vector<uint64_t> &off) {
 	std::vector<uint32_t> rows, crow;
 	std::vector<uint64_t> c0, cbeg;
+	uint64_t chunk = kChunk;
+	if (const char *e = std::getenv("ALP_HIP_CHUNK")) {  // diagnostics: another chunk size
+		const long v = std::atol(e);
+		if (v >= 256 && v <= 65536) chunk = (uint64_t)v;
+	}
+	S.chunk = (uint32_t)chunk;
 	for (size_t r = 0; r < S.nmajor; ++r) {
 		const uint64_t len = off[r + 1] - off[r];
 		if (len <= kLongRow) continue;
 		c0.push_back(cbeg.size());
-		for (uint64_t k = off[r]; k < off[r + 1]; k += kChunk) {
+		for (uint64_t k = off[r]; k < off[r + 1]; k += chunk) {
 			crow.push_back((uint32_t)rows.size());
 			cbeg.push_back(k);
 		}

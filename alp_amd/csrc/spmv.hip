@@ -817,7 +817,8 @@ __global__ __launch_bounds__(kBlock) void k_chunk_part(size_t nchunks, const uin
                                                        const uint64_t *__restrict__ off,
                                                        const uint32_t *__restrict__ idx, const void *__restrict__ vval,
                                                        const typename R::X *__restrict__ x,
-                                                       typename R::A *__restrict__ part, const int *__restrict__ skip) {
+                                                       typename R::A *__restrict__ part, const int *__restrict__ skip,
+                                                       uint64_t chunk) {
 	if (skip && *skip) return;  // a solver loop that has stopped
 	using A = typename R::A;
 	__shared__ A red[kBlock / kWave];
@@ -825,9 +826,27 @@ __global__ __launch_bounds__(kBlock) void k_chunk_part(size_t nchunks, const uin
 	for (size_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
 		const uint32_t r = lr_rows[ch_row[c]];
 		const uint64_t k0 = ch_beg[c], rend = off[r + 1];
-		const uint64_t k1 = k0 + kChunk < rend ? k0 + kChunk : rend;
+		const uint64_t k1 = k0 + chunk < rend ? k0 + chunk : rend;
+		// four entries' index, value and x loads in flight per thread, folded in
+		// index order (the same sequence as one entry per step)
+		using VT = typename Val<R>::T;
 		A acc = R::zero();
-		for (uint64_t k = k0 + threadIdx.x; k < k1; k += kBlock)
+		uint64_t k = k0 + threadIdx.x;
+		for (; k + 3 * kBlock < k1; k += 4 * kBlock) {
+			uint32_t j[4];
+			VT v[4];
+#pragma unroll
+			for (int u = 0; u < 4; ++u) {
+				j[u] = __builtin_nontemporal_load(idx + k + u * kBlock);
+				v[u] = Val<R>::load(val, k + u * kBlock);
+			}
+			typename R::X xv[4];
+#pragma unroll
+			for (int u = 0; u < 4; ++u) xv[u] = x[j[u]];
+#pragma unroll
+			for (int u = 0; u < 4; ++u) acc = R::add(acc, Val<R>::mul(v[u], xv[u]));
+		}
+		for (; k < k1; k += kBlock)
 			acc = R::add(acc, Val<R>::mul(Val<R>::load(val, k), x[__builtin_nontemporal_load(idx + k)]));
 		acc = group_fold<kWave, R>(acc);
 		__syncthreads();
@@ -866,7 +885,8 @@ static void launch_long_rows(const Storage &S, const void *val, const void *x, v
 	size_t g = S.n_chunks < (size_t)fill_blocks(8) ? S.n_chunks : fill_blocks(8);
 	hipLaunchKernelGGL((k_chunk_part<R>), dim3(g), dim3(kBlock), 0, s, S.n_chunks, (const uint32_t *)S.ch_row,
 	                   (const uint64_t *)S.ch_beg, (const uint32_t *)S.lr_rows, (const uint64_t *)S.off,
-	                   (const uint32_t *)S.idx, val, (const typename R::X *)x, (A *)S.ch_part, skip);
+	                   (const uint32_t *)S.idx, val, (const typename R::X *)x, (A *)S.ch_part, skip,
+	                   (uint64_t)(S.chunk ? S.chunk : kChunk));
 	const size_t g2 = (S.n_long + kBlock - 1) / kBlock;
 	hipLaunchKernelGGL((k_long_fold<R, MASK, OW>), dim3(g2), dim3(kBlock), 0, s, S.n_long,
 	                   (const uint32_t *)S.lr_rows, (const uint64_t *)S.lr_c0, (const A *)S.ch_part,
