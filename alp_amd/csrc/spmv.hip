@@ -190,7 +190,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_seg(
 	using Y = typename R::Y;
 	using VT = typename Val<R>::T;
 	constexpr int kTile = J * kBlock;
-	constexpr int U = 4;
+	// gathers in flight per thread: 8 for patterns (no value registers; PageRank
+	// at R-MAT scale 22: 34.2 against 35.0 ms), 4 with values (8 cost the 192^3
+	// FP64 stencil 490 -> 525 us)
+	constexpr int U = R::kPattern ? 8 : 4;
 	__shared__ uint32_t sidx[kTile];
 	__shared__ VT sval[R::kPattern ? 1 : kTile];
 	__shared__ A red[kBlock / kWave];
@@ -827,24 +830,24 @@ __global__ __launch_bounds__(kBlock) void k_chunk_part(size_t nchunks, const uin
 		const uint32_t r = lr_rows[ch_row[c]];
 		const uint64_t k0 = ch_beg[c], rend = off[r + 1];
 		const uint64_t k1 = k0 + chunk < rend ? k0 + chunk : rend;
-		// four entries' index, value and x loads in flight per thread, folded in
+		// eight entries' index, value and x loads in flight per thread, folded in
 		// index order (the same sequence as one entry per step)
 		using VT = typename Val<R>::T;
 		A acc = R::zero();
 		uint64_t k = k0 + threadIdx.x;
-		for (; k + 3 * kBlock < k1; k += 4 * kBlock) {
-			uint32_t j[4];
-			VT v[4];
+		for (; k + 7 * kBlock < k1; k += 8 * kBlock) {
+			uint32_t j[8];
+			VT v[8];
 #pragma unroll
-			for (int u = 0; u < 4; ++u) {
+			for (int u = 0; u < 8; ++u) {
 				j[u] = __builtin_nontemporal_load(idx + k + u * kBlock);
 				v[u] = Val<R>::load(val, k + u * kBlock);
 			}
-			typename R::X xv[4];
+			typename R::X xv[8];
 #pragma unroll
-			for (int u = 0; u < 4; ++u) xv[u] = x[j[u]];
+			for (int u = 0; u < 8; ++u) xv[u] = x[j[u]];
 #pragma unroll
-			for (int u = 0; u < 4; ++u) acc = R::add(acc, Val<R>::mul(v[u], xv[u]));
+			for (int u = 0; u < 8; ++u) acc = R::add(acc, Val<R>::mul(v[u], xv[u]));
 		}
 		for (; k < k1; k += kBlock)
 			acc = R::add(acc, Val<R>::mul(Val<R>::load(val, k), x[__builtin_nontemporal_load(idx + k)]));
