@@ -20,6 +20,8 @@
  *   hip/blas2.hpp       mxv, vxm (masks, descriptors, sparse inputs and outputs);
  *   hip/blas3.hpp       mxm;
  *   hip/exec.hpp        Launcher, Benchmarker, spmd, collectives, utils::Timer;
+ *   hip/pipeline.hpp    Pipeline: a recurring sequence of primitives launched as
+ *                       hipGraphs (this backend's form of nonblocking execution);
  * and graphblas/algorithms/: conjugate_gradient, hpcg, simple_pagerank, knn.
  *
  * Every primitive is generic over the program's operators, monoids and
@@ -29,8 +31,11 @@
  * the device CG / HPCG / PageRank / k-hop solvers), the layer calls those.
  *
  * Differences from ALP's reference backend, all from device residence:
- *   - a call returns once its result is complete (blocking, as the reference);
- *     nonzero counts are exact after every call (one small readback);
+ *   - execution is lazy, as on ALP's nonblocking backend: a primitive enqueues
+ *     its kernels on the library's stream and returns; the host waits at forcing
+ *     points only (a scalar output, a nonzero count the host does not know, a
+ *     copy to or from host memory, freeing device memory, grb::wait). Errors of
+ *     enqueued work surface at the next forcing point;
  *   - eWiseLambda takes a device callable, which reaches vector elements through
  *     grb::view(vector) captured by value (device code cannot dereference host
  *     references);
@@ -48,5 +53,6 @@
 #include "graphblas/hip/blas2.hpp"
 #include "graphblas/hip/blas3.hpp"
 #include "graphblas/hip/exec.hpp"
+#include "graphblas/hip/pipeline.hpp"
 
 #endif  // ALP_HIP_GRAPHBLAS_HPP

@@ -127,6 +127,8 @@ RC cg_device(Vector<double, b, C> &x, const Matrix<double, b, R, Cc, N> &A, cons
 	if (ret) return ret;
 	size_t it = 0;
 	double res = 0.0;
+	grb::internal::hip::Forcing force;  // the library's solver is blocking
+	if (force.rc) return force.rc;
 	const RC rc = grb::internal::rc(
 	    alp_hip_cg(A.handle(), bv.raw(), x.raw(), max_iterations, (double)tol, 0, &it, &res));
 	iterations = it;

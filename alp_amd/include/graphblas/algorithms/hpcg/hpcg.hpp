@@ -27,6 +27,8 @@ grb::RC hpcg(hpcg_data<IOType, NonzeroType, InputType> &data, bool with_precondi
 	static_assert(std::is_same<IOType, double>::value && std::is_same<NonzeroType, double>::value &&
 	                  std::is_same<InputType, double>::value,
 	              "hpcg on the hip backend: double systems");
+	grb::internal::hip::Forcing force;  // the library's solver is blocking
+	if (force.rc) return force.rc;
 	if (!data.device || data.presmooth != presmoother_steps || data.postsmooth != postsmoother_steps) {
 		if (data.device) alp_hip_hpcg_destroy(data.device);
 		data.device = nullptr;

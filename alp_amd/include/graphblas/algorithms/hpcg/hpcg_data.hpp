@@ -64,7 +64,10 @@ struct hpcg_data : public system_data<IOType, NonzeroType> {
 	hpcg_data(std::size_t n) : system_data<IOType, NonzeroType>(n), b(n), u(n), p(n), x(n) {}
 	virtual ~hpcg_data() {
 		delete coarser_level;
-		if (device) alp_hip_hpcg_destroy(device);
+		if (device) {
+			grb::internal::hip::Forcing force;
+			alp_hip_hpcg_destroy(device);
+		}
 	}
 };
 

@@ -41,6 +41,8 @@ grb::RC build_hpcg_system(std::unique_ptr<hpcg_data<T, T, T>> &holder, hpcg_syst
 	hpcg_data<T, T, T> *data = new hpcg_data<T, T, T>(n);
 	holder = std::unique_ptr<hpcg_data<T, T, T>>(data);
 	alp_hip_matrix_t h = nullptr;
+	grb::internal::hip::Forcing force;  // a library build is blocking
+	if (force.rc) return force.rc;
 	RC rc = grb::internal::rc(alp_hip_matrix_stencil27(&h, s[0], s[1], s[2], params.diag_value,
 	                                                   params.non_diag_value, 0));
 	if (rc) return rc;

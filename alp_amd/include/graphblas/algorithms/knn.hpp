@@ -34,6 +34,8 @@ RC knn(Vector<bool, b, C> &u, const Matrix<InputType, b, R, Cc, N> &A, const siz
 		RC r = clear(u);
 		return r ? r : setElement(u, true, source);
 	}
+	grb::internal::hip::Forcing force;  // the library's solver is blocking
+	if (force.rc) return force.rc;
 	RC r = grb::internal::rc(alp_hip_khop(A.handle(), (uint32_t)source, k, reinterpret_cast<uint8_t *>(u.raw())));
 	if (r) return r;
 	hipLaunchKernelGGL(internal::k_khop_assigned<bool>, dim3(grb::internal::hip::grid(n)),

@@ -37,6 +37,8 @@ RC simple_pagerank(Vector<IOType, b, C> &pr, const Matrix<NonzeroT, b, R, Cc, N>
 	}
 	size_t it = 0;
 	double q = 0;
+	grb::internal::hip::Forcing force;  // the library's solver is blocking
+	if (force.rc) return force.rc;
 	r = grb::internal::rc(alp_hip_pagerank(L.handle(), pr.raw(), alpha, conv, max, &it, &q));
 	if (iterations) *iterations = it;
 	if (quality) *quality = q;

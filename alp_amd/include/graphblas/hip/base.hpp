@@ -66,13 +66,6 @@ inline RC rc(alp_rc_t r) { return static_cast<RC>(r); }
 
 }  // namespace internal
 
-// grb::init / grb::finalize (base/init.hpp:144-210) and grb::wait (the hip
-// backend is blocking: wait only drains the library's stream)
-inline RC init() { return internal::rc(alp_hip_init(0)); }
-inline RC init(size_t /*s*/, size_t /*P*/, void * /*lpf*/) { return init(); }
-inline RC finalize() { return internal::rc(alp_hip_finalize()); }
-inline RC wait() { return internal::rc(alp_hip_sync()); }
-
 namespace internal {
 namespace hip {
 
@@ -80,7 +73,7 @@ constexpr int kBlock = 256;
 
 // the library's stream: every kernel of this layer runs on it, so it is ordered
 // with the library's own work
-inline hipStream_t stream() {
+inline hipStream_t library_stream() {
 	void *s = nullptr;
 	if (alp_hip_get_stream(&s) != ALP_SUCCESS) throw std::runtime_error("grb: no device");
 	return static_cast<hipStream_t>(s);
@@ -92,7 +85,58 @@ inline RC from_hip(hipError_t e) {
 	return e == hipErrorOutOfMemory ? OUTOFMEM : PANIC;
 }
 
-inline RC sync() { return from_hip(hipStreamSynchronize(stream())); }
+// Lazy execution (the device form of the nonblocking backend's lazy evaluation,
+// docs/Nonblocking_backend.md "Lazy evaluation"). A primitive only enqueues its
+// kernels on the library's stream and returns; the stream keeps them in program
+// order. The host waits only at a forcing point, where it needs what the device
+// computed: a scalar output, a nonzero count that only the device knows, a copy
+// between host and device memory, the release of device memory, grb::wait.
+//
+// Inside a grb::Pipeline call, the work enqueued between two forcing points is
+// captured into a hipGraph (a segment) and launched as one graph at the next
+// forcing point or at the end of the call. SegmentSink is that capture.
+struct SegmentSink {
+	virtual void open() = 0;   // begin capturing the stream, if not already
+	virtual RC close() = 0;    // end the capture, if any, and launch what it holds
+	virtual ~SegmentSink() = default;
+};
+struct LazyState {
+	SegmentSink *sink = nullptr;  // the running grb::Pipeline call, if any
+	int forcing = 0;              // depth of forcing scopes: nothing is captured inside one
+};
+inline LazyState &lazy() {
+	static thread_local LazyState s;
+	return s;
+}
+
+// the stream a primitive enqueues on (inside a pipeline this opens a segment)
+inline hipStream_t stream() {
+	const hipStream_t s = library_stream();
+	LazyState &l = lazy();
+	if (l.sink && l.forcing == 0) l.sink->open();
+	return s;
+}
+
+// a forcing point: the open segment, if any, is launched first, and work in the
+// scope runs directly on the stream
+struct Forcing {
+	RC rc = SUCCESS;
+	Forcing() {
+		LazyState &l = lazy();
+		if (l.sink && l.forcing == 0) rc = l.sink->close();
+		++l.forcing;
+	}
+	~Forcing() { --lazy().forcing; }
+	Forcing(const Forcing &) = delete;
+	Forcing &operator=(const Forcing &) = delete;
+};
+
+// wait for everything enqueued so far
+inline RC sync() {
+	Forcing f;
+	if (f.rc) return f.rc;
+	return from_hip(hipStreamSynchronize(library_stream()));
+}
 
 inline unsigned grid(size_t work, size_t cap_blocks = 256 * 16) {
 	size_t g = (work + kBlock - 1) / kBlock;
@@ -120,13 +164,19 @@ struct DevArray {
 		return *this;
 	}
 	~DevArray() { release(); }
+	// allocating and freeing are forcing points: hipFree waits for the device, and
+	// a captured segment must not name memory freed before it runs
 	void release() {
-		if (p) (void)alp_hip_free(p);
+		if (p) {
+			Forcing f;
+			(void)alp_hip_free(p);
+		}
 		p = nullptr;
 		n = 0;
 	}
 	void alloc(size_t count) {
 		release();
+		Forcing f;
 		void *q = nullptr;
 		if (alp_hip_malloc(&q, sizeof(T) * (count ? count : 1)) != ALP_SUCCESS) throw std::bad_alloc();
 		p = static_cast<T *>(q);
@@ -134,15 +184,22 @@ struct DevArray {
 	}
 };
 
-// one counter on the device, read back after the kernels that add to it
-struct Counter {
-	DevArray<unsigned long long> c{1};
-	RC reset() { return from_hip(hipMemsetAsync(c.p, 0, sizeof(unsigned long long), stream())); }
-	RC read(unsigned long long &out) {
-		RC r = from_hip(hipMemcpyAsync(&out, c.p, sizeof(out), hipMemcpyDeviceToHost, stream()));
-		return r ? r : sync();
+// device memory that a forcing primitive (a reduction) reuses from call to call:
+// grown when a call needs more, kept until the process ends
+inline void *workspace(size_t bytes) {
+	static thread_local void *p = nullptr;
+	static thread_local size_t cap = 0;
+	if (bytes > cap) {
+		Forcing f;
+		if (p) (void)alp_hip_free(p);
+		p = nullptr;
+		cap = 0;
+		size_t want = bytes < 4096 ? 4096 : bytes;
+		if (alp_hip_malloc(&p, want) != ALP_SUCCESS) throw std::bad_alloc();
+		cap = want;
 	}
-};
+	return p;
+}
 
 template <typename T>
 __global__ void k_fill(size_t n, T *x, T v) {
@@ -162,6 +219,14 @@ __global__ void k_count(size_t n, const F *a, unsigned long long *count) {
 
 }  // namespace hip
 }  // namespace internal
+
+// grb::init / grb::finalize (base/init.hpp:144-210); grb::wait
+// (base/io.hpp:1792-1880) is a forcing point: it returns once all enqueued work
+// is done, with the first error that work met
+inline RC init() { return internal::rc(alp_hip_init(0)); }
+inline RC init(size_t /*s*/, size_t /*P*/, void * /*lpf*/) { return init(); }
+inline RC finalize() { return internal::rc(alp_hip_finalize()); }
+inline RC wait() { return internal::hip::sync(); }
 
 }  // namespace grb
 

@@ -35,11 +35,13 @@ struct Arg {
 	__device__ __forceinline__ bool has(size_t i) const { return scalar || !a || a[i]; }
 	__device__ __forceinline__ T get(size_t i) const { return scalar ? s : v[i]; }
 };
+// a vector operand: without its flags when the host knows it is dense (a pending
+// count is not waited for: the flags are passed instead)
 template <typename T, enum Backend b, typename C>
 Arg<T> arg(const Vector<T, b, C> &x) {
 	Arg<T> r;
 	r.v = x.raw();
-	r.a = x.dense() ? nullptr : x.assigned();
+	r.a = x.known_dense() ? nullptr : x.assigned();
 	return r;
 }
 template <typename T>
@@ -141,6 +143,7 @@ __global__ void k_ewise(size_t n, Z *z, uint8_t *za, Arg<X> x, Arg<Y> y, Arg<W> 
 		za[i] = zout ? 1 : 0;
 		c += zout ? 1 : 0;
 	}
+	if (!count) return;
 	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
 	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
 }
@@ -199,24 +202,30 @@ __global__ void k_reduce_final(unsigned nb, const T *part, const uint8_t *part_h
 	*out_has = rh ? 1 : 0;
 }
 
+// a scalar output: a forcing point (the host waits for the value)
 template <class OP, typename T, class Term>
 RC reduce(size_t n, Term term, T &result, bool &present) {
 	present = false;
+	Forcing f;
+	if (f.rc) return f.rc;
 	if (n == 0) return SUCCESS;
 	unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
 	if (nb > kReduceBlocks) nb = kReduceBlocks;
-	DevArray<T> part(nb), out(1);
-	DevArray<uint8_t> part_has(nb), out_has(1);
-	hipLaunchKernelGGL((k_reduce_blocks<OP, T, Term>), dim3(nb), dim3(kBlock), 0, stream(), n, term, part.p,
-	                   part_has.p);
-	hipLaunchKernelGGL((k_reduce_final<OP, T>), dim3(1), dim3(64), 0, stream(), nb, (const T *)part.p,
-	                   (const uint8_t *)part_has.p, out.p, out_has.p);
+	// workspace: nb + 1 values (block results, then the result), then nb + 1 flags
+	const size_t vbytes = (sizeof(T) * (nb + 1) + 15) / 16 * 16;
+	char *ws = static_cast<char *>(workspace(vbytes + nb + 1));
+	T *part = reinterpret_cast<T *>(ws), *out = part + nb;
+	uint8_t *part_has = reinterpret_cast<uint8_t *>(ws + vbytes), *out_has = part_has + nb;
+	hipLaunchKernelGGL((k_reduce_blocks<OP, T, Term>), dim3(nb), dim3(kBlock), 0, library_stream(), n, term, part,
+	                   part_has);
+	hipLaunchKernelGGL((k_reduce_final<OP, T>), dim3(1), dim3(64), 0, library_stream(), nb, (const T *)part,
+	                   (const uint8_t *)part_has, out, out_has);
 	RC r = from_hip(hipGetLastError());
 	if (r) return r;
 	uint8_t h = 0;
-	r = from_hip(hipMemcpyAsync(&result, out.p, sizeof(T), hipMemcpyDeviceToHost, stream()));
-	r = r ? r : from_hip(hipMemcpyAsync(&h, out_has.p, 1, hipMemcpyDeviceToHost, stream()));
-	r = r ? r : sync();
+	r = from_hip(hipMemcpyAsync(&result, out, sizeof(T), hipMemcpyDeviceToHost, library_stream()));
+	r = r ? r : from_hip(hipMemcpyAsync(&h, out_has, 1, hipMemcpyDeviceToHost, library_stream()));
+	r = r ? r : from_hip(hipStreamSynchronize(library_stream()));
 	present = h != 0;
 	return r;
 }
@@ -247,28 +256,44 @@ struct FoldTerm {
 	}
 };
 
+// whether z comes out dense whatever the values are, from what the host knows:
+// dense (flag-free) operands, a dense z for the in-place modes, and no mask or a
+// structural mask that holds everywhere
+template <int MODE, bool MONOID, typename X, typename Y, typename W>
+bool ewise_dense_result(bool z_dense, const Arg<X> &x, const Arg<Y> &y, const Arg<W> &w, bool mask_all) {
+	const bool xd = x.scalar || !x.a, yd = y.scalar || !y.a, wd = w.scalar || !w.a;
+	if (MODE == kApplyOp) return mask_all && xd && yd;
+	if (MODE == kApplyMonoid) return mask_all && (xd || yd);
+	if (z_dense) return true;  // an in-place result keeps every entry it had
+	if (!mask_all) return false;
+	if (MODE == kMulIn) return xd && yd;
+	if (MODE == kAddIn) return xd || yd;
+	if (MODE == kMulAddIn) return (xd && yd) || wd;
+	if (MODE == kFoldl) return MONOID && yd;
+	return MONOID && xd;  // kFoldr
+}
+
 template <int MODE, bool MONOID, Descriptor descr, class ADD, class MUL, typename Z, typename X, typename Y,
           typename W, typename M, enum Backend b, typename C>
 RC ewise(Vector<Z, b, C> &z, const Arg<X> &x, const Arg<Y> &y, const Arg<W> &w, const Vector<M, b, C> *mask) {
 	const size_t n = z.size();
-	Counter cnt;
-	RC r = cnt.reset();
-	if (r) return r;
 	const bool masked = mask && mask->size() > 0;
+	const bool mask_all = !masked || ((descr & descriptors::structural) && !(descr & descriptors::invert_mask) &&
+	                                  mask->known_dense());
+	const bool dense_out = ewise_dense_result<MODE, MONOID>(z.known_dense(), x, y, w, mask_all);
+	unsigned long long *count = dense_out ? nullptr : z.count_slot();
 	if (n) {
 		if (masked)
 			hipLaunchKernelGGL((k_ewise<MODE, MONOID, descr, true, ADD, MUL, Z, X, Y, W, M>), dim3(grid(n)),
 			                   dim3(kBlock), 0, stream(), n, z.raw(), z.assigned(), x, y, w, mask->raw(),
-			                   mask->dense() ? nullptr : mask->assigned(), cnt.c.p);
+			                   mask->known_dense() ? nullptr : mask->assigned(), count);
 		else
 			hipLaunchKernelGGL((k_ewise<MODE, MONOID, descr, false, ADD, MUL, Z, X, Y, W, M>), dim3(grid(n)),
 			                   dim3(kBlock), 0, stream(), n, z.raw(), z.assigned(), x, y, w, (const M *)nullptr,
-			                   (const uint8_t *)nullptr, cnt.c.p);
+			                   (const uint8_t *)nullptr, count);
 	}
-	unsigned long long nz = 0;
-	r = cnt.read(nz);
-	z.set_nnz((size_t)nz);
-	return r;
+	if (dense_out) z.set_nnz(n); else z.mark_counted();
+	return from_hip(hipGetLastError());
 }
 
 template <typename V, typename T, enum Backend b, typename C>
@@ -319,8 +344,9 @@ RC axpy_fast(Z &, const X &, const Y &, bool &done, std::false_type) {
 template <enum Backend b, typename C, typename X>
 RC axpy_fast(Vector<double, b, C> &z, const X &alpha, const Vector<double, b, C> &y, bool &done, std::true_type) {
 	done = false;
-	if (!z.dense() || !y.dense() || z.raw() == y.raw()) return SUCCESS;
+	if (!z.known_dense() || !y.known_dense() || z.raw() == y.raw()) return SUCCESS;
 	done = true;
+	(void)stream();  // enqueued: inside a pipeline this joins the open segment
 	return rc(alp_hip_axpy_f64(z.size(), (double)alpha, y.raw(), z.raw()));
 }
 
@@ -350,6 +376,8 @@ RC dot(Z &z, const Vector<X, b, C> &x, const Vector<Y, b, C> &y, const Ring &rin
 	if (std::is_same<ADD, operators::add<double>>::value && std::is_same<MUL, operators::mul<double>>::value &&
 	    std::is_same<X, double>::value && std::is_same<Y, double>::value && x.dense() && y.dense()) {
 		double v = 0.0;
+		internal::hip::Forcing f;
+		if (f.rc) return f.rc;
 		RC r = internal::rc(alp_hip_dot_f64(x.size(), (const double *)x.raw(), (const double *)y.raw(), &v));
 		if (r) return r;
 		internal::foldl_op<ADD>(z, v);
@@ -598,11 +626,10 @@ RC eWiseLambda(const F f, const Vector<T, b, C> &x, const Vs &...others) {
 	RC r = internal::hip::lambda_sizes(x.size(), others...);
 	if (r) return r;
 	const size_t n = x.size();
-	if (n == 0 || x.nnz() == 0) return SUCCESS;
+	if (n == 0 || (!x.count_pending() && x.nnz() == 0)) return SUCCESS;
 	hipLaunchKernelGGL((internal::hip::k_lambda<F>), dim3(internal::hip::grid(n)), dim3(internal::hip::kBlock), 0,
-	                   internal::hip::stream(), n, x.dense() ? nullptr : x.assigned(), f);
-	r = internal::hip::from_hip(hipGetLastError());
-	return r ? r : internal::hip::sync();
+	                   internal::hip::stream(), n, x.known_dense() ? nullptr : x.assigned(), f);
+	return internal::hip::from_hip(hipGetLastError());
 }
 
 }  // namespace grb

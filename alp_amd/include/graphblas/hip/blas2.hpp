@@ -135,6 +135,7 @@ __global__ void k_mv_gather(size_t m, size_t nsrc, const uint64_t *off, const ui
 		ya[i] = yout ? 1 : 0;
 		c += yout ? 1 : 0;
 	}
+	if (!count) return;
 	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
 	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
 }
@@ -214,6 +215,7 @@ __global__ void k_mv_push_finish(size_t m, size_t nsrc, Arg<X> x, const unsigned
 		ya[i] = yout ? 1 : 0;
 		c += yout ? 1 : 0;
 	}
+	if (!count) return;
 	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
 	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
 }
@@ -257,6 +259,7 @@ bool mv_library(Vector<IO, b, C> &u, const Vector<M, b, C> *mask, const Matrix<T
 	}
 	const unsigned d = descr & (descriptors::invert_mask | descriptors::transpose_matrix | descriptors::dense |
 	                            descriptors::add_identity);
+	(void)stream();  // enqueued: inside a pipeline this joins the open segment
 	rc = MXV ? grb::internal::rc(alp_hip_mxv(A.handle(), (alp_semiring_t)RING, d, u.raw(), v.raw(), mp))
 	         : grb::internal::rc(alp_hip_vxm(A.handle(), (alp_semiring_t)RING, d, u.raw(), v.raw(), mp));
 	return true;
@@ -309,20 +312,20 @@ RC mv(Vector<IO, b, C> &u, const Vector<M, b, C> *mask, const Matrix<T, b, R, Cc
 		RC r = SUCCESS;
 		if (mv_library<descr, ring, MXV>(u, mask, A, v, r,
 		                                 std::integral_constant<bool, (ring >= 0) && lib_matrix_ok<ring, T>::value>())) {
-			if (r) return r;
-			return sync();
+			return r;  // u was dense and stays so
 		}
 	}
 	typename Matrix<T, b, R, Cc, N>::Storage S;
 	RC r = A.storage(use_crs ? 0 : 1, S);
 	if (r) return r;
-	Counter cnt;
-	r = cnt.reset();
-	if (r) return r;
+	// an output that was dense stays dense (entries are only added); otherwise the
+	// kernel counts into u's device slot
+	const bool dense_out = u.known_dense();
+	unsigned long long *count = dense_out ? nullptr : u.count_slot();
 	const M *mval = masked ? mask->raw() : nullptr;
-	const uint8_t *masg = masked && !mask->dense() ? mask->assigned() : nullptr;
+	const uint8_t *masg = masked && !mask->known_dense() ? mask->assigned() : nullptr;
 	const VM *vmval = in_masked ? v_mask->raw() : nullptr;
-	const uint8_t *vmasg = in_masked && !v_mask->dense() ? v_mask->assigned() : nullptr;
+	const uint8_t *vmasg = in_masked && !v_mask->known_dense() ? v_mask->assigned() : nullptr;
 	typedef IdentityOf<ADDM::template Identity> ID0;
 	typedef IdentityOf<One> ONE;
 	constexpr bool exact = std::is_integral<OUT>::value && exact_fold_op<ADD>::value && sizeof(OUT) <= 8 &&
@@ -333,15 +336,16 @@ RC mv(Vector<IO, b, C> &u, const Vector<M, b, C> *mask, const Matrix<T, b, R, Cc
 		typename Matrix<T, b, R, Cc, N>::Storage ST;
 		r = A.storage(use_crs ? 1 : 0, ST);
 		if (r) return r;
-		DevArray<unsigned long long> acc(out_n);
-		DevArray<uint8_t> touched(out_n);
-		hipLaunchKernelGGL(k_fill_identity<OUT>, dim3(grid(out_n)), dim3(kBlock), 0, stream(), out_n, acc.p,
-		                   touched.p, ADDM::template getIdentity<OUT>());
+		// per-output accumulators and touched flags in the reused workspace
+		unsigned long long *acc = static_cast<unsigned long long *>(workspace(out_n * (sizeof(unsigned long long) + 1)));
+		uint8_t *touched = reinterpret_cast<uint8_t *>(acc + out_n);
+		hipLaunchKernelGGL(k_fill_identity<OUT>, dim3(grid(out_n)), dim3(kBlock), 0, stream(), out_n, acc,
+		                   touched, ADDM::template getIdentity<OUT>());
 #define ALP_HIP_PUSH(MK, IM)                                                                                     \
 	hipLaunchKernelGGL((k_mv_push<descr, !MXV, MK, IM, ADD, MUL, ID0, ONE, X, typename std::remove_cv<         \
 	                                 typename std::remove_pointer<decltype(ST.val)>::type>::type, M, VM>),      \
-	                   dim3(grid(in_n)), dim3(kBlock), 0, stream(), in_n, ST.off, ST.idx, ST.val, arg(v), acc.p, \
-	                   touched.p, mval, masg, vmval, vmasg)
+	                   dim3(grid(in_n)), dim3(kBlock), 0, stream(), in_n, ST.off, ST.idx, ST.val, arg(v), acc, \
+	                   touched, mval, masg, vmval, vmasg)
 		if (masked) {
 			if (in_masked) ALP_HIP_PUSH(true, true); else ALP_HIP_PUSH(true, false);
 		} else {
@@ -350,19 +354,19 @@ RC mv(Vector<IO, b, C> &u, const Vector<M, b, C> *mask, const Matrix<T, b, R, Cc
 #undef ALP_HIP_PUSH
 		if (masked)
 			hipLaunchKernelGGL((k_mv_push_finish<descr, true, ADD, ID0, IO, X, M>), dim3(grid(out_n)), dim3(kBlock), 0,
-			                   stream(), out_n, in_n, arg(v), (const unsigned long long *)acc.p,
-			                   (const uint8_t *)touched.p, u.raw(), u.assigned(), mval, masg, cnt.c.p);
+			                   stream(), out_n, in_n, arg(v), (const unsigned long long *)acc,
+			                   (const uint8_t *)touched, u.raw(), u.assigned(), mval, masg, count);
 		else
 			hipLaunchKernelGGL((k_mv_push_finish<descr, false, ADD, ID0, IO, X, M>), dim3(grid(out_n)), dim3(kBlock),
-			                   0, stream(), out_n, in_n, arg(v), (const unsigned long long *)acc.p,
-			                   (const uint8_t *)touched.p, u.raw(), u.assigned(), mval, masg, cnt.c.p);
+			                   0, stream(), out_n, in_n, arg(v), (const unsigned long long *)acc,
+			                   (const uint8_t *)touched, u.raw(), u.assigned(), mval, masg, count);
 	} else {
 		typedef typename std::remove_cv<typename std::remove_pointer<decltype(S.val)>::type>::type VT;
 		constexpr bool imm = has_immutable_nonzeroes<ADDM>::value;
 #define ALP_HIP_GATHER(MK, IM)                                                                                    \
 	hipLaunchKernelGGL((k_mv_gather<descr, !MXV, MK, IM, imm, ADD, MUL, ID0, ONE, IO, X, VT, M, VM>),            \
 	                   dim3(grid(out_n)), dim3(kBlock), 0, stream(), out_n, in_n, S.off, S.idx, S.val, arg(v),     \
-	                   u.raw(), u.assigned(), mval, masg, vmval, vmasg, cnt.c.p)
+	                   u.raw(), u.assigned(), mval, masg, vmval, vmasg, count)
 		if (masked) {
 			if (in_masked) ALP_HIP_GATHER(true, true); else ALP_HIP_GATHER(true, false);
 		} else {
@@ -370,12 +374,8 @@ RC mv(Vector<IO, b, C> &u, const Vector<M, b, C> *mask, const Matrix<T, b, R, Cc
 		}
 #undef ALP_HIP_GATHER
 	}
-	r = from_hip(hipGetLastError());
-	if (r) return r;
-	unsigned long long nz = 0;
-	r = cnt.read(nz);
-	u.set_nnz((size_t)nz);
-	return r;
+	if (dense_out) u.set_nnz(out_n); else u.mark_counted();
+	return from_hip(hipGetLastError());
 }
 
 }  // namespace hip

@@ -90,6 +90,10 @@ RC mxm(Matrix<OutT, b, R, C, N> &Cm, const Matrix<T1, b, R, C, N> &A, const Matr
 	if (phase == RESIZE) return SUCCESS;
 	typedef typename ADDM::Operator ADD;
 	if (!A.handle() || !B.handle() || A.nonzeroes() == 0 || B.nonzeroes() == 0) return SUCCESS;  // C unchanged
+	// a forcing point: the product's size is decided on the host (the reference's
+	// nonblocking backend also runs level-3 primitives blocking)
+	Forcing f;
+	if (f.rc) return f.rc;
 	constexpr int ring = lib_mxm<ADD, MUL, OutT, T1, T2>::value;
 	const unsigned d = (tl ? ALP_DESC_TRANSPOSE_LEFT : 0u) | (trr ? ALP_DESC_TRANSPOSE_RIGHT : 0u);
 	if (ring >= 0) return mxm_library<ring>(Cm, A, B, d, std::integral_constant<bool, (ring >= 0)>());

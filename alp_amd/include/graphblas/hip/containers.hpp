@@ -4,10 +4,13 @@
  *
  * Vector<T> (reference/vector.hpp:191 with Coordinates<reference>,
  * reference/coordinates.hpp:66-1905): a value array and an `assigned` byte per
- * element -- the Coordinates' sparsity -- plus the exact nonzero count, kept on
- * the host and updated by every primitive that may change it. A vector is dense
- * when its count equals its size. T is any trivially copyable type (bool is one
- * byte, 0 or 1, as the C ABI's boolean vectors).
+ * element -- the Coordinates' sparsity -- plus the exact nonzero count. A
+ * primitive whose result count follows from what the host knows (a dense result
+ * of dense operands) sets it on the host; otherwise its kernel counts into the
+ * vector's device slot and the count stays pending until something asks for it
+ * (nnz, dense, a read): that is a forcing point. A vector is dense when its count
+ * equals its size. T is any trivially copyable type (bool is one byte, 0 or 1, as
+ * the C ABI's boolean vectors).
  *
  * Matrix<T> (reference/matrix.hpp:1018) wraps a libalp_hip.so handle that owns
  * the CRS and CCS in ALP's Compressed_Storage layout. Values of double and float
@@ -53,28 +56,33 @@ public:
 	explicit Vector(const size_t n = 0) : n_(n) {
 		if (internal::rc(alp_hip_init(0)) != SUCCESS) throw std::runtime_error("grb::Vector: no device");
 		val_.alloc(n_);
-		asg_.alloc(n_);
-		if (n_) {
-			hipLaunchKernelGGL(internal::hip::k_fill<uint8_t>, dim3(internal::hip::grid(n_)),
-			                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), n_, asg_.p, (uint8_t)0);
-		}
+		// the assigned flags, then (8-byte aligned) the device slot of the count
+		asg_.alloc(slot_offset() + sizeof(unsigned long long));
+		const size_t bytes = slot_offset() + sizeof(unsigned long long);
+		hipLaunchKernelGGL(internal::hip::k_fill<uint8_t>, dim3(internal::hip::grid(bytes)),
+		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), bytes, asg_.p, (uint8_t)0);
 	}
 	Vector(const size_t n, const size_t /*capacity*/) : Vector(n) {}
 	Vector(const Vector &o) : Vector(o.n_) {
+		nz_ = o.nnz();
 		if (n_) {
 			(void)hipMemcpyAsync(val_.p, o.val_.p, sizeof(T) * n_, hipMemcpyDeviceToDevice, internal::hip::stream());
 			(void)hipMemcpyAsync(asg_.p, o.asg_.p, n_, hipMemcpyDeviceToDevice, internal::hip::stream());
 		}
-		nz_ = o.nz_;
 	}
-	Vector(Vector &&o) noexcept : val_(std::move(o.val_)), asg_(std::move(o.asg_)), n_(o.n_), nz_(o.nz_) {
+	Vector(Vector &&o) noexcept
+	    : val_(std::move(o.val_)), asg_(std::move(o.asg_)), n_(o.n_), nz_(o.nz_), pending_(o.pending_),
+	      slot_dirty_(o.slot_dirty_) {
 		o.n_ = o.nz_ = 0;
+		o.pending_ = o.slot_dirty_ = false;
 	}
 	Vector &operator=(Vector &&o) noexcept {
 		std::swap(val_, o.val_);
 		std::swap(asg_, o.asg_);
 		std::swap(n_, o.n_);
 		std::swap(nz_, o.nz_);
+		std::swap(pending_, o.pending_);
+		std::swap(slot_dirty_, o.slot_dirty_);
 		return *this;
 	}
 	Vector &operator=(const Vector &o) {
@@ -84,8 +92,15 @@ public:
 	}
 
 	size_t size() const { return n_; }
-	size_t nnz() const { return nz_; }
-	bool dense() const { return nz_ == n_; }
+	// the exact count: a forcing point while the device holds it
+	size_t nnz() const {
+		resolve();
+		return nz_;
+	}
+	bool dense() const { return nnz() == n_; }
+	// dense as far as the host knows without waiting (false while a count is pending)
+	bool known_dense() const { return !pending_ && nz_ == n_; }
+	bool count_pending() const { return pending_; }
 
 	// device arrays (internal::getRaw, internal::getCoordinates)
 	T *raw() { return val_.p; }
@@ -93,32 +108,43 @@ public:
 	uint8_t *assigned() { return asg_.p; }
 	const uint8_t *assigned() const { return asg_.p; }
 
-	// the count after kernels that changed `assigned` in place
-	void set_nnz(size_t z) { nz_ = z; }
+	// The count after kernels that changed `assigned`: either known on the host
+	// (set_nnz), or added up by the next kernel into count_slot() (zeroed here
+	// first) and left pending (mark_counted) until it is asked for.
+	void set_nnz(size_t z) {
+		nz_ = z;
+		pending_ = false;
+	}
+	unsigned long long *count_slot() {
+		unsigned long long *c = slot();
+		if (slot_dirty_)
+			hipLaunchKernelGGL(internal::hip::k_fill<unsigned long long>, dim3(1), dim3(64), 0,
+			                   internal::hip::stream(), (size_t)1, c, 0ull);
+		slot_dirty_ = true;
+		return c;
+	}
+	void mark_counted() { pending_ = true; }
 	RC recount() {
-		internal::hip::Counter c;
-		RC r = c.reset();
-		if (r) return r;
+		unsigned long long *c = count_slot();
 		if (n_)
 			hipLaunchKernelGGL(internal::hip::k_count<uint8_t>, dim3(internal::hip::grid(n_)),
-			                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), n_, (const uint8_t *)asg_.p,
-			                   c.c.p);
-		unsigned long long v = 0;
-		r = c.read(v);
-		if (!r) nz_ = (size_t)v;
-		return r;
+			                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), n_, (const uint8_t *)asg_.p, c);
+		mark_counted();
+		return internal::hip::from_hip(hipGetLastError());
 	}
 
 	// host snapshot of the nonzeroes as (index, value) pairs, in index order
 	// (ALP's const_iterator over a vector yields the same pairs)
 	std::vector<std::pair<size_t, T>> nonzeroes() const {
+		internal::hip::Forcing f;
 		std::vector<std::pair<size_t, T>> out;
+		if (f.rc) throw std::runtime_error("grb::Vector: enqueued work failed");
 		if (!n_) return out;
 		std::unique_ptr<T[]> v(new T[n_]);
 		std::vector<uint8_t> a(n_);
 		if (alp_hip_memcpy_d2h(v.get(), val_.p, sizeof(T) * n_) || alp_hip_memcpy_d2h(a.data(), asg_.p, n_))
 			throw std::runtime_error("grb::Vector: download failed");
-		out.reserve(nz_);
+		out.reserve(nnz());
 		for (size_t i = 0; i < n_; ++i)
 			if (a[i]) out.emplace_back(i, v[i]);
 		return out;
@@ -154,9 +180,32 @@ public:
 	const_iterator end() const { return cend(); }
 
 private:
+	size_t slot_offset() const { return (n_ + 7) / 8 * 8; }
+	unsigned long long *slot() const { return reinterpret_cast<unsigned long long *>(asg_.p + slot_offset()); }
+	// a forcing point: reads the pending count back (the slot keeps the value until
+	// the next count zeroes it)
+	void resolve() const {
+		if (!pending_) return;
+		internal::hip::Forcing f;
+		unsigned long long v = 0;
+		hipError_t e = f.rc ? hipErrorUnknown : hipSuccess;
+		if (e == hipSuccess)
+			e = hipMemcpyAsync(&v, slot(), sizeof(v), hipMemcpyDeviceToHost, internal::hip::library_stream());
+		if (e == hipSuccess) e = hipStreamSynchronize(internal::hip::library_stream());
+		if (e != hipSuccess) {
+			(void)hipGetLastError();
+			throw std::runtime_error("grb::Vector: nonzero count readback failed");
+		}
+		nz_ = (size_t)v;
+		pending_ = false;
+	}
+
 	internal::hip::DevArray<T> val_;
 	internal::hip::DevArray<uint8_t> asg_;
-	size_t n_ = 0, nz_ = 0;
+	size_t n_ = 0;
+	mutable size_t nz_ = 0;
+	mutable bool pending_ = false;  // the count is in the device slot, not yet read
+	bool slot_dirty_ = false;       // the slot holds a count (zero it before the next)
 	mutable std::shared_ptr<const std::vector<std::pair<size_t, T>>> snap_;
 };
 
@@ -204,7 +253,10 @@ public:
 	Matrix(const Matrix &) = delete;
 	Matrix &operator=(const Matrix &) = delete;
 	~Matrix() {
-		if (h_) alp_hip_matrix_destroy(h_);
+		if (h_) {
+			internal::hip::Forcing f;
+			alp_hip_matrix_destroy(h_);
+		}
 	}
 
 	size_t rows() const { return m_; }
@@ -240,7 +292,10 @@ public:
 	// adopt a library handle (with, for values the library does not hold, the
 	// values in CRS and CCS order)
 	void reset(alp_hip_matrix_t h) {
-		if (h_) alp_hip_matrix_destroy(h_);
+		if (h_) {
+			internal::hip::Forcing f;
+			alp_hip_matrix_destroy(h_);
+		}
 		h_ = h;
 	}
 	void reset(alp_hip_matrix_t h, internal::hip::DevArray<VT> &&crs_val, internal::hip::DevArray<VT> &&ccs_val) {
@@ -251,6 +306,7 @@ public:
 
 	// host snapshot of the nonzeroes ((row, column), value) in CRS order
 	std::vector<std::pair<std::pair<size_t, size_t>, VT>> nonzero_list() const {
+		internal::hip::Forcing f;
 		std::vector<std::pair<std::pair<size_t, size_t>, VT>> out;
 		const size_t z = nonzeroes();
 		if (!h_) return out;
@@ -302,6 +358,16 @@ template <typename T, enum Backend b, typename R, typename C, typename N>
 RC resize(Matrix<T, b, R, C, N> &, const size_t) { return SUCCESS; }
 template <typename T, enum Backend b, typename C> uintptr_t getID(const Vector<T, b, C> &x) {
 	return reinterpret_cast<uintptr_t>(x.raw());
+}
+
+// grb::wait on given containers (base/io.hpp:1829-1880): waits for everything
+template <typename T, enum Backend b, typename C, typename... Args>
+RC wait(const Vector<T, b, C> &, const Args &...) {
+	return wait();
+}
+template <typename T, enum Backend b, typename R, typename C, typename N, typename... Args>
+RC wait(const Matrix<T, b, R, C, N> &, const Args &...) {
+	return wait();
 }
 
 namespace internal {

@@ -43,6 +43,7 @@ __global__ void k_set_masked(size_t n, T *x, uint8_t *xa, const M *mval, const u
 		xa[i] = on ? 1 : 0;
 		c += on ? 1 : 0;
 	}
+	if (!count) return;
 	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
 	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
 }
@@ -59,6 +60,7 @@ __global__ void k_copy(size_t n, T *x, uint8_t *xa, const U *y, const uint8_t *y
 		xa[i] = on ? 1 : 0;
 		c += on ? 1 : 0;
 	}
+	if (!count) return;
 	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
 	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
 }
@@ -90,7 +92,7 @@ RC clear(Vector<T, b, C> &x) {
 		hipLaunchKernelGGL(internal::hip::k_fill<uint8_t>, dim3(internal::hip::grid(x.size())),
 		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.assigned(), (uint8_t)0);
 	x.set_nnz(0);
-	return internal::hip::sync();
+	return internal::hip::from_hip(hipGetLastError());
 }
 
 // x_i = val for every i (reference/io.hpp:893): x becomes dense
@@ -103,7 +105,14 @@ RC set(Vector<T, b, C> &x, const V val, const Phase phase = EXECUTE,
 		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
 		                   static_cast<T>(val));
 	x.set_nnz(x.size());
-	return internal::hip::sync();
+	return internal::hip::from_hip(hipGetLastError());
+}
+
+// a structural mask that holds everywhere (dense, not inverted), as far as the
+// host knows without waiting
+template <Descriptor descr, typename M, enum Backend b, typename C>
+bool mask_holds_everywhere(const Vector<M, b, C> &mask) {
+	return (descr & descriptors::structural) && !(descr & descriptors::invert_mask) && mask.known_dense();
 }
 
 // x_i = val where the mask holds; x holds nothing else afterwards (io.hpp:977)
@@ -120,17 +129,14 @@ RC set(Vector<T, b, C> &x, const Vector<M, b, C> &mask, const V val, const Phase
 		RC c = clear(x);
 		if (c) return c;
 	}
-	internal::hip::Counter cnt;
-	RC r = cnt.reset();
-	if (r) return r;
+	const bool all = mask_holds_everywhere<descr>(mask);
+	unsigned long long *count = all ? nullptr : x.count_slot();
 	if (x.size())
 		hipLaunchKernelGGL((internal::hip::k_set_masked<descr, T, M, T>), dim3(internal::hip::grid(x.size())),
 		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
-		                   mask.raw(), mask.dense() ? nullptr : mask.assigned(), static_cast<T>(val), cnt.c.p);
-	unsigned long long z = 0;
-	r = cnt.read(z);
-	x.set_nnz((size_t)z);
-	return r;
+		                   mask.raw(), mask.known_dense() ? nullptr : mask.assigned(), static_cast<T>(val), count);
+	if (all) x.set_nnz(x.size()); else x.mark_counted();
+	return internal::hip::from_hip(hipGetLastError());
 }
 
 // x = y (io.hpp:1186), with y's sparsity
@@ -139,17 +145,15 @@ RC set(Vector<T, b, C> &x, const Vector<U, b, C> &y, const Phase phase = EXECUTE
 	if (x.size() != y.size()) return MISMATCH;
 	if ((const void *)&x == (const void *)&y) return ILLEGAL;
 	if (phase == RESIZE) return SUCCESS;
-	internal::hip::Counter cnt;
-	RC r = cnt.reset();
-	if (r) return r;
+	// x takes y's count: from the host when it is known there, else counted
+	const bool known = !y.count_pending();
+	unsigned long long *count = known ? nullptr : x.count_slot();
 	if (x.size())
 		hipLaunchKernelGGL((internal::hip::k_copy<descr, false, T, U, uint8_t>), dim3(internal::hip::grid(x.size())),
 		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
-		                   y.raw(), y.assigned(), (const uint8_t *)nullptr, (const uint8_t *)nullptr, cnt.c.p);
-	unsigned long long z = 0;
-	r = cnt.read(z);
-	x.set_nnz((size_t)z);
-	return r;
+		                   y.raw(), y.assigned(), (const uint8_t *)nullptr, (const uint8_t *)nullptr, count);
+	if (known) x.set_nnz(y.nnz()); else x.mark_counted();
+	return internal::hip::from_hip(hipGetLastError());
 }
 
 // x = y where the mask holds (io.hpp:1298)
@@ -160,17 +164,14 @@ RC set(Vector<T, b, C> &x, const Vector<M, b, C> &mask, const Vector<U, b, C> &y
 	if (x.size() != y.size() || mask.size() != x.size()) return MISMATCH;
 	if ((const void *)&x == (const void *)&y) return ILLEGAL;
 	if (phase == RESIZE) return SUCCESS;
-	internal::hip::Counter cnt;
-	RC r = cnt.reset();
-	if (r) return r;
+	const bool known = !y.count_pending() && mask_holds_everywhere<descr>(mask);
+	unsigned long long *count = known ? nullptr : x.count_slot();
 	if (x.size())
 		hipLaunchKernelGGL((internal::hip::k_copy<descr, true, T, U, M>), dim3(internal::hip::grid(x.size())),
 		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
-		                   y.raw(), y.assigned(), mask.raw(), mask.dense() ? nullptr : mask.assigned(), cnt.c.p);
-	unsigned long long z = 0;
-	r = cnt.read(z);
-	x.set_nnz((size_t)z);
-	return r;
+		                   y.raw(), y.assigned(), mask.raw(), mask.known_dense() ? nullptr : mask.assigned(), count);
+	if (known) x.set_nnz(y.nnz()); else x.mark_counted();
+	return internal::hip::from_hip(hipGetLastError());
 }
 
 // x_i = val (io.hpp setElement)
@@ -178,6 +179,8 @@ template <Descriptor descr = descriptors::no_operation, typename T, enum Backend
 RC setElement(Vector<T, b, C> &x, const V val, const size_t i, const Phase phase = EXECUTE) {
 	if (i >= x.size()) return MISMATCH;
 	if (phase == RESIZE) return SUCCESS;
+	internal::hip::Forcing f;  // copies from and to host memory
+	if (f.rc) return f.rc;
 	uint8_t was = 0;
 	if (alp_hip_memcpy_d2h(&was, x.assigned() + i, 1)) return PANIC;
 	const T v = static_cast<T>(val);
@@ -194,13 +197,16 @@ RC buildVector(Vector<T, b, C> &x, It begin, const It end, const IOMode) {
 	std::vector<T> h;
 	for (; begin != end; ++begin) h.push_back(static_cast<T>(*begin));
 	if (h.size() != x.size()) return MISMATCH;
+	internal::hip::Forcing f;  // a copy from host memory
+	if (f.rc) return f.rc;
 	if (!h.empty()) {
 		if (alp_hip_memcpy_h2d(x.raw(), h.data(), sizeof(T) * h.size())) return PANIC;
 		hipLaunchKernelGGL(internal::hip::k_fill<uint8_t>, dim3(internal::hip::grid(x.size())),
-		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.assigned(), (uint8_t)1);
+		                   dim3(internal::hip::kBlock), 0, internal::hip::library_stream(), x.size(), x.assigned(),
+		                   (uint8_t)1);
 	}
 	x.set_nnz(x.size());
-	return internal::hip::sync();
+	return internal::hip::from_hip(hipGetLastError());
 }
 
 // sparse build from index and value ranges; repeated indices fold with dup in
@@ -218,14 +224,17 @@ RC buildVector(Vector<T, b, C> &x, ItI ind, const ItI ind_end, ItV val, const It
 	}
 	if (ind != ind_end || val != val_end) return MISMATCH;
 	if (hi.empty()) return SUCCESS;
+	internal::hip::Forcing f;  // copies from host memory
+	if (f.rc) return f.rc;
 	internal::hip::DevArray<size_t> di(hi.size());
 	internal::hip::DevArray<T> dv(hv.size());
 	if (alp_hip_memcpy_h2d(di.p, hi.data(), sizeof(size_t) * hi.size()) ||
 	    alp_hip_memcpy_h2d(dv.p, hv.data(), sizeof(T) * hv.size()))
 		return PANIC;
-	hipLaunchKernelGGL((internal::hip::k_scatter_build<Dup, T>), dim3(1), dim3(64), 0, internal::hip::stream(),
-	                   hi.size(), (const size_t *)di.p, (const T *)dv.p, x.raw(), x.assigned());
-	return x.recount();
+	hipLaunchKernelGGL((internal::hip::k_scatter_build<Dup, T>), dim3(1), dim3(64), 0,
+	                   internal::hip::library_stream(), hi.size(), (const size_t *)di.p, (const T *)dv.p, x.raw(),
+	                   x.assigned());
+	return x.recount();  // di and dv are freed on return, after the scatter (hipFree waits)
 }
 
 // ---- matrices -------------------------------------------------------------------
@@ -236,6 +245,8 @@ template <typename T, enum Backend b, typename R, typename C, typename N, typena
 RC build_matrix(Matrix<T, b, R, C, N> &A, std::vector<uint32_t> &hi, std::vector<uint32_t> &hj,
                 const std::vector<VT> &hv) {
 	const size_t nz = hi.size();
+	hip::Forcing f;  // copies from host memory
+	if (f.rc) return f.rc;
 	alp_hip_matrix_t h = nullptr;
 	// the reference's buildMatrixUnique fills CRS and CCS (matrix.hpp:1854-1951)
 	if (std::is_void<T>::value || hip::lib_holds_values<T>::value) {

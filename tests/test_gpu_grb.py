@@ -250,3 +250,38 @@ def test_grb_hpcg_program_as_the_reference_smoke_test(golden):
     assert r["residual"] <= 1e-30 and g["residual"][0] <= 1e-30
     assert r["square_norm_diff"] < (4e-14) ** 2
     assert r["pinned_nonzeroes"] == 16 ** 3
+
+
+PIPE_PROG = os.path.join(ROOT, "tests", "cpp", "_build", "grb_pipeline_program")
+
+
+@pytest.mark.parametrize("sizes", [(16, 16, 16, 40), (12, 10, 9, 25)])
+def test_lazy_primitives_and_pipeline_replay(sizes):
+    """Lazy execution (primitives enqueue, the host waits at forcing points) and
+    grb::Pipeline (each call's device work captured into hipGraphs, segment by
+    segment, and launched by updating the previous call's graphs): every loop of
+    tests/cpp/grb_pipeline_program.cpp runs once primitive by primitive and once
+    as a Pipeline body from the same start, and the results must be the same
+    bits -- a dense Jacobi sweep (one segment per call), scalars that change
+    every call, a power iteration with its norm inside the body (the dot splits
+    the call into two segments), a value-masked set whose device-side count an
+    mxv inside the body asks for, and eight sweeps per call (one 32-launch
+    segment). The Jacobi-preconditioned CG in grb::
+    primitives must take the oracle CG's iteration count: with D = 26 I the
+    preconditioned iterates equal the plain ones in exact arithmetic."""
+    nx, ny, nz, sweeps = sizes
+    r = _run(PIPE_PROG, nx, ny, nz, sweeps)
+    assert r["rc"] == 0
+    assert r["a_equal"] == 1 and r["b_equal"] == 1 and r["c_equal"] == 1 and r["d_equal"] == 1
+    assert r["a_segments"] == 1 and r["c_segments"] == 2 and r["d_segments"] == 2
+    # after the first call, each call updates its graph in place
+    assert r["a_instantiated"] == 1 and r["a_updated"] == sweeps - 1
+    assert r["f_equal"] == 1 and r["f_segments"] == 1
+    assert r["b_updated"] == 19
+    n = nx * ny * nz
+    assert 0 < r["d_nnz"] < n
+    b = closed_form_A1(nx, ny, nz)
+    rc_o, _, it_o, _, _ = O.cg(O.stencil(nx, ny, nz), b, np.zeros(n), 1000, 1e-10)
+    assert r["pcg_rc"] == 0 and rc_o == 0
+    assert abs(r["pcg_iterations"] - it_o) <= 1
+    assert r["pcg_max_err"] <= 1e-7
