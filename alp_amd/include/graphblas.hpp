@@ -22,7 +22,7 @@
  *   hip/exec.hpp        Launcher, Benchmarker, spmd, collectives, utils::Timer;
  *   hip/pipeline.hpp    Pipeline: a recurring sequence of primitives launched as
  *                       hipGraphs (this backend's form of nonblocking execution);
- * and graphblas/algorithms/: conjugate_gradient, hpcg, simple_pagerank, knn.
+ * and graphblas/algorithms/: conjugate_gradient, hpcg, simple_pagerank, knn, mpv.
  *
  * Every primitive is generic over the program's operators, monoids and
  * semirings: its device kernels are templates instantiated for them. Where the

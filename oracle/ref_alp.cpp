@@ -19,6 +19,7 @@
  *   - hpcg                          include/graphblas/algorithms/hpcg/hpcg.hpp:97
  *   - simple_pagerank               include/graphblas/algorithms/simple_pagerank.hpp:146
  *   - knn                           include/graphblas/algorithms/knn.hpp:82
+ *   - mpv                           include/graphblas/algorithms/mpv.hpp:94
  *   - grb::mxm                      include/graphblas/reference/blas3.hpp:1744
  */
 #include <graphblas.hpp>
@@ -282,6 +283,31 @@ REF_API int ref_knn( size_t n, size_t nedges, const uint32_t * src, const uint32
 	rc = static_cast< int >( grb::algorithms::knn< grb::descriptors::no_operation >( u, A, source, k, buf1 ) );
 	std::fill( out, out + n, 0 );
 	for( const auto &pair : u ) { out[ pair.first ] = pair.second ? 1 : 2; }
+	return rc;
+}
+
+/** mpv (mpv.hpp:94): u = A^k v over plus-times, u and temp fresh; out holds u's
+ *  values and out_assigned 1 where u holds an entry. */
+REF_API int ref_mpv( size_t n, size_t nz, const uint32_t * I, const uint32_t * J, const double * V,
+	size_t k, const double * v, double * out, uint8_t * out_assigned
+) {
+	if( ensure_init() ) { return -1; }
+	grb::Matrix< double > A( n, n );
+	std::vector< size_t > i( I, I + nz ), j( J, J + nz );
+	std::vector< double > w( V, V + nz );
+	int rc = static_cast< int >( grb::buildMatrixUnique( A, i.data(), j.data(), w.data(), nz, grb::SEQUENTIAL ) );
+	if( rc ) { return rc; }
+	grb::Vector< double > u( n ), vv( n ), temp( n );
+	rc = from_host( vv, v );
+	if( rc ) { return rc; }
+	RealRing ring;
+	rc = static_cast< int >( grb::algorithms::mpv< grb::descriptors::no_operation >( u, A, k, vv, temp, ring ) );
+	std::fill( out, out + n, 0.0 );
+	std::fill( out_assigned, out_assigned + n, 0 );
+	for( const auto &pair : u ) {
+		out[ pair.first ] = pair.second;
+		out_assigned[ pair.first ] = 1;
+	}
 	return rc;
 }
 

@@ -8,6 +8,7 @@ must have built ``oracle/_ref/libalpref_seq.so``, ALP's sequential
     python tests/golden/make_golden.py transition   # the transition CG group only
     python tests/golden/make_golden.py cg_converge  # CG to 1e-10 at 64^3 and 192^3 only
     python tests/golden/make_golden.py mxm          # grb::mxm cases only
+    python tests/golden/make_golden.py mpv          # algorithms::mpv cases only
 
 (the transition group also needs ``oracle/_ref/libalpref_solver.so``).
 
@@ -226,13 +227,39 @@ def mxm_cases(manifest):
          **out)
 
 
+def mpv_cases(manifest):
+    """ALP's algorithms::mpv (mpv.hpp:94) over plus-times on a random 60 x 60
+    matrix (random_coo, seed 11, about 3 entries per row, some rows empty) and a
+    dense v (seed 12), for k = 0 .. 5: u's values and which entries u holds. The
+    reference accumulates each product into the vector it overwrites (mxv is
+    in place), so for k >= 3 u is not A^k v (k = 3 gives A v + A^3 v)."""
+    R = O.ref("seq")
+    n = 60
+    rng = np.random.default_rng(11)
+    I, J, V = random_coo(n, n, 3, rng)
+    v = np.random.default_rng(12).standard_normal(n)
+    out = dict(I=I, J=J, V=V, v=v)
+    for k in range(6):
+        u = np.zeros(n)
+        a = np.zeros(n, np.uint8)
+        rc = R.ref_mpv(n, len(I), P(I), P(J), P(V), k, P(v), P(u), P(a))
+        assert rc == 0, (k, rc)
+        out[f"u_{k}"] = u
+        out[f"assigned_{k}"] = a
+    save("mpv", manifest,
+         dict(kind="mpv", n=n, ring="plus-times f64", k=list(range(6)),
+              inputs="random_coo(60, 60, 3, default_rng(11)) in input order; v = default_rng(12).standard_normal(60)"),
+         **out)
+
+
 def main(only=None):
     manifest = {}
     mpath = os.path.join(HERE, "manifest.json")
     if only:
         with open(mpath) as f:
             manifest = json.load(f)
-        {"transition": transition_cg, "cg_converge": cg_converge, "mxm": mxm_cases}[only](manifest)
+        {"transition": transition_cg, "cg_converge": cg_converge, "mxm": mxm_cases,
+         "mpv": mpv_cases}[only](manifest)
         with open(mpath, "w") as f:
             json.dump(manifest, f, indent=1, sort_keys=True)
         print(f"updated {only}")
@@ -373,6 +400,7 @@ def main(only=None):
     transition_cg(manifest)
     cg_converge(manifest)
     mxm_cases(manifest)
+    mpv_cases(manifest)
 
     with open(mpath, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)

@@ -285,3 +285,102 @@ def test_lazy_primitives_and_pipeline_replay(sizes):
     assert r["pcg_rc"] == 0 and rc_o == 0
     assert abs(r["pcg_iterations"] - it_o) <= 1
     assert r["pcg_max_err"] <= 1e-7
+
+
+ALG_PROG = os.path.join(ROOT, "tests", "cpp", "_build", "grb_algorithms_program")
+
+
+def _write_general_mtx(path, n, I, J, V=None):
+    """MatrixMarket 'coordinate general' in the given entry order (1-based)."""
+    with open(path, "w") as f:
+        kind = "pattern" if V is None else "real"
+        f.write(f"%%MatrixMarket matrix coordinate {kind} general\n{n} {n} {len(I)}\n")
+        for t in range(len(I)):
+            if V is None:
+                f.write(f"{int(I[t]) + 1} {int(J[t]) + 1}\n")
+            else:
+                f.write(f"{int(I[t]) + 1} {int(J[t]) + 1} {float(V[t]):.17g}\n")
+
+
+def _hops(n, I, J, source, k, reverse=False):
+    """Vertices within k hops of source along the edges i -> j (j -> i reversed):
+    knn.hpp's k lor-land products with add_identity, by breadth-first search."""
+    adj = [[] for _ in range(n)]
+    for i, j in zip(I.tolist(), J.tolist()):
+        if reverse:
+            adj[j].append(i)
+        else:
+            adj[i].append(j)
+    seen = {source}
+    front = [source]
+    for _ in range(k):
+        nxt = []
+        for v in front:
+            for w in adj[v]:
+                if w not in seen:
+                    seen.add(w)
+                    nxt.append(w)
+        front = nxt
+    return sorted(seen)
+
+
+@pytest.mark.parametrize("name,conv", [("graph_rmat_s9", 1e-9), ("graph_rmat_s11", 1e-9),
+                                       ("pagerank_smoke10", 1e-7)])
+def test_grb_pagerank_and_knn_device_and_primitive_paths(tmp_path, golden, name, conv):
+    """simple_pagerank and knn through the device solvers (no descriptor) and
+    through the reference's loops in grb:: primitives (a descriptor that changes
+    nothing selects them), on the reference's own fixture graphs read from a
+    MatrixMarket file in the fixture's entry order: ranks within 1e-12 of the
+    reference's and iterations within one, both ways; k-hop sets equal to the
+    reference's (k = 1..3 on the R-MAT graphs), to a breadth-first search on the
+    directed ring (knn follows the edges i -> j; transpose_matrix the reverse),
+    and MISMATCH for a source out of range (setElement's code in the
+    reference)."""
+    g = golden(name)
+    I, J = g["I"], g["J"]
+    n = len(g["pr"])
+    src = int(g["source"][0]) if "source" in g.files else 0
+    path = os.path.join(str(tmp_path), "g.mtx")
+    _write_general_mtx(path, n, I, J)
+    r = _run(ALG_PROG, "graph", path, src, conv)
+    assert r["rc"] == 0 and r["n"] == n
+    it_ref = int(g["iterations"][0])
+    for way in ("device", "generic"):
+        assert r[f"pr_rc_{way}"] == int(g["rc"][0]) == 0, way
+        assert abs(r[f"pr_it_{way}"] - it_ref) <= 1, (way, r[f"pr_it_{way}"], it_ref)
+        assert np.max(np.abs(np.array(r[f"pr_{way}"]) - g["pr"])) <= 1e-12, way
+    for k in range(4):
+        fwd = _hops(n, I, J, src, k)
+        rev = _hops(n, I, J, src, k, reverse=True)
+        if f"knn_{k}" in g.files:
+            assert np.flatnonzero(g[f"knn_{k}"]).tolist() == fwd, k
+        assert r[f"knn_device_{k}"] == fwd, k
+        assert r[f"knn_generic_{k}"] == fwd, k
+        assert r[f"knn_transposed_{k}"] == rev, k
+    assert r["knn_rc"] == 0
+    assert r["knn_bad_source_rc"] == 3  # MISMATCH
+
+
+def test_grb_mpv_against_reference_fixture(tmp_path, golden):
+    """algorithms::mpv over plus-times for k = 0..5 against ALP's own mpv on the
+    same matrix (entries in the same input order) and v (fixture mpv, pinned to
+    the loop's closed form in test_oracle.py): which entries u holds, exactly,
+    and the values within 1e-14 of the largest."""
+    g = golden("mpv")
+    n = len(g["v"])
+    path = os.path.join(str(tmp_path), "a.mtx")
+    _write_general_mtx(path, n, g["I"], g["J"], g["V"])
+    vpath = os.path.join(str(tmp_path), "v.txt")
+    with open(vpath, "w") as f:
+        f.write("\n".join(f"{float(x):.17g}" for x in g["v"]) + "\n")
+    r = _run(ALG_PROG, "mpv", path, vpath)
+    assert r["rc"] == 0 and r["n"] == n
+    for k in range(6):
+        assert r[f"rc_{k}"] == 0
+        want_a = g[f"assigned_{k}"].astype(bool)
+        assert np.array_equal(np.array(r[f"assigned_{k}"]) > 0, want_a), k
+        assert r[f"nnz_{k}"] == int(want_a.sum())
+        u, want = np.array(r[f"u_{k}"]), g[f"u_{k}"]
+        err = np.max(np.abs(u - want)) / np.max(np.abs(want))
+        print(f"mpv k={k}: max difference {err:.2e} of the largest", flush=True)
+        assert err <= 1e-14, k

@@ -495,3 +495,31 @@ def test_mxm_shifted_identities_closed_form():
     assert np.array_equal(C.off, E.off) and np.array_equal(C.idx, E.idx) and np.array_equal(C.val, E.val)
     C2 = O.mxm(eye(1, 1), eye(2, 2), 0, C)
     assert np.array_equal(C2.idx, E.idx) and np.array_equal(C2.val, 2 * E.val)
+
+
+def test_mpv_fixture_closed_form(golden):
+    """The mpv fixture (ALP's algorithms::mpv over plus-times, written from the
+    reference by make_golden.py) against what its loop computes in closed form:
+    each product is folded into the vector it writes and only temp is cleared
+    once, so k = 0..5 give v, Av, A^2 v, Av + A^3 v, 2A^2 v + A^4 v and
+    Av + 3A^3 v + A^5 v (dense products here; the fixture's unassigned entries
+    are the rows the products never reach)."""
+    g = golden("mpv")
+    n = len(g["v"])
+    A = np.zeros((n, n))
+    A[g["I"], g["J"]] = g["V"]
+    v = g["v"]
+    p = [v]
+    for _ in range(5):
+        p.append(A @ p[-1])
+    want = {0: p[0], 1: p[1], 2: p[2], 3: p[1] + p[3], 4: 2 * p[2] + p[4], 5: p[1] + 3 * p[3] + p[5]}
+    reach = [np.ones(n, bool)]
+    S = A != 0
+    for _ in range(5):
+        reach.append(S.astype(int) @ reach[-1].astype(int) > 0)
+    assigned = {0: reach[0], 1: reach[1], 2: reach[2], 3: reach[1] | reach[3], 4: reach[2] | reach[4],
+                5: reach[1] | reach[3] | reach[5]}
+    for k in range(6):
+        u = g[f"u_{k}"]
+        assert np.array_equal(g[f"assigned_{k}"].astype(bool), assigned[k]), k
+        assert np.max(np.abs(u - want[k])) <= 1e-13 * np.max(np.abs(want[k])), k
