@@ -14,6 +14,7 @@
  *   hip/base.hpp        RC, descriptors, phases, init / finalize / wait;
  *   hip/algebra.hpp     identities, operators, Monoid, Semiring;
  *   hip/containers.hpp  Vector (with sparsity), Matrix, PinnedVector;
+ *   hip/blas0.hpp       apply, foldl, foldr on scalars;
  *   hip/io.hpp          set, setElement, clear, buildVector, buildMatrixUnique;
  *   hip/blas1.hpp       dot, eWiseApply, eWiseMul, eWiseAdd, eWiseMulAdd, foldl,
  *                       foldr, eWiseLambda;
@@ -48,6 +49,7 @@
 #include "graphblas/hip/base.hpp"
 #include "graphblas/hip/algebra.hpp"
 #include "graphblas/hip/containers.hpp"
+#include "graphblas/hip/blas0.hpp"
 #include "graphblas/hip/io.hpp"
 #include "graphblas/hip/blas1.hpp"
 #include "graphblas/hip/blas2.hpp"

@@ -13,6 +13,7 @@
 #ifndef ALP_HIP_GRB_ALGEBRA_HPP
 #define ALP_HIP_GRB_ALGEBRA_HPP
 
+#include "../utils/iscomplex.hpp"
 #include "base.hpp"
 
 namespace grb {
@@ -89,6 +90,27 @@ ALP_HIP_OPERATOR(greater_than, false, false, *c = static_cast<D3>(*a > *b);)
 ALP_HIP_OPERATOR(geq, false, false, *c = static_cast<D3>(*a >= *b);)
 
 #undef ALP_HIP_OPERATOR
+
+// c = conj(a) b and c = a conj(b) (ops.hpp:900-990 over internalops.hpp's
+// conjugate_mul); for real domains conjugation is the identity, and the
+// operators are associative and commutative unless a domain is complex
+#define ALP_HIP_CONJ_MUL(NAME, BODY)                                                        \
+	template <typename D1_, typename D2_ = D1_, typename D3_ = D2_, enum Backend impl = hip> \
+	struct NAME {                                                                           \
+		typedef D1_ D1;                                                                     \
+		typedef D2_ D2;                                                                     \
+		typedef D3_ D3;                                                                     \
+		typedef D1_ left_type;                                                              \
+		typedef D2_ right_type;                                                             \
+		typedef D3_ result_type;                                                            \
+		static constexpr bool is_associative =                                              \
+		    !utils::is_complex<D1>::value && !utils::is_complex<D2>::value;                 \
+		static constexpr bool is_commutative = is_associative;                              \
+		static ALP_HD void apply(const D1 *a, const D2 *b, D3 *c) { BODY }                  \
+	};
+ALP_HIP_CONJ_MUL(conjugate_left_mul, *c = static_cast<D3>(utils::is_complex<D1>::conjugate(*a) * *b);)
+ALP_HIP_CONJ_MUL(conjugate_right_mul, *c = static_cast<D3>(*a * utils::is_complex<D2>::conjugate(*b));)
+#undef ALP_HIP_CONJ_MUL
 
 }  // namespace operators
 

@@ -220,6 +220,17 @@ __global__ void k_count(size_t n, const F *a, unsigned long long *count) {
 }  // namespace hip
 }  // namespace internal
 
+// what an algorithm may assume of this backend (base/properties.hpp): lambdas
+// run on the device and cannot write captured host scalars, and execution is
+// lazy (nonblocking), as on ALP's nonblocking backend (nonblocking/properties.hpp)
+template <enum Backend backend = config::default_backend>
+class Properties {
+public:
+	static constexpr bool writableCaptured = false;
+	static constexpr bool isBlockingExecution = false;
+	static constexpr bool isNonblockingExecution = true;
+};
+
 // grb::init / grb::finalize (base/init.hpp:144-210); grb::wait
 // (base/io.hpp:1792-1880) is a forcing point: it returns once all enqueued work
 // is done, with the first error that work met

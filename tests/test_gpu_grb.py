@@ -384,3 +384,70 @@ def test_grb_mpv_against_reference_fixture(tmp_path, golden):
         err = np.max(np.abs(u - want)) / np.max(np.abs(want))
         print(f"mpv k={k}: max difference {err:.2e} of the largest", flush=True)
         assert err <= 1e-14, k
+
+
+REF_ON_HIP = os.path.join(ROOT, "oracle", "_ref", "ref_on_hip")
+
+
+def _need_ref_on_hip():
+    if not os.path.exists(REF_ON_HIP):
+        pytest.skip("oracle/_ref/ref_on_hip is built only where the reference sources are (make -C oracle ref)")
+
+
+@pytest.mark.parametrize("name", ["cg_8x8x8_it1000", "cg_12x10x9_it1000", "cg_16x16x16_it20"])
+def test_reference_cg_template_runs_on_the_hip_layer(tmp_path, golden, name):
+    """ALP's own conjugate_gradient.hpp, compiled unchanged from the reference
+    sources over the hip grb:: layer (oracle/ref_on_hip.cpp): every primitive of
+    the reference's loop runs on the device through this layer. Against the
+    reference backend's run of the same template (the fixture: b = A 1, x0 from
+    the fixture): return code, iterations within one, x within 1e-9 (converged)
+    or 1e-10 of the largest entry (20 iterations)."""
+    _need_ref_on_hip()
+    g = golden(name)
+    nx, ny, nz = (int(v) for v in name.split("_")[1].split("x"))
+    max_iter = int(name.split("_it")[1])
+    x0 = os.path.join(str(tmp_path), "x0.txt")
+    with open(x0, "w") as f:
+        f.write("\n".join(f"{float(v):.17g}" for v in g["x0"]) + "\n")
+    r = _run(REF_ON_HIP, "cg", nx, ny, nz, max_iter, 1e-10, x0)
+    assert r["rc"] == 0 and r["cg_rc"] == int(g["rc"][0])
+    assert abs(r["iterations"] - int(g["iterations"][0])) <= 1
+    x = np.array(r["x"])
+    tol = 1e-9 if max_iter == 1000 else 1e-10 * np.max(np.abs(g["x"]))
+    assert np.max(np.abs(x - g["x"])) <= tol
+
+
+def test_reference_knn_and_mpv_templates_run_on_the_hip_layer(tmp_path, golden):
+    """ALP's own knn.hpp and mpv.hpp over the hip grb:: layer: the k-hop sets of
+    the reference's fixture graphs (k = 1..3 on R-MAT scale 9; the directed ring
+    against a breadth-first search) and mpv's u for k = 0..5 against the
+    reference backend's (fixture mpv)."""
+    _need_ref_on_hip()
+    g = golden("graph_rmat_s9")
+    n = len(g["pr"])
+    path = os.path.join(str(tmp_path), "g.mtx")
+    _write_general_mtx(path, n, g["I"], g["J"])
+    src = int(g["source"][0])
+    for k in (1, 2, 3):
+        r = _run(REF_ON_HIP, "knn", path, src, k)
+        assert r["rc"] == 0 and r["knn_rc"] == 0
+        assert r["reached"] == np.flatnonzero(g[f"knn_{k}"]).tolist(), k
+    s = golden("pagerank_smoke10")
+    ring = os.path.join(str(tmp_path), "ring.mtx")
+    _write_general_mtx(ring, 10, s["I"], s["J"])
+    for k in (0, 1, 2, 4):
+        r = _run(REF_ON_HIP, "knn", ring, 0, k)
+        assert r["reached"] == _hops(10, s["I"], s["J"], 0, k), k
+    m = golden("mpv")
+    nm = len(m["v"])
+    apath = os.path.join(str(tmp_path), "a.mtx")
+    _write_general_mtx(apath, nm, m["I"], m["J"], m["V"])
+    vpath = os.path.join(str(tmp_path), "v.txt")
+    with open(vpath, "w") as f:
+        f.write("\n".join(f"{float(x):.17g}" for x in m["v"]) + "\n")
+    for k in range(6):
+        r = _run(REF_ON_HIP, "mpv", apath, vpath, k)
+        assert r["rc"] == 0 and r["mpv_rc"] == 0
+        assert np.array_equal(np.array(r["assigned"]) > 0, m[f"assigned_{k}"].astype(bool)), k
+        u, want = np.array(r["u"]), m[f"u_{k}"]
+        assert np.max(np.abs(u - want)) <= 1e-14 * np.max(np.abs(want)), k
