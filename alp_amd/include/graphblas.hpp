@@ -36,7 +36,10 @@
  *     its kernels on the library's stream and returns; the host waits at forcing
  *     points only (a scalar output, a nonzero count the host does not know, a
  *     copy to or from host memory, freeing device memory, grb::wait). Errors of
- *     enqueued work surface at the next forcing point;
+ *     enqueued work surface at the next forcing point. Consecutive element-wise
+ *     primitives with host-known result counts are fused into one kernel, run
+ *     when other work is launched; code that uses a vector's device array
+ *     directly (getRaw, the C ABI) calls grb::wait first;
  *   - eWiseLambda takes a device callable, which reaches vector elements through
  *     grb::view(vector) captured by value (device code cannot dereference host
  *     references);

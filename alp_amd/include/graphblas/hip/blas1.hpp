@@ -62,90 +62,118 @@ enum EwMode {
 	kFoldr         // z = x op z likewise
 };
 
-// one element-wise pass over n entries; masked-out entries of an out-of-place
-// result are removed, those of an in-place one left alone; counts z's entries
+// element i of an element-wise pass: masked-out entries of an out-of-place
+// result are removed, those of an in-place one left alone; returns whether z
+// holds element i afterwards
+template <int MODE, bool MONOID, Descriptor descr, bool MASKED, class ADD, class MUL, typename Z, typename X,
+          typename Y, typename W, typename M>
+__device__ __forceinline__ bool ew_element(size_t i, Z *z, uint8_t *za, const Arg<X> &x, const Arg<Y> &y,
+                                           const Arg<W> &w, const M *mval, const uint8_t *masg) {
+	const bool allowed = !MASKED || mask_at<descr>(mval, masg, i);
+	const bool zin = za[i] != 0;
+	bool zout = zin;
+	if (MODE == kApplyOp || MODE == kApplyMonoid) {
+		zout = false;
+		if (allowed) {
+			const bool hx = x.has(i), hy = y.has(i);
+			if (hx && hy) {
+				apply_op<ADD>(z[i], x.get(i), y.get(i));
+				zout = true;
+			} else if (MODE == kApplyMonoid && hx) {
+				z[i] = static_cast<Z>(x.get(i));
+				zout = true;
+			} else if (MODE == kApplyMonoid && hy) {
+				z[i] = static_cast<Z>(y.get(i));
+				zout = true;
+			}
+		}
+	} else if (allowed) {
+		if (MODE == kMulIn) {
+			if (x.has(i) && y.has(i)) {
+				typename MUL::D3 t = typename MUL::D3();
+				apply_op<MUL>(t, x.get(i), y.get(i));
+				if (zin) foldl_op<ADD>(z[i], t); else z[i] = static_cast<Z>(t);
+				zout = true;
+			}
+		} else if (MODE == kAddIn) {
+			const bool hx = x.has(i), hy = y.has(i);
+			if (hx || hy) {
+				typename ADD::D3 t;
+				if (hx && hy) apply_op<ADD>(t, x.get(i), y.get(i));
+				else if (hx) t = static_cast<typename ADD::D3>(x.get(i));
+				else t = static_cast<typename ADD::D3>(y.get(i));
+				if (zin) foldl_op<ADD>(z[i], t); else z[i] = static_cast<Z>(t);
+				zout = true;
+			}
+		} else if (MODE == kMulAddIn) {
+			const bool hp = x.has(i) && y.has(i), hw = w.has(i);
+			if (hp || hw) {
+				typename ADD::D3 t;
+				if (hp) {
+					typename MUL::D3 p = typename MUL::D3();
+					apply_op<MUL>(p, x.get(i), y.get(i));
+					if (hw) apply_op<ADD>(t, p, w.get(i)); else t = static_cast<typename ADD::D3>(p);
+				} else {
+					t = static_cast<typename ADD::D3>(w.get(i));
+				}
+				if (zin) foldl_op<ADD>(z[i], t); else z[i] = static_cast<Z>(t);
+				zout = true;
+			}
+		} else if (MODE == kFoldl) {
+			if (y.has(i)) {
+				if (zin) {
+					foldl_op<ADD>(z[i], y.get(i));
+				} else if (MONOID) {
+					z[i] = static_cast<Z>(y.get(i));
+					zout = true;
+				}
+			}
+		} else if (MODE == kFoldr) {
+			if (x.has(i)) {
+				if (zin) {
+					foldr_op<ADD>(x.get(i), z[i]);
+				} else if (MONOID) {
+					z[i] = static_cast<Z>(x.get(i));
+					zout = true;
+				}
+			}
+		}
+	}
+	za[i] = zout ? 1 : 0;
+	return zout;
+}
+
+// one element-wise pass over n entries, counting z's entries (count may be null)
 template <int MODE, bool MONOID, Descriptor descr, bool MASKED, class ADD, class MUL, typename Z, typename X,
           typename Y, typename W, typename M>
 __global__ void k_ewise(size_t n, Z *z, uint8_t *za, Arg<X> x, Arg<Y> y, Arg<W> w, const M *mval, const uint8_t *masg,
                         unsigned long long *count) {
 	unsigned long long c = 0;
-	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-		const bool allowed = !MASKED || mask_at<descr>(mval, masg, i);
-		const bool zin = za[i] != 0;
-		bool zout = zin;
-		if (MODE == kApplyOp || MODE == kApplyMonoid) {
-			zout = false;
-			if (allowed) {
-				const bool hx = x.has(i), hy = y.has(i);
-				if (hx && hy) {
-					apply_op<ADD>(z[i], x.get(i), y.get(i));
-					zout = true;
-				} else if (MODE == kApplyMonoid && hx) {
-					z[i] = static_cast<Z>(x.get(i));
-					zout = true;
-				} else if (MODE == kApplyMonoid && hy) {
-					z[i] = static_cast<Z>(y.get(i));
-					zout = true;
-				}
-			}
-		} else if (allowed) {
-			if (MODE == kMulIn) {
-				if (x.has(i) && y.has(i)) {
-					typename MUL::D3 t = typename MUL::D3();
-					apply_op<MUL>(t, x.get(i), y.get(i));
-					if (zin) foldl_op<ADD>(z[i], t); else z[i] = static_cast<Z>(t);
-					zout = true;
-				}
-			} else if (MODE == kAddIn) {
-				const bool hx = x.has(i), hy = y.has(i);
-				if (hx || hy) {
-					typename ADD::D3 t;
-					if (hx && hy) apply_op<ADD>(t, x.get(i), y.get(i));
-					else if (hx) t = static_cast<typename ADD::D3>(x.get(i));
-					else t = static_cast<typename ADD::D3>(y.get(i));
-					if (zin) foldl_op<ADD>(z[i], t); else z[i] = static_cast<Z>(t);
-					zout = true;
-				}
-			} else if (MODE == kMulAddIn) {
-				const bool hp = x.has(i) && y.has(i), hw = w.has(i);
-				if (hp || hw) {
-					typename ADD::D3 t;
-					if (hp) {
-						typename MUL::D3 p = typename MUL::D3();
-						apply_op<MUL>(p, x.get(i), y.get(i));
-						if (hw) apply_op<ADD>(t, p, w.get(i)); else t = static_cast<typename ADD::D3>(p);
-					} else {
-						t = static_cast<typename ADD::D3>(w.get(i));
-					}
-					if (zin) foldl_op<ADD>(z[i], t); else z[i] = static_cast<Z>(t);
-					zout = true;
-				}
-			} else if (MODE == kFoldl) {
-				if (y.has(i)) {
-					if (zin) {
-						foldl_op<ADD>(z[i], y.get(i));
-					} else if (MONOID) {
-						z[i] = static_cast<Z>(y.get(i));
-						zout = true;
-					}
-				}
-			} else if (MODE == kFoldr) {
-				if (x.has(i)) {
-					if (zin) {
-						foldr_op<ADD>(x.get(i), z[i]);
-					} else if (MONOID) {
-						z[i] = static_cast<Z>(x.get(i));
-						zout = true;
-					}
-				}
-			}
-		}
-		za[i] = zout ? 1 : 0;
-		c += zout ? 1 : 0;
-	}
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		c += ew_element<MODE, MONOID, descr, MASKED, ADD, MUL, Z, X, Y, W, M>(i, z, za, x, y, w, mval, masg) ? 1 : 0;
 	if (!count) return;
 	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
 	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
+// the same pass as a stage of a fused kernel (its result count is known on the host)
+template <typename Z, typename X, typename Y, typename W, typename M>
+struct EwArgs {
+	Z *z;
+	uint8_t *za;
+	Arg<X> x;
+	Arg<Y> y;
+	Arg<W> w;
+	const M *mval;
+	const uint8_t *masg;
+};
+template <int MODE, bool MONOID, Descriptor descr, bool MASKED, class ADD, class MUL, typename Z, typename X,
+          typename Y, typename W, typename M>
+__device__ void ew_stage(size_t b, size_t e, const void *args) {
+	const EwArgs<Z, X, Y, W, M> a = *static_cast<const EwArgs<Z, X, Y, W, M> *>(args);
+	for (size_t i = b + threadIdx.x; i < e; i += blockDim.x)
+		(void)ew_element<MODE, MONOID, descr, MASKED, ADD, MUL, Z, X, Y, W, M>(i, a.z, a.za, a.x, a.y, a.w, a.mval,
+		                                                                          a.masg);
 }
 
 // ---- ordered reductions ------------------------------------------------------------
@@ -281,7 +309,16 @@ RC ewise(Vector<Z, b, C> &z, const Arg<X> &x, const Arg<Y> &y, const Arg<W> &w, 
 	const bool mask_all = !masked || ((descr & descriptors::structural) && !(descr & descriptors::invert_mask) &&
 	                                  mask->known_dense());
 	const bool dense_out = ewise_dense_result<MODE, MONOID>(z.known_dense(), x, y, w, mask_all);
-	unsigned long long *count = dense_out ? nullptr : z.count_slot();
+	if (dense_out) {
+		// the count is known: a stage of the next fused pass
+		z.set_nnz(n);
+		const M *mv = masked ? mask->raw() : nullptr;
+		const uint8_t *ma = masked && !mask->known_dense() ? mask->assigned() : nullptr;
+		const EwArgs<Z, X, Y, W, M> a{z.raw(), z.assigned(), x, y, w, mv, ma};
+		return masked ? enqueue_stage<ew_stage<MODE, MONOID, descr, true, ADD, MUL, Z, X, Y, W, M>>(n, a)
+		              : enqueue_stage<ew_stage<MODE, MONOID, descr, false, ADD, MUL, Z, X, Y, W, M>>(n, a);
+	}
+	unsigned long long *count = z.count_slot();
 	if (n) {
 		if (masked)
 			hipLaunchKernelGGL((k_ewise<MODE, MONOID, descr, true, ADD, MUL, Z, X, Y, W, M>), dim3(grid(n)),
@@ -292,7 +329,7 @@ RC ewise(Vector<Z, b, C> &z, const Arg<X> &x, const Arg<Y> &y, const Arg<W> &w, 
 			                   dim3(kBlock), 0, stream(), n, z.raw(), z.assigned(), x, y, w, (const M *)nullptr,
 			                   (const uint8_t *)nullptr, count);
 	}
-	if (dense_out) z.set_nnz(n); else z.mark_counted();
+	z.mark_counted();
 	return from_hip(hipGetLastError());
 }
 
@@ -333,21 +370,6 @@ RC check_args(const Z &z, const As &...as) {
 	for (bool d : dense)
 		if (!d) return ILLEGAL;
 	return SUCCESS;
-}
-
-// z += alpha y on the library's axpy when z and y are dense double vectors
-template <typename Z, typename X, typename Y>
-RC axpy_fast(Z &, const X &, const Y &, bool &done, std::false_type) {
-	done = false;
-	return SUCCESS;
-}
-template <enum Backend b, typename C, typename X>
-RC axpy_fast(Vector<double, b, C> &z, const X &alpha, const Vector<double, b, C> &y, bool &done, std::true_type) {
-	done = false;
-	if (!z.known_dense() || !y.known_dense() || z.raw() == y.raw()) return SUCCESS;
-	done = true;
-	(void)stream();  // enqueued: inside a pipeline this joins the open segment
-	return rc(alp_hip_axpy_f64(z.size(), (double)alpha, y.raw(), z.raw()));
 }
 
 template <typename T, enum Backend b, typename C> const void *addr(const Vector<T, b, C> &v) { return v.raw(); }
@@ -446,14 +468,6 @@ RC eWiseMul(Vector<Z, b, C> &z, const X &x, const Y &y, const Ring & = Ring(), c
 	if (r || phase == RESIZE) return r;
 	typedef typename internal::hip::val_of<X>::type XT;
 	typedef typename internal::hip::val_of<Y>::type YT;
-	// the library's axpy for a dense double z += alpha * y
-	constexpr bool axpy_form = std::is_same<typename Ring::AdditiveOperator, operators::add<double>>::value &&
-	                           std::is_same<typename Ring::MultiplicativeOperator, operators::mul<double>>::value &&
-	                           std::is_same<Z, double>::value && std::is_same<YT, double>::value &&
-	                           !internal::hip::is_vec<X>::value && internal::hip::is_vec<Y>::value;
-	bool done = false;
-	r = internal::hip::axpy_fast(z, x, y, done, std::integral_constant<bool, axpy_form>());
-	if (done || r) return r;
 	return internal::hip::ewise<internal::hip::kMulIn, false, descr, typename Ring::AdditiveOperator,
 	                            typename Ring::MultiplicativeOperator, Z, XT, YT, char, bool>(
 	    z, internal::hip::as_arg(x), internal::hip::as_arg(y), internal::hip::Arg<char>(),

@@ -371,8 +371,17 @@ RC wait(const Matrix<T, b, R, C, N> &, const Args &...) {
 }
 
 namespace internal {
-template <typename T, enum Backend b, typename C> T *getRaw(Vector<T, b, C> &x) { return x.raw(); }
-template <typename T, enum Backend b, typename C> const T *getRaw(const Vector<T, b, C> &x) { return x.raw(); }
+// the device array for code outside this layer: enqueued element-wise stages
+// are launched first (they are not yet on the stream); the caller still waits
+// (grb::wait) before its own work on the array
+template <typename T, enum Backend b, typename C> T *getRaw(Vector<T, b, C> &x) {
+	hip::flush_stages();
+	return x.raw();
+}
+template <typename T, enum Backend b, typename C> const T *getRaw(const Vector<T, b, C> &x) {
+	hip::flush_stages();
+	return x.raw();
+}
 }  // namespace internal
 
 // A device-side view of a vector for the lambdas of eWiseLambda: operator[] reads

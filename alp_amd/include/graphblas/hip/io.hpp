@@ -24,14 +24,6 @@ __device__ __forceinline__ bool mask_at(const M *mval, const uint8_t *masg, cons
 	return (descr & descriptors::invert_mask) ? !ret : ret;
 }
 
-template <typename T, typename V>
-__global__ void k_set_scalar(size_t n, T *x, uint8_t *xa, V v) {
-	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-		x[i] = static_cast<T>(v);
-		xa[i] = 1;
-	}
-}
-
 // x_i = v where the mask holds, x_i removed elsewhere; counts the result
 template <Descriptor descr, typename T, typename M, typename V>
 __global__ void k_set_masked(size_t n, T *x, uint8_t *xa, const M *mval, const uint8_t *masg, V v,
@@ -46,6 +38,46 @@ __global__ void k_set_masked(size_t n, T *x, uint8_t *xa, const M *mval, const u
 	if (!count) return;
 	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
 	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
+// stages of the fused element-wise pass (base.hpp): x_i = v; x_i removed; x_i = y_i
+// with y's sparsity
+template <typename T>
+struct SetArgs {
+	T *x;
+	uint8_t *xa;
+	T v;
+};
+template <typename T>
+__device__ void set_stage(size_t b, size_t e, const void *args) {
+	const SetArgs<T> a = *static_cast<const SetArgs<T> *>(args);
+	for (size_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+		a.x[i] = a.v;
+		a.xa[i] = 1;
+	}
+}
+struct ClearArgs {
+	uint8_t *xa;
+};
+__device__ inline void clear_stage(size_t b, size_t e, const void *args) {
+	uint8_t *xa = static_cast<const ClearArgs *>(args)->xa;
+	for (size_t i = b + threadIdx.x; i < e; i += blockDim.x) xa[i] = 0;
+}
+template <typename T, typename U>
+struct CopyArgs {
+	T *x;
+	uint8_t *xa;
+	const U *y;
+	const uint8_t *ya;
+};
+template <typename T, typename U>
+__device__ void copy_stage(size_t b, size_t e, const void *args) {
+	const CopyArgs<T, U> a = *static_cast<const CopyArgs<T, U> *>(args);
+	for (size_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+		const bool on = a.ya[i] != 0;
+		if (on) a.x[i] = static_cast<T>(a.y[i]);
+		a.xa[i] = on ? 1 : 0;
+	}
 }
 
 // x = y (values cast) with y's sparsity, optionally restricted to a mask
@@ -88,11 +120,8 @@ __global__ void k_scatter_build(size_t nz, const size_t *idx, const T *vals, T *
 
 template <typename T, enum Backend b, typename C>
 RC clear(Vector<T, b, C> &x) {
-	if (x.size())
-		hipLaunchKernelGGL(internal::hip::k_fill<uint8_t>, dim3(internal::hip::grid(x.size())),
-		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.assigned(), (uint8_t)0);
 	x.set_nnz(0);
-	return internal::hip::from_hip(hipGetLastError());
+	return internal::hip::enqueue_stage<internal::hip::clear_stage>(x.size(), internal::hip::ClearArgs{x.assigned()});
 }
 
 // x_i = val for every i (reference/io.hpp:893): x becomes dense
@@ -100,12 +129,9 @@ template <Descriptor descr = descriptors::no_operation, typename T, enum Backend
 RC set(Vector<T, b, C> &x, const V val, const Phase phase = EXECUTE,
        typename std::enable_if<std::is_convertible<V, T>::value>::type * = nullptr) {
 	if (phase == RESIZE) return SUCCESS;
-	if (x.size())
-		hipLaunchKernelGGL((internal::hip::k_set_scalar<T, T>), dim3(internal::hip::grid(x.size())),
-		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
-		                   static_cast<T>(val));
 	x.set_nnz(x.size());
-	return internal::hip::from_hip(hipGetLastError());
+	return internal::hip::enqueue_stage<internal::hip::set_stage<T>>(
+	    x.size(), internal::hip::SetArgs<T>{x.raw(), x.assigned(), static_cast<T>(val)});
 }
 
 // a structural mask that holds everywhere (dense, not inverted), as far as the
@@ -129,13 +155,17 @@ RC set(Vector<T, b, C> &x, const Vector<M, b, C> &mask, const V val, const Phase
 		RC c = clear(x);
 		if (c) return c;
 	}
-	const bool all = mask_holds_everywhere<descr>(mask);
-	unsigned long long *count = all ? nullptr : x.count_slot();
+	if (mask_holds_everywhere<descr>(mask)) {  // the mask selects every element
+		x.set_nnz(x.size());
+		return internal::hip::enqueue_stage<internal::hip::set_stage<T>>(
+		    x.size(), internal::hip::SetArgs<T>{x.raw(), x.assigned(), static_cast<T>(val)});
+	}
+	unsigned long long *count = x.count_slot();
 	if (x.size())
 		hipLaunchKernelGGL((internal::hip::k_set_masked<descr, T, M, T>), dim3(internal::hip::grid(x.size())),
 		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
 		                   mask.raw(), mask.known_dense() ? nullptr : mask.assigned(), static_cast<T>(val), count);
-	if (all) x.set_nnz(x.size()); else x.mark_counted();
+	x.mark_counted();
 	return internal::hip::from_hip(hipGetLastError());
 }
 
@@ -145,14 +175,18 @@ RC set(Vector<T, b, C> &x, const Vector<U, b, C> &y, const Phase phase = EXECUTE
 	if (x.size() != y.size()) return MISMATCH;
 	if ((const void *)&x == (const void *)&y) return ILLEGAL;
 	if (phase == RESIZE) return SUCCESS;
-	// x takes y's count: from the host when it is known there, else counted
-	const bool known = !y.count_pending();
-	unsigned long long *count = known ? nullptr : x.count_slot();
+	// x takes y's count: from the host when it is known there (a stage), else counted
+	if (!y.count_pending()) {
+		x.set_nnz(y.nnz());
+		return internal::hip::enqueue_stage<internal::hip::copy_stage<T, U>>(
+		    x.size(), internal::hip::CopyArgs<T, U>{x.raw(), x.assigned(), y.raw(), y.assigned()});
+	}
+	unsigned long long *count = x.count_slot();
 	if (x.size())
 		hipLaunchKernelGGL((internal::hip::k_copy<descr, false, T, U, uint8_t>), dim3(internal::hip::grid(x.size())),
 		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
 		                   y.raw(), y.assigned(), (const uint8_t *)nullptr, (const uint8_t *)nullptr, count);
-	if (known) x.set_nnz(y.nnz()); else x.mark_counted();
+	x.mark_counted();
 	return internal::hip::from_hip(hipGetLastError());
 }
 
@@ -164,13 +198,17 @@ RC set(Vector<T, b, C> &x, const Vector<M, b, C> &mask, const Vector<U, b, C> &y
 	if (x.size() != y.size() || mask.size() != x.size()) return MISMATCH;
 	if ((const void *)&x == (const void *)&y) return ILLEGAL;
 	if (phase == RESIZE) return SUCCESS;
-	const bool known = !y.count_pending() && mask_holds_everywhere<descr>(mask);
-	unsigned long long *count = known ? nullptr : x.count_slot();
+	if (!y.count_pending() && mask_holds_everywhere<descr>(mask)) {  // as the unmasked copy
+		x.set_nnz(y.nnz());
+		return internal::hip::enqueue_stage<internal::hip::copy_stage<T, U>>(
+		    x.size(), internal::hip::CopyArgs<T, U>{x.raw(), x.assigned(), y.raw(), y.assigned()});
+	}
+	unsigned long long *count = x.count_slot();
 	if (x.size())
 		hipLaunchKernelGGL((internal::hip::k_copy<descr, true, T, U, M>), dim3(internal::hip::grid(x.size())),
 		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), x.size(), x.raw(), x.assigned(),
 		                   y.raw(), y.assigned(), mask.raw(), mask.known_dense() ? nullptr : mask.assigned(), count);
-	if (known) x.set_nnz(y.nnz()); else x.mark_counted();
+	x.mark_counted();
 	return internal::hip::from_hip(hipGetLastError());
 }
 

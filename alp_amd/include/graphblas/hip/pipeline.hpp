@@ -75,10 +75,12 @@ public:
 		try {
 			r = body();
 		} catch (...) {
+			internal::hip::flush_stages();
 			(void)close();
 			l.sink = nullptr;
 			throw;
 		}
+		internal::hip::flush_stages();  // the body's last fused pass joins its last segment
 		const RC c = close();
 		l.sink = nullptr;
 		segments_ = seg_;

@@ -24,10 +24,14 @@ static int run(const char *path) {
 	rc = rc ? rc : set(y, 0.0);
 	std::vector<double> h(parser.m(), 0.0);
 	if (std::is_void<T>::value) {
-		// a pattern matrix: the plus/pattern product counts each row's nonzeroes
+		// a pattern matrix: the plus/pattern product counts each row's nonzeroes,
+		// through the C ABI on the vectors' device arrays: grb::wait first, so that
+		// the enqueued work on x and y is done
+		rc = rc ? rc : wait();
 		rc = rc ? rc : internal::rc(alp_hip_mxv(A.handle(), ALP_PLUS_PATTERN_F64, 0, y.raw(), x.raw(), nullptr));
 	} else {
 		rc = rc ? rc : mxv(y, A, x, ring);
+		rc = rc ? rc : wait();
 	}
 	rc = rc ? rc : internal::rc(alp_hip_memcpy_d2h(h.data(), y.raw(), sizeof(double) * h.size()));
 	std::printf("{\"rc\": %d, \"m\": %zu, \"n\": %zu, \"nz\": %zu, \"entries\": %zu, \"pattern\": %d, "

@@ -255,7 +255,7 @@ def test_grb_hpcg_program_as_the_reference_smoke_test(golden):
 PIPE_PROG = os.path.join(ROOT, "tests", "cpp", "_build", "grb_pipeline_program")
 
 
-@pytest.mark.parametrize("sizes", [(16, 16, 16, 40), (12, 10, 9, 25)])
+@pytest.mark.parametrize("sizes", [(16, 16, 16, 40), (12, 10, 9, 25), (104, 104, 104, 6)])
 def test_lazy_primitives_and_pipeline_replay(sizes):
     """Lazy execution (primitives enqueue, the host waits at forcing points) and
     grb::Pipeline (each call's device work captured into hipGraphs, segment by
@@ -268,7 +268,9 @@ def test_lazy_primitives_and_pipeline_replay(sizes):
     mxv inside the body asks for, and eight sweeps per call (one 32-launch
     segment). The Jacobi-preconditioned CG in grb::
     primitives must take the oracle CG's iteration count: with D = 26 I the
-    preconditioned iterates equal the plain ones in exact arithmetic."""
+    preconditioned iterates equal the plain ones in exact arithmetic. The small
+    grids run the element-wise primitives fused into one kernel per run of them;
+    104^3 (over 2^20 elements) runs each as its own kernel."""
     nx, ny, nz, sweeps = sizes
     r = _run(PIPE_PROG, nx, ny, nz, sweeps)
     assert r["rc"] == 0
