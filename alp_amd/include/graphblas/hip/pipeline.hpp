@@ -32,11 +32,11 @@
  *
  * On MI355X with ROCm 7.2 this is not faster than the direct (lazy) launches: a
  * graph launch costs ~16 us whatever its size and the capture plus update ~2 us
- * per kernel on the host, against ~2.7 us per direct launch. A 4-launch Jacobi
- * sweep on 16^3 takes 32-36 us per call this way against 12 us direct, and eight
- * sweeps in one call (32 launches) 96-131 against 87-90 us. The Pipeline is for callers
+ * per kernel on the host, against ~2.7 us per direct launch. A Jacobi sweep on
+ * 16^3 (four primitives) takes 27 us per call this way against 8 us direct, and
+ * eight sweeps in one call 86 against 64 us. The Pipeline is for callers
  * that want a loop's device work as graphs; the speed of user sequences comes
- * from the lazy launches alone.
+ * from the lazy launches and the fusion of element-wise runs.
  *
  * The call returns once the work is enqueued (as any primitive); grb::wait or any
  * forcing point waits for it. Calls may not nest, and may not run inside a
