@@ -295,7 +295,12 @@ __global__ void k_stage_address(StageFn *out) {
 }
 template <StageFn F>
 RC stage_address(StageFn &fn) {
+	// a function's address is per device: fetched again after a change of device
 	static thread_local StageFn cached = nullptr;
+	static thread_local int cached_device = -1;
+	int device = -1;
+	if (hipGetDevice(&device) != hipSuccess) return PANIC;
+	if (device != cached_device) cached = nullptr;
 	if (!cached) {
 		Forcing f;
 		if (f.rc) return f.rc;
@@ -306,6 +311,7 @@ RC stage_address(StageFn &fn) {
 		if (e == hipSuccess) e = hipStreamSynchronize(library_stream());
 		if (e != hipSuccess || !h) return e != hipSuccess ? from_hip(e) : PANIC;
 		cached = h;
+		cached_device = device;
 	}
 	fn = cached;
 	return SUCCESS;

@@ -294,6 +294,23 @@ int main(int argc, char **argv) {
 	Vector<double> ms(m);
 	rc = rc ? rc : set(ms, mask, 4.0);
 	emit_vec("masked_set", ms);
+
+	// 11. a chain of sparse primitives with nothing read in between: each result
+	//     count stays on the device until a later primitive needs it (the mxv)
+	//     or the results are read at the end
+	Vector<double> c1(n), c2(n), c3(n), c4(n), yc(m);
+	rc = rc ? rc : eWiseApply(c1, u, w, operators::subtract<double>());
+	rc = rc ? rc : eWiseAdd(c2, c1, u, plus_times);
+	rc = rc ? rc : set(c3, c2);
+	rc = rc ? rc : foldl(c3, c1, Monoid<operators::max<double>, identities::negative_infinity>());
+	rc = rc ? rc : set(c4, c1, 7.0);  // a value mask: c1's nonzero entries
+	rc = rc ? rc : eWiseMul(c4, c3, w, plus_times);
+	rc = rc ? rc : mxv(yc, A, c3, plus_times);
+	emit_vec("chain_c1", c1);
+	emit_vec("chain_c2", c2);
+	emit_vec("chain_c3", c3);
+	emit_vec("chain_c4", c4);
+	emit_vec("chain_y", yc);
 	emit_num("rc", (double)rc);
 	out += "}";
 	std::printf("%s\n", out.c_str());

@@ -234,6 +234,21 @@ def test_grb_generic_semirings_and_sparse_semantics(seed):
         got = {(int(i), int(j)): v for i, j, v in zip(r[name + "_I"], r[name + "_J"], r[name + "_V"])}
         assert got == want, name
     _assert_vec(r, "masked_set", {i: 4.0 for i in mask})
+    # 11: the chain with counts left on the device between primitives
+    c1 = {k: u[k] - w[k] for k in u if k in w}
+    c2 = {k: (c1[k] + u[k] if (k in c1 and k in u) else c1.get(k, u.get(k))) for k in set(c1) | set(u)}
+    c3 = dict(c2)
+    for k, v in c1.items():
+        c3[k] = max(c3[k], v) if k in c3 else v
+    c4 = {k: 7.0 for k, v in c1.items() if v != 0}
+    for k in set(c3) & set(w):
+        t = c3[k] * w[k]
+        c4[k] = c4[k] + t if k in c4 else t
+    _assert_vec(r, "chain_c1", c1)
+    _assert_vec(r, "chain_c2", c2)
+    _assert_vec(r, "chain_c3", c3)
+    _assert_vec(r, "chain_c4", c4)
+    _assert_vec(r, "chain_y", _gather(rows, m, c3, mul, add, 0.0))
 
 
 def test_grb_hpcg_program_as_the_reference_smoke_test(golden):
