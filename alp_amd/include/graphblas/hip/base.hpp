@@ -154,6 +154,7 @@ struct LazyState {
 	SegmentSink *sink = nullptr;  // the running grb::Pipeline call, if any
 	int forcing = 0;              // depth of forcing scopes: nothing is captured inside one
 	StageList stages;             // element-wise stages not yet launched
+	RC deferred = SUCCESS;        // a failed launch of stages, reported at the next forcing point
 };
 inline LazyState &lazy() {
 	static thread_local LazyState s;
@@ -175,14 +176,19 @@ inline unsigned grid(size_t work, size_t cap_blocks = 256 * 16) {
 	return static_cast<unsigned>(g);
 }
 
-// launch the pending stages as one kernel, one element per thread and tile
+// launch the pending stages as one kernel, one element per thread and tile; a
+// launch error is kept for the next forcing point (the primitives that made the
+// stages have returned already, as deferred errors do in ALP's nonblocking mode)
 inline void flush_stages() {
-	StageList &st = lazy().stages;
+	LazyState &l = lazy();
+	StageList &st = l.stages;
 	if (st.count == 0) return;
 	const hipStream_t s = enqueue_stream();
 	st.tile = kBlock;
 	hipLaunchKernelGGL(k_stages<0>, dim3(grid(st.n)), dim3(kBlock), 0, s, st);
 	st.count = 0;
+	const RC r = from_hip(hipGetLastError());
+	if (r && !l.deferred) l.deferred = r;
 }
 
 // Vectors longer than this run each element-wise primitive as its own kernel:
@@ -215,7 +221,12 @@ struct Forcing {
 		LazyState &l = lazy();
 		if (l.forcing == 0) {
 			flush_stages();
-			if (l.sink) rc = l.sink->close();
+			rc = l.deferred;
+			l.deferred = SUCCESS;
+			if (l.sink) {
+				const RC c = l.sink->close();
+				if (!rc) rc = c;
+			}
 		}
 		++l.forcing;
 	}
