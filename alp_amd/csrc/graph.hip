@@ -199,8 +199,10 @@ struct PrState {
 	unsigned long long max_iter;
 };
 
-// the partials regions of one iteration: dangling mass, then the L1 residual
-constexpr unsigned kPrRegion = kMaxPartials / 2;
+// the partials regions: the dangling mass of even and of odd iterations (one
+// iteration's update writes the next one's while its own is still being read),
+// then the L1 residual
+constexpr unsigned kPrRegion = kMaxPartials / 4;
 
 // row_sum[i] = alpha / out-degree (0 for dangling rows), pr = 1/n (:206-297);
 // entry i is vertex perm[i] (perm null: vertex i)
@@ -235,22 +237,36 @@ __global__ __launch_bounds__(kBlock) void k_pr_scale(size_t n, const double *pr,
 }
 
 // pr = pr_nextnext + (alpha * dangling + 1 - alpha) / n and the partials of the L1
-// residual (:441-455, :490)
-__global__ __launch_bounds__(kBlock) void k_pr_update(size_t n, double *pr, const double *pnn, double *partials,
-                                                      unsigned np, const PrScal *in, const PrState *st) {
+// residual (:441-455, :490), fused with the next iteration's k_pr_scale on the
+// new pr: pr_next = pr * row_sum and the next dangling mass into region dout
+// (the same grid and per-thread order as k_pr_scale, so the same partials)
+__global__ __launch_bounds__(kBlock) void k_pr_update(size_t n, double *pr, const double *pnn, const double *rs,
+                                                      double *pn, double *partials, unsigned np, unsigned din,
+                                                      unsigned dout, const PrScal *in, const PrState *st) {
 	__shared__ double lds[kBlock / kWave];
 	if (in->done) return;
-	const double d = block_sum_partials(partials, np, lds);
+	const double d = block_sum_partials(partials + din, np, lds);
 	const double dang = (st->alpha * d + 1.0 - st->alpha) / (double)n;
-	double res = 0.0;
+	double res = 0.0, next = 0.0;
 	grid_for_each<4>(n, [&](size_t i) {
 		const double old = pr[i];
 		const double v = pnn[i] + dang;
 		pr[i] = v;
 		res += old > v ? old - v : v - old;
+		const double r = rs[i];
+		if (r == 0.0) {
+			next += v;
+			pn[i] = 0.0;
+		} else {
+			pn[i] = v * r;
+		}
 	});
 	res = block_sum(res, lds);
-	if (threadIdx.x == 0) partials[kPrRegion + blockIdx.x] = res;
+	next = block_sum(next, lds);
+	if (threadIdx.x == 0) {
+		partials[2 * kPrRegion + blockIdx.x] = res;
+		partials[dout + blockIdx.x] = next;
+	}
 }
 
 // one block: the residual, the iteration count and the stopping test into the
@@ -262,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_fin(const double *partials, unsig
 		if (threadIdx.x == 0) *out = *in;
 		return;
 	}
-	const double r = block_sum_partials(partials + kPrRegion, np, lds);
+	const double r = block_sum_partials(partials + 2 * kPrRegion, np, lds);
 	if (threadIdx.x == 0) {
 		PrScal o;
 		o.residual = r;
@@ -800,16 +816,19 @@ ALP_EXPORT alp_rc_t alp_hip_pagerank(alp_hip_matrix_t L, double *d_pr, double al
 	unsigned rg = g_rgrid(n);
 	if (rg > kPrRegion) rg = kPrRegion;
 	double *part = ctx().d_partials;
+	// the first iteration's pr_next and dangling mass; every later one's comes
+	// from the previous update
+	hipLaunchKernelGGL(k_pr_scale, dim3(rg), dim3(kBlock), 0, s, n, (const double *)pr, (const double *)rs.p, pn.p,
+	                   part, &st.p->s[0]);
 	for (;;) {
-		for (int k = 0; k < 8; ++k) {  // even: every check reads slot 0
+		for (int k = 0; k < 8; ++k) {  // even: every check reads slot 0, and iteration k's dangling region is k % 2
 			const PrScal *in = &st.p->s[k % 2];
-			hipLaunchKernelGGL(k_pr_scale, dim3(rg), dim3(kBlock), 0, s, n, (const double *)pr, (const double *)rs.p,
-			                   pn.p, part, in);
 			// pr_nextnext = pr_next L: the CCS gather of vxm (blas2.hpp:1432-1466)
 			ALP_TRY(spmv(P, L->value_type, ALP_PLUS_PATTERN_F64, ALP_DESC_OVERWRITE, pnn.p, pn.p, nullptr, s,
 			             &in->done));
-			hipLaunchKernelGGL(k_pr_update, dim3(rg), dim3(kBlock), 0, s, n, pr, (const double *)pnn.p, part, rg, in,
-			                   (const PrState *)st.p);
+			hipLaunchKernelGGL(k_pr_update, dim3(rg), dim3(kBlock), 0, s, n, pr, (const double *)pnn.p,
+			                   (const double *)rs.p, pn.p, part, rg, (unsigned)(k % 2) * kPrRegion,
+			                   (unsigned)(1 - k % 2) * kPrRegion, in, (const PrState *)st.p);
 			hipLaunchKernelGGL(k_pr_fin, dim3(1), dim3(kBlock), 0, s, (const double *)part, rg, in,
 			                   &st.p->s[1 - k % 2], (const PrState *)st.p);
 		}
