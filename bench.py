@@ -149,6 +149,24 @@ def timed(fn, sync):
     return out, time.perf_counter() - t0
 
 
+def leg_grb_loops() -> dict:
+    """A program's own loops in grb:: primitives (tests/cpp/grb_pipeline_program,
+    built by `make -C alp_amd`): a Jacobi sweep of four primitives and a
+    Jacobi-preconditioned CG iteration at 64^3, with lazy execution and fused
+    element-wise runs; the same sweep as a grb::Pipeline of hipGraphs."""
+    import subprocess
+    prog = os.path.join(ROOT, "tests", "cpp", "_build", "grb_pipeline_program")
+    if not os.path.exists(prog):
+        return {"skipped": "tests/cpp/_build/grb_pipeline_program not built"}
+    out = subprocess.run([prog, "64", "64", "64", "48"], capture_output=True, text=True, timeout=300)
+    if out.returncode != 0:
+        raise RuntimeError("grb_pipeline_program failed: " + out.stdout[-500:] + out.stderr[-500:])
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"grid": "64^3", "jacobi_sweep_us": r["a_eager_us"], "jacobi_sweep_pipeline_us": r["a_pipeline_us"],
+            "pcg_iterations": r["pcg_iterations"], "pcg_us_per_iteration": r["pcg_us_per_iteration"],
+            "bit_identical_pipeline": bool(r["a_equal"] and r["b_equal"] and r["c_equal"] and r["d_equal"])}
+
+
 def leg_c1(alp, L) -> dict:
     """64^3 stencil, b = A 1, x0 ~ U[0,1) (numpy seed 42), 50 unpreconditioned
     FP64 CG iterations; GB/s counts the SpMV overwrite form plus 72 B/row of
@@ -547,7 +565,8 @@ def main() -> int:
     configs = {}
     if not args.no_configs:
         for name, fn in (("c1_cg50_64cubed", lambda: leg_c1(alp, L)), ("c2_rmat_s22", lambda: leg_c2(alp)),
-                         ("c4_batched_64x8", lambda: leg_c4(alp)), ("c5_adjoint_128cubed", lambda: leg_c5(alp, L))):
+                         ("c4_batched_64x8", lambda: leg_c4(alp)), ("c5_adjoint_128cubed", lambda: leg_c5(alp, L)),
+                         ("grb_loops_64cubed", leg_grb_loops)):
             log(f"[bench] {name}")
             configs[name] = fn()
             log(f"[bench] {name}: {json.dumps(configs[name])}")
