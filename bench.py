@@ -440,6 +440,17 @@ def main() -> int:
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / args.steps
     kern_ms = ev.value / args.steps
+    # 20 launches timed one by one (HIP events around each), for the spread
+    single = []
+    for _ in range(20):
+        alp.check(L.alp_hip_timer_start(), "timer_start")
+        L.alp_hip_mxv(A.h, alp.PLUS_TIMES_F64, alp.DENSE, y.ptr, x.ptr, None)
+        alp.check(L.alp_hip_timer_stop(ctypes.byref(ev)), "timer_stop")
+        single.append(ev.value * 1e3)
+    single.sort()
+    per_launch = {"min": round(single[0], 2), "median": round(0.5 * (single[9] + single[10]), 2),
+                  "max": round(single[-1], 2), "launches": 20, "unit": "us",
+                  "note": "each launch between its own HIP events, after the timed region"}
     crs_bytes = spmv_bytes(n, nnz)
     fbytes = format_bytes(n, nnz, kernel, layout)
     achieved = fbytes / (kern_ms * 1e-3) / 1e9
@@ -593,6 +604,7 @@ def main() -> int:
                      "note": "achieved = the streamed layout's algorithmic bytes per launch "
                              "(format_bytes_per_launch) / the kernel's HIP-event time per launch; traffic = the "
                              "measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated)"},
+        "per_launch_us": per_launch,
         "format_bytes_per_launch": fbytes,
         "crs_bytes_per_launch": crs_bytes,
         "crs_equivalent_gbs": round(crs_equiv, 1),
