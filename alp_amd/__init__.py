@@ -53,8 +53,10 @@ class AlpError(RuntimeError):
 
 
 def build(quiet: bool = True) -> None:
-    """Compile libalp_hip.so for gfx950 in-tree (hipcc cross-compiles without a GPU)."""
-    out = subprocess.run(["make", "-C", HERE, f"-j{min(16, os.cpu_count() or 4)}"],
+    """Compile libalp_hip.so and the grb:: test programs for gfx950 in-tree (hipcc
+    cross-compiles without a GPU). Every source is compiled on every call (make
+    -B), so the call checks that the tree builds, whatever is already built."""
+    out = subprocess.run(["make", "-B", "-C", HERE, f"-j{min(16, os.cpu_count() or 4)}"],
                          capture_output=True, text=True)
     if out.returncode != 0:
         raise RuntimeError("libalp_hip build failed:\n" + out.stdout[-4000:] + out.stderr[-8000:])

@@ -35,8 +35,9 @@ def _ptr(a: np.ndarray):
 
 
 def build(quiet: bool = True) -> None:
-    """Compile liboracle.so (and oracle/_ref when the reference sources exist)."""
-    out = subprocess.run(["make", "-C", HERE, "all"], capture_output=True, text=True)
+    """Compile liboracle.so (and oracle/_ref when the reference sources exist),
+    every source on every call (make -B)."""
+    out = subprocess.run(["make", "-B", "-C", HERE, "all"], capture_output=True, text=True)
     if out.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
     if not quiet:
