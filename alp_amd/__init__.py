@@ -510,9 +510,12 @@ def max_degree_vertex(A: Matrix):
     return v.value, d.value
 
 
-def bfs(A: Matrix, source: int):
-    """(levels as an int32 Vector, number of levels, vertices reached)."""
-    lev = Vector(A.nrows, np.int32)
+def bfs(A: Matrix, source: int, out: "Vector" = None):
+    """(levels as an int32 Vector, number of levels, vertices reached); `out`, an
+    int32 Vector of A.nrows entries, receives the levels instead of a new one."""
+    if out is not None and (out.n != A.nrows or out.dtype != np.int32):
+        raise ValueError("bfs: out must be an int32 Vector of A.nrows entries")
+    lev = Vector(A.nrows, np.int32) if out is None else out
     nl, reached = _S(), _S()
     check(lib().alp_hip_bfs(A.h, source, lev.ptr, ctypes.byref(nl), ctypes.byref(reached)), "bfs")
     return lev, nl.value, reached.value
@@ -526,8 +529,12 @@ def pagerank(L: Matrix, alpha=0.85, conv=1e-7, max_iter=1000):
     return rc, pr, it.value, q.value
 
 
-def khop(A: Matrix, source: int, k: int) -> Vector:
-    out = Vector(A.nrows, np.uint8)
+def khop(A: Matrix, source: int, k: int, out: "Vector" = None) -> Vector:
+    """The k-hop set of source as a uint8 Vector (1: within k hops); `out`, a
+    uint8 Vector of A.nrows entries, receives it instead of a new one."""
+    if out is not None and (out.n != A.nrows or out.dtype != np.uint8):
+        raise ValueError("khop: out must be a uint8 Vector of A.nrows entries")
+    out = Vector(A.nrows, np.uint8) if out is None else out
     check(lib().alp_hip_khop(A.h, source, k, out.ptr), "khop")
     return out
 

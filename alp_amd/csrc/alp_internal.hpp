@@ -283,6 +283,7 @@ struct alp_hip_matrix_s {
 	void *graph_cache = nullptr;
 	void (*free_graph_cache)(void *) = nullptr;
 	bool symmetric_pattern = false;  // built symmetrised (alp_hip_graph_from_edges)
+	bool unique_entries = false;     // no position stored twice (alp_hip_graph_from_edges)
 	// ccs is a shallow copy of crs (a symmetrised pattern, whose CCS lists the
 	// same entries in the same order): the arrays are owned, and freed, by crs
 	bool ccs_alias = false;

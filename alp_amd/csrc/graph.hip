@@ -19,6 +19,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -93,52 +94,204 @@ __global__ __launch_bounds__(kBlock) void k_offsets_from_sorted(size_t nmajor, c
 // ---- BFS ---------------------------------------------------------------------------
 
 struct BfsCounters {
-	unsigned long long next_vertices;  // vertices discovered in this level
-	unsigned long long next_edges;     // sum of their degrees
+	unsigned long long queued;     // push: the length of the next queue so far
+	unsigned long long collected;  // collect: the length of the queue so far
 };
 
-__global__ __launch_bounds__(kBlock) void k_bfs_init(size_t n, int32_t *level, uint32_t source) {
-	grid_for_each<4>(n, [&](size_t i) { level[i] = i == source ? 0 : -1; });
+// Queue slots for one step of a loop that steps whole blocks (every thread of
+// the block calls it): a thread with `take` gets the next free slot after the
+// block's earlier takers, in thread order, and the block takes its slots from
+// *tail with one atomic. Per-thread atomics on one word serialise: each waits
+// for the one before.
+__device__ __forceinline__ unsigned long long block_slots(bool take, unsigned long long *tail) {
+	__shared__ unsigned s_off[kBlock / kWave];
+	__shared__ unsigned long long s_base;
+	const unsigned lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+	const unsigned long long mask = __ballot(take);
+	__syncthreads();  // the previous step has read s_off and s_base
+	if (lane == 0) s_off[wave] = (unsigned)__popcll(mask);
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		unsigned t = 0;  // exclusive scan of the wave counts
+		for (int i = 0; i < kBlock / kWave; ++i) {
+			const unsigned c = s_off[i];
+			s_off[i] = t;
+			t += c;
+		}
+		s_base = t ? atomicAdd(tail, (unsigned long long)t) : 0;
+	}
+	__syncthreads();
+	return s_base + s_off[wave] + (unsigned long long)__popcll(mask & ((1ull << lane) - 1));
 }
 
+// A level's totals as the device hands them to the host, in pinned fine-grained
+// host memory: k_bfs_report writes the two counts, then `seq` (release, system
+// scope), so a host that reads `seq` equal to the number it waits for reads the
+// counts of that level. It replaces a copy to the host and a stream
+// synchronisation per level.
+struct BfsReport {
+	unsigned long long vertices, edges;
+	unsigned long long seq;
+};
+
+// A level's kernel leaves per-block totals, part[b] vertices found and
+// part[gridDim.x + b] the sum of their out-degrees, which k_bfs_report folds.
+// Counting into two global words instead serialises the kernel on them: every
+// same-address atomic waits for the one before, and a push level whose 638
+// blocks found 163 K vertices spent 66 us on it.
+__device__ __forceinline__ void bfs_block_totals(unsigned long long nv, unsigned long long ne,
+                                                 unsigned long long *part) {
+	__shared__ unsigned long long s_nv[kBlock / kWave], s_ne[kBlock / kWave];
+	for (int o = kWave / 2; o > 0; o >>= 1) {
+		nv += __shfl_xor(nv, o, kWave);
+		ne += __shfl_xor(ne, o, kWave);
+	}
+	if (threadIdx.x % kWave == 0) {
+		s_nv[threadIdx.x / kWave] = nv;
+		s_ne[threadIdx.x / kWave] = ne;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		unsigned long long a = 0, b = 0;
+		for (int i = 0; i < kBlock / kWave; ++i) {
+			a += s_nv[i];
+			b += s_ne[i];
+		}
+		part[blockIdx.x] = a;
+		part[gridDim.x + blockIdx.x] = b;
+	}
+}
+
+// After a level's final kernel, one block: folds its nb per-block totals into the
+// report, publishes seq, and empties the queue counters for the next level.
+__global__ __launch_bounds__(kBlock) void k_bfs_report(const unsigned long long *part, unsigned nb,
+                                                       BfsCounters *cnt, BfsReport *rep, unsigned long long seq) {
+	__shared__ unsigned long long lds[kBlock / kWave];
+	const unsigned long long v = block_sum_partials(part, nb, lds);
+	const unsigned long long e = block_sum_partials(part + nb, nb, lds);
+	if (threadIdx.x == 0) {
+		cnt->queued = 0;
+		cnt->collected = 0;
+		__hip_atomic_store(&rep->vertices, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+		__hip_atomic_store(&rep->edges, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+		__hip_atomic_store(&rep->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+	}
+}
+
+// levels = -1, the source 0, the first queue {source}, and level 0's totals
+// (one vertex, the source's out-degree) as one block's
+__global__ __launch_bounds__(kBlock) void k_bfs_init(size_t n, int32_t *level, uint32_t source, const uint64_t *off,
+                                                     uint32_t *queue, unsigned long long *part) {
+	grid_for_each<4>(n, [&](size_t i) { level[i] = i == source ? 0 : -1; });
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		queue[0] = source;
+		part[0] = 1;
+		part[1] = off[source + 1] - off[source];
+	}
+}
+
+// degrees of the queued vertices, and a zero after them (the scan's total)
 __global__ __launch_bounds__(kBlock) void k_frontier_degrees(size_t q, const uint32_t *queue, const uint64_t *off,
                                                              uint64_t *deg) {
 	grid_for_each<4>(q, [&](size_t i) {
 		const uint32_t v = queue[i];
 		deg[i] = off[v + 1] - off[v];
 	});
+	if (blockIdx.x == 0 && threadIdx.x == 0) deg[q] = 0;
+}
+
+// The exclusive scan of the degrees of a queue of at most kFrontierScanMax
+// vertices (scan[q] = their total) by one block, each thread holding
+// kFrontierItems consecutive ones: one launch where the degrees and a library
+// scan take three, for the small frontiers of the first and last levels.
+constexpr int kFrontierItems = 16;
+constexpr size_t kFrontierScanMax = (size_t)kBlock * kFrontierItems;
+
+__global__ __launch_bounds__(kBlock) void k_frontier_scan(size_t q, const uint32_t *queue, const uint64_t *off,
+                                                          uint64_t *scan) {
+	__shared__ uint64_t s_wave[kBlock / kWave];
+	const unsigned lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+	const size_t first = (size_t)threadIdx.x * kFrontierItems;
+	uint64_t d[kFrontierItems];
+	uint64_t sum = 0;
+#pragma unroll
+	for (int k = 0; k < kFrontierItems; ++k) {
+		const size_t i = first + k;
+		d[k] = 0;
+		if (i < q) {
+			const uint32_t v = queue[i];
+			d[k] = off[v + 1] - off[v];
+		}
+		sum += d[k];
+	}
+	uint64_t inc = sum;  // inclusive scan of the thread sums over the wave
+	for (int o = 1; o < kWave; o <<= 1) {
+		const uint64_t t = __shfl_up(inc, o, kWave);
+		if ((int)lane >= o) inc += t;
+	}
+	if (lane == kWave - 1) s_wave[wave] = inc;
+	__syncthreads();
+	uint64_t run = inc - sum;
+	for (unsigned w = 0; w < wave; ++w) run += s_wave[w];
+#pragma unroll
+	for (int k = 0; k < kFrontierItems; ++k) {
+		const size_t i = first + k;
+		if (i < q) scan[i] = run;
+		run += d[k];
+	}
+	if (threadIdx.x == kBlock - 1) scan[q] = run;  // the total: items past q add nothing
 }
 
 // push: one thread per frontier edge; vertex found by binary search in the
-// exclusive scan of frontier degrees
+// exclusive scan of frontier degrees. With `single` (a frontier of one vertex in
+// a pattern that stores no position twice) each neighbour is reached by one edge
+// only, so its claim is a plain store; otherwise by compare-and-swap.
 __global__ __launch_bounds__(kBlock) void k_bfs_push(size_t q, size_t fedges, const uint32_t *queue,
                                                      const uint64_t *scan, const uint64_t *off,
                                                      const uint32_t *idx, int32_t *level, int32_t next,
-                                                     uint32_t *nqueue, BfsCounters *cnt) {
-	grid_for_each<1>(fedges, [&](size_t e) {
-		size_t lo = 0, hi = q;  // largest i with scan[i] <= e
-		while (hi - lo > 1) {
-			const size_t mid = (lo + hi) / 2;
-			if (scan[mid] <= e) lo = mid;
-			else hi = mid;
+                                                     uint32_t *nqueue, BfsCounters *cnt, unsigned long long *part,
+                                                     int single) {
+	unsigned long long nv = 0, ne = 0;
+	const size_t stride = (size_t)gridDim.x * kBlock;
+	for (size_t e0 = (size_t)blockIdx.x * kBlock; e0 < fedges; e0 += stride) {
+		const size_t e = e0 + threadIdx.x;
+		bool found = false;
+		uint32_t w = 0;
+		if (e < fedges) {
+			size_t lo = 0, hi = q;  // largest i with scan[i] <= e
+			while (hi - lo > 1) {
+				const size_t mid = (lo + hi) / 2;
+				if (scan[mid] <= e) lo = mid;
+				else hi = mid;
+			}
+			const uint32_t v = queue[lo];
+			w = idx[off[v] + (e - scan[lo])];
+			if (level[w] < 0) {
+				if (single) {
+					level[w] = next;
+					found = true;
+				} else {
+					found = atomicCAS(level + w, -1, next) == -1;
+				}
+			}
 		}
-		const uint32_t v = queue[lo];
-		const uint32_t w = idx[off[v] + (e - scan[lo])];
-		if (level[w] < 0 && atomicCAS(level + w, -1, next) == -1) {
-			const unsigned long long pos = atomicAdd(&cnt->next_vertices, 1ull);
+		const unsigned long long pos = block_slots(found, &cnt->queued);
+		if (found) {
 			nqueue[pos] = w;
-			atomicAdd(&cnt->next_edges, (unsigned long long)(off[w + 1] - off[w]));
+			nv += 1;
+			ne += off[w + 1] - off[w];
 		}
-	});
+	}
+	bfs_block_totals(nv, ne, part);
 }
 
 // pull: every unvisited vertex scans its in-neighbours (the CCS column of w; the
 // CRS row when the pattern is symmetric) for one in the frontier (level == cur)
-// and stops at the first one (lor early exit). The edge counter adds w's
+// and stops at the first one (lor early exit). The edge total adds w's
 // out-degree (out_off), which is what a following push step expands.
 __global__ __launch_bounds__(kBlock) void k_bfs_pull(size_t n, const uint64_t *off, const uint32_t *idx,
                                                      const uint64_t *out_off, int32_t *level, int32_t cur,
-                                                     BfsCounters *cnt) {
+                                                     unsigned long long *part) {
 	unsigned long long nv = 0, ne = 0;
 	grid_for_each<1>(n, [&](size_t w) {
 		if (level[w] >= 0) return;
@@ -152,34 +305,19 @@ __global__ __launch_bounds__(kBlock) void k_bfs_pull(size_t n, const uint64_t *o
 			}
 		}
 	});
-	// block totals, then one atomic per block (integer sums: order-free)
-	__shared__ unsigned long long s_nv[kBlock / kWave], s_ne[kBlock / kWave];
-	for (int s = kWave / 2; s > 0; s >>= 1) {
-		nv += __shfl_xor(nv, s, kWave);
-		ne += __shfl_xor(ne, s, kWave);
-	}
-	if (threadIdx.x % kWave == 0) {
-		s_nv[threadIdx.x / kWave] = nv;
-		s_ne[threadIdx.x / kWave] = ne;
-	}
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		unsigned long long a = 0, b = 0;
-		for (int i = 0; i < kBlock / kWave; ++i) {
-			a += s_nv[i];
-			b += s_ne[i];
-		}
-		if (a) atomicAdd(&cnt->next_vertices, a);
-		if (b) atomicAdd(&cnt->next_edges, b);
-	}
+	bfs_block_totals(nv, ne, part);
 }
 
 // queue of the vertices at level `lev` (after a pull step, to switch back to push)
 __global__ __launch_bounds__(kBlock) void k_bfs_collect(size_t n, const int32_t *level, int32_t lev,
-                                                        uint32_t *queue, unsigned long long *count) {
-	grid_for_each<4>(n, [&](size_t i) {
-		if (level[i] == lev) queue[atomicAdd(count, 1ull)] = (uint32_t)i;
-	});
+                                                        uint32_t *queue, BfsCounters *cnt) {
+	const size_t stride = (size_t)gridDim.x * kBlock;
+	for (size_t i0 = (size_t)blockIdx.x * kBlock; i0 < n; i0 += stride) {
+		const size_t i = i0 + threadIdx.x;
+		const bool take = i < n && level[i] == lev;
+		const unsigned long long pos = block_slots(take, &cnt->collected);
+		if (take) queue[pos] = (uint32_t)i;
+	}
 }
 
 // ---- PageRank (simple_pagerank.hpp:146-533) -------------------------------------------
@@ -515,6 +653,7 @@ ALP_EXPORT alp_rc_t alp_hip_graph_from_edges(alp_hip_matrix_t *A, size_t n, size
 	}
 	if (nnz_out) *nnz_out = M->nnz;
 	M->symmetric_pattern = sym;
+	M->unique_entries = true;  // built from sorted unique keys
 	*A = M;
 	return ALP_SUCCESS;
 }
@@ -547,6 +686,7 @@ struct BfsWork {
 	DevBuf<uint32_t> q0, q1;
 	DevBuf<uint64_t> deg, scan;
 	DevBuf<BfsCounters> cnt;
+	DevBuf<unsigned long long> part;  // per-block level totals (two per block of the widest grid)
 	DevBuf<char> tmp;
 	DevBuf<int32_t> lev;  // k-hop's levels (alp_hip_bfs writes the caller's array)
 	size_t scan_tb = 0;
@@ -557,7 +697,14 @@ struct BfsWork {
 	Storage pr_pat;
 	DevBuf<uint32_t> pr_perm;
 	DevBuf<double> pr_x;
-	~BfsWork() { storage_free(pr_pat); }
+	// the level reports (pinned, fine-grained), their device address and the
+	// number of the last one asked for
+	BfsReport *rep = nullptr, *rep_dev = nullptr;
+	unsigned long long seq = 0;
+	~BfsWork() {
+		storage_free(pr_pat);
+		if (rep) hipHostFree(rep);
+	}
 };
 
 static void free_bfs_work(void *p) { delete static_cast<BfsWork *>(p); }
@@ -580,16 +727,33 @@ static alp_rc_t bfs_work(alp_hip_matrix_s *A, BfsWork **out) {
 		const size_t n = A->nrows;
 		alp_rc_t rc = ALP_SUCCESS;
 		size_t tb = 0;
+		if (!w->rep) {
+			if (hipHostMalloc(&w->rep, sizeof(BfsReport), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) {
+				w->rep = nullptr;
+				return ALP_OUTOFMEM;
+			}
+			*w->rep = BfsReport{};
+			if (hipHostGetDevicePointer((void **)&w->rep_dev, w->rep, 0) != hipSuccess) {
+				hipHostFree(w->rep);
+				w->rep = nullptr;
+				return ALP_PANIC;
+			}
+			w->seq = 0;
+		}
 		if ((rc = w->q0.alloc(n)) || (rc = w->q1.alloc(n)) || (rc = w->deg.alloc(n + 1)) ||
 		    (rc = w->scan.alloc(n + 1)) || (rc = w->cnt.alloc(1)) || (rc = w->lev.alloc(n)) ||
+		    (rc = w->part.alloc(2 * (size_t)fill_blocks(8))) ||  // g_grid's largest grid
 		    (hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w->deg.p, w->scan.p, n + 1, stream()) != hipSuccess
 		         ? (rc = ALP_PANIC)
-		         : (rc = w->tmp.alloc(tb)))) {
+		         : (rc = w->tmp.alloc(tb))) ||
+		    // the queue counter starts at zero; each level's report leaves it so
+		    (hipMemsetAsync(w->cnt.p, 0, sizeof(BfsCounters), stream()) != hipSuccess ? (rc = ALP_PANIC) : 0)) {
 			w->q0.release();
 			w->q1.release();
 			w->deg.release();
 			w->scan.release();
 			w->cnt.release();
+			w->part.release();
 			w->lev.release();
 			w->tmp.release();
 			return rc;
@@ -597,6 +761,28 @@ static alp_rc_t bfs_work(alp_hip_matrix_s *A, BfsWork **out) {
 		w->scan_tb = tb;
 	}
 	*out = w;
+	return ALP_SUCCESS;
+}
+
+// Waits for report w->seq and returns its counts. Spins on the pinned word for up
+// to 500 us (a level's kernels at scale 22 take 2-90 us; a spin sees the report
+// microseconds after the last block writes it, where a copy and a stream
+// synchronisation take tens), then blocks on the stream, after which the report
+// is there unless the kernel failed.
+static alp_rc_t bfs_wait(hipStream_t s, const BfsWork *w, unsigned long long *vertices,
+                         unsigned long long *edges) {
+	const BfsReport *r = w->rep;
+	const auto t0 = std::chrono::steady_clock::now();
+	unsigned spins = 0;
+	while (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != w->seq) {
+		if ((++spins & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(500)) {
+			ALP_HIP_TRY(hipStreamSynchronize(s));
+			if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != w->seq) return ALP_PANIC;
+			break;
+		}
+	}
+	*vertices = __atomic_load_n(&r->vertices, __ATOMIC_RELAXED);
+	*edges = __atomic_load_n(&r->edges, __ATOMIC_RELAXED);
 	return ALP_SUCCESS;
 }
 
@@ -618,12 +804,14 @@ static alp_rc_t bfs_run(alp_hip_matrix_s *A, uint32_t source, int32_t *d_level, 
 	DevBuf<BfsCounters> &cnt = w->cnt;
 	DevBuf<char> &tmp = w->tmp;
 	size_t scan_tb = w->scan_tb;
-	hipLaunchKernelGGL(k_bfs_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, d_level, source);
-	ALP_HIP_TRY(hipStreamSynchronize(s));
-	ALP_HIP_TRY(copy_sync(q0.p, &source, sizeof(uint32_t), hipMemcpyHostToDevice));
-	uint64_t src_deg[2];
-	ALP_HIP_TRY(copy_sync(src_deg, S.off + source, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
-	unsigned long long fq = 1, fe = src_deg[1] - src_deg[0];
+	unsigned long long *part = w->part.p;
+	hipLaunchKernelGGL(k_bfs_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, d_level, source, (const uint64_t *)S.off,
+	                   q0.p, part);
+	hipLaunchKernelGGL(k_bfs_report, dim3(1), dim3(kBlock), 0, s, (const unsigned long long *)part, 1u, cnt.p,
+	                   w->rep_dev, ++w->seq);
+	ALP_LAUNCH_CHECK();
+	unsigned long long fq = 0, fe = 0;
+	ALP_TRY(bfs_wait(s, w, &fq, &fe));
 	unsigned long long unexplored = A->nnz - fe;
 	bool queue_valid = true;  // q0 holds the frontier
 	size_t total = 1;
@@ -634,39 +822,44 @@ static alp_rc_t bfs_run(alp_hip_matrix_s *A, uint32_t source, int32_t *d_level, 
 		// ALP's choice between scatter and gather, by the frontier's edge count
 		if (!pull && (double)fe > (double)unexplored / alpha) pull = true;
 		else if (pull && (double)fq < (double)n / beta) pull = false;
-		ALP_HIP_TRY(hipMemsetAsync(cnt.p, 0, sizeof(BfsCounters), s));
+		unsigned nb = 0;  // blocks of the level's final kernel, one total each
 		if (pull) {
-			hipLaunchKernelGGL(k_bfs_pull, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const uint64_t *)P.off,
-			                   (const uint32_t *)P.idx, (const uint64_t *)S.off, d_level, cur, cnt.p);
+			nb = g_grid(n);
+			hipLaunchKernelGGL(k_bfs_pull, dim3(nb), dim3(kBlock), 0, s, n, (const uint64_t *)P.off,
+			                   (const uint32_t *)P.idx, (const uint64_t *)S.off, d_level, cur, part);
 			queue_valid = false;
 		} else {
+			if (fe == 0) break;  // a frontier without edges reaches nothing
 			if (!queue_valid) {
-				unsigned long long *c = reinterpret_cast<unsigned long long *>(scan.p);  // scratch counter
-				ALP_HIP_TRY(hipMemsetAsync(c, 0, sizeof(unsigned long long), s));
 				hipLaunchKernelGGL(k_bfs_collect, dim3(g_grid(n)), dim3(kBlock), 0, s, n, (const int32_t *)d_level, cur,
-				                   q0.p, c);
+				                   q0.p, cnt.p);
 				queue_valid = true;
 			}
-			hipLaunchKernelGGL(k_frontier_degrees, dim3(g_grid(fq)), dim3(kBlock), 0, s, (size_t)fq,
-			                   (const uint32_t *)q0.p, (const uint64_t *)S.off, deg.p);
-			ALP_HIP_TRY(hipMemsetAsync(deg.p + fq, 0, sizeof(uint64_t), s));
-			ALP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, scan_tb, deg.p, scan.p, (size_t)fq + 1, s));
-			if (fe > 0)
-				hipLaunchKernelGGL(k_bfs_push, dim3(g_grid(fe)), dim3(kBlock), 0, s, (size_t)fq, (size_t)fe,
-				                   (const uint32_t *)q0.p, (const uint64_t *)scan.p, (const uint64_t *)S.off,
-				                   (const uint32_t *)S.idx, d_level, cur + 1, q1.p, cnt.p);
+			if (fq <= kFrontierScanMax) {
+				hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(kBlock), 0, s, (size_t)fq, (const uint32_t *)q0.p,
+				                   (const uint64_t *)S.off, scan.p);
+			} else {
+				hipLaunchKernelGGL(k_frontier_degrees, dim3(g_grid(fq)), dim3(kBlock), 0, s, (size_t)fq,
+				                   (const uint32_t *)q0.p, (const uint64_t *)S.off, deg.p);
+				ALP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, scan_tb, deg.p, scan.p, (size_t)fq + 1, s));
+			}
+			nb = g_grid(fe);
+			hipLaunchKernelGGL(k_bfs_push, dim3(nb), dim3(kBlock), 0, s, (size_t)fq, (size_t)fe,
+			                   (const uint32_t *)q0.p, (const uint64_t *)scan.p, (const uint64_t *)S.off,
+			                   (const uint32_t *)S.idx, d_level, cur + 1, q1.p, cnt.p, part,
+			                   fq == 1 && A->unique_entries ? 1 : 0);
 			std::swap(q0.p, q1.p);
 		}
+		hipLaunchKernelGGL(k_bfs_report, dim3(1), dim3(kBlock), 0, s, (const unsigned long long *)part, nb, cnt.p,
+		                   w->rep_dev, ++w->seq);
 		ALP_LAUNCH_CHECK();
-		BfsCounters h{};
-		ALP_HIP_TRY(hipMemcpyAsync(&h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
-		ALP_HIP_TRY(hipStreamSynchronize(s));
-		fq = h.next_vertices;
-		fe = h.next_edges;
+		ALP_TRY(bfs_wait(s, w, &fq, &fe));
 		unexplored = unexplored > fe ? unexplored - fe : 0;
 		total += fq;
 		if (fq > 0) ++cur;
 	}
+	// the last kernel has published; the levels are complete when this returns
+	ALP_HIP_TRY(hipStreamSynchronize(s));
 	if (nlevels) *nlevels = (size_t)cur + 1;
 	if (reached) *reached = total;
 	return ALP_SUCCESS;

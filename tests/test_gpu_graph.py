@@ -147,6 +147,36 @@ def test_bfs_directed_pull_walks_in_neighbours(A):
     assert list(lev.numpy()) == [0, 1, -1] and nl == 2 and reached == 2
 
 
+def test_bfs_on_a_pattern_with_repeated_entries(A):
+    """A COO pattern that stores positions more than once (create_coo keeps
+    them, as buildMatrixUnique keeps what it is handed): every vertex is still
+    claimed once per level, so the levels and the count of reached vertices
+    match the oracle's queue BFS over the same CRS. Vertex 0's row repeats each
+    of its 300 neighbours three times, so its first level may not take the
+    plain-store claim of a duplicate-free single-vertex frontier."""
+    rng = np.random.default_rng(5)
+    n = 4096
+    I = rng.integers(0, n, 40000)
+    J = rng.integers(0, n, 40000)
+    nb = rng.choice(np.arange(1, n), 300, replace=False)
+    I = np.concatenate([I, np.zeros(900, np.int64)])
+    J = np.concatenate([J, np.tile(nb, 3)])
+    G = A.Matrix.from_coo(n, n, I, J)
+    off, idx, _ = G.download(0, with_values=False)
+    ref = O.CSR(n, n, off, idx, None)
+    out = A.Vector(n, np.int32)
+    for source in (0, 17, int(rng.integers(0, n))):
+        lev, nl, reached = A.bfs(G, source)
+        want = O.bfs_levels(ref, source)
+        assert np.array_equal(lev.numpy(), want), source
+        assert reached == int((want >= 0).sum()) and nl == int(want.max()) + 1
+        # into a caller-owned vector, reused across sources
+        lev2, nl2, reached2 = A.bfs(G, source, out=out)
+        assert lev2 is out and np.array_equal(out.numpy(), want) and (nl2, reached2) == (nl, reached)
+    with pytest.raises(ValueError):
+        A.bfs(G, 0, out=A.Vector(n, np.uint8))
+
+
 @pytest.mark.parametrize("scale,seed", [(12, 4), (16, 9)])
 def test_bfs_directed_rmat_bitexact(A, scale, seed):
     """Directed R-MAT graphs (self loops dropped, no symmetrisation): levels
