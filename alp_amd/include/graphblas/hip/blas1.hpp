@@ -152,8 +152,7 @@ __global__ void k_ewise(size_t n, Z *z, uint8_t *za, Arg<X> x, Arg<Y> y, Arg<W> 
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
 		c += ew_element<MODE, MONOID, descr, MASKED, ADD, MUL, Z, X, Y, W, M>(i, z, za, x, y, w, mval, masg) ? 1 : 0;
 	if (!count) return;
-	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
-	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+	count_add(c, count);
 }
 
 // the same pass as a stage of a fused kernel (its result count is known on the host)

@@ -136,8 +136,7 @@ __global__ void k_mv_gather(size_t m, size_t nsrc, const uint64_t *off, const ui
 		c += yout ? 1 : 0;
 	}
 	if (!count) return;
-	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
-	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+	count_add(c, count);
 }
 
 // the push: for each input nonzero j, fold its products into the outputs i of
@@ -216,8 +215,7 @@ __global__ void k_mv_push_finish(size_t m, size_t nsrc, Arg<X> x, const unsigned
 		c += yout ? 1 : 0;
 	}
 	if (!count) return;
-	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
-	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+	count_add(c, count);
 }
 
 template <typename T>

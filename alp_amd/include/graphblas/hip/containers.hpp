@@ -56,9 +56,9 @@ public:
 	explicit Vector(const size_t n = 0) : n_(n) {
 		if (internal::rc(alp_hip_init(0)) != SUCCESS) throw std::runtime_error("grb::Vector: no device");
 		val_.alloc(n_);
-		// the assigned flags, then (8-byte aligned) the device slot of the count
-		asg_.alloc(slot_offset() + sizeof(unsigned long long));
-		const size_t bytes = slot_offset() + sizeof(unsigned long long);
+		// the assigned flags, then (128-byte aligned) the device slot of the count
+		const size_t bytes = slot_offset() + sizeof(unsigned long long) * internal::hip::kCountWords;
+		asg_.alloc(bytes);
 		hipLaunchKernelGGL(internal::hip::k_fill<uint8_t>, dim3(internal::hip::grid(bytes)),
 		                   dim3(internal::hip::kBlock), 0, internal::hip::stream(), bytes, asg_.p, (uint8_t)0);
 	}
@@ -118,8 +118,9 @@ public:
 	unsigned long long *count_slot() {
 		unsigned long long *c = slot();
 		if (slot_dirty_)
-			hipLaunchKernelGGL(internal::hip::k_fill<unsigned long long>, dim3(1), dim3(64), 0,
-			                   internal::hip::stream(), (size_t)1, c, 0ull);
+			hipLaunchKernelGGL(internal::hip::k_fill<unsigned long long>, dim3(internal::hip::kCountLanes),
+			                   dim3(internal::hip::kCountStride), 0, internal::hip::stream(),
+			                   internal::hip::kCountWords, c, 0ull);
 		slot_dirty_ = true;
 		return c;
 	}
@@ -180,22 +181,24 @@ public:
 	const_iterator end() const { return cend(); }
 
 private:
-	size_t slot_offset() const { return (n_ + 7) / 8 * 8; }
+	size_t slot_offset() const { return (n_ + 127) / 128 * 128; }
 	unsigned long long *slot() const { return reinterpret_cast<unsigned long long *>(asg_.p + slot_offset()); }
-	// a forcing point: reads the pending count back (the slot keeps the value until
-	// the next count zeroes it)
+	// a forcing point: reads the pending count back, the sum of the slot's lanes
+	// (the slot keeps them until the next count zeroes it)
 	void resolve() const {
 		if (!pending_) return;
 		internal::hip::Forcing f;
-		unsigned long long v = 0;
+		unsigned long long lanes[internal::hip::kCountWords];
 		hipError_t e = f.rc ? hipErrorUnknown : hipSuccess;
 		if (e == hipSuccess)
-			e = hipMemcpyAsync(&v, slot(), sizeof(v), hipMemcpyDeviceToHost, internal::hip::library_stream());
+			e = hipMemcpyAsync(lanes, slot(), sizeof(lanes), hipMemcpyDeviceToHost, internal::hip::library_stream());
 		if (e == hipSuccess) e = hipStreamSynchronize(internal::hip::library_stream());
 		if (e != hipSuccess) {
 			(void)hipGetLastError();
 			throw std::runtime_error("grb::Vector: nonzero count readback failed");
 		}
+		unsigned long long v = 0;
+		for (size_t l = 0; l < internal::hip::kCountLanes; ++l) v += lanes[l * internal::hip::kCountStride];
 		nz_ = (size_t)v;
 		pending_ = false;
 	}

@@ -36,8 +36,7 @@ __global__ void k_set_masked(size_t n, T *x, uint8_t *xa, const M *mval, const u
 		c += on ? 1 : 0;
 	}
 	if (!count) return;
-	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
-	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+	count_add(c, count);
 }
 
 // stages of the fused element-wise pass (base.hpp): x_i = v; x_i removed; x_i = y_i
@@ -93,8 +92,7 @@ __global__ void k_copy(size_t n, T *x, uint8_t *xa, const U *y, const uint8_t *y
 		c += on ? 1 : 0;
 	}
 	if (!count) return;
-	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s, 64);
-	if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+	count_add(c, count);
 }
 
 // sparse build: entry k sets x[idx[k]]; a repeated index folds with Dup in input
@@ -229,10 +227,20 @@ RC setElement(Vector<T, b, C> &x, const V val, const size_t i, const Phase phase
 	return SUCCESS;
 }
 
+namespace internal {
+namespace hip {
+// host staging of values bound for a device array of T: std::vector<bool> packs
+// bits and has no data(), so bool values stage as bytes (one byte per bool here)
+template <typename T>
+using HostStage = std::vector<typename std::conditional<std::is_same<T, bool>::value, uint8_t, T>::type>;
+static_assert(sizeof(bool) == 1, "bool vectors stage their values as bytes");
+}  // namespace hip
+}  // namespace internal
+
 // dense build from a range of size(x) values (io.hpp buildVector)
 template <Descriptor descr = descriptors::no_operation, typename T, enum Backend b, typename C, typename It>
 RC buildVector(Vector<T, b, C> &x, It begin, const It end, const IOMode) {
-	std::vector<T> h;
+	internal::hip::HostStage<T> h;
 	for (; begin != end; ++begin) h.push_back(static_cast<T>(*begin));
 	if (h.size() != x.size()) return MISMATCH;
 	internal::hip::Forcing f;  // a copy from host memory
@@ -254,7 +262,7 @@ template <Descriptor descr = descriptors::no_operation, typename T, enum Backend
 RC buildVector(Vector<T, b, C> &x, ItI ind, const ItI ind_end, ItV val, const ItV val_end, const IOMode,
                const Dup & = Dup()) {
 	std::vector<size_t> hi;
-	std::vector<T> hv;
+	internal::hip::HostStage<T> hv;
 	for (; ind != ind_end && val != val_end; ++ind, ++val) {
 		if ((size_t)*ind >= x.size()) return MISMATCH;
 		hi.push_back((size_t)*ind);

@@ -311,6 +311,34 @@ int main(int argc, char **argv) {
 	emit_vec("chain_c3", c3);
 	emit_vec("chain_c4", c4);
 	emit_vec("chain_y", yc);
+
+	// 12. boolean vectors built from index and value ranges and from a dense
+	//     range, and a count on the device over 2^21 entries (4096 blocks, so
+	//     every lane of the count slot takes a share)
+	{
+		std::vector<size_t> bi;
+		std::vector<bool> bv;
+		for (size_t i = 0; i < n; i += 2) {
+			bi.push_back(i);
+			bv.push_back(i % 4 == 0);
+		}
+		Vector<bool> b1(n), b2(n);
+		rc = rc ? rc : buildVector(b1, bi.begin(), bi.end(), bv.begin(), bv.end(), SEQUENTIAL);
+		std::vector<bool> dv(n);
+		for (size_t i = 0; i < n; ++i) dv[i] = i % 3 == 0;
+		rc = rc ? rc : buildVector(b2, dv.begin(), dv.end(), SEQUENTIAL);
+		emit_vec("bool_sparse", b1);
+		emit_vec("bool_dense", b2);
+		const size_t L = size_t(1) << 21;
+		std::vector<bool> lmv(L);
+		for (size_t i = 0; i < L; ++i) lmv[i] = i % 5 == 0;
+		Vector<bool> lm(L);
+		Vector<double> lv(L);
+		rc = rc ? rc : buildVector(lm, lmv.begin(), lmv.end(), SEQUENTIAL);
+		rc = rc ? rc : set(lv, lm, 1.0);  // a value mask: counted on the device
+		emit_num("long_n", (double)L);
+		emit_num("long_nnz", (double)nnz(lv));
+	}
 	emit_num("rc", (double)rc);
 	out += "}";
 	std::printf("%s\n", out.c_str());

@@ -154,7 +154,8 @@ def test_grb_generic_semirings_and_sparse_semantics(seed):
     program-defined operator; lor-land nonzero counts, the pushed sparse input,
     masks and add_identity; OVERLAP, ILLEGAL and MISMATCH; element-wise
     operations, dots and folds under sparsity; a device eWiseLambda; mxm over
-    plus-min on int (the generic SpGEMM) and plus-times; masked set."""
+    plus-min on int (the generic SpGEMM) and plus-times; masked set; boolean
+    vectors built from ranges; a device count over 2^21 entries."""
     r = _run(SEM_PROG, seed)
     assert r["rc"] == 0
     m, n = int(r["m"]), int(r["n"])
@@ -249,6 +250,11 @@ def test_grb_generic_semirings_and_sparse_semantics(seed):
     _assert_vec(r, "chain_c3", c3)
     _assert_vec(r, "chain_c4", c4)
     _assert_vec(r, "chain_y", _gather(rows, m, c3, mul, add, 0.0))
+    # 12: boolean builds (false entries are entries) and a count over all lanes
+    _assert_vec(r, "bool_sparse", {i: float(i % 4 == 0) for i in range(0, n, 2)})
+    _assert_vec(r, "bool_dense", {i: float(i % 3 == 0) for i in range(n)})
+    L = int(r["long_n"])
+    assert int(r["long_nnz"]) == (L + 4) // 5
 
 
 def test_grb_hpcg_program_as_the_reference_smoke_test(golden):
