@@ -6,6 +6,8 @@
 #ifndef ALP_HIP_GRB_IO_HPP
 #define ALP_HIP_GRB_IO_HPP
 
+#include <algorithm>
+
 #include "containers.hpp"
 
 namespace grb {
@@ -95,19 +97,22 @@ __global__ void k_copy(size_t n, T *x, uint8_t *xa, const U *y, const uint8_t *y
 	count_add(c, count);
 }
 
-// sparse build: entry k sets x[idx[k]]; a repeated index folds with Dup in input
-// order (one thread walks the entries in order: the fold order is the input's)
+// sparse build: one thread per distinct index idx[s], whose values are
+// vals[seg[s] .. seg[s + 1]) in input order (seg null: one value each, vals[s]).
+// The first value is set where x holds nothing at idx[s]; every other folds into
+// x with Dup, in input order, as a walk over the input in order would fold them.
 template <class Dup, typename T>
-__global__ void k_scatter_build(size_t nz, const size_t *idx, const T *vals, T *x, uint8_t *xa) {
-	if (blockIdx.x != 0 || threadIdx.x != 0) return;
-	for (size_t k = 0; k < nz; ++k) {
-		const size_t i = idx[k];
-		if (xa[i]) {
-			foldl_op<Dup>(x[i], vals[k]);
-		} else {
-			x[i] = vals[k];
+__global__ void k_scatter_build(size_t nseg, const size_t *idx, const size_t *seg, const T *vals, T *x,
+                                uint8_t *xa) {
+	for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += (size_t)gridDim.x * blockDim.x) {
+		size_t k = seg ? seg[s] : s;
+		const size_t e = seg ? seg[s + 1] : s + 1;
+		const size_t i = idx[s];
+		if (!xa[i]) {
+			x[i] = vals[k++];
 			xa[i] = 1;
 		}
+		for (; k < e; ++k) foldl_op<Dup>(x[i], vals[k]);
 	}
 }
 
@@ -270,17 +275,50 @@ RC buildVector(Vector<T, b, C> &x, ItI ind, const ItI ind_end, ItV val, const It
 	}
 	if (ind != ind_end || val != val_end) return MISMATCH;
 	if (hi.empty()) return SUCCESS;
+	// a repeated index: group each index's values, in input order, for one thread
+	std::vector<size_t> seg;
+	{
+		std::vector<uint8_t> seen((x.size() + 7) / 8, 0);
+		bool repeats = false;
+		for (const size_t i : hi) {
+			if (seen[i / 8] & (1u << (i % 8))) {
+				repeats = true;
+				break;
+			}
+			seen[i / 8] |= (uint8_t)(1u << (i % 8));
+		}
+		if (repeats) {
+			std::vector<size_t> order(hi.size());
+			for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+			std::stable_sort(order.begin(), order.end(), [&](size_t p, size_t q) { return hi[p] < hi[q]; });
+			std::vector<size_t> ui;
+			internal::hip::HostStage<T> gv;
+			gv.reserve(hv.size());
+			for (size_t k = 0; k < order.size(); ++k) {
+				if (k == 0 || hi[order[k]] != hi[order[k - 1]]) {
+					ui.push_back(hi[order[k]]);
+					seg.push_back(k);
+				}
+				gv.push_back(hv[order[k]]);
+			}
+			seg.push_back(order.size());
+			hi.swap(ui);
+			hv.swap(gv);
+		}
+	}
 	internal::hip::Forcing f;  // copies from host memory
 	if (f.rc) return f.rc;
-	internal::hip::DevArray<size_t> di(hi.size());
+	internal::hip::DevArray<size_t> di(hi.size()), ds(seg.size());
 	internal::hip::DevArray<T> dv(hv.size());
 	if (alp_hip_memcpy_h2d(di.p, hi.data(), sizeof(size_t) * hi.size()) ||
-	    alp_hip_memcpy_h2d(dv.p, hv.data(), sizeof(T) * hv.size()))
+	    alp_hip_memcpy_h2d(dv.p, hv.data(), sizeof(T) * hv.size()) ||
+	    (!seg.empty() && alp_hip_memcpy_h2d(ds.p, seg.data(), sizeof(size_t) * seg.size())))
 		return PANIC;
-	hipLaunchKernelGGL((internal::hip::k_scatter_build<Dup, T>), dim3(1), dim3(64), 0,
-	                   internal::hip::library_stream(), hi.size(), (const size_t *)di.p, (const T *)dv.p, x.raw(),
-	                   x.assigned());
-	return x.recount();  // di and dv are freed on return, after the scatter (hipFree waits)
+	hipLaunchKernelGGL((internal::hip::k_scatter_build<Dup, T>), dim3(internal::hip::grid(hi.size())),
+	                   dim3(internal::hip::kBlock), 0, internal::hip::library_stream(), hi.size(),
+	                   (const size_t *)di.p, seg.empty() ? (const size_t *)nullptr : (const size_t *)ds.p,
+	                   (const T *)dv.p, x.raw(), x.assigned());
+	return x.recount();  // di, ds and dv are freed on return, after the scatter (hipFree waits)
 }
 
 // ---- matrices -------------------------------------------------------------------

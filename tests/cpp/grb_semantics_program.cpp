@@ -338,6 +338,37 @@ int main(int argc, char **argv) {
 		rc = rc ? rc : set(lv, lm, 1.0);  // a value mask: counted on the device
 		emit_num("long_n", (double)L);
 		emit_num("long_nnz", (double)nnz(lv));
+		// repeated indices fold with Dup in input order, into an empty vector and
+		// into one that holds entries; subtraction does not commute, so the order shows
+		const std::vector<size_t> ri = {3, 5, 3, 3, 5, 0};
+		const std::vector<double> rv = {10, 1, 2, 4, 8, 0.5};
+		Vector<double> r1(n), r2(n);
+		rc = rc ? rc : buildVector(r1, ri.begin(), ri.end(), rv.begin(), rv.end(), SEQUENTIAL,
+		                           operators::subtract<double>());
+		rc = rc ? rc : setElement(r2, 100.0, 3);
+		rc = rc ? rc : setElement(r2, -1.0, 7);
+		rc = rc ? rc : buildVector(r2, ri.begin(), ri.end(), rv.begin(), rv.end(), SEQUENTIAL,
+		                           operators::subtract<double>());
+		emit_vec("dup_empty", r1);
+		emit_vec("dup_held", r2);
+		// a long build with no repeats: every third index, value index / 3
+		std::vector<size_t> li;
+		std::vector<double> lvv;
+		for (size_t i = L - 1; i < L; i -= 3) {  // descending: input order is not index order
+			li.push_back(i);
+			lvv.push_back((double)(i / 3));
+		}
+		Vector<double> lb(L);
+		rc = rc ? rc : buildVector(lb, li.begin(), li.end(), lvv.begin(), lvv.end(), SEQUENTIAL);
+		double lsum = 0;
+		size_t lbad = 0;
+		for (const auto &p : lb) {
+			lsum += p.second;
+			if (p.first % 3 != (L - 1) % 3 || p.second != (double)(p.first / 3)) ++lbad;
+		}
+		emit_num("long_build_nnz", (double)nnz(lb));
+		emit_num("long_build_sum", lsum);
+		emit_num("long_build_bad", (double)lbad);
 	}
 	emit_num("rc", (double)rc);
 	out += "}";

@@ -255,6 +255,12 @@ def test_grb_generic_semirings_and_sparse_semantics(seed):
     _assert_vec(r, "bool_dense", {i: float(i % 3 == 0) for i in range(n)})
     L = int(r["long_n"])
     assert int(r["long_nnz"]) == (L + 4) // 5
+    # repeated indices fold with subtract in input order; held entries fold first
+    _assert_vec(r, "dup_empty", {3: 10.0 - 2 - 4, 5: 1.0 - 8, 0: 0.5})
+    _assert_vec(r, "dup_held", {3: 100.0 - 10 - 2 - 4, 5: 1.0 - 8, 0: 0.5, 7: -1.0})
+    idx = range((L - 1) % 3, L, 3)
+    assert int(r["long_build_nnz"]) == len(idx) and int(r["long_build_bad"]) == 0
+    assert r["long_build_sum"] == float(sum(i // 3 for i in idx))
 
 
 def test_grb_hpcg_program_as_the_reference_smoke_test(golden):
