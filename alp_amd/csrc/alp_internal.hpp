@@ -43,16 +43,24 @@ void report_hip_error(hipError_t e, const char *what, const char *file, int line
 
 // ---- runtime context (grb::init<hip>) ---------------------------------------
 
+struct ScalarReport {
+	double v[64];
+	unsigned long long seq;
+};
+
 struct Context {
 	int device = -1;
 	hipStream_t own_stream = nullptr;
 	hipStream_t stream = nullptr;   // current stream (own or caller's)
 	int num_cus = 0;
-	// small device scratch for reductions: partial sums, tickets and results
+	// small device scratch for reductions: partial sums and tickets
 	double *d_partials = nullptr;   // kMaxPartials doubles
 	unsigned *d_tickets = nullptr;  // 64 counters, kept at zero between uses
-	double *d_scalars = nullptr;    // 64 result slots
-	double *h_pinned = nullptr;     // 64 pinned host doubles
+	// scalars reported to the host: pinned fine-grained memory that a one-thread
+	// kernel fills and releases (ScalarReport), its device address and the number
+	// of the last report asked for
+	struct ScalarReport *h_report = nullptr, *d_report = nullptr;
+	unsigned long long report_seq = 0;
 	// device address of the file-scope scratch array of alp_hip_cg_noalloc on
 	// this context's device (hipGetSymbolAddress at init)
 	unsigned char *d_noalloc = nullptr;
@@ -337,7 +345,9 @@ template <typename T>
 alp_rc_t cg_native(alp_hip_matrix_s *A, const T *b, T *x, size_t max_iter, double tol, HostPrecond<T> pc,
                    size_t *iters, double *residual);
 
-// device scalars shared with the host
-alp_rc_t read_scalars(double *h, const double *d, int count);
+// scalars reported to the host (Context::h_report): the number of the next
+// report, and the wait for report seq, whose first count values go to h
+unsigned long long next_report();
+alp_rc_t wait_report(double *h, unsigned long long seq, int count);
 
 }  // namespace alp

@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
@@ -296,6 +297,53 @@ inline void *workspace(size_t bytes) {
 		cap = want;
 	}
 	return p;
+}
+
+// A forcing reduction's result read back to the host: the device writes it into
+// pinned fine-grained host memory, then publishes `seq` with a system-scope
+// release; the host spins on seq for up to 500 us, then synchronises the stream.
+// A round trip after a small kernel took 9.2 us this way on MI355X, against 24
+// us for a copy to pageable memory and a stream synchronisation.
+struct HostReport {
+	unsigned long long seq;
+	unsigned int has;
+	alignas(16) unsigned char value[240];
+};
+struct HostReportSlot {
+	HostReport *h = nullptr, *d = nullptr;  // host and device addresses
+	unsigned long long seq = 0;             // the last report asked for
+};
+// this thread's report (made on first use, kept until the process ends)
+inline RC host_report(HostReportSlot *&out) {
+	static thread_local HostReportSlot slot;
+	if (!slot.h) {
+		void *p = nullptr;
+		if (hipHostMalloc(&p, sizeof(HostReport), hipHostMallocCoherent | hipHostMallocMapped | hipHostMallocPortable) !=
+		    hipSuccess)
+			return OUTOFMEM;
+		std::memset(p, 0, sizeof(HostReport));
+		void *d = nullptr;
+		if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+			(void)hipHostFree(p);
+			return PANIC;
+		}
+		slot.h = static_cast<HostReport *>(p);
+		slot.d = static_cast<HostReport *>(d);
+	}
+	out = &slot;
+	return SUCCESS;
+}
+inline RC host_report_wait(const HostReportSlot &s, unsigned long long seq) {
+	const auto t0 = std::chrono::steady_clock::now();
+	unsigned spins = 0;
+	while (__atomic_load_n(&s.h->seq, __ATOMIC_ACQUIRE) != seq) {
+		if ((++spins & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(500)) {
+			const RC r = from_hip(hipStreamSynchronize(library_stream()));
+			if (r) return r;
+			return __atomic_load_n(&s.h->seq, __ATOMIC_ACQUIRE) == seq ? SUCCESS : PANIC;
+		}
+	}
+	return SUCCESS;
 }
 
 // the device address of stage function F (fetched once per function, at a

@@ -214,9 +214,11 @@ __global__ void k_reduce_blocks(size_t n, Term term, T *part, uint8_t *part_has)
 	}
 }
 
-// out = the fold of the block results in order (out_has = any present)
+// the fold of the block results in order (has = any present), into the host
+// report, then seq
 template <class OP, typename T>
-__global__ void k_reduce_final(unsigned nb, const T *part, const uint8_t *part_has, T *out, uint8_t *out_has) {
+__global__ void k_reduce_final(unsigned nb, const T *part, const uint8_t *part_has, HostReport *rep,
+                               unsigned long long seq) {
 	if (blockIdx.x != 0 || threadIdx.x != 0) return;
 	T r = T();
 	bool rh = false;
@@ -225,8 +227,9 @@ __global__ void k_reduce_final(unsigned nb, const T *part, const uint8_t *part_h
 		if (rh) foldl_op<OP>(r, part[k]); else r = part[k];
 		rh = true;
 	}
-	*out = r;
-	*out_has = rh ? 1 : 0;
+	memcpy(rep->value, &r, sizeof(T));
+	rep->has = rh ? 1u : 0u;
+	__hip_atomic_store(&rep->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // a scalar output: a forcing point (the host waits for the value)
@@ -238,23 +241,27 @@ RC reduce(size_t n, Term term, T &result, bool &present) {
 	if (n == 0) return SUCCESS;
 	unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
 	if (nb > kReduceBlocks) nb = kReduceBlocks;
-	// workspace: nb + 1 values (block results, then the result), then nb + 1 flags
-	const size_t vbytes = (sizeof(T) * (nb + 1) + 15) / 16 * 16;
-	char *ws = static_cast<char *>(workspace(vbytes + nb + 1));
-	T *part = reinterpret_cast<T *>(ws), *out = part + nb;
-	uint8_t *part_has = reinterpret_cast<uint8_t *>(ws + vbytes), *out_has = part_has + nb;
+	static_assert(sizeof(T) <= sizeof(HostReport::value), "reduce: result type too large for the host report");
+	HostReportSlot *rs = nullptr;
+	RC r = host_report(rs);
+	if (r) return r;
+	// workspace: nb block results, then nb flags
+	const size_t vbytes = (sizeof(T) * nb + 15) / 16 * 16;
+	char *ws = static_cast<char *>(workspace(vbytes + nb));
+	T *part = reinterpret_cast<T *>(ws);
+	uint8_t *part_has = reinterpret_cast<uint8_t *>(ws + vbytes);
+	const unsigned long long seq = ++rs->seq;
 	hipLaunchKernelGGL((k_reduce_blocks<OP, T, Term>), dim3(nb), dim3(kBlock), 0, library_stream(), n, term, part,
 	                   part_has);
 	hipLaunchKernelGGL((k_reduce_final<OP, T>), dim3(1), dim3(64), 0, library_stream(), nb, (const T *)part,
-	                   (const uint8_t *)part_has, out, out_has);
-	RC r = from_hip(hipGetLastError());
+	                   (const uint8_t *)part_has, rs->d, seq);
+	r = from_hip(hipGetLastError());
 	if (r) return r;
-	uint8_t h = 0;
-	r = from_hip(hipMemcpyAsync(&result, out, sizeof(T), hipMemcpyDeviceToHost, library_stream()));
-	r = r ? r : from_hip(hipMemcpyAsync(&h, out_has, 1, hipMemcpyDeviceToHost, library_stream()));
-	r = r ? r : from_hip(hipStreamSynchronize(library_stream()));
-	present = h != 0;
-	return r;
+	r = host_report_wait(*rs, seq);
+	if (r) return r;
+	std::memcpy(&result, rs->h->value, sizeof(T));
+	present = rs->h->has != 0;
+	return SUCCESS;
 }
 
 // term of a dot product: x_i (x) y_i where both exist
