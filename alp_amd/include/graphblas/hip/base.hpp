@@ -429,6 +429,14 @@ __device__ __forceinline__ void count_add(unsigned long long c, unsigned long lo
 	}
 }
 
+// the count in a slot (the sum of its lanes) into the host report, then seq
+__global__ void k_count_report(const unsigned long long *count, HostReport *rep, unsigned long long seq) {
+	unsigned long long v = 0;
+	for (size_t l = 0; l < kCountLanes; ++l) v += count[l * kCountStride];
+	memcpy(rep->value, &v, sizeof(v));
+	__hip_atomic_store(&rep->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // number of nonzero bytes of a (assigned flags) added to the count slot
 template <typename F>
 __global__ void k_count(size_t n, const F *a, unsigned long long *count) {

@@ -183,22 +183,27 @@ public:
 private:
 	size_t slot_offset() const { return (n_ + 127) / 128 * 128; }
 	unsigned long long *slot() const { return reinterpret_cast<unsigned long long *>(asg_.p + slot_offset()); }
-	// a forcing point: reads the pending count back, the sum of the slot's lanes
-	// (the slot keeps them until the next count zeroes it)
+	// a forcing point: reads the pending count back, the sum of the slot's lanes,
+	// through the host report (the slot keeps them until the next count zeroes it)
 	void resolve() const {
 		if (!pending_) return;
 		internal::hip::Forcing f;
-		unsigned long long lanes[internal::hip::kCountWords];
-		hipError_t e = f.rc ? hipErrorUnknown : hipSuccess;
-		if (e == hipSuccess)
-			e = hipMemcpyAsync(lanes, slot(), sizeof(lanes), hipMemcpyDeviceToHost, internal::hip::library_stream());
-		if (e == hipSuccess) e = hipStreamSynchronize(internal::hip::library_stream());
-		if (e != hipSuccess) {
+		internal::hip::HostReportSlot *rs = nullptr;
+		RC r = f.rc ? f.rc : internal::hip::host_report(rs);
+		unsigned long long seq = 0;
+		if (r == SUCCESS) {
+			seq = ++rs->seq;
+			hipLaunchKernelGGL(internal::hip::k_count_report, dim3(1), dim3(1), 0, internal::hip::library_stream(),
+			                   (const unsigned long long *)slot(), rs->d, seq);
+			r = internal::hip::from_hip(hipGetLastError());
+		}
+		if (r == SUCCESS) r = internal::hip::host_report_wait(*rs, seq);
+		if (r != SUCCESS) {
 			(void)hipGetLastError();
 			throw std::runtime_error("grb::Vector: nonzero count readback failed");
 		}
 		unsigned long long v = 0;
-		for (size_t l = 0; l < internal::hip::kCountLanes; ++l) v += lanes[l * internal::hip::kCountStride];
+		std::memcpy(&v, rs->h->value, sizeof(v));
 		nz_ = (size_t)v;
 		pending_ = false;
 	}
