@@ -131,6 +131,7 @@ __device__ __forceinline__ unsigned long long block_slots(bool take, unsigned lo
 // synchronisation per level.
 struct BfsReport {
 	unsigned long long vertices, edges;
+	unsigned long long extra;  // a value the level's kernels left on the device (the source's degree)
 	unsigned long long seq;
 };
 
@@ -163,9 +164,11 @@ __device__ __forceinline__ void bfs_block_totals(unsigned long long nv, unsigned
 }
 
 // After a level's final kernel, one block: folds its nb per-block totals into the
-// report, publishes seq, and empties the queue counters for the next level.
+// report (with *extra, when given), publishes seq, and empties the queue counters
+// for the next level.
 __global__ __launch_bounds__(kBlock) void k_bfs_report(const unsigned long long *part, unsigned nb,
-                                                       BfsCounters *cnt, BfsReport *rep, unsigned long long seq) {
+                                                       BfsCounters *cnt, BfsReport *rep, unsigned long long seq,
+                                                       const uint64_t *extra) {
 	__shared__ unsigned long long lds[kBlock / kWave];
 	const unsigned long long v = block_sum_partials(part, nb, lds);
 	const unsigned long long e = block_sum_partials(part + nb, nb, lds);
@@ -174,20 +177,15 @@ __global__ __launch_bounds__(kBlock) void k_bfs_report(const unsigned long long 
 		cnt->collected = 0;
 		__hip_atomic_store(&rep->vertices, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 		__hip_atomic_store(&rep->edges, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+		if (extra) __hip_atomic_store(&rep->extra, (unsigned long long)*extra, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 		__hip_atomic_store(&rep->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 	}
 }
 
-// levels = -1, the source 0, the first queue {source}, and level 0's totals
-// (one vertex, the source's out-degree) as one block's
-__global__ __launch_bounds__(kBlock) void k_bfs_init(size_t n, int32_t *level, uint32_t source, const uint64_t *off,
-                                                     uint32_t *queue, unsigned long long *part) {
+// levels = -1, the source 0, and the first queue {source}
+__global__ __launch_bounds__(kBlock) void k_bfs_init(size_t n, int32_t *level, uint32_t source, uint32_t *queue) {
 	grid_for_each<4>(n, [&](size_t i) { level[i] = i == source ? 0 : -1; });
-	if (blockIdx.x == 0 && threadIdx.x == 0) {
-		queue[0] = source;
-		part[0] = 1;
-		part[1] = off[source + 1] - off[source];
-	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) queue[0] = source;
 }
 
 // degrees of the queued vertices, and a zero after them (the scan's total)
@@ -243,14 +241,16 @@ __global__ __launch_bounds__(kBlock) void k_frontier_scan(size_t q, const uint32
 }
 
 // push: one thread per frontier edge; vertex found by binary search in the
-// exclusive scan of frontier degrees. With `single` (a frontier of one vertex in
-// a pattern that stores no position twice) each neighbour is reached by one edge
-// only, so its claim is a plain store; otherwise by compare-and-swap.
-__global__ __launch_bounds__(kBlock) void k_bfs_push(size_t q, size_t fedges, const uint32_t *queue,
-                                                     const uint64_t *scan, const uint64_t *off,
-                                                     const uint32_t *idx, int32_t *level, int32_t next,
-                                                     uint32_t *nqueue, BfsCounters *cnt, unsigned long long *part,
-                                                     int single) {
+// exclusive scan of frontier degrees, whose last entry scan[q] is the number of
+// edges (read here, so the host can size the grid without knowing it). With
+// `single` (a frontier of one vertex in a pattern that stores no position twice)
+// each neighbour is reached by one edge only, so its claim is a plain store;
+// otherwise by compare-and-swap.
+__global__ __launch_bounds__(kBlock) void k_bfs_push(size_t q, const uint32_t *queue, const uint64_t *scan,
+                                                     const uint64_t *off, const uint32_t *idx, int32_t *level,
+                                                     int32_t next, uint32_t *nqueue, BfsCounters *cnt,
+                                                     unsigned long long *part, int single) {
+	const size_t fedges = (size_t)scan[q];
 	unsigned long long nv = 0, ne = 0;
 	const size_t stride = (size_t)gridDim.x * kBlock;
 	for (size_t e0 = (size_t)blockIdx.x * kBlock; e0 < fedges; e0 += stride) {
@@ -770,7 +770,7 @@ static alp_rc_t bfs_work(alp_hip_matrix_s *A, BfsWork **out) {
 // synchronisation take tens), then blocks on the stream, after which the report
 // is there unless the kernel failed.
 static alp_rc_t bfs_wait(hipStream_t s, const BfsWork *w, unsigned long long *vertices,
-                         unsigned long long *edges) {
+                         unsigned long long *edges, unsigned long long *extra = nullptr) {
 	const BfsReport *r = w->rep;
 	const auto t0 = std::chrono::steady_clock::now();
 	unsigned spins = 0;
@@ -783,6 +783,7 @@ static alp_rc_t bfs_wait(hipStream_t s, const BfsWork *w, unsigned long long *ve
 	}
 	*vertices = __atomic_load_n(&r->vertices, __ATOMIC_RELAXED);
 	*edges = __atomic_load_n(&r->edges, __ATOMIC_RELAXED);
+	if (extra) *extra = __atomic_load_n(&r->extra, __ATOMIC_RELAXED);
 	return ALP_SUCCESS;
 }
 
@@ -805,17 +806,34 @@ static alp_rc_t bfs_run(alp_hip_matrix_s *A, uint32_t source, int32_t *d_level, 
 	DevBuf<char> &tmp = w->tmp;
 	size_t scan_tb = w->scan_tb;
 	unsigned long long *part = w->part.p;
-	hipLaunchKernelGGL(k_bfs_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, d_level, source, (const uint64_t *)S.off,
-	                   q0.p, part);
-	hipLaunchKernelGGL(k_bfs_report, dim3(1), dim3(kBlock), 0, s, (const unsigned long long *)part, 1u, cnt.p,
-	                   w->rep_dev, ++w->seq);
+	hipLaunchKernelGGL(k_bfs_init, dim3(g_grid(n)), dim3(kBlock), 0, s, n, d_level, source, q0.p);
 	ALP_LAUNCH_CHECK();
 	unsigned long long fq = 0, fe = 0;
-	ALP_TRY(bfs_wait(s, w, &fq, &fe));
-	unsigned long long unexplored = A->nnz - fe;
+	unsigned long long unexplored = A->nnz;
 	bool queue_valid = true;  // q0 holds the frontier
 	size_t total = 1;
 	int32_t cur = 0;
+	if (max_depth > 0) {
+		// level 0 pushes from the source with no round trip before it: the push
+		// reads the source's degree from the scan, and the report hands it back
+		// (its direction is not chosen: a pull here would scan every vertex)
+		hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(kBlock), 0, s, (size_t)1, (const uint32_t *)q0.p,
+		                   (const uint64_t *)S.off, scan.p);
+		const unsigned nb = g_grid(A->nnz);
+		hipLaunchKernelGGL(k_bfs_push, dim3(nb), dim3(kBlock), 0, s, (size_t)1, (const uint32_t *)q0.p,
+		                   (const uint64_t *)scan.p, (const uint64_t *)S.off, (const uint32_t *)S.idx, d_level, 1, q1.p,
+		                   cnt.p, part, A->unique_entries ? 1 : 0);
+		std::swap(q0.p, q1.p);
+		hipLaunchKernelGGL(k_bfs_report, dim3(1), dim3(kBlock), 0, s, (const unsigned long long *)part, nb, cnt.p,
+		                   w->rep_dev, ++w->seq, (const uint64_t *)(scan.p + 1));
+		ALP_LAUNCH_CHECK();
+		unsigned long long fe0 = 0;
+		ALP_TRY(bfs_wait(s, w, &fq, &fe, &fe0));
+		unexplored = unexplored > fe0 ? unexplored - fe0 : 0;
+		unexplored = unexplored > fe ? unexplored - fe : 0;
+		total += fq;
+		if (fq > 0) ++cur;
+	}
 	const double alpha = 14.0, beta = 24.0;  // direction switching (Beamer et al.)
 	bool pull = false;
 	while (fq > 0 && cur < max_depth) {
@@ -844,14 +862,13 @@ static alp_rc_t bfs_run(alp_hip_matrix_s *A, uint32_t source, int32_t *d_level, 
 				ALP_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, scan_tb, deg.p, scan.p, (size_t)fq + 1, s));
 			}
 			nb = g_grid(fe);
-			hipLaunchKernelGGL(k_bfs_push, dim3(nb), dim3(kBlock), 0, s, (size_t)fq, (size_t)fe,
-			                   (const uint32_t *)q0.p, (const uint64_t *)scan.p, (const uint64_t *)S.off,
-			                   (const uint32_t *)S.idx, d_level, cur + 1, q1.p, cnt.p, part,
-			                   fq == 1 && A->unique_entries ? 1 : 0);
+			hipLaunchKernelGGL(k_bfs_push, dim3(nb), dim3(kBlock), 0, s, (size_t)fq, (const uint32_t *)q0.p,
+			                   (const uint64_t *)scan.p, (const uint64_t *)S.off, (const uint32_t *)S.idx, d_level,
+			                   cur + 1, q1.p, cnt.p, part, fq == 1 && A->unique_entries ? 1 : 0);
 			std::swap(q0.p, q1.p);
 		}
 		hipLaunchKernelGGL(k_bfs_report, dim3(1), dim3(kBlock), 0, s, (const unsigned long long *)part, nb, cnt.p,
-		                   w->rep_dev, ++w->seq);
+		                   w->rep_dev, ++w->seq, (const uint64_t *)nullptr);
 		ALP_LAUNCH_CHECK();
 		ALP_TRY(bfs_wait(s, w, &fq, &fe));
 		unexplored = unexplored > fe ? unexplored - fe : 0;

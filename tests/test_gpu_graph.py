@@ -137,14 +137,21 @@ def test_rmat_scale22_bfs_and_pagerank_against_oracle(A):
 
 def test_bfs_directed_pull_walks_in_neighbours(A):
     """Directed pattern (no symmetrisation): a pull step must look at each
-    vertex's in-neighbours. Edges 0->1 and 2->0 from source 0: the first step
-    already pulls (the frontier's edges exceed the unexplored ones / 14), and
-    only vertex 1 is one hop away; 2 is unreachable."""
-    src = np.array([0, 2], np.uint32)
-    dst = np.array([1, 0], np.uint32)
-    G = A.graph_from_edges(3, A.Vector.from_numpy(src), A.Vector.from_numpy(dst), flags=0)
+    vertex's in-neighbours. Edges 0->1, 1->2, 1->3, 4->2 from source 0: the
+    source's level pushes (to 1); the next level pulls (the frontier's 2 edges
+    exceed the 1 unexplored / 14) and finds 2 and 3 through their in-neighbour 1,
+    which they do not point to; 4 is unreachable. With edges 2->0 and 3->0 added,
+    the source's level reaches nothing new through them."""
+    src = np.array([0, 1, 1, 4], np.uint32)
+    dst = np.array([1, 2, 3, 2], np.uint32)
+    G = A.graph_from_edges(5, A.Vector.from_numpy(src), A.Vector.from_numpy(dst), flags=0)
     lev, nl, reached = A.bfs(G, 0)
-    assert list(lev.numpy()) == [0, 1, -1] and nl == 2 and reached == 2
+    assert list(lev.numpy()) == [0, 1, 2, 2, -1] and nl == 3 and reached == 4
+    src = np.array([0, 1, 1, 4, 2, 3], np.uint32)
+    dst = np.array([1, 2, 3, 2, 0, 0], np.uint32)
+    G = A.graph_from_edges(5, A.Vector.from_numpy(src), A.Vector.from_numpy(dst), flags=0)
+    lev, nl, reached = A.bfs(G, 0)
+    assert list(lev.numpy()) == [0, 1, 2, 2, -1] and nl == 3 and reached == 4
 
 
 def test_bfs_on_a_pattern_with_repeated_entries(A):
