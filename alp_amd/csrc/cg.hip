@@ -641,14 +641,26 @@ struct CgRunner {
 			return ALP_SUCCESS;
 		};
 		alp_rc_t rc = ALP_SUCCESS;
+		int tail = 0;  // iterations to launch directly instead of the next batch
 		for (;;) {
-			if ((rc = launch()) != ALP_SUCCESS) break;
+			if (tail > 0) {
+				for (int i = 0; i < tail && rc == ALP_SUCCESS; ++i) rc = iteration(s, i % 2);
+				if (rc != ALP_SUCCESS) break;
+			} else if ((rc = launch()) != ALP_SUCCESS) {
+				break;
+			}
 			if (hipMemcpyAsync(h_state, a.st, sizeof(CgState), hipMemcpyDeviceToHost, s) != hipSuccess ||
 			    hipStreamSynchronize(s) != hipSuccess) {
 				rc = ALP_PANIC;
 				break;
 			}
 			if (h_state->s[0].done) break;
+			// fewer iterations than a batch left before max_iter: launch just those
+			// (rounded up to even, so the run ends in slot 0) rather than a batch
+			// whose surplus iterations only find `done` set (50 iterations at 64^3:
+			// 14 idle iterations of the fourth batch, 4 kernels each)
+			const unsigned long long it = h_state->s[0].iter, mx = h_state->max_iter;
+			tail = mx > it && mx - it < (unsigned long long)batch ? (int)((mx - it + 1) / 2 * 2) : 0;
 		}
 		if (!slot || slot->exec != exec) {
 			if (exec) hipGraphExecDestroy(exec);
